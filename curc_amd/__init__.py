@@ -1,0 +1,284 @@
+"""curc_amd: MI355X-native pseudogenome construction and read mapping for CURC.
+
+Python mirror of the reference's block interface (src/compress.cuh) over the
+C ABI in include/curc_gpu.h.  The compute runs in libcurc_gpu.so (HIP kernels
+for gfx950); this module only moves arguments and results.  There is no CPU
+fallback: if the library is missing or no GPU is visible, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcurc_gpu.so")
+
+CURC_OK = 0
+STAGE_DEDUP, STAGE_OVERLAP, STAGE_PRUNE, STAGE_FULLMATCH, STAGE_ASSEMBLY, STAGE_MAP, STAGE_LQ = range(1, 8)
+STAGE_ALL = STAGE_LQ
+RUN_INPUT_ON_DEVICE = 1
+RUN_KEEP_STAGES = 2
+
+# field id -> numpy dtype (ids shared with the oracle)
+FIELDS = {
+    "packed": (0, np.uint64), "nonN_ids": (1, np.uint32), "N_ids": (2, np.uint32),
+    "sorted_ids": (3, np.uint32), "next_dedup": (4, np.uint32), "next_overlap": (5, np.uint32),
+    "prev_overlap": (6, np.uint32), "offset_overlap": (7, np.uint16), "next": (8, np.uint32),
+    "prev": (9, np.uint32), "offset": (10, np.uint16), "hq_ids": (11, np.uint32),
+    "unmatched_ids": (12, np.uint32), "hq_pg": (13, np.uint8), "hq_rec": (14, "rec"),
+    "map_rec": (15, "rec"), "map_mis": (16, np.uint8), "unmapped": (17, np.uint32),
+    "unmapped_N": (18, np.uint32), "lq_pg": (19, np.uint8), "lq_rec": (20, "rec"),
+    "prev_dedup": (23, np.uint32), "next_prune": (24, np.uint32), "prev_prune": (25, np.uint32),
+    "hq_pg2": (26, np.uint64), "lq_next": (27, np.uint32), "lq_prev": (28, np.uint32),
+    "lq_offset": (29, np.uint16), "N_ascii": (30, np.uint8),
+}
+
+# curc_alignment: u32 read_id, u32 pad, u64 word (isN | strand << 1 | pos << 2)
+REC_DTYPE = np.dtype([("read_id", "<u4"), ("pad", "<u4"), ("word", "<u8")])
+
+
+class CurcError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("L", "W", "max_off", "k", "kip", "max_mm")]
+
+
+class StageStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("launches", ctypes.c_uint64),
+                ("alg_bytes", ctypes.c_double), ("items", ctypes.c_double)]
+
+
+class SynthCfg(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64), ("genome_len", ctypes.c_uint64), ("gc", ctypes.c_double),
+        ("repeat_frac", ctypes.c_double), ("rep_min", ctypes.c_uint32), ("rep_max", ctypes.c_uint32),
+        ("div_min", ctypes.c_double), ("div_max", ctypes.c_double), ("rep_families", ctypes.c_uint32),
+        ("n_reads", ctypes.c_uint64), ("L", ctypes.c_uint32), ("paired", ctypes.c_int32),
+        ("frag_mean", ctypes.c_double), ("frag_sd", ctypes.c_double), ("rc_prob", ctypes.c_double),
+        ("sub_first", ctypes.c_double), ("sub_last", ctypes.c_double), ("n_rate", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libcurc_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CurcError(f"{LIB_PATH} is missing; run curc_amd/build.py (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "curc_make_params": (i32, [u32, ctypes.POINTER(Params)]),
+        "curc_device_count": (i32, [ctypes.POINTER(i32)]),
+        "curc_ctx_create": (vp, [i32]),
+        "curc_ctx_destroy": (None, [vp]),
+        "curc_ctx_last_error": (ctypes.c_char_p, [vp]),
+        "curc_ctx_trim": (i32, [vp]),
+        "curc_host_alloc": (i32, [vp, ctypes.POINTER(vp), u64]),
+        "curc_host_free": (i32, [vp, vp]),
+        "curc_dev_alloc": (i32, [vp, ctypes.POINTER(vp), u64]),
+        "curc_dev_free": (i32, [vp, vp]),
+        "curc_memcpy_h2d": (i32, [vp, vp, vp, u64]),
+        "curc_memcpy_d2h": (i32, [vp, vp, vp, u64]),
+        "curc_sync": (i32, [vp]),
+        "curc_pack": (i32, [vp, vp, u64, u32, vp, vp]),
+        "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
+        "curc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
+        "curc_block_free": (None, [vp]),
+        "curc_ctx_set_timing": (i32, [vp, i32]),
+        "curc_ctx_stats": (i32, [vp, ctypes.POINTER(StageStat), i32]),
+        "curc_ctx_reset_stats": (None, [vp]),
+        "curc_synth_default": (None, [ctypes.POINTER(SynthCfg)]),
+        "curc_synth_genome": (i32, [ctypes.POINTER(SynthCfg), vp]),
+        "curc_synth_reads": (i32, [ctypes.POINTER(SynthCfg), vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def make_params(L):
+    p = Params()
+    if lib().curc_make_params(L, ctypes.byref(p)) != CURC_OK:
+        raise CurcError(f"unsupported read length {L}")
+    return p
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    lib().curc_device_count(ctypes.byref(n))
+    return n.value
+
+
+# ----------------------------------------------------------------- synth ---
+def synth_config(**kw):
+    c = SynthCfg()
+    lib().curc_synth_default(ctypes.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+def synth_genome(cfg):
+    g = np.empty(cfg.genome_len, dtype=np.uint8)
+    rc = lib().curc_synth_genome(ctypes.byref(cfg), g.ctypes.data)
+    if rc != CURC_OK:
+        raise CurcError(f"curc_synth_genome failed ({rc})")
+    return g
+
+
+def synth_reads(cfg, genome=None):
+    """Returns (genome, reads) with reads an (n_reads, L) uint8 array."""
+    if genome is None:
+        genome = synth_genome(cfg)
+    out = np.empty((cfg.n_reads, cfg.L), dtype=np.uint8)
+    rc = lib().curc_synth_reads(ctypes.byref(cfg), genome.ctypes.data, out.ctypes.data)
+    if rc != CURC_OK:
+        raise CurcError(f"curc_synth_reads failed ({rc})")
+    return genome, out
+
+
+# workload presets (sizes as the reference's headline configurations)
+def config_small(seed=11):
+    """1 Mbp uniform genome, 133,333 x 150 bp SE, 50% RC, 0.5% subst, 0.01% N."""
+    return synth_config(seed=seed, genome_len=1_000_000, gc=0.5, n_reads=133_333, L=150, rc_prob=0.5,
+                        sub_first=0.005, sub_last=0.005, n_rate=0.0001)
+
+
+def config_scale(seed=22, n_reads=20_000_000, genome_len=100_000_000):
+    """100 Mbp genome (GC 41%), 20M x 150 bp SE, subst 0.1%->1.5% along the read, 0.02% N."""
+    return synth_config(seed=seed, genome_len=genome_len, gc=0.41, n_reads=n_reads, L=150, rc_prob=0.5,
+                        sub_first=0.001, sub_last=0.015, n_rate=0.0002)
+
+
+# --------------------------------------------------------------- context ---
+class Block:
+    """Results of one block (host memory owned by the library)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+
+    def get(self, name):
+        fid, dt = FIELDS[name]
+        ptr = ctypes.c_void_p()
+        n = lib().curc_block_get(self._h, fid, ctypes.byref(ptr))
+        if dt == "rec":
+            dt = REC_DTYPE
+        dt = np.dtype(dt)
+        if n == 0 or not ptr.value:
+            return np.zeros(0, dtype=dt)
+        buf = (ctypes.c_char * (n * dt.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dt, count=n).copy()
+
+    def free(self):
+        if self._h is not None and self._h.value:
+            lib().curc_block_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context (stream + cached device scratch)."""
+
+    def __init__(self, device=0):
+        h = lib().curc_ctx_create(device)
+        if not h:
+            raise CurcError(f"curc_ctx_create({device}) failed: no usable HIP device")
+        self._h = ctypes.c_void_p(h)
+
+    def _check(self, rc, what):
+        if rc != CURC_OK:
+            msg = lib().curc_ctx_last_error(self._h)
+            raise CurcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def run_block(self, reads, L=None, stop_after=STAGE_ALL, keep_stages=False, device_ptr=None, n_reads=None):
+        """Runs the GPU part of block_compress on reads ((n, L) uint8 host
+        array) or on a device buffer (device_ptr, n_reads, L)."""
+        flags = RUN_KEEP_STAGES if keep_stages else 0
+        out = ctypes.c_void_p()
+        if device_ptr is not None:
+            flags |= RUN_INPUT_ON_DEVICE
+            rc = lib().curc_block_run(self._h, ctypes.c_void_p(device_ptr), n_reads, L, stop_after, flags,
+                                      ctypes.byref(out))
+        else:
+            reads = np.ascontiguousarray(reads, dtype=np.uint8)
+            n, L = reads.shape
+            rc = lib().curc_block_run(self._h, reads.ctypes.data, n, L, stop_after, flags, ctypes.byref(out))
+        self._check(rc, "curc_block_run")
+        return Block(out.value)
+
+    def dev_alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        self._check(lib().curc_dev_alloc(self._h, ctypes.byref(p), nbytes), "curc_dev_alloc")
+        return p.value
+
+    def dev_free(self, p):
+        self._check(lib().curc_dev_free(self._h, ctypes.c_void_p(p)), "curc_dev_free")
+
+    def h2d(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        self._check(lib().curc_memcpy_h2d(self._h, ctypes.c_void_p(dst), arr.ctypes.data, arr.nbytes), "h2d")
+
+    def d2h(self, arr, src):
+        self._check(lib().curc_memcpy_d2h(self._h, arr.ctypes.data, ctypes.c_void_p(src), arr.nbytes), "d2h")
+
+    def pack(self, d_ascii, n, L, d_packed, d_isN=None):
+        self._check(lib().curc_pack(self._h, ctypes.c_void_p(d_ascii), n, L, ctypes.c_void_p(d_packed),
+                                    ctypes.c_void_p(d_isN) if d_isN else None), "curc_pack")
+
+    def sync(self):
+        self._check(lib().curc_sync(self._h), "curc_sync")
+
+    def set_timing(self, on=True):
+        lib().curc_ctx_set_timing(self._h, 1 if on else 0)
+
+    def reset_stats(self):
+        lib().curc_ctx_reset_stats(self._h)
+
+    def stats(self):
+        arr = (StageStat * 64)()
+        n = lib().curc_ctx_stats(self._h, arr, 64)
+        return [dict(name=arr[i].name.decode(), ms=arr[i].ms, launches=arr[i].launches,
+                     alg_bytes=arr[i].alg_bytes, items=arr[i].items) for i in range(min(n, 64))]
+
+    def trim(self):
+        lib().curc_ctx_trim(self._h)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().curc_ctx_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rec_fields(rec):
+    """Splits curc_alignment records into (read_id, isN, strand, pos) arrays."""
+    w = rec["word"]
+    return rec["read_id"], (w & 1).astype(np.uint8), ((w >> 1) & 1).astype(np.uint8), w >> np.uint64(2)

@@ -1,0 +1,675 @@
+// HIP kernels of the CURC pseudogenome path (gfx950, wave64).
+//
+// Design notes (the reference runs the same stages as thrust calls over
+// __device__ lambdas, compress.cu:1349-1722):
+//  * every loop over offsets keeps its list lengths in device memory, so a
+//    whole loop is enqueued without host round trips;
+//  * the reference's racy first-come claims (compress.cu:1518) and the
+//    merge-join of full_match_gpu (compress.cu:1089-1160) are both computed
+//    here as serial dictatorship inside each group of equal overlap strings:
+//    a suffix proposes its first free candidate; a group where two suffixes
+//    propose the same prefix is handed to one leader thread that replays the
+//    group in the reference's suffix order.
+#pragma once
+#include "common.cuh"
+#include "prims.cuh"
+
+namespace curc {
+
+#define GRID_STRIDE(i, n) \
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
+    GRID_STRIDE(i, n) p[i] = v;
+}
+
+__global__ void k_iota(uint32_t* __restrict__ p, uint64_t n) {
+    GRID_STRIDE(i, n) p[i] = (uint32_t)i;
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+// ------------------------------------------------------------------ pack ---
+// One block packs 64 reads: the 64*L ASCII bytes are staged in LDS with
+// coalesced dword loads, each thread then packs one 32-base word, and the
+// 64*W packed words leave through LDS again as one contiguous store.
+constexpr uint32_t PACK_READS = 64;
+
+template <bool ALIGNED4>
+__global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii, uint64_t n, uint32_t L,
+                                              uint32_t W, uint64_t* __restrict__ packed,
+                                              uint8_t* __restrict__ isN) {
+    extern __shared__ uint8_t smem[];
+    uint8_t* chars = smem;                                            // 64*L bytes, padded to 8
+    uint64_t* words = (uint64_t*)(smem + ((PACK_READS * L + 15) & ~15u)); // 64*W words
+    __shared__ uint32_t nflag[PACK_READS];
+    const uint64_t r0 = (uint64_t)blockIdx.x * PACK_READS;
+    const uint32_t nr = (uint32_t)min<uint64_t>(PACK_READS, n - r0);
+    const uint32_t nbytes = nr * L;
+    const uint8_t* src = ascii + r0 * L;
+    if (threadIdx.x < PACK_READS) nflag[threadIdx.x] = 0;
+    if (ALIGNED4) {
+        const uint32_t nw = nbytes >> 2;
+        const uint32_t* s4 = (const uint32_t*)src;
+        uint32_t* d4 = (uint32_t*)chars;
+        for (uint32_t i = threadIdx.x; i < nw; i += 256) d4[i] = s4[i];
+        for (uint32_t i = (nw << 2) + threadIdx.x; i < nbytes; i += 256) chars[i] = src[i];
+    } else {
+        for (uint32_t i = threadIdx.x; i < nbytes; i += 256) chars[i] = src[i];
+    }
+    __syncthreads();
+    const uint32_t tasks = nr * W;
+    for (uint32_t t = threadIdx.x; t < tasks; t += 256) {
+        uint32_t r = t / W, w = t - r * W;
+        uint32_t b0 = w * 32;
+        uint32_t nb = min(32u, L - b0);
+        const uint8_t* c = chars + r * L + b0;
+        uint64_t v = 0;
+        uint32_t hasN = 0;
+        for (uint32_t j = 0; j < nb; ++j) {
+            uint8_t ch = c[j];
+            v |= (uint64_t)(enc_base(ch) & 3u) << (62u - 2u * j);
+            hasN |= (ch == 'N');
+        }
+        words[t] = v;
+        if (hasN) atomicOr(&nflag[r], 1u);
+    }
+    __syncthreads();
+    uint64_t* dst = packed + r0 * W;
+    for (uint32_t t = threadIdx.x; t < tasks; t += 256) dst[t] = words[t];
+    if (isN && threadIdx.x < nr) isN[r0 + threadIdx.x] = (uint8_t)nflag[threadIdx.x];
+}
+
+struct PredIsN {
+    const uint8_t* isN;
+    bool want;
+    __device__ bool operator()(uint32_t i) const { return (isN[i] != 0) == want; }
+};
+struct EmitIndex {
+    uint32_t* out;
+    __device__ void operator()(uint32_t i, uint32_t j) const { out[j] = i; }
+};
+
+__global__ void k_gather_ascii(const uint8_t* __restrict__ ascii, const uint32_t* __restrict__ ids,
+                               uint64_t n, uint32_t L, uint8_t* __restrict__ out) {
+    GRID_STRIDE(b, n * L) {
+        uint64_t r = b / L, j = b - r * L;
+        out[b] = ascii[(uint64_t)ids[r] * L + j];
+    }
+}
+
+__global__ void k_gather_packed(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids,
+                                uint64_t n, uint32_t W, uint64_t* __restrict__ out) {
+    GRID_STRIDE(t, n * W) {
+        uint64_t r = t / W, w = t - r * W;
+        out[t] = R[(uint64_t)ids[r] * W + w];
+    }
+}
+
+// ----------------------------------------------------------------- dedup ---
+__global__ void k_gather_word(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids, uint64_t n,
+                              uint32_t W, uint32_t w, uint64_t* __restrict__ keys) {
+    GRID_STRIDE(i, n) keys[i] = R[(uint64_t)ids[i] * W + w];
+}
+
+// compress.cu:1409-1420: adjacent equal reads in sorted order are linked.
+__global__ void k_dedup_link(const uint64_t* __restrict__ R, const uint32_t* __restrict__ sorted, uint64_t n,
+                             uint32_t W, uint32_t* __restrict__ next, uint32_t* __restrict__ prev,
+                             uint16_t* __restrict__ offset) {
+    GRID_STRIDE(i, n - 1) {
+        uint32_t a = sorted[i], b = sorted[i + 1];
+        const uint64_t* ra = R + (uint64_t)a * W;
+        const uint64_t* rb = R + (uint64_t)b * W;
+        bool eq = true;
+        for (uint32_t w = 0; w < W; ++w)
+            if (ra[w] != rb[w]) {
+                eq = false;
+                break;
+            }
+        if (eq) {
+            next[a] = b;
+            prev[b] = a;
+            offset[a] = 0;
+        }
+    }
+}
+
+struct PredLinkState {
+    const uint32_t* list;
+    const uint32_t* link;  // next or prev
+    __device__ bool operator()(uint32_t i) const { return link[list[i]] == INV; }
+};
+struct EmitGather {
+    const uint32_t* list;
+    uint32_t* out;
+    __device__ void operator()(uint32_t i, uint32_t j) const { out[j] = list[i]; }
+};
+struct EmitGatherKey {
+    const uint32_t* list;
+    const uint64_t* R;
+    uint32_t W;
+    uint32_t* out;
+    uint64_t* key;
+    __device__ void operator()(uint32_t i, uint32_t j) const {
+        uint32_t id = list[i];
+        out[j] = id;
+        key[j] = R[(uint64_t)id * W];
+    }
+};
+
+// -------------------------------------------------------- prefix index ---
+__global__ void k_prefix_hash(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids, uint64_t n,
+                              uint32_t W, uint32_t kip, uint32_t k, uint32_t mask, uint32_t* __restrict__ hk) {
+    GRID_STRIDE(i, n) hk[i] = (uint32_t)(murmur64(xw(R + (uint64_t)ids[i] * W, kip, k)) & mask);
+}
+
+// CSR offsets from sorted keys: kr[b] = first index with key >= b, for
+// b in [0, nb]; every entry is written by exactly one element.
+__global__ void k_gap_fill(const uint32_t* __restrict__ keys, uint64_t n, uint64_t nb, uint32_t* __restrict__ kr) {
+    GRID_STRIDE(i, n + 1) {
+        uint64_t lo = (i == 0) ? 0 : (uint64_t)keys[i - 1] + 1;
+        uint64_t hi = (i == n) ? nb : (uint64_t)keys[i];
+        for (uint64_t b = lo; b <= hi; ++b) kr[b] = (uint32_t)i;
+    }
+}
+
+__global__ void k_gather_key0(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids, uint64_t n,
+                              uint32_t W, uint64_t* __restrict__ key) {
+    GRID_STRIDE(i, n) key[i] = R[(uint64_t)ids[i] * W];
+}
+
+__global__ void k_gather_key0_dev(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids,
+                                  const uint32_t* __restrict__ d_n, uint32_t W, uint64_t* __restrict__ key) {
+    const uint32_t n = *d_n;
+    GRID_STRIDE(i, n) key[i] = R[(uint64_t)ids[i] * W];
+}
+
+// p[0:len) vs s[off:off+len), with p's first word given and s's first
+// min(32,len) bases pre-extracted in sx0.
+__device__ __forceinline__ int cmp_ps(const uint64_t* __restrict__ R, uint32_t W, uint32_t p, uint64_t pkey,
+                                      const uint64_t* __restrict__ rs, uint64_t sx0, uint32_t off, uint32_t len) {
+    uint32_t n = min(32u, len);
+    uint64_t a = pkey >> ((32u - n) << 1);
+    if (a != sx0) return a < sx0 ? -1 : 1;
+    const uint64_t* rp = R + (uint64_t)p * W;
+    for (uint32_t j = 32; j < len; j += 32) {
+        n = min(32u, len - j);
+        uint64_t x = xw(rp, j, n), y = xw(rs, off + j, n);
+        if (x != y) return x < y ? -1 : 1;
+    }
+    return 0;
+}
+
+// Order of two suffixes of one full-match group (equal s[off:L]): the
+// reference's suffix list is sorted by read[off:] then by base off-1,
+// off-2, ..., 0 (compress.cu:1013-1059), i.e. by the reversed s[0:off).
+__device__ __forceinline__ int cmp_rev_prefix(const uint64_t* __restrict__ R, uint32_t W, uint32_t a,
+                                              uint32_t b, uint32_t off) {
+    const uint64_t* ra = R + (uint64_t)a * W;
+    const uint64_t* rb = R + (uint64_t)b * W;
+    for (uint32_t t = 0; t < off; t += 32) {
+        uint32_t n = min(32u, off - t);
+        uint32_t st = off - t - n;
+        uint64_t mask = (n == 32) ? ~0ull : ((1ull << (2 * n)) - 1);
+        uint64_t x = revcomp(xw(ra, st, n), n) ^ mask;
+        uint64_t y = revcomp(xw(rb, st, n), n) ^ mask;
+        if (x != y) return x < y ? -1 : 1;
+    }
+    return 0;
+}
+
+// State shared by the overlap (a6) and full-match (a8) claim rounds.
+struct ClaimState {
+    const uint64_t* R;
+    uint32_t W, L;
+    const uint32_t* sl;     // suffix list (ids)
+    const uint32_t* d_ns;   // suffix count (device)
+    const uint32_t* pl;     // prefix array (ids): a6 hash-ordered, a8 lexicographic list
+    const uint64_t* pk;     // first word of each prefix
+    const uint32_t* d_np;   // prefix count (device)
+    uint32_t* next;
+    uint32_t* prev;
+    uint16_t* offset;
+    uint32_t* choice;       // per suffix index: chosen prefix position or INV
+    uint32_t* grp;          // per suffix index: group id (first equal position)
+    uint32_t* owner;        // per prefix position: proposing suffix index
+    uint32_t* contended;    // per prefix position (group id): 1 if contended
+    uint32_t* head;         // per group id: member list head
+    uint32_t* memb;         // per suffix index: next member
+    uint32_t* res_list;
+    uint32_t* res_n;
+};
+
+// a6 probe (compress.cu:1474-1528): hash the k-mer at kip+off, binary
+// search the bucket, take the first free non-self candidate if it matches.
+__global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t* __restrict__ kr, uint32_t mask,
+                                                  uint32_t k, uint32_t kip, uint32_t off) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = 0;
+    const uint32_t ns = *cs.d_ns;
+    const uint32_t len = cs.L - off;
+    const uint32_t n1 = min(32u, len);
+    GRID_STRIDE(i, ns) {
+        uint32_t s = cs.sl[i];
+        const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
+        uint32_t h = (uint32_t)(murmur64(xw(rs, kip + off, k)) & mask);
+        uint32_t lo = kr[h], e = kr[h + 1];
+        uint64_t sx0 = xw(rs, off, n1);
+        uint32_t cnt = e - lo;
+        while (cnt > 0) {
+            uint32_t step = cnt >> 1, it = lo + step;
+            if (cmp_ps(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
+                lo = it + 1;
+                cnt -= step + 1;
+            } else {
+                cnt = step;
+            }
+        }
+        uint32_t ch = INV;
+        for (uint32_t x = lo; x < e; ++x) {
+            uint32_t p = cs.pl[x];
+            if (p == s || cs.prev[p] != INV) continue;
+            if (cmp_ps(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) ch = x;
+            break;
+        }
+        cs.choice[i] = ch;
+        cs.grp[i] = lo;
+        if (ch != INV) {
+            uint32_t old = atomicCAS(&cs.owner[ch], INV, (uint32_t)i);
+            if (old != INV) cs.contended[lo] = 1;
+        }
+    }
+}
+
+// a8 probe (compress.cu:1061-1160): the 10-mer block of s[off:] bounds the
+// search in the lexicographic prefix list; first equal non-self prefix.
+__global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t* __restrict__ rng, uint32_t bp,
+                                                  uint32_t off) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = 0;
+    const uint32_t ns = *cs.d_ns;
+    const uint32_t len = cs.L - off;
+    const uint32_t n1 = min(32u, len);
+    GRID_STRIDE(i, ns) {
+        uint32_t s = cs.sl[i];
+        const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
+        uint32_t b = (uint32_t)xw(rs, off, bp);
+        uint32_t lo = rng[b], e = rng[b + 1];
+        uint64_t sx0 = xw(rs, off, n1);
+        uint32_t cnt = e - lo;
+        while (cnt > 0) {
+            uint32_t step = cnt >> 1, it = lo + step;
+            if (cmp_ps(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
+                lo = it + 1;
+                cnt -= step + 1;
+            } else {
+                cnt = step;
+            }
+        }
+        uint32_t ch = INV;
+        if (lo < e && cmp_ps(cs.R, cs.W, cs.pl[lo], cs.pk[lo], rs, sx0, off, len) == 0) {
+            if (cs.pl[lo] != s) {
+                ch = lo;
+            } else if (lo + 1 < e && cmp_ps(cs.R, cs.W, cs.pl[lo + 1], cs.pk[lo + 1], rs, sx0, off, len) == 0) {
+                ch = lo + 1;
+            }
+        }
+        cs.choice[i] = ch;
+        cs.grp[i] = lo;
+        if (ch != INV) {
+            uint32_t old = atomicCAS(&cs.owner[ch], INV, (uint32_t)i);
+            if (old != INV) cs.contended[lo] = 1;
+        }
+    }
+}
+
+// Uncontended groups link at once; members of contended groups are queued
+// for their leader.
+__global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t off) {
+    const uint32_t ns = *cs.d_ns;
+    GRID_STRIDE(i, ns) {
+        uint32_t ch = cs.choice[i];
+        if (ch == INV) continue;
+        cs.owner[ch] = INV;
+        uint32_t g = cs.grp[i];
+        if (cs.contended[g]) {
+            cs.memb[i] = atomicExch(&cs.head[g], (uint32_t)i);
+            uint32_t q = atomicAdd(cs.res_n, 1u);
+            cs.res_list[q] = (uint32_t)i;
+        } else {
+            uint32_t s = cs.sl[i], p = cs.pl[ch];
+            cs.next[s] = p;
+            cs.prev[p] = s;
+            cs.offset[s] = (uint16_t)off;
+        }
+    }
+}
+
+// One leader per contended group replays it sequentially: members in the
+// reference's suffix order (a6: list order; a8: reversed s[0:off)), each
+// taking the first free non-self equal prefix from the group start.
+template <bool FULLMATCH>
+__global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t off) {
+    const uint32_t nq = *cs.res_n;
+    const uint32_t np = *cs.d_np;
+    const uint32_t len = cs.L - off;
+    const uint32_t n1 = min(32u, len);
+    GRID_STRIDE(q, nq) {
+        uint32_t lead = cs.res_list[q];
+        uint32_t g = cs.grp[lead];
+        if (cs.head[g] != lead) continue;
+        uint32_t last = INV;
+        for (;;) {
+            uint32_t best = INV;
+            for (uint32_t u = lead; u != INV; u = cs.memb[u]) {
+                if (FULLMATCH) {
+                    if (last != INV && cmp_rev_prefix(cs.R, cs.W, cs.sl[u], cs.sl[last], off) <= 0) continue;
+                    if (best != INV && cmp_rev_prefix(cs.R, cs.W, cs.sl[u], cs.sl[best], off) >= 0) continue;
+                } else {
+                    if (last != INV && u <= last) continue;
+                    if (best != INV && u >= best) continue;
+                }
+                best = u;
+            }
+            if (best == INV) break;
+            last = best;
+            uint32_t s = cs.sl[best];
+            const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
+            uint64_t sx0 = xw(rs, off, n1);
+            for (uint32_t x = g; x < np; ++x) {
+                uint32_t p = cs.pl[x];
+                if (p == s || cs.prev[p] != INV) continue;
+                if (cmp_ps(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) {
+                    cs.next[s] = p;
+                    cs.prev[p] = s;
+                    cs.offset[s] = (uint16_t)off;
+                }
+                break;
+            }
+        }
+        cs.head[g] = INV;
+        cs.contended[g] = 0;
+    }
+}
+
+// Block ranges of the lexicographic prefix list by its first bp bases.
+__global__ void k_fm_ranges(const uint64_t* __restrict__ pk, const uint32_t* __restrict__ d_np, uint32_t bp,
+                            uint32_t* __restrict__ rng) {
+    const uint32_t np = *d_np;
+    const uint64_t nb = 1ull << (2 * bp);
+    const uint32_t sh = 64 - 2 * bp;
+    GRID_STRIDE(i, (uint64_t)np + 1) {
+        uint64_t lo = (i == 0) ? 0 : (pk[i - 1] >> sh) + 1;
+        uint64_t hi = (i == np) ? nb : (pk[i] >> sh);
+        for (uint64_t b = lo; b <= hi; ++b) rng[b] = (uint32_t)i;
+    }
+}
+
+struct PredSufFree {
+    const uint32_t* sl;
+    const uint32_t* next;
+    __device__ bool operator()(uint32_t i) const { return next[sl[i]] == INV; }
+};
+struct PredPreFree {
+    const uint32_t* pl;
+    const uint32_t* prev;
+    __device__ bool operator()(uint32_t i) const { return prev[pl[i]] == INV; }
+};
+struct EmitPre {
+    const uint32_t* pl;
+    const uint64_t* pk;
+    uint32_t* pl2;
+    uint64_t* pk2;
+    __device__ void operator()(uint32_t i, uint32_t j) const {
+        pl2[j] = pl[i];
+        pk2[j] = pk[i];
+    }
+};
+
+// ----------------------------------------------------------------- prune ---
+// compress.cu:1539-1564
+__global__ void k_prune_mark(const uint32_t* __restrict__ next, const uint32_t* __restrict__ prev,
+                             const uint16_t* __restrict__ offset, uint64_t n, uint8_t* __restrict__ res) {
+    GRID_STRIDE(r, n) {
+        uint32_t nx = next[r], pv = prev[r];
+        bool keep = (nx != INV && pv != INV) || (nx != INV && offset[r] == 0) || (pv != INV && offset[pv] == 0);
+        res[r] = keep ? 1 : 0;
+    }
+}
+
+// Links whose either end is dropped are cleared; the outcome does not depend
+// on thread order (each cleared link is cleared from both ends).
+__global__ void k_prune_unlink(uint32_t* __restrict__ next, uint32_t* __restrict__ prev,
+                               const uint8_t* __restrict__ res, uint64_t n) {
+    GRID_STRIDE(r, n) {
+        if (res[r]) continue;
+        uint32_t nx = next[r];
+        if (nx != INV) {
+            prev[nx] = INV;
+            next[r] = INV;
+        }
+        uint32_t pv = prev[r];
+        if (pv != INV) {
+            next[pv] = INV;
+            prev[r] = INV;
+        }
+    }
+}
+
+struct PredRes {
+    const uint32_t* list;
+    const uint8_t* res;
+    bool want;
+    __device__ bool operator()(uint32_t i) const { return (res[list[i]] != 0) == want; }
+};
+
+// ------------------------------------------------------- pseudogenome ----
+// Writes nb bases of a read at PG position pos into a zeroed 2-bit array.
+__global__ void k_pg_write(const uint64_t* __restrict__ R, uint32_t W, const uint32_t* __restrict__ ids,
+                           const uint64_t* __restrict__ pos, const uint32_t* __restrict__ nbs, uint64_t n,
+                           unsigned long long* __restrict__ pg) {
+    GRID_STRIDE(i, n) {
+        const uint64_t* rd = R + (uint64_t)ids[i] * W;
+        uint64_t p = pos[i];
+        uint32_t nb = nbs[i];
+        for (uint32_t j = 0; j < nb; j += 32) {
+            uint32_t c = min(32u, nb - j);
+            uint64_t v = xw(rd, j, c);            // c bases, right-aligned
+            uint64_t q = p + j;
+            uint32_t o2 = (uint32_t)(q & 31u);     // bases already used in the first word
+            // place the c bases at base offset o2 of word q/32
+            uint64_t hi = (o2 + c <= 32) ? (v << (2 * (32 - o2 - c))) : (v >> (2 * (o2 + c - 32)));
+            atomicOr(&pg[q >> 5], (unsigned long long)hi);
+            if (o2 + c > 32) {
+                uint32_t spill = o2 + c - 32;
+                uint64_t lo = v << (2 * (32 - spill));
+                atomicOr(&pg[(q >> 5) + 1], (unsigned long long)lo);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- mapping ---
+constexpr uint32_t MAP_K = 28;       // Param.hpp:73 k1
+constexpr uint32_t MAP_REF_STEP = 5; // ref_index_step1
+constexpr uint32_t MAP_READ_STEP = 2;// read_index_step1
+constexpr uint32_t MAP_TARGET = 1;   // target_mismatch_count1
+constexpr uint32_t MAP_BUCKET = 12;  // bucket_limit
+
+__global__ void k_map_hash(const uint64_t* __restrict__ pg, uint64_t start, uint64_t count, bool rc, uint32_t mask,
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    GRID_STRIDE(i, count) {
+        uint64_t km = xwu(pg, start + i * MAP_REF_STEP, MAP_K);
+        if (rc) km = revcomp(km, MAP_K);
+        keys[i] = (uint32_t)(murmur64(km) & mask);
+        vals[i] = (uint32_t)i;
+    }
+}
+
+// Capped bucket sizes: min(run length, 12), entries [0, H+2).
+__global__ void k_map_caps(const uint32_t* __restrict__ krf, uint64_t H, uint32_t* __restrict__ caps) {
+    GRID_STRIDE(b, H + 2) {
+        uint32_t c = 0;
+        if (b < H) c = min(krf[b + 1] - krf[b], MAP_BUCKET);
+        caps[b] = c;
+    }
+}
+
+// The first min(n,12) occurrences (ascending i, as sorted) land in
+// descending order (the sequential meaning of compress.cu:369-406).
+__global__ void k_map_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint64_t count,
+                           const uint32_t* __restrict__ krf, const uint32_t* __restrict__ kr,
+                           uint32_t* __restrict__ pos) {
+    GRID_STRIDE(t, count) {
+        uint32_t h = keys[t];
+        uint32_t q = (uint32_t)t - krf[h];
+        uint32_t c = min(krf[h + 1] - krf[h], MAP_BUCKET);
+        if (q < c) pos[kr[h] + c - 1 - q] = vals[t];
+    }
+}
+
+__device__ __forceinline__ uint32_t fwd_mm(const uint64_t* __restrict__ ref, uint64_t o1,
+                                           const uint64_t* __restrict__ rd, uint32_t L, uint32_t bound) {
+    uint32_t mm = 0, i = 0;
+    for (i = 0; (i + 32 < L) && (mm <= bound); i += 32) mm += mm_words(xwu(ref, o1 + i, 32), xw(rd, i, 32));
+    if ((i < L) && (mm <= bound)) mm += mm_words(xwu(ref, o1 + i, L - i), xw(rd, i, L - i));
+    return mm;
+}
+
+__device__ __forceinline__ uint32_t rc_mm(const uint64_t* __restrict__ ref, uint64_t o1,
+                                          const uint64_t* __restrict__ rd, uint32_t L, uint32_t bound) {
+    uint32_t mm = 0, i = 0;
+    for (i = 0; (i + 32 < L) && (mm <= bound); i += 32)
+        mm += mm_words(xwu(ref, o1 + i, 32), revcomp(xw(rd, L - (i + 32), 32), 32));
+    if ((i < L) && (mm <= bound)) mm += mm_words(xwu(ref, o1 + i, L - i), revcomp(xw(rd, 0, L - i), L - i));
+    return mm;
+}
+
+__device__ __forceinline__ uint64_t kmer_ascii(const uint8_t* s, uint32_t k) {
+    uint64_t bits = 0;
+    for (uint32_t i = 0; i < k; ++i) bits |= (uint64_t)(enc_base(s[i]) & 3u) << (2 * (k - 1 - i));
+    return bits;
+}
+
+__device__ __forceinline__ uint8_t base_char(uint32_t c) {
+    return (uint8_t)(c == 0 ? 'A' : c == 1 ? 'C' : c == 2 ? 'G' : 'T');
+}
+
+struct MapArgs {
+    const uint64_t* pg;
+    uint64_t pg_len;
+    const uint32_t* kr;
+    const uint32_t* pos;
+    uint32_t mask;
+    uint64_t start;
+    const uint64_t* R;         // packed reads, by global id
+    uint32_t W, L;
+    const uint32_t* un_ids;    // unmatched non-N ids
+    uint64_t n_un;
+    const uint8_t* nascii;     // N-reads ASCII
+    uint64_t n_N;
+    uint8_t* last;
+    uint64_t* word;            // record word per mapping slot
+};
+
+// compress.cu:411-521: one thread per read; candidates in bucket order,
+// a strictly better mismatch count replaces the record, stop at <= 1.
+template <bool RC>
+__global__ void __launch_bounds__(256) k_map(MapArgs a) {
+    GRID_STRIDE(i, a.n_un + a.n_N) {
+        uint32_t last = a.last[i];
+        if (last <= MAP_TARGET) continue;
+        const bool isN = i >= a.n_un;
+        uint64_t w = a.word[i];
+        if (!isN) {
+            const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * a.W;
+            for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
+                uint32_t h = (uint32_t)(murmur64(xw(rd, j, MAP_K)) & a.mask);
+                uint32_t b = a.kr[h], e = a.kr[h + 1];
+                if (b == e) continue;
+                e = min(b + MAP_BUCKET, e);
+                const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
+                for (uint32_t p = b; p < e; ++p) {
+                    uint64_t rp = a.start + (uint64_t)a.pos[p] * MAP_REF_STEP;
+                    if (rp < ls) continue;
+                    uint64_t mp = rp - ls;
+                    if (mp + a.L > a.pg_len) continue;
+                    uint32_t c = RC ? rc_mm(a.pg, mp, rd, a.L, last - 1) : fwd_mm(a.pg, mp, rd, a.L, last - 1);
+                    if (c < last) {
+                        w = (RC ? 2ull : 0ull) | (mp << 2);
+                        last = c;
+                        if (last <= MAP_TARGET) goto done;
+                    }
+                }
+            }
+        } else {
+            const uint8_t* ra = a.nascii + (i - a.n_un) * a.L;
+            for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
+                uint32_t h = (uint32_t)(murmur64(kmer_ascii(ra + j, MAP_K)) & a.mask);
+                uint32_t b = a.kr[h], e = a.kr[h + 1];
+                if (b == e) continue;
+                e = min(b + MAP_BUCKET, e);
+                const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
+                for (uint32_t p = b; p < e; ++p) {
+                    uint64_t rp = a.start + (uint64_t)a.pos[p] * MAP_REF_STEP;
+                    if (rp < ls) continue;
+                    uint64_t mp = rp - ls;
+                    if (mp + a.L > a.pg_len) continue;
+                    uint32_t c = 0;
+                    for (uint32_t o = 0; o < a.L; ++o) {
+                        uint8_t rb = base_char(base2(a.pg, mp + o));
+                        uint8_t qb = RC ? rev_base_ascii(ra[a.L - 1 - o]) : ra[o];
+                        if (rb != qb) {
+                            c++;
+                            if (c >= last) break;
+                        }
+                    }
+                    if (c < last) {
+                        w = 1ull | (RC ? 2ull : 0ull) | (mp << 2);
+                        last = c;
+                        if (last <= MAP_TARGET) goto done;
+                    }
+                }
+            }
+        }
+    done:
+        a.last[i] = (uint8_t)last;
+        a.word[i] = w;
+    }
+}
+
+__global__ void k_map_init(uint8_t* __restrict__ last, uint64_t* __restrict__ word, uint64_t n_un, uint64_t n,
+                           uint32_t max_mm) {
+    GRID_STRIDE(i, n) {
+        last[i] = (uint8_t)(max_mm + 1);
+        word[i] = (i >= n_un) ? 1ull : 0ull;
+    }
+}
+
+struct PredMapped {
+    const uint8_t* last;
+    uint32_t max_mm;
+    uint32_t base;
+    bool want;
+    __device__ bool operator()(uint32_t i) const { return (last[base + i] <= max_mm) == want; }
+};
+struct EmitMapRec {
+    const uint8_t* last;
+    const uint64_t* word;
+    const uint32_t* un_ids;
+    uint32_t n_un;
+    uint32_t* rid;
+    uint64_t* rword;
+    uint8_t* mis;
+    __device__ void operator()(uint32_t i, uint32_t j) const {
+        rid[j] = (i < n_un) ? un_ids[i] : i - n_un;
+        rword[j] = word[i];
+        mis[j] = last[i];
+    }
+};
+struct EmitUnmapped {
+    const uint32_t* un_ids;  // null for N-reads (emit the index)
+    uint32_t* out;
+    __device__ void operator()(uint32_t i, uint32_t j) const { out[j] = un_ids ? un_ids[i] : i; }
+};
+
+}  // namespace curc

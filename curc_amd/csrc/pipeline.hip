@@ -1,0 +1,968 @@
+// Block driver and C ABI of libcurc_gpu (see include/curc_gpu.h).
+//
+// One block runs as a single stream of kernels; the host is consulted only
+// where a size decides an allocation (after the N split, after dedup, after
+// the partition, after mapping) and for the pseudogenome walk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/curc_gpu.h"
+#include "kernels.cuh"
+
+using namespace curc;
+
+namespace {
+
+enum Slot {
+    S_ASCII, S_PACKED, S_ISN, S_NONN, S_NIDS, S_NASCII, S_CNT,
+    S_K0, S_K1, S_V0, S_V1, S_NEXT, S_PREV, S_OFF,
+    S_VA, S_VK, S_KR, S_SL0, S_SL1, S_CHOICE, S_GRP, S_OWNER, S_CONT, S_HEAD, S_MEMB, S_RESL,
+    S_PL0, S_PL1, S_PK0, S_PK1, S_RNG, S_RES, S_HQ, S_UNM, S_PRE, S_SUF,
+    S_PG_HQ, S_PG_LQ, S_PGIDS, S_PGPOS, S_PGNB, S_SORTED,
+    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_LAST, S_WORD,
+    S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N,
+    S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
+    S_PRIM = 72  // primitives use slots 72.. (scan recursion, sort hist, compaction)
+};
+
+// Device counters (u32) in slot S_CNT.
+enum Cnt {
+    C_N, C_NONN, C_NN, C_NPRE, C_NSUF, C_NS0, C_NS1, C_RESN, C_NP0, C_NP1,
+    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_COUNT = 32
+};
+
+inline uint32_t grid_for(uint64_t n, uint32_t cap = 8192) {
+    uint64_t g = (n + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+inline uint32_t hash_size(uint64_t n) {  // util.cuh:72-80
+    uint32_t hs;
+    uint32_t i = 24;
+    do {
+        hs = 1u << (i++);
+    } while (i <= 31 && hs < n);
+    return hs;
+}
+
+inline uint32_t log2u(uint64_t v) {
+    uint32_t r = 0;
+    while ((1ull << r) < v) ++r;
+    return r;
+}
+
+struct StageStat {
+    std::string name;
+    double ms = 0;
+    uint64_t launches = 0;
+    double alg_bytes = 0;
+    double items = 0;
+};
+
+struct PendingTimer {
+    int stat;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct curc_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    Scratch S;
+    std::string err;
+    bool timing = false;
+    std::vector<StageStat> stats;
+    std::vector<PendingTimer> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    int stat_id(const char* name) {
+        for (size_t i = 0; i < stats.size(); ++i)
+            if (stats[i].name == name) return (int)i;
+        stats.push_back(StageStat{name});
+        return (int)stats.size() - 1;
+    }
+    hipEvent_t ev() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        CURC_HIP(hipEventCreate(&e));
+        return e;
+    }
+    // Sums pending event pairs into stats (call after a stream sync).
+    void harvest() {
+        for (auto& p : pending) {
+            float ms = 0;
+            CURC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            stats[p.stat].ms += ms;
+            stats[p.stat].launches += 1;
+            event_pool.push_back(p.a);
+            event_pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void add_bytes(const char* name, double bytes, double items) {
+        int s = stat_id(name);
+        stats[s].alg_bytes += bytes;
+        stats[s].items += items;
+    }
+};
+
+namespace {
+
+// Brackets a region of stream work with HIP events when timing is on.
+struct Region {
+    curc_ctx* c;
+    int stat = -1;
+    hipEvent_t a{}, b{};
+    Region(curc_ctx* ctx, const char* name) : c(ctx) {
+        if (!c->timing) return;
+        stat = c->stat_id(name);
+        a = c->ev();
+        b = c->ev();
+        CURC_HIP(hipEventRecord(a, c->st));
+    }
+    ~Region() {
+        if (stat < 0) return;
+        if (hipEventRecord(b, c->st) == hipSuccess) c->pending.push_back(PendingTimer{stat, a, b});
+    }
+};
+
+template <typename T>
+void fill(curc_ctx* c, T* p, uint64_t n, T v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(256), 0, c->st, p, n, v);
+}
+
+void set_cnt(curc_ctx* c, uint32_t* p, uint32_t v) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->st, p, v);
+}
+
+uint32_t read_cnt(curc_ctx* c, const uint32_t* p) {
+    uint32_t v = 0;
+    CURC_HIP(hipMemcpyAsync(&v, p, 4, hipMemcpyDeviceToHost, c->st));
+    CURC_HIP(hipStreamSynchronize(c->st));
+    return v;
+}
+
+template <typename T>
+void d2h(curc_ctx* c, std::vector<T>& out, const T* src, uint64_t n) {
+    out.resize(n);
+    if (n) CURC_HIP(hipMemcpyAsync(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, c->st));
+}
+
+template <typename T>
+void h2d(curc_ctx* c, T* dst, const T* src, uint64_t n) {
+    if (n) CURC_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->st));
+}
+
+// Pseudogenome walk (compress.cu:107-165): components in the order their
+// first member appears in `visit`; a cycle is cut after its first
+// largest-offset read.  Emits (read, position, bases written) per record.
+struct Walk {
+    std::vector<uint32_t> ids;
+    std::vector<uint64_t> pos;
+    std::vector<uint32_t> nb;
+    uint64_t len = 0;
+};
+
+void walk_chains(Walk& out, const uint32_t* visit_ids, uint64_t n_visit, const uint32_t* next,
+                 const uint32_t* prev, const uint16_t* offset, uint64_t n_reads, uint32_t L) {
+    std::vector<uint8_t> seen(n_reads, 0);
+    out.ids.clear();
+    out.pos.clear();
+    out.nb.clear();
+    out.ids.reserve(n_visit);
+    out.pos.reserve(n_visit);
+    out.nb.reserve(n_visit);
+    uint64_t p = 0;
+    auto emit = [&](uint32_t id, uint32_t nb) {
+        out.ids.push_back(id);
+        out.pos.push_back(p);
+        out.nb.push_back(nb);
+        p += nb;
+    };
+    // A well-formed graph has symmetric links, so every component is a path
+    // or a cycle; the step budget turns a malformed one into an error.
+    uint64_t budget = 4 * n_reads + 16;
+    auto step = [&]() {
+        if (budget-- == 0) throw std::runtime_error("pseudogenome walk: link graph is not a set of paths and cycles");
+    };
+    for (uint64_t r = 0; r < n_visit; ++r) {
+        const uint32_t rid = visit_ids[r];
+        if (seen[rid]) continue;
+        uint32_t i = rid;
+        while (prev[i] != INV && prev[i] != rid) {
+            i = prev[i];
+            step();
+        }
+        if (prev[i] == rid) {
+            uint16_t mo = offset[i];
+            uint32_t mo_id = i;
+            while (i != rid) {
+                step();
+                seen[i] = 1;
+                i = next[i];
+                if (offset[i] > mo) {
+                    mo = offset[i];
+                    mo_id = i;
+                }
+            }
+            seen[rid] = 1;
+            for (uint32_t s = next[mo_id]; s != mo_id; s = next[s]) {
+                step();
+                emit(s, offset[s]);
+            }
+            emit(mo_id, L);
+        } else {
+            while (next[i] != INV) {
+                step();
+                seen[i] = 1;
+                emit(i, offset[i]);
+                i = next[i];
+            }
+            seen[i] = 1;
+            emit(i, L);
+        }
+    }
+    out.len = p;
+}
+
+}  // namespace
+
+struct curc_block {
+    uint32_t L = 0, W = 0;
+    uint64_t n_reads = 0;
+    std::vector<uint64_t> packed;
+    std::vector<uint32_t> nonN, Nids, sorted;
+    std::vector<uint32_t> next_dedup, prev_dedup, next_ov, prev_ov, next_pr, prev_pr;
+    std::vector<uint16_t> off_ov;
+    std::vector<uint32_t> next, prev;
+    std::vector<uint16_t> offset;
+    std::vector<uint32_t> hq, unmatched;
+    std::vector<uint64_t> hq_pg2, lq_pg2;
+    uint64_t hq_len = 0, lq_len = 0;
+    std::string hq_text, lq_text;
+    bool hq_text_ok = false, lq_text_ok = false;
+    std::vector<curc_alignment> hq_rec, map_rec, lq_rec;
+    std::vector<uint8_t> map_mis;
+    std::vector<uint32_t> unmapped, unmapped_N;
+    std::vector<uint32_t> lq_next, lq_prev;
+    std::vector<uint16_t> lq_off;
+    std::vector<char> Nascii;
+};
+
+namespace {
+
+std::string decode_pg(const std::vector<uint64_t>& pg2, uint64_t len) {
+    static const char D[4] = {'A', 'C', 'G', 'T'};
+    std::string s(len, 'A');
+    for (uint64_t i = 0; i < len; ++i) s[i] = D[(pg2[i >> 5] >> (2 * (31 - (i & 31)))) & 3];
+    return s;
+}
+
+// Builds the 2-bit pseudogenome on the device from walk records over the
+// packed reads R; returns the device array (ceil(len/32)+1 words, zero tail).
+uint64_t* build_pg(curc_ctx* c, const uint64_t* R, uint32_t W, const Walk& wk, int slot) {
+    const uint64_t words = (wk.len + 31) / 32 + 1;
+    uint64_t* pg = c->S.as<uint64_t>(slot, words);
+    fill<uint64_t>(c, pg, words, 0ull);
+    const uint64_t n = wk.ids.size();
+    if (n) {
+        uint32_t* ids = c->S.as<uint32_t>(S_PGIDS, n);
+        uint64_t* pos = c->S.as<uint64_t>(S_PGPOS, n);
+        uint32_t* nb = c->S.as<uint32_t>(S_PGNB, n);
+        h2d(c, ids, wk.ids.data(), n);
+        h2d(c, pos, wk.pos.data(), n);
+        h2d(c, nb, wk.nb.data(), n);
+        hipLaunchKernelGGL(k_pg_write, dim3(grid_for(n)), dim3(256), 0, c->st, R, W, (const uint32_t*)ids,
+                           (const uint64_t*)pos, (const uint32_t*)nb, n, (unsigned long long*)pg);
+    }
+    return pg;
+}
+
+void records_from_walk(const Walk& wk, const uint32_t* id_map, std::vector<curc_alignment>& rec) {
+    rec.resize(wk.ids.size());
+    for (size_t i = 0; i < wk.ids.size(); ++i) {
+        rec[i].read_id = id_map ? id_map[wk.ids[i]] : wk.ids[i];
+        rec[i].pad = 0;
+        rec[i].word = wk.pos[i] << 2;
+    }
+}
+
+// Exact all-offset matching (a8/a11; compress.cu:1003-1174).  Initial
+// prefix/suffix lists are lexicographic; their lengths are the device
+// counters d_npre/d_nsuf, both <= n_max.
+void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const uint32_t* d_pre,
+                const uint32_t* d_suf, uint32_t* cnt, uint32_t n_max, uint32_t* next, uint32_t* prev,
+                uint16_t* offset) {
+    if (n_max == 0) return;
+    Scratch& S = c->S;
+    uint32_t* pl[2] = {S.as<uint32_t>(S_PL0, n_max), S.as<uint32_t>(S_PL1, n_max)};
+    uint64_t* pk[2] = {S.as<uint64_t>(S_PK0, n_max), S.as<uint64_t>(S_PK1, n_max)};
+    uint32_t* sl[2] = {S.as<uint32_t>(S_SL0, n_max), S.as<uint32_t>(S_SL1, n_max)};
+    uint32_t* choice = S.as<uint32_t>(S_CHOICE, n_max);
+    uint32_t* grp = S.as<uint32_t>(S_GRP, n_max);
+    uint32_t* memb = S.as<uint32_t>(S_MEMB, n_max);
+    uint32_t* resl = S.as<uint32_t>(S_RESL, n_max);
+    uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)n_max + 1);
+    uint32_t* cont = S.as<uint32_t>(S_CONT, (uint64_t)n_max + 1);
+    uint32_t* head = S.as<uint32_t>(S_HEAD, (uint64_t)n_max + 1);
+    uint32_t* rng = S.as<uint32_t>(S_RNG, (1u << 20) + 1);
+    fill<uint32_t>(c, owner, (uint64_t)n_max + 1, INV);
+    fill<uint32_t>(c, head, (uint64_t)n_max + 1, INV);
+    fill<uint32_t>(c, cont, (uint64_t)n_max + 1, 0u);
+    uint32_t* np[2] = {cnt + C_NP0, cnt + C_NP1};
+    uint32_t* ns[2] = {cnt + C_NS0, cnt + C_NS1};
+    CURC_HIP(hipMemcpyAsync(np[0], cnt + C_PRE, 4, hipMemcpyDeviceToDevice, c->st));
+    CURC_HIP(hipMemcpyAsync(ns[0], cnt + C_SUF, 4, hipMemcpyDeviceToDevice, c->st));
+    CURC_HIP(hipMemcpyAsync(pl[0], d_pre, (uint64_t)n_max * 4, hipMemcpyDeviceToDevice, c->st));
+    CURC_HIP(hipMemcpyAsync(sl[0], d_suf, (uint64_t)n_max * 4, hipMemcpyDeviceToDevice, c->st));
+    hipLaunchKernelGGL(k_gather_key0_dev, dim3(grid_for(n_max)), dim3(256), 0, c->st, R, (const uint32_t*)pl[0],
+                       (const uint32_t*)np[0], W, pk[0]);
+    const uint32_t g = grid_for(n_max, 4096);
+    int cur = 0;
+    for (uint32_t off = 1; off < L; ++off) {
+        const uint32_t bp = (L - off >= 10) ? 10 : (L - off);
+        hipLaunchKernelGGL(k_fm_ranges, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, c->st,
+                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng);
+        ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
+                      choice, grp, owner, cont, head, memb, resl, cnt + C_RESN};
+        hipLaunchKernelGGL(k_fm_probe, dim3(g), dim3(256), 0, c->st, cs, (const uint32_t*)rng, bp, off);
+        hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off);
+        hipLaunchKernelGGL(k_claim_resolve<true>, dim3(256), dim3(256), 0, c->st, cs, off);
+        compact(ns[cur], PredSufFree{sl[cur], next}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
+                S_PRIM, c->st);
+        compact(np[cur], PredPreFree{pl[cur], prev}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
+                np[1 - cur], S, S_PRIM, c->st);
+        cur = 1 - cur;
+    }
+}
+
+void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop, uint32_t flags,
+               curc_block* B) {
+    curc_params P;
+    curc_make_params(L, &P);
+    const uint32_t W = P.W;
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    const bool keep = (flags & CURC_RUN_KEEP_STAGES) != 0;
+    B->L = L;
+    B->W = W;
+    B->n_reads = N;
+    uint32_t* cnt = S.as<uint32_t>(S_CNT, C_COUNT);
+    fill<uint32_t>(c, cnt, C_COUNT, 0u);
+
+    // ---- a1/a2: N split + pack (preprocess.cpp:147-199, compress.cu:2151-2180)
+    const uint8_t* d_ascii;
+    if (flags & CURC_RUN_INPUT_ON_DEVICE) {
+        d_ascii = (const uint8_t*)ascii;
+    } else {
+        uint8_t* a = S.as<uint8_t>(S_ASCII, N * L + 16);
+        h2d(c, a, (const uint8_t*)ascii, N * L);
+        d_ascii = a;
+    }
+    uint64_t* packed = S.as<uint64_t>(S_PACKED, N * W + 1);
+    uint8_t* isN = S.as<uint8_t>(S_ISN, N + 1);
+    uint32_t* nonN = S.as<uint32_t>(S_NONN, N + 1);
+    uint32_t* Nids = S.as<uint32_t>(S_NIDS, N + 1);
+    uint32_t n, nN;
+    {
+        Region r(c, "pack");
+        fill<uint64_t>(c, packed + N * W, 1, 0ull);
+        size_t lds = ((PACK_READS * L + 15) & ~15u) + PACK_READS * W * 8;
+        uint32_t grid = (uint32_t)((N + PACK_READS - 1) / PACK_READS);
+        if (grid) {
+            if (((uintptr_t)d_ascii & 3) == 0)
+                hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), lds, st, d_ascii, N, L, W, packed, isN);
+            else
+                hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), lds, st, d_ascii, N, L, W, packed, isN);
+        }
+        set_cnt(c, cnt + C_N, (uint32_t)N);
+        compact(cnt + C_N, PredIsN{isN, false}, EmitIndex{nonN}, cnt + C_NONN, S, S_PRIM, st);
+        compact(cnt + C_N, PredIsN{isN, true}, EmitIndex{Nids}, cnt + C_NN, S, S_PRIM, st);
+        c->add_bytes("pack", (double)N * (L + 8.0 * W) + 4.0 * N, (double)N);
+    }
+    n = read_cnt(c, cnt + C_NONN);
+    nN = read_cnt(c, cnt + C_NN);
+    uint8_t* Nascii = S.as<uint8_t>(S_NASCII, (uint64_t)nN * L + 16);
+    if (nN)
+        hipLaunchKernelGGL(k_gather_ascii, dim3(grid_for((uint64_t)nN * L)), dim3(256), 0, st, d_ascii,
+                           (const uint32_t*)Nids, (uint64_t)nN, L, Nascii);
+
+    // ---- a4: dedup (compress.cu:1396-1420)
+    uint32_t* next = S.as<uint32_t>(S_NEXT, N + 1);
+    uint32_t* prev = S.as<uint32_t>(S_PREV, N + 1);
+    uint16_t* off = S.as<uint16_t>(S_OFF, N + 1);
+    uint32_t* sorted;
+    {
+        Region r(c, "dedup");
+        fill<uint32_t>(c, next, N, INV);
+        fill<uint32_t>(c, prev, N, INV);
+        fill<uint16_t>(c, off, N, INVOFF);
+        uint64_t* k0 = S.as<uint64_t>(S_K0, (uint64_t)n + 1);
+        uint64_t* k1 = S.as<uint64_t>(S_K1, (uint64_t)n + 1);
+        uint32_t* v0 = S.as<uint32_t>(S_V0, (uint64_t)n + 1);
+        uint32_t* v1 = S.as<uint32_t>(S_V1, (uint64_t)n + 1);
+        CURC_HIP(hipMemcpyAsync(v0, nonN, (uint64_t)n * 4, hipMemcpyDeviceToDevice, st));
+        // LSD over words, least significant word first; each word sorted in
+        // 8-bit digits (stable), so ties keep ascending id.
+        for (int w = (int)W - 1; w >= 0; --w) {
+            uint32_t nbases = (w == (int)W - 1) ? (L - 32 * (W - 1)) : 32;
+            uint32_t begin = ((64 - 2 * nbases) / 8) * 8;
+            hipLaunchKernelGGL(k_gather_word, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)packed,
+                               (const uint32_t*)v0, (uint64_t)n, W, (uint32_t)w, k0);
+            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, begin, 64, S, S_PRIM, st);
+        }
+        sorted = v0;
+        if (n > 1)
+            hipLaunchKernelGGL(k_dedup_link, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)packed,
+                               (const uint32_t*)sorted, (uint64_t)n, W, next, prev, off);
+        c->add_bytes("dedup", (double)n * 8 * W + 2.0 * n * (8 * W + 4), (double)n);
+    }
+    // keep the sorted ids in a stable slot (v0/v1 are reused by later sorts)
+    uint32_t* srt = S.as<uint32_t>(S_SORTED, (uint64_t)n + 1);
+    CURC_HIP(hipMemcpyAsync(srt, sorted, (uint64_t)n * 4, hipMemcpyDeviceToDevice, st));
+    sorted = srt;
+    set_cnt(c, cnt + C_TMP, n);
+    if (keep) {
+        d2h(c, B->next_dedup, next, N);
+        d2h(c, B->prev_dedup, prev, N);
+    }
+    d2h(c, B->sorted, sorted, n);
+    d2h(c, B->nonN, nonN, n);
+    d2h(c, B->Nids, Nids, nN);
+    if (stop <= CURC_STAGE_DEDUP) {
+        d2h(c, B->next, next, N);
+        d2h(c, B->prev, prev, N);
+        d2h(c, B->offset, off, N);
+        CURC_HIP(hipStreamSynchronize(st));
+        return;
+    }
+
+    // ---- a5: prefix k-mer index (compress.cu:1427-1471)
+    uint32_t* va = S.as<uint32_t>(S_VA, (uint64_t)n + 1);
+    uint32_t* sl0 = S.as<uint32_t>(S_SL0, (uint64_t)n + 1);
+    compact(cnt + C_TMP, PredLinkState{sorted, prev}, EmitGather{sorted, va}, cnt + C_NPRE, S, S_PRIM, st);
+    compact(cnt + C_TMP, PredLinkState{sorted, next}, EmitGather{sorted, sl0}, cnt + C_NSUF, S, S_PRIM, st);
+    const uint32_t npre = read_cnt(c, cnt + C_NPRE);
+    const uint32_t nsuf = read_cnt(c, cnt + C_NSUF);
+    {
+        const uint32_t H = hash_size(npre);
+        const uint32_t mask = H - 1;
+        uint32_t* kr = S.as<uint32_t>(S_KR, (uint64_t)H + 2);
+        uint64_t* vk = S.as<uint64_t>(S_VK, (uint64_t)npre + 1);
+        {
+            Region r(c, "prefix_index");
+            uint32_t* hk0 = S.as<uint32_t>(S_MK0, (uint64_t)npre + 1);
+            uint32_t* hk1 = S.as<uint32_t>(S_MK1, (uint64_t)npre + 1);
+            uint32_t* vb = S.as<uint32_t>(S_MV1, (uint64_t)npre + 1);
+            uint32_t* va_ = va;
+            hipLaunchKernelGGL(k_prefix_hash, dim3(grid_for(npre)), dim3(256), 0, st, (const uint64_t*)packed,
+                               (const uint32_t*)va, (uint64_t)npre, W, P.kip, P.k, mask, hk0);
+            radix_sort_pairs<uint32_t>(hk0, va_, hk1, vb, npre, 0, log2u(H), S, S_PRIM, st);
+            if (va_ != va) CURC_HIP(hipMemcpyAsync(va, va_, (uint64_t)npre * 4, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_gap_fill, dim3(grid_for((uint64_t)npre + 1)), dim3(256), 0, st,
+                               (const uint32_t*)hk0, (uint64_t)npre, (uint64_t)H + 1, kr);
+            hipLaunchKernelGGL(k_gather_key0, dim3(grid_for(npre)), dim3(256), 0, st, (const uint64_t*)packed,
+                               (const uint32_t*)va, (uint64_t)npre, W, vk);
+            c->add_bytes("prefix_index", (double)npre * (16 + 8 + 16) + 8.0 * H, (double)npre);
+        }
+        // ---- a6: greedy overlap loop (compress.cu:1473-1532)
+        {
+            Region r(c, "overlap");
+            uint32_t* sl[2] = {sl0, S.as<uint32_t>(S_SL1, (uint64_t)nsuf + 1)};
+            uint32_t* choice = S.as<uint32_t>(S_CHOICE, (uint64_t)nsuf + 1);
+            uint32_t* grp = S.as<uint32_t>(S_GRP, (uint64_t)nsuf + 1);
+            uint32_t* memb = S.as<uint32_t>(S_MEMB, (uint64_t)nsuf + 1);
+            uint32_t* resl = S.as<uint32_t>(S_RESL, (uint64_t)nsuf + 1);
+            uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)npre + 1);
+            uint32_t* cont = S.as<uint32_t>(S_CONT, (uint64_t)npre + 1);
+            uint32_t* head = S.as<uint32_t>(S_HEAD, (uint64_t)npre + 1);
+            fill<uint32_t>(c, owner, (uint64_t)npre + 1, INV);
+            fill<uint32_t>(c, head, (uint64_t)npre + 1, INV);
+            fill<uint32_t>(c, cont, (uint64_t)npre + 1, 0u);
+            uint32_t* ns[2] = {cnt + C_NS0, cnt + C_NS1};
+            CURC_HIP(hipMemcpyAsync(ns[0], cnt + C_NSUF, 4, hipMemcpyDeviceToDevice, st));
+            const uint32_t g = grid_for(nsuf, 4096);
+            int cur = 0;
+            for (uint32_t o = 1; o < P.max_off; ++o) {
+                ClaimState cs{packed, W, L, sl[cur], ns[cur], va, vk, cnt + C_NPRE, next, prev, off,
+                              choice, grp, owner, cont, head, memb, resl, cnt + C_RESN};
+                hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr, mask, P.k,
+                                   P.kip, o);
+                hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
+                hipLaunchKernelGGL(k_claim_resolve<false>, dim3(256), dim3(256), 0, st, cs, o);
+                compact(ns[cur], PredSufFree{sl[cur], next}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
+                        S_PRIM, st);
+                cur = 1 - cur;
+            }
+        }
+    }
+    if (keep) {
+        d2h(c, B->next_ov, next, N);
+        d2h(c, B->prev_ov, prev, N);
+        d2h(c, B->off_ov, off, N);
+    }
+    if (stop <= CURC_STAGE_OVERLAP) {
+        d2h(c, B->next, next, N);
+        d2h(c, B->prev, prev, N);
+        d2h(c, B->offset, off, N);
+        CURC_HIP(hipStreamSynchronize(st));
+        return;
+    }
+
+    // ---- a7: prune + stable partition (compress.cu:1539-1572)
+    uint32_t* hq = S.as<uint32_t>(S_HQ, (uint64_t)n + 1);
+    uint32_t* unm = S.as<uint32_t>(S_UNM, (uint64_t)n + 1);
+    uint32_t n_hq, n_unm;
+    {
+        Region r(c, "prune");
+        uint8_t* res = S.as<uint8_t>(S_RES, N + 1);
+        hipLaunchKernelGGL(k_prune_mark, dim3(grid_for(N)), dim3(256), 0, st, (const uint32_t*)next,
+                           (const uint32_t*)prev, (const uint16_t*)off, N, res);
+        hipLaunchKernelGGL(k_prune_unlink, dim3(grid_for(N)), dim3(256), 0, st, next, prev, (const uint8_t*)res, N);
+        set_cnt(c, cnt + C_TMP, n);
+        compact(cnt + C_TMP, PredRes{sorted, res, true}, EmitGather{sorted, hq}, cnt + C_HQ, S, S_PRIM, st);
+        compact(cnt + C_TMP, PredRes{sorted, res, false}, EmitGather{sorted, unm}, cnt + C_UNM, S, S_PRIM, st);
+        c->add_bytes("prune", 28.0 * N, (double)N);
+    }
+    n_hq = read_cnt(c, cnt + C_HQ);
+    n_unm = read_cnt(c, cnt + C_UNM);
+    if (keep) {
+        d2h(c, B->next_pr, next, N);
+        d2h(c, B->prev_pr, prev, N);
+    }
+    d2h(c, B->hq, hq, n_hq);
+    d2h(c, B->unmatched, unm, n_unm);
+    if (stop <= CURC_STAGE_PRUNE) {
+        d2h(c, B->next, next, N);
+        d2h(c, B->prev, prev, N);
+        d2h(c, B->offset, off, N);
+        CURC_HIP(hipStreamSynchronize(st));
+        return;
+    }
+
+    // ---- a8: exact all-offset matching of HQ chain ends (compress.cu:1574-1591)
+    {
+        Region r(c, "full_match");
+        uint32_t* pre = S.as<uint32_t>(S_PRE, (uint64_t)n_hq + 1);
+        uint32_t* suf = S.as<uint32_t>(S_SUF, (uint64_t)n_hq + 1);
+        set_cnt(c, cnt + C_TMP, n_hq);
+        compact(cnt + C_TMP, PredLinkState{hq, prev}, EmitGather{hq, pre}, cnt + C_PRE, S, S_PRIM, st);
+        compact(cnt + C_TMP, PredLinkState{hq, next}, EmitGather{hq, suf}, cnt + C_SUF, S, S_PRIM, st);
+        full_match(c, packed, W, L, pre, suf, cnt, n_hq, next, prev, off);
+    }
+    d2h(c, B->next, next, N);
+    d2h(c, B->prev, prev, N);
+    d2h(c, B->offset, off, N);
+    CURC_HIP(hipStreamSynchronize(st));
+    if (stop <= CURC_STAGE_FULLMATCH) return;
+
+    // ---- a9: pseudogenome assembly (compress.cu:1613-1618)
+    uint64_t* hq_pg;
+    {
+        Region r(c, "assembly");
+        Walk wk;
+        walk_chains(wk, B->hq.data(), B->hq.size(), B->next.data(), B->prev.data(), B->offset.data(), N, L);
+        records_from_walk(wk, nullptr, B->hq_rec);
+        B->hq_len = wk.len;
+        hq_pg = build_pg(c, packed, W, wk, S_PG_HQ);
+        d2h(c, B->hq_pg2, hq_pg, (wk.len + 31) / 32 + 1);
+        CURC_HIP(hipStreamSynchronize(st));
+    }
+    if (stop <= CURC_STAGE_ASSEMBLY) return;
+
+    // ---- a10: mapping (compress.cu:1620-1640, 268-595)
+    {
+        const uint64_t ref_len = B->hq_len;
+        const uint64_t nm = (uint64_t)n_unm + nN;
+        uint8_t* last = S.as<uint8_t>(S_LAST, nm + 1);
+        uint64_t* word = S.as<uint64_t>(S_WORD, nm + 1);
+        if (nm) hipLaunchKernelGGL(k_map_init, dim3(grid_for(nm)), dim3(256), 0, st, last, word, (uint64_t)n_unm, nm, P.max_mm);
+        uint64_t index_count = 0;
+        if (ref_len + 1 >= MAP_K) index_count = (ref_len - MAP_K + 1 + MAP_REF_STEP - 1) / MAP_REF_STEP;
+        if (L >= MAP_K && nm > 0) {
+            for (int strand = 0; strand < 2; ++strand) {
+                const bool rc = strand == 1;
+                uint64_t parts[2][2];
+                int np = 1;
+                if (index_count > (1ull << 30)) {  // compress.cu:341-353
+                    uint64_t c1 = index_count / 2;
+                    parts[0][0] = 0;
+                    parts[0][1] = c1;
+                    parts[1][0] = c1 * MAP_REF_STEP;
+                    parts[1][1] = index_count - c1;
+                    np = 2;
+                } else {
+                    parts[0][0] = 0;
+                    parts[0][1] = index_count;
+                }
+                for (int pi = 0; pi < np; ++pi) {
+                    const uint64_t start = parts[pi][0], count = parts[pi][1];
+                    const uint32_t H = hash_size(count);
+                    uint32_t* kr = S.as<uint32_t>(S_MKR, (uint64_t)H + 2);
+                    uint32_t* pos = S.as<uint32_t>(S_MPOS, count + 2);
+                    {
+                        Region r(c, "map_index");
+                        uint32_t* k0 = S.as<uint32_t>(S_MK0, count + 1);
+                        uint32_t* k1 = S.as<uint32_t>(S_MK1, count + 1);
+                        uint32_t* v0 = S.as<uint32_t>(S_MV0, count + 1);
+                        uint32_t* v1 = S.as<uint32_t>(S_MV1, count + 1);
+                        uint32_t* krf = S.as<uint32_t>(S_KRF, (uint64_t)H + 2);
+                        if (count)
+                            hipLaunchKernelGGL(k_map_hash, dim3(grid_for(count)), dim3(256), 0, st,
+                                               (const uint64_t*)hq_pg, start, count, rc, H - 1, k0, v0);
+                        radix_sort_pairs<uint32_t>(k0, v0, k1, v1, count, 0, log2u(H), S, S_PRIM, st);
+                        hipLaunchKernelGGL(k_gap_fill, dim3(grid_for(count + 1)), dim3(256), 0, st,
+                                           (const uint32_t*)k0, count, (uint64_t)H + 1, krf);
+                        hipLaunchKernelGGL(k_map_caps, dim3(grid_for((uint64_t)H + 2)), dim3(256), 0, st,
+                                           (const uint32_t*)krf, (uint64_t)H, kr);
+                        scan_excl(kr, kr, (uint64_t)H + 2, S, S_PRIM, st);
+                        if (count)
+                            hipLaunchKernelGGL(k_map_fill, dim3(grid_for(count)), dim3(256), 0, st,
+                                               (const uint32_t*)k0, (const uint32_t*)v0, count,
+                                               (const uint32_t*)krf, (const uint32_t*)kr, pos);
+                        c->add_bytes("map_index", (double)count * (16 + 8) + 8.0 * H, (double)count);
+                    }
+                    {
+                        Region r(c, "mapping");
+                        MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
+                                  Nascii, nN, last, word};
+                        if (rc)
+                            hipLaunchKernelGGL(k_map<true>, dim3(grid_for(nm, 1u << 20)), dim3(256), 0, st, a);
+                        else
+                            hipLaunchKernelGGL(k_map<false>, dim3(grid_for(nm, 1u << 20)), dim3(256), 0, st, a);
+                    }
+                }
+            }
+        }
+        // compress.cu:524-594 (get_result)
+        uint32_t* rid = S.as<uint32_t>(S_MREC_ID, nm + 1);
+        uint64_t* rw = S.as<uint64_t>(S_MREC_W, nm + 1);
+        uint8_t* mis = S.as<uint8_t>(S_MIS, nm + 1);
+        uint32_t* um = S.as<uint32_t>(S_UNMAPPED, (uint64_t)n_unm + 1);
+        uint32_t* umN = S.as<uint32_t>(S_UNMAPPED_N, (uint64_t)nN + 1);
+        set_cnt(c, cnt + C_TMP, (uint32_t)nm);
+        compact(cnt + C_TMP, PredMapped{last, P.max_mm, 0, true},
+                EmitMapRec{last, word, unm, n_unm, rid, rw, mis}, cnt + C_NMAP, S, S_PRIM, st);
+        set_cnt(c, cnt + C_TMP, n_unm);
+        compact(cnt + C_TMP, PredMapped{last, P.max_mm, 0, false}, EmitUnmapped{unm, um}, cnt + C_NUNMAP, S,
+                S_PRIM, st);
+        set_cnt(c, cnt + C_TMP, nN);
+        compact(cnt + C_TMP, PredMapped{last, P.max_mm, n_unm, false}, EmitUnmapped{nullptr, umN},
+                cnt + C_NUNMAPN, S, S_PRIM, st);
+        const uint32_t nmap = read_cnt(c, cnt + C_NMAP);
+        const uint32_t nun = read_cnt(c, cnt + C_NUNMAP);
+        const uint32_t nunN = read_cnt(c, cnt + C_NUNMAPN);
+        std::vector<uint32_t> hid;
+        std::vector<uint64_t> hw;
+        d2h(c, hid, rid, nmap);
+        d2h(c, hw, rw, nmap);
+        d2h(c, B->map_mis, mis, nmap);
+        d2h(c, B->unmapped, um, nun);
+        d2h(c, B->unmapped_N, umN, nunN);
+        CURC_HIP(hipStreamSynchronize(st));
+        B->map_rec.resize(nmap);
+        for (uint32_t i = 0; i < nmap; ++i) {
+            B->map_rec[i].read_id = hid[i];
+            B->map_rec[i].pad = 0;
+            B->map_rec[i].word = hw[i];
+        }
+    }
+    if (stop <= CURC_STAGE_MAP) return;
+
+    // ---- a11: LQ pseudogenome (compress.cu:1649-1722)
+    const uint32_t n_lq = (uint32_t)B->unmapped.size();
+    if (n_lq) {
+        Region r(c, "lq");
+        uint64_t* lqp = S.as<uint64_t>(S_LQPACKED, (uint64_t)n_lq * W + 1);
+        uint32_t* um = S.as<uint32_t>(S_UNMAPPED, (uint64_t)n_lq + 1);
+        hipLaunchKernelGGL(k_gather_packed, dim3(grid_for((uint64_t)n_lq * W)), dim3(256), 0, st,
+                           (const uint64_t*)packed, (const uint32_t*)um, (uint64_t)n_lq, W, lqp);
+        fill<uint64_t>(c, lqp + (uint64_t)n_lq * W, 1, 0ull);
+        uint32_t* lnext = S.as<uint32_t>(S_LQNEXT, n_lq);
+        uint32_t* lprev = S.as<uint32_t>(S_LQPREV, n_lq);
+        uint16_t* loff = S.as<uint16_t>(S_LQOFF, n_lq);
+        fill<uint32_t>(c, lnext, n_lq, INV);
+        fill<uint32_t>(c, lprev, n_lq, INV);
+        fill<uint16_t>(c, loff, n_lq, INVOFF);
+        uint32_t* ids = S.as<uint32_t>(S_PRE, n_lq);
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(n_lq)), dim3(256), 0, st, ids, (uint64_t)n_lq);
+        set_cnt(c, cnt + C_PRE, n_lq);
+        set_cnt(c, cnt + C_SUF, n_lq);
+        full_match(c, lqp, W, L, ids, ids, cnt, n_lq, lnext, lprev, loff);
+        d2h(c, B->lq_next, lnext, n_lq);
+        d2h(c, B->lq_prev, lprev, n_lq);
+        d2h(c, B->lq_off, loff, n_lq);
+        CURC_HIP(hipStreamSynchronize(st));
+        std::vector<uint32_t> visit(n_lq);
+        for (uint32_t i = 0; i < n_lq; ++i) visit[i] = i;
+        Walk wk;
+        walk_chains(wk, visit.data(), n_lq, B->lq_next.data(), B->lq_prev.data(), B->lq_off.data(), n_lq, L);
+        records_from_walk(wk, B->unmapped.data(), B->lq_rec);
+        B->lq_len = wk.len;
+        uint64_t* lpg = build_pg(c, lqp, W, wk, S_PG_LQ);
+        d2h(c, B->lq_pg2, lpg, (wk.len + 31) / 32 + 1);
+        CURC_HIP(hipStreamSynchronize(st));
+    }
+}
+
+int fail(curc_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int curc_make_params(uint32_t L, curc_params* p) {
+    // preprocess.cpp:296-318
+    if (!p || L == 0 || L > 512) return CURC_EINVAL;
+    p->L = L;
+    p->W = (L + 31) / 32;
+    p->max_off = L * 65 / 100;
+    p->k = (L < 89) ? 36 * L / 100 : 32;
+    if (p->k == 0 || p->k + p->max_off > L) return CURC_EINVAL;
+    p->kip = L - p->k - p->max_off;
+    p->max_mm = L / 5;
+    return CURC_OK;
+}
+
+int curc_device_count(int* count) {
+    if (!count) return CURC_EINVAL;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return CURC_EHIP;
+    }
+    *count = n;
+    return CURC_OK;
+}
+
+curc_ctx* curc_ctx_create(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    curc_ctx* c = new curc_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void curc_ctx_destroy(curc_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->st);
+    c->S.release();
+    for (auto& p : c->pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char* curc_ctx_last_error(curc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int curc_ctx_trim(curc_ctx* c) {
+    if (!c) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->st);
+    c->S.release();
+    return CURC_OK;
+}
+
+int curc_host_alloc(curc_ctx* c, void** p, uint64_t bytes) {
+    if (!c || !p) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    if (hipHostMalloc(p, bytes, hipHostMallocPortable) != hipSuccess) return fail(c, CURC_ENOMEM, "hipHostMalloc failed");
+    return CURC_OK;
+}
+int curc_host_free(curc_ctx* c, void* p) {
+    if (!c) return CURC_EINVAL;
+    return hipHostFree(p) == hipSuccess ? CURC_OK : fail(c, CURC_EHIP, "hipHostFree failed");
+}
+int curc_dev_alloc(curc_ctx* c, void** p, uint64_t bytes) {
+    if (!c || !p) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    if (hipMalloc(p, bytes) != hipSuccess) return fail(c, CURC_ENOMEM, "hipMalloc failed");
+    return CURC_OK;
+}
+int curc_dev_free(curc_ctx* c, void* p) {
+    if (!c) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    return hipFree(p) == hipSuccess ? CURC_OK : fail(c, CURC_EHIP, "hipFree failed");
+}
+int curc_memcpy_h2d(curc_ctx* c, void* d, const void* s, uint64_t bytes) {
+    if (!c) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    if (hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+        return fail(c, CURC_EHIP, "host to device copy failed");
+    return CURC_OK;
+}
+int curc_memcpy_d2h(curc_ctx* c, void* d, const void* s, uint64_t bytes) {
+    if (!c) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    if (hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+        return fail(c, CURC_EHIP, "device to host copy failed");
+    return CURC_OK;
+}
+int curc_sync(curc_ctx* c) {
+    if (!c) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    return hipStreamSynchronize(c->st) == hipSuccess ? CURC_OK : fail(c, CURC_EHIP, "stream sync failed");
+}
+
+int curc_pack(curc_ctx* c, const char* d_ascii, uint64_t n, uint32_t L, uint64_t* d_packed, uint8_t* d_isN) {
+    if (!c || (!d_ascii && n) || (!d_packed && n) || L == 0 || L > 512) return CURC_EINVAL;
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        const uint32_t W = (L + 31) / 32;
+        size_t lds = ((PACK_READS * L + 15) & ~15u) + PACK_READS * W * 8;
+        uint32_t grid = (uint32_t)((n + PACK_READS - 1) / PACK_READS);
+        if (grid) {
+            if (((uintptr_t)d_ascii & 3) == 0)
+                hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), lds, c->st, (const uint8_t*)d_ascii, n, L,
+                                   W, d_packed, d_isN);
+            else
+                hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), lds, c->st, (const uint8_t*)d_ascii, n,
+                                   L, W, d_packed, d_isN);
+            CURC_HIP(hipGetLastError());
+        }
+        CURC_HIP(hipStreamSynchronize(c->st));
+    } catch (const std::exception& e) {
+        return fail(c, CURC_EHIP, e.what());
+    }
+    return CURC_OK;
+}
+
+int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L, int stop_after, uint32_t flags,
+                   curc_block** out) {
+    if (!c || !out) return CURC_EINVAL;
+    *out = nullptr;
+    curc_params P;
+    if (curc_make_params(L, &P) != CURC_OK) return fail(c, CURC_EINVAL, "unsupported read length");
+    if (n_reads >= (uint64_t)INV) return fail(c, CURC_ELIMIT, "a block holds at most 2^32-2 reads");
+    if (n_reads && !ascii) return fail(c, CURC_EINVAL, "null input");
+    curc_block* B = new curc_block();
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        run_block(c, ascii, n_reads, L, stop_after <= 0 ? CURC_STAGE_ALL : stop_after, flags, B);
+        CURC_HIP(hipGetLastError());
+        CURC_HIP(hipStreamSynchronize(c->st));
+        c->harvest();
+        if (flags & CURC_RUN_KEEP_STAGES) {
+            // expose the packed reads and N-read text for checking
+            d2h(c, B->packed, c->S.as<uint64_t>(S_PACKED, n_reads * P.W + 1), n_reads * P.W);
+            std::vector<uint8_t> na;
+            d2h(c, na, c->S.as<uint8_t>(S_NASCII, (uint64_t)B->Nids.size() * L + 16), (uint64_t)B->Nids.size() * L);
+            CURC_HIP(hipStreamSynchronize(c->st));
+            B->Nascii.assign(na.begin(), na.end());
+        }
+    } catch (const std::exception& e) {
+        delete B;
+        hipStreamSynchronize(c->st);
+        c->pending.clear();
+        std::string m = e.what();
+        return fail(c, m.find("out of memory") != std::string::npos ? CURC_ENOMEM : CURC_EHIP, m);
+    }
+    *out = B;
+    return CURC_OK;
+}
+
+uint64_t curc_block_get(curc_block* B, int field, const void** ptr) {
+    if (!B || !ptr) return 0;
+    auto put = [&](const auto& v) -> uint64_t {
+        *ptr = v.data();
+        return v.size();
+    };
+    switch (field) {
+        case CURC_F_PACKED: return put(B->packed);
+        case CURC_F_NONN_IDS: return put(B->nonN);
+        case CURC_F_N_IDS: return put(B->Nids);
+        case CURC_F_SORTED_IDS: return put(B->sorted);
+        case CURC_F_NEXT_DEDUP: return put(B->next_dedup);
+        case CURC_F_PREV_DEDUP: return put(B->prev_dedup);
+        case CURC_F_NEXT_OVERLAP: return put(B->next_ov);
+        case CURC_F_PREV_OVERLAP: return put(B->prev_ov);
+        case CURC_F_OFFSET_OVERLAP: return put(B->off_ov);
+        case CURC_F_NEXT_PRUNE: return put(B->next_pr);
+        case CURC_F_PREV_PRUNE: return put(B->prev_pr);
+        case CURC_F_NEXT: return put(B->next);
+        case CURC_F_PREV: return put(B->prev);
+        case CURC_F_OFFSET: return put(B->offset);
+        case CURC_F_HQ_IDS: return put(B->hq);
+        case CURC_F_UNMATCHED_IDS: return put(B->unmatched);
+        case CURC_F_HQ_PG:
+            if (!B->hq_text_ok) {
+                B->hq_text = B->hq_pg2.empty() ? std::string() : decode_pg(B->hq_pg2, B->hq_len);
+                B->hq_text_ok = true;
+            }
+            return put(B->hq_text);
+        case CURC_F_LQ_PG:
+            if (!B->lq_text_ok) {
+                B->lq_text = B->lq_pg2.empty() ? std::string() : decode_pg(B->lq_pg2, B->lq_len);
+                B->lq_text_ok = true;
+            }
+            return put(B->lq_text);
+        case CURC_F_HQ_PG2: return put(B->hq_pg2);
+        case CURC_F_HQ_REC: return put(B->hq_rec);
+        case CURC_F_MAP_REC: return put(B->map_rec);
+        case CURC_F_MAP_MIS: return put(B->map_mis);
+        case CURC_F_UNMAPPED: return put(B->unmapped);
+        case CURC_F_UNMAPPED_N: return put(B->unmapped_N);
+        case CURC_F_LQ_REC: return put(B->lq_rec);
+        case CURC_F_LQ_NEXT: return put(B->lq_next);
+        case CURC_F_LQ_PREV: return put(B->lq_prev);
+        case CURC_F_LQ_OFFSET: return put(B->lq_off);
+        case CURC_F_N_ASCII: return put(B->Nascii);
+        default: *ptr = nullptr; return 0;
+    }
+}
+
+void curc_block_free(curc_block* B) { delete B; }
+
+int curc_ctx_set_timing(curc_ctx* c, int on) {
+    if (!c) return CURC_EINVAL;
+    c->timing = on != 0;
+    return CURC_OK;
+}
+
+int curc_ctx_stats(curc_ctx* c, curc_stage_stat* out, int max) {
+    if (!c) return CURC_EINVAL;
+    int n = (int)c->stats.size();
+    for (int i = 0; i < n && i < max; ++i) {
+        memset(&out[i], 0, sizeof(out[i]));
+        strncpy(out[i].name, c->stats[i].name.c_str(), sizeof(out[i].name) - 1);
+        out[i].ms = c->stats[i].ms;
+        out[i].launches = c->stats[i].launches;
+        out[i].alg_bytes = c->stats[i].alg_bytes;
+        out[i].items = c->stats[i].items;
+    }
+    return n;
+}
+
+void curc_ctx_reset_stats(curc_ctx* c) {
+    if (c) c->stats.clear();
+}
+
+}  // extern "C"

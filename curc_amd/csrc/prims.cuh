@@ -1,0 +1,318 @@
+// Device-wide primitives written for wave64 / gfx950: exclusive scan,
+// stable LSD radix sort of (key, u32 value) pairs, and stable compaction
+// whose element count lives in device memory (so loops over offsets need no
+// host round trip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "common.cuh"
+
+namespace curc {
+
+#define CURC_HIP(x)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) +     \
+                                     " at " + __FILE__ + ":" + std::to_string(__LINE__));    \
+    } while (0)
+
+// Growable device scratch owned by a context: get(slot, bytes) returns a
+// buffer of at least `bytes`, reallocating only when it must grow.
+struct Scratch {
+    static constexpr int SLOTS = 128;
+    void* ptr[SLOTS] = {};
+    size_t cap[SLOTS] = {};
+    size_t total = 0;
+    void* get(int slot, size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        if (cap[slot] < bytes) {
+            if (ptr[slot]) CURC_HIP(hipFree(ptr[slot]));
+            total -= cap[slot];
+            size_t c = bytes + bytes / 8 + 256;
+            CURC_HIP(hipMalloc(&ptr[slot], c));
+            cap[slot] = c;
+            total += c;
+        }
+        return ptr[slot];
+    }
+    template <typename T>
+    T* as(int slot, size_t count) {
+        return static_cast<T*>(get(slot, count * sizeof(T)));
+    }
+    void release() {
+        for (int i = 0; i < SLOTS; ++i) {
+            if (ptr[i]) hipFree(ptr[i]);
+            ptr[i] = nullptr;
+            cap[i] = 0;
+        }
+        total = 0;
+    }
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    uint32_t l = __lane_id();
+    return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
+}
+
+// Inclusive wave64 scan of a u32.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    uint32_t lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+// Exclusive block scan for 256 threads (4 waves); returns the exclusive
+// prefix and writes the block total to *total.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* lds4, uint32_t* total) {
+    uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) lds4[w] = inc;
+    __syncthreads();
+    uint32_t wo = 0, tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        uint32_t s = lds4[i];
+        if (i < w) wo += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wo + inc - v;
+}
+
+// ---------------------------------------------------------------- scan ----
+constexpr uint32_t SCAN_ITEMS = 8;
+constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                     uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lds[4];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
+        uint64_t i = base + r * 256 + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_excl_scan256(s, lds, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Exclusive scan of one tile per block plus the block offset.  Thread t owns
+// elements [t*ITEMS, t*ITEMS+ITEMS) of the tile, staged through LDS so global
+// loads stay coalesced.
+__global__ void __launch_bounds__(256) k_scan_apply(const uint32_t* __restrict__ in,
+                                                    uint32_t* __restrict__ out, uint64_t n,
+                                                    const uint32_t* __restrict__ offs) {
+    __shared__ uint32_t tile[SCAN_TILE];
+    __shared__ uint32_t lds[4];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
+        uint64_t i = base + r * 256 + threadIdx.x;
+        tile[r * 256 + threadIdx.x] = (i < n) ? in[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
+        v[r] = tile[threadIdx.x * SCAN_ITEMS + r];
+        s += v[r];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan256(s, lds, &tot) + (offs ? offs[blockIdx.x] : 0u);
+#pragma unroll
+    for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
+        tile[threadIdx.x * SCAN_ITEMS + r] = ex;
+        ex += v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
+        uint64_t i = base + r * 256 + threadIdx.x;
+        if (i < n) out[i] = tile[r * 256 + threadIdx.x];
+    }
+}
+
+// Exclusive scan of n u32 (in may equal out).  Uses scratch slots
+// [slot, slot+depth).
+inline void scan_excl(const uint32_t* in, uint32_t* out, uint64_t n, Scratch& S, int slot,
+                      hipStream_t st) {
+    if (n == 0) return;
+    uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles == 1) {
+        hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(256), 0, st, in, out, n, (const uint32_t*)nullptr);
+        return;
+    }
+    uint32_t* sums = S.as<uint32_t>(slot, tiles);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)tiles), dim3(256), 0, st, in, n, sums);
+    scan_excl(sums, sums, tiles, S, slot + 1, st);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)tiles), dim3(256), 0, st, in, out, n,
+                       (const uint32_t*)sums);
+}
+
+// ----------------------------------------------------------- radix sort ----
+constexpr uint32_t RADIX_ITEMS = 16;
+constexpr uint32_t RADIX_TILE = 256 * RADIX_ITEMS;
+
+template <typename K>
+__global__ void __launch_bounds__(256) k_radix_hist(const K* __restrict__ keys, uint64_t n, uint32_t shift,
+                                                    uint32_t* __restrict__ hist, uint32_t tiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * RADIX_TILE;
+#pragma unroll 4
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        uint64_t i = base + r * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: within a tile, element order is (round, wave, lane); each
+// wave ranks its lanes by digit with 8 ballots, waves are ordered through
+// per-wave digit counts in LDS.
+template <typename K>
+__global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                       K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                       uint64_t n, uint32_t shift,
+                                                       const uint32_t* __restrict__ offs, uint32_t tiles) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const uint32_t t = threadIdx.x, w = t >> 6;
+    run[t] = offs[(uint64_t)t * tiles + blockIdx.x];
+    wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * RADIX_TILE;
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+        uint64_t i = base + r * 256 + t;
+        bool valid = i < n;
+        K k = valid ? kin[i] : (K)0;
+        uint32_t v = valid ? vin[i] : 0u;
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (uint32_t q = 0; q < w; ++q) pos += wcnt[q][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
+        __syncthreads();
+    }
+}
+
+// Sorts n (key, value) pairs by key bits [begin_bit, end_bit) in 8-bit
+// digits, stably.  Ping-pongs between (k, v) and (kt, vt); on return k and v
+// point at the sorted data.
+template <typename K>
+void radix_sort_pairs(K*& k, uint32_t*& v, K*& kt, uint32_t*& vt, uint64_t n, uint32_t begin_bit,
+                      uint32_t end_bit, Scratch& S, int slot, hipStream_t st) {
+    if (n <= 1) return;
+    uint32_t tiles = (uint32_t)((n + RADIX_TILE - 1) / RADIX_TILE);
+    uint32_t* hist = S.as<uint32_t>(slot, (size_t)256 * tiles);
+    for (uint32_t sh = begin_bit; sh < end_bit; sh += 8) {
+        hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(256), 0, st, (const K*)k, n, sh, hist, tiles);
+        scan_excl(hist, hist, (uint64_t)256 * tiles, S, slot + 1, st);
+        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(256), 0, st, (const K*)k, (const uint32_t*)v,
+                           kt, vt, n, sh, (const uint32_t*)hist, tiles);
+        K* tk = k; k = kt; kt = tk;
+        uint32_t* tv = v; v = vt; vt = tv;
+    }
+}
+
+// ------------------------------------------------------- compaction ------
+// Stable compaction over [0, *d_n) with a fixed grid: block b owns a
+// contiguous chunk, so output order equals input order.
+constexpr uint32_t COMPACT_GRID = 1024;
+
+template <typename Pred>
+__global__ void __launch_bounds__(256) k_compact_count(const uint32_t* __restrict__ d_n, Pred pred,
+                                                       uint32_t* __restrict__ counts) {
+    __shared__ uint32_t lds[4];
+    uint32_t n = *d_n;
+    uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    uint32_t lo = blockIdx.x * chunk;
+    uint32_t hi = min(n, lo + chunk);
+    uint32_t c = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) c += pred(i) ? 1u : 0u;
+    uint32_t tot;
+    block_excl_scan256(c, lds, &tot);
+    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_compact_offsets(uint32_t* __restrict__ counts, uint32_t nblk,
+                                                          uint32_t* __restrict__ d_out_n) {
+    // nblk <= 1024: one element per thread.
+    __shared__ uint32_t lds[16];
+    uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
+    uint32_t v = (t < nblk) ? counts[t] : 0u;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    uint32_t wo = 0, tot = 0;
+    for (uint32_t i = 0; i < 16; ++i) {
+        if (i < w) wo += lds[i];
+        tot += lds[i];
+    }
+    if (t < nblk) counts[t] = wo + inc - v;
+    if (t == 0) *d_out_n = tot;
+}
+
+template <typename Pred, typename Emit>
+__global__ void __launch_bounds__(256) k_compact_scatter(const uint32_t* __restrict__ d_n, Pred pred, Emit emit,
+                                                         const uint32_t* __restrict__ offs) {
+    __shared__ uint32_t lds[4];
+    uint32_t n = *d_n;
+    uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    uint32_t lo = blockIdx.x * chunk;
+    uint32_t hi = min(n, lo + chunk);
+    uint32_t o = offs[blockIdx.x];
+    for (uint32_t i0 = lo; i0 < hi; i0 += 256) {
+        uint32_t i = i0 + threadIdx.x;
+        bool f = (i < hi) && pred(i);
+        uint32_t tot;
+        uint32_t ex = block_excl_scan256(f ? 1u : 0u, lds, &tot);
+        if (f) emit(i, o + ex);
+        o += tot;
+    }
+}
+
+// Stable compaction: emit(i, j) is called for the j-th i in [0, *d_n) with
+// pred(i); the number of emitted items is written to *d_out_n (device).
+template <typename Pred, typename Emit>
+void compact(const uint32_t* d_n, Pred pred, Emit emit, uint32_t* d_out_n, Scratch& S, int slot,
+             hipStream_t st, uint32_t grid = COMPACT_GRID) {
+    uint32_t* counts = S.as<uint32_t>(slot, grid);
+    hipLaunchKernelGGL(k_compact_count<Pred>, dim3(grid), dim3(256), 0, st, d_n, pred, counts);
+    hipLaunchKernelGGL(k_compact_offsets, dim3(1), dim3(1024), 0, st, counts, grid, d_out_n);
+    hipLaunchKernelGGL((k_compact_scatter<Pred, Emit>), dim3(grid), dim3(256), 0, st, d_n, pred, emit,
+                       (const uint32_t*)counts);
+}
+
+}  // namespace curc

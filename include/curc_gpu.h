@@ -1,0 +1,182 @@
+/*
+ * libcurc_gpu: MI355X (gfx950) implementation of CURC's pseudogenome hot path
+ * behind a C ABI.
+ *
+ * The reference's GPU/host boundary is src/compress.cuh (C++ linkage):
+ *   block_compress(...)             compress.cuh:48-54
+ *   encode(...)                     compress.cuh:56-57
+ *   get_device_count_and_init(...)  compress.cuh:59
+ *   gpu_malloc_host / gpu_malloc / gpu_free_host / gpu_free /
+ *   gpu_mem_H2D / gpu_mem_D2H       compress.cuh:61-71
+ * Each declaration below names the reference entry it replaces.
+ *
+ * Conventions: every function returns an int status (CURC_OK = 0) and never
+ * exits the process; the caller owns every buffer it passes in, and the
+ * library never frees caller memory; results land in caller buffers or in a
+ * curc_block handle released with curc_block_free.  One context per host
+ * thread; contexts are independent.
+ */
+#ifndef CURC_GPU_H
+#define CURC_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CURC_OK 0
+#define CURC_EINVAL -1 /* bad argument or unsupported input (e.g. L > 512) */
+#define CURC_EHIP -2   /* HIP runtime error; text in curc_ctx_last_error */
+#define CURC_ENOMEM -3 /* device or host allocation failed */
+#define CURC_ELIMIT -4 /* input exceeds a documented limit */
+
+/* Derived per-read-length parameters (preprocess.cpp:296-313, Param.hpp:47-76). */
+typedef struct curc_params {
+    uint32_t L;       /* read length, 1..512 */
+    uint32_t W;       /* 64-bit words per packed read, ceil(L/32) */
+    uint32_t max_off; /* floor(65 L / 100) */
+    uint32_t k;       /* overlap k-mer: 32 if L >= 89 else floor(36 L / 100) */
+    uint32_t kip;     /* k-mer index position, L - k - max_off */
+    uint32_t max_mm;  /* mapping mismatch limit, floor(L / 5) */
+} curc_params;
+
+int curc_make_params(uint32_t L, curc_params *p);
+
+/* 16-byte record, layout-compatible with AlignmentRecord (compress.cu:68-77):
+ * word = is_contain_N | strand_id << 1 | pos << 2. */
+typedef struct curc_alignment {
+    uint32_t read_id;
+    uint32_t pad;
+    uint64_t word;
+} curc_alignment;
+
+typedef struct curc_ctx curc_ctx;
+
+/* Replaces get_device_count_and_init (compress.cu:2183-2193). */
+int curc_device_count(int *count);
+/* One context per device and host thread; holds a stream and device scratch. */
+curc_ctx *curc_ctx_create(int device);
+void curc_ctx_destroy(curc_ctx *ctx);
+const char *curc_ctx_last_error(curc_ctx *ctx);
+/* Drops cached device scratch (it is otherwise kept for the next block). */
+int curc_ctx_trim(curc_ctx *ctx);
+
+/* Memory helpers; replace gpu_malloc_host / gpu_malloc / gpu_free_host /
+ * gpu_free / gpu_mem_H2D / gpu_mem_D2H (compress.cu:2195-2220). */
+int curc_host_alloc(curc_ctx *ctx, void **ptr, uint64_t bytes);
+int curc_host_free(curc_ctx *ctx, void *ptr);
+int curc_dev_alloc(curc_ctx *ctx, void **ptr, uint64_t bytes);
+int curc_dev_free(curc_ctx *ctx, void *ptr);
+int curc_memcpy_h2d(curc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int curc_memcpy_d2h(curc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int curc_sync(curc_ctx *ctx);
+
+/* 2-bit pack of n ASCII reads of length L (all device pointers); replaces
+ * encode (compress.cu:2151-2180).  d_packed holds n*ceil(L/32) words; d_isN
+ * (optional) receives 1 for reads containing 'N' (preprocess.cpp:151). */
+int curc_pack(curc_ctx *ctx, const char *d_ascii, uint64_t n, uint32_t L, uint64_t *d_packed,
+              uint8_t *d_isN);
+
+/* Stages of one block (compress.cu:1349-1728).  curc_block_run stops after
+ * the requested stage. */
+enum {
+    CURC_STAGE_DEDUP = 1,     /* a4: duplicate links */
+    CURC_STAGE_OVERLAP = 2,   /* a5-a6: prefix index + greedy overlap */
+    CURC_STAGE_PRUNE = 3,     /* a7: prune + HQ/unmatched partition */
+    CURC_STAGE_FULLMATCH = 4, /* a8: exact all-offset matching of HQ ends */
+    CURC_STAGE_ASSEMBLY = 5,  /* a9: HQ pseudogenome */
+    CURC_STAGE_MAP = 6,       /* a10: mapping onto the HQ pseudogenome */
+    CURC_STAGE_LQ = 7,        /* a11: LQ pseudogenome of unmapped reads */
+    CURC_STAGE_ALL = 7
+};
+
+/* Result fields (numbering shared with the oracle's field ids). */
+enum {
+    CURC_F_PACKED = 0,         /* u64[n_reads * W] */
+    CURC_F_NONN_IDS = 1,       /* u32 */
+    CURC_F_N_IDS = 2,          /* u32 */
+    CURC_F_SORTED_IDS = 3,     /* u32 non-N ids in dedup order */
+    CURC_F_NEXT_DEDUP = 4,     /* u32[n_reads] */
+    CURC_F_NEXT_OVERLAP = 5,   /* u32[n_reads] */
+    CURC_F_PREV_OVERLAP = 6,   /* u32[n_reads] */
+    CURC_F_OFFSET_OVERLAP = 7, /* u16[n_reads] */
+    CURC_F_NEXT = 8,           /* u32[n_reads] final */
+    CURC_F_PREV = 9,
+    CURC_F_OFFSET = 10,        /* u16 */
+    CURC_F_HQ_IDS = 11,        /* u32 */
+    CURC_F_UNMATCHED_IDS = 12, /* u32 */
+    CURC_F_HQ_PG = 13,         /* char, HQ pseudogenome text */
+    CURC_F_HQ_REC = 14,        /* curc_alignment */
+    CURC_F_MAP_REC = 15,       /* curc_alignment */
+    CURC_F_MAP_MIS = 16,       /* u8 mismatch count per map record */
+    CURC_F_UNMAPPED = 17,      /* u32 global ids of unmapped non-N reads */
+    CURC_F_UNMAPPED_N = 18,    /* u32 N indices of unmapped N-reads */
+    CURC_F_LQ_PG = 19,         /* char */
+    CURC_F_LQ_REC = 20,        /* curc_alignment, global ids */
+    CURC_F_PREV_DEDUP = 23,
+    CURC_F_NEXT_PRUNE = 24,
+    CURC_F_PREV_PRUNE = 25,
+    CURC_F_HQ_PG2 = 26,        /* u64 2-bit HQ pseudogenome, ceil(len/32)+1 words */
+    CURC_F_LQ_NEXT = 27,       /* u32, local ids of the LQ graph */
+    CURC_F_LQ_PREV = 28,
+    CURC_F_LQ_OFFSET = 29,     /* u16 */
+    CURC_F_N_ASCII = 30        /* char[n_N * L] N-reads in N-index order */
+};
+
+/* Flags for curc_block_run. */
+#define CURC_RUN_INPUT_ON_DEVICE 1u /* `ascii` is a device pointer */
+#define CURC_RUN_KEEP_STAGES 2u     /* also return the per-stage link snapshots */
+
+typedef struct curc_block curc_block;
+
+/* GPU part of block_compress (compress.cu:1349-1722): N-split and pack,
+ * dedup, prefix index, greedy overlap, prune/partition, exact all-offset
+ * matching, assembly, mapping and the LQ pseudogenome, for n_reads ASCII
+ * reads of length L (block-global ids = input order).  The N-read LQ
+ * pseudogenome, record sorting, stream building and entropy coding stay on
+ * the CPU as in the reference. */
+int curc_block_run(curc_ctx *ctx, const char *ascii, uint64_t n_reads, uint32_t L,
+                   int stop_after, uint32_t flags, curc_block **out);
+uint64_t curc_block_get(curc_block *b, int field, const void **ptr);
+void curc_block_free(curc_block *b);
+
+/* Per-stage timing (HIP events on the context's stream) and algorithmic
+ * byte counts accumulated since the last reset. */
+typedef struct curc_stage_stat {
+    char name[32];
+    double ms;         /* summed event time */
+    uint64_t launches; /* timed regions recorded */
+    double alg_bytes;  /* algorithmic bytes moved by the stage */
+    double items;      /* work items (probes, candidates, ...) */
+} curc_stage_stat;
+
+int curc_ctx_set_timing(curc_ctx *ctx, int on);
+int curc_ctx_stats(curc_ctx *ctx, curc_stage_stat *out, int max_stats);
+void curc_ctx_reset_stats(curc_ctx *ctx);
+
+/* Seeded synthetic workloads (no reference counterpart; test/bench input). */
+typedef struct curc_synth_cfg {
+    uint64_t seed;
+    uint64_t genome_len;
+    double gc;             /* P(G)+P(C) */
+    double repeat_frac;    /* fraction of the genome covered by repeat copies */
+    uint32_t rep_min, rep_max;
+    double div_min, div_max;
+    uint32_t rep_families;
+    uint64_t n_reads;      /* total reads (pairs interleaved when paired) */
+    uint32_t L;
+    int32_t paired;
+    double frag_mean, frag_sd;
+    double rc_prob;
+    double sub_first, sub_last; /* substitution rate at base 1 and base L, linear between */
+    double n_rate;
+} curc_synth_cfg;
+
+void curc_synth_default(curc_synth_cfg *cfg);
+int curc_synth_genome(const curc_synth_cfg *cfg, char *genome);
+int curc_synth_reads(const curc_synth_cfg *cfg, const char *genome, char *reads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+"""ctypes binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product package never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBS = {"st": os.path.join(HERE, "liboracle_st.so"), "omp": os.path.join(HERE, "liboracle_omp.so")}
+
+FIELDS = {
+    "packed": (0, np.uint64), "nonN_ids": (1, np.uint32), "N_ids": (2, np.uint32),
+    "sorted_ids": (3, np.uint32), "next_dedup": (4, np.uint32), "next_overlap": (5, np.uint32),
+    "prev_overlap": (6, np.uint32), "offset_overlap": (7, np.uint16), "next": (8, np.uint32),
+    "prev": (9, np.uint32), "offset": (10, np.uint16), "hq_ids": (11, np.uint32),
+    "unmatched_ids": (12, np.uint32), "hq_pg": (13, np.uint8), "hq_rec": (14, "rec"),
+    "map_rec": (15, "rec"), "map_mis": (16, np.uint8), "unmapped": (17, np.uint32),
+    "unmapped_N": (18, np.uint32), "lq_pg": (19, np.uint8), "lq_rec": (20, "rec"),
+    "nlq_pg": (21, np.uint8), "nlq_rec": (22, "rec"), "prev_dedup": (23, np.uint32),
+    "next_prune": (24, np.uint32), "prev_prune": (25, np.uint32), "hq_pg2": (26, np.uint64),
+    "lq_next": (27, np.uint32), "lq_prev": (28, np.uint32), "lq_offset": (29, np.uint16),
+}
+REC_DTYPE = np.dtype([("read_id", "<u4"), ("pad", "<u4"), ("word", "<u8")])
+STAGE = dict(dedup=1, overlap=2, prune=3, fullmatch=4, assembly=5, map=6, lq=7, nlq=8, all=8)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("L", "W", "max_off", "k", "kip", "max_mm")]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_libs = {}
+
+
+def lib(kind="st"):
+    if kind in _libs:
+        return _libs[kind]
+    path = LIBS[kind]
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    u32, u64, vp, i32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int
+    sig = {
+        "orc_make_params": (i32, [u32, ctypes.POINTER(OrcParams)]),
+        "orc_murmur64": (u64, [u64]),
+        "orc_hash_size": (u32, [u64]),
+        "orc_extract_word": (u64, [vp, u64, u64, u32]),
+        "orc_extract_word_unaligned": (u64, [vp, u64, u32]),
+        "orc_revcomp_kmer": (u64, [u64, u32]),
+        "orc_mismatch_words": (i32, [u64, u64]),
+        "orc_forward_mismatch": (i32, [vp, u64, vp, u32, u32]),
+        "orc_rc_mismatch": (i32, [vp, u64, vp, u32, u32]),
+        "orc_pack": (None, [vp, u64, u32, vp]),
+        "orc_block_run": (vp, [vp, u64, u32, i32]),
+        "orc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
+        "orc_block_free": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _libs[kind] = L
+    return L
+
+
+def params(L):
+    p = OrcParams()
+    if lib().orc_make_params(L, ctypes.byref(p)) != 0:
+        raise ValueError(L)
+    return p
+
+
+class OracleBlock:
+    def __init__(self, reads, stop="all", kind="st"):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        n, L = reads.shape
+        self.lib = lib(kind)
+        self._keep = reads
+        h = self.lib.orc_block_run(reads.ctypes.data, n, L, STAGE[stop])
+        if not h:
+            raise ValueError("oracle rejected the input")
+        self._h = ctypes.c_void_p(h)
+
+    def get(self, name):
+        fid, dt = FIELDS[name]
+        ptr = ctypes.c_void_p()
+        n = self.lib.orc_block_get(self._h, fid, ctypes.byref(ptr))
+        dt = np.dtype(REC_DTYPE if dt == "rec" else dt)
+        if n == 0 or not ptr.value:
+            return np.zeros(0, dtype=dt)
+        buf = (ctypes.c_char * (n * dt.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dt, count=n).copy()
+
+    def free(self):
+        if getattr(self, "_h", None) is not None:
+            self.lib.orc_block_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def pack(reads):
+    reads = np.ascontiguousarray(reads, dtype=np.uint8)
+    n, L = reads.shape
+    W = (L + 31) // 32
+    out = np.zeros(n * W + 1, dtype=np.uint64)
+    lib().orc_pack(reads.ctypes.data, n, L, out.ctypes.data)
+    return out[: n * W].reshape(n, W)

@@ -1,0 +1,112 @@
+"""GPU path vs the CPU oracle, field by field, on seeded synthetic blocks.
+
+Every comparison is exact (integer/byte work).  Intermediate link arrays are
+compared stage by stage, so a failure names the first stage that diverges.
+"""
+import numpy as np
+import pytest
+
+import curc_amd as ca
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+STAGE_FIELDS = [
+    ("nonN_ids", "N_ids", "sorted_ids", "next_dedup", "prev_dedup"),
+    ("next_overlap", "prev_overlap", "offset_overlap"),
+    ("next_prune", "prev_prune", "hq_ids", "unmatched_ids"),
+    ("next", "prev", "offset"),
+    ("hq_pg", "hq_rec", "hq_pg2"),
+    ("map_rec", "map_mis", "unmapped", "unmapped_N"),
+    ("lq_next", "lq_prev", "lq_offset", "lq_pg", "lq_rec"),
+]
+
+
+def compare(gpu_blk, orc_blk, fields):
+    for f in fields:
+        a, b = gpu_blk.get(f), orc_blk.get(f)
+        assert a.shape == b.shape, f"{f}: length {a.shape} vs oracle {b.shape}"
+        if not np.array_equal(a, b):
+            bad = np.flatnonzero(a != b)
+            i = int(bad[0])
+            raise AssertionError(f"{f}: {len(bad)} entries differ; first at {i}: gpu {a[i]} oracle {b[i]}")
+
+
+def run_both(ctx, reads):
+    print(f"case: {reads.shape[0]} reads x {reads.shape[1]} bp", flush=True)
+    g = ctx.run_block(reads, keep_stages=True)
+    o = orc.OracleBlock(reads, stop="lq")
+    for group in STAGE_FIELDS:
+        compare(g, o, group)
+    packed = g.get("packed")
+    assert np.array_equal(packed, o.get("packed")), "packed reads differ"
+    return g, o
+
+
+def test_small_block_l150(gpu_ctx):
+    cfg = ca.synth_config(seed=101, genome_len=50_000, gc=0.5, n_reads=8_000, L=150,
+                          sub_first=0.003, sub_last=0.008, n_rate=0.0005)
+    run_both(gpu_ctx, ca.synth_reads(cfg)[1])
+
+
+def test_config_small(gpu_ctx):
+    """The 1 Mbp / 133,333-read / 150 bp workload."""
+    run_both(gpu_ctx, ca.synth_reads(ca.config_small())[1])
+
+
+@pytest.mark.parametrize("L", [75, 100, 250])
+def test_read_lengths(gpu_ctx, L):
+    cfg = ca.synth_config(seed=200 + L, genome_len=40_000, gc=0.48, n_reads=6_000, L=L,
+                          sub_first=0.002, sub_last=0.01, n_rate=0.0003)
+    run_both(gpu_ctx, ca.synth_reads(cfg)[1])
+
+
+def test_high_coverage_duplicates(gpu_ctx):
+    """Tiny genome at ~1500x: many exact duplicates and contended claims."""
+    cfg = ca.synth_config(seed=7, genome_len=2_000, gc=0.5, n_reads=20_000, L=100,
+                          sub_first=0.0, sub_last=0.002, n_rate=0.0)
+    run_both(gpu_ctx, ca.synth_reads(cfg)[1])
+
+
+def test_repeats(gpu_ctx):
+    cfg = ca.synth_config(seed=9, genome_len=200_000, gc=0.45, repeat_frac=0.45, rep_min=300,
+                          rep_max=3000, rep_families=8, div_min=0.0, div_max=0.02,
+                          n_reads=30_000, L=100, sub_first=0.002, sub_last=0.004, n_rate=0.0002)
+    run_both(gpu_ctx, ca.synth_reads(cfg)[1])
+
+
+def test_periodic_sequence(gpu_ctx):
+    """Reads from a short-period sequence: self-overlaps, cycles, self-avoidance."""
+    rng = np.random.default_rng(3)
+    unit = np.frombuffer(b"ACGTTGCAAC", dtype=np.uint8)
+    genome = np.tile(unit, 200)
+    n, L = 600, 60
+    starts = rng.integers(0, len(genome) - L, size=n)
+    reads = np.stack([genome[s:s + L] for s in starts]).copy()
+    flip = rng.random((n, L)) < 0.01
+    reads[flip] = np.frombuffer(b"G", dtype=np.uint8)[0]
+    run_both(gpu_ctx, reads)
+
+
+def test_n_heavy(gpu_ctx):
+    cfg = ca.synth_config(seed=31, genome_len=30_000, gc=0.5, n_reads=5_000, L=120,
+                          sub_first=0.004, sub_last=0.004, n_rate=0.01)
+    run_both(gpu_ctx, ca.synth_reads(cfg)[1])
+
+
+def test_tiny_blocks(gpu_ctx):
+    for n in (1, 2, 3, 17):
+        cfg = ca.synth_config(seed=40 + n, genome_len=1_000, n_reads=n, L=90, n_rate=0.0)
+        run_both(gpu_ctx, ca.synth_reads(cfg)[1])
+    # all reads identical
+    reads = np.tile(np.frombuffer(b"ACGT" * 25, dtype=np.uint8), (50, 1))
+    run_both(gpu_ctx, reads)
+
+
+def test_determinism(gpu_ctx):
+    cfg = ca.synth_config(seed=55, genome_len=30_000, n_reads=6_000, L=150)
+    reads = ca.synth_reads(cfg)[1]
+    a = gpu_ctx.run_block(reads)
+    b = gpu_ctx.run_block(reads)
+    for f in ("next", "prev", "offset", "hq_pg", "map_rec", "map_mis", "lq_pg", "lq_rec"):
+        assert np.array_equal(a.get(f), b.get(f)), f
