@@ -1,0 +1,162 @@
+"""Benchmark of the CURC pseudogenome path on one GPU.
+
+Workload: 100 Mbp synthetic genome (GC 41%), 20,000,000 single-end 150 bp
+reads (3.0 Gbases), 50% reverse complemented, substitutions rising linearly
+from 0.1% at base 1 to 1.5% at base 150, N at 0.02% per base, one block.
+
+A step is one block through the whole GPU path: N split + 2-bit pack, dedup,
+prefix k-mer index, greedy overlap, prune/partition, exact all-offset
+matching, pseudogenome assembly, mapping onto it and the LQ pseudogenome,
+starting from the ASCII reads resident in HBM and ending with the
+pseudogenomes, map records and unmapped lists in host memory.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import curc_amd as ca  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--reads", type=int, default=20_000_000)
+    p.add_argument("--genome", type=int, default=100_000_000)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-reads", type=int, default=1_000_000)
+    return p.parse_args()
+
+
+def dominant_roofline(stats, steps):
+    """Roofline entry for the stage with the largest event time whose region
+    brackets single kernel launches."""
+    single = [s for s in stats if s["name"] in ("mapping",)]
+    if not single:
+        return None
+    s = max(single, key=lambda x: x["ms"])
+    per_launch_ms = s["ms"] / max(1, s["launches"])
+    per_launch_bytes = s["alg_bytes"] / max(1, s["launches"])
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    return dict(kernel=s["name"], bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
+                avg_launch_ms=round(per_launch_ms, 4), alg_bytes_per_launch=per_launch_bytes)
+
+
+def cpu_baseline(n_reads, L):
+    """Times the OpenMP build of the oracle on a scaled sample of the same
+    workload (same coverage, error and N profile) and cross-checks the GPU
+    path on it."""
+    from oracle import oracle as orc
+
+    genome = max(1_000_000, n_reads * L // 30)
+    cfg = ca.config_scale(seed=23, n_reads=n_reads, genome_len=genome)
+    _, reads = ca.synth_reads(cfg)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    ob = orc.OracleBlock(reads, stop="lq", kind="omp")
+    dt = time.perf_counter() - t0
+    with ca.Context(0) as ctx:
+        gb = ctx.run_block(reads)
+        same = all(np.array_equal(gb.get(f), ob.get(f)) for f in
+                   ("hq_pg", "hq_rec", "map_rec", "map_mis", "unmapped", "unmapped_N", "lq_pg", "lq_rec"))
+    gbases = n_reads * L / 1e9
+    return dict(value=round(gbases / dt, 6), unit="Gbases/s", cores=threads, kind="port",
+                sample=f"oracle_omp on {n_reads} reads x {L} bp from a {genome} bp genome "
+                       f"(same 30x coverage and error profile), {dt:.1f} s",
+                seconds=round(dt, 3), gpu_matches_oracle=bool(same))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
+              f"(blocks shard across GPUs as independent jobs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    L = 150
+    cfg = ca.config_scale(n_reads=a.reads, genome_len=a.genome)
+    t0 = time.perf_counter()
+    _, reads = ca.synth_reads(cfg)
+    log(f"generated {a.reads} reads in {time.perf_counter() - t0:.1f} s")
+    ctx = ca.Context(0)
+    nbytes = reads.nbytes
+    d_ascii = ctx.dev_alloc(nbytes)
+    ctx.h2d(d_ascii, reads)
+    del reads
+    ctx.device_sync()
+
+    def step():
+        b = ctx.run_block(None, L=L, device_ptr=d_ascii, n_reads=a.reads)
+        return b
+
+    for i in range(a.warmup):
+        t = time.perf_counter()
+        b = step()
+        log(f"warmup {i}: {time.perf_counter() - t:.3f} s, HQ PG {len(b.get('hq_pg2')) * 32} bases")
+        b.free()
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    ctx.device_sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        b = step()
+        b.free()
+    ctx.device_sync()
+    dt = time.perf_counter() - t0
+    ctx.set_timing(False)
+    stats = ctx.stats()
+    ms = dt * 1e3 / a.steps
+    gbases = a.reads * L / 1e9
+    value = gbases / (ms / 1e3)
+    log("stage breakdown (ms per step):")
+    for s in sorted(stats, key=lambda s: -s["ms"]):
+        log(f"  {s['name']:<14} {s['ms'] / a.steps:9.2f} ms  ({s['launches'] // max(1, a.steps)} regions/step)")
+    roof = dominant_roofline(stats, a.steps)
+    ctx.dev_free(d_ascii)
+    ctx.close()
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("timing the CPU oracle on a bounded sample ...")
+        cpu = cpu_baseline(a.cpu_reads, L)
+        log(f"  {cpu['sample']}: {cpu['value']} Gbases/s, GPU==oracle: {cpu['gpu_matches_oracle']}")
+    out = {
+        "metric": "Gbases/s (pseudogenome build+mapping, 1 GPU)",
+        "value": round(value, 4),
+        "unit": "Gbases/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"plain path, {a.genome // 1_000_000} Mbp genome GC 41%, {a.reads} x {L} bp SE, "
+                               "subst 0.1%->1.5%, N 0.02%, one block",
+                   "reads": a.reads, "read_len": L, "genome_bp": a.genome},
+        "stages_ms": {s["name"]: round(s["ms"] / a.steps, 3) for s in stats},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -87,6 +87,7 @@ def lib():
         "curc_memcpy_h2d": (i32, [vp, vp, vp, u64]),
         "curc_memcpy_d2h": (i32, [vp, vp, vp, u64]),
         "curc_sync": (i32, [vp]),
+        "curc_device_sync": (i32, [vp]),
         "curc_pack": (i32, [vp, vp, u64, u32, vp, vp]),
         "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
         "curc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
@@ -244,6 +245,9 @@ class Context:
 
     def sync(self):
         self._check(lib().curc_sync(self._h), "curc_sync")
+
+    def device_sync(self):
+        self._check(lib().curc_device_sync(self._h), "curc_device_sync")
 
     def set_timing(self, on=True):
         lib().curc_ctx_set_timing(self._h, 1 if on else 0)

@@ -832,6 +832,12 @@ int curc_sync(curc_ctx* c) {
     return hipStreamSynchronize(c->st) == hipSuccess ? CURC_OK : fail(c, CURC_EHIP, "stream sync failed");
 }
 
+int curc_device_sync(curc_ctx* c) {
+    if (!c) return CURC_EINVAL;
+    hipSetDevice(c->device);
+    return hipDeviceSynchronize() == hipSuccess ? CURC_OK : fail(c, CURC_EHIP, "device sync failed");
+}
+
 int curc_pack(curc_ctx* c, const char* d_ascii, uint64_t n, uint32_t L, uint64_t* d_packed, uint8_t* d_isN) {
     if (!c || (!d_ascii && n) || (!d_packed && n) || L == 0 || L > 512) return CURC_EINVAL;
     try {

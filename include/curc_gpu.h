@@ -70,6 +70,8 @@ int curc_dev_free(curc_ctx *ctx, void *ptr);
 int curc_memcpy_h2d(curc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int curc_memcpy_d2h(curc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int curc_sync(curc_ctx *ctx);
+/* Waits for all work on the context's device (hipDeviceSynchronize). */
+int curc_device_sync(curc_ctx *ctx);
 
 /* 2-bit pack of n ASCII reads of length L (all device pointers); replaces
  * encode (compress.cu:2151-2180).  d_packed holds n*ceil(L/32) words; d_isN
