@@ -1,0 +1,177 @@
+// Pseudogenome assembly on the GPU (compress.cu:107-165, RefRecord::assembly).
+//
+// The reference walks the link graph on the host: for each read of the visit
+// list not yet placed, it walks back to the chain head (or around a cycle,
+// which it cuts after the first largest-offset read met from next[rid]) and
+// emits the chain.  Here the same output comes from list ranking:
+//  1. nodes are renamed to their rank in the visit list;
+//  2. pointer jumping along prev gives, per node, the bases emitted before it
+//     in its chain, its index in the chain, its head and the minimum rank in
+//     its chain (or, for a cycle, in the whole cycle);
+//  3. one thread per cycle (the member of minimum rank, i.e. the read the
+//     reference visits first) finds the reference's cut and the cycle is
+//     ranked again as a path;
+//  4. chains are emitted in order of their minimum rank, which is the order in
+//     which the reference's visit loop first touches them.
+#pragma once
+#include "kernels.cuh"
+
+namespace curc {
+
+struct JumpState {
+    uint32_t p;  // jump pointer (rank) or INV
+    uint32_t c;  // nodes in the window
+    uint32_t h;  // last node of the window (chain head once p == INV)
+    uint32_t m;  // minimum rank in the window
+    uint64_t s;  // bases emitted by the window's nodes
+};
+
+__global__ void k_asm_rank(const uint32_t* __restrict__ V, uint64_t n, uint32_t* __restrict__ rank) {
+    GRID_STRIDE(j, n) rank[V[j]] = (uint32_t)j;
+}
+
+__global__ void k_asm_links(const uint32_t* __restrict__ V, uint64_t n, const uint32_t* __restrict__ next,
+                            const uint32_t* __restrict__ prev, const uint16_t* __restrict__ offset,
+                            const uint32_t* __restrict__ rank, uint32_t* __restrict__ nxtR,
+                            uint32_t* __restrict__ prvR, uint16_t* __restrict__ offR) {
+    GRID_STRIDE(j, n) {
+        uint32_t i = V[j];
+        uint32_t nx = next[i], pv = prev[i];
+        nxtR[j] = (nx == INV) ? INV : rank[nx];
+        prvR[j] = (pv == INV) ? INV : rank[pv];
+        offR[j] = offset[i];
+    }
+}
+
+// A node emits offset[i] bases, or the whole read if it ends its chain.
+__device__ __forceinline__ uint32_t emitted(const uint32_t* nxtR, const uint16_t* offR, uint64_t j, uint32_t L) {
+    return nxtR[j] == INV ? L : (uint32_t)offR[j];
+}
+
+__global__ void k_asm_init(const uint32_t* __restrict__ list, uint64_t n, const uint32_t* __restrict__ nxtR,
+                           const uint32_t* __restrict__ prvR, const uint16_t* __restrict__ offR, uint32_t L,
+                           JumpState* __restrict__ st) {
+    GRID_STRIDE(q, n) {
+        uint32_t j = list ? list[q] : (uint32_t)q;
+        JumpState a;
+        a.p = prvR[j];
+        a.c = 1;
+        a.h = j;
+        a.m = j;
+        a.s = emitted(nxtR, offR, j, L);
+        st[j] = a;
+    }
+}
+
+__global__ void k_asm_round(const uint32_t* __restrict__ list, uint64_t n, const JumpState* __restrict__ in,
+                            JumpState* __restrict__ out) {
+    GRID_STRIDE(q, n) {
+        uint32_t j = list ? list[q] : (uint32_t)q;
+        JumpState a = in[j];
+        if (a.p != INV) {
+            JumpState b = in[a.p];
+            a.s += b.s;
+            a.c += b.c;
+            a.h = b.h;
+            a.m = min(a.m, b.m);
+            a.p = b.p;
+        }
+        out[j] = a;
+    }
+}
+
+// Cycle nodes keep a live jump pointer after enough rounds; the member of
+// minimum rank cuts its cycle where the reference does (compress.cu:125-145)
+// and lists the cycle's members for re-ranking.
+__global__ void k_asm_cycles(uint64_t n, const JumpState* __restrict__ st, uint32_t* __restrict__ nxtR,
+                             uint32_t* __restrict__ prvR, const uint16_t* __restrict__ offR,
+                             uint32_t* __restrict__ members, uint32_t* __restrict__ n_members) {
+    GRID_STRIDE(j, n) {
+        JumpState a = st[j];
+        if (a.p == INV || a.m != (uint32_t)j) continue;
+        uint32_t t = nxtR[j];
+        uint32_t mo = t;
+        uint16_t mof = offR[t];
+        uint32_t len = 1;
+        while (t != (uint32_t)j) {
+            t = nxtR[t];
+            ++len;
+            if (offR[t] > mof) {
+                mof = offR[t];
+                mo = t;
+            }
+        }
+        uint32_t base = atomicAdd(n_members, len);
+        t = (uint32_t)j;
+        for (uint32_t k = 0; k < len; ++k) {
+            members[base + k] = t;
+            t = nxtR[t];
+        }
+        uint32_t hd = nxtR[mo];
+        nxtR[mo] = INV;
+        prvR[hd] = INV;
+    }
+}
+
+// Chain tails publish their chain's key (minimum rank), length and size at
+// the chain head.
+__global__ void k_asm_tails(uint64_t n, const JumpState* __restrict__ st, const uint32_t* __restrict__ nxtR,
+                            uint32_t* __restrict__ ckey, uint64_t* __restrict__ clen, uint32_t* __restrict__ ccnt) {
+    GRID_STRIDE(j, n) {
+        if (nxtR[j] != INV) continue;
+        JumpState a = st[j];
+        ckey[a.h] = a.m;
+        clen[a.h] = a.s;
+        ccnt[a.h] = a.c;
+    }
+}
+
+struct PredHead {
+    const uint32_t* prvR;
+    __device__ bool operator()(uint32_t j) const { return prvR[j] == INV; }
+};
+struct EmitHead {
+    const uint32_t* ckey;
+    uint32_t* keys;
+    uint32_t* heads;
+    __device__ void operator()(uint32_t j, uint32_t q) const {
+        keys[q] = ckey[j];
+        heads[q] = j;
+    }
+};
+
+__global__ void k_asm_gather_sizes(const uint32_t* __restrict__ heads, uint64_t nh, const uint64_t* __restrict__ clen,
+                                   const uint32_t* __restrict__ ccnt, uint64_t* __restrict__ olen,
+                                   uint32_t* __restrict__ ocnt) {
+    GRID_STRIDE(q, nh) {
+        uint32_t h = heads[q];
+        olen[q] = clen[h];
+        ocnt[q] = ccnt[h];
+    }
+}
+
+__global__ void k_asm_scatter_bases(const uint32_t* __restrict__ heads, uint64_t nh, const uint64_t* __restrict__ olen,
+                                    const uint32_t* __restrict__ ocnt, uint64_t* __restrict__ cpos,
+                                    uint32_t* __restrict__ crec) {
+    GRID_STRIDE(q, nh) {
+        uint32_t h = heads[q];
+        cpos[h] = olen[q];
+        crec[h] = ocnt[q];
+    }
+}
+
+__global__ void k_asm_records(const uint32_t* __restrict__ V, uint64_t n, const JumpState* __restrict__ st,
+                              const uint32_t* __restrict__ nxtR, const uint16_t* __restrict__ offR, uint32_t L,
+                              const uint64_t* __restrict__ cpos, const uint32_t* __restrict__ crec,
+                              uint32_t* __restrict__ rid, uint64_t* __restrict__ rpos, uint32_t* __restrict__ rnb) {
+    GRID_STRIDE(j, n) {
+        JumpState a = st[j];
+        uint32_t e = emitted(nxtR, offR, j, L);
+        uint32_t idx = crec[a.h] + a.c - 1;
+        rid[idx] = V[j];
+        rpos[idx] = cpos[a.h] + a.s - e;
+        rnb[idx] = e;
+    }
+}
+
+}  // namespace curc

@@ -185,17 +185,52 @@ __global__ void k_gather_key0_dev(const uint64_t* __restrict__ R, const uint32_t
     GRID_STRIDE(i, n) key[i] = R[(uint64_t)ids[i] * W];
 }
 
+// Base encodings the exact-matching kernels run over.  Enc2: ACGT reads,
+// 32 bases per word, 2 bits MSB-first (the packed reads).  Enc3: N-reads over
+// the alphabet A<C<G<N<T (compress.cu:1205), 21 bases per word, 3 bits
+// MSB-first below an unused top bit, so numeric order is ASCII order.
+struct Enc2 {
+    static constexpr uint32_t BPW = 32;
+    __device__ static uint64_t word(const uint64_t* rd, uint32_t off, uint32_t n) { return xw(rd, off, n); }
+    // first n bases of a read from its first word
+    __device__ static uint64_t head(uint64_t w0, uint32_t n) { return w0 >> ((32u - n) << 1); }
+    // n right-aligned bases in reverse order
+    __device__ static uint64_t reversed(uint64_t x, uint32_t n) {
+        uint64_t mask = (n == 32) ? ~0ull : ((1ull << (2 * n)) - 1);
+        return revcomp(x, n) ^ mask;
+    }
+};
+
+struct Enc3 {
+    static constexpr uint32_t BPW = 21;
+    __device__ static uint64_t word(const uint64_t* rd, uint32_t off, uint32_t n) {
+        uint32_t o1 = off / 21, o2 = off - o1 * 21;
+        uint64_t w = rd[o1];
+        if (o2 + n <= 21) return (w >> (3 * (21 - o2 - n))) & ((1ull << (3 * n)) - 1);
+        uint32_t k1 = 21 - o2, k2 = n - k1;
+        uint64_t hi = w & ((1ull << (3 * k1)) - 1);
+        return (hi << (3 * k2)) | (rd[o1 + 1] >> (3 * (21 - k2)));
+    }
+    __device__ static uint64_t head(uint64_t w0, uint32_t n) { return w0 >> (3 * (21 - n)); }
+    __device__ static uint64_t reversed(uint64_t x, uint32_t n) {
+        uint64_t r = 0;
+        for (uint32_t t = 0; t < n; ++t) r = (r << 3) | ((x >> (3 * t)) & 7u);
+        return r;
+    }
+};
+
 // p[0:len) vs s[off:off+len), with p's first word given and s's first
-// min(32,len) bases pre-extracted in sx0.
+// min(BPW,len) bases pre-extracted in sx0.
+template <class E>
 __device__ __forceinline__ int cmp_ps(const uint64_t* __restrict__ R, uint32_t W, uint32_t p, uint64_t pkey,
                                       const uint64_t* __restrict__ rs, uint64_t sx0, uint32_t off, uint32_t len) {
-    uint32_t n = min(32u, len);
-    uint64_t a = pkey >> ((32u - n) << 1);
+    uint32_t n = min(E::BPW, len);
+    uint64_t a = E::head(pkey, n);
     if (a != sx0) return a < sx0 ? -1 : 1;
     const uint64_t* rp = R + (uint64_t)p * W;
-    for (uint32_t j = 32; j < len; j += 32) {
-        n = min(32u, len - j);
-        uint64_t x = xw(rp, j, n), y = xw(rs, off + j, n);
+    for (uint32_t j = E::BPW; j < len; j += E::BPW) {
+        n = min(E::BPW, len - j);
+        uint64_t x = E::word(rp, j, n), y = E::word(rs, off + j, n);
         if (x != y) return x < y ? -1 : 1;
     }
     return 0;
@@ -203,17 +238,19 @@ __device__ __forceinline__ int cmp_ps(const uint64_t* __restrict__ R, uint32_t W
 
 // Order of two suffixes of one full-match group (equal s[off:L]): the
 // reference's suffix list is sorted by read[off:] then by base off-1,
-// off-2, ..., 0 (compress.cu:1013-1059), i.e. by the reversed s[0:off).
+// off-2, ..., 0 (compress.cu:1013-1059; full_match_cpu keeps the same order
+// with its stable multiway merge, compress.cu:1230-1242), i.e. by the
+// reversed s[0:off).
+template <class E>
 __device__ __forceinline__ int cmp_rev_prefix(const uint64_t* __restrict__ R, uint32_t W, uint32_t a,
                                               uint32_t b, uint32_t off) {
     const uint64_t* ra = R + (uint64_t)a * W;
     const uint64_t* rb = R + (uint64_t)b * W;
-    for (uint32_t t = 0; t < off; t += 32) {
-        uint32_t n = min(32u, off - t);
+    for (uint32_t t = 0; t < off; t += E::BPW) {
+        uint32_t n = min(E::BPW, off - t);
         uint32_t st = off - t - n;
-        uint64_t mask = (n == 32) ? ~0ull : ((1ull << (2 * n)) - 1);
-        uint64_t x = revcomp(xw(ra, st, n), n) ^ mask;
-        uint64_t y = revcomp(xw(rb, st, n), n) ^ mask;
+        uint64_t x = E::reversed(E::word(ra, st, n), n);
+        uint64_t y = E::reversed(E::word(rb, st, n), n);
         if (x != y) return x < y ? -1 : 1;
     }
     return 0;
@@ -258,7 +295,7 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
         uint32_t cnt = e - lo;
         while (cnt > 0) {
             uint32_t step = cnt >> 1, it = lo + step;
-            if (cmp_ps(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
+            if (cmp_ps<Enc2>(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
                 lo = it + 1;
                 cnt -= step + 1;
             } else {
@@ -269,7 +306,7 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
         for (uint32_t x = lo; x < e; ++x) {
             uint32_t p = cs.pl[x];
             if (p == s || cs.prev[p] != INV) continue;
-            if (cmp_ps(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) ch = x;
+            if (cmp_ps<Enc2>(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) ch = x;
             break;
         }
         cs.choice[i] = ch;
@@ -281,24 +318,26 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
     }
 }
 
-// a8 probe (compress.cu:1061-1160): the 10-mer block of s[off:] bounds the
-// search in the lexicographic prefix list; first equal non-self prefix.
+// a8 probe (compress.cu:1061-1160): the block of s[off:] (its first bp
+// bases) bounds the search in the lexicographic prefix list; first equal
+// non-self prefix.
+template <class E>
 __global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t* __restrict__ rng, uint32_t bp,
                                                   uint32_t off) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = 0;
     const uint32_t ns = *cs.d_ns;
     const uint32_t len = cs.L - off;
-    const uint32_t n1 = min(32u, len);
+    const uint32_t n1 = min(E::BPW, len);
     GRID_STRIDE(i, ns) {
         uint32_t s = cs.sl[i];
         const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-        uint32_t b = (uint32_t)xw(rs, off, bp);
+        uint32_t b = (uint32_t)E::word(rs, off, bp);
         uint32_t lo = rng[b], e = rng[b + 1];
-        uint64_t sx0 = xw(rs, off, n1);
+        uint64_t sx0 = E::word(rs, off, n1);
         uint32_t cnt = e - lo;
         while (cnt > 0) {
             uint32_t step = cnt >> 1, it = lo + step;
-            if (cmp_ps(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
+            if (cmp_ps<E>(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
                 lo = it + 1;
                 cnt -= step + 1;
             } else {
@@ -306,10 +345,10 @@ __global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t*
             }
         }
         uint32_t ch = INV;
-        if (lo < e && cmp_ps(cs.R, cs.W, cs.pl[lo], cs.pk[lo], rs, sx0, off, len) == 0) {
+        if (lo < e && cmp_ps<E>(cs.R, cs.W, cs.pl[lo], cs.pk[lo], rs, sx0, off, len) == 0) {
             if (cs.pl[lo] != s) {
                 ch = lo;
-            } else if (lo + 1 < e && cmp_ps(cs.R, cs.W, cs.pl[lo + 1], cs.pk[lo + 1], rs, sx0, off, len) == 0) {
+            } else if (lo + 1 < e && cmp_ps<E>(cs.R, cs.W, cs.pl[lo + 1], cs.pk[lo + 1], rs, sx0, off, len) == 0) {
                 ch = lo + 1;
             }
         }
@@ -347,12 +386,12 @@ __global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t 
 // One leader per contended group replays it sequentially: members in the
 // reference's suffix order (a6: list order; a8: reversed s[0:off)), each
 // taking the first free non-self equal prefix from the group start.
-template <bool FULLMATCH>
+template <bool FULLMATCH, class E>
 __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t off) {
     const uint32_t nq = *cs.res_n;
     const uint32_t np = *cs.d_np;
     const uint32_t len = cs.L - off;
-    const uint32_t n1 = min(32u, len);
+    const uint32_t n1 = min(E::BPW, len);
     GRID_STRIDE(q, nq) {
         uint32_t lead = cs.res_list[q];
         uint32_t g = cs.grp[lead];
@@ -362,8 +401,8 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
             uint32_t best = INV;
             for (uint32_t u = lead; u != INV; u = cs.memb[u]) {
                 if (FULLMATCH) {
-                    if (last != INV && cmp_rev_prefix(cs.R, cs.W, cs.sl[u], cs.sl[last], off) <= 0) continue;
-                    if (best != INV && cmp_rev_prefix(cs.R, cs.W, cs.sl[u], cs.sl[best], off) >= 0) continue;
+                    if (last != INV && cmp_rev_prefix<E>(cs.R, cs.W, cs.sl[u], cs.sl[last], off) <= 0) continue;
+                    if (best != INV && cmp_rev_prefix<E>(cs.R, cs.W, cs.sl[u], cs.sl[best], off) >= 0) continue;
                 } else {
                     if (last != INV && u <= last) continue;
                     if (best != INV && u >= best) continue;
@@ -374,11 +413,11 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
             last = best;
             uint32_t s = cs.sl[best];
             const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-            uint64_t sx0 = xw(rs, off, n1);
+            uint64_t sx0 = E::word(rs, off, n1);
             for (uint32_t x = g; x < np; ++x) {
                 uint32_t p = cs.pl[x];
                 if (p == s || cs.prev[p] != INV) continue;
-                if (cmp_ps(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) {
+                if (cmp_ps<E>(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) {
                     cs.next[s] = p;
                     cs.prev[p] = s;
                     cs.offset[s] = (uint16_t)off;
@@ -391,15 +430,15 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
     }
 }
 
-// Block ranges of the lexicographic prefix list by its first bp bases.
+// Block ranges of the lexicographic prefix list by its first bp bases
+// (nb = number of block values, 4^bp or 8^bp).
+template <class E>
 __global__ void k_fm_ranges(const uint64_t* __restrict__ pk, const uint32_t* __restrict__ d_np, uint32_t bp,
-                            uint32_t* __restrict__ rng) {
+                            uint64_t nb, uint32_t* __restrict__ rng) {
     const uint32_t np = *d_np;
-    const uint64_t nb = 1ull << (2 * bp);
-    const uint32_t sh = 64 - 2 * bp;
     GRID_STRIDE(i, (uint64_t)np + 1) {
-        uint64_t lo = (i == 0) ? 0 : (pk[i - 1] >> sh) + 1;
-        uint64_t hi = (i == np) ? nb : (pk[i] >> sh);
+        uint64_t lo = (i == 0) ? 0 : E::head(pk[i - 1], bp) + 1;
+        uint64_t hi = (i == np) ? nb : E::head(pk[i], bp);
         for (uint64_t b = lo; b <= hi; ++b) rng[b] = (uint32_t)i;
     }
 }
@@ -485,6 +524,39 @@ __global__ void k_pg_write(const uint64_t* __restrict__ R, uint32_t W, const uin
                 atomicOr(&pg[(q >> 5) + 1], (unsigned long long)lo);
             }
         }
+    }
+}
+
+// ------------------------------------------------------- N-read LQ -------
+// 3-bit packing of N-reads for exact matching over A<C<G<N<T (Enc3): code
+// A=0 C=1 G=2 N=3 T=4; other bytes fall outside the reference's alphabet
+// (compress.cu:1205) and are coded as N.
+__device__ __forceinline__ uint32_t enc5(uint8_t c) {
+    return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : c == 'T' ? 4u : 3u;
+}
+
+__global__ void k_pack3(const uint8_t* __restrict__ ascii, uint64_t n, uint32_t L, uint32_t W3,
+                        uint64_t* __restrict__ out) {
+    GRID_STRIDE(t, n * W3) {
+        uint64_t r = t / W3;
+        uint32_t w = (uint32_t)(t - r * W3);
+        uint32_t b0 = w * 21, nb = min(21u, L - b0);
+        const uint8_t* s = ascii + r * L + b0;
+        uint64_t v = 0;
+        for (uint32_t j = 0; j < nb; ++j) v |= (uint64_t)enc5(s[j]) << (3 * (20 - j));
+        out[t] = v;
+    }
+}
+
+// Copies each record's contribution into an ASCII pseudogenome
+// (compress.cu:167-217: the N-read pseudogenome is kept as text).
+__global__ void k_pg_write_ascii(const uint8_t* __restrict__ ascii, uint32_t L, const uint32_t* __restrict__ ids,
+                                 const uint64_t* __restrict__ pos, const uint32_t* __restrict__ nbs, uint64_t n,
+                                 uint8_t* __restrict__ pg) {
+    GRID_STRIDE(i, n) {
+        const uint8_t* s = ascii + (uint64_t)ids[i] * L;
+        uint8_t* d = pg + pos[i];
+        for (uint32_t j = 0; j < nbs[i]; ++j) d[j] = s[j];
     }
 }
 

@@ -2,7 +2,7 @@
 //
 // One block runs as a single stream of kernels; the host is consulted only
 // where a size decides an allocation (after the N split, after dedup, after
-// the partition, after mapping) and for the pseudogenome walk.
+// the partition, inside assembly, after mapping).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/curc_gpu.h"
+#include "assembly.cuh"
 #include "kernels.cuh"
 
 using namespace curc;
@@ -24,16 +25,22 @@ enum Slot {
     S_VA, S_VK, S_KR, S_SL0, S_SL1, S_CHOICE, S_GRP, S_OWNER, S_CONT, S_HEAD, S_MEMB, S_RESL,
     S_PL0, S_PL1, S_PK0, S_PK1, S_RNG, S_RES, S_HQ, S_UNM, S_PRE, S_SUF,
     S_PG_HQ, S_PG_LQ, S_PGIDS, S_PGPOS, S_PGNB, S_SORTED,
+    S_A_RANK, S_A_NXT, S_A_PRV, S_A_OFF, S_A_ST0, S_A_ST1, S_A_MEMB, S_A_CKEY, S_A_CLEN, S_A_CCNT,
+    S_A_HK0, S_A_HK1, S_A_HV0, S_A_HV1, S_A_OLEN, S_A_OCNT, S_A_CPOS, S_A_CREC,
     S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_LAST, S_WORD,
     S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
-    S_PRIM = 72  // primitives use slots 72.. (scan recursion, sort hist, compaction)
+    S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG,
+    S_LAST_NAMED,
+    S_PRIM = 104  // primitives use slots 104.. (scan recursion, sort hist, compaction)
 };
 
 // Device counters (u32) in slot S_CNT.
+static_assert(S_LAST_NAMED <= S_PRIM, "scratch slots overlap");
+
 enum Cnt {
     C_N, C_NONN, C_NN, C_NPRE, C_NSUF, C_NS0, C_NS1, C_RESN, C_NP0, C_NP1,
-    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_COUNT = 32
+    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_COUNT = 32
 };
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap = 8192) {
@@ -166,77 +173,14 @@ void h2d(curc_ctx* c, T* dst, const T* src, uint64_t n) {
     if (n) CURC_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->st));
 }
 
-// Pseudogenome walk (compress.cu:107-165): components in the order their
-// first member appears in `visit`; a cycle is cut after its first
-// largest-offset read.  Emits (read, position, bases written) per record.
-struct Walk {
-    std::vector<uint32_t> ids;
-    std::vector<uint64_t> pos;
-    std::vector<uint32_t> nb;
+// Records of one assembled pseudogenome, in emission order (device).
+struct AsmOut {
+    uint32_t* rid = nullptr;  // read id in the id space of the link arrays
+    uint64_t* pos = nullptr;  // position in the pseudogenome
+    uint32_t* nb = nullptr;   // bases the read contributes
+    uint32_t n = 0;
     uint64_t len = 0;
 };
-
-void walk_chains(Walk& out, const uint32_t* visit_ids, uint64_t n_visit, const uint32_t* next,
-                 const uint32_t* prev, const uint16_t* offset, uint64_t n_reads, uint32_t L) {
-    std::vector<uint8_t> seen(n_reads, 0);
-    out.ids.clear();
-    out.pos.clear();
-    out.nb.clear();
-    out.ids.reserve(n_visit);
-    out.pos.reserve(n_visit);
-    out.nb.reserve(n_visit);
-    uint64_t p = 0;
-    auto emit = [&](uint32_t id, uint32_t nb) {
-        out.ids.push_back(id);
-        out.pos.push_back(p);
-        out.nb.push_back(nb);
-        p += nb;
-    };
-    // A well-formed graph has symmetric links, so every component is a path
-    // or a cycle; the step budget turns a malformed one into an error.
-    uint64_t budget = 4 * n_reads + 16;
-    auto step = [&]() {
-        if (budget-- == 0) throw std::runtime_error("pseudogenome walk: link graph is not a set of paths and cycles");
-    };
-    for (uint64_t r = 0; r < n_visit; ++r) {
-        const uint32_t rid = visit_ids[r];
-        if (seen[rid]) continue;
-        uint32_t i = rid;
-        while (prev[i] != INV && prev[i] != rid) {
-            i = prev[i];
-            step();
-        }
-        if (prev[i] == rid) {
-            uint16_t mo = offset[i];
-            uint32_t mo_id = i;
-            while (i != rid) {
-                step();
-                seen[i] = 1;
-                i = next[i];
-                if (offset[i] > mo) {
-                    mo = offset[i];
-                    mo_id = i;
-                }
-            }
-            seen[rid] = 1;
-            for (uint32_t s = next[mo_id]; s != mo_id; s = next[s]) {
-                step();
-                emit(s, offset[s]);
-            }
-            emit(mo_id, L);
-        } else {
-            while (next[i] != INV) {
-                step();
-                seen[i] = 1;
-                emit(i, offset[i]);
-                i = next[i];
-            }
-            seen[i] = 1;
-            emit(i, L);
-        }
-    }
-    out.len = p;
-}
 
 }  // namespace
 
@@ -260,6 +204,8 @@ struct curc_block {
     std::vector<uint32_t> lq_next, lq_prev;
     std::vector<uint16_t> lq_off;
     std::vector<char> Nascii;
+    std::string nlq_text;
+    std::vector<curc_alignment> nlq_rec;
 };
 
 namespace {
@@ -271,41 +217,130 @@ std::string decode_pg(const std::vector<uint64_t>& pg2, uint64_t len) {
     return s;
 }
 
-// Builds the 2-bit pseudogenome on the device from walk records over the
-// packed reads R; returns the device array (ceil(len/32)+1 words, zero tail).
-uint64_t* build_pg(curc_ctx* c, const uint64_t* R, uint32_t W, const Walk& wk, int slot) {
-    const uint64_t words = (wk.len + 31) / 32 + 1;
+// Builds the 2-bit pseudogenome on the device from assembled records over
+// the packed reads R; returns the device array (ceil(len/32)+1 words, zero
+// tail).
+uint64_t* build_pg(curc_ctx* c, const uint64_t* R, uint32_t W, const AsmOut& a, int slot) {
+    const uint64_t words = (a.len + 31) / 32 + 1;
     uint64_t* pg = c->S.as<uint64_t>(slot, words);
     fill<uint64_t>(c, pg, words, 0ull);
-    const uint64_t n = wk.ids.size();
-    if (n) {
-        uint32_t* ids = c->S.as<uint32_t>(S_PGIDS, n);
-        uint64_t* pos = c->S.as<uint64_t>(S_PGPOS, n);
-        uint32_t* nb = c->S.as<uint32_t>(S_PGNB, n);
-        h2d(c, ids, wk.ids.data(), n);
-        h2d(c, pos, wk.pos.data(), n);
-        h2d(c, nb, wk.nb.data(), n);
-        hipLaunchKernelGGL(k_pg_write, dim3(grid_for(n)), dim3(256), 0, c->st, R, W, (const uint32_t*)ids,
-                           (const uint64_t*)pos, (const uint32_t*)nb, n, (unsigned long long*)pg);
-    }
+    if (a.n)
+        hipLaunchKernelGGL(k_pg_write, dim3(grid_for(a.n)), dim3(256), 0, c->st, R, W, (const uint32_t*)a.rid,
+                           (const uint64_t*)a.pos, (const uint32_t*)a.nb, (uint64_t)a.n, (unsigned long long*)pg);
     return pg;
 }
 
-void records_from_walk(const Walk& wk, const uint32_t* id_map, std::vector<curc_alignment>& rec) {
-    rec.resize(wk.ids.size());
-    for (size_t i = 0; i < wk.ids.size(); ++i) {
-        rec[i].read_id = id_map ? id_map[wk.ids[i]] : wk.ids[i];
+// Assembly over the visit list V (n reads; ids in the space of the link
+// arrays, which hold id_space entries).  See assembly.cuh.
+AsmOut assemble(curc_ctx* c, const uint32_t* V, uint32_t n, uint64_t id_space, const uint32_t* next,
+                const uint32_t* prev, const uint16_t* offset, uint32_t L, uint32_t* cnt) {
+    AsmOut out;
+    out.n = n;
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    out.rid = S.as<uint32_t>(S_PGIDS, (uint64_t)n + 1);
+    out.pos = S.as<uint64_t>(S_PGPOS, (uint64_t)n + 1);
+    out.nb = S.as<uint32_t>(S_PGNB, (uint64_t)n + 1);
+    if (n == 0) return out;
+    uint32_t* rank = S.as<uint32_t>(S_A_RANK, id_space + 1);
+    uint32_t* nxtR = S.as<uint32_t>(S_A_NXT, n);
+    uint32_t* prvR = S.as<uint32_t>(S_A_PRV, n);
+    uint16_t* offR = S.as<uint16_t>(S_A_OFF, n);
+    JumpState* js[2] = {S.as<JumpState>(S_A_ST0, n), S.as<JumpState>(S_A_ST1, n)};
+    const uint32_t g = grid_for(n);
+    hipLaunchKernelGGL(k_asm_rank, dim3(g), dim3(256), 0, st, V, (uint64_t)n, rank);
+    hipLaunchKernelGGL(k_asm_links, dim3(g), dim3(256), 0, st, V, (uint64_t)n, next, prev, offset,
+                       (const uint32_t*)rank, nxtR, prvR, offR);
+    hipLaunchKernelGGL(k_asm_init, dim3(g), dim3(256), 0, st, (const uint32_t*)nullptr, (uint64_t)n,
+                       (const uint32_t*)nxtR, (const uint32_t*)prvR, (const uint16_t*)offR, L, js[0]);
+    // 2^K > n >= any chain or cycle length, so every path node's pointer runs
+    // out and every cycle node's window wraps its whole cycle.
+    uint32_t K = 1;
+    while ((1ull << K) <= n) ++K;
+    int cur = 0;
+    for (uint32_t r = 0; r < K; ++r) {
+        hipLaunchKernelGGL(k_asm_round, dim3(g), dim3(256), 0, st, (const uint32_t*)nullptr, (uint64_t)n,
+                           (const JumpState*)js[cur], js[1 - cur]);
+        cur = 1 - cur;
+    }
+    uint32_t* members = S.as<uint32_t>(S_A_MEMB, n);
+    set_cnt(c, cnt + C_ASM, 0);
+    hipLaunchKernelGGL(k_asm_cycles, dim3(g), dim3(256), 0, st, (uint64_t)n, (const JumpState*)js[cur], nxtR, prvR,
+                       (const uint16_t*)offR, members, cnt + C_ASM);
+    const uint32_t ncyc = read_cnt(c, cnt + C_ASM);
+    if (ncyc) {
+        // re-rank the cut cycles as paths; the other nodes are final in both
+        // buffers after this copy
+        CURC_HIP(hipMemcpyAsync(js[1 - cur], js[cur], (uint64_t)n * sizeof(JumpState), hipMemcpyDeviceToDevice, st));
+        const uint32_t gc = grid_for(ncyc);
+        hipLaunchKernelGGL(k_asm_init, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
+                           (const uint32_t*)nxtR, (const uint32_t*)prvR, (const uint16_t*)offR, L, js[cur]);
+        uint32_t Kc = 1;
+        while ((1ull << Kc) <= ncyc) ++Kc;
+        for (uint32_t r = 0; r < Kc; ++r) {
+            hipLaunchKernelGGL(k_asm_round, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
+                               (const JumpState*)js[cur], js[1 - cur]);
+            cur = 1 - cur;
+        }
+    }
+    uint32_t* ckey = S.as<uint32_t>(S_A_CKEY, n);
+    uint64_t* clen = S.as<uint64_t>(S_A_CLEN, n);
+    uint32_t* ccnt = S.as<uint32_t>(S_A_CCNT, n);
+    hipLaunchKernelGGL(k_asm_tails, dim3(g), dim3(256), 0, st, (uint64_t)n, (const JumpState*)js[cur],
+                       (const uint32_t*)nxtR, ckey, clen, ccnt);
+    uint32_t* hk0 = S.as<uint32_t>(S_A_HK0, n);
+    uint32_t* hk1 = S.as<uint32_t>(S_A_HK1, n);
+    uint32_t* hv0 = S.as<uint32_t>(S_A_HV0, n);
+    uint32_t* hv1 = S.as<uint32_t>(S_A_HV1, n);
+    set_cnt(c, cnt + C_TMP, n);
+    compact(cnt + C_TMP, PredHead{prvR}, EmitHead{ckey, hk0, hv0}, cnt + C_ASM, S, S_PRIM, st);
+    const uint32_t nh = read_cnt(c, cnt + C_ASM);
+    radix_sort_pairs<uint32_t>(hk0, hv0, hk1, hv1, nh, 0, log2u((uint64_t)n + 1), S, S_PRIM, st);
+    uint64_t* olen = S.as<uint64_t>(S_A_OLEN, (uint64_t)nh + 1);
+    uint32_t* ocnt = S.as<uint32_t>(S_A_OCNT, (uint64_t)nh + 1);
+    fill<uint64_t>(c, olen + nh, 1, 0ull);
+    fill<uint32_t>(c, ocnt + nh, 1, 0u);
+    hipLaunchKernelGGL(k_asm_gather_sizes, dim3(grid_for(nh)), dim3(256), 0, st, (const uint32_t*)hv0, (uint64_t)nh,
+                       (const uint64_t*)clen, (const uint32_t*)ccnt, olen, ocnt);
+    scan_excl_t<uint64_t>(olen, olen, (uint64_t)nh + 1, S, S_PRIM, st);
+    scan_excl_t<uint32_t>(ocnt, ocnt, (uint64_t)nh + 1, S, S_PRIM, st);
+    uint64_t* cpos = S.as<uint64_t>(S_A_CPOS, n);
+    uint32_t* crec = S.as<uint32_t>(S_A_CREC, n);
+    hipLaunchKernelGGL(k_asm_scatter_bases, dim3(grid_for(nh)), dim3(256), 0, st, (const uint32_t*)hv0, (uint64_t)nh,
+                       (const uint64_t*)olen, (const uint32_t*)ocnt, cpos, crec);
+    hipLaunchKernelGGL(k_asm_records, dim3(g), dim3(256), 0, st, V, (uint64_t)n, (const JumpState*)js[cur],
+                       (const uint32_t*)nxtR, (const uint16_t*)offR, L, (const uint64_t*)cpos,
+                       (const uint32_t*)crec, out.rid, out.pos, out.nb);
+    CURC_HIP(hipMemcpyAsync(&out.len, olen + nh, 8, hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipStreamSynchronize(st));
+    return out;
+}
+
+void records_to_host(curc_ctx* c, const AsmOut& a, const uint32_t* id_map, std::vector<curc_alignment>& rec) {
+    std::vector<uint32_t> ids;
+    std::vector<uint64_t> pos;
+    d2h(c, ids, a.rid, a.n);
+    d2h(c, pos, a.pos, a.n);
+    CURC_HIP(hipStreamSynchronize(c->st));
+    rec.resize(a.n);
+    for (uint32_t i = 0; i < a.n; ++i) {
+        rec[i].read_id = id_map ? id_map[ids[i]] : ids[i];
         rec[i].pad = 0;
-        rec[i].word = wk.pos[i] << 2;
+        rec[i].word = pos[i] << 2;
     }
 }
 
-// Exact all-offset matching (a8/a11; compress.cu:1003-1174).  Initial
-// prefix/suffix lists are lexicographic; their lengths are the device
-// counters d_npre/d_nsuf, both <= n_max.
+// Exact all-offset matching (a8/a11 over 2-bit reads, compress.cu:1003-1174;
+// a12 over 3-bit N-reads, compress.cu:1176-1347).  Initial prefix/suffix
+// lists are lexicographic; their lengths are the device counters
+// cnt[C_PRE]/cnt[C_SUF], both <= n_max.  The block width bp only partitions
+// the search; it does not change which prefixes a suffix meets.
+template <class E>
 void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const uint32_t* d_pre,
                 const uint32_t* d_suf, uint32_t* cnt, uint32_t n_max, uint32_t* next, uint32_t* prev,
                 uint16_t* offset) {
+    constexpr uint32_t BITS = (E::BPW == 32) ? 2 : 3;
+    constexpr uint32_t MAX_BP = (BITS == 2) ? 10 : 6;  // 2^20 or 2^18 block values
     if (n_max == 0) return;
     Scratch& S = c->S;
     uint32_t* pl[2] = {S.as<uint32_t>(S_PL0, n_max), S.as<uint32_t>(S_PL1, n_max)};
@@ -333,14 +368,15 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     const uint32_t g = grid_for(n_max, 4096);
     int cur = 0;
     for (uint32_t off = 1; off < L; ++off) {
-        const uint32_t bp = (L - off >= 10) ? 10 : (L - off);
-        hipLaunchKernelGGL(k_fm_ranges, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, c->st,
-                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng);
+        const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
+        const uint64_t nb = 1ull << (BITS * bp);
+        hipLaunchKernelGGL(k_fm_ranges<E>, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, c->st,
+                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, nb, rng);
         ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
                       choice, grp, owner, cont, head, memb, resl, cnt + C_RESN};
-        hipLaunchKernelGGL(k_fm_probe, dim3(g), dim3(256), 0, c->st, cs, (const uint32_t*)rng, bp, off);
+        hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs, (const uint32_t*)rng, bp, off);
         hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off);
-        hipLaunchKernelGGL(k_claim_resolve<true>, dim3(256), dim3(256), 0, c->st, cs, off);
+        hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
         compact(ns[cur], PredSufFree{sl[cur], next}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
                 S_PRIM, c->st);
         compact(np[cur], PredPreFree{pl[cur], prev}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
@@ -439,8 +475,10 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->next_dedup, next, N);
         d2h(c, B->prev_dedup, prev, N);
     }
-    d2h(c, B->sorted, sorted, n);
-    d2h(c, B->nonN, nonN, n);
+    if (keep) {
+        d2h(c, B->sorted, sorted, n);
+        d2h(c, B->nonN, nonN, n);
+    }
     d2h(c, B->Nids, Nids, nN);
     if (stop <= CURC_STAGE_DEDUP) {
         d2h(c, B->next, next, N);
@@ -502,7 +540,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr, mask, P.k,
                                    P.kip, o);
                 hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
-                hipLaunchKernelGGL(k_claim_resolve<false>, dim3(256), dim3(256), 0, st, cs, o);
+                hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
                 compact(ns[cur], PredSufFree{sl[cur], next}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
                         S_PRIM, st);
                 cur = 1 - cur;
@@ -543,8 +581,10 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->next_pr, next, N);
         d2h(c, B->prev_pr, prev, N);
     }
-    d2h(c, B->hq, hq, n_hq);
-    d2h(c, B->unmatched, unm, n_unm);
+    if (keep || stop <= CURC_STAGE_PRUNE) {
+        d2h(c, B->hq, hq, n_hq);
+        d2h(c, B->unmatched, unm, n_unm);
+    }
     if (stop <= CURC_STAGE_PRUNE) {
         d2h(c, B->next, next, N);
         d2h(c, B->prev, prev, N);
@@ -561,24 +601,25 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         set_cnt(c, cnt + C_TMP, n_hq);
         compact(cnt + C_TMP, PredLinkState{hq, prev}, EmitGather{hq, pre}, cnt + C_PRE, S, S_PRIM, st);
         compact(cnt + C_TMP, PredLinkState{hq, next}, EmitGather{hq, suf}, cnt + C_SUF, S, S_PRIM, st);
-        full_match(c, packed, W, L, pre, suf, cnt, n_hq, next, prev, off);
+        full_match<Enc2>(c, packed, W, L, pre, suf, cnt, n_hq, next, prev, off);
     }
-    d2h(c, B->next, next, N);
-    d2h(c, B->prev, prev, N);
-    d2h(c, B->offset, off, N);
-    CURC_HIP(hipStreamSynchronize(st));
+    if (keep || stop <= CURC_STAGE_FULLMATCH) {
+        d2h(c, B->next, next, N);
+        d2h(c, B->prev, prev, N);
+        d2h(c, B->offset, off, N);
+        CURC_HIP(hipStreamSynchronize(st));
+    }
     if (stop <= CURC_STAGE_FULLMATCH) return;
 
     // ---- a9: pseudogenome assembly (compress.cu:1613-1618)
     uint64_t* hq_pg;
     {
         Region r(c, "assembly");
-        Walk wk;
-        walk_chains(wk, B->hq.data(), B->hq.size(), B->next.data(), B->prev.data(), B->offset.data(), N, L);
-        records_from_walk(wk, nullptr, B->hq_rec);
-        B->hq_len = wk.len;
-        hq_pg = build_pg(c, packed, W, wk, S_PG_HQ);
-        d2h(c, B->hq_pg2, hq_pg, (wk.len + 31) / 32 + 1);
+        AsmOut a = assemble(c, hq, n_hq, N, next, prev, off, L, cnt);
+        B->hq_len = a.len;
+        hq_pg = build_pg(c, packed, W, a, S_PG_HQ);
+        records_to_host(c, a, nullptr, B->hq_rec);
+        d2h(c, B->hq_pg2, hq_pg, (a.len + 31) / 32 + 1);
         CURC_HIP(hipStreamSynchronize(st));
     }
     if (stop <= CURC_STAGE_ASSEMBLY) return;
@@ -701,19 +742,17 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         hipLaunchKernelGGL(k_iota, dim3(grid_for(n_lq)), dim3(256), 0, st, ids, (uint64_t)n_lq);
         set_cnt(c, cnt + C_PRE, n_lq);
         set_cnt(c, cnt + C_SUF, n_lq);
-        full_match(c, lqp, W, L, ids, ids, cnt, n_lq, lnext, lprev, loff);
-        d2h(c, B->lq_next, lnext, n_lq);
-        d2h(c, B->lq_prev, lprev, n_lq);
-        d2h(c, B->lq_off, loff, n_lq);
-        CURC_HIP(hipStreamSynchronize(st));
-        std::vector<uint32_t> visit(n_lq);
-        for (uint32_t i = 0; i < n_lq; ++i) visit[i] = i;
-        Walk wk;
-        walk_chains(wk, visit.data(), n_lq, B->lq_next.data(), B->lq_prev.data(), B->lq_off.data(), n_lq, L);
-        records_from_walk(wk, B->unmapped.data(), B->lq_rec);
-        B->lq_len = wk.len;
-        uint64_t* lpg = build_pg(c, lqp, W, wk, S_PG_LQ);
-        d2h(c, B->lq_pg2, lpg, (wk.len + 31) / 32 + 1);
+        full_match<Enc2>(c, lqp, W, L, ids, ids, cnt, n_lq, lnext, lprev, loff);
+        if (keep) {
+            d2h(c, B->lq_next, lnext, n_lq);
+            d2h(c, B->lq_prev, lprev, n_lq);
+            d2h(c, B->lq_off, loff, n_lq);
+        }
+        AsmOut a = assemble(c, ids, n_lq, n_lq, lnext, lprev, loff, L, cnt);
+        B->lq_len = a.len;
+        uint64_t* lpg = build_pg(c, lqp, W, a, S_PG_LQ);
+        records_to_host(c, a, B->unmapped.data(), B->lq_rec);
+        d2h(c, B->lq_pg2, lpg, (a.len + 31) / 32 + 1);
         CURC_HIP(hipStreamSynchronize(st));
     }
 }

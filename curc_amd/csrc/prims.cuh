@@ -93,45 +93,69 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* lds
 constexpr uint32_t SCAN_ITEMS = 8;
 constexpr uint32_t SCAN_TILE = 256 * SCAN_ITEMS;
 
-__global__ void __launch_bounds__(256) k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
-                                                     uint32_t* __restrict__ sums) {
-    __shared__ uint32_t lds[4];
+// Exclusive block scan of a T for 256 threads (4 waves).
+template <typename T>
+__device__ __forceinline__ T block_excl_scan256_t(T v, T* lds4, T* total) {
+    uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
+    T inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T u = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += u;
+    }
+    if (lane == 63) lds4[w] = inc;
+    __syncthreads();
+    T wo = 0, tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        T s = lds4[i];
+        if (i < w) wo += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wo + inc - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scan_reduce(const T* __restrict__ in, uint64_t n, T* __restrict__ sums) {
+    __shared__ T lds[4];
     uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
-    uint32_t s = 0;
+    T s = 0;
 #pragma unroll
     for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
         uint64_t i = base + r * 256 + threadIdx.x;
         if (i < n) s += in[i];
     }
-    uint32_t tot;
-    block_excl_scan256(s, lds, &tot);
+    T tot;
+    block_excl_scan256_t<T>(s, lds, &tot);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
 // Exclusive scan of one tile per block plus the block offset.  Thread t owns
 // elements [t*ITEMS, t*ITEMS+ITEMS) of the tile, staged through LDS so global
 // loads stay coalesced.
-__global__ void __launch_bounds__(256) k_scan_apply(const uint32_t* __restrict__ in,
-                                                    uint32_t* __restrict__ out, uint64_t n,
-                                                    const uint32_t* __restrict__ offs) {
-    __shared__ uint32_t tile[SCAN_TILE];
-    __shared__ uint32_t lds[4];
+template <typename T>
+__global__ void __launch_bounds__(256) k_scan_apply(const T* __restrict__ in, T* __restrict__ out, uint64_t n,
+                                                    const T* __restrict__ offs) {
+    __shared__ T tile[SCAN_TILE];
+    __shared__ T lds[4];
     uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
 #pragma unroll
     for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
         uint64_t i = base + r * 256 + threadIdx.x;
-        tile[r * 256 + threadIdx.x] = (i < n) ? in[i] : 0u;
+        tile[r * 256 + threadIdx.x] = (i < n) ? in[i] : (T)0;
     }
     __syncthreads();
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
+    T v[SCAN_ITEMS];
+    T s = 0;
 #pragma unroll
     for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
         v[r] = tile[threadIdx.x * SCAN_ITEMS + r];
         s += v[r];
     }
-    uint32_t tot;
-    uint32_t ex = block_excl_scan256(s, lds, &tot) + (offs ? offs[blockIdx.x] : 0u);
+    T tot;
+    T ex = block_excl_scan256_t<T>(s, lds, &tot) + (offs ? offs[blockIdx.x] : (T)0);
 #pragma unroll
     for (uint32_t r = 0; r < SCAN_ITEMS; ++r) {
         tile[threadIdx.x * SCAN_ITEMS + r] = ex;
@@ -145,21 +169,24 @@ __global__ void __launch_bounds__(256) k_scan_apply(const uint32_t* __restrict__
     }
 }
 
-// Exclusive scan of n u32 (in may equal out).  Uses scratch slots
+// Exclusive scan of n elements (in may equal out).  Uses scratch slots
 // [slot, slot+depth).
-inline void scan_excl(const uint32_t* in, uint32_t* out, uint64_t n, Scratch& S, int slot,
-                      hipStream_t st) {
+template <typename T>
+inline void scan_excl_t(const T* in, T* out, uint64_t n, Scratch& S, int slot, hipStream_t st) {
     if (n == 0) return;
     uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (tiles == 1) {
-        hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(256), 0, st, in, out, n, (const uint32_t*)nullptr);
+        hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(256), 0, st, in, out, n, (const T*)nullptr);
         return;
     }
-    uint32_t* sums = S.as<uint32_t>(slot, tiles);
-    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)tiles), dim3(256), 0, st, in, n, sums);
-    scan_excl(sums, sums, tiles, S, slot + 1, st);
-    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)tiles), dim3(256), 0, st, in, out, n,
-                       (const uint32_t*)sums);
+    T* sums = S.as<T>(slot, tiles);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((uint32_t)tiles), dim3(256), 0, st, in, n, sums);
+    scan_excl_t<T>(sums, sums, tiles, S, slot + 1, st);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((uint32_t)tiles), dim3(256), 0, st, in, out, n, (const T*)sums);
+}
+
+inline void scan_excl(const uint32_t* in, uint32_t* out, uint64_t n, Scratch& S, int slot, hipStream_t st) {
+    scan_excl_t<uint32_t>(in, out, n, S, slot, st);
 }
 
 // ----------------------------------------------------------- radix sort ----

@@ -647,6 +647,14 @@ struct orc_block {
     std::vector<uint32_t> unmapped, unmapped_N;
     std::vector<uint32_t> lq_next, lq_prev;
     std::vector<uint16_t> lq_off;
+    // pre-entropy streams (orc_block_streams)
+    std::string s_strand;
+    std::vector<uint16_t> s_read_off, s_lq_off, s_nlq_off;
+    std::vector<uint8_t> s_mis_count, s_mis_base, s_mis_table;
+    std::vector<uint32_t> s_pe_order;
+    std::vector<uint64_t> s_id_pos;
+    std::vector<orc_rec> s_sorted;
+    std::vector<std::vector<uint16_t>> s_mis_off;
 };
 
 extern "C" {
@@ -967,8 +975,166 @@ uint64_t orc_block_get(orc_block *B, int field, const void **ptr) {
         case ORC_F_LQ_NEXT: return put(B->lq_next);
         case ORC_F_LQ_PREV: return put(B->lq_prev);
         case ORC_F_LQ_OFFSET: return put(B->lq_off);
-        default: *ptr = nullptr; return 0;
+        case ORC_F_S_STRAND: return put(B->s_strand);
+        case ORC_F_S_READ_OFF: return put(B->s_read_off);
+        case ORC_F_S_MIS_COUNT: return put(B->s_mis_count);
+        case ORC_F_S_MIS_BASE: return put(B->s_mis_base);
+        case ORC_F_S_MIS_TABLE: return put(B->s_mis_table);
+        case ORC_F_S_LQ_OFF: return put(B->s_lq_off);
+        case ORC_F_S_NLQ_OFF: return put(B->s_nlq_off);
+        case ORC_F_S_PE_ORDER: return put(B->s_pe_order);
+        case ORC_F_S_ID_POS: return put(B->s_id_pos);
+        case ORC_F_S_SORTED_REC: return put(B->s_sorted);
+        default:
+            if (field > ORC_F_S_MIS_OFF && field - ORC_F_S_MIS_OFF < (int)B->s_mis_off.size())
+                return put(B->s_mis_off[field - ORC_F_S_MIS_OFF]);
+            *ptr = nullptr;
+            return 0;
     }
+}
+
+// compress.cu:1730-2044: record sort and the stream builders that define the
+// block's parity surface (positions, mismatch lists, read order).
+int orc_block_streams(orc_block *B, int preserve, int paired) {
+    const uint32_t L = B->P.L, W = B->P.W, max_mm = B->P.max_mm;
+    const std::string &ref = B->hq_pg.text;
+    std::vector<orc_rec> rec = B->hq_pg.rec;
+    rec.insert(rec.end(), B->map_rec.begin(), B->map_rec.end());
+    std::vector<uint8_t> mis(B->hq_pg.rec.size(), 0);
+    mis.insert(mis.end(), B->map_mis.begin(), B->map_mis.end());
+    auto gid = [&](const orc_rec &r) -> uint64_t { return (r.word & 1) ? B->Nids[r.read_id] : r.read_id; };
+    // compress.cu:1743-1756, stable (R17)
+    std::vector<size_t> ord(rec.size());
+    std::iota(ord.begin(), ord.end(), (size_t)0);
+    if (!preserve)
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return rec_pos(rec[a]) < rec_pos(rec[b]); });
+    else
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return gid(rec[a]) < gid(rec[b]); });
+    std::vector<orc_rec> R(rec.size());
+    std::vector<uint8_t> M(rec.size());
+    for (size_t i = 0; i < ord.size(); ++i) {
+        R[i] = rec[ord[i]];
+        M[i] = mis[ord[i]];
+    }
+    B->s_sorted = R;
+    const size_t n = R.size();
+    if (preserve) B->s_id_pos.assign(B->n_reads, 0);
+    B->s_strand.assign(n, '0');
+    if (!preserve) B->s_read_off.assign(n, 0);
+    B->s_pe_order.clear();
+    if (!preserve && paired) B->s_pe_order.assign(n, 0);
+    B->s_mis_count = M;
+    auto width = [](uint32_t m) -> uint64_t { return m == 1 ? 2 : m == 2 ? 3 : (uint64_t)m + 2; };
+    std::vector<uint64_t> count_m(max_mm + 1, 0);
+    std::vector<uint64_t> off_idx(n, 0);
+    uint64_t total_mis = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (M[i]) {
+            off_idx[i] = width(M[i]) * count_m[M[i]];
+            count_m[M[i]]++;
+        }
+        total_mis += M[i];
+    }
+    B->s_mis_off.assign(max_mm + 1, {});
+    for (uint32_t m = 1; m <= max_mm; ++m) B->s_mis_off[m].assign(width(m) * count_m[m], 0);
+    std::string ctx_s(total_mis, 0), base_s(total_mis, 0);
+    uint64_t bidx = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t rid = R[i].read_id;
+        const uint64_t isN = R[i].word & 1, strand = (R[i].word >> 1) & 1, pos = rec_pos(R[i]);
+        const uint32_t m = M[i];
+        if (preserve) {
+            B->s_id_pos[gid(R[i])] = pos;
+        } else {
+            if (paired) B->s_pe_order[i] = (uint32_t)gid(R[i]);
+            B->s_read_off[i] = (uint16_t)((i == 0) ? 0 : (pos - rec_pos(R[i - 1])));
+        }
+        B->s_strand[i] = (char)('0' + strand);
+        if (!m) continue;
+        std::vector<uint16_t> offs;
+        for (uint32_t p = 0; p < L; ++p) {
+            uint32_t bo = strand == 0 ? p : L - 1 - p;
+            char rb = isN ? B->Nascii[(uint64_t)rid * L + bo] : base_at(B->packed.data(), (uint64_t)rid * W, bo);
+            if (strand == 1) rb = rev_base(rb);
+            if (ref[pos + p] != rb) {
+                ctx_s[bidx] = ref[pos + p];
+                base_s[bidx] = rb;
+                bidx++;
+                offs.push_back((uint16_t)p);
+            }
+        }
+        if (offs.size() != m) return -1;  // recount disagrees with the mapping count
+        std::vector<uint16_t> &S = B->s_mis_off[m];
+        uint64_t k = off_idx[i];
+        if (offs[0] < (L - 1 - offs[m - 1])) {  // left to right (compress.cu:1848-1867)
+            S[k++] = 0;
+            S[k++] = offs[0];
+            if (m == 2) {
+                S[k++] = (uint16_t)(offs[1] - offs[0] - 1);
+            } else if (m > 2) {
+                uint32_t mn = 0xFFFFFFFFu;
+                for (uint32_t q = 1; q < m; ++q) mn = std::min<uint32_t>(mn, offs[q] - offs[q - 1]);
+                S[k++] = (uint16_t)(mn - 1);
+                for (uint32_t q = 1; q < m; ++q) S[k++] = (uint16_t)(offs[q] - offs[q - 1] - mn);
+            }
+        } else {  // right to left (compress.cu:1868-1886)
+            S[k++] = 1;
+            S[k++] = (uint16_t)(L - 1 - offs[m - 1]);
+            if (m == 2) {
+                S[k++] = (uint16_t)(offs[1] - offs[0] - 1);
+            } else if (m > 2) {
+                uint32_t mn = 0xFFFFFFFFu;
+                for (uint32_t q = m - 1; q > 0; --q) mn = std::min<uint32_t>(mn, offs[q] - offs[q - 1]);
+                S[k++] = (uint16_t)(mn - 1);
+                for (uint32_t q = m - 1; q > 0; --q) S[k++] = (uint16_t)(offs[q] - offs[q - 1] - mn);
+            }
+        }
+    }
+    // compress.cu:1894-1929: per-context symbol ranks by frequency; ties keep
+    // the destination order (std::sort of four elements is an insertion sort).
+    static uint64_t cnt[256][256];
+    std::memset(cnt, 0, sizeof(cnt));
+    for (uint64_t i = 0; i < total_mis; ++i) cnt[(uint8_t)ctx_s[i]][(uint8_t)base_s[i]]++;
+    static const char CTX[4] = {'A', 'T', 'C', 'G'};
+    static const char DEST[4][4] = {{'C', 'G', 'N', 'T'}, {'A', 'C', 'G', 'N'}, {'A', 'G', 'N', 'T'}, {'A', 'C', 'N', 'T'}};
+    uint8_t rank[256][256];
+    std::memset(rank, 0, sizeof(rank));
+    B->s_mis_table.assign(16, 0);
+    for (int c = 0; c < 4; ++c) {
+        int idx[4] = {0, 1, 2, 3};
+        std::stable_sort(idx, idx + 4, [&](int a, int b) {
+            return cnt[(uint8_t)CTX[c]][(uint8_t)DEST[c][a]] > cnt[(uint8_t)CTX[c]][(uint8_t)DEST[c][b]];
+        });
+        for (int i = 0; i < 4; ++i) rank[(uint8_t)CTX[c]][(uint8_t)DEST[c][idx[i]]] = (uint8_t)i;
+        for (int i = 0; i < 4; ++i) B->s_mis_table[c * 4 + rank[(uint8_t)CTX[c]][(uint8_t)DEST[c][i]]] = (uint8_t)DEST[c][i];
+    }
+    B->s_mis_base.assign(total_mis, 0);
+    for (uint64_t i = 0; i < total_mis; ++i) B->s_mis_base[i] = rank[(uint8_t)ctx_s[i]][(uint8_t)base_s[i]];
+    // compress.cu:1968-2001: LQ and N-LQ offsets, pair order, id -> position
+    const uint64_t hq_len = B->hq_pg.text.size(), lq_len = B->lq_pg.text.size();
+    B->s_lq_off.clear();
+    B->s_nlq_off.clear();
+    uint64_t last = 0;
+    for (const auto &r : B->lq_pg.rec) {
+        if (!preserve) {
+            if (paired) B->s_pe_order.push_back(r.read_id);
+            B->s_lq_off.push_back((uint16_t)(L - (rec_pos(r) - last)));
+            last = rec_pos(r);
+        } else {
+            B->s_id_pos[r.read_id] = hq_len + rec_pos(r);
+        }
+    }
+    last = 0;
+    for (const auto &r : B->nlq_pg.rec) {
+        if (!preserve) {
+            if (paired) B->s_pe_order.push_back(B->Nids[r.read_id]);
+            B->s_nlq_off.push_back((uint16_t)(L - (rec_pos(r) - last)));
+            last = rec_pos(r);
+        } else {
+            B->s_id_pos[B->Nids[r.read_id]] = hq_len + lq_len + rec_pos(r);
+        }
+    }
+    return 0;
 }
 
 void orc_block_free(orc_block *B) { delete B; }

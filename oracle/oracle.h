@@ -82,7 +82,19 @@ enum {
     ORC_F_LQ_NEXT = 27,      /* u32[n_unmapped], local ids */
     ORC_F_LQ_PREV = 28,
     ORC_F_LQ_OFFSET = 29,
-    ORC_F_COUNT = 30
+    /* pre-entropy streams of one block (orc_block_streams) */
+    ORC_F_S_STRAND = 40,     /* char '0'/'1' per sorted record */
+    ORC_F_S_READ_OFF = 41,   /* u16 per sorted record (order not preserved) */
+    ORC_F_S_MIS_COUNT = 42,  /* u8 per sorted record */
+    ORC_F_S_MIS_BASE = 43,   /* u8 recoded mismatch symbol per mismatch */
+    ORC_F_S_MIS_TABLE = 44,  /* 16 bytes: decoder rows for contexts A, T, C, G */
+    ORC_F_S_LQ_OFF = 45,     /* u16 per LQ record */
+    ORC_F_S_NLQ_OFF = 46,    /* u16 per N-LQ record */
+    ORC_F_S_PE_ORDER = 47,   /* u32 global ids in emission order (paired, not preserved) */
+    ORC_F_S_ID_POS = 48,     /* u64 per read: joined-PG position (order preserved) */
+    ORC_F_S_SORTED_REC = 49, /* orc_rec, HQ+mapped records after the record sort */
+    ORC_F_S_MIS_OFF = 64,    /* + m (1..max_mm): u16 mismatch-offset stream for m mismatches */
+    ORC_F_COUNT = 64 + 512
 };
 
 /* 16-byte alignment record, layout-compatible with the reference's
@@ -116,6 +128,9 @@ orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int st
 /* Returns the element count of a field and sets *ptr to its data (owned by
  * the block). */
 uint64_t orc_block_get(orc_block *b, int field, const void **ptr);
+/* Record sort and stream builders of block_compress (compress.cu:1730-2044)
+ * for a block run to ORC_STAGE_NLQ; results via the ORC_F_S_* fields. */
+int orc_block_streams(orc_block *b, int preserve_order, int paired_end);
 void orc_block_free(orc_block *b);
 
 #ifdef __cplusplus

@@ -23,7 +23,12 @@ FIELDS = {
     "nlq_pg": (21, np.uint8), "nlq_rec": (22, "rec"), "prev_dedup": (23, np.uint32),
     "next_prune": (24, np.uint32), "prev_prune": (25, np.uint32), "hq_pg2": (26, np.uint64),
     "lq_next": (27, np.uint32), "lq_prev": (28, np.uint32), "lq_offset": (29, np.uint16),
+    "s_strand": (40, np.uint8), "s_read_off": (41, np.uint16), "s_mis_count": (42, np.uint8),
+    "s_mis_base": (43, np.uint8), "s_mis_table": (44, np.uint8), "s_lq_off": (45, np.uint16),
+    "s_nlq_off": (46, np.uint16), "s_pe_order": (47, np.uint32), "s_id_pos": (48, np.uint64),
+    "s_sorted_rec": (49, "rec"),
 }
+MIS_OFF_BASE = 64
 REC_DTYPE = np.dtype([("read_id", "<u4"), ("pad", "<u4"), ("word", "<u8")])
 STAGE = dict(dedup=1, overlap=2, prune=3, fullmatch=4, assembly=5, map=6, lq=7, nlq=8, all=8)
 
@@ -61,6 +66,7 @@ def lib(kind="st"):
         "orc_block_run": (vp, [vp, u64, u32, i32]),
         "orc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "orc_block_free": (None, [vp]),
+        "orc_block_streams": (i32, [vp, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -87,6 +93,23 @@ class OracleBlock:
         if not h:
             raise ValueError("oracle rejected the input")
         self._h = ctypes.c_void_p(h)
+
+    def streams(self, preserve_order=False, paired_end=False):
+        """Builds the block's pre-entropy streams (compress.cu:1730-2044)."""
+        if self.lib.orc_block_streams(self._h, int(preserve_order), int(paired_end)) != 0:
+            raise ValueError("mismatch recount disagrees with the mapping counts")
+
+    def mis_off(self, m):
+        return self._field(MIS_OFF_BASE + m, np.uint16)
+
+    def _field(self, fid, dt):
+        ptr = ctypes.c_void_p()
+        n = self.lib.orc_block_get(self._h, fid, ctypes.byref(ptr))
+        dt = np.dtype(dt)
+        if n == 0 or not ptr.value:
+            return np.zeros(0, dtype=dt)
+        buf = (ctypes.c_char * (n * dt.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=dt, count=n).copy()
 
     def get(self, name):
         fid, dt = FIELDS[name]
