@@ -108,7 +108,9 @@ def main():
     for i in range(a.warmup):
         t = time.perf_counter()
         b = step()
-        log(f"warmup {i}: {time.perf_counter() - t:.3f} s, HQ PG {len(b.get('hq_pg2')) * 32} bases")
+        log(f"warmup {i}: {time.perf_counter() - t:.3f} s; HQ records {len(b.get('hq_rec'))}, "
+            f"HQ PG ~{len(b.get('hq_pg2')) * 32} bases, mapped {len(b.get('map_rec'))}, "
+            f"LQ records {len(b.get('lq_rec'))}, N-LQ records {len(b.get('nlq_rec'))}")
         b.free()
     ctx.reset_stats()
     ctx.set_timing(True)

@@ -14,8 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libcurc_gpu.so")
 
 CURC_OK = 0
-STAGE_DEDUP, STAGE_OVERLAP, STAGE_PRUNE, STAGE_FULLMATCH, STAGE_ASSEMBLY, STAGE_MAP, STAGE_LQ = range(1, 8)
-STAGE_ALL = STAGE_LQ
+STAGE_DEDUP, STAGE_OVERLAP, STAGE_PRUNE, STAGE_FULLMATCH, STAGE_ASSEMBLY, STAGE_MAP, STAGE_LQ, STAGE_NLQ = range(1, 9)
+STAGE_ALL = STAGE_NLQ
 RUN_INPUT_ON_DEVICE = 1
 RUN_KEEP_STAGES = 2
 
@@ -28,7 +28,7 @@ FIELDS = {
     "unmatched_ids": (12, np.uint32), "hq_pg": (13, np.uint8), "hq_rec": (14, "rec"),
     "map_rec": (15, "rec"), "map_mis": (16, np.uint8), "unmapped": (17, np.uint32),
     "unmapped_N": (18, np.uint32), "lq_pg": (19, np.uint8), "lq_rec": (20, "rec"),
-    "prev_dedup": (23, np.uint32), "next_prune": (24, np.uint32), "prev_prune": (25, np.uint32),
+    "nlq_pg": (21, np.uint8), "nlq_rec": (22, "rec"), "prev_dedup": (23, np.uint32), "next_prune": (24, np.uint32), "prev_prune": (25, np.uint32),
     "hq_pg2": (26, np.uint64), "lq_next": (27, np.uint32), "lq_prev": (28, np.uint32),
     "lq_offset": (29, np.uint16), "N_ascii": (30, np.uint8),
 }

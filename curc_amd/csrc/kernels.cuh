@@ -276,13 +276,15 @@ struct ClaimState {
     uint32_t* memb;         // per suffix index: next member
     uint32_t* res_list;
     uint32_t* res_n;
+    uint32_t* pool;         // member lists of contended groups, carved per leader
+    uint32_t* pool_n;
 };
 
 // a6 probe (compress.cu:1474-1528): hash the k-mer at kip+off, binary
 // search the bucket, take the first free non-self candidate if it matches.
 __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t* __restrict__ kr, uint32_t mask,
                                                   uint32_t k, uint32_t kip, uint32_t off) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
     const uint32_t ns = *cs.d_ns;
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(32u, len);
@@ -324,7 +326,7 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
 template <class E>
 __global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t* __restrict__ rng, uint32_t bp,
                                                   uint32_t off) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
     const uint32_t ns = *cs.d_ns;
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(E::BPW, len);
@@ -383,9 +385,56 @@ __global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t 
     }
 }
 
-// One leader per contended group replays it sequentially: members in the
-// reference's suffix order (a6: list order; a8: reversed s[0:off)), each
-// taking the first free non-self equal prefix from the group start.
+// Suffix order inside a group: a6 keeps the suffix list order (compress.cu:
+// 1474, one thread per list entry, sequential meaning); a8 orders by the
+// reversed s[0:off) (see cmp_rev_prefix).
+template <bool FULLMATCH, class E>
+__device__ __forceinline__ bool member_before(const ClaimState& cs, uint32_t u, uint32_t v, uint32_t off) {
+    if (FULLMATCH) return cmp_rev_prefix<E>(cs.R, cs.W, cs.sl[u], cs.sl[v], off) < 0;
+    return u < v;
+}
+
+template <bool FULLMATCH, class E>
+__device__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint32_t off) {
+    if (m <= 16) {
+        for (uint32_t i = 1; i < m; ++i) {
+            uint32_t x = a[i], j = i;
+            while (j > 0 && member_before<FULLMATCH, E>(cs, x, a[j - 1], off)) {
+                a[j] = a[j - 1];
+                --j;
+            }
+            a[j] = x;
+        }
+        return;
+    }
+    // heapsort (members are distinct and strictly ordered)
+    auto sift = [&](uint32_t root, uint32_t end) {
+        for (;;) {
+            uint32_t c = 2 * root + 1;
+            if (c >= end) return;
+            if (c + 1 < end && member_before<FULLMATCH, E>(cs, a[c], a[c + 1], off)) ++c;
+            if (!member_before<FULLMATCH, E>(cs, a[root], a[c], off)) return;
+            uint32_t t = a[root];
+            a[root] = a[c];
+            a[c] = t;
+            root = c;
+        }
+    };
+    for (uint32_t i = m / 2; i-- > 0;) sift(i, m);
+    for (uint32_t end = m; end-- > 1;) {
+        uint32_t t = a[0];
+        a[0] = a[end];
+        a[end] = t;
+        sift(0, end);
+    }
+}
+
+// One leader per contended group replays it as the reference's sequential
+// order would: members sorted by suffix order, each taking the first free
+// non-self prefix of the group's run (compress.cu:1500-1527 for a6; the
+// merge join with self-avoid rotation, compress.cu:1089-1160, for a8).  The
+// free part of the run is always {P[a]} followed by P[b..], so one pass
+// suffices.
 template <bool FULLMATCH, class E>
 __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t off) {
     const uint32_t nq = *cs.res_n;
@@ -396,33 +445,36 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
         uint32_t lead = cs.res_list[q];
         uint32_t g = cs.grp[lead];
         if (cs.head[g] != lead) continue;
-        uint32_t last = INV;
-        for (;;) {
-            uint32_t best = INV;
-            for (uint32_t u = lead; u != INV; u = cs.memb[u]) {
-                if (FULLMATCH) {
-                    if (last != INV && cmp_rev_prefix<E>(cs.R, cs.W, cs.sl[u], cs.sl[last], off) <= 0) continue;
-                    if (best != INV && cmp_rev_prefix<E>(cs.R, cs.W, cs.sl[u], cs.sl[best], off) >= 0) continue;
-                } else {
-                    if (last != INV && u <= last) continue;
-                    if (best != INV && u >= best) continue;
-                }
-                best = u;
-            }
-            if (best == INV) break;
-            last = best;
-            uint32_t s = cs.sl[best];
+        uint32_t m = 0;
+        for (uint32_t u = lead; u != INV; u = cs.memb[u]) ++m;
+        uint32_t* arr = cs.pool + atomicAdd(cs.pool_n, m);
+        m = 0;
+        for (uint32_t u = lead; u != INV; u = cs.memb[u]) arr[m++] = u;
+        sort_members<FULLMATCH, E>(cs, arr, m, off);
+        auto next_free = [&](uint32_t x) {
+            while (x < np && cs.prev[cs.pl[x]] != INV) ++x;
+            return x;
+        };
+        uint32_t a = next_free(g);
+        uint32_t b = next_free(a + 1);
+        for (uint32_t k = 0; k < m && a < np; ++k) {
+            const uint32_t s = cs.sl[arr[k]];
             const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-            uint64_t sx0 = E::word(rs, off, n1);
-            for (uint32_t x = g; x < np; ++x) {
-                uint32_t p = cs.pl[x];
-                if (p == s || cs.prev[p] != INV) continue;
-                if (cmp_ps<E>(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) {
-                    cs.next[s] = p;
-                    cs.prev[p] = s;
-                    cs.offset[s] = (uint16_t)off;
-                }
-                break;
+            const uint64_t sx0 = E::word(rs, off, n1);
+            const uint32_t pa = cs.pl[a];
+            if (pa != s) {
+                if (cmp_ps<E>(cs.R, cs.W, pa, cs.pk[a], rs, sx0, off, len) != 0) break;  // run exhausted
+                cs.next[s] = pa;
+                cs.prev[pa] = s;
+                cs.offset[s] = (uint16_t)off;
+                a = b;
+                b = next_free(b + 1);
+            } else if (b < np && cmp_ps<E>(cs.R, cs.W, cs.pl[b], cs.pk[b], rs, sx0, off, len) == 0) {
+                const uint32_t pb = cs.pl[b];
+                cs.next[s] = pb;
+                cs.prev[pb] = s;
+                cs.offset[s] = (uint16_t)off;
+                b = next_free(b + 1);
             }
         }
         cs.head[g] = INV;

@@ -30,7 +30,7 @@ enum Slot {
     S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_LAST, S_WORD,
     S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
-    S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG,
+    S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG, S_POOL,
     S_LAST_NAMED,
     S_PRIM = 104  // primitives use slots 104.. (scan recursion, sort hist, compaction)
 };
@@ -40,7 +40,7 @@ static_assert(S_LAST_NAMED <= S_PRIM, "scratch slots overlap");
 
 enum Cnt {
     C_N, C_NONN, C_NN, C_NPRE, C_NSUF, C_NS0, C_NS1, C_RESN, C_NP0, C_NP1,
-    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_COUNT = 32
+    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_COUNT = 32
 };
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap = 8192) {
@@ -350,6 +350,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     uint32_t* grp = S.as<uint32_t>(S_GRP, n_max);
     uint32_t* memb = S.as<uint32_t>(S_MEMB, n_max);
     uint32_t* resl = S.as<uint32_t>(S_RESL, n_max);
+    uint32_t* pool = S.as<uint32_t>(S_POOL, n_max);
     uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)n_max + 1);
     uint32_t* cont = S.as<uint32_t>(S_CONT, (uint64_t)n_max + 1);
     uint32_t* head = S.as<uint32_t>(S_HEAD, (uint64_t)n_max + 1);
@@ -373,7 +374,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
         hipLaunchKernelGGL(k_fm_ranges<E>, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, c->st,
                            (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, nb, rng);
         ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
-                      choice, grp, owner, cont, head, memb, resl, cnt + C_RESN};
+                      choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL};
         hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs, (const uint32_t*)rng, bp, off);
         hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off);
         hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
@@ -524,6 +525,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             uint32_t* grp = S.as<uint32_t>(S_GRP, (uint64_t)nsuf + 1);
             uint32_t* memb = S.as<uint32_t>(S_MEMB, (uint64_t)nsuf + 1);
             uint32_t* resl = S.as<uint32_t>(S_RESL, (uint64_t)nsuf + 1);
+            uint32_t* pool = S.as<uint32_t>(S_POOL, (uint64_t)nsuf + 1);
             uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)npre + 1);
             uint32_t* cont = S.as<uint32_t>(S_CONT, (uint64_t)npre + 1);
             uint32_t* head = S.as<uint32_t>(S_HEAD, (uint64_t)npre + 1);
@@ -536,7 +538,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             int cur = 0;
             for (uint32_t o = 1; o < P.max_off; ++o) {
                 ClaimState cs{packed, W, L, sl[cur], ns[cur], va, vk, cnt + C_NPRE, next, prev, off,
-                              choice, grp, owner, cont, head, memb, resl, cnt + C_RESN};
+                              choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL};
                 hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr, mask, P.k,
                                    P.kip, o);
                 hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
@@ -753,6 +755,62 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint64_t* lpg = build_pg(c, lqp, W, a, S_PG_LQ);
         records_to_host(c, a, B->unmapped.data(), B->lq_rec);
         d2h(c, B->lq_pg2, lpg, (a.len + 31) / 32 + 1);
+        CURC_HIP(hipStreamSynchronize(st));
+    }
+    if (stop <= CURC_STAGE_LQ) return;
+
+    // ---- a12: N-read LQ pseudogenome (compress.cu:1724-1728, 1176-1347).
+    // Same exact matching over the N-reads in the 3-bit ACGNT encoding; ids
+    // are N indices, and the visit order is the sorted unmapped list.
+    const uint32_t n_nlq = (uint32_t)B->unmapped_N.size();
+    if (n_nlq) {
+        Region r(c, "nlq");
+        const uint32_t W3 = (L + 20) / 21;
+        uint64_t* p3 = S.as<uint64_t>(S_N3, (uint64_t)nN * W3 + 1);
+        hipLaunchKernelGGL(k_pack3, dim3(grid_for((uint64_t)nN * W3)), dim3(256), 0, st, (const uint8_t*)Nascii,
+                           (uint64_t)nN, L, W3, p3);
+        fill<uint64_t>(c, p3 + (uint64_t)nN * W3, 1, 0ull);
+        uint32_t* nnext = S.as<uint32_t>(S_NNEXT, nN);
+        uint32_t* nprev = S.as<uint32_t>(S_NPREV, nN);
+        uint16_t* noff = S.as<uint16_t>(S_NOFF, nN);
+        fill<uint32_t>(c, nnext, nN, INV);
+        fill<uint32_t>(c, nprev, nN, INV);
+        fill<uint16_t>(c, noff, nN, INVOFF);
+        // compress.cu:1179-1193: sort by content (ties by N index), link equal reads
+        uint64_t* k0 = S.as<uint64_t>(S_K0, (uint64_t)n_nlq + 1);
+        uint64_t* k1 = S.as<uint64_t>(S_K1, (uint64_t)n_nlq + 1);
+        uint32_t* v0 = S.as<uint32_t>(S_V0, (uint64_t)n_nlq + 1);
+        uint32_t* v1 = S.as<uint32_t>(S_V1, (uint64_t)n_nlq + 1);
+        CURC_HIP(hipMemcpyAsync(v0, S.as<uint32_t>(S_UNMAPPED_N, n_nlq), (uint64_t)n_nlq * 4,
+                                hipMemcpyDeviceToDevice, st));
+        for (int w = (int)W3 - 1; w >= 0; --w) {
+            uint32_t nd = (w == (int)W3 - 1) ? (L - 21 * (W3 - 1)) : 21;
+            uint32_t begin = ((63 - 3 * nd) / 8) * 8;
+            hipLaunchKernelGGL(k_gather_word, dim3(grid_for(n_nlq)), dim3(256), 0, st, (const uint64_t*)p3,
+                               (const uint32_t*)v0, (uint64_t)n_nlq, W3, (uint32_t)w, k0);
+            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n_nlq, begin, 64, S, S_PRIM, st);
+        }
+        uint32_t* nsorted = S.as<uint32_t>(S_NSORT, n_nlq);
+        CURC_HIP(hipMemcpyAsync(nsorted, v0, (uint64_t)n_nlq * 4, hipMemcpyDeviceToDevice, st));
+        if (n_nlq > 1)
+            hipLaunchKernelGGL(k_dedup_link, dim3(grid_for(n_nlq)), dim3(256), 0, st, (const uint64_t*)p3,
+                               (const uint32_t*)nsorted, (uint64_t)n_nlq, W3, nnext, nprev, noff);
+        uint32_t* pre = S.as<uint32_t>(S_PRE, n_nlq);
+        uint32_t* suf = S.as<uint32_t>(S_SUF, n_nlq);
+        set_cnt(c, cnt + C_TMP, n_nlq);
+        compact(cnt + C_TMP, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE, S, S_PRIM, st);
+        compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st);
+        full_match<Enc3>(c, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff);
+        AsmOut a = assemble(c, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt);
+        uint8_t* npg = S.as<uint8_t>(S_NPG, a.len + 16);
+        if (a.n)
+            hipLaunchKernelGGL(k_pg_write_ascii, dim3(grid_for(a.n)), dim3(256), 0, st, (const uint8_t*)Nascii, L,
+                               (const uint32_t*)a.rid, (const uint64_t*)a.pos, (const uint32_t*)a.nb,
+                               (uint64_t)a.n, npg);
+        B->nlq_text.resize(a.len);
+        if (a.len) CURC_HIP(hipMemcpyAsync(&B->nlq_text[0], npg, a.len, hipMemcpyDeviceToHost, st));
+        records_to_host(c, a, nullptr, B->nlq_rec);
+        for (auto& rr : B->nlq_rec) rr.word |= 1;  // is_contain_N
         CURC_HIP(hipStreamSynchronize(st));
     }
 }
@@ -980,6 +1038,8 @@ uint64_t curc_block_get(curc_block* B, int field, const void** ptr) {
         case CURC_F_LQ_PREV: return put(B->lq_prev);
         case CURC_F_LQ_OFFSET: return put(B->lq_off);
         case CURC_F_N_ASCII: return put(B->Nascii);
+        case CURC_F_NLQ_PG: return put(B->nlq_text);
+        case CURC_F_NLQ_REC: return put(B->nlq_rec);
         default: *ptr = nullptr; return 0;
     }
 }

@@ -89,7 +89,8 @@ enum {
     CURC_STAGE_ASSEMBLY = 5,  /* a9: HQ pseudogenome */
     CURC_STAGE_MAP = 6,       /* a10: mapping onto the HQ pseudogenome */
     CURC_STAGE_LQ = 7,        /* a11: LQ pseudogenome of unmapped reads */
-    CURC_STAGE_ALL = 7
+    CURC_STAGE_NLQ = 8,       /* a12: N-read LQ pseudogenome of unmapped N-reads */
+    CURC_STAGE_ALL = 8
 };
 
 /* Result fields (numbering shared with the oracle's field ids). */
@@ -115,6 +116,8 @@ enum {
     CURC_F_UNMAPPED_N = 18,    /* u32 N indices of unmapped N-reads */
     CURC_F_LQ_PG = 19,         /* char */
     CURC_F_LQ_REC = 20,        /* curc_alignment, global ids */
+    CURC_F_NLQ_PG = 21,        /* char, N-read LQ pseudogenome text */
+    CURC_F_NLQ_REC = 22,       /* curc_alignment, N indices, is_contain_N set */
     CURC_F_PREV_DEDUP = 23,
     CURC_F_NEXT_PRUNE = 24,
     CURC_F_PREV_PRUNE = 25,
@@ -131,12 +134,11 @@ enum {
 
 typedef struct curc_block curc_block;
 
-/* GPU part of block_compress (compress.cu:1349-1722): N-split and pack,
- * dedup, prefix index, greedy overlap, prune/partition, exact all-offset
- * matching, assembly, mapping and the LQ pseudogenome, for n_reads ASCII
- * reads of length L (block-global ids = input order).  The N-read LQ
- * pseudogenome, record sorting, stream building and entropy coding stay on
- * the CPU as in the reference. */
+/* Pseudogenome part of block_compress (compress.cu:1349-1728): N-split and
+ * pack, dedup, prefix index, greedy overlap, prune/partition, exact
+ * all-offset matching, assembly, mapping, the LQ pseudogenome and the N-read
+ * LQ pseudogenome (which the reference runs on the CPU, full_match_cpu), for
+ * n_reads ASCII reads of length L (block-global ids = input order). */
 int curc_block_run(curc_ctx *ctx, const char *ascii, uint64_t n_reads, uint32_t L,
                    int stop_after, uint32_t flags, curc_block **out);
 uint64_t curc_block_get(curc_block *b, int field, const void **ptr);

@@ -19,6 +19,7 @@ STAGE_FIELDS = [
     ("hq_pg", "hq_rec", "hq_pg2"),
     ("map_rec", "map_mis", "unmapped", "unmapped_N"),
     ("lq_next", "lq_prev", "lq_offset", "lq_pg", "lq_rec"),
+    ("nlq_pg", "nlq_rec"),
 ]
 
 
@@ -35,7 +36,7 @@ def compare(gpu_blk, orc_blk, fields):
 def run_both(ctx, reads):
     print(f"case: {reads.shape[0]} reads x {reads.shape[1]} bp", flush=True)
     g = ctx.run_block(reads, keep_stages=True)
-    o = orc.OracleBlock(reads, stop="lq")
+    o = orc.OracleBlock(reads, stop="all")
     for group in STAGE_FIELDS:
         compare(g, o, group)
     packed = g.get("packed")
