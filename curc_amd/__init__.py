@@ -18,6 +18,10 @@ STAGE_DEDUP, STAGE_OVERLAP, STAGE_PRUNE, STAGE_FULLMATCH, STAGE_ASSEMBLY, STAGE_
 STAGE_ALL = STAGE_NLQ
 RUN_INPUT_ON_DEVICE = 1
 RUN_KEEP_STAGES = 2
+RUN_STREAMS = 4
+RUN_PRESERVE_ORDER = 8
+RUN_PAIRED = 16
+MIS_OFF_BASE = 64
 
 # field id -> numpy dtype (ids shared with the oracle)
 FIELDS = {
@@ -31,6 +35,10 @@ FIELDS = {
     "nlq_pg": (21, np.uint8), "nlq_rec": (22, "rec"), "prev_dedup": (23, np.uint32), "next_prune": (24, np.uint32), "prev_prune": (25, np.uint32),
     "hq_pg2": (26, np.uint64), "lq_next": (27, np.uint32), "lq_prev": (28, np.uint32),
     "lq_offset": (29, np.uint16), "N_ascii": (30, np.uint8),
+    "s_strand": (40, np.uint8), "s_read_off": (41, np.uint16), "s_mis_count": (42, np.uint8),
+    "s_mis_base": (43, np.uint8), "s_mis_table": (44, np.uint8), "s_lq_off": (45, np.uint16),
+    "s_nlq_off": (46, np.uint16), "s_pe_order": (47, np.uint32), "s_id_pos": (48, np.uint64),
+    "s_sorted_rec": (49, "rec"),
 }
 
 # curc_alignment: u32 read_id, u32 pad, u64 word (isN | strand << 1 | pos << 2)
@@ -170,8 +178,15 @@ class Block:
     def __init__(self, handle):
         self._h = ctypes.c_void_p(handle)
 
+    def mis_off(self, m):
+        """Mismatch-offset stream for reads with m mismatches."""
+        return self._field(MIS_OFF_BASE + m, np.uint16)
+
     def get(self, name):
         fid, dt = FIELDS[name]
+        return self._field(fid, dt)
+
+    def _field(self, fid, dt):
         ptr = ctypes.c_void_p()
         n = lib().curc_block_get(self._h, fid, ctypes.byref(ptr))
         if dt == "rec":
@@ -208,10 +223,14 @@ class Context:
             msg = lib().curc_ctx_last_error(self._h)
             raise CurcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def run_block(self, reads, L=None, stop_after=STAGE_ALL, keep_stages=False, device_ptr=None, n_reads=None):
+    def run_block(self, reads, L=None, stop_after=STAGE_ALL, keep_stages=False, device_ptr=None, n_reads=None,
+                  streams=False, preserve_order=False, paired=False):
         """Runs the GPU part of block_compress on reads ((n, L) uint8 host
-        array) or on a device buffer (device_ptr, n_reads, L)."""
+        array) or on a device buffer (device_ptr, n_reads, L); with streams,
+        also the record sort and stream builders."""
         flags = RUN_KEEP_STAGES if keep_stages else 0
+        if streams:
+            flags |= RUN_STREAMS | (RUN_PRESERVE_ORDER if preserve_order else 0) | (RUN_PAIRED if paired else 0)
         out = ctypes.c_void_p()
         if device_ptr is not None:
             flags |= RUN_INPUT_ON_DEVICE

@@ -1,0 +1,320 @@
+"""The .curc archive: container, entropy coding and decoding.
+
+Layout (preprocess.cpp:261-276, compress.cu:1364-2147; read back by
+decompress.cu:760-1060):
+
+    archive := u8 preserve_order, u8 paired_end, u16 read_len, u8 block_count, block*
+    block   := u32 reads_count, 16 B mismatch decoder rows (contexts A, T, C, G),
+               [u8 is_join_ref_len_std if preserve_order], u8 is_ref_len_std,
+               u64 hq_len, u64 lq_len, u64 n_len,
+               sub-files, each u64 size + payload, in this order:
+                 [id_pos if preserve_order], strand_id, ref,
+                 [read_off, unmapping_N_read_off, unmapping_read_off, [pe_order if paired]
+                  if not preserve_order],
+                 ref_off.map, ref_len.map, unmap_ref_off.map, unmap_ref_len.map,
+                 unmap_n_ref_off.map, unmap_n_ref_len.map, mismatch_count, mismatch_base,
+               u8 max_mismatch, mismatch_off_1 .. mismatch_off_max (each u64 size + payload)
+
+The pre-entropy streams are CURC's.  Each sub-file payload here is an xz
+(LZMA2) stream; the pseudogenome text is first packed to 2 bits per base
+with an exception list for other symbols.  The reference's coders (PgRC's
+PPMd/LZMA framing, VarLenDNA + fast-lzma2, bsc, a range coder) are
+third-party code this package does not carry, so archives are not
+byte-compatible with CURC's; the streams they hold are.
+"""
+import io
+import lzma
+import struct
+
+import numpy as np
+
+XZ_PRESET = 6
+COMP = np.zeros(256, dtype=np.uint8)
+for _a, _b in zip(b"ACGTNacgtn", b"TGCANtgcan"):
+    COMP[_a] = _b
+
+
+# ------------------------------------------------------------------ coders ---
+def xz(data: bytes) -> bytes:
+    return lzma.compress(data, format=lzma.FORMAT_XZ, preset=XZ_PRESET)
+
+
+def unxz(data: bytes) -> bytes:
+    return lzma.decompress(data, format=lzma.FORMAT_XZ)
+
+
+def pack_dna(text: bytes) -> bytes:
+    """2-bit ACGT packing with (position, byte) exceptions, then xz."""
+    a = np.frombuffer(text, dtype=np.uint8)
+    code = np.full(256, 255, dtype=np.uint8)
+    code[[65, 67, 71, 84]] = [0, 1, 2, 3]
+    c = code[a]
+    exc = np.flatnonzero(c == 255)
+    c = np.where(c == 255, 0, c).astype(np.uint8)
+    pad = (-len(c)) % 4
+    c4 = np.concatenate([c, np.zeros(pad, dtype=np.uint8)]).reshape(-1, 4)
+    packed = (c4[:, 0] << 6) | (c4[:, 1] << 4) | (c4[:, 2] << 2) | c4[:, 3]
+    head = struct.pack("<QQ", len(a), len(exc))
+    body = head + exc.astype("<u8").tobytes() + a[exc].tobytes() + packed.astype(np.uint8).tobytes()
+    return xz(body)
+
+
+def unpack_dna(blob: bytes) -> bytes:
+    body = unxz(blob)
+    n, ne = struct.unpack_from("<QQ", body, 0)
+    off = 16
+    exc = np.frombuffer(body, dtype="<u8", count=ne, offset=off)
+    off += 8 * ne
+    excv = np.frombuffer(body, dtype=np.uint8, count=ne, offset=off)
+    off += ne
+    packed = np.frombuffer(body, dtype=np.uint8, offset=off)
+    c = np.stack([(packed >> 6) & 3, (packed >> 4) & 3, (packed >> 2) & 3, packed & 3], axis=1).reshape(-1)[:n]
+    out = np.frombuffer(b"ACGT", dtype=np.uint8)[c].copy()
+    out[exc.astype(np.int64)] = excv
+    return out.tobytes()
+
+
+# ------------------------------------------------------------ block streams ---
+class BlockStreams:
+    """Pre-entropy streams of one block (the fields CURC writes)."""
+
+    FIELDS = ("reads_count", "mis_table", "hq_text", "lq_text", "nlq_text", "strand", "read_off", "lq_off",
+              "nlq_off", "pe_order", "id_pos", "mis_count", "mis_base", "mis_off", "max_mm", "maps")
+
+    def __init__(self, **kw):
+        for f in self.FIELDS:
+            setattr(self, f, kw.get(f))
+        if self.maps is None:
+            self.maps = {k: b"" for k in ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len",
+                                          "unmap_n_ref_off", "unmap_n_ref_len")}
+
+    @classmethod
+    def from_block(cls, blk, n_reads, L, preserve, paired):
+        """From a curc_amd.Block run with streams=True (or an oracle block
+        after .streams(); both expose the same field names)."""
+        max_mm = L // 5
+        hq = blk.get("hq_pg").tobytes()
+        return cls(
+            reads_count=n_reads, mis_table=blk.get("s_mis_table").tobytes(), hq_text=hq,
+            lq_text=blk.get("lq_pg").tobytes(), nlq_text=blk.get("nlq_pg").tobytes(),
+            strand=blk.get("s_strand").tobytes(), read_off=blk.get("s_read_off"), lq_off=blk.get("s_lq_off"),
+            nlq_off=blk.get("s_nlq_off"), pe_order=blk.get("s_pe_order"), id_pos=blk.get("s_id_pos"),
+            mis_count=blk.get("s_mis_count"), mis_base=blk.get("s_mis_base"),
+            mis_off={m: blk.mis_off(m) for m in range(1, max_mm + 1)}, max_mm=max_mm)
+
+
+def _pe_order_encode(pe_order):
+    """Pairs in emission order (paired_end_reads_order_compress,
+    util.cuh:444-525, carries the same information): for each pair, first
+    seen at emission index e, the distance to its mate's index and whether
+    the first-seen read is the second mate."""
+    pe_order = np.asarray(pe_order, dtype=np.int64)
+    n = len(pe_order)
+    idx = np.empty(n, dtype=np.int64)
+    idx[pe_order] = np.arange(n)
+    mate_idx = idx[pe_order ^ 1]
+    first = mate_idx > np.arange(n)
+    e = np.flatnonzero(first)
+    delta = (mate_idx[e] - e).astype("<u4")
+    second = (pe_order[e] & 1).astype(np.uint8)
+    return delta.tobytes(), second.tobytes()
+
+
+def _pe_order_decode(delta, second, n):
+    """Emission indices in output order: even positions are first mates."""
+    delta = np.frombuffer(delta, dtype="<u4").astype(np.int64)
+    second = np.frombuffer(second, dtype=np.uint8)
+    taken = np.zeros(n, dtype=bool)
+    out = np.empty(n, dtype=np.int64)
+    k = 0
+    j = 0
+    for i in range(n):
+        if taken[i]:
+            continue
+        m = i + int(delta[k])
+        taken[m] = True
+        a, b = (m, i) if second[k] else (i, m)
+        out[j], out[j + 1] = a, b
+        j += 2
+        k += 1
+    return out
+
+
+def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired: bool):
+    out.write(struct.pack("<I", bs.reads_count))
+    out.write(bs.mis_table)
+    joined = len(bs.hq_text) + len(bs.lq_text) + len(bs.nlq_text)
+    if preserve:
+        out.write(struct.pack("<B", 1 if joined <= 0xFFFFFFFF else 0))
+    out.write(struct.pack("<B", 1 if len(bs.hq_text) <= 0xFFFFFFFF else 0))
+    out.write(struct.pack("<QQQ", len(bs.hq_text), len(bs.lq_text), len(bs.nlq_text)))
+
+    def sub(payload: bytes):
+        out.write(struct.pack("<Q", len(payload)))
+        out.write(payload)
+
+    if preserve:
+        dt = "<u4" if joined <= 0xFFFFFFFF else "<u8"
+        sub(xz(np.asarray(bs.id_pos).astype(dt).tobytes()))
+    sub(xz(bs.strand))
+    sub(pack_dna(bs.hq_text + bs.lq_text + bs.nlq_text))
+    if not preserve:
+        sub(xz(np.asarray(bs.read_off, dtype="<u2").tobytes()))
+        sub(xz(np.asarray(bs.nlq_off, dtype="<u2").tobytes()))
+        sub(xz(np.asarray(bs.lq_off, dtype="<u2").tobytes()))
+        if paired:
+            d, s = _pe_order_encode(bs.pe_order)
+            sub(xz(struct.pack("<Q", len(d)) + d + s))
+    for k in ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len", "unmap_n_ref_off", "unmap_n_ref_len"):
+        sub(xz(bs.maps[k]))
+    sub(xz(np.asarray(bs.mis_count, dtype=np.uint8).tobytes()))
+    sub(xz(np.asarray(bs.mis_base, dtype=np.uint8).tobytes()))
+    out.write(struct.pack("<B", bs.max_mm))
+    for m in range(1, bs.max_mm + 1):
+        sub(xz(np.asarray(bs.mis_off.get(m, np.zeros(0)), dtype="<u2").tobytes()))
+
+
+def read_block(inp: io.BufferedIOBase, preserve: bool, paired: bool, L: int) -> BlockStreams:
+    def rd(fmt):
+        return struct.unpack(fmt, inp.read(struct.calcsize(fmt)))
+
+    (reads_count,) = rd("<I")
+    mis_table = inp.read(16)
+    if preserve:
+        (join_std,) = rd("<B")
+    rd("<B")
+    hq_len, lq_len, n_len = rd("<QQQ")
+
+    def sub():
+        (n,) = rd("<Q")
+        return inp.read(n)
+
+    id_pos = None
+    if preserve:
+        dt = "<u4" if join_std else "<u8"
+        id_pos = np.frombuffer(unxz(sub()), dtype=dt).astype(np.uint64)
+    strand = unxz(sub())
+    text = unpack_dna(sub())
+    read_off = lq_off = nlq_off = pe_order = None
+    pe_pairs = None
+    if not preserve:
+        read_off = np.frombuffer(unxz(sub()), dtype="<u2")
+        nlq_off = np.frombuffer(unxz(sub()), dtype="<u2")
+        lq_off = np.frombuffer(unxz(sub()), dtype="<u2")
+        if paired:
+            raw = unxz(sub())
+            (nd,) = struct.unpack_from("<Q", raw, 0)
+            pe_pairs = (raw[8:8 + nd], raw[8 + nd:])
+    maps = {}
+    for k in ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len", "unmap_n_ref_off", "unmap_n_ref_len"):
+        maps[k] = unxz(sub())
+    mis_count = np.frombuffer(unxz(sub()), dtype=np.uint8)
+    mis_base = np.frombuffer(unxz(sub()), dtype=np.uint8)
+    (max_mm,) = rd("<B")
+    mis_off = {m: np.frombuffer(unxz(sub()), dtype="<u2") for m in range(1, max_mm + 1)}
+    bs = BlockStreams(reads_count=reads_count, mis_table=mis_table, hq_text=text[:hq_len],
+                      lq_text=text[hq_len:hq_len + lq_len], nlq_text=text[hq_len + lq_len:],
+                      strand=strand, read_off=read_off, lq_off=lq_off, nlq_off=nlq_off, pe_order=pe_order,
+                      id_pos=id_pos, mis_count=mis_count, mis_base=mis_base, mis_off=mis_off, max_mm=max_mm,
+                      maps=maps)
+    bs.pe_pairs = pe_pairs
+    return bs
+
+
+def write_archive(path, blocks, preserve, paired, L):
+    if len(blocks) >= 255:  # preprocess.cpp:215-216
+        raise ValueError("too many blocks")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<BBHB", int(preserve), int(paired), L, len(blocks)))
+        for bs in blocks:
+            write_block(f, bs, preserve, paired)
+
+
+def read_archive(path):
+    with open(path, "rb") as f:
+        preserve, paired, L, nb = struct.unpack("<BBHB", f.read(5))
+        blocks = [read_block(f, bool(preserve), bool(paired), L) for _ in range(nb)]
+    return bool(preserve), bool(paired), L, blocks
+
+
+# ------------------------------------------------------------------ decode ---
+def _offsets(stream, m, n_rec, L):
+    """Mismatch positions (n_rec x m) from the offset stream for m mismatches
+    (inverse of compress.cu:1848-1886)."""
+    w = 2 if m == 1 else 3 if m == 2 else m + 2
+    v = np.asarray(stream, dtype=np.int64).reshape(n_rec, w)
+    flag, first = v[:, 0], v[:, 1]
+    l2r = flag == 0
+    if m == 1:
+        return np.where(l2r, first, L - 1 - first)[:, None]
+    if m == 2:
+        gap = v[:, 2] + 1
+        a = np.where(l2r, first, L - 1 - first - gap)
+        return np.stack([a, a + gap], axis=1)
+    mn = v[:, 2:3] + 1
+    gaps = v[:, 3:] + mn
+    cum = np.cumsum(gaps, axis=1)
+    fwd = np.concatenate([first[:, None], first[:, None] + cum], axis=1)
+    last = L - 1 - first
+    bwd = np.concatenate([last[:, None], last[:, None] - cum], axis=1)[:, ::-1]
+    return np.where(l2r[:, None], fwd, bwd)
+
+
+def _apply_records(pg, pos, strand, mis_count, mis_base, mis_off, table, L):
+    """Reads of the HQ/mapped records (compress.cu:1815-1846 inverted)."""
+    n = len(pos)
+    pga = np.frombuffer(pg, dtype=np.uint8)
+    reads = pga[pos[:, None].astype(np.int64) + np.arange(L)[None, :]] if n else np.zeros((0, L), np.uint8)
+    reads = reads.copy()
+    if n and len(mis_base):
+        row = np.zeros(256, dtype=np.int64)
+        row[[ord("A"), ord("T"), ord("C"), ord("G")]] = [0, 1, 2, 3]
+        tab = np.frombuffer(table, dtype=np.uint8).reshape(4, 4)
+        base_idx = np.concatenate([[0], np.cumsum(mis_count.astype(np.int64))])
+        for m in np.unique(mis_count):
+            if m == 0:
+                continue
+            sel = np.flatnonzero(mis_count == m)
+            offs = _offsets(mis_off[int(m)], int(m), len(sel), L)
+            syms = mis_base[(base_idx[sel][:, None] + np.arange(m)[None, :]).reshape(-1)].reshape(len(sel), m)
+            r = np.repeat(sel, m)
+            o = offs.reshape(-1)
+            ctx = reads[r, o]
+            reads[r, o] = tab[row[ctx], syms.reshape(-1)]
+    rc = np.frombuffer(strand, dtype=np.uint8) == ord("1")
+    if rc.any():
+        reads[rc] = COMP[reads[rc][:, ::-1]]
+    return reads
+
+
+def decode_block(bs: BlockStreams, L: int, preserve: bool, paired: bool):
+    """Reads of one block in CURC's output order (decompress.cu:909-1005):
+    without preserve_order HQ/mapped records in position order, then LQ,
+    then N-LQ (pairs regrouped when paired); with it, by read id."""
+    n_rec = len(bs.mis_count)
+    if not preserve:
+        pos = np.cumsum(bs.read_off.astype(np.int64))
+        hq = _apply_records(bs.hq_text, pos, bs.strand, bs.mis_count, bs.mis_base, bs.mis_off, bs.mis_table, L)
+
+        def chain(text, off):
+            p = np.cumsum(L - off.astype(np.int64))
+            a = np.frombuffer(text, dtype=np.uint8)
+            return a[p[:, None] + np.arange(L)[None, :]] if len(p) else np.zeros((0, L), np.uint8)
+
+        reads = np.concatenate([hq, chain(bs.lq_text, bs.lq_off), chain(bs.nlq_text, bs.nlq_off)])
+        if paired:
+            order = _pe_order_decode(bs.pe_pairs[0], bs.pe_pairs[1], len(reads))
+            reads = reads[order]
+        return reads
+    # order preserved: id_pos gives each read's joined-pseudogenome position
+    id_pos = bs.id_pos.astype(np.int64)
+    hq_len, lq_len = len(bs.hq_text), len(bs.lq_text)
+    joined = np.frombuffer(bs.hq_text + bs.lq_text + bs.nlq_text, dtype=np.uint8)
+    out = joined[id_pos[:, None] + np.arange(L)[None, :]].copy()
+    in_hq = id_pos < hq_len
+    del lq_len
+    hq_ids = np.flatnonzero(in_hq)
+    assert len(hq_ids) == n_rec
+    out[hq_ids] = _apply_records(bs.hq_text, id_pos[hq_ids], bs.strand, bs.mis_count, bs.mis_base, bs.mis_off,
+                                 bs.mis_table, L)
+    return out

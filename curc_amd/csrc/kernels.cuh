@@ -482,16 +482,31 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
     }
 }
 
-// Block ranges of the lexicographic prefix list by its first bp bases
-// (nb = number of block values, 4^bp or 8^bp).
+// Histogram of the prefix list's block values (its first bp bases); an
+// exclusive scan of it gives the block ranges.  The list is sorted, so equal
+// blocks are adjacent: each wave adds one count per run it holds.
 template <class E>
-__global__ void k_fm_ranges(const uint64_t* __restrict__ pk, const uint32_t* __restrict__ d_np, uint32_t bp,
-                            uint64_t nb, uint32_t* __restrict__ rng) {
+__global__ void __launch_bounds__(256) k_fm_block_hist(const uint64_t* __restrict__ pk,
+                                                       const uint32_t* __restrict__ d_np, uint32_t bp,
+                                                       uint32_t* __restrict__ hist) {
     const uint32_t np = *d_np;
-    GRID_STRIDE(i, (uint64_t)np + 1) {
-        uint64_t lo = (i == 0) ? 0 : E::head(pk[i - 1], bp) + 1;
-        uint64_t hi = (i == np) ? nb : E::head(pk[i], bp);
-        for (uint64_t b = lo; b <= hi; ++b) rng[b] = (uint32_t)i;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < np; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        const bool valid = i < np;
+        const uint64_t key = valid ? E::head(pk[i], bp) : ~0ull;
+        uint64_t pkey = __shfl_up(key, 1, 64);
+        const bool start = valid && (lane == 0 || pkey != key);
+        const uint64_t starts = __ballot(start);
+        const uint64_t vmask = __ballot(valid);
+        if (start) {
+            // lanes of this run: from this lane up to the next start (or the last valid lane)
+            const uint64_t after = starts & ~(lt | (1ull << lane));
+            const uint64_t upto = after ? (after & (~after + 1)) - 1 : ~0ull;  // lanes below the next start
+            const uint32_t count = (uint32_t)__popcll(vmask & upto & ~lt);
+            atomicAdd(&hist[key], count);
+        }
     }
 }
 
@@ -693,20 +708,31 @@ struct MapArgs {
     uint64_t n_N;
     uint8_t* last;
     uint64_t* word;            // record word per mapping slot
+    unsigned long long* work;  // [active reads, k-mer probes, verified candidates]
 };
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
+    uint64_t s = v;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (__lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
 
 // compress.cu:411-521: one thread per read; candidates in bucket order,
 // a strictly better mismatch count replaces the record, stop at <= 1.
 template <bool RC>
 __global__ void __launch_bounds__(256) k_map(MapArgs a) {
+    uint32_t n_act = 0, n_probe = 0, n_cand = 0;
     GRID_STRIDE(i, a.n_un + a.n_N) {
         uint32_t last = a.last[i];
         if (last <= MAP_TARGET) continue;
+        ++n_act;
         const bool isN = i >= a.n_un;
         uint64_t w = a.word[i];
         if (!isN) {
             const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * a.W;
             for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
+                ++n_probe;
                 uint32_t h = (uint32_t)(murmur64(xw(rd, j, MAP_K)) & a.mask);
                 uint32_t b = a.kr[h], e = a.kr[h + 1];
                 if (b == e) continue;
@@ -717,6 +743,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
                     if (rp < ls) continue;
                     uint64_t mp = rp - ls;
                     if (mp + a.L > a.pg_len) continue;
+                    ++n_cand;
                     uint32_t c = RC ? rc_mm(a.pg, mp, rd, a.L, last - 1) : fwd_mm(a.pg, mp, rd, a.L, last - 1);
                     if (c < last) {
                         w = (RC ? 2ull : 0ull) | (mp << 2);
@@ -728,6 +755,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         } else {
             const uint8_t* ra = a.nascii + (i - a.n_un) * a.L;
             for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
+                ++n_probe;
                 uint32_t h = (uint32_t)(murmur64(kmer_ascii(ra + j, MAP_K)) & a.mask);
                 uint32_t b = a.kr[h], e = a.kr[h + 1];
                 if (b == e) continue;
@@ -738,6 +766,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
                     if (rp < ls) continue;
                     uint64_t mp = rp - ls;
                     if (mp + a.L > a.pg_len) continue;
+                    ++n_cand;
                     uint32_t c = 0;
                     for (uint32_t o = 0; o < a.L; ++o) {
                         uint8_t rb = base_char(base2(a.pg, mp + o));
@@ -759,6 +788,9 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         a.last[i] = (uint8_t)last;
         a.word[i] = w;
     }
+    wave_add_u64(&a.work[0], n_act);
+    wave_add_u64(&a.work[1], n_probe);
+    wave_add_u64(&a.work[2], n_cand);
 }
 
 __global__ void k_map_init(uint8_t* __restrict__ last, uint64_t* __restrict__ word, uint64_t n_un, uint64_t n,

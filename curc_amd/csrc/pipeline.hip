@@ -14,6 +14,7 @@
 #include "../../include/curc_gpu.h"
 #include "assembly.cuh"
 #include "kernels.cuh"
+#include "streams.cuh"
 
 using namespace curc;
 
@@ -31,8 +32,12 @@ enum Slot {
     S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
     S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG, S_POOL,
+    S_LQIDS, S_LQPOS, S_LQNB, S_NLQIDS, S_NLQPOS, S_NLQNB,
+    S_ST_RID, S_ST_RWORD, S_ST_STRAND, S_ST_MISC, S_ST_ROFF, S_ST_PE, S_ST_IDPOS, S_ST_MKEY, S_ST_MIDX,
+    S_ST_MKEY2, S_ST_MIDX2, S_ST_BIDX, S_ST_OIDX, S_ST_GSTART, S_ST_MBASE, S_ST_PAIR, S_ST_OFFS, S_ST_HIST,
+    S_ST_RANK, S_ST_SYM, S_ST_LQOFF, S_ST_NLQOFF, S_MWORK,
     S_LAST_NAMED,
-    S_PRIM = 104  // primitives use slots 104.. (scan recursion, sort hist, compaction)
+    S_PRIM = 176  // primitives use slots 176.. (scan recursion, sort hist, compaction)
 };
 
 // Device counters (u32) in slot S_CNT.
@@ -206,6 +211,13 @@ struct curc_block {
     std::vector<char> Nascii;
     std::string nlq_text;
     std::vector<curc_alignment> nlq_rec;
+    // pre-entropy streams (CURC_RUN_STREAMS)
+    std::vector<char> s_strand;
+    std::vector<uint16_t> s_read_off, s_lq_off, s_nlq_off, s_mis_off;
+    std::vector<uint8_t> s_mis_count, s_mis_base, s_mis_table;
+    std::vector<uint32_t> s_pe_order;
+    std::vector<uint64_t> s_id_pos, s_mis_off_start;
+    std::vector<curc_alignment> s_sorted;
 };
 
 namespace {
@@ -231,16 +243,17 @@ uint64_t* build_pg(curc_ctx* c, const uint64_t* R, uint32_t W, const AsmOut& a, 
 }
 
 // Assembly over the visit list V (n reads; ids in the space of the link
-// arrays, which hold id_space entries).  See assembly.cuh.
+// arrays, which hold id_space entries).  Records land in the three scratch
+// slots starting at out_slot.  See assembly.cuh.
 AsmOut assemble(curc_ctx* c, const uint32_t* V, uint32_t n, uint64_t id_space, const uint32_t* next,
-                const uint32_t* prev, const uint16_t* offset, uint32_t L, uint32_t* cnt) {
+                const uint32_t* prev, const uint16_t* offset, uint32_t L, uint32_t* cnt, int out_slot) {
     AsmOut out;
     out.n = n;
     Scratch& S = c->S;
     hipStream_t st = c->st;
-    out.rid = S.as<uint32_t>(S_PGIDS, (uint64_t)n + 1);
-    out.pos = S.as<uint64_t>(S_PGPOS, (uint64_t)n + 1);
-    out.nb = S.as<uint32_t>(S_PGNB, (uint64_t)n + 1);
+    out.rid = S.as<uint32_t>(out_slot, (uint64_t)n + 1);
+    out.pos = S.as<uint64_t>(out_slot + 1, (uint64_t)n + 1);
+    out.nb = S.as<uint32_t>(out_slot + 2, (uint64_t)n + 1);
     if (n == 0) return out;
     uint32_t* rank = S.as<uint32_t>(S_A_RANK, id_space + 1);
     uint32_t* nxtR = S.as<uint32_t>(S_A_NXT, n);
@@ -340,7 +353,9 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
                 const uint32_t* d_suf, uint32_t* cnt, uint32_t n_max, uint32_t* next, uint32_t* prev,
                 uint16_t* offset) {
     constexpr uint32_t BITS = (E::BPW == 32) ? 2 : 3;
-    constexpr uint32_t MAX_BP = (BITS == 2) ? 10 : 6;  // 2^20 or 2^18 block values
+    // block width: about one block per list entry, at most 2^20 block values
+    uint32_t MAX_BP = 1;
+    while (MAX_BP < ((BITS == 2) ? 10u : 6u) && (1ull << (BITS * MAX_BP)) < n_max) ++MAX_BP;
     if (n_max == 0) return;
     Scratch& S = c->S;
     uint32_t* pl[2] = {S.as<uint32_t>(S_PL0, n_max), S.as<uint32_t>(S_PL1, n_max)};
@@ -371,8 +386,10 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     for (uint32_t off = 1; off < L; ++off) {
         const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
         const uint64_t nb = 1ull << (BITS * bp);
-        hipLaunchKernelGGL(k_fm_ranges<E>, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, c->st,
-                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, nb, rng);
+        fill<uint32_t>(c, rng, nb + 1, 0u);
+        hipLaunchKernelGGL(k_fm_block_hist<E>, dim3(grid_for(n_max, 4096)), dim3(256), 0, c->st,
+                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng);
+        scan_excl(rng, rng, nb + 1, S, S_PRIM, c->st);
         ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
                       choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL};
         hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs, (const uint32_t*)rng, bp, off);
@@ -383,6 +400,149 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
         compact(np[cur], PredPreFree{pl[cur], prev}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
                 np[1 - cur], S, S_PRIM, c->st);
         cur = 1 - cur;
+    }
+}
+
+// Record sort and stream builders (a13; compress.cu:1730-2044), on the
+// device where the records are.  Fills the s_* fields of B.
+void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOut& lq_asm, const AsmOut& nlq_asm,
+                   uint32_t* cnt, bool preserve, bool paired) {
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    const uint32_t L = B->L, W = B->W;
+    curc_params P;
+    curc_make_params(L, &P);
+    const uint32_t max_mm = P.max_mm;
+    const uint64_t N = B->n_reads;
+    const uint32_t nmap = (uint32_t)B->map_rec.size();
+    const uint32_t n = hq_asm.n + nmap;
+    const uint32_t nN = (uint32_t)B->Nids.size();
+    RecSrc src{hq_asm.rid, hq_asm.pos, hq_asm.n, S.as<uint32_t>(S_MREC_ID, (uint64_t)nmap + 1),
+               S.as<uint64_t>(S_MREC_W, (uint64_t)nmap + 1), S.as<uint8_t>(S_MIS, (uint64_t)nmap + 1),
+               S.as<uint32_t>(S_NIDS, (uint64_t)nN + 1)};
+    // compress.cu:1743-1756 (stable; HQ records precede map records on ties)
+    uint64_t* k0 = S.as<uint64_t>(S_K0, (uint64_t)n + 1);
+    uint64_t* k1 = S.as<uint64_t>(S_K1, (uint64_t)n + 1);
+    uint32_t* v0 = S.as<uint32_t>(S_V0, (uint64_t)n + 1);
+    uint32_t* v1 = S.as<uint32_t>(S_V1, (uint64_t)n + 1);
+    const uint32_t g = grid_for(n);
+    if (n) hipLaunchKernelGGL(k_rec_keys, dim3(g), dim3(256), 0, st, src, n, preserve, k0, v0);
+    const uint32_t kbits = preserve ? log2u(N + 1) : log2u(B->hq_len + 1);
+    radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, 0, ((kbits + 7) / 8) * 8, S, S_PRIM, st);
+    StreamOut o;
+    o.rid = S.as<uint32_t>(S_ST_RID, (uint64_t)n + 1);
+    o.rword = S.as<uint64_t>(S_ST_RWORD, (uint64_t)n + 1);
+    o.strand = S.as<uint8_t>(S_ST_STRAND, (uint64_t)n + 1);
+    o.mis_count = S.as<uint8_t>(S_ST_MISC, (uint64_t)n + 1);
+    o.read_off = S.as<uint16_t>(S_ST_ROFF, (uint64_t)n + 1);
+    const uint64_t n_pe = (uint64_t)n + lq_asm.n + nlq_asm.n;
+    o.pe_order = S.as<uint32_t>(S_ST_PE, n_pe + 1);
+    o.id_pos = S.as<uint64_t>(S_ST_IDPOS, N + 1);
+    o.mkey = S.as<uint32_t>(S_ST_MKEY, (uint64_t)n + 1);
+    o.midx = S.as<uint32_t>(S_ST_MIDX, (uint64_t)n + 1);
+    if (preserve) fill<uint64_t>(c, o.id_pos, N, 0ull);
+    if (n) hipLaunchKernelGGL(k_rec_streams, dim3(g), dim3(256), 0, st, src, (const uint32_t*)v0, n, preserve, paired, o);
+    // mismatch-symbol index: exclusive scan of counts (total at [n])
+    uint32_t* bidx = S.as<uint32_t>(S_ST_BIDX, (uint64_t)n + 1);
+    CURC_HIP(hipMemcpyAsync(bidx, o.mkey, (uint64_t)n * 4, hipMemcpyDeviceToDevice, st));
+    fill<uint32_t>(c, bidx + n, 1, 0u);
+    scan_excl(bidx, bidx, (uint64_t)n + 1, S, S_PRIM, st);
+    // offset-stream index: group records by mismatch count (stable)
+    uint32_t* mk2 = S.as<uint32_t>(S_ST_MKEY2, (uint64_t)n + 1);
+    uint32_t* mi2 = S.as<uint32_t>(S_ST_MIDX2, (uint64_t)n + 1);
+    uint32_t* mk = o.mkey;
+    uint32_t* mi = o.midx;
+    radix_sort_pairs<uint32_t>(mk, mi, mk2, mi2, n, 0, 8, S, S_PRIM, st);
+    uint32_t* gstart = S.as<uint32_t>(S_ST_GSTART, (uint64_t)max_mm + 3);
+    hipLaunchKernelGGL(k_gap_fill, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, st, (const uint32_t*)mk, (uint64_t)n,
+                       (uint64_t)max_mm + 1, gstart);
+    std::vector<uint32_t> hgs(max_mm + 2);
+    uint32_t total_mis = 0;
+    CURC_HIP(hipMemcpyAsync(hgs.data(), gstart, (max_mm + 2) * 4, hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipMemcpyAsync(&total_mis, bidx + n, 4, hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> mbase(max_mm + 2, 0);
+    B->s_mis_off_start.assign(max_mm + 2, 0);
+    uint64_t acc = 0;
+    for (uint32_t m = 1; m <= max_mm; ++m) {
+        mbase[m] = acc;
+        B->s_mis_off_start[m] = acc;
+        acc += (uint64_t)mis_width(m) * (hgs[m + 1] - hgs[m]);
+    }
+    B->s_mis_off_start[max_mm + 1] = acc;
+    uint64_t* dmbase = S.as<uint64_t>(S_ST_MBASE, (uint64_t)max_mm + 2);
+    h2d(c, dmbase, mbase.data(), max_mm + 2);
+    uint32_t* oidx = S.as<uint32_t>(S_ST_OIDX, (uint64_t)n + 1);
+    if (n)
+        hipLaunchKernelGGL(k_rec_offidx, dim3(g), dim3(256), 0, st, (const uint32_t*)mk, (const uint32_t*)mi, n,
+                           (const uint32_t*)gstart, (const uint64_t*)dmbase, oidx);
+    // compress.cu:1815-1886: mismatch symbols and offsets
+    uint8_t* pair = S.as<uint8_t>(S_ST_PAIR, (uint64_t)total_mis + 1);
+    uint16_t* offs = S.as<uint16_t>(S_ST_OFFS, acc + 1);
+    uint32_t* hist = S.as<uint32_t>(S_ST_HIST, 20);
+    fill<uint32_t>(c, hist, 20, 0u);
+    MisArgs ma{S.as<uint64_t>(S_PACKED, N * W + 1), W, L, S.as<uint8_t>(S_NASCII, (uint64_t)nN * L + 16),
+               S.as<uint64_t>(S_PG_HQ, (B->hq_len + 31) / 32 + 1), o.rid, o.rword, o.mis_count, bidx, oidx,
+               pair, offs, hist};
+    if (n) hipLaunchKernelGGL(k_rec_mismatch, dim3(grid_for(n, 4096)), dim3(256), 0, st, ma, n);
+    // compress.cu:1894-1929: per-context ranks by frequency, ties in
+    // destination order (std::sort of four elements is an insertion sort)
+    uint32_t hh[20];
+    CURC_HIP(hipMemcpyAsync(hh, hist, sizeof(hh), hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipStreamSynchronize(st));
+    static const char CTX[4] = {'A', 'T', 'C', 'G'};
+    static const char DEST[4][4] = {{'C', 'G', 'N', 'T'}, {'A', 'C', 'G', 'N'}, {'A', 'G', 'N', 'T'}, {'A', 'C', 'N', 'T'}};
+    auto sym = [](char ch) { return ch == 'A' ? 0 : ch == 'C' ? 1 : ch == 'G' ? 2 : ch == 'T' ? 4 : 3; };
+    uint8_t rank[20] = {0};
+    B->s_mis_table.assign(16, 0);
+    for (int cx = 0; cx < 4; ++cx) {
+        int idx[4] = {0, 1, 2, 3};
+        std::stable_sort(idx, idx + 4, [&](int a, int b) { return hh[cx * 5 + sym(DEST[cx][a])] > hh[cx * 5 + sym(DEST[cx][b])]; });
+        for (int r = 0; r < 4; ++r) rank[cx * 5 + sym(DEST[cx][idx[r]])] = (uint8_t)r;
+        for (int d = 0; d < 4; ++d) B->s_mis_table[cx * 4 + rank[cx * 5 + sym(DEST[cx][d])]] = (uint8_t)DEST[cx][d];
+    }
+    (void)CTX;
+    uint8_t* drank = S.as<uint8_t>(S_ST_RANK, 32);
+    h2d(c, drank, rank, 20);
+    uint8_t* symb = S.as<uint8_t>(S_ST_SYM, (uint64_t)total_mis + 1);
+    if (total_mis)
+        hipLaunchKernelGGL(k_rec_recode, dim3(grid_for(total_mis)), dim3(256), 0, st, (const uint8_t*)pair,
+                           (uint64_t)total_mis, (const uint8_t*)drank, symb);
+    // compress.cu:1968-2001: LQ and N-LQ records
+    uint16_t* lqoff = S.as<uint16_t>(S_ST_LQOFF, (uint64_t)lq_asm.n + 1);
+    uint16_t* nlqoff = S.as<uint16_t>(S_ST_NLQOFF, (uint64_t)nlq_asm.n + 1);
+    if (lq_asm.n)
+        hipLaunchKernelGGL(k_lq_streams, dim3(grid_for(lq_asm.n)), dim3(256), 0, st, (const uint32_t*)lq_asm.rid,
+                           (const uint64_t*)lq_asm.pos, lq_asm.n,
+                           (const uint32_t*)S.as<uint32_t>(S_UNMAPPED, (uint64_t)lq_asm.n + 1), L, preserve, paired,
+                           B->hq_len, lqoff, o.pe_order + n, o.id_pos);
+    if (nlq_asm.n)
+        hipLaunchKernelGGL(k_lq_streams, dim3(grid_for(nlq_asm.n)), dim3(256), 0, st, (const uint32_t*)nlq_asm.rid,
+                           (const uint64_t*)nlq_asm.pos, nlq_asm.n, (const uint32_t*)src.Nids, L, preserve, paired,
+                           B->hq_len + B->lq_len, nlqoff, o.pe_order + n + lq_asm.n, o.id_pos);
+    // to host
+    std::vector<uint32_t> rid;
+    std::vector<uint64_t> rword;
+    d2h(c, rid, o.rid, n);
+    d2h(c, rword, o.rword, n);
+    d2h(c, B->s_strand, (const char*)o.strand, n);
+    d2h(c, B->s_mis_count, o.mis_count, n);
+    d2h(c, B->s_mis_base, symb, total_mis);
+    d2h(c, B->s_mis_off, offs, acc);
+    if (!preserve) {
+        d2h(c, B->s_read_off, o.read_off, n);
+        d2h(c, B->s_lq_off, lqoff, lq_asm.n);
+        d2h(c, B->s_nlq_off, nlqoff, nlq_asm.n);
+        if (paired) d2h(c, B->s_pe_order, o.pe_order, n_pe);
+    } else {
+        d2h(c, B->s_id_pos, o.id_pos, N);
+    }
+    CURC_HIP(hipStreamSynchronize(st));
+    B->s_sorted.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        B->s_sorted[i].read_id = rid[i];
+        B->s_sorted[i].pad = 0;
+        B->s_sorted[i].word = rword[i];
     }
 }
 
@@ -398,6 +558,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     B->W = W;
     B->n_reads = N;
     uint32_t* cnt = S.as<uint32_t>(S_CNT, C_COUNT);
+    AsmOut hq_asm, lq_asm, nlq_asm;
     fill<uint32_t>(c, cnt, C_COUNT, 0u);
 
     // ---- a1/a2: N split + pack (preprocess.cpp:147-199, compress.cu:2151-2180)
@@ -617,7 +778,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     uint64_t* hq_pg;
     {
         Region r(c, "assembly");
-        AsmOut a = assemble(c, hq, n_hq, N, next, prev, off, L, cnt);
+        hq_asm = assemble(c, hq, n_hq, N, next, prev, off, L, cnt, S_PGIDS);
+        const AsmOut& a = hq_asm;
         B->hq_len = a.len;
         hq_pg = build_pg(c, packed, W, a, S_PG_HQ);
         records_to_host(c, a, nullptr, B->hq_rec);
@@ -678,14 +840,28 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                                                (const uint32_t*)krf, (const uint32_t*)kr, pos);
                         c->add_bytes("map_index", (double)count * (16 + 8) + 8.0 * H, (double)count);
                     }
+                    unsigned long long* work = S.as<unsigned long long>(S_MWORK, 4);
+                    fill<unsigned long long>(c, work, 4, 0ull);
                     {
                         Region r(c, "mapping");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
-                                  Nascii, nN, last, word};
+                                  Nascii, nN, last, word, work};
                         if (rc)
                             hipLaunchKernelGGL(k_map<true>, dim3(grid_for(nm, 1u << 20)), dim3(256), 0, st, a);
                         else
                             hipLaunchKernelGGL(k_map<false>, dim3(grid_for(nm, 1u << 20)), dim3(256), 0, st, a);
+                    }
+                    if (c->timing) {
+                        // algorithmic bytes of one launch: each active read's words, its
+                        // state write (1 + 8 B), 16 B per k-mer probe (two CSR offsets and
+                        // the hash of a register-resident k-mer) and, per verified
+                        // candidate, its 4 B position plus the P bytes of pseudogenome
+                        unsigned long long wk[4];
+                        CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
+                        CURC_HIP(hipStreamSynchronize(st));
+                        const double P8 = 8.0 * W;
+                        c->add_bytes("mapping", (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
+                                     (double)wk[2]);
                     }
                 }
             }
@@ -750,7 +926,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             d2h(c, B->lq_prev, lprev, n_lq);
             d2h(c, B->lq_off, loff, n_lq);
         }
-        AsmOut a = assemble(c, ids, n_lq, n_lq, lnext, lprev, loff, L, cnt);
+        lq_asm = assemble(c, ids, n_lq, n_lq, lnext, lprev, loff, L, cnt, S_LQIDS);
+        const AsmOut& a = lq_asm;
         B->lq_len = a.len;
         uint64_t* lpg = build_pg(c, lqp, W, a, S_PG_LQ);
         records_to_host(c, a, B->unmapped.data(), B->lq_rec);
@@ -801,7 +978,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         compact(cnt + C_TMP, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE, S, S_PRIM, st);
         compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st);
         full_match<Enc3>(c, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff);
-        AsmOut a = assemble(c, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt);
+        nlq_asm = assemble(c, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt, S_NLQIDS);
+        const AsmOut& a = nlq_asm;
         uint8_t* npg = S.as<uint8_t>(S_NPG, a.len + 16);
         if (a.n)
             hipLaunchKernelGGL(k_pg_write_ascii, dim3(grid_for(a.n)), dim3(256), 0, st, (const uint8_t*)Nascii, L,
@@ -812,6 +990,13 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         records_to_host(c, a, nullptr, B->nlq_rec);
         for (auto& rr : B->nlq_rec) rr.word |= 1;  // is_contain_N
         CURC_HIP(hipStreamSynchronize(st));
+    }
+
+    // ---- a13: record sort + stream builders (compress.cu:1730-2044)
+    if (flags & CURC_RUN_STREAMS) {
+        Region r(c, "streams");
+        build_streams(c, B, hq_asm, lq_asm, nlq_asm, cnt, (flags & CURC_RUN_PRESERVE_ORDER) != 0,
+                      (flags & CURC_RUN_PAIRED) != 0);
     }
 }
 
@@ -1040,7 +1225,25 @@ uint64_t curc_block_get(curc_block* B, int field, const void** ptr) {
         case CURC_F_N_ASCII: return put(B->Nascii);
         case CURC_F_NLQ_PG: return put(B->nlq_text);
         case CURC_F_NLQ_REC: return put(B->nlq_rec);
-        default: *ptr = nullptr; return 0;
+        case CURC_F_S_STRAND: return put(B->s_strand);
+        case CURC_F_S_READ_OFF: return put(B->s_read_off);
+        case CURC_F_S_MIS_COUNT: return put(B->s_mis_count);
+        case CURC_F_S_MIS_BASE: return put(B->s_mis_base);
+        case CURC_F_S_MIS_TABLE: return put(B->s_mis_table);
+        case CURC_F_S_LQ_OFF: return put(B->s_lq_off);
+        case CURC_F_S_NLQ_OFF: return put(B->s_nlq_off);
+        case CURC_F_S_PE_ORDER: return put(B->s_pe_order);
+        case CURC_F_S_ID_POS: return put(B->s_id_pos);
+        case CURC_F_S_SORTED_REC: return put(B->s_sorted);
+        default:
+            if (field > CURC_F_S_MIS_OFF && (size_t)(field - CURC_F_S_MIS_OFF) + 1 < B->s_mis_off_start.size()) {
+                const size_t m = (size_t)(field - CURC_F_S_MIS_OFF);
+                const uint64_t a = B->s_mis_off_start[m], b = B->s_mis_off_start[m + 1];
+                *ptr = B->s_mis_off.data() + a;
+                return b - a;
+            }
+            *ptr = nullptr;
+            return 0;
     }
 }
 

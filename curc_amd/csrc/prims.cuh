@@ -24,7 +24,7 @@ namespace curc {
 // Growable device scratch owned by a context: get(slot, bytes) returns a
 // buffer of at least `bytes`, reallocating only when it must grow.
 struct Scratch {
-    static constexpr int SLOTS = 128;
+    static constexpr int SLOTS = 200;
     void* ptr[SLOTS] = {};
     size_t cap[SLOTS] = {};
     size_t total = 0;

@@ -125,12 +125,27 @@ enum {
     CURC_F_LQ_NEXT = 27,       /* u32, local ids of the LQ graph */
     CURC_F_LQ_PREV = 28,
     CURC_F_LQ_OFFSET = 29,     /* u16 */
-    CURC_F_N_ASCII = 30        /* char[n_N * L] N-reads in N-index order */
+    CURC_F_N_ASCII = 30,       /* char[n_N * L] N-reads in N-index order */
+    /* pre-entropy streams (CURC_RUN_STREAMS; compress.cu:1730-2044) */
+    CURC_F_S_STRAND = 40,      /* char '0'/'1' per sorted HQ+map record */
+    CURC_F_S_READ_OFF = 41,    /* u16 position delta per sorted record (order not preserved) */
+    CURC_F_S_MIS_COUNT = 42,   /* u8 mismatch count per sorted record */
+    CURC_F_S_MIS_BASE = 43,    /* u8 recoded mismatch symbol per mismatch */
+    CURC_F_S_MIS_TABLE = 44,   /* 16 bytes: decoder rows for contexts A, T, C, G */
+    CURC_F_S_LQ_OFF = 45,      /* u16 L - position delta per LQ record */
+    CURC_F_S_NLQ_OFF = 46,     /* u16 L - position delta per N-LQ record */
+    CURC_F_S_PE_ORDER = 47,    /* u32 global ids in emission order (paired, order not preserved) */
+    CURC_F_S_ID_POS = 48,      /* u64 joined-pseudogenome position per read (order preserved) */
+    CURC_F_S_SORTED_REC = 49,  /* curc_alignment, HQ+map records after the record sort */
+    CURC_F_S_MIS_OFF = 64      /* + m (1..max_mm): u16 mismatch-offset stream for m mismatches */
 };
 
 /* Flags for curc_block_run. */
 #define CURC_RUN_INPUT_ON_DEVICE 1u /* `ascii` is a device pointer */
 #define CURC_RUN_KEEP_STAGES 2u     /* also return the per-stage link snapshots */
+#define CURC_RUN_STREAMS 4u         /* also build the block's pre-entropy streams */
+#define CURC_RUN_PRESERVE_ORDER 8u  /* streams for --preserve_order */
+#define CURC_RUN_PAIRED 16u         /* streams for paired-end input (ids 2k / 2k+1) */
 
 typedef struct curc_block curc_block;
 
