@@ -59,6 +59,29 @@ def dominant_roofline(stats, steps):
                 avg_launch_ms=round(per_launch_ms, 4), alg_bytes_per_launch=per_launch_bytes)
 
 
+def bits_per_base():
+    """Archive size per base on the 1 Mbp / 133,333-read workload: GPU
+    streams -> xz-coded .curc archive (written to a temporary file)."""
+    import tempfile
+
+    from curc_amd import archive
+
+    cfg = ca.config_small()
+    _, reads = ca.synth_reads(cfg)
+    with ca.Context(0) as ctx:
+        blk = ctx.run_block(reads, streams=True)
+        bs = archive.BlockStreams.from_block(blk, len(reads), reads.shape[1], False, False)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "c1.curc")
+        archive.write_archive(p, [bs], False, False, reads.shape[1])
+        size = os.path.getsize(p)
+        preserve, paired, L, blocks = archive.read_archive(p)
+        dec = archive.decode_block(blocks[0], L, preserve, paired)
+    lossless = sorted(map(bytes, dec)) == sorted(map(bytes, reads))
+    return dict(value=round(8.0 * size / reads.size, 5), archive_bytes=size, lossless=bool(lossless),
+                workload="1 Mbp uniform genome, 133,333 x 150 bp SE, 0.5% subst, 0.01% N, order not preserved")
+
+
 def cpu_baseline(n_reads, L):
     """Times the OpenMP build of the oracle on a scaled sample of the same
     workload (same coverage, error and N profile) and cross-checks the GPU
@@ -132,6 +155,8 @@ def main():
     roof = dominant_roofline(stats, a.steps)
     ctx.dev_free(d_ascii)
     ctx.close()
+    bpb = bits_per_base()
+    log(f"bits/base on the 1 Mbp workload: {bpb['value']} (lossless round trip: {bpb['lossless']})")
     cpu = None
     if not a.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample ...")
@@ -156,6 +181,7 @@ def main():
         "stages_ms": {s["name"]: round(s["ms"] / a.steps, 3) for s in stats},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "bits_per_base": bpb,
     }
     print(json.dumps(out))
 

@@ -278,6 +278,21 @@ struct ClaimState {
     uint32_t* res_n;
     uint32_t* pool;         // member lists of contended groups, carved per leader
     uint32_t* pool_n;
+    uint8_t* slinked;       // per suffix index: linked at this offset
+    uint8_t* plinked;       // per prefix position: linked at this offset (a8 only)
+    __device__ __forceinline__ void link(uint32_t si, uint32_t px, uint32_t off) const {
+        const uint32_t s = sl[si], p = pl[px];
+        next[s] = p;
+        prev[p] = s;
+        offset[s] = (uint16_t)off;
+        slinked[si] = 1;
+        if (plinked) plinked[px] = 1;
+    }
+};
+
+struct PredFlagClear {
+    const uint8_t* flags;
+    __device__ bool operator()(uint32_t i) const { return flags[i] == 0; }
 };
 
 // a6 probe (compress.cu:1474-1528): hash the k-mer at kip+off, binary
@@ -289,6 +304,7 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(32u, len);
     GRID_STRIDE(i, ns) {
+        cs.slinked[i] = 0;
         uint32_t s = cs.sl[i];
         const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
         uint32_t h = (uint32_t)(murmur64(xw(rs, kip + off, k)) & mask);
@@ -331,6 +347,7 @@ __global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t*
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(E::BPW, len);
     GRID_STRIDE(i, ns) {
+        cs.slinked[i] = 0;
         uint32_t s = cs.sl[i];
         const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
         uint32_t b = (uint32_t)E::word(rs, off, bp);
@@ -377,10 +394,7 @@ __global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t 
             uint32_t q = atomicAdd(cs.res_n, 1u);
             cs.res_list[q] = (uint32_t)i;
         } else {
-            uint32_t s = cs.sl[i], p = cs.pl[ch];
-            cs.next[s] = p;
-            cs.prev[p] = s;
-            cs.offset[s] = (uint16_t)off;
+            cs.link((uint32_t)i, ch, off);
         }
     }
 }
@@ -464,16 +478,11 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
             const uint32_t pa = cs.pl[a];
             if (pa != s) {
                 if (cmp_ps<E>(cs.R, cs.W, pa, cs.pk[a], rs, sx0, off, len) != 0) break;  // run exhausted
-                cs.next[s] = pa;
-                cs.prev[pa] = s;
-                cs.offset[s] = (uint16_t)off;
+                cs.link(arr[k], a, off);
                 a = b;
                 b = next_free(b + 1);
             } else if (b < np && cmp_ps<E>(cs.R, cs.W, cs.pl[b], cs.pk[b], rs, sx0, off, len) == 0) {
-                const uint32_t pb = cs.pl[b];
-                cs.next[s] = pb;
-                cs.prev[pb] = s;
-                cs.offset[s] = (uint16_t)off;
+                cs.link(arr[k], b, off);
                 b = next_free(b + 1);
             }
         }
@@ -488,13 +497,14 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
 template <class E>
 __global__ void __launch_bounds__(256) k_fm_block_hist(const uint64_t* __restrict__ pk,
                                                        const uint32_t* __restrict__ d_np, uint32_t bp,
-                                                       uint32_t* __restrict__ hist) {
+                                                       uint32_t* __restrict__ hist, uint8_t* __restrict__ plinked) {
     const uint32_t np = *d_np;
     const uint32_t lane = __lane_id();
     const uint64_t lt = lanemask_lt();
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < np; base += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < np;
+        if (valid) plinked[i] = 0;
         const uint64_t key = valid ? E::head(pk[i], bp) : ~0ull;
         uint64_t pkey = __shfl_up(key, 1, 64);
         const bool start = valid && (lane == 0 || pkey != key);
@@ -718,6 +728,21 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
     if (__lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
 }
 
+// Positions verified earlier in the same strand pass.  Skipping a repeat is
+// exact: its mismatch count is the same, and since `last` only decreases a
+// candidate rejected (or accepted) once cannot be strictly better later.
+struct SeenCache {
+    uint64_t s0 = ~0ull, s1 = ~0ull, s2 = ~0ull, s3 = ~0ull;
+    __device__ __forceinline__ bool check_insert(uint64_t mp) {
+        if (mp == s0 || mp == s1 || mp == s2 || mp == s3) return true;
+        s3 = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = mp;
+        return false;
+    }
+};
+
 // compress.cu:411-521: one thread per read; candidates in bucket order,
 // a strictly better mismatch count replaces the record, stop at <= 1.
 template <bool RC>
@@ -729,6 +754,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
         ++n_act;
         const bool isN = i >= a.n_un;
         uint64_t w = a.word[i];
+        SeenCache seen;
         if (!isN) {
             const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * a.W;
             for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
@@ -743,6 +769,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
                     if (rp < ls) continue;
                     uint64_t mp = rp - ls;
                     if (mp + a.L > a.pg_len) continue;
+                    if (seen.check_insert(mp)) continue;
                     ++n_cand;
                     uint32_t c = RC ? rc_mm(a.pg, mp, rd, a.L, last - 1) : fwd_mm(a.pg, mp, rd, a.L, last - 1);
                     if (c < last) {
@@ -766,6 +793,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
                     if (rp < ls) continue;
                     uint64_t mp = rp - ls;
                     if (mp + a.L > a.pg_len) continue;
+                    if (seen.check_insert(mp)) continue;
                     ++n_cand;
                     uint32_t c = 0;
                     for (uint32_t o = 0; o < a.L; ++o) {

@@ -35,7 +35,7 @@ enum Slot {
     S_LQIDS, S_LQPOS, S_LQNB, S_NLQIDS, S_NLQPOS, S_NLQNB,
     S_ST_RID, S_ST_RWORD, S_ST_STRAND, S_ST_MISC, S_ST_ROFF, S_ST_PE, S_ST_IDPOS, S_ST_MKEY, S_ST_MIDX,
     S_ST_MKEY2, S_ST_MIDX2, S_ST_BIDX, S_ST_OIDX, S_ST_GSTART, S_ST_MBASE, S_ST_PAIR, S_ST_OFFS, S_ST_HIST,
-    S_ST_RANK, S_ST_SYM, S_ST_LQOFF, S_ST_NLQOFF, S_MWORK,
+    S_ST_RANK, S_ST_SYM, S_ST_LQOFF, S_ST_NLQOFF, S_MWORK, S_SLINK, S_PLINK,
     S_LAST_NAMED,
     S_PRIM = 176  // primitives use slots 176.. (scan recursion, sort hist, compaction)
 };
@@ -366,6 +366,8 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     uint32_t* memb = S.as<uint32_t>(S_MEMB, n_max);
     uint32_t* resl = S.as<uint32_t>(S_RESL, n_max);
     uint32_t* pool = S.as<uint32_t>(S_POOL, n_max);
+    uint8_t* slinked = S.as<uint8_t>(S_SLINK, n_max);
+    uint8_t* plinked = S.as<uint8_t>(S_PLINK, n_max);
     uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)n_max + 1);
     uint32_t* cont = S.as<uint32_t>(S_CONT, (uint64_t)n_max + 1);
     uint32_t* head = S.as<uint32_t>(S_HEAD, (uint64_t)n_max + 1);
@@ -388,16 +390,17 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
         const uint64_t nb = 1ull << (BITS * bp);
         fill<uint32_t>(c, rng, nb + 1, 0u);
         hipLaunchKernelGGL(k_fm_block_hist<E>, dim3(grid_for(n_max, 4096)), dim3(256), 0, c->st,
-                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng);
+                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng, plinked);
         scan_excl(rng, rng, nb + 1, S, S_PRIM, c->st);
         ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
-                      choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL};
+                      choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
+                      slinked, plinked};
         hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs, (const uint32_t*)rng, bp, off);
         hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off);
         hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
-        compact(ns[cur], PredSufFree{sl[cur], next}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
-                S_PRIM, c->st);
-        compact(np[cur], PredPreFree{pl[cur], prev}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
+        compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
+                c->st);
+        compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
                 np[1 - cur], S, S_PRIM, c->st);
         cur = 1 - cur;
     }
@@ -697,15 +700,17 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             CURC_HIP(hipMemcpyAsync(ns[0], cnt + C_NSUF, 4, hipMemcpyDeviceToDevice, st));
             const uint32_t g = grid_for(nsuf, 4096);
             int cur = 0;
+            uint8_t* slinked = S.as<uint8_t>(S_SLINK, (uint64_t)nsuf + 1);
             for (uint32_t o = 1; o < P.max_off; ++o) {
                 ClaimState cs{packed, W, L, sl[cur], ns[cur], va, vk, cnt + C_NPRE, next, prev, off,
-                              choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL};
+                              choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
+                              slinked, nullptr};
                 hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr, mask, P.k,
                                    P.kip, o);
                 hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
                 hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
-                compact(ns[cur], PredSufFree{sl[cur], next}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
-                        S_PRIM, st);
+                compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
+                        st);
                 cur = 1 - cur;
             }
         }
