@@ -32,6 +32,13 @@ XZ_PRESET = 6
 COMP = np.zeros(256, dtype=np.uint8)
 for _a, _b in zip(b"ACGTNacgtn", b"TGCANtgcan"):
     COMP[_a] = _b
+# complement table of the pseudogenome self-match (helper.cpp:253-268):
+# IUPAC symbols map to their complements (lower case to upper), the rest to 0
+_RM_COMP = bytearray(256)
+for _a, _b in zip(b"AaCcGgTtNnUuYyRrKkMmBbDdHhVv", b"TTGGCCAANNAARRYYMMKKVVHHDDBB"):
+    _RM_COMP[_a] = _b
+RM_COMP = bytes(_RM_COMP)
+MAP_NAMES = ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len", "unmap_n_ref_off", "unmap_n_ref_len")
 
 
 # ------------------------------------------------------------------ coders ---
@@ -76,31 +83,83 @@ def unpack_dna(blob: bytes) -> bytes:
 
 # ------------------------------------------------------------ block streams ---
 class BlockStreams:
-    """Pre-entropy streams of one block (the fields CURC writes)."""
+    """Pre-entropy streams of one block (the fields CURC writes).
+
+    hq_text / lq_text / nlq_text are the full pseudogenomes.  When the
+    self-match ran, rm_texts holds the three rewritten ('%'-marked) texts
+    that the archive stores and maps the six MapOff / MapLen streams."""
 
     FIELDS = ("reads_count", "mis_table", "hq_text", "lq_text", "nlq_text", "strand", "read_off", "lq_off",
-              "nlq_off", "pe_order", "id_pos", "mis_count", "mis_base", "mis_off", "max_mm", "maps")
+              "nlq_off", "pe_order", "id_pos", "mis_count", "mis_base", "mis_off", "max_mm", "maps", "rm_texts")
 
     def __init__(self, **kw):
         for f in self.FIELDS:
             setattr(self, f, kw.get(f))
         if self.maps is None:
-            self.maps = {k: b"" for k in ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len",
-                                          "unmap_n_ref_off", "unmap_n_ref_len")}
+            self.maps = {k: b"" for k in MAP_NAMES}
 
     @classmethod
     def from_block(cls, blk, n_reads, L, preserve, paired):
         """From a curc_amd.Block run with streams=True (or an oracle block
-        after .streams(); both expose the same field names)."""
+        after .streams(); both expose the same field names), with the
+        self-match outputs when the block carries them."""
         max_mm = L // 5
         hq = blk.get("hq_pg").tobytes()
-        return cls(
+        bs = cls(
             reads_count=n_reads, mis_table=blk.get("s_mis_table").tobytes(), hq_text=hq,
             lq_text=blk.get("lq_pg").tobytes(), nlq_text=blk.get("nlq_pg").tobytes(),
             strand=blk.get("s_strand").tobytes(), read_off=blk.get("s_read_off"), lq_off=blk.get("s_lq_off"),
             nlq_off=blk.get("s_nlq_off"), pe_order=blk.get("s_pe_order"), id_pos=blk.get("s_id_pos"),
             mis_count=blk.get("s_mis_count"), mis_base=blk.get("s_mis_base"),
             mis_off={m: blk.mis_off(m) for m in range(1, max_mm + 1)}, max_mm=max_mm)
+        if getattr(blk, "has_refmatch", False):
+            bs.rm_texts = tuple(blk.get(f"rm_{p}_text").tobytes() for p in ("hq", "lq", "nlq"))
+            bs.maps = {}
+            for p, name in (("hq", "ref"), ("lq", "unmap_ref"), ("nlq", "unmap_n_ref")):
+                bs.maps[name + "_off"] = blk.get(f"rm_{p}_off").tobytes()
+                bs.maps[name + "_len"] = blk.get(f"rm_{p}_len").tobytes()
+        return bs
+
+
+def _read_frugal(buf, i):
+    """helper.h:191-200; returns (value, next index)."""
+    v = 0
+    base = 1
+    while True:
+        b = buf[i] if i < len(buf) else 0
+        i += 1
+        v += base * (b % 128)
+        base *= 128
+        if b < 128:
+            return v, i
+
+
+def recover_ref(dest: bytes, src, map_off: bytes, map_len: bytes, ref_len_std: bool, src_is_dest: bool) -> bytes:
+    """recoverRef (decompress.cpp:339-366): every '%' of dest expands to the
+    reverse complement of src[off : off + len].  With src_is_dest the source
+    is the text being rebuilt, whose earlier part each reference points to."""
+    out = bytearray()
+    srcb = out if src_is_dest else src
+    if not map_len:
+        return bytes(dest)
+    target, li = _read_frugal(map_len, 0)
+    osz = 4 if ref_len_std else 8
+    ofmt = "<I" if ref_len_std else "<Q"
+    oi = 0
+    pos = 0
+    while True:
+        mark = dest.find(b"%", pos)
+        if mark < 0:
+            break
+        out += dest[pos:mark]
+        pos = mark + 1
+        (s,) = struct.unpack_from(ofmt, map_off, oi)
+        oi += osz
+        n, li = _read_frugal(map_len, li)
+        n += target
+        out += bytes(srcb[s:s + n]).translate(RM_COMP)[::-1]
+    out += dest[pos:]
+    return bytes(out)
 
 
 def _pe_order_encode(pe_order):
@@ -147,7 +206,8 @@ def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired
     if preserve:
         out.write(struct.pack("<B", 1 if joined <= 0xFFFFFFFF else 0))
     out.write(struct.pack("<B", 1 if len(bs.hq_text) <= 0xFFFFFFFF else 0))
-    out.write(struct.pack("<QQQ", len(bs.hq_text), len(bs.lq_text), len(bs.nlq_text)))
+    texts = bs.rm_texts if bs.rm_texts is not None else (bs.hq_text, bs.lq_text, bs.nlq_text)
+    out.write(struct.pack("<QQQ", *(len(t) for t in texts)))
 
     def sub(payload: bytes):
         out.write(struct.pack("<Q", len(payload)))
@@ -157,7 +217,7 @@ def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired
         dt = "<u4" if joined <= 0xFFFFFFFF else "<u8"
         sub(xz(np.asarray(bs.id_pos).astype(dt).tobytes()))
     sub(xz(bs.strand))
-    sub(pack_dna(bs.hq_text + bs.lq_text + bs.nlq_text))
+    sub(pack_dna(b"".join(texts)))
     if not preserve:
         sub(xz(np.asarray(bs.read_off, dtype="<u2").tobytes()))
         sub(xz(np.asarray(bs.nlq_off, dtype="<u2").tobytes()))
@@ -165,7 +225,7 @@ def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired
         if paired:
             d, s = _pe_order_encode(bs.pe_order)
             sub(xz(struct.pack("<Q", len(d)) + d + s))
-    for k in ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len", "unmap_n_ref_off", "unmap_n_ref_len"):
+    for k in MAP_NAMES:
         sub(xz(bs.maps[k]))
     sub(xz(np.asarray(bs.mis_count, dtype=np.uint8).tobytes()))
     sub(xz(np.asarray(bs.mis_base, dtype=np.uint8).tobytes()))
@@ -182,7 +242,7 @@ def read_block(inp: io.BufferedIOBase, preserve: bool, paired: bool, L: int) -> 
     mis_table = inp.read(16)
     if preserve:
         (join_std,) = rd("<B")
-    rd("<B")
+    (ref_len_std,) = rd("<B")
     hq_len, lq_len, n_len = rd("<QQQ")
 
     def sub():
@@ -206,14 +266,21 @@ def read_block(inp: io.BufferedIOBase, preserve: bool, paired: bool, L: int) -> 
             (nd,) = struct.unpack_from("<Q", raw, 0)
             pe_pairs = (raw[8:8 + nd], raw[8 + nd:])
     maps = {}
-    for k in ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len", "unmap_n_ref_off", "unmap_n_ref_len"):
+    for k in MAP_NAMES:
         maps[k] = unxz(sub())
     mis_count = np.frombuffer(unxz(sub()), dtype=np.uint8)
     mis_base = np.frombuffer(unxz(sub()), dtype=np.uint8)
     (max_mm,) = rd("<B")
     mis_off = {m: np.frombuffer(unxz(sub()), dtype="<u2") for m in range(1, max_mm + 1)}
-    bs = BlockStreams(reads_count=reads_count, mis_table=mis_table, hq_text=text[:hq_len],
-                      lq_text=text[hq_len:hq_len + lq_len], nlq_text=text[hq_len + lq_len:],
+    # decompress.cpp:602-608: the HQ text first (its references point into
+    # itself), then the LQ and N-read LQ texts against it
+    rm = (text[:hq_len], text[hq_len:hq_len + lq_len], text[hq_len + lq_len:])
+    std = bool(ref_len_std)
+    hq = recover_ref(rm[0], None, maps["ref_off"], maps["ref_len"], std, True)
+    lq = recover_ref(rm[1], hq, maps["unmap_ref_off"], maps["unmap_ref_len"], std, False)
+    nlq = recover_ref(rm[2], hq, maps["unmap_n_ref_off"], maps["unmap_n_ref_len"], std, False)
+    bs = BlockStreams(reads_count=reads_count, mis_table=mis_table, hq_text=hq, lq_text=lq, nlq_text=nlq,
+                      rm_texts=rm,
                       strand=strand, read_off=read_off, lq_off=lq_off, nlq_off=nlq_off, pe_order=pe_order,
                       id_pos=id_pos, mis_count=mis_count, mis_base=mis_base, mis_off=mis_off, max_mm=max_mm,
                       maps=maps)

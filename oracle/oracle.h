@@ -133,6 +133,26 @@ uint64_t orc_block_get(orc_block *b, int field, const void **ptr);
 int orc_block_streams(orc_block *b, int preserve_order, int paired_end);
 void orc_block_free(orc_block *b);
 
+/* RefMatcher (compress.cu:597-1001, refmatch.cpp): matches the reverse
+ * complement of dest against a k=36 index of src (the HQ pseudogenome),
+ * rewrites dest with '%' markers and builds the MapOff / MapLen streams.
+ * dest_contain_N selects the character path (compress.cu:809-853) over the
+ * binary one (:746-807); dest_is_ref marks the self match of src. */
+typedef struct orc_refmatch orc_refmatch;
+enum {
+    ORC_RM_TEXT = 0,      /* char: rewritten destination text */
+    ORC_RM_MAP_OFF = 1,   /* bytes: u32 (u64 if src_len > 2^32-1) source positions */
+    ORC_RM_MAP_LEN = 2,   /* bytes: byte-frugal 50, then byte-frugal (length - 50) */
+    ORC_RM_FOUND = 3,     /* u64 triples (src, len, dest) in collection order */
+    ORC_RM_MATCHES = 4,   /* u64 triples after transform, sort and unique */
+    ORC_RM_CAPACITY = 5   /* u64 */
+};
+uint32_t orc_marush36(const char *s);
+orc_refmatch *orc_ref_match(const char *src, uint64_t src_len, const char *dest, uint64_t dest_len,
+                            int dest_contain_N, int dest_is_ref);
+uint64_t orc_refmatch_get(orc_refmatch *r, int field, const void **ptr);
+void orc_refmatch_free(orc_refmatch *r);
+
 #ifdef __cplusplus
 }
 #endif

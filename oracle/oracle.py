@@ -67,6 +67,10 @@ def lib(kind="st"):
         "orc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "orc_block_free": (None, [vp]),
         "orc_block_streams": (i32, [vp, i32, i32]),
+        "orc_marush36": (u32, [vp]),
+        "orc_ref_match": (vp, [vp, u64, vp, u64, i32, i32]),
+        "orc_refmatch_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
+        "orc_refmatch_free": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -93,6 +97,15 @@ class OracleBlock:
         if not h:
             raise ValueError("oracle rejected the input")
         self._h = ctypes.c_void_p(h)
+        self.kind = kind
+        self.has_refmatch = False
+
+    def refmatch(self):
+        """The block's three self-match runs (compress.cu:2053-2069); results
+        via get("rm_<hq|lq|nlq>_<text|off|len|matches|found>")."""
+        self._rm = ref_match_block(self.get("hq_pg").tobytes(), self.get("lq_pg").tobytes(),
+                                   self.get("nlq_pg").tobytes(), self.kind)
+        self.has_refmatch = True
 
     def streams(self, preserve_order=False, paired_end=False):
         """Builds the block's pre-entropy streams (compress.cu:1730-2044)."""
@@ -112,6 +125,13 @@ class OracleBlock:
         return np.frombuffer(buf, dtype=dt, count=n).copy()
 
     def get(self, name):
+        if name.startswith("rm_"):
+            _, part, what = name.split("_")
+            r = self._rm[part]
+            if what in ("matches", "found"):
+                return r.get(what, np.zeros((0, 3), dtype=np.uint64))
+            v = r[{"text": "text", "off": "map_off", "len": "map_len"}[what]]
+            return np.frombuffer(v, dtype=np.uint8).copy()
         fid, dt = FIELDS[name]
         ptr = ctypes.c_void_p()
         n = self.lib.orc_block_get(self._h, fid, ctypes.byref(ptr))
@@ -131,6 +151,45 @@ class OracleBlock:
             self.free()
         except Exception:
             pass
+
+
+def ref_match(src, dest, dest_contain_n=False, dest_is_ref=False, kind="st"):
+    """RefMatcher::match (compress.cu:855-994) of dest against src; returns
+    dict(text, map_off, map_len, found, matches, capacity)."""
+    src = bytes(src)
+    dest = bytes(dest)
+    L = lib(kind)
+    h = L.orc_ref_match(src, len(src), dest, len(dest), int(dest_contain_n), int(dest_is_ref))
+    try:
+        def get(fid, dt):
+            ptr = ctypes.c_void_p()
+            n = L.orc_refmatch_get(h, fid, ctypes.byref(ptr))
+            if dt is bytes:
+                return ctypes.string_at(ptr.value, n) if n else b""
+            if n == 0:
+                return np.zeros((0, 3), dtype=np.uint64)
+            buf = (ctypes.c_char * (n * 24)).from_address(ptr.value)
+            return np.frombuffer(buf, dtype=np.uint64, count=3 * n).reshape(n, 3).copy()
+
+        out = dict(text=get(0, bytes), map_off=get(1, bytes), map_len=get(2, bytes), found=get(3, None),
+                   matches=get(4, None))
+        ptr = ctypes.c_void_p()
+        L.orc_refmatch_get(h, 5, ctypes.byref(ptr))
+        out["capacity"] = ctypes.c_uint64.from_address(ptr.value).value
+        return out
+    finally:
+        L.orc_refmatch_free(h)
+
+
+def ref_match_block(hq, lq, nlq, kind="st"):
+    """The three matches of block_compress (compress.cu:2053-2069), in its
+    order: the LQ and N-read LQ pseudogenomes against the HQ one (skipped
+    when empty), then the HQ pseudogenome against itself."""
+    empty = dict(text=b"", map_off=b"", map_len=b"")
+    out = {"lq": ref_match(hq, lq, False, False, kind) if len(lq) else dict(empty),
+           "nlq": ref_match(hq, nlq, True, False, kind) if len(nlq) else dict(empty),
+           "hq": ref_match(hq, hq, False, True, kind)}
+    return out
 
 
 def pack(reads):

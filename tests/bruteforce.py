@@ -229,3 +229,128 @@ def map_reads(pg, reads_fwd, P, k=28, step=5, rstep=2, limit=12, mutant=False):
         else:
             out.append(None)
     return out
+
+
+# ---------------------------------------------------------------- RefMatcher ---
+# compress.cu:597-1001.  Closed forms instead of the reference's loops: the
+# seed's extensions are a longest common prefix / suffix, and the boundary
+# rule of the left loop (it gives up the base at a text start) is written as
+# an explicit case.
+RM_COMP = {ord(a): ord(b) for a, b in zip("AaCcGgTtNnUuYyRrKkMmBbDdHhVv", "TTGGCCAANNAARRYYMMKKVVHHDDBB")}
+
+
+def rm_comp(c):
+    return RM_COMP.get(c, 0)
+
+
+def rm_hash(s):
+    """maRushPrime1 over 36 bytes (compress.cu:617-629)."""
+    import struct
+    h = 36
+    for j, k in enumerate(struct.unpack("<9I", bytes(s[:36]))):
+        h ^= (k + j) & 0xFFFFFFFF
+        h = (h * 171717) & M64
+    return h & 0xFFFFFFFF
+
+
+def _lcp(a, i, b, j, limit):
+    n = 0
+    while n < limit and a[i + n] == b[j + n]:
+        n += 1
+    return n
+
+
+def ref_match(src, dest, contain_n, is_ref, mutant=None):
+    """Returns (text, map_off, map_len, found, matches).  mutant plants one
+    plausible mistake: 'left' (no boundary rule on the left extension),
+    'bucket' (ascending bucket order), 'segment' (one result per probe
+    position on a binary destination), 'threshold' (49-base matches)."""
+    src, dest = bytes(src), bytes(dest)
+    n, m = len(src), len(dest)
+    rc = bytes(rm_comp(c) for c in reversed(dest))
+    text = bytes(rm_comp(c) for c in reversed(rc))
+    if n < 36 or m < 36:
+        return text, b"", b"", [], []
+    cnt = (n - 35 + 4) // 5
+    H = hash_size(cnt)
+    buckets = {}
+    for i in range(cnt):
+        b = buckets.setdefault(rm_hash(src[5 * i:]) & (H - 1), [])
+        if len(b) < 12:
+            b.append(i)
+    minlen = 49 if mutant == "threshold" else 50
+
+    def probe(dp):
+        cand = buckets.get(rm_hash(rc[dp:]) & (H - 1), [])
+        for i in (cand if mutant == "bucket" else cand[::-1]):
+            sp = 5 * i
+            if not contain_n and is_ref and m - sp < dp:
+                continue
+            if rc[dp:dp + 36] != src[sp:sp + 36]:
+                continue
+            right = dp + 36 + _lcp(rc, dp + 36, src, sp + 36, min(m - dp - 36, n - sp - 36))
+            b = min(dp, sp)
+            left = 0
+            while left < b and rc[dp - 1 - left] == src[sp - 1 - left]:
+                left += 1
+            start = dp - left if (left < b or mutant == "left") else dp - b + 1
+            if right - start >= minlen:
+                return (sp - (dp - start), right - start, start)
+        return None
+
+    dic = (m - 35 + 2) // 3
+    found = []
+    if not contain_n and mutant != "segment":
+        for g in range((dic + 3) // 4):
+            for s in range(4):
+                dp = (4 * g + s) * 3
+                if dp + 36 <= m:
+                    r = probe(dp)
+                    if r:
+                        found.append(r)
+                        break
+    else:
+        for i in range(dic):
+            r = probe(3 * i)
+            if r:
+                found.append(r)
+    if not found:
+        return text, b"", b"", [], []
+    cap = int(dic * (0.2 if is_ref else 0.21))
+    res = []
+    for s, ln, d in found[:cap]:
+        d = m - (d + ln)
+        if is_ref:
+            s, d = min(s, d), max(s, d)
+            if s + ln > d:
+                mg = (s + ln - d + 1) // 2
+                ln -= mg
+                d += mg
+        res.append((d, s, ln))
+    res = sorted(set(res))
+    matches = [(s, ln, d) for d, s, ln in res]
+
+    def frugal(v):
+        o = bytearray()
+        while v >= 128:
+            o.append(128 + v % 128)
+            v //= 128
+        o.append(v)
+        return bytes(o)
+
+    out, moff, mlen = bytearray(), bytearray(), bytearray(frugal(50))
+    pos = 0
+    for s, ln, d in matches:
+        if d < pos:
+            if pos - d >= ln:
+                continue
+            ln -= pos - d
+            d = pos
+        if ln < 50:
+            continue
+        out += text[pos:d] + b"%"
+        moff += s.to_bytes(4 if n <= 0xFFFFFFFF else 8, "little")
+        mlen += frugal(ln - 50)
+        pos = d + ln
+    out += text[pos:]
+    return bytes(out), bytes(moff), bytes(mlen), found, matches
