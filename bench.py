@@ -40,7 +40,9 @@ def parse():
     p.add_argument("--reads", type=int, default=20_000_000)
     p.add_argument("--genome", type=int, default=100_000_000)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-reads", type=int, default=1_000_000)
+    p.add_argument("--cpu-reads", type=int, default=2_000_000)
+    p.add_argument("--no-full-block", action="store_true")
+    p.add_argument("--no-bits-per-base", action="store_true")
     return p.parse_args()
 
 
@@ -61,7 +63,8 @@ def dominant_roofline(stats, steps):
 
 def bits_per_base():
     """Archive size per base on the 1 Mbp / 133,333-read workload: GPU
-    streams -> xz-coded .curc archive (written to a temporary file)."""
+    streams and pseudogenome self-match -> xz-coded .curc archive (written
+    to a temporary file)."""
     import tempfile
 
     from curc_amd import archive
@@ -69,7 +72,7 @@ def bits_per_base():
     cfg = ca.config_small()
     _, reads = ca.synth_reads(cfg)
     with ca.Context(0) as ctx:
-        blk = ctx.run_block(reads, streams=True)
+        blk = ctx.run_block(reads, streams=True, refmatch=True)
         bs = archive.BlockStreams.from_block(blk, len(reads), reads.shape[1], False, False)
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "c1.curc")
@@ -153,10 +156,31 @@ def main():
     for s in sorted(stats, key=lambda s: -s["ms"]):
         log(f"  {s['name']:<14} {s['ms'] / a.steps:9.2f} ms  ({s['launches'] // max(1, a.steps)} regions/step)")
     roof = dominant_roofline(stats, a.steps)
+    full = None
+    if not a.no_full_block:
+        # the same block once more with the stream builders and the
+        # pseudogenome self-match (outside the metric's region)
+        ctx.reset_stats()
+        ctx.set_timing(True)
+        ctx.device_sync()
+        t = time.perf_counter()
+        b = ctx.run_block(None, L=L, device_ptr=d_ascii, n_reads=a.reads, streams=True, refmatch=True)
+        ctx.device_sync()
+        full_ms = (time.perf_counter() - t) * 1e3
+        ctx.set_timing(False)
+        fst = ctx.stats()
+        full = {"ms": round(full_ms, 3), "stages_ms": {s["name"]: round(s["ms"], 3) for s in fst},
+                "pg_bases": {p: int(len(b.get(p + "_pg"))) for p in ("hq", "lq", "nlq")},
+                "pg_bases_after_refmatch": {p: int(len(b.get(f"rm_{p}_text"))) for p in ("hq", "lq", "nlq")}}
+        log(f"block with streams + self-match: {full_ms:.1f} ms; " +
+            ", ".join(f"{k} {v:.1f}" for k, v in full["stages_ms"].items() if k in ("streams", "refmatch")))
+        b.free()
     ctx.dev_free(d_ascii)
     ctx.close()
-    bpb = bits_per_base()
-    log(f"bits/base on the 1 Mbp workload: {bpb['value']} (lossless round trip: {bpb['lossless']})")
+    bpb = None
+    if not a.no_bits_per_base:
+        bpb = bits_per_base()
+        log(f"bits/base on the 1 Mbp workload: {bpb['value']} (lossless round trip: {bpb['lossless']})")
     cpu = None
     if not a.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample ...")
@@ -182,6 +206,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "bits_per_base": bpb,
+        "full_block": full,
     }
     print(json.dumps(out))
 

@@ -21,7 +21,11 @@ RUN_KEEP_STAGES = 2
 RUN_STREAMS = 4
 RUN_PRESERVE_ORDER = 8
 RUN_PAIRED = 16
+RUN_REFMATCH = 32
 MIS_OFF_BASE = 64
+RM_BASE = 1000
+RM_PARTS = {"hq": 0, "lq": 1, "nlq": 2}
+RM_KINDS = {"text": 0, "off": 1, "len": 2, "matches": 3, "found": 4}
 
 # field id -> numpy dtype (ids shared with the oracle)
 FIELDS = {
@@ -175,14 +179,23 @@ def config_scale(seed=22, n_reads=20_000_000, genome_len=100_000_000):
 class Block:
     """Results of one block (host memory owned by the library)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, has_refmatch=False):
         self._h = ctypes.c_void_p(handle)
+        self.has_refmatch = has_refmatch
 
     def mis_off(self, m):
         """Mismatch-offset stream for reads with m mismatches."""
         return self._field(MIS_OFF_BASE + m, np.uint16)
 
     def get(self, name):
+        if name.startswith("rm_"):
+            # rm_<hq|lq|nlq>_<text|off|len|matches|found>: self-match outputs
+            _, part, kind = name.split("_")
+            k = RM_KINDS[kind]
+            fid = RM_BASE + 5 * RM_PARTS[part] + k
+            if k >= 3:
+                return self._field(fid, np.dtype((np.uint64, 3)))
+            return self._field(fid, np.uint8)
         fid, dt = FIELDS[name]
         return self._field(fid, dt)
 
@@ -193,7 +206,7 @@ class Block:
             dt = REC_DTYPE
         dt = np.dtype(dt)
         if n == 0 or not ptr.value:
-            return np.zeros(0, dtype=dt)
+            return np.zeros((0,) + dt.shape, dtype=dt.base)
         buf = (ctypes.c_char * (n * dt.itemsize)).from_address(ptr.value)
         return np.frombuffer(buf, dtype=dt, count=n).copy()
 
@@ -224,13 +237,16 @@ class Context:
             raise CurcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
     def run_block(self, reads, L=None, stop_after=STAGE_ALL, keep_stages=False, device_ptr=None, n_reads=None,
-                  streams=False, preserve_order=False, paired=False):
+                  streams=False, preserve_order=False, paired=False, refmatch=False):
         """Runs the GPU part of block_compress on reads ((n, L) uint8 host
         array) or on a device buffer (device_ptr, n_reads, L); with streams,
-        also the record sort and stream builders."""
+        also the record sort and stream builders; with refmatch, the
+        pseudogenome self-match."""
         flags = RUN_KEEP_STAGES if keep_stages else 0
         if streams:
             flags |= RUN_STREAMS | (RUN_PRESERVE_ORDER if preserve_order else 0) | (RUN_PAIRED if paired else 0)
+        if refmatch:
+            flags |= RUN_REFMATCH
         out = ctypes.c_void_p()
         if device_ptr is not None:
             flags |= RUN_INPUT_ON_DEVICE
@@ -241,7 +257,7 @@ class Context:
             n, L = reads.shape
             rc = lib().curc_block_run(self._h, reads.ctypes.data, n, L, stop_after, flags, ctypes.byref(out))
         self._check(rc, "curc_block_run")
-        return Block(out.value)
+        return Block(out.value, has_refmatch=refmatch)
 
     def dev_alloc(self, nbytes):
         p = ctypes.c_void_p()

@@ -28,7 +28,7 @@ def compress(inputs, output_name, block_ratio=1.0, preserve_order=False, working
     with Context(device) as ctx:
         for reads in fastq.read_blocks(inputs[0], inputs[1] if paired else None, block_ratio):
             L = reads.shape[1]
-            blk = ctx.run_block(reads, streams=True, preserve_order=preserve_order, paired=paired)
+            blk = ctx.run_block(reads, streams=True, preserve_order=preserve_order, paired=paired, refmatch=True)
             blocks.append(archive.BlockStreams.from_block(blk, len(reads), L, preserve_order, paired))
             blk.free()
             total += len(reads)

@@ -14,6 +14,7 @@
 #include "../../include/curc_gpu.h"
 #include "assembly.cuh"
 #include "kernels.cuh"
+#include "refmatch.cuh"
 #include "streams.cuh"
 
 using namespace curc;
@@ -36,6 +37,10 @@ enum Slot {
     S_ST_RID, S_ST_RWORD, S_ST_STRAND, S_ST_MISC, S_ST_ROFF, S_ST_PE, S_ST_IDPOS, S_ST_MKEY, S_ST_MIDX,
     S_ST_MKEY2, S_ST_MIDX2, S_ST_BIDX, S_ST_OIDX, S_ST_GSTART, S_ST_MBASE, S_ST_PAIR, S_ST_OFFS, S_ST_HIST,
     S_ST_RANK, S_ST_SYM, S_ST_LQOFF, S_ST_NLQOFF, S_MWORK, S_SLINK, S_PLINK,
+    S_RM_K0, S_RM_K1, S_RM_V0, S_RM_V1, S_RM_KRF, S_RM_KR, S_RM_POS, S_RM_RC, S_RM_S, S_RM_L, S_RM_D,
+    S_RM_FLAG, S_RM_FS, S_RM_FL, S_RM_FD, S_RM_SK0, S_RM_SK1, S_RM_SV0, S_RM_SV1, S_RM_US, S_RM_UL, S_RM_UD,
+    S_RM_T, S_RM_TMAX, S_RM_J0, S_RM_J1, S_RM_MARK, S_RM_AS, S_RM_AL, S_RM_AD, S_RM_ASTART, S_RM_AEND,
+    S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_LAST_NAMED,
     S_PRIM = 176  // primitives use slots 176.. (scan recursion, sort hist, compaction)
 };
@@ -45,7 +50,7 @@ static_assert(S_LAST_NAMED <= S_PRIM, "scratch slots overlap");
 
 enum Cnt {
     C_N, C_NONN, C_NN, C_NPRE, C_NSUF, C_NS0, C_NS1, C_RESN, C_NP0, C_NP1,
-    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_COUNT = 32
+    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_RM, C_COUNT = 32
 };
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap = 8192) {
@@ -218,6 +223,11 @@ struct curc_block {
     std::vector<uint32_t> s_pe_order;
     std::vector<uint64_t> s_id_pos, s_mis_off_start;
     std::vector<curc_alignment> s_sorted;
+    // pseudogenome self-match (CURC_RUN_REFMATCH): parts HQ, LQ, N-read LQ
+    struct RmPart {
+        std::string text, map_off, map_len;
+        std::vector<uint64_t> matches, found;  // (src, len, dest) triples, with KEEP_STAGES
+    } rm[3];
 };
 
 namespace {
@@ -408,6 +418,243 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
 
 // Record sort and stream builders (a13; compress.cu:1730-2044), on the
 // device where the records are.  Fills the s_* fields of B.
+// ------------------------------------------------ pseudogenome self-match ---
+// RefMatcher::init (compress.cu:698-741): CSR index of source positions 5i,
+// the first 12 per bucket in descending order.
+struct RmIndex {
+    uint64_t count = 0;
+    uint32_t H = 0;
+    uint32_t* kr = nullptr;
+    uint32_t* pos = nullptr;
+};
+
+RmIndex rm_index(curc_ctx* c, const uint64_t* src, uint64_t src_len) {
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    RmIndex ix;
+    if (src_len < RM_K) return ix;
+    ix.count = (src_len - RM_K + 1 + RM_SRC_STEP - 1) / RM_SRC_STEP;
+    if (ix.count > (1ull << 30)) throw std::length_error("pseudogenome too long for the self-match index");
+    ix.H = hash_size(ix.count);
+    uint32_t* k0 = S.as<uint32_t>(S_RM_K0, ix.count + 1);
+    uint32_t* k1 = S.as<uint32_t>(S_RM_K1, ix.count + 1);
+    uint32_t* v0 = S.as<uint32_t>(S_RM_V0, ix.count + 1);
+    uint32_t* v1 = S.as<uint32_t>(S_RM_V1, ix.count + 1);
+    uint32_t* krf = S.as<uint32_t>(S_RM_KRF, (uint64_t)ix.H + 2);
+    ix.kr = S.as<uint32_t>(S_RM_KR, (uint64_t)ix.H + 2);
+    ix.pos = S.as<uint32_t>(S_RM_POS, ix.count + 2);
+    hipLaunchKernelGGL(k_rm_hash, dim3(grid_for(ix.count)), dim3(256), 0, st, src, ix.count, ix.H - 1, k0, v0);
+    radix_sort_pairs<uint32_t>(k0, v0, k1, v1, ix.count, 0, log2u(ix.H), S, S_PRIM, st);
+    hipLaunchKernelGGL(k_gap_fill, dim3(grid_for(ix.count + 1)), dim3(256), 0, st, (const uint32_t*)k0, ix.count,
+                       (uint64_t)ix.H + 1, krf);
+    hipLaunchKernelGGL(k_map_caps, dim3(grid_for((uint64_t)ix.H + 2)), dim3(256), 0, st, (const uint32_t*)krf,
+                       (uint64_t)ix.H, ix.kr);
+    scan_excl(ix.kr, ix.kr, (uint64_t)ix.H + 2, S, S_PRIM, st);
+    hipLaunchKernelGGL(k_map_fill, dim3(grid_for(ix.count)), dim3(256), 0, st, (const uint32_t*)k0,
+                       (const uint32_t*)v0, ix.count, (const uint32_t*)krf, (const uint32_t*)ix.kr, ix.pos);
+    c->add_bytes("refmatch_index", (double)ix.count * (16 + 8) + 8.0 * ix.H, (double)ix.count);
+    return ix;
+}
+
+template <typename T>
+T read_one(curc_ctx* c, const T* p) {
+    T v{};
+    CURC_HIP(hipMemcpyAsync(&v, p, sizeof(T), hipMemcpyDeviceToHost, c->st));
+    CURC_HIP(hipStreamSynchronize(c->st));
+    return v;
+}
+
+void put_triples(curc_ctx* c, std::vector<uint64_t>& out, const uint64_t* s, const uint64_t* l, const uint64_t* d,
+                 uint64_t n) {
+    std::vector<uint64_t> a, b, e;
+    d2h(c, a, s, n);
+    d2h(c, b, l, n);
+    d2h(c, e, d, n);
+    CURC_HIP(hipStreamSynchronize(c->st));
+    out.resize(3 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        out[3 * i] = a[i];
+        out[3 * i + 1] = b[i];
+        out[3 * i + 2] = e[i];
+    }
+}
+
+// RefMatcher::match (compress.cu:855-994) of one destination text, given as
+// 2-bit words (dest2) or ASCII (desta, the character path for N-reads).
+void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_len, const uint64_t* dest2,
+              const uint8_t* desta, uint64_t dest_len, bool is_ref, bool keep, uint32_t* cnt,
+              curc_block::RmPart& out) {
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    const bool binary = desta == nullptr;
+    uint64_t n_found = 0, n_acc = 0, a_lastend = 0;
+    bool any = false;
+    uint64_t *as_ = nullptr, *a_start = nullptr, *a_end = nullptr, *goff = nullptr;
+    uint32_t* len_off = nullptr;
+    if (dest_len >= RM_K && ix.count) {
+        const uint64_t dic = (dest_len - RM_K + 1 + RM_DEST_STEP - 1) / RM_DEST_STEP;
+        const uint64_t capacity = (uint64_t)((double)dic * (is_ref ? 0.2 : 0.21));  // compress.cu:681-682
+        const uint64_t n_slots = binary ? (dic + RM_SEG - 1) / RM_SEG : dic;
+        if (n_slots >= 0xFFFFFFFFull) throw std::length_error("self-match destination too long");
+        RmArgs a{};
+        a.src = src;
+        a.src_len = src_len;
+        a.kr = ix.kr;
+        a.pos = ix.pos;
+        a.mask = ix.H - 1;
+        a.dest_len = dest_len;
+        a.n_slots = n_slots;
+        a.is_ref = is_ref;
+        if (binary) {
+            const uint64_t words = (dest_len + 31) / 32 + 1;
+            uint64_t* rc2 = S.as<uint64_t>(S_RM_RC, words);
+            hipLaunchKernelGGL(k_rm_rc2, dim3(grid_for(words)), dim3(256), 0, st, dest2, dest_len, rc2, words);
+            a.rc2 = rc2;
+        } else {
+            uint8_t* rca = S.as<uint8_t>(S_RM_RC, dest_len + 1);
+            hipLaunchKernelGGL(k_rm_rc_ascii, dim3(grid_for(dest_len)), dim3(256), 0, st, desta, dest_len, rca);
+            a.rca = rca;
+        }
+        a.r_src = S.as<uint64_t>(S_RM_S, n_slots);
+        a.r_len = S.as<uint64_t>(S_RM_L, n_slots);
+        a.r_dest = S.as<uint64_t>(S_RM_D, n_slots);
+        a.flag = S.as<uint8_t>(S_RM_FLAG, n_slots);
+        a.work = S.as<unsigned long long>(S_RM_WORK, 2);
+        fill<unsigned long long>(c, a.work, 2, 0ull);
+        if (binary)
+            hipLaunchKernelGGL(k_rm_match2, dim3(grid_for(n_slots, 1u << 20)), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL(k_rm_match_ascii, dim3(grid_for(n_slots, 1u << 20)), dim3(256), 0, st, a);
+        // results in collection order (compress.cu:794-797, 843-846)
+        uint64_t* fs = S.as<uint64_t>(S_RM_FS, n_slots);
+        uint64_t* fl = S.as<uint64_t>(S_RM_FL, n_slots);
+        uint64_t* fd = S.as<uint64_t>(S_RM_FD, n_slots);
+        set_cnt(c, cnt + C_TMP, (uint32_t)n_slots);
+        compact(cnt + C_TMP, PredFlag8{a.flag}, EmitTriple{a.r_src, a.r_len, a.r_dest, fs, fl, fd}, cnt + C_RM, S,
+                S_PRIM, st);
+        n_found = read_cnt(c, cnt + C_RM);
+        if (c->timing) {
+            unsigned long long wk[2];
+            CURC_HIP(hipMemcpyAsync(wk, a.work, sizeof(wk), hipMemcpyDeviceToHost, st));
+            CURC_HIP(hipStreamSynchronize(st));
+            // per probe: the 36-mer (9 B of 2-bit text) and two CSR offsets;
+            // per verified candidate: its position and both 36-mers; per
+            // extended base: a quarter byte of each text
+            const double probes = binary ? (double)dic : (double)dic * 4.5;
+            c->add_bytes("refmatch", probes * 17 + (double)wk[0] * 22 + (double)wk[1] * 0.5, (double)dic);
+        }
+        if (keep) put_triples(c, out.found, fs, fl, fd, n_found);
+        any = n_found > 0;
+        const uint64_t n = std::min(n_found, capacity);
+        uint64_t n_u = 0;
+        uint64_t *us = nullptr, *ul = nullptr, *ud = nullptr;
+        if (n) {
+            // compress.cu:914-932: transform, sort by (dest, src, len), unique
+            uint64_t* k0 = S.as<uint64_t>(S_RM_SK0, n);
+            uint64_t* k1 = S.as<uint64_t>(S_RM_SK1, n);
+            uint32_t* v0 = S.as<uint32_t>(S_RM_SV0, n);
+            uint32_t* v1 = S.as<uint32_t>(S_RM_SV1, n);
+            hipLaunchKernelGGL(k_rm_transform, dim3(grid_for(n)), dim3(256), 0, st, fs, fl, fd, n, dest_len, is_ref,
+                               k0, v0);
+            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, 0, log2u(dest_len + 1), S, S_PRIM, st);
+            hipLaunchKernelGGL(k_rm_gather_key, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)fs,
+                               (const uint32_t*)v0, n, k0);
+            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, 0, log2u(std::max(src_len, dest_len) + 1), S, S_PRIM, st);
+            hipLaunchKernelGGL(k_rm_gather_key, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)fd,
+                               (const uint32_t*)v0, n, k0);
+            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, 0, log2u(dest_len + 1), S, S_PRIM, st);
+            us = S.as<uint64_t>(S_RM_US, n);
+            ul = S.as<uint64_t>(S_RM_UL, n);
+            ud = S.as<uint64_t>(S_RM_UD, n);
+            set_cnt(c, cnt + C_TMP, (uint32_t)n);
+            compact(cnt + C_TMP, PredUniqueTriple{fs, fl, fd, v0}, EmitTripleIdx{fs, fl, fd, v0, us, ul, ud},
+                    cnt + C_RM, S, S_PRIM, st);
+            n_u = read_cnt(c, cnt + C_RM);
+            if (keep) put_triples(c, out.matches, us, ul, ud, n_u);
+        }
+        if (n_u) {
+            // accepted matches (compress.cu:960-986) as a pointer chain
+            const uint64_t ntile = (n_u + RM_TILE - 1) / RM_TILE;
+            uint64_t* T = S.as<uint64_t>(S_RM_T, n_u);
+            uint64_t* tmax = S.as<uint64_t>(S_RM_TMAX, ntile);
+            hipLaunchKernelGGL(k_rm_reach, dim3((uint32_t)ntile), dim3(RM_TILE), 0, st, (const uint64_t*)ul,
+                               (const uint64_t*)ud, n_u, T, tmax);
+            uint32_t* J[2] = {S.as<uint32_t>(S_RM_J0, n_u + 2), S.as<uint32_t>(S_RM_J1, n_u + 2)};
+            hipLaunchKernelGGL(k_rm_next, dim3(grid_for(n_u + 2)), dim3(256), 0, st, (const uint64_t*)ul,
+                               (const uint64_t*)ud, (const uint64_t*)T, (const uint64_t*)tmax, n_u, J[0]);
+            uint8_t* mark = S.as<uint8_t>(S_RM_MARK, n_u + 2);
+            fill<uint8_t>(c, mark, n_u + 2, 0);
+            fill<uint8_t>(c, mark + n_u, 1, 1);
+            // after round r every chain node within 2^(r+1)-1 steps is marked
+            uint32_t R = 0;
+            while ((1ull << R) < n_u + 2) ++R;
+            int cur = 0;
+            for (uint32_t r = 0; r < R; ++r) {
+                hipLaunchKernelGGL(k_rm_mark, dim3(grid_for(n_u + 2)), dim3(256), 0, st, (const uint32_t*)J[cur],
+                                   n_u + 2, mark);
+                if (r + 1 < R) {
+                    hipLaunchKernelGGL(k_rm_jump, dim3(grid_for(n_u + 2)), dim3(256), 0, st, (const uint32_t*)J[cur],
+                                       n_u + 2, J[1 - cur]);
+                    cur = 1 - cur;
+                }
+            }
+            as_ = S.as<uint64_t>(S_RM_AS, n_u);
+            uint64_t* al = S.as<uint64_t>(S_RM_AL, n_u);
+            uint64_t* ad = S.as<uint64_t>(S_RM_AD, n_u);
+            set_cnt(c, cnt + C_TMP, (uint32_t)n_u);
+            compact(cnt + C_TMP, PredFlag8{mark}, EmitTriple{us, ul, ud, as_, al, ad}, cnt + C_RM, S, S_PRIM, st);
+            n_acc = read_cnt(c, cnt + C_RM);
+            a_start = S.as<uint64_t>(S_RM_ASTART, n_acc + 1);
+            a_end = S.as<uint64_t>(S_RM_AEND, n_acc + 1);
+            len_off = S.as<uint32_t>(S_RM_LB, n_acc + 1);
+            goff = S.as<uint64_t>(S_RM_GAP, n_acc + 1);
+            fill<uint32_t>(c, len_off + n_acc, 1, 0u);
+            fill<uint64_t>(c, goff + n_acc, 1, 0ull);
+            if (n_acc)
+                hipLaunchKernelGGL(k_rm_accept, dim3(grid_for(n_acc)), dim3(256), 0, st, (const uint64_t*)as_,
+                                   (const uint64_t*)al, (const uint64_t*)ad, n_acc, a_start, a_end, len_off, goff);
+            scan_excl(len_off, len_off, n_acc + 1, S, S_PRIM, st);
+            scan_excl_t<uint64_t>(goff, goff, n_acc + 1, S, S_PRIM, st);
+            if (n_acc) a_lastend = read_one(c, a_end + n_acc - 1);
+        }
+    }
+    // rewritten text (compress.cu:953-993; texts are the reverse complement
+    // of a reverse complement, compress.cu:873 and 954)
+    uint64_t gap_total = 0;
+    if (goff) gap_total = read_one(c, goff + n_acc);
+    const uint64_t out_len = gap_total + (dest_len - a_lastend);
+    uint8_t* text = S.as<uint8_t>(S_RM_TEXT, out_len + 1);
+    if (!goff) {
+        goff = S.as<uint64_t>(S_RM_GAP, 1);
+        fill<uint64_t>(c, goff, 1, 0ull);
+    }
+    if (out_len)
+        hipLaunchKernelGGL(k_rm_text, dim3(grid_for((out_len + 63) / 64)), dim3(256), 0, st, dest2, desta,
+                           (const uint64_t*)a_start, (const uint64_t*)a_end, (const uint64_t*)goff, n_acc, dest_len,
+                           out_len, text);
+    out.text.resize(out_len);
+    if (out_len) CURC_HIP(hipMemcpyAsync(&out.text[0], text, out_len, hipMemcpyDeviceToHost, st));
+    out.map_off.clear();
+    out.map_len.clear();
+    if (any) {
+        const bool off32 = src_len <= 0xFFFFFFFFull;
+        const uint32_t w = off32 ? 4 : 8;
+        const uint32_t len_bytes = n_acc ? read_one(c, len_off + n_acc) : 0;
+        uint8_t* moff = S.as<uint8_t>(S_RM_MOFF, n_acc * w + 1);
+        uint8_t* mlen = S.as<uint8_t>(S_RM_MLEN, (uint64_t)len_bytes + 1);
+        fill<uint8_t>(c, mlen, 1, (uint8_t)RM_TARGET);  // helper.cpp:179-187 of 50: one byte
+        if (n_acc)
+            hipLaunchKernelGGL(k_rm_maps, dim3(grid_for(n_acc)), dim3(256), 0, st, (const uint64_t*)as_,
+                               (const uint64_t*)a_start, (const uint64_t*)a_end, (const uint32_t*)len_off, n_acc,
+                               off32, moff, mlen);
+        out.map_off.resize(n_acc * w);
+        out.map_len.resize((uint64_t)len_bytes + 1);
+        if (n_acc) CURC_HIP(hipMemcpyAsync(&out.map_off[0], moff, n_acc * w, hipMemcpyDeviceToHost, st));
+        CURC_HIP(hipMemcpyAsync(&out.map_len[0], mlen, (uint64_t)len_bytes + 1, hipMemcpyDeviceToHost, st));
+    }
+    CURC_HIP(hipStreamSynchronize(st));
+}
+
 void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOut& lq_asm, const AsmOut& nlq_asm,
                    uint32_t* cnt, bool preserve, bool paired) {
     Scratch& S = c->S;
@@ -1003,6 +1250,23 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         build_streams(c, B, hq_asm, lq_asm, nlq_asm, cnt, (flags & CURC_RUN_PRESERVE_ORDER) != 0,
                       (flags & CURC_RUN_PAIRED) != 0);
     }
+
+    // ---- a14: pseudogenome self-match (compress.cu:2053-2069): the LQ and
+    // N-read LQ texts (when present) and then the HQ text against the HQ text
+    if (flags & CURC_RUN_REFMATCH) {
+        Region r(c, "refmatch");
+        const uint64_t hq_len = B->hq_len;
+        const uint64_t* hq2 = S.as<uint64_t>(S_PG_HQ, (hq_len + 31) / 32 + 1);
+        const RmIndex ix = rm_index(c, hq2, hq_len);
+        const bool keepm = keep;
+        if (B->lq_len)
+            rm_match(c, ix, hq2, hq_len, S.as<uint64_t>(S_PG_LQ, (B->lq_len + 31) / 32 + 1), nullptr, B->lq_len,
+                     false, keepm, cnt, B->rm[1]);
+        if (!B->nlq_text.empty())
+            rm_match(c, ix, hq2, hq_len, nullptr, S.as<uint8_t>(S_NPG, B->nlq_text.size() + 16),
+                     B->nlq_text.size(), false, keepm, cnt, B->rm[2]);
+        rm_match(c, ix, hq2, hq_len, hq2, nullptr, hq_len, true, keepm, cnt, B->rm[0]);
+    }
 }
 
 int fail(curc_ctx* c, int code, const std::string& msg) {
@@ -1171,6 +1435,11 @@ int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L,
             CURC_HIP(hipStreamSynchronize(c->st));
             B->Nascii.assign(na.begin(), na.end());
         }
+    } catch (const std::length_error& e) {
+        delete B;
+        hipStreamSynchronize(c->st);
+        c->pending.clear();
+        return fail(c, CURC_ELIMIT, e.what());
     } catch (const std::exception& e) {
         delete B;
         hipStreamSynchronize(c->st);
@@ -1241,6 +1510,17 @@ uint64_t curc_block_get(curc_block* B, int field, const void** ptr) {
         case CURC_F_S_ID_POS: return put(B->s_id_pos);
         case CURC_F_S_SORTED_REC: return put(B->s_sorted);
         default:
+            if (field >= CURC_F_RM_BASE && field < CURC_F_RM_BASE + 15) {
+                const int p = (field - CURC_F_RM_BASE) / 5, k = (field - CURC_F_RM_BASE) % 5;
+                const auto& R = B->rm[p];
+                switch (k) {
+                    case 0: *ptr = R.text.data(); return R.text.size();
+                    case 1: *ptr = R.map_off.data(); return R.map_off.size();
+                    case 2: *ptr = R.map_len.data(); return R.map_len.size();
+                    case 3: *ptr = R.matches.data(); return R.matches.size() / 3;
+                    default: *ptr = R.found.data(); return R.found.size() / 3;
+                }
+            }
             if (field > CURC_F_S_MIS_OFF && (size_t)(field - CURC_F_S_MIS_OFF) + 1 < B->s_mis_off_start.size()) {
                 const size_t m = (size_t)(field - CURC_F_S_MIS_OFF);
                 const uint64_t a = B->s_mis_off_start[m], b = B->s_mis_off_start[m + 1];

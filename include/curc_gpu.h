@@ -137,7 +137,16 @@ enum {
     CURC_F_S_PE_ORDER = 47,    /* u32 global ids in emission order (paired, order not preserved) */
     CURC_F_S_ID_POS = 48,      /* u64 joined-pseudogenome position per read (order preserved) */
     CURC_F_S_SORTED_REC = 49,  /* curc_alignment, HQ+map records after the record sort */
-    CURC_F_S_MIS_OFF = 64      /* + m (1..max_mm): u16 mismatch-offset stream for m mismatches */
+    CURC_F_S_MIS_OFF = 64,     /* + m (1..max_mm): u16 mismatch-offset stream for m mismatches */
+    /* pseudogenome self-match (CURC_RUN_REFMATCH; RefMatcher, compress.cu:597-1001, called at
+     * :2053-2069).  Part p (0 = HQ text against itself, 1 = LQ text, 2 = N-read LQ text, each
+     * against the HQ text) adds 5p to: */
+    CURC_F_RM_BASE = 1000,
+    CURC_F_RM_TEXT = 1000,     /* char: text with each reference replaced by '%' */
+    CURC_F_RM_MAP_OFF = 1001,  /* bytes: u32 source positions (u64 if the HQ text exceeds 2^32-1) */
+    CURC_F_RM_MAP_LEN = 1002,  /* bytes: byte-frugal 50, then byte-frugal (length - 50) per reference */
+    CURC_F_RM_MATCHES = 1003,  /* u64 (src, len, dest) triples after sort + unique (KEEP_STAGES) */
+    CURC_F_RM_FOUND = 1004     /* u64 triples in collection order, before the capacity cut (KEEP_STAGES) */
 };
 
 /* Flags for curc_block_run. */
@@ -146,6 +155,7 @@ enum {
 #define CURC_RUN_STREAMS 4u         /* also build the block's pre-entropy streams */
 #define CURC_RUN_PRESERVE_ORDER 8u  /* streams for --preserve_order */
 #define CURC_RUN_PAIRED 16u         /* streams for paired-end input (ids 2k / 2k+1) */
+#define CURC_RUN_REFMATCH 32u       /* also self-match the pseudogenomes (CURC_F_RM_*) */
 
 typedef struct curc_block curc_block;
 

@@ -114,11 +114,21 @@ orc_refmatch *orc_ref_match(const char *src, uint64_t src_len, const char *dest,
     const uint64_t index_count = (src_len - RM_K + 1 + RM_SRC_STEP - 1) / RM_SRC_STEP;
     const uint32_t H = orc_hash_size(index_count);
     const uint32_t mask = H - 1;
+    std::vector<uint32_t> hs(index_count);
     std::vector<uint8_t> cnt(H, 0);
-    std::vector<uint32_t> bucket((uint64_t)H * RM_BUCKET);
     for (uint64_t i = 0; i < index_count; ++i) {
-        const uint32_t h = marush36(src + i * RM_SRC_STEP) & mask;
-        if (cnt[h] < RM_BUCKET) bucket[(uint64_t)h * RM_BUCKET + cnt[h]++] = (uint32_t)i;
+        hs[i] = marush36(src + i * RM_SRC_STEP) & mask;
+        if (cnt[hs[i]] < RM_BUCKET) ++cnt[hs[i]];
+    }
+    std::vector<uint32_t> range((uint64_t)H + 1, 0);
+    for (uint64_t h = 0; h < H; ++h) range[h + 1] = range[h] + cnt[h];
+    // bucket h holds its first cnt[h] occurrences at range[h] + 0.. in
+    // ascending index order; probes visit them from the last
+    std::vector<uint32_t> bucket(range[H]);
+    std::vector<uint8_t> fillc(H, 0);
+    for (uint64_t i = 0; i < index_count; ++i) {
+        const uint32_t h = hs[i];
+        if (fillc[h] < cnt[h]) bucket[range[h] + fillc[h]++] = (uint32_t)i;
     }
 
     // one probe (compress.cu:759-803 binary, :815-851 character)
@@ -126,7 +136,7 @@ orc_refmatch *orc_ref_match(const char *src, uint64_t src_len, const char *dest,
     auto probe = [&](uint64_t dp, Match &m) -> bool {
         const uint32_t h = marush36(&rc[dp]) & mask;
         for (int q = (int)cnt[h] - 1; q >= 0; --q) {
-            const uint64_t sp = (uint64_t)bucket[(uint64_t)h * RM_BUCKET + q] * RM_SRC_STEP;
+            const uint64_t sp = (uint64_t)bucket[range[h] + q] * RM_SRC_STEP;
             if (binary && dest_is_ref && (dest_len - sp < dp)) continue;
             uint64_t p1 = dp + RM_K - 1, p2 = sp + RM_K - 1;
             while (++p1 != dest_len && ++p2 != src_len && rc[p1] == src[p2]) {

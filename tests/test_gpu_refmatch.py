@@ -1,0 +1,84 @@
+"""GPU pseudogenome self-match (RefMatcher, compress.cu:597-1001) vs the
+oracle: the collected results, the sorted unique matches, the rewritten
+texts and the MapOff / MapLen streams of all three parts, then a lossless
+archive round trip of GPU-built blocks."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import curc_amd as ca
+from curc_amd import archive
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+PARTS = ("hq", "lq", "nlq")
+
+
+def check(ctx, reads, label):
+    print(f"refmatch {label}: {reads.shape[0]} reads x {reads.shape[1]}", flush=True)
+    g = ctx.run_block(reads, keep_stages=True, refmatch=True)
+    o = orc.OracleBlock(reads, stop="all")
+    o.refmatch()
+    for p in PARTS:
+        for k in ("found", "matches", "text", "off", "len"):
+            a, b = g.get(f"rm_{p}_{k}"), o.get(f"rm_{p}_{k}")
+            assert a.shape == b.shape and np.array_equal(a, b), f"{label}: rm_{p}_{k} differs"
+    return g, o
+
+
+def test_refmatch_small_block(gpu_ctx):
+    cfg = ca.synth_config(seed=71, genome_len=80_000, gc=0.45, n_reads=12_000, L=150, sub_first=0.002,
+                          sub_last=0.02, n_rate=0.002)
+    g, o = check(gpu_ctx, ca.synth_reads(cfg)[1], "small")
+    assert len(o.get("rm_hq_matches")) > 10 and len(o.get("rm_lq_found")) > 0
+
+
+def test_refmatch_config_small(gpu_ctx):
+    check(gpu_ctx, ca.synth_reads(ca.config_small())[1], "config_small")
+
+
+@pytest.mark.parametrize("L", [75, 100, 250])
+def test_refmatch_read_lengths(gpu_ctx, L):
+    cfg = ca.synth_config(seed=72 + L, genome_len=40_000, gc=0.5, n_reads=40_000 * 15 // L, L=L,
+                          sub_first=0.002, sub_last=0.01, n_rate=0.002)
+    check(gpu_ctx, ca.synth_reads(cfg)[1], f"L{L}")
+
+
+def test_refmatch_repeats(gpu_ctx):
+    """Repeat-rich genome: buckets fill up and candidate order decides."""
+    cfg = ca.synth_config(seed=75, genome_len=100_000, gc=0.45, repeat_frac=0.45, rep_min=300, rep_max=3000,
+                          rep_families=8, div_min=0.0, div_max=0.02, n_reads=15_000, L=150, sub_first=0.001,
+                          sub_last=0.004, n_rate=0.001)
+    check(gpu_ctx, ca.synth_reads(cfg)[1], "repeats")
+
+
+def test_refmatch_tiny_block(gpu_ctx):
+    cfg = ca.synth_config(seed=76, genome_len=2_000, gc=0.5, n_reads=300, L=100, n_rate=0.004)
+    check(gpu_ctx, ca.synth_reads(cfg)[1], "tiny")
+
+
+@pytest.mark.parametrize("preserve,paired", [(False, False), (True, False), (False, True)])
+def test_gpu_archive_round_trip(gpu_ctx, preserve, paired):
+    cfg = ca.synth_config(seed=77, genome_len=60_000, gc=0.45, n_reads=10_000, L=100, paired=int(paired),
+                          sub_first=0.002, sub_last=0.01, n_rate=0.002)
+    _, reads = ca.synth_reads(cfg)
+    print(f"round trip preserve={preserve} paired={paired}", flush=True)
+    blk = gpu_ctx.run_block(reads, streams=True, preserve_order=preserve, paired=paired, refmatch=True)
+    bs = archive.BlockStreams.from_block(blk, len(reads), 100, preserve, paired)
+    blk.free()
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "a.curc")
+        archive.write_archive(p, [bs], preserve, paired, 100)
+        pr, pa, L, blocks = archive.read_archive(p)
+        dec = archive.decode_block(blocks[0], L, pr, pa)
+    if preserve:
+        assert np.array_equal(dec, reads)
+    elif paired:
+        got = sorted(zip(map(bytes, dec[0::2]), map(bytes, dec[1::2])))
+        want = sorted(zip(map(bytes, reads[0::2]), map(bytes, reads[1::2])))
+        assert got == want
+    else:
+        assert sorted(map(bytes, dec)) == sorted(map(bytes, reads))
