@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--cpu-reads", type=int, default=2_000_000)
     p.add_argument("--no-full-block", action="store_true")
     p.add_argument("--no-bits-per-base", action="store_true")
+    p.add_argument("--no-batch", action="store_true")
     return p.parse_args()
 
 
@@ -83,6 +84,49 @@ def bits_per_base():
     lossless = sorted(map(bytes, dec)) == sorted(map(bytes, reads))
     return dict(value=round(8.0 * size / reads.size, 5), archive_bytes=size, lossless=bool(lossless),
                 workload="1 Mbp uniform genome, 133,333 x 150 bp SE, 0.5% subst, 0.01% N, order not preserved")
+
+
+def batched_inputs(n_inputs=64, seed=404):
+    """Many-input workload: genomes of 0.2-10 Mbp, read lengths from
+    {75, 100, 150, 250}, read counts log-uniform in [10k, 500k], 0.3-1%
+    substitutions, 0.02% N."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inputs):
+        L = int(rng.choice([75, 100, 150, 250]))
+        n = int(np.exp(rng.uniform(np.log(10_000), np.log(500_000))))
+        sub = float(rng.uniform(0.003, 0.01))
+        g = int(min(10_000_000, max(200_000, n * L // 20)))
+        cfg = ca.synth_config(seed=seed * 1000 + i, genome_len=g, gc=0.45, n_reads=n, L=L, sub_first=sub,
+                              sub_last=sub, n_rate=0.0002)
+        out.append(ca.synth_reads(cfg)[1])
+    return out
+
+
+def batched(ins, reps=2):
+    """The many-input workload one input after another on one stream, then
+    as one batch on concurrent streams; both from host memory."""
+    gbases = sum(r.size for r in ins) / 1e9
+    with ca.Context(0) as ctx:
+        for b in ctx.batch_run(ins[:4], workers=4):
+            b.free()
+        t = time.perf_counter()
+        for _ in range(reps):
+            for r in ins:
+                ctx.run_block(r).free()
+        ctx.device_sync()
+        seq = (time.perf_counter() - t) / reps
+        t = time.perf_counter()
+        for _ in range(reps):
+            for b in ctx.batch_run(ins, workers=0):
+                b.free()
+        ctx.device_sync()
+        bat = (time.perf_counter() - t) / reps
+    return dict(inputs=len(ins), gbases=round(gbases, 4), sequential_s=round(seq, 4), batched_s=round(bat, 4),
+                sequential_gbases_s=round(gbases / seq, 4), batched_gbases_s=round(gbases / bat, 4),
+                speedup=round(seq / bat, 3),
+                workload="64 inputs: genomes 0.2-10 Mbp, L in {75,100,150,250}, 10k-500k reads log-uniform, "
+                         "0.3-1% subst, input in host memory")
 
 
 def cpu_baseline(n_reads, L):
@@ -181,6 +225,12 @@ def main():
     if not a.no_bits_per_base:
         bpb = bits_per_base()
         log(f"bits/base on the 1 Mbp workload: {bpb['value']} (lossless round trip: {bpb['lossless']})")
+    bat = None
+    if not a.no_batch:
+        log("many-input batch ...")
+        bat = batched(batched_inputs())
+        log(f"  {bat['inputs']} inputs, {bat['gbases']} Gbases: one by one {bat['sequential_s']} s, "
+            f"batched {bat['batched_s']} s ({bat['speedup']}x)")
     cpu = None
     if not a.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample ...")
@@ -207,6 +257,7 @@ def main():
         "cpu_baseline": cpu,
         "bits_per_base": bpb,
         "full_block": full,
+        "batched": bat,
     }
     print(json.dumps(out))
 

@@ -57,6 +57,11 @@ class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("L", "W", "max_off", "k", "kip", "max_mm")]
 
 
+class Job(ctypes.Structure):
+    _fields_ = [("ascii", ctypes.c_void_p), ("n_reads", ctypes.c_uint64), ("L", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
+
+
 class StageStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("launches", ctypes.c_uint64),
                 ("alg_bytes", ctypes.c_double), ("items", ctypes.c_double)]
@@ -102,6 +107,7 @@ def lib():
         "curc_device_sync": (i32, [vp]),
         "curc_pack": (i32, [vp, vp, u64, u32, vp, vp]),
         "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
+        "curc_batch_run": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(i32)]),
         "curc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "curc_block_free": (None, [vp]),
         "curc_ctx_set_timing": (i32, [vp, i32]),
@@ -258,6 +264,32 @@ class Context:
             rc = lib().curc_block_run(self._h, reads.ctypes.data, n, L, stop_after, flags, ctypes.byref(out))
         self._check(rc, "curc_block_run")
         return Block(out.value, has_refmatch=refmatch)
+
+    def batch_run(self, inputs, workers=0, keep_stages=False, streams=False, preserve_order=False, paired=False,
+                  refmatch=False):
+        """Runs independent blocks together (curc_batch_run): inputs is a
+        list of (n, L) uint8 host arrays (lengths may differ); returns one
+        Block per input, in input order."""
+        arrs = [np.ascontiguousarray(r, dtype=np.uint8) for r in inputs]
+        flags = RUN_KEEP_STAGES if keep_stages else 0
+        if streams:
+            flags |= RUN_STREAMS | (RUN_PRESERVE_ORDER if preserve_order else 0) | (RUN_PAIRED if paired else 0)
+        if refmatch:
+            flags |= RUN_REFMATCH
+        n = len(arrs)
+        jobs = (Job * max(1, n))()
+        for i, a in enumerate(arrs):
+            jobs[i] = Job(a.ctypes.data, a.shape[0], a.shape[1], flags)
+        out = (ctypes.c_void_p * max(1, n))()
+        status = (ctypes.c_int32 * max(1, n))()
+        rc = lib().curc_batch_run(self._h, jobs, n, workers, out, status)
+        blocks = [Block(out[i], has_refmatch=refmatch) if out[i] else None for i in range(n)]
+        if rc != CURC_OK:
+            for b in blocks:
+                if b is not None:
+                    b.free()
+            self._check(rc, "curc_batch_run")
+        return blocks
 
     def dev_alloc(self, nbytes):
         p = ctypes.c_void_p()

@@ -346,12 +346,22 @@ __global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t*
     const uint32_t ns = *cs.d_ns;
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(E::BPW, len);
+    const uint32_t np = *cs.d_np;
+    if (!rng) {
+        // no block ranges (short lists): search the whole prefix list, and
+        // clear the prefix flags the ranges kernel would have cleared
+        GRID_STRIDE(i, np) cs.plinked[i] = 0;
+    }
     GRID_STRIDE(i, ns) {
         cs.slinked[i] = 0;
         uint32_t s = cs.sl[i];
         const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-        uint32_t b = (uint32_t)E::word(rs, off, bp);
-        uint32_t lo = rng[b], e = rng[b + 1];
+        uint32_t lo = 0, e = np;
+        if (rng) {
+            const uint32_t b = (uint32_t)E::word(rs, off, bp);
+            lo = rng[b];
+            e = rng[b + 1];
+        }
         uint64_t sx0 = E::word(rs, off, n1);
         uint32_t cnt = e - lo;
         while (cnt > 0) {
@@ -382,7 +392,11 @@ __global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t*
 
 // Uncontended groups link at once; members of contended groups are queued
 // for their leader.
-__global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t off) {
+// zero_p / zero_n: a buffer the next round needs cleared (the block ranges
+// of the exact-matching loop), folded in here to save a launch.
+__global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t off, uint32_t* zero_p = nullptr,
+                                                        uint64_t zero_n = 0) {
+    GRID_STRIDE(z, zero_n) zero_p[z] = 0;
     const uint32_t ns = *cs.d_ns;
     GRID_STRIDE(i, ns) {
         uint32_t ch = cs.choice[i];

@@ -6,7 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <numeric>
+#include <thread>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -96,6 +99,7 @@ struct curc_ctx {
     Scratch S;
     std::string err;
     bool timing = false;
+    std::vector<curc_ctx*> workers;  // extra stream/arena pairs of curc_batch_run
     std::vector<StageStat> stats;
     std::vector<PendingTimer> pending;
     std::vector<hipEvent_t> event_pool;
@@ -316,7 +320,7 @@ AsmOut assemble(curc_ctx* c, const uint32_t* V, uint32_t n, uint64_t id_space, c
     uint32_t* hv0 = S.as<uint32_t>(S_A_HV0, n);
     uint32_t* hv1 = S.as<uint32_t>(S_A_HV1, n);
     set_cnt(c, cnt + C_TMP, n);
-    compact(cnt + C_TMP, PredHead{prvR}, EmitHead{ckey, hk0, hv0}, cnt + C_ASM, S, S_PRIM, st);
+    compact(cnt + C_TMP, PredHead{prvR}, EmitHead{ckey, hk0, hv0}, cnt + C_ASM, S, S_PRIM, st, n);
     const uint32_t nh = read_cnt(c, cnt + C_ASM);
     radix_sort_pairs<uint32_t>(hk0, hv0, hk1, hv1, nh, 0, log2u((uint64_t)n + 1), S, S_PRIM, st);
     uint64_t* olen = S.as<uint64_t>(S_A_OLEN, (uint64_t)nh + 1);
@@ -363,9 +367,13 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
                 const uint32_t* d_suf, uint32_t* cnt, uint32_t n_max, uint32_t* next, uint32_t* prev,
                 uint16_t* offset) {
     constexpr uint32_t BITS = (E::BPW == 32) ? 2 : 3;
-    // block width: about one block per list entry, at most 2^20 block values
+    // block width: about one block per list entry, at most 2^20 block values;
+    // short lists (<= FM_SHORT) go without block ranges, searching the whole
+    // prefix list, which saves the fill / histogram / scan launches
+    constexpr uint32_t FM_SHORT = 1u << 12;
     uint32_t MAX_BP = 1;
     while (MAX_BP < ((BITS == 2) ? 10u : 6u) && (1ull << (BITS * MAX_BP)) < n_max) ++MAX_BP;
+    const bool ranged = n_max > FM_SHORT;
     if (n_max == 0) return;
     Scratch& S = c->S;
     uint32_t* pl[2] = {S.as<uint32_t>(S_PL0, n_max), S.as<uint32_t>(S_PL1, n_max)};
@@ -395,23 +403,28 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
                        (const uint32_t*)np[0], W, pk[0]);
     const uint32_t g = grid_for(n_max, 4096);
     int cur = 0;
+    if (ranged) fill<uint32_t>(c, rng, (1ull << (BITS * std::min(MAX_BP, L - 1))) + 1, 0u);
     for (uint32_t off = 1; off < L; ++off) {
         const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
         const uint64_t nb = 1ull << (BITS * bp);
-        fill<uint32_t>(c, rng, nb + 1, 0u);
-        hipLaunchKernelGGL(k_fm_block_hist<E>, dim3(grid_for(n_max, 4096)), dim3(256), 0, c->st,
-                           (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng, plinked);
-        scan_excl(rng, rng, nb + 1, S, S_PRIM, c->st);
+        if (ranged) {
+            hipLaunchKernelGGL(k_fm_block_hist<E>, dim3(grid_for(n_max, 4096)), dim3(256), 0, c->st,
+                               (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng, plinked);
+            scan_excl(rng, rng, nb + 1, S, S_PRIM, c->st);
+        }
         ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
                       choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                       slinked, plinked};
-        hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs, (const uint32_t*)rng, bp, off);
-        hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off);
+        hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs, ranged ? (const uint32_t*)rng : nullptr,
+                           bp, off);
+        // the ranges are rebuilt next round over at most nb + 1 values
+        hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, ranged ? rng : nullptr,
+                           ranged ? nb + 1 : 0ull);
         hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
         compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
-                c->st);
+                c->st, n_max);
         compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
-                np[1 - cur], S, S_PRIM, c->st);
+                np[1 - cur], S, S_PRIM, c->st, n_max);
         cur = 1 - cur;
     }
 }
@@ -531,7 +544,7 @@ void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_
         uint64_t* fd = S.as<uint64_t>(S_RM_FD, n_slots);
         set_cnt(c, cnt + C_TMP, (uint32_t)n_slots);
         compact(cnt + C_TMP, PredFlag8{a.flag}, EmitTriple{a.r_src, a.r_len, a.r_dest, fs, fl, fd}, cnt + C_RM, S,
-                S_PRIM, st);
+                S_PRIM, st, n_slots);
         n_found = read_cnt(c, cnt + C_RM);
         if (c->timing) {
             unsigned long long wk[2];
@@ -568,7 +581,7 @@ void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_
             ud = S.as<uint64_t>(S_RM_UD, n);
             set_cnt(c, cnt + C_TMP, (uint32_t)n);
             compact(cnt + C_TMP, PredUniqueTriple{fs, fl, fd, v0}, EmitTripleIdx{fs, fl, fd, v0, us, ul, ud},
-                    cnt + C_RM, S, S_PRIM, st);
+                    cnt + C_RM, S, S_PRIM, st, n);
             n_u = read_cnt(c, cnt + C_RM);
             if (keep) put_triples(c, out.matches, us, ul, ud, n_u);
         }
@@ -602,7 +615,8 @@ void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_
             uint64_t* al = S.as<uint64_t>(S_RM_AL, n_u);
             uint64_t* ad = S.as<uint64_t>(S_RM_AD, n_u);
             set_cnt(c, cnt + C_TMP, (uint32_t)n_u);
-            compact(cnt + C_TMP, PredFlag8{mark}, EmitTriple{us, ul, ud, as_, al, ad}, cnt + C_RM, S, S_PRIM, st);
+            compact(cnt + C_TMP, PredFlag8{mark}, EmitTriple{us, ul, ud, as_, al, ad}, cnt + C_RM, S, S_PRIM, st,
+                    n_u);
             n_acc = read_cnt(c, cnt + C_RM);
             a_start = S.as<uint64_t>(S_RM_ASTART, n_acc + 1);
             a_end = S.as<uint64_t>(S_RM_AEND, n_acc + 1);
@@ -837,8 +851,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), lds, st, d_ascii, N, L, W, packed, isN);
         }
         set_cnt(c, cnt + C_N, (uint32_t)N);
-        compact(cnt + C_N, PredIsN{isN, false}, EmitIndex{nonN}, cnt + C_NONN, S, S_PRIM, st);
-        compact(cnt + C_N, PredIsN{isN, true}, EmitIndex{Nids}, cnt + C_NN, S, S_PRIM, st);
+        compact(cnt + C_N, PredIsN{isN, false}, EmitIndex{nonN}, cnt + C_NONN, S, S_PRIM, st, N);
+        compact(cnt + C_N, PredIsN{isN, true}, EmitIndex{Nids}, cnt + C_NN, S, S_PRIM, st, N);
         c->add_bytes("pack", (double)N * (L + 8.0 * W) + 4.0 * N, (double)N);
     }
     n = read_cnt(c, cnt + C_NONN);
@@ -903,8 +917,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     // ---- a5: prefix k-mer index (compress.cu:1427-1471)
     uint32_t* va = S.as<uint32_t>(S_VA, (uint64_t)n + 1);
     uint32_t* sl0 = S.as<uint32_t>(S_SL0, (uint64_t)n + 1);
-    compact(cnt + C_TMP, PredLinkState{sorted, prev}, EmitGather{sorted, va}, cnt + C_NPRE, S, S_PRIM, st);
-    compact(cnt + C_TMP, PredLinkState{sorted, next}, EmitGather{sorted, sl0}, cnt + C_NSUF, S, S_PRIM, st);
+    compact(cnt + C_TMP, PredLinkState{sorted, prev}, EmitGather{sorted, va}, cnt + C_NPRE, S, S_PRIM, st, n);
+    compact(cnt + C_TMP, PredLinkState{sorted, next}, EmitGather{sorted, sl0}, cnt + C_NSUF, S, S_PRIM, st, n);
     const uint32_t npre = read_cnt(c, cnt + C_NPRE);
     const uint32_t nsuf = read_cnt(c, cnt + C_NSUF);
     {
@@ -957,7 +971,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
                 hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
                 compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
-                        st);
+                        st, nsuf);
                 cur = 1 - cur;
             }
         }
@@ -986,8 +1000,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                            (const uint32_t*)prev, (const uint16_t*)off, N, res);
         hipLaunchKernelGGL(k_prune_unlink, dim3(grid_for(N)), dim3(256), 0, st, next, prev, (const uint8_t*)res, N);
         set_cnt(c, cnt + C_TMP, n);
-        compact(cnt + C_TMP, PredRes{sorted, res, true}, EmitGather{sorted, hq}, cnt + C_HQ, S, S_PRIM, st);
-        compact(cnt + C_TMP, PredRes{sorted, res, false}, EmitGather{sorted, unm}, cnt + C_UNM, S, S_PRIM, st);
+        compact(cnt + C_TMP, PredRes{sorted, res, true}, EmitGather{sorted, hq}, cnt + C_HQ, S, S_PRIM, st, n);
+        compact(cnt + C_TMP, PredRes{sorted, res, false}, EmitGather{sorted, unm}, cnt + C_UNM, S, S_PRIM, st, n);
         c->add_bytes("prune", 28.0 * N, (double)N);
     }
     n_hq = read_cnt(c, cnt + C_HQ);
@@ -1014,8 +1028,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint32_t* pre = S.as<uint32_t>(S_PRE, (uint64_t)n_hq + 1);
         uint32_t* suf = S.as<uint32_t>(S_SUF, (uint64_t)n_hq + 1);
         set_cnt(c, cnt + C_TMP, n_hq);
-        compact(cnt + C_TMP, PredLinkState{hq, prev}, EmitGather{hq, pre}, cnt + C_PRE, S, S_PRIM, st);
-        compact(cnt + C_TMP, PredLinkState{hq, next}, EmitGather{hq, suf}, cnt + C_SUF, S, S_PRIM, st);
+        compact(cnt + C_TMP, PredLinkState{hq, prev}, EmitGather{hq, pre}, cnt + C_PRE, S, S_PRIM, st, n_hq);
+        compact(cnt + C_TMP, PredLinkState{hq, next}, EmitGather{hq, suf}, cnt + C_SUF, S, S_PRIM, st, n_hq);
         full_match<Enc2>(c, packed, W, L, pre, suf, cnt, n_hq, next, prev, off);
     }
     if (keep || stop <= CURC_STAGE_FULLMATCH) {
@@ -1126,13 +1140,13 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint32_t* umN = S.as<uint32_t>(S_UNMAPPED_N, (uint64_t)nN + 1);
         set_cnt(c, cnt + C_TMP, (uint32_t)nm);
         compact(cnt + C_TMP, PredMapped{last, P.max_mm, 0, true},
-                EmitMapRec{last, word, unm, n_unm, rid, rw, mis}, cnt + C_NMAP, S, S_PRIM, st);
+                EmitMapRec{last, word, unm, n_unm, rid, rw, mis}, cnt + C_NMAP, S, S_PRIM, st, nm);
         set_cnt(c, cnt + C_TMP, n_unm);
         compact(cnt + C_TMP, PredMapped{last, P.max_mm, 0, false}, EmitUnmapped{unm, um}, cnt + C_NUNMAP, S,
-                S_PRIM, st);
+                S_PRIM, st, n_unm);
         set_cnt(c, cnt + C_TMP, nN);
         compact(cnt + C_TMP, PredMapped{last, P.max_mm, n_unm, false}, EmitUnmapped{nullptr, umN},
-                cnt + C_NUNMAPN, S, S_PRIM, st);
+                cnt + C_NUNMAPN, S, S_PRIM, st, nN);
         const uint32_t nmap = read_cnt(c, cnt + C_NMAP);
         const uint32_t nun = read_cnt(c, cnt + C_NUNMAP);
         const uint32_t nunN = read_cnt(c, cnt + C_NUNMAPN);
@@ -1227,8 +1241,10 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint32_t* pre = S.as<uint32_t>(S_PRE, n_nlq);
         uint32_t* suf = S.as<uint32_t>(S_SUF, n_nlq);
         set_cnt(c, cnt + C_TMP, n_nlq);
-        compact(cnt + C_TMP, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE, S, S_PRIM, st);
-        compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st);
+        compact(cnt + C_TMP, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE, S, S_PRIM, st,
+                n_nlq);
+        compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st,
+                n_nlq);
         full_match<Enc3>(c, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff);
         nlq_asm = assemble(c, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt, S_NLQIDS);
         const AsmOut& a = nlq_asm;
@@ -1318,6 +1334,8 @@ curc_ctx* curc_ctx_create(int device) {
 
 void curc_ctx_destroy(curc_ctx* c) {
     if (!c) return;
+    for (curc_ctx* w : c->workers) curc_ctx_destroy(w);
+    c->workers.clear();
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
     c->S.release();
@@ -1556,6 +1574,75 @@ int curc_ctx_stats(curc_ctx* c, curc_stage_stat* out, int max) {
 
 void curc_ctx_reset_stats(curc_ctx* c) {
     if (c) c->stats.clear();
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Jobs run on up to CURC_BATCH_MAX_WORKERS streams of one device, each with
+// its own scratch arena, fed largest job first from a shared counter (the
+// reference instead starts one std::async task per block and serialises
+// their GPU phases with a per-device try_lock, compress.cu:52-65,
+// preprocess.cpp:205-276).
+int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t workers, curc_block** out,
+                   int* status) {
+    if (!c || (!jobs && n_jobs) || (!out && n_jobs)) return CURC_EINVAL;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        out[j] = nullptr;
+        if (status) status[j] = CURC_OK;
+    }
+    if (n_jobs == 0) return CURC_OK;
+    if (workers == 0) workers = CURC_BATCH_MAX_WORKERS;
+    workers = std::min<uint32_t>(std::min<uint32_t>(workers, CURC_BATCH_MAX_WORKERS), n_jobs);
+    while (c->workers.size() + 1 < workers) {
+        curc_ctx* w = curc_ctx_create(c->device);
+        if (!w) return fail(c, CURC_EHIP, "could not create a worker stream");
+        c->workers.push_back(w);
+    }
+    std::vector<curc_ctx*> pool{c};
+    for (uint32_t w = 1; w < workers; ++w) {
+        c->workers[w - 1]->timing = c->timing;
+        pool.push_back(c->workers[w - 1]);
+    }
+    std::vector<uint32_t> order(n_jobs);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return jobs[a].n_reads * (uint64_t)jobs[a].L > jobs[b].n_reads * (uint64_t)jobs[b].L;
+    });
+    std::atomic<uint32_t> next{0};
+    std::vector<int> rc(n_jobs, CURC_OK);
+    auto work = [&](curc_ctx* wc) {
+        hipSetDevice(wc->device);
+        for (uint32_t k = next++; k < n_jobs; k = next++) {
+            const uint32_t j = order[k];
+            rc[j] = curc_block_run(wc, jobs[j].ascii, jobs[j].n_reads, jobs[j].L, CURC_STAGE_ALL, jobs[j].flags,
+                                   &out[j]);
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t w = 1; w < workers; ++w) th.emplace_back(work, pool[w]);
+    work(c);
+    for (auto& t : th) t.join();
+    int first = CURC_OK;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        if (status) status[j] = rc[j];
+        if (rc[j] != CURC_OK && first == CURC_OK) first = rc[j];
+    }
+    // worker errors and stage times land on the calling context
+    for (uint32_t w = 1; w < workers; ++w) {
+        curc_ctx* wc = pool[w];
+        if (first != CURC_OK && c->err.empty()) c->err = wc->err;
+        for (const StageStat& st : wc->stats) {
+            const int id = c->stat_id(st.name.c_str());
+            c->stats[id].ms += st.ms;
+            c->stats[id].launches += st.launches;
+            c->stats[id].alg_bytes += st.alg_bytes;
+            c->stats[id].items += st.items;
+        }
+        wc->stats.clear();
+    }
+    return first;
 }
 
 }  // extern "C"

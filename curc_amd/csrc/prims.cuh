@@ -277,6 +277,10 @@ void radix_sort_pairs(K*& k, uint32_t*& v, K*& kt, uint32_t*& vt, uint64_t n, ui
 // Stable compaction over [0, *d_n) with a fixed grid: block b owns a
 // contiguous chunk, so output order equals input order.
 constexpr uint32_t COMPACT_GRID = 1024;
+// Lists whose host-known bound is at most COMPACT_SMALL go through one
+// workgroup in one launch (16 items per thread per round).
+constexpr uint32_t COMPACT_SMALL = 1u << 15;
+constexpr uint32_t COMPACT1_ITEMS = 16;
 
 template <typename Pred>
 __global__ void __launch_bounds__(256) k_compact_count(const uint32_t* __restrict__ d_n, Pred pred,
@@ -330,11 +334,57 @@ __global__ void __launch_bounds__(256) k_compact_scatter(const uint32_t* __restr
     }
 }
 
+// One workgroup of 1024 threads; each round covers 16 consecutive items per
+// thread.  Correct for any n, fast for short lists.
+template <typename Pred, typename Emit>
+__global__ void __launch_bounds__(1024) k_compact_small(const uint32_t* __restrict__ d_n, Pred pred, Emit emit,
+                                                        uint32_t* __restrict__ d_out_n) {
+    __shared__ uint32_t lds[16];
+    const uint32_t n = *d_n;
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
+    uint32_t o = 0;
+    for (uint64_t base = 0; base < n; base += 1024ull * COMPACT1_ITEMS) {
+        const uint64_t first = base + (uint64_t)t * COMPACT1_ITEMS;
+        uint32_t bits = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < COMPACT1_ITEMS; ++r) {
+            const uint64_t i = first + r;
+            if (i < n && pred((uint32_t)i)) bits |= 1u << r;
+        }
+        const uint32_t c = (uint32_t)__popc(bits);
+        const uint32_t inc = wave_incl_scan(c);
+        if (lane == 63) lds[w] = inc;
+        __syncthreads();
+        uint32_t wo = 0, tot = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 16; ++i) {
+            const uint32_t s = lds[i];
+            if (i < w) wo += s;
+            tot += s;
+        }
+        __syncthreads();
+        uint32_t j = o + wo + inc - c;
+        while (bits) {
+            const uint32_t r = (uint32_t)__builtin_ctz(bits);
+            bits &= bits - 1;
+            emit((uint32_t)(first + r), j++);
+        }
+        o += tot;
+    }
+    if (t == 0) *d_out_n = o;
+}
+
 // Stable compaction: emit(i, j) is called for the j-th i in [0, *d_n) with
 // pred(i); the number of emitted items is written to *d_out_n (device).
+// `bound` is a host-known upper bound of *d_n.
 template <typename Pred, typename Emit>
 void compact(const uint32_t* d_n, Pred pred, Emit emit, uint32_t* d_out_n, Scratch& S, int slot,
-             hipStream_t st, uint32_t grid = COMPACT_GRID) {
+             hipStream_t st, uint64_t bound = ~0ull) {
+    if (bound <= COMPACT_SMALL) {
+        hipLaunchKernelGGL((k_compact_small<Pred, Emit>), dim3(1), dim3(1024), 0, st, d_n, pred, emit, d_out_n);
+        return;
+    }
+    const uint32_t grid = COMPACT_GRID;
     uint32_t* counts = S.as<uint32_t>(slot, grid);
     hipLaunchKernelGGL(k_compact_count<Pred>, dim3(grid), dim3(256), 0, st, d_n, pred, counts);
     hipLaunchKernelGGL(k_compact_offsets, dim3(1), dim3(1024), 0, st, counts, grid, d_out_n);

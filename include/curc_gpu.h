@@ -205,6 +205,25 @@ void curc_synth_default(curc_synth_cfg *cfg);
 int curc_synth_genome(const curc_synth_cfg *cfg, char *genome);
 int curc_synth_reads(const curc_synth_cfg *cfg, const char *genome, char *reads);
 
+/* ---- batching ------------------------------------------------------------
+ * Many independent blocks or inputs (uneven read counts and lengths) in one
+ * call.  Replaces the per-block std::async fan-out of preprocess.cpp:205-276
+ * whose GPU phases serialise on a per-device try_lock (compress.cu:52-65):
+ * here the jobs run concurrently on up to CURC_BATCH_MAX_WORKERS streams of
+ * the context's device, each with its own scratch arena, largest job first.
+ * out[i] receives job i's block (NULL if it failed; status[i], when given,
+ * holds its code).  workers = 0 picks the maximum.  Returns the first
+ * failing job's code, or CURC_OK. */
+#define CURC_BATCH_MAX_WORKERS 4u
+typedef struct curc_job {
+    const char *ascii; /* n_reads * L ASCII bytes (device pointer with CURC_RUN_INPUT_ON_DEVICE) */
+    uint64_t n_reads;
+    uint32_t L;
+    uint32_t flags; /* CURC_RUN_* */
+} curc_job;
+int curc_batch_run(curc_ctx *ctx, const curc_job *jobs, uint32_t n_jobs, uint32_t workers, curc_block **out,
+                   int *status);
+
 #ifdef __cplusplus
 }
 #endif
