@@ -193,12 +193,15 @@ def main():
     dt = time.perf_counter() - t0
     ctx.set_timing(False)
     stats = ctx.stats()
+    work = {s["name"]: int(s["items"] / a.steps) for s in stats if s["name"].startswith("map_") and s["ms"] == 0}
+    stats = [s for s in stats if not (s["name"].startswith("map_") and s["ms"] == 0)]
     ms = dt * 1e3 / a.steps
     gbases = a.reads * L / 1e9
     value = gbases / (ms / 1e3)
     log("stage breakdown (ms per step):")
     for s in sorted(stats, key=lambda s: -s["ms"]):
         log(f"  {s['name']:<14} {s['ms'] / a.steps:9.2f} ms  ({s['launches'] // max(1, a.steps)} regions/step)")
+    log("mapping work per step (both strand passes): " + ", ".join(f"{k[4:]} {v}" for k, v in work.items()))
     roof = dominant_roofline(stats, a.steps)
     full = None
     if not a.no_full_block:
@@ -212,7 +215,7 @@ def main():
         ctx.device_sync()
         full_ms = (time.perf_counter() - t) * 1e3
         ctx.set_timing(False)
-        fst = ctx.stats()
+        fst = [s for s in ctx.stats() if s["ms"] > 0]
         full = {"ms": round(full_ms, 3), "stages_ms": {s["name"]: round(s["ms"], 3) for s in fst},
                 "pg_bases": {p: int(len(b.get(p + "_pg"))) for p in ("hq", "lq", "nlq")},
                 "pg_bases_after_refmatch": {p: int(len(b.get(f"rm_{p}_text"))) for p in ("hq", "lq", "nlq")}}
@@ -253,6 +256,7 @@ def main():
                                "subst 0.1%->1.5%, N 0.02%, one block",
                    "reads": a.reads, "read_len": L, "genome_bp": a.genome},
         "stages_ms": {s["name"]: round(s["ms"] / a.steps, 3) for s in stats},
+        "mapping_work": work,
         "roofline": roof,
         "cpu_baseline": cpu,
         "bits_per_base": bpb,

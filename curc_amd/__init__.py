@@ -108,6 +108,13 @@ def lib():
         "curc_pack": (i32, [vp, vp, u64, u32, vp, vp]),
         "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
         "curc_batch_run": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(i32)]),
+        "curc_mlp_create": (vp, [vp]),
+        "curc_mlp_destroy": (None, [vp]),
+        "curc_mlp_set_params": (i32, [vp, vp, u32]),
+        "curc_mlp_get_params": (i32, [vp, vp, u32]),
+        "curc_mlp_forward": (i32, [vp, vp, u64, vp]),
+        "curc_mlp_gradients": (i32, [vp, vp, vp, u64, vp]),
+        "curc_mlp_train_step": (i32, [vp, vp, vp, u64, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
         "curc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "curc_block_free": (None, [vp]),
         "curc_ctx_set_timing": (i32, [vp, i32]),

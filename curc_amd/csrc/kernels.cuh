@@ -732,7 +732,10 @@ struct MapArgs {
     uint64_t n_N;
     uint8_t* last;
     uint64_t* word;            // record word per mapping slot
-    unsigned long long* work;  // [active reads, k-mer probes, verified candidates]
+    // [0] active reads, [1] k-mer probes, [2] verified candidates,
+    // [3] reads settled (<= 1 mismatch) at their first probe, [4] probes and
+    // [5] candidates of reads left above 1 mismatch by this pass
+    unsigned long long* work;
 };
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
@@ -761,11 +764,12 @@ struct SeenCache {
 // a strictly better mismatch count replaces the record, stop at <= 1.
 template <bool RC>
 __global__ void __launch_bounds__(256) k_map(MapArgs a) {
-    uint32_t n_act = 0, n_probe = 0, n_cand = 0;
+    uint32_t n_act = 0, n_probe = 0, n_cand = 0, n_first = 0, n_open_probe = 0, n_open_cand = 0;
     GRID_STRIDE(i, a.n_un + a.n_N) {
         uint32_t last = a.last[i];
         if (last <= MAP_TARGET) continue;
         ++n_act;
+        const uint32_t probe0 = n_probe, cand0 = n_cand;
         const bool isN = i >= a.n_un;
         uint64_t w = a.word[i];
         SeenCache seen;
@@ -829,10 +833,19 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     done:
         a.last[i] = (uint8_t)last;
         a.word[i] = w;
+        if (last <= MAP_TARGET) {
+            n_first += (n_probe - probe0 == 1) ? 1u : 0u;
+        } else {
+            n_open_probe += n_probe - probe0;
+            n_open_cand += n_cand - cand0;
+        }
     }
     wave_add_u64(&a.work[0], n_act);
     wave_add_u64(&a.work[1], n_probe);
     wave_add_u64(&a.work[2], n_cand);
+    wave_add_u64(&a.work[3], n_first);
+    wave_add_u64(&a.work[4], n_open_probe);
+    wave_add_u64(&a.work[5], n_open_cand);
 }
 
 __global__ void k_map_init(uint8_t* __restrict__ last, uint64_t* __restrict__ word, uint64_t n_un, uint64_t n,

@@ -17,6 +17,7 @@
 #include "../../include/curc_gpu.h"
 #include "assembly.cuh"
 #include "kernels.cuh"
+#include "mlp.cuh"
 #include "refmatch.cuh"
 #include "streams.cuh"
 
@@ -1106,8 +1107,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                                                (const uint32_t*)krf, (const uint32_t*)kr, pos);
                         c->add_bytes("map_index", (double)count * (16 + 8) + 8.0 * H, (double)count);
                     }
-                    unsigned long long* work = S.as<unsigned long long>(S_MWORK, 4);
-                    fill<unsigned long long>(c, work, 4, 0ull);
+                    unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
+                    fill<unsigned long long>(c, work, 8, 0ull);
                     {
                         Region r(c, "mapping");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
@@ -1122,12 +1123,16 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         // state write (1 + 8 B), 16 B per k-mer probe (two CSR offsets and
                         // the hash of a register-resident k-mer) and, per verified
                         // candidate, its 4 B position plus the P bytes of pseudogenome
-                        unsigned long long wk[4];
+                        unsigned long long wk[8];
                         CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
                         CURC_HIP(hipStreamSynchronize(st));
                         const double P8 = 8.0 * W;
                         c->add_bytes("mapping", (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
                                      (double)wk[2]);
+                        // work profile of the pass (stats entries named map_*)
+                        const char* names[6] = {"map_active", "map_probes", "map_cands", "map_first_probe",
+                                                "map_open_probes", "map_open_cands"};
+                        for (int q = 0; q < 6; ++q) c->add_bytes(names[q], 0.0, (double)wk[q]);
                     }
                 }
             }
@@ -1646,3 +1651,182 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ scorer net ---
+struct curc_mlp {
+    curc_ctx* c = nullptr;
+    MlpDev d{};
+    uint64_t t = 0;  // Adam steps taken
+    // training intermediates, grown on demand
+    uint64_t cap = 0;
+    uint16_t *xt = nullptr, *h1t = nullptr, *dz2t = nullptr, *dz1t = nullptr;
+    float *h2 = nullptr, *ds = nullptr, *logit = nullptr, *lsum = nullptr;
+    void grow(uint64_t n) {
+        if (n <= cap) return;
+        free_tmp();
+        cap = n + n / 8 + 64;
+        CURC_HIP(hipMalloc(&xt, cap * MLP_IN * 2));
+        CURC_HIP(hipMalloc(&h1t, cap * MLP_H * 2));
+        CURC_HIP(hipMalloc(&dz2t, cap * MLP_H * 2));
+        CURC_HIP(hipMalloc(&dz1t, cap * MLP_H * 2));
+        CURC_HIP(hipMalloc(&h2, cap * MLP_H * 4));
+        CURC_HIP(hipMalloc(&ds, cap * 4));
+        CURC_HIP(hipMalloc(&logit, cap * 4));
+    }
+    void free_tmp() {
+        for (void* q : {(void*)xt, (void*)h1t, (void*)dz2t, (void*)dz1t, (void*)h2, (void*)ds, (void*)logit})
+            if (q) hipFree(q);
+        xt = h1t = dz2t = dz1t = nullptr;
+        h2 = ds = logit = nullptr;
+        cap = 0;
+    }
+};
+
+namespace {
+
+uint32_t mlp_grid(uint64_t n) {
+    const uint64_t tiles = (n + MLP_TILE - 1) / MLP_TILE;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((tiles + 3) / 4, 1024));
+}
+
+// Forward, loss gradient and all parameter gradients of the mean
+// cross-entropy over n samples, accumulated into d.g.
+void mlp_backprop(curc_mlp* m, const float* x, const float* y, uint64_t n) {
+    hipStream_t st = m->c->st;
+    m->grow(n);
+    const uint32_t g = mlp_grid(n);
+    hipLaunchKernelGGL(k_mlp_train_fwd, dim3(g), dim3(256), 0, st, m->d, x, y, n, 1.0f / (float)n, m->xt, m->h1t,
+                       m->h2, m->ds, m->logit);
+    hipLaunchKernelGGL(k_mlp_train_bwd, dim3(g), dim3(256), 0, st, m->d, n, (const float*)m->h2,
+                       (const float*)m->ds, (const uint16_t*)m->h1t, m->dz2t, m->dz1t);
+    const uint32_t parts = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, n / 512));
+    hipLaunchKernelGGL(k_mlp_train_wgrad, dim3(20, parts), dim3(64), 0, st, m->d, n, (const uint16_t*)m->xt,
+                       (const uint16_t*)m->h1t, (const uint16_t*)m->dz2t, (const uint16_t*)m->dz1t, parts);
+    hipLaunchKernelGGL(k_mlp_train_b1, dim3(MLP_H), dim3(256), 0, st, m->d, n, (const uint16_t*)m->dz1t);
+}
+
+}  // namespace
+
+extern "C" {
+
+curc_mlp* curc_mlp_create(curc_ctx* c) {
+    if (!c) return nullptr;
+    curc_mlp* m = new curc_mlp();
+    m->c = c;
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        for (float** q : {&m->d.p, &m->d.m, &m->d.v, &m->d.g}) CURC_HIP(hipMalloc(q, MLP_NPARAMS * 4));
+        CURC_HIP(hipMalloc(&m->d.w1t, MLP_IN * MLP_H * 2));
+        CURC_HIP(hipMalloc(&m->d.w2t, MLP_H * MLP_H * 2));
+        CURC_HIP(hipMalloc(&m->d.w2, MLP_H * MLP_H * 2));
+        CURC_HIP(hipMalloc(&m->lsum, 4));
+        for (float* q : {m->d.p, m->d.m, m->d.v, m->d.g})
+            hipLaunchKernelGGL(k_fill<float>, dim3(32), dim3(256), 0, c->st, q, (uint64_t)MLP_NPARAMS, 0.f);
+        hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, c->st, m->d);
+        CURC_HIP(hipStreamSynchronize(c->st));
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        curc_mlp_destroy(m);
+        return nullptr;
+    }
+    return m;
+}
+
+void curc_mlp_destroy(curc_mlp* m) {
+    if (!m) return;
+    hipStreamSynchronize(m->c->st);
+    m->free_tmp();
+    for (void* q : {(void*)m->d.p, (void*)m->d.m, (void*)m->d.v, (void*)m->d.g, (void*)m->d.w1t, (void*)m->d.w2t,
+                    (void*)m->d.w2, (void*)m->lsum})
+        if (q) hipFree(q);
+    delete m;
+}
+
+int curc_mlp_set_params(curc_mlp* m, const float* params, uint32_t n) {
+    if (!m || !params || n != MLP_NPARAMS) return CURC_EINVAL;
+    try {
+        hipStream_t st = m->c->st;
+        CURC_HIP(hipMemcpyAsync(m->d.p, params, MLP_NPARAMS * 4, hipMemcpyHostToDevice, st));
+        for (float* q : {m->d.m, m->d.v, m->d.g})
+            hipLaunchKernelGGL(k_fill<float>, dim3(32), dim3(256), 0, st, q, (uint64_t)MLP_NPARAMS, 0.f);
+        hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, st, m->d);
+        CURC_HIP(hipStreamSynchronize(st));
+        m->t = 0;
+    } catch (const std::exception& e) {
+        return fail(m->c, CURC_EHIP, e.what());
+    }
+    return CURC_OK;
+}
+
+int curc_mlp_get_params(curc_mlp* m, float* params, uint32_t n) {
+    if (!m || !params || n != MLP_NPARAMS) return CURC_EINVAL;
+    try {
+        CURC_HIP(hipMemcpyAsync(params, m->d.p, MLP_NPARAMS * 4, hipMemcpyDeviceToHost, m->c->st));
+        CURC_HIP(hipStreamSynchronize(m->c->st));
+    } catch (const std::exception& e) {
+        return fail(m->c, CURC_EHIP, e.what());
+    }
+    return CURC_OK;
+}
+
+int curc_mlp_forward(curc_mlp* m, const float* d_x, uint64_t n, float* d_logit) {
+    if (!m || (n && (!d_x || !d_logit))) return CURC_EINVAL;
+    if (n == 0) return CURC_OK;
+    try {
+        hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(n)), dim3(256), 0, m->c->st, m->d, d_x, n, d_logit);
+        CURC_HIP(hipGetLastError());
+        CURC_HIP(hipStreamSynchronize(m->c->st));
+    } catch (const std::exception& e) {
+        return fail(m->c, CURC_EHIP, e.what());
+    }
+    return CURC_OK;
+}
+
+int curc_mlp_gradients(curc_mlp* m, const float* d_x, const float* d_y, uint64_t n, float* grads) {
+    if (!m || !d_x || !d_y || !grads || n == 0) return CURC_EINVAL;
+    try {
+        hipStream_t st = m->c->st;
+        hipLaunchKernelGGL(k_fill<float>, dim3(32), dim3(256), 0, st, m->d.g, (uint64_t)MLP_NPARAMS, 0.f);
+        mlp_backprop(m, d_x, d_y, n);
+        CURC_HIP(hipMemcpyAsync(grads, m->d.g, MLP_NPARAMS * 4, hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_fill<float>, dim3(32), dim3(256), 0, st, m->d.g, (uint64_t)MLP_NPARAMS, 0.f);
+        CURC_HIP(hipGetLastError());
+        CURC_HIP(hipStreamSynchronize(st));
+    } catch (const std::exception& e) {
+        return fail(m->c, CURC_EHIP, e.what());
+    }
+    return CURC_OK;
+}
+
+int curc_mlp_train_step(curc_mlp* m, const float* d_x, const float* d_y, uint64_t n, float lr, float* loss) {
+    if (!m || !d_x || !d_y || n == 0) return CURC_EINVAL;
+    try {
+        hipStream_t st = m->c->st;
+        mlp_backprop(m, d_x, d_y, n);
+        if (loss) {
+            hipLaunchKernelGGL(k_fill<float>, dim3(1), dim3(64), 0, st, m->lsum, (uint64_t)1, 0.f);
+            hipLaunchKernelGGL(k_bce_sum, dim3((uint32_t)std::min<uint64_t>(1024, (n + 255) / 256)), dim3(256), 0,
+                               st, (const float*)m->logit, d_y, n, m->lsum);
+        }
+        m->t += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)m->t)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)m->t)));
+        hipLaunchKernelGGL(k_mlp_adam, dim3(32), dim3(256), 0, st, m->d, lr, b1, b2, 1e-8f, c1, c2);
+        hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, st, m->d);
+        CURC_HIP(hipGetLastError());
+        if (loss) {
+            float sum = 0;
+            CURC_HIP(hipMemcpyAsync(&sum, m->lsum, 4, hipMemcpyDeviceToHost, st));
+            CURC_HIP(hipStreamSynchronize(st));
+            *loss = sum / (float)n;
+        }
+    } catch (const std::exception& e) {
+        return fail(m->c, CURC_EHIP, e.what());
+    }
+    return CURC_OK;
+}
+
+}  // extern "C"
+
+static_assert(CURC_MLP_NPARAMS == curc::MLP_NPARAMS, "scorer parameter count");

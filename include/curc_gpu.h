@@ -224,6 +224,27 @@ typedef struct curc_job {
 int curc_batch_run(curc_ctx *ctx, const curc_job *jobs, uint32_t n_jobs, uint32_t workers, curc_block **out,
                    int *status);
 
+/* ---- learned candidate scorer ----------------------------------------------
+ * No counterpart in the reference.  A 32 -> 128 -> 128 -> 1 network (ReLU,
+ * bfloat16 matrix products on the MFMA units, float32 accumulation and
+ * master weights) trained with the library's own forward / backward / Adam
+ * kernels on mapping outcomes that exact verification produced.  Parameters
+ * are float32 in the order W1[32][128], b1[128], W2[128][128], b2[128],
+ * w3[128], b3 (CURC_MLP_NPARAMS values).  Sample features are float32
+ * [n][32]; labels float32 in {0, 1}; x / y / logit are device pointers. */
+#define CURC_MLP_NPARAMS 20865u
+typedef struct curc_mlp curc_mlp;
+curc_mlp *curc_mlp_create(curc_ctx *ctx);
+void curc_mlp_destroy(curc_mlp *m);
+int curc_mlp_set_params(curc_mlp *m, const float *params, uint32_t n);  /* also resets Adam */
+int curc_mlp_get_params(curc_mlp *m, float *params, uint32_t n);
+int curc_mlp_forward(curc_mlp *m, const float *d_x, uint64_t n, float *d_logit);
+/* gradients of the mean binary cross-entropy of sigmoid(logit), no update */
+int curc_mlp_gradients(curc_mlp *m, const float *d_x, const float *d_y, uint64_t n, float *grads);
+/* one Adam step (lr; beta 0.9 / 0.999, eps 1e-8) on the batch; *loss, when
+ * given, receives the batch's mean loss before the step */
+int curc_mlp_train_step(curc_mlp *m, const float *d_x, const float *d_y, uint64_t n, float lr, float *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,3 +99,14 @@ def test_synth_paired_end_fragments():
         fwd = [x for x in (a, b) if x in gs]
         rev = [x for x in (a, b) if x[::-1].translate(comp) in gs]
         assert len(fwd) >= 1 and len(rev) >= 1
+
+
+def test_scorer_parameter_layout_matches_reference():
+    import re
+
+    from curc_amd import scorer
+    from oracle import mlp
+
+    hdr = open(os.path.join(ROOT, "include", "curc_gpu.h")).read()
+    n = int(re.search(r"#define CURC_MLP_NPARAMS (\d+)u", hdr).group(1))
+    assert n == scorer.N_PARAMS == mlp.N_PARAMS == scorer.init_params(0).size
