@@ -22,6 +22,8 @@ RUN_STREAMS = 4
 RUN_PRESERVE_ORDER = 8
 RUN_PAIRED = 16
 RUN_REFMATCH = 32
+RUN_TRACE = 64
+RUN_LEARNED = 128
 MIS_OFF_BASE = 64
 RM_BASE = 1000
 RM_PARTS = {"hq": 0, "lq": 1, "nlq": 2}
@@ -43,6 +45,7 @@ FIELDS = {
     "s_mis_base": (43, np.uint8), "s_mis_table": (44, np.uint8), "s_lq_off": (45, np.uint16),
     "s_nlq_off": (46, np.uint16), "s_pe_order": (47, np.uint32), "s_id_pos": (48, np.uint64),
     "s_sorted_rec": (49, "rec"),
+    "trace_x": (60, np.dtype((np.float32, 32))), "trace_y": (61, np.float32),
 }
 
 # curc_alignment: u32 read_id, u32 pad, u64 word (isN | strand << 1 | pos << 2)
@@ -108,6 +111,8 @@ def lib():
         "curc_pack": (i32, [vp, vp, u64, u32, vp, vp]),
         "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
         "curc_batch_run": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(i32)]),
+        "curc_ctx_set_scorer": (i32, [vp, vp, ctypes.c_float]),
+        "curc_ctx_set_trace": (i32, [vp, u32, u64]),
         "curc_mlp_create": (vp, [vp]),
         "curc_mlp_destroy": (None, [vp]),
         "curc_mlp_set_params": (i32, [vp, vp, u32]),
@@ -250,12 +255,14 @@ class Context:
             raise CurcError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
     def run_block(self, reads, L=None, stop_after=STAGE_ALL, keep_stages=False, device_ptr=None, n_reads=None,
-                  streams=False, preserve_order=False, paired=False, refmatch=False):
+                  streams=False, preserve_order=False, paired=False, refmatch=False, trace=False, learned=False):
         """Runs the GPU part of block_compress on reads ((n, L) uint8 host
         array) or on a device buffer (device_ptr, n_reads, L); with streams,
         also the record sort and stream builders; with refmatch, the
-        pseudogenome self-match."""
+        pseudogenome self-match; with trace, scorer training records; with
+        learned, mapping under the context's scorer (set_scorer)."""
         flags = RUN_KEEP_STAGES if keep_stages else 0
+        flags |= (RUN_TRACE if trace else 0) | (RUN_LEARNED if learned else 0)
         if streams:
             flags |= RUN_STREAMS | (RUN_PRESERVE_ORDER if preserve_order else 0) | (RUN_PAIRED if paired else 0)
         if refmatch:
@@ -297,6 +304,16 @@ class Context:
                     b.free()
             self._check(rc, "curc_batch_run")
         return blocks
+
+    def set_scorer(self, mlp, threshold=0.0):
+        """Scorer network (curc_amd.scorer.Mlp, kept alive by the caller) for
+        runs with learned=True; None disables."""
+        self._scorer = mlp
+        self._check(lib().curc_ctx_set_scorer(self._h, mlp._h if mlp is not None else None,
+                                              ctypes.c_float(threshold)), "curc_ctx_set_scorer")
+
+    def set_trace(self, every=16, max_records=1 << 22):
+        self._check(lib().curc_ctx_set_trace(self._h, every, max_records), "curc_ctx_set_trace")
 
     def dev_alloc(self, nbytes):
         p = ctypes.c_void_p()

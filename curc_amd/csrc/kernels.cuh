@@ -736,6 +736,7 @@ struct MapArgs {
     // [3] reads settled (<= 1 mismatch) at their first probe, [4] probes and
     // [5] candidates of reads left above 1 mismatch by this pass
     unsigned long long* work;
+    uint32_t max_mm = 0;       // floor(L / 5) (used by the probe-range kernels)
 };
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {

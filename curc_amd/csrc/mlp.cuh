@@ -51,6 +51,11 @@ __device__ __forceinline__ uint32_t acc_row(uint32_t r, uint32_t lane) {
     return 8u * (r >> 2) + 4u * (lane >> 5) + (r & 3u);
 }
 
+inline uint32_t mlp_grid(uint64_t n) {
+    const uint64_t tiles = (n + MLP_TILE - 1) / MLP_TILE;
+    return (uint32_t)(tiles < 4 ? 1 : ((tiles + 3) / 4 > 1024 ? 1024 : (tiles + 3) / 4));
+}
+
 // Device copies of one network: float32 master parameters, Adam moments,
 // gradient accumulator, and bfloat16 weight copies in both orders.
 struct MlpDev {
@@ -170,9 +175,10 @@ __device__ __forceinline__ void mlp_stage_weights(const MlpDev& net, MlpLds& L) 
 
 // Inference over n samples (x: float32 [n][32]); writes logits.  Blocks of
 // 4 waves, one 32-sample tile per wave per step.
-__global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __restrict__ x, uint64_t n,
-                                                   float* __restrict__ logit) {
+__global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __restrict__ x, uint64_t n_host,
+                                                   const uint32_t* __restrict__ d_n, float* __restrict__ logit) {
     __shared__ MlpLds L;
+    const uint64_t n = d_n ? (uint64_t)*d_n : n_host;
     mlp_stage_weights(net, L);
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();

@@ -17,6 +17,7 @@
 #include "../../include/curc_gpu.h"
 #include "assembly.cuh"
 #include "kernels.cuh"
+#include "mapphase.cuh"
 #include "mlp.cuh"
 #include "refmatch.cuh"
 #include "streams.cuh"
@@ -45,6 +46,7 @@ enum Slot {
     S_RM_FLAG, S_RM_FS, S_RM_FL, S_RM_FD, S_RM_SK0, S_RM_SK1, S_RM_SV0, S_RM_SV1, S_RM_US, S_RM_UL, S_RM_UD,
     S_RM_T, S_RM_TMAX, S_RM_J0, S_RM_J1, S_RM_MARK, S_RM_AS, S_RM_AL, S_RM_AD, S_RM_ASTART, S_RM_AEND,
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
+    S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
     S_LAST_NAMED,
     S_PRIM = 176  // primitives use slots 176.. (scan recursion, sort hist, compaction)
 };
@@ -54,7 +56,8 @@ static_assert(S_LAST_NAMED <= S_PRIM, "scratch slots overlap");
 
 enum Cnt {
     C_N, C_NONN, C_NN, C_NPRE, C_NSUF, C_NS0, C_NS1, C_RESN, C_NP0, C_NP1,
-    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_RM, C_COUNT = 32
+    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_RM, C_MA0, C_MA1,
+    C_COUNT = 32
 };
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap = 8192) {
@@ -101,6 +104,14 @@ struct curc_ctx {
     std::string err;
     bool timing = false;
     std::vector<curc_ctx*> workers;  // extra stream/arena pairs of curc_batch_run
+    // learned mapping (CURC_RUN_LEARNED): the scorer's device arrays
+    bool has_scorer = false;
+    MlpDev scorer_net{};
+    float scorer_threshold = 0.f;
+    const MlpDev& scorer_dev() const { return scorer_net; }
+    // traced mapping (CURC_RUN_TRACE): sampling of reads and record capacity
+    uint32_t trace_every = 16;
+    uint64_t trace_cap = 1u << 22;
     std::vector<StageStat> stats;
     std::vector<PendingTimer> pending;
     std::vector<hipEvent_t> event_pool;
@@ -228,6 +239,8 @@ struct curc_block {
     std::vector<uint32_t> s_pe_order;
     std::vector<uint64_t> s_id_pos, s_mis_off_start;
     std::vector<curc_alignment> s_sorted;
+    // traced mapping (CURC_RUN_TRACE): scorer features and labels
+    std::vector<float> trace_x, trace_y;
     // pseudogenome self-match (CURC_RUN_REFMATCH): parts HQ, LQ, N-read LQ
     struct RmPart {
         std::string text, map_off, map_len;
@@ -670,6 +683,74 @@ void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_
     CURC_HIP(hipStreamSynchronize(st));
 }
 
+// ------------------------------------------------- mapping in probe ranges ---
+// Probe counts after which the scorer's checkpoints run.
+constexpr uint32_t MAP_CHECKPOINTS[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48};
+
+// One strand pass (one index part) of the mapping as probe ranges with the
+// loop state in HBM.  Learned: at each checkpoint the network scores every
+// active read and reads below the threshold end the pass.  Trace: sampled
+// reads' features are recorded at each checkpoint and labelled at the end
+// of the pass; no read is stopped, so the results equal k_map's.
+void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* cnt, bool learned, bool trace) {
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    if (nm == 0) return;
+    uint32_t* act[2] = {S.as<uint32_t>(S_MACT0, nm + 1), S.as<uint32_t>(S_MACT1, nm + 1)};
+    uint32_t* nact[2] = {cnt + C_MA0, cnt + C_MA1};
+    set_cnt(c, cnt + C_TMP, (uint32_t)nm);
+    compact(cnt + C_TMP, PredNeedsMap{a.last}, EmitIota{act[0]}, nact[0], S, S_PRIM, st, nm);
+    MapState ms{S.as<uint64_t>(S_MSEEN, 4 * nm), S.as<uint32_t>(S_MCNT, 4 * nm), S.as<uint8_t>(S_MSTOP, nm), nm};
+    const uint32_t g = grid_for(nm, 1u << 20);
+    hipLaunchKernelGGL(k_map_state_reset, dim3(g), dim3(256), 0, st, ms, (const uint32_t*)act[0],
+                       (const uint32_t*)nact[0]);
+    float* X = (learned || trace) ? S.as<float>(S_MX, nm * MAP_NFEAT) : nullptr;
+    float* logit = learned ? S.as<float>(S_MLOGIT, nm) : nullptr;
+    unsigned long long* tcount = trace ? S.as<unsigned long long>(S_TRCNT, 1) : nullptr;
+    uint64_t r0 = 0;
+    if (trace) r0 = read_one(c, tcount);
+    const uint32_t nprobe = (a.L - MAP_K) / MAP_READ_STEP + 1;
+    int cur = 0;
+    uint32_t t0 = 0;
+    for (uint32_t ci = 0;; ++ci) {
+        const uint32_t t1 = ci < sizeof(MAP_CHECKPOINTS) / 4 ? std::min(MAP_CHECKPOINTS[ci], nprobe) : nprobe;
+        if (t1 > t0) {
+            if (rc)
+                hipLaunchKernelGGL(k_map_range<true>, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
+                                   (const uint32_t*)nact[cur], t0, t1);
+            else
+                hipLaunchKernelGGL(k_map_range<false>, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
+                                   (const uint32_t*)nact[cur], t0, t1);
+        }
+        if (t1 >= nprobe) break;
+        compact(nact[cur], PredMapActive{act[cur], a.last, ms.stop}, EmitGather{act[cur], act[1 - cur]},
+                nact[1 - cur], S, S_PRIM, st, nm);
+        cur = 1 - cur;
+        hipLaunchKernelGGL(k_map_features, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
+                           (const uint32_t*)nact[cur], t1, t0, rc, X);
+        if (trace)
+            hipLaunchKernelGGL(k_map_trace_emit, dim3(g), dim3(256), 0, st, (const uint32_t*)act[cur],
+                               (const uint32_t*)nact[cur], (const float*)X, (const uint8_t*)a.last, c->trace_every,
+                               S.as<float>(S_TRX, c->trace_cap * MAP_NFEAT), S.as<uint8_t>(S_TRLAST, c->trace_cap),
+                               S.as<uint32_t>(S_TRSLOT, c->trace_cap), tcount, c->trace_cap);
+        if (learned) {
+            hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(nm)), dim3(256), 0, st, c->scorer_dev(), (const float*)X,
+                               nm, (const uint32_t*)nact[cur], logit);
+            hipLaunchKernelGGL(k_map_stop, dim3(g), dim3(256), 0, st, ms, (const uint32_t*)act[cur],
+                               (const uint32_t*)nact[cur], (const float*)logit, c->scorer_threshold);
+        }
+        t0 = t1;
+    }
+    if (trace) {
+        const uint64_t r1 = std::min<uint64_t>(read_one(c, tcount), c->trace_cap);
+        if (r1 > r0)
+            hipLaunchKernelGGL(k_map_trace_label, dim3(grid_for(r1 - r0)), dim3(256), 0, st,
+                               (const uint8_t*)S.as<uint8_t>(S_TRLAST, c->trace_cap),
+                               (const uint32_t*)S.as<uint32_t>(S_TRSLOT, c->trace_cap), (const uint8_t*)a.last,
+                               std::min(r0, c->trace_cap), r1, S.as<float>(S_TRY, c->trace_cap));
+    }
+}
+
 void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOut& lq_asm, const AsmOut& nlq_asm,
                    uint32_t* cnt, bool preserve, bool paired) {
     Scratch& S = c->S;
@@ -1057,6 +1138,13 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
 
     // ---- a10: mapping (compress.cu:1620-1640, 268-595)
     {
+        const bool learned = (flags & CURC_RUN_LEARNED) && c->has_scorer;
+        const bool trace = (flags & CURC_RUN_TRACE) != 0;
+        const bool phased = learned || trace;
+        if (trace) {
+            S.as<unsigned long long>(S_TRCNT, 1);
+            fill<unsigned long long>(c, S.as<unsigned long long>(S_TRCNT, 1), 1, 0ull);
+        }
         const uint64_t ref_len = B->hq_len;
         const uint64_t nm = (uint64_t)n_unm + nN;
         uint8_t* last = S.as<uint8_t>(S_LAST, nm + 1);
@@ -1109,7 +1197,12 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     }
                     unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
                     fill<unsigned long long>(c, work, 8, 0ull);
-                    {
+                    if (phased) {
+                        Region r(c, learned ? "mapping_learned" : "mapping_traced");
+                        MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
+                                  Nascii, nN, last, word, work, P.max_mm};
+                        map_phased(c, a, rc, nm, cnt, learned, trace);
+                    } else {
                         Region r(c, "mapping");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
                                   Nascii, nN, last, word, work};
@@ -1127,7 +1220,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
                         CURC_HIP(hipStreamSynchronize(st));
                         const double P8 = 8.0 * W;
-                        c->add_bytes("mapping", (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
+                        c->add_bytes(phased ? (learned ? "mapping_learned" : "mapping_traced") : "mapping",
+                                     (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
                                      (double)wk[2]);
                         // work profile of the pass (stats entries named map_*)
                         const char* names[6] = {"map_active", "map_probes", "map_cands", "map_first_probe",
@@ -1163,6 +1257,12 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->unmapped, um, nun);
         d2h(c, B->unmapped_N, umN, nunN);
         CURC_HIP(hipStreamSynchronize(st));
+        if (trace) {
+            const uint64_t nr = std::min<uint64_t>(read_one(c, S.as<unsigned long long>(S_TRCNT, 1)), c->trace_cap);
+            d2h(c, B->trace_x, (const float*)S.as<float>(S_TRX, c->trace_cap * MAP_NFEAT), nr * MAP_NFEAT);
+            d2h(c, B->trace_y, (const float*)S.as<float>(S_TRY, c->trace_cap), nr);
+            CURC_HIP(hipStreamSynchronize(st));
+        }
         B->map_rec.resize(nmap);
         for (uint32_t i = 0; i < nmap; ++i) {
             B->map_rec[i].read_id = hid[i];
@@ -1532,6 +1632,8 @@ uint64_t curc_block_get(curc_block* B, int field, const void** ptr) {
         case CURC_F_S_PE_ORDER: return put(B->s_pe_order);
         case CURC_F_S_ID_POS: return put(B->s_id_pos);
         case CURC_F_S_SORTED_REC: return put(B->s_sorted);
+        case CURC_F_TRACE_X: *ptr = B->trace_x.data(); return B->trace_x.size() / MAP_NFEAT;
+        case CURC_F_TRACE_Y: return put(B->trace_y);
         default:
             if (field >= CURC_F_RM_BASE && field < CURC_F_RM_BASE + 15) {
                 const int p = (field - CURC_F_RM_BASE) / 5, k = (field - CURC_F_RM_BASE) % 5;
@@ -1684,11 +1786,6 @@ struct curc_mlp {
 
 namespace {
 
-uint32_t mlp_grid(uint64_t n) {
-    const uint64_t tiles = (n + MLP_TILE - 1) / MLP_TILE;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((tiles + 3) / 4, 1024));
-}
-
 // Forward, loss gradient and all parameter gradients of the mean
 // cross-entropy over n samples, accumulated into d.g.
 void mlp_backprop(curc_mlp* m, const float* x, const float* y, uint64_t n) {
@@ -1773,7 +1870,8 @@ int curc_mlp_forward(curc_mlp* m, const float* d_x, uint64_t n, float* d_logit) 
     if (!m || (n && (!d_x || !d_logit))) return CURC_EINVAL;
     if (n == 0) return CURC_OK;
     try {
-        hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(n)), dim3(256), 0, m->c->st, m->d, d_x, n, d_logit);
+        hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(n)), dim3(256), 0, m->c->st, m->d, d_x, n,
+                           (const uint32_t*)nullptr, d_logit);
         CURC_HIP(hipGetLastError());
         CURC_HIP(hipStreamSynchronize(m->c->st));
     } catch (const std::exception& e) {
@@ -1830,3 +1928,22 @@ int curc_mlp_train_step(curc_mlp* m, const float* d_x, const float* d_y, uint64_
 }  // extern "C"
 
 static_assert(CURC_MLP_NPARAMS == curc::MLP_NPARAMS, "scorer parameter count");
+
+extern "C" {
+
+int curc_ctx_set_scorer(curc_ctx* c, curc_mlp* m, float threshold) {
+    if (!c) return CURC_EINVAL;
+    c->has_scorer = m != nullptr;
+    if (m) c->scorer_net = m->d;
+    c->scorer_threshold = threshold;
+    return CURC_OK;
+}
+
+int curc_ctx_set_trace(curc_ctx* c, uint32_t every, uint64_t max_records) {
+    if (!c || every == 0 || max_records == 0) return CURC_EINVAL;
+    c->trace_every = every;
+    c->trace_cap = max_records;
+    return CURC_OK;
+}
+
+}  // extern "C"

@@ -137,6 +137,9 @@ enum {
     CURC_F_S_PE_ORDER = 47,    /* u32 global ids in emission order (paired, order not preserved) */
     CURC_F_S_ID_POS = 48,      /* u64 joined-pseudogenome position per read (order preserved) */
     CURC_F_S_SORTED_REC = 49,  /* curc_alignment, HQ+map records after the record sort */
+    /* traced mapping (CURC_RUN_TRACE): one record per sampled read and checkpoint */
+    CURC_F_TRACE_X = 60,       /* float32 [n][32] scorer features (count: n records) */
+    CURC_F_TRACE_Y = 61,       /* float32 [n]: 1 if the rest of the strand pass lowered the best count */
     CURC_F_S_MIS_OFF = 64,     /* + m (1..max_mm): u16 mismatch-offset stream for m mismatches */
     /* pseudogenome self-match (CURC_RUN_REFMATCH; RefMatcher, compress.cu:597-1001, called at
      * :2053-2069).  Part p (0 = HQ text against itself, 1 = LQ text, 2 = N-read LQ text, each
@@ -156,6 +159,8 @@ enum {
 #define CURC_RUN_PRESERVE_ORDER 8u  /* streams for --preserve_order */
 #define CURC_RUN_PAIRED 16u         /* streams for paired-end input (ids 2k / 2k+1) */
 #define CURC_RUN_REFMATCH 32u       /* also self-match the pseudogenomes (CURC_F_RM_*) */
+#define CURC_RUN_TRACE 64u          /* record scorer features and labels while mapping (CURC_F_TRACE_*) */
+#define CURC_RUN_LEARNED 128u       /* map with the context's scorer (curc_ctx_set_scorer) */
 
 typedef struct curc_block curc_block;
 
@@ -244,6 +249,16 @@ int curc_mlp_gradients(curc_mlp *m, const float *d_x, const float *d_y, uint64_t
 /* one Adam step (lr; beta 0.9 / 0.999, eps 1e-8) on the batch; *loss, when
  * given, receives the batch's mean loss before the step */
 int curc_mlp_train_step(curc_mlp *m, const float *d_x, const float *d_y, uint64_t n, float lr, float *loss);
+/* Learned mapping: with CURC_RUN_LEARNED, the mapping stage checks each
+ * unsettled read after 1, 2, 3, 4, 6, 8, 12, 16, 24, 32 and 48 probes of a
+ * strand pass; a read whose logit is below `threshold` tries no further
+ * probes in that pass.  Records stay exact (verification decides); reads
+ * may map with more mismatches than the exhaustive loop finds, or not at
+ * all.  The network must outlive its use; NULL disables. */
+int curc_ctx_set_scorer(curc_ctx *ctx, curc_mlp *m, float threshold);
+/* Traced mapping (CURC_RUN_TRACE): reads whose slot is a multiple of
+ * `every` contribute one record per checkpoint, up to max_records. */
+int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);
 
 #ifdef __cplusplus
 }
