@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--no-full-block", action="store_true")
     p.add_argument("--no-bits-per-base", action="store_true")
     p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--no-learned", action="store_true")
     return p.parse_args()
 
 
@@ -62,19 +63,28 @@ def dominant_roofline(stats, steps):
                 avg_launch_ms=round(per_launch_ms, 4), alg_bytes_per_launch=per_launch_bytes)
 
 
-def bits_per_base():
+def bits_per_base(learned=False):
     """Archive size per base on the 1 Mbp / 133,333-read workload: GPU
     streams and pseudogenome self-match -> xz-coded .curc archive (written
-    to a temporary file)."""
+    to a temporary file); with learned, mapping under the trained scorer."""
     import tempfile
 
-    from curc_amd import archive
+    from curc_amd import archive, scorer, train_scorer
 
     cfg = ca.config_small()
     _, reads = ca.synth_reads(cfg)
     with ca.Context(0) as ctx:
-        blk = ctx.run_block(reads, streams=True, refmatch=True)
+        net = None
+        if learned:
+            params, thr = train_scorer.load()
+            net = scorer.Mlp(ctx, params)
+            ctx.set_scorer(net, thr)
+        blk = ctx.run_block(reads, streams=True, refmatch=True, learned=learned)
         bs = archive.BlockStreams.from_block(blk, len(reads), reads.shape[1], False, False)
+        blk.free()
+        if net is not None:
+            ctx.set_scorer(None)
+            net.close()
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "c1.curc")
         archive.write_archive(p, [bs], False, False, reads.shape[1])
@@ -84,6 +94,47 @@ def bits_per_base():
     lossless = sorted(map(bytes, dec)) == sorted(map(bytes, reads))
     return dict(value=round(8.0 * size / reads.size, 5), archive_bytes=size, lossless=bool(lossless),
                 workload="1 Mbp uniform genome, 133,333 x 150 bp SE, 0.5% subst, 0.01% N, order not preserved")
+
+
+def learned_run(ctx, a, L, d_ascii, plain_ms, plain_stats):
+    """The same block with mapping under the trained scorer."""
+    from curc_amd import scorer, train_scorer
+
+    params, thr = train_scorer.load()
+    net = scorer.Mlp(ctx, params)
+    ctx.set_scorer(net, thr)
+
+    def outcome(b):
+        mis = b.get("map_mis").astype(np.int64)
+        return dict(mapped=int(len(mis)), mismatches=int(mis.sum()), lq_reads=int(len(b.get("lq_rec"))),
+                    nlq_reads=int(len(b.get("nlq_rec"))), lq_pg_bases=int(len(b.get("lq_pg"))))
+
+    b = ctx.run_block(None, L=L, device_ptr=d_ascii, n_reads=a.reads)
+    plain_out = outcome(b)
+    b.free()
+    b = ctx.run_block(None, L=L, device_ptr=d_ascii, n_reads=a.reads, learned=True)  # warm-up
+    b.free()
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    ctx.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        b = ctx.run_block(None, L=L, device_ptr=d_ascii, n_reads=a.reads, learned=True)
+        out = outcome(b)
+        b.free()
+    ctx.device_sync()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    ctx.set_timing(False)
+    st = {s["name"]: round(s["ms"] / a.steps, 3) for s in ctx.stats() if s["ms"] > 0}
+    ctx.set_scorer(None)
+    net.close()
+    pm = sum(s["ms"] for s in plain_stats if s["name"] == "mapping") / a.steps
+    log(f"learned mapping: {ms:.1f} ms per block (plain {plain_ms:.1f}); mapping {st.get('mapping_learned', 0):.1f} ms "
+        f"(plain {pm:.1f}); mapped {out['mapped']} (plain {plain_out['mapped']}), mismatches {out['mismatches']} "
+        f"(plain {plain_out['mismatches']})")
+    return dict(ms_per_step=round(ms, 3), gbases_s=round(a.reads * L / 1e9 / (ms / 1e3), 4),
+                speedup_vs_plain=round(plain_ms / ms, 4), mapping_ms=st.get("mapping_learned"),
+                mapping_ms_plain=round(pm, 3), threshold=thr, outcome=out, outcome_plain=plain_out)
 
 
 def batched_inputs(n_inputs=64, seed=404):
@@ -203,6 +254,9 @@ def main():
         log(f"  {s['name']:<14} {s['ms'] / a.steps:9.2f} ms  ({s['launches'] // max(1, a.steps)} regions/step)")
     log("mapping work per step (both strand passes): " + ", ".join(f"{k[4:]} {v}" for k, v in work.items()))
     roof = dominant_roofline(stats, a.steps)
+    learned = None
+    if not a.no_learned:
+        learned = learned_run(ctx, a, L, d_ascii, ms, stats)
     full = None
     if not a.no_full_block:
         # the same block once more with the stream builders and the
@@ -228,6 +282,12 @@ def main():
     if not a.no_bits_per_base:
         bpb = bits_per_base()
         log(f"bits/base on the 1 Mbp workload: {bpb['value']} (lossless round trip: {bpb['lossless']})")
+        if not a.no_learned:
+            bl = bits_per_base(learned=True)
+            bpb["learned"] = bl["value"]
+            bpb["learned_lossless"] = bl["lossless"]
+            bpb["learned_vs_plain"] = round(bl["value"] / bpb["value"], 5)
+            log(f"  with the learned scorer: {bl['value']} (lossless round trip: {bl['lossless']})")
     bat = None
     if not a.no_batch:
         log("many-input batch ...")
@@ -262,6 +322,7 @@ def main():
         "bits_per_base": bpb,
         "full_block": full,
         "batched": bat,
+        "learned": learned,
     }
     print(json.dumps(out))
 

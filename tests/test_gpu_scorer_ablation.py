@@ -1,0 +1,70 @@
+"""What the trained scorer gains over mapping without it, on a workload whose
+seed is disjoint from the training seeds: far fewer k-mer probes and
+verified candidates, nearly every read still mapped, and a lossless
+archive no more than 1% larger.  A network with random weights, stopping as
+often, loses more of the mappings' quality."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import curc_amd as ca
+from curc_amd import archive, scorer, train_scorer
+
+pytestmark = pytest.mark.gpu
+
+
+def run(ctx, reads, learned):
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    b = ctx.run_block(reads, learned=learned, streams=True, refmatch=True)
+    ctx.set_timing(False)
+    st = {s["name"]: s for s in ctx.stats()}
+    work = {k: st[k]["items"] for k in ("map_probes", "map_cands") if k in st}
+    mis = b.get("map_mis").astype(np.int64)
+    bs = archive.BlockStreams.from_block(b, len(reads), reads.shape[1], False, False)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "a.curc")
+        archive.write_archive(p, [bs], False, False, reads.shape[1])
+        size = os.path.getsize(p)
+        _, _, L, blocks = archive.read_archive(p)
+        dec = archive.decode_block(blocks[0], L, False, False)
+    assert sorted(map(bytes, dec)) == sorted(map(bytes, reads))
+    b.free()
+    return dict(probes=work["map_probes"], cands=work["map_cands"], mapped=len(mis), mis=int(mis.sum()),
+                size=size)
+
+
+def test_trained_scorer_gains(gpu_ctx):
+    cfg = ca.config_scale(seed=31, n_reads=600_000, genome_len=3_000_000)
+    _, reads = ca.synth_reads(cfg)
+    print("ablation: plain", flush=True)
+    plain = run(gpu_ctx, reads, False)
+    params, thr = train_scorer.load()
+    net = scorer.Mlp(gpu_ctx, params)
+    gpu_ctx.set_scorer(net, thr)
+    print("ablation: trained scorer", flush=True)
+    got = run(gpu_ctx, reads, True)
+    print(plain, got, flush=True)
+    assert got["probes"] < 0.6 * plain["probes"]
+    assert got["cands"] < 0.7 * plain["cands"]
+    assert got["mapped"] >= 0.99 * plain["mapped"]
+    assert got["size"] <= 1.01 * plain["size"]
+    # random weights, threshold set to stop at least as much probing
+    rnd = scorer.Mlp(gpu_ctx, scorer.init_params(11))
+    x = np.random.default_rng(0).random((4096, 32)).astype(np.float32)
+    lr = rnd.forward(x)
+    best = None
+    for q in (0.5, 0.7, 0.85):
+        gpu_ctx.set_scorer(rnd, float(np.quantile(lr, q)))
+        print(f"ablation: random network, quantile {q}", flush=True)
+        r = run(gpu_ctx, reads, True)
+        if r["probes"] <= got["probes"]:
+            best = r
+            break
+    gpu_ctx.set_scorer(None)
+    net.close()
+    rnd.close()
+    if best is not None:
+        assert got["mis"] - plain["mis"] < best["mis"] - plain["mis"]
