@@ -111,6 +111,7 @@ def lib():
         "curc_pack": (i32, [vp, vp, u64, u32, vp, vp]),
         "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
         "curc_batch_run": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(i32)]),
+        "curc_ref_match": (i32, [vp, vp, u64, vp, u64, i32, i32, u32, ctypes.POINTER(vp)]),
         "curc_ctx_set_scorer": (i32, [vp, vp, ctypes.c_float]),
         "curc_ctx_set_trace": (i32, [vp, u32, u64]),
         "curc_mlp_create": (vp, [vp]),
@@ -248,6 +249,7 @@ class Context:
         if not h:
             raise CurcError(f"curc_ctx_create({device}) failed: no usable HIP device")
         self._h = ctypes.c_void_p(h)
+        self._dependents = []  # objects holding device memory of this context (scorer networks)
 
     def _check(self, rc, what):
         if rc != CURC_OK:
@@ -305,6 +307,23 @@ class Context:
             self._check(rc, "curc_batch_run")
         return blocks
 
+    def ref_match(self, src, dest, dest_contain_n=False, dest_is_ref=False, keep_stages=False):
+        """RefMatcher of dest against src (bytes); returns dict(text,
+        map_off, map_len[, found, matches])."""
+        src, dest = bytes(src), bytes(dest)
+        out = ctypes.c_void_p()
+        rc = lib().curc_ref_match(self._h, src, len(src), dest, len(dest), int(dest_contain_n), int(dest_is_ref),
+                                  RUN_KEEP_STAGES if keep_stages else 0, ctypes.byref(out))
+        self._check(rc, "curc_ref_match")
+        b = Block(out.value, has_refmatch=True)
+        res = dict(text=b.get("rm_hq_text").tobytes(), map_off=b.get("rm_hq_off").tobytes(),
+                   map_len=b.get("rm_hq_len").tobytes())
+        if keep_stages:
+            res["found"] = b.get("rm_hq_found")
+            res["matches"] = b.get("rm_hq_matches")
+        b.free()
+        return res
+
     def set_scorer(self, mlp, threshold=0.0):
         """Scorer network (curc_amd.scorer.Mlp, kept alive by the caller) for
         runs with learned=True; None disables."""
@@ -356,6 +375,8 @@ class Context:
         lib().curc_ctx_trim(self._h)
 
     def close(self):
+        for d in list(getattr(self, "_dependents", [])):
+            d.close()
         if self._h is not None and self._h.value:
             lib().curc_ctx_destroy(self._h)
         self._h = None

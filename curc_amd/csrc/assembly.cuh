@@ -18,7 +18,8 @@
 
 namespace curc {
 
-struct JumpState {
+// 32-byte aligned so that the compiler's 16-byte loads of it are aligned.
+struct alignas(32) JumpState {
     uint32_t p;  // jump pointer (rank) or INV
     uint32_t c;  // nodes in the window
     uint32_t h;  // last node of the window (chain head once p == INV)

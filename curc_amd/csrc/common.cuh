@@ -23,15 +23,31 @@ __device__ __forceinline__ uint64_t murmur64(uint64_t x) {
     return x;
 }
 
+// One 8-byte load that the compiler keeps apart from loads of the
+// neighbouring word.  Two adjacent 8-byte loads are otherwise merged into a
+// single 16-byte load at an address that is only 8-byte aligned, and such
+// loads have returned wrong data at random on gfx950 (a kernel reading only
+// constant arrays gave different results on identical launches); a relaxed
+// wavefront-scope atomic load is a plain global_load_dwordx2 that is never
+// merged.
+__device__ __forceinline__ uint64_t ld64(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// The same for one byte (byte arrays read at arbitrary offsets).
+__device__ __forceinline__ uint8_t ld8(const uint8_t* p) {
+    return __hip_atomic_load(const_cast<uint8_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // n (1..32) bases starting at base `off` of the read whose first word is
 // `rd`, right-aligned (util.cuh:180-190).  Reads word off/32+1 only when the
 // window straddles a word boundary.
 __device__ __forceinline__ uint64_t xw(const uint64_t* __restrict__ rd, uint32_t off, uint32_t n) {
     uint32_t o1 = off >> 5;
     uint32_t o2 = (off & 31u) << 1;
-    uint64_t left = rd[o1] << o2;
+    uint64_t left = ld64(rd + o1) << o2;
     if ((64u - o2) >= (n << 1)) return left >> ((32u - n) << 1);
-    uint64_t right = (rd[o1 + 1] >> (62u - o2)) >> 2;
+    uint64_t right = (ld64(rd + o1 + 1) >> (62u - o2)) >> 2;
     return (left | right) >> ((32u - n) << 1);
 }
 
@@ -51,13 +67,13 @@ __device__ __forceinline__ uint64_t xw_regs(const uint64_t (&w)[MAXW], uint32_t 
 __device__ __forceinline__ uint64_t xwu(const uint64_t* __restrict__ bits, uint64_t off, uint32_t n) {
     uint64_t o1 = off >> 5;
     uint32_t o2 = (uint32_t)(off & 31u) << 1;
-    uint64_t left = bits[o1] << o2;
-    uint64_t right = (bits[o1 + 1] >> (62u - o2)) >> 2;
+    uint64_t left = ld64(bits + o1) << o2;
+    uint64_t right = (ld64(bits + o1 + 1) >> (62u - o2)) >> 2;
     return (left | right) >> ((32u - n) << 1);
 }
 
 __device__ __forceinline__ uint32_t base2(const uint64_t* __restrict__ bits, uint64_t off) {
-    return (uint32_t)((bits[off >> 5] >> (((~off) & 31u) << 1)) & 3u);
+    return (uint32_t)((ld64(bits + (off >> 5)) >> (((~off) & 31u) << 1)) & 3u);
 }
 
 // Reverse complement of a right-aligned k-mer (util.cuh:220-226).

@@ -205,11 +205,11 @@ struct Enc3 {
     static constexpr uint32_t BPW = 21;
     __device__ static uint64_t word(const uint64_t* rd, uint32_t off, uint32_t n) {
         uint32_t o1 = off / 21, o2 = off - o1 * 21;
-        uint64_t w = rd[o1];
+        uint64_t w = ld64(rd + o1);
         if (o2 + n <= 21) return (w >> (3 * (21 - o2 - n))) & ((1ull << (3 * n)) - 1);
         uint32_t k1 = 21 - o2, k2 = n - k1;
         uint64_t hi = w & ((1ull << (3 * k1)) - 1);
-        return (hi << (3 * k2)) | (rd[o1 + 1] >> (3 * (21 - k2)));
+        return (hi << (3 * k2)) | (ld64(rd + o1 + 1) >> (3 * (21 - k2)));
     }
     __device__ static uint64_t head(uint64_t w0, uint32_t n) { return w0 >> (3 * (21 - n)); }
     __device__ static uint64_t reversed(uint64_t x, uint32_t n) {

@@ -1831,7 +1831,7 @@ curc_mlp* curc_mlp_create(curc_ctx* c) {
 
 void curc_mlp_destroy(curc_mlp* m) {
     if (!m) return;
-    hipStreamSynchronize(m->c->st);
+    // the context may already be gone: hipFree waits for pending work itself
     m->free_tmp();
     for (void* q : {(void*)m->d.p, (void*)m->d.m, (void*)m->d.v, (void*)m->d.g, (void*)m->d.w1t, (void*)m->d.w2t,
                     (void*)m->d.w2, (void*)m->lsum})
@@ -1943,6 +1943,60 @@ int curc_ctx_set_trace(curc_ctx* c, uint32_t every, uint64_t max_records) {
     if (!c || every == 0 || max_records == 0) return CURC_EINVAL;
     c->trace_every = every;
     c->trace_cap = max_records;
+    return CURC_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// RefMatcher on its own (compress.cu:630-1001): init on src (the HQ
+// pseudogenome, A/C/G/T) and one match of dest; results in the block's
+// CURC_F_RM_* fields of part 0 (matches and found with KEEP_STAGES).
+int curc_ref_match(curc_ctx* c, const char* src, uint64_t src_len, const char* dest, uint64_t dest_len,
+                   int dest_contain_n, int dest_is_ref, uint32_t flags, curc_block** out) {
+    if (!c || !out || (src_len && !src) || (dest_len && !dest)) return CURC_EINVAL;
+    *out = nullptr;
+    curc_block* B = new curc_block();
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        Scratch& S = c->S;
+        hipStream_t st = c->st;
+        uint32_t* cnt = S.as<uint32_t>(S_CNT, C_COUNT);
+        const uint64_t sw = (src_len + 31) / 32 + 1;
+        uint8_t* dsrc = S.as<uint8_t>(S_ASCII, src_len + 1);
+        uint64_t* src2 = S.as<uint64_t>(S_PG_HQ, sw);
+        h2d(c, dsrc, (const uint8_t*)src, src_len);
+        hipLaunchKernelGGL(k_rm_pack_text, dim3(grid_for(sw)), dim3(256), 0, st, (const uint8_t*)dsrc, src_len, src2,
+                           sw);
+        const uint64_t dw = (dest_len + 31) / 32 + 1;
+        uint8_t* ddest = S.as<uint8_t>(S_NPG, dest_len + 16);
+        h2d(c, ddest, (const uint8_t*)dest, dest_len);
+        uint64_t* dest2 = nullptr;
+        if (!dest_contain_n) {
+            dest2 = S.as<uint64_t>(S_PG_LQ, dw);
+            hipLaunchKernelGGL(k_rm_pack_text, dim3(grid_for(dw)), dim3(256), 0, st, (const uint8_t*)ddest, dest_len,
+                               dest2, dw);
+        }
+        Region r(c, "refmatch");
+        const RmIndex ix = rm_index(c, src2, src_len);
+        rm_match(c, ix, src2, src_len, dest2, dest_contain_n ? ddest : nullptr, dest_len, dest_is_ref != 0,
+                 (flags & CURC_RUN_KEEP_STAGES) != 0, cnt, B->rm[0]);
+        CURC_HIP(hipGetLastError());
+        CURC_HIP(hipStreamSynchronize(st));
+        c->harvest();
+    } catch (const std::length_error& e) {
+        delete B;
+        hipStreamSynchronize(c->st);
+        c->pending.clear();
+        return fail(c, CURC_ELIMIT, e.what());
+    } catch (const std::exception& e) {
+        delete B;
+        hipStreamSynchronize(c->st);
+        c->pending.clear();
+        return fail(c, CURC_EHIP, e.what());
+    }
+    *out = B;
     return CURC_OK;
 }
 

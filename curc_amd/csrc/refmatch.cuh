@@ -72,8 +72,8 @@ __device__ __forceinline__ uint32_t rm_hash_ascii(const uint8_t* __restrict__ s)
     uint64_t h = RM_K;
 #pragma unroll
     for (uint32_t j = 0; j < RM_K / 4; ++j) {
-        uint32_t k = (uint32_t)s[4 * j] | (uint32_t)s[4 * j + 1] << 8 | (uint32_t)s[4 * j + 2] << 16 |
-                     (uint32_t)s[4 * j + 3] << 24;
+        uint32_t k = (uint32_t)ld8(s + 4 * j) | (uint32_t)ld8(s + 4 * j + 1) << 8 |
+                     (uint32_t)ld8(s + 4 * j + 2) << 16 | (uint32_t)ld8(s + 4 * j + 3) << 24;
         k += j;
         h ^= k;
         h *= 171717ull;
@@ -107,6 +107,21 @@ __global__ void k_rm_rc2(const uint64_t* __restrict__ in, uint64_t n, uint64_t* 
 
 __global__ void k_rm_rc_ascii(const uint8_t* __restrict__ in, uint64_t n, uint8_t* __restrict__ out) {
     GRID_STRIDE(i, n) out[i] = rm_comp(in[n - 1 - i]);
+}
+
+// 2-bit packing of an n-base ASCII text (one word per thread; bases other
+// than A/C/G/T pack as 0, as in encode, compress.cu:2151-2180).
+__global__ void k_rm_pack_text(const uint8_t* __restrict__ text, uint64_t n, uint64_t* __restrict__ out,
+                               uint64_t nwords) {
+    GRID_STRIDE(w, nwords) {
+        uint64_t v = 0;
+        for (uint32_t k = 0; k < 32; ++k) {
+            const uint64_t i = w * 32 + k;
+            const uint32_t c = i < n ? (enc_base(text[i]) & 3u) : 0u;
+            v |= (uint64_t)c << (62u - 2u * k);
+        }
+        out[w] = v;
+    }
 }
 
 struct RmArgs {
@@ -225,15 +240,15 @@ __global__ void __launch_bounds__(256) k_rm_match_ascii(RmArgs a) {
         for (uint32_t q = qb; q < qe && !hit; ++q) {
             const uint64_t sp = (uint64_t)a.pos[q] * RM_SRC_STEP;
             bool eq = true;
-            for (uint32_t j = 0; j < RM_K && eq; ++j) eq = s[j] == rm_char(base2(a.src, sp + j));
+            for (uint32_t j = 0; j < RM_K && eq; ++j) eq = ld8(s + j) == rm_char(base2(a.src, sp + j));
             if (!eq) continue;
             ++n_cand;
             const uint64_t lim = min(a.dest_len - dp - RM_K, a.src_len - sp - RM_K);
             uint64_t r = 0;
-            while (r < lim && a.rca[dp + RM_K + r] == rm_char(base2(a.src, sp + RM_K + r))) ++r;
+            while (r < lim && ld8(a.rca + dp + RM_K + r) == rm_char(base2(a.src, sp + RM_K + r))) ++r;
             const uint64_t bnd = min(dp, sp);
             uint64_t l = 0;
-            while (l < bnd && a.rca[dp - 1 - l] == rm_char(base2(a.src, sp - 1 - l))) ++l;
+            while (l < bnd && ld8(a.rca + dp - 1 - l) == rm_char(base2(a.src, sp - 1 - l))) ++l;
             n_ext += r + l;
             const uint64_t start = l < bnd ? dp - l : dp - bnd + 1;
             const uint64_t right = dp + RM_K + r;

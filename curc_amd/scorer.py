@@ -59,6 +59,7 @@ class Mlp:
         if not h:
             raise CurcError("curc_mlp_create failed")
         self._h = ctypes.c_void_p(h)
+        ctx._dependents.append(self)
         if params is not None:
             self.set_params(params)
 
@@ -111,6 +112,10 @@ class Mlp:
         if getattr(self, "_h", None) is not None:
             lib().curc_mlp_destroy(self._h)
             self._h = None
+            try:
+                self.ctx._dependents.remove(self)
+            except ValueError:
+                pass
 
     def __del__(self):
         try:

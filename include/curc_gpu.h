@@ -210,6 +210,14 @@ void curc_synth_default(curc_synth_cfg *cfg);
 int curc_synth_genome(const curc_synth_cfg *cfg, char *genome);
 int curc_synth_reads(const curc_synth_cfg *cfg, const char *genome, char *reads);
 
+/* RefMatcher on its own (compress.cu:630-1001): RefMatcher::init on src
+ * (the HQ pseudogenome text) and one RefMatcher::match of dest (a text with
+ * N when dest_contain_n, the character path; dest_is_ref for the self match
+ * of src).  Results in the block's CURC_F_RM_* fields of part 0 (found and
+ * matches with CURC_RUN_KEEP_STAGES in flags). */
+int curc_ref_match(curc_ctx *ctx, const char *src, uint64_t src_len, const char *dest, uint64_t dest_len,
+                   int dest_contain_n, int dest_is_ref, uint32_t flags, curc_block **out);
+
 /* ---- batching ------------------------------------------------------------
  * Many independent blocks or inputs (uneven read counts and lengths) in one
  * call.  Replaces the per-block std::async fan-out of preprocess.cpp:205-276
@@ -240,6 +248,7 @@ int curc_batch_run(curc_ctx *ctx, const curc_job *jobs, uint32_t n_jobs, uint32_
 #define CURC_MLP_NPARAMS 20865u
 typedef struct curc_mlp curc_mlp;
 curc_mlp *curc_mlp_create(curc_ctx *ctx);
+/* may be called after the context was destroyed */
 void curc_mlp_destroy(curc_mlp *m);
 int curc_mlp_set_params(curc_mlp *m, const float *params, uint32_t n);  /* also resets Adam */
 int curc_mlp_get_params(curc_mlp *m, float *params, uint32_t n);

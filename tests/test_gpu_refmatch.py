@@ -82,3 +82,32 @@ def test_gpu_archive_round_trip(gpu_ctx, preserve, paired):
         assert got == want
     else:
         assert sorted(map(bytes, dec)) == sorted(map(bytes, reads))
+
+
+def test_standalone_long_destination_repeatable(gpu_ctx):
+    """RefMatcher on its own with a destination several times longer than
+    the source (the shape of a large LQ text): two runs are identical and
+    equal the oracle (a kernel reading adjacent words through misaligned
+    16-byte loads gave a few different extensions per run on this shape)."""
+    rng = np.random.default_rng(99)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    src = acgt[rng.integers(0, 4, 1_500_000)].tobytes()
+    comp = bytes.maketrans(b"ACGT", b"TGCA")
+    pieces = []
+    for _ in range(20000):
+        a = int(rng.integers(0, len(src) - 400))
+        n = int(rng.integers(60, 400))
+        p = bytearray(src[a:a + n].translate(comp)[::-1])
+        for i in np.flatnonzero(rng.random(n) < 0.01):
+            p[i] = b"ACGT"[(b"ACGT".index(p[i]) + 1) % 4]
+        pieces.append(bytes(p))
+        pieces.append(acgt[rng.integers(0, 4, int(rng.integers(0, 600)))].tobytes())
+    dest = b"".join(pieces)
+    print(f"standalone refmatch: src {len(src)}, dest {len(dest)}", flush=True)
+    a = gpu_ctx.ref_match(src, dest, keep_stages=True)
+    b = gpu_ctx.ref_match(src, dest, keep_stages=True)
+    assert np.array_equal(a["found"], b["found"])
+    o = orc.ref_match(src, dest, False, False, kind="omp")
+    assert np.array_equal(a["found"], o["found"])
+    assert (a["text"], a["map_off"], a["map_len"]) == (o["text"], o["map_off"], o["map_len"])
+    assert archive.recover_ref(a["text"], src, a["map_off"], a["map_len"], True, False) == dest

@@ -1,8 +1,8 @@
 """What the trained scorer gains over mapping without it, on a workload whose
 seed is disjoint from the training seeds: far fewer k-mer probes and
 verified candidates, nearly every read still mapped, and a lossless
-archive no more than 1% larger.  A network with random weights, stopping as
-often, loses more of the mappings' quality."""
+archive no more than 1% larger.  A network with random weights, probing as
+little, maps fewer reads and writes a larger archive."""
 import os
 import tempfile
 
@@ -66,5 +66,6 @@ def test_trained_scorer_gains(gpu_ctx):
     gpu_ctx.set_scorer(None)
     net.close()
     rnd.close()
+    print("random network:", best, flush=True)
     if best is not None:
-        assert got["mis"] - plain["mis"] < best["mis"] - plain["mis"]
+        assert got["mapped"] > best["mapped"] and got["size"] < best["size"]
