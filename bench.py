@@ -118,14 +118,20 @@ def learned_run(ctx, a, L, d_ascii, plain_ms, plain_stats):
     ctx.set_timing(True)
     ctx.device_sync()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for k in range(a.steps):
         b = ctx.run_block(None, L=L, device_ptr=d_ascii, n_reads=a.reads, learned=True)
-        out = outcome(b)
-        b.free()
+        if k + 1 < a.steps:
+            b.free()
     ctx.device_sync()
     ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    out = outcome(b)
+    b.free()
     ctx.set_timing(False)
-    st = {s["name"]: round(s["ms"] / a.steps, 3) for s in ctx.stats() if s["ms"] > 0}
+    raw = ctx.stats()
+    st = {s["name"]: round(s["ms"] / a.steps, 3) for s in raw if s["ms"] > 0}
+    work = {s["name"]: int(s["items"] / a.steps) for s in raw if s["name"].startswith("map_") and s["ms"] == 0}
+    log("  learned stages: " + ", ".join(f"{k} {v:.1f}" for k, v in st.items()))
+    log("  learned mapping work: " + ", ".join(f"{k[4:]} {v}" for k, v in work.items()))
     ctx.set_scorer(None)
     net.close()
     pm = sum(s["ms"] for s in plain_stats if s["name"] == "mapping") / a.steps
@@ -134,7 +140,8 @@ def learned_run(ctx, a, L, d_ascii, plain_ms, plain_stats):
         f"(plain {plain_out['mismatches']})")
     return dict(ms_per_step=round(ms, 3), gbases_s=round(a.reads * L / 1e9 / (ms / 1e3), 4),
                 speedup_vs_plain=round(plain_ms / ms, 4), mapping_ms=st.get("mapping_learned"),
-                mapping_ms_plain=round(pm, 3), threshold=thr, outcome=out, outcome_plain=plain_out)
+                mapping_ms_plain=round(pm, 3), threshold=thr, outcome=out, outcome_plain=plain_out,
+                stages_ms=st, mapping_work=work)
 
 
 def batched_inputs(n_inputs=64, seed=404):

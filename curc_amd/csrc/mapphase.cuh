@@ -34,6 +34,7 @@ struct MapCounters {
     uint32_t impr = 0;      // improvements of the best count
     uint32_t tbest = 255;   // probe index of the last improvement (255: none)
     uint32_t full = 0;      // probes with a full bucket
+    uint32_t gc = 0;        // G/C share of the read, in 1/255
 
     __device__ void load(const MapState& s, uint64_t i) {
         const uint32_t a = s.cnt[i], b = s.cnt[s.n + i], c = s.cnt[2 * s.n + i], d = s.cnt[3 * s.n + i];
@@ -47,36 +48,110 @@ struct MapCounters {
         impr = c >> 24;
         tbest = d & 0xFF;
         full = (d >> 8) & 0xFF;
+        gc = (d >> 16) & 0xFF;
     }
     __device__ void store(const MapState& s, uint64_t i) const {
         s.cnt[i] = cands;
         s.cnt[s.n + i] = min(votes, 255u) | min(empty, 255u) << 8 | min(blast, 255u) << 16 | min(second, 255u) << 24;
         s.cnt[2 * s.n + i] = min(bsum, 0xFFFFu) | min(near, 255u) << 16 | min(impr, 255u) << 24;
-        s.cnt[3 * s.n + i] = min(tbest, 255u) | min(full, 255u) << 8;
+        s.cnt[3 * s.n + i] = min(tbest, 255u) | min(full, 255u) << 8 | min(gc, 255u) << 16;
     }
 };
 
-__global__ void k_map_state_reset(MapState s, const uint32_t* __restrict__ act, const uint32_t* __restrict__ d_n) {
+// G/C bases of a mapping slot's read.
+__device__ __forceinline__ uint32_t read_gc(const MapArgs& a, uint64_t i) {
+    uint32_t gc = 0;
+    if (i < a.n_un) {
+        const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * a.W;
+        for (uint32_t k = 0; k < a.W; ++k) {
+            const uint32_t nb = min(32u, a.L - 32u * k);
+            const uint64_t v = xw(rd, 32u * k, nb);
+            // codes 1 (C) and 2 (G): the two bits differ
+            gc += (uint32_t)__popcll((v ^ (v >> 1)) & 0x5555555555555555ull);
+        }
+    } else {
+        const uint8_t* ra = a.nascii + (i - a.n_un) * a.L;
+        for (uint32_t o = 0; o < a.L; ++o) {
+            const uint8_t u = ld8(ra + o) & 0xDF;
+            gc += (u == 'C' || u == 'G') ? 1u : 0u;
+        }
+    }
+    return gc;
+}
+
+__global__ void k_map_state_reset(MapArgs a, MapState s, const uint32_t* __restrict__ act,
+                                  const uint32_t* __restrict__ d_n) {
     GRID_STRIDE(q, *d_n) {
         const uint64_t i = act[q];
         for (int k = 0; k < 4; ++k) s.seen[k * s.n + i] = ~0ull;
         MapCounters c;
+        c.gc = (255u * read_gc(a, i) + a.L / 2) / a.L;
         c.store(s, i);
         s.stop[i] = 0;
     }
 }
 
+// The 32 features of a read after t_done probes of the pass (t_prev: the
+// probe count at the previous checkpoint).
+__device__ __forceinline__ void map_features(const MapArgs& a, const MapCounters& c, uint32_t last, bool isN,
+                                             bool rc, uint32_t t_done, uint32_t t_prev, float* __restrict__ o) {
+    const float np = (float)((a.L - MAP_K) / MAP_READ_STEP + 1);
+    const float mx = (float)(a.max_mm + 1);
+    float f[MAP_NFEAT];
+    const bool none = last > a.max_mm;
+    f[0] = (float)t_done / np;
+    f[1] = (float)min(last, a.max_mm + 1) / mx;
+    f[2] = none ? 0.f : (float)(c.tbest + 1) / np;
+    f[3] = (float)c.cands / 64.f;
+    f[4] = (float)c.near / 16.f;
+    f[5] = (float)c.votes / 16.f;
+    f[6] = (float)c.empty / np;
+    f[7] = (float)c.bsum / (12.f * (float)max(t_done, 1u));
+    f[8] = (float)c.blast / 12.f;
+    f[9] = (float)c.gc / 255.f;
+    f[10] = rc ? 1.f : 0.f;
+    f[11] = isN ? 1.f : 0.f;
+    f[12] = (float)a.L / 256.f;
+    f[13] = (float)min(c.second, a.max_mm + 1) / mx;
+    f[14] = (float)c.impr / 8.f;
+    f[15] = none ? 1.f : (float)(t_done - 1 - min(c.tbest, t_done - 1)) / np;
+    f[16] = (float)c.full / (float)max(t_done, 1u);
+    f[17] = none ? 1.f : 0.f;
+#pragma unroll
+    for (uint32_t k = 0; k < 7; ++k) f[18 + k] = 0.f;
+    if (!none) f[18 + min(max(last, 2u), 8u) - 2] = 1.f;
+    f[25] = (!none && c.tbest >= t_prev && c.tbest != 255) ? 1.f : 0.f;
+    f[26] = 1.f;
+    f[27] = (float)a.max_mm / 64.f;
+    f[28] = t_done >= 32 ? 1.f : 0.f;
+    f[29] = (float)min(last, 2 * a.max_mm) / (float)a.L;
+    f[30] = (float)(a.max_mm + 1 - min(c.second, a.max_mm + 1)) / mx;
+    f[31] = 0.f;
+#pragma unroll
+    for (uint32_t k = 0; k < MAP_NFEAT; ++k) o[k] = f[k];
+}
+
 // Probes [t0, t1) (probe t reads the k-mer at base 2t) of every read in the
 // active list, continuing k_map's loop from the saved state.
+// With x, each read of the list also writes its features after the range
+// (row q of x, t1 probes done); reads already settled write a row too.
 template <bool RC>
 __global__ void __launch_bounds__(256) k_map_range(MapArgs a, MapState s, const uint32_t* __restrict__ act,
-                                                   const uint32_t* __restrict__ d_n, uint32_t t0, uint32_t t1) {
+                                                   const uint32_t* __restrict__ d_n, uint32_t t0, uint32_t t1,
+                                                   float* __restrict__ x) {
     uint32_t n_probe = 0, n_cand = 0;
     const uint32_t n_act = *d_n;
     GRID_STRIDE(q, n_act) {
         const uint64_t i = act[q];
         uint32_t last = a.last[i];
-        if (last <= MAP_TARGET) continue;
+        if (last <= MAP_TARGET) {
+            if (x) {
+                MapCounters c;
+                c.load(s, i);
+                map_features(a, c, last, i >= a.n_un, RC, t1, t0, x + q * MAP_NFEAT);
+            }
+            continue;
+        }
         const bool isN = i >= a.n_un;
         uint64_t w = a.word[i];
         SeenCache seen;
@@ -150,77 +225,11 @@ __global__ void __launch_bounds__(256) k_map_range(MapArgs a, MapState s, const 
         s.seen[2 * s.n + i] = seen.s2;
         s.seen[3 * s.n + i] = seen.s3;
         c.store(s, i);
+        if (x) map_features(a, c, last, isN, RC, t1, t0, x + q * MAP_NFEAT);
     }
     if (a.work) {
         wave_add_u64(&a.work[1], n_probe);
         wave_add_u64(&a.work[2], n_cand);
-    }
-}
-
-// The 32 features of an active read after t_done probes of the pass
-// (x: float32 [n_act][32], in active-list order).
-__global__ void k_map_features(MapArgs a, MapState s, const uint32_t* __restrict__ act,
-                               const uint32_t* __restrict__ d_n, uint32_t t_done, uint32_t t_prev, bool rc,
-                               float* __restrict__ x) {
-    const uint32_t n_act = *d_n;
-    const float np = (float)((a.L - MAP_K) / MAP_READ_STEP + 1);
-    const float mx = (float)(a.max_mm + 1);
-    GRID_STRIDE(q, n_act) {
-        const uint64_t i = act[q];
-        const uint32_t last = a.last[i];
-        const bool isN = i >= a.n_un;
-        MapCounters c;
-        c.load(s, i);
-        uint32_t gc = 0;
-        if (!isN) {
-            const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * a.W;
-            for (uint32_t k = 0; k < a.W; ++k) {
-                const uint32_t nb = min(32u, a.L - 32u * k);
-                const uint64_t v = xw(rd, 32u * k, nb);
-                // codes 1 (C) and 2 (G): the two bits differ
-                const uint64_t d = (v ^ (v >> 1)) & 0x5555555555555555ull;
-                gc += (uint32_t)__popcll(d);
-            }
-        } else {
-            const uint8_t* ra = a.nascii + (i - a.n_un) * a.L;
-            for (uint32_t o = 0; o < a.L; ++o) {
-                const uint8_t u = ra[o] & 0xDF;
-                gc += (u == 'C' || u == 'G') ? 1u : 0u;
-            }
-        }
-        float f[MAP_NFEAT];
-        const bool none = last > a.max_mm;
-        f[0] = (float)t_done / np;
-        f[1] = (float)min(last, a.max_mm + 1) / mx;
-        f[2] = none ? 0.f : (float)(c.tbest + 1) / np;
-        f[3] = (float)c.cands / 64.f;
-        f[4] = (float)c.near / 16.f;
-        f[5] = (float)c.votes / 16.f;
-        f[6] = (float)c.empty / np;
-        f[7] = (float)c.bsum / (12.f * (float)max(t_done, 1u));
-        f[8] = (float)c.blast / 12.f;
-        f[9] = (float)gc / (float)a.L;
-        f[10] = rc ? 1.f : 0.f;
-        f[11] = isN ? 1.f : 0.f;
-        f[12] = (float)a.L / 256.f;
-        f[13] = (float)min(c.second, a.max_mm + 1) / mx;
-        f[14] = (float)c.impr / 8.f;
-        f[15] = none ? 1.f : (float)(t_done - 1 - min(c.tbest, t_done - 1)) / np;
-        f[16] = (float)c.full / (float)max(t_done, 1u);
-        f[17] = none ? 1.f : 0.f;
-#pragma unroll
-        for (uint32_t k = 0; k < 7; ++k) f[18 + k] = 0.f;
-        if (!none) f[18 + min(max(last, 2u), 8u) - 2] = 1.f;
-        f[25] = (!none && c.tbest >= t_prev && c.tbest != 255) ? 1.f : 0.f;
-        f[26] = 1.f;
-        f[27] = (float)a.max_mm / 64.f;
-        f[28] = t_done >= 32 ? 1.f : 0.f;
-        f[29] = (float)min(last, 2 * a.max_mm) / (float)a.L;
-        f[30] = (float)(a.max_mm + 1 - min(c.second, a.max_mm + 1)) / mx;
-        f[31] = 0.f;
-        float* o = x + (uint64_t)q * MAP_NFEAT;
-#pragma unroll
-        for (uint32_t k = 0; k < MAP_NFEAT; ++k) o[k] = f[k];
     }
 }
 
@@ -249,7 +258,7 @@ __global__ void k_map_trace_emit(const uint32_t* __restrict__ act, const uint32_
                                  unsigned long long* __restrict__ tcount, uint64_t cap) {
     GRID_STRIDE(q, *d_n) {
         const uint32_t i = act[q];
-        if (i % every != 0) continue;
+        if (i % every != 0 || last[i] <= MAP_TARGET) continue;
         const unsigned long long r = atomicAdd(tcount, 1ull);
         if (r >= cap) continue;
         for (uint32_t k = 0; k < MAP_NFEAT; ++k) tx[r * MAP_NFEAT + k] = x[(uint64_t)q * MAP_NFEAT + k];

@@ -685,7 +685,7 @@ void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_
 
 // ------------------------------------------------- mapping in probe ranges ---
 // Probe counts after which the scorer's checkpoints run.
-constexpr uint32_t MAP_CHECKPOINTS[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48};
+constexpr uint32_t MAP_CHECKPOINTS[] = {2, 4, 8, 16, 32};
 
 // One strand pass (one index part) of the mapping as probe ranges with the
 // loop state in HBM.  Learned: at each checkpoint the network scores every
@@ -702,7 +702,7 @@ void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* c
     compact(cnt + C_TMP, PredNeedsMap{a.last}, EmitIota{act[0]}, nact[0], S, S_PRIM, st, nm);
     MapState ms{S.as<uint64_t>(S_MSEEN, 4 * nm), S.as<uint32_t>(S_MCNT, 4 * nm), S.as<uint8_t>(S_MSTOP, nm), nm};
     const uint32_t g = grid_for(nm, 1u << 20);
-    hipLaunchKernelGGL(k_map_state_reset, dim3(g), dim3(256), 0, st, ms, (const uint32_t*)act[0],
+    hipLaunchKernelGGL(k_map_state_reset, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[0],
                        (const uint32_t*)nact[0]);
     float* X = (learned || trace) ? S.as<float>(S_MX, nm * MAP_NFEAT) : nullptr;
     float* logit = learned ? S.as<float>(S_MLOGIT, nm) : nullptr;
@@ -714,20 +714,18 @@ void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* c
     uint32_t t0 = 0;
     for (uint32_t ci = 0;; ++ci) {
         const uint32_t t1 = ci < sizeof(MAP_CHECKPOINTS) / 4 ? std::min(MAP_CHECKPOINTS[ci], nprobe) : nprobe;
+        const bool last_range = t1 >= nprobe;
+        // the range kernel writes each listed read's features at its end
+        float* Xr = last_range ? nullptr : X;
         if (t1 > t0) {
             if (rc)
                 hipLaunchKernelGGL(k_map_range<true>, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
-                                   (const uint32_t*)nact[cur], t0, t1);
+                                   (const uint32_t*)nact[cur], t0, t1, Xr);
             else
                 hipLaunchKernelGGL(k_map_range<false>, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
-                                   (const uint32_t*)nact[cur], t0, t1);
+                                   (const uint32_t*)nact[cur], t0, t1, Xr);
         }
-        if (t1 >= nprobe) break;
-        compact(nact[cur], PredMapActive{act[cur], a.last, ms.stop}, EmitGather{act[cur], act[1 - cur]},
-                nact[1 - cur], S, S_PRIM, st, nm);
-        cur = 1 - cur;
-        hipLaunchKernelGGL(k_map_features, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
-                           (const uint32_t*)nact[cur], t1, t0, rc, X);
+        if (last_range) break;
         if (trace)
             hipLaunchKernelGGL(k_map_trace_emit, dim3(g), dim3(256), 0, st, (const uint32_t*)act[cur],
                                (const uint32_t*)nact[cur], (const float*)X, (const uint8_t*)a.last, c->trace_every,
@@ -739,6 +737,9 @@ void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* c
             hipLaunchKernelGGL(k_map_stop, dim3(g), dim3(256), 0, st, ms, (const uint32_t*)act[cur],
                                (const uint32_t*)nact[cur], (const float*)logit, c->scorer_threshold);
         }
+        compact(nact[cur], PredMapActive{act[cur], a.last, ms.stop}, EmitGather{act[cur], act[1 - cur]},
+                nact[1 - cur], S, S_PRIM, st, nm);
+        cur = 1 - cur;
         t0 = t1;
     }
     if (trace) {

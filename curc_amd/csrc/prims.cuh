@@ -279,7 +279,7 @@ void radix_sort_pairs(K*& k, uint32_t*& v, K*& kt, uint32_t*& vt, uint64_t n, ui
 constexpr uint32_t COMPACT_GRID = 1024;
 // Lists whose host-known bound is at most COMPACT_SMALL go through one
 // workgroup in one launch (16 items per thread per round).
-constexpr uint32_t COMPACT_SMALL = 1u << 15;
+constexpr uint32_t COMPACT_SMALL = 1u << 12;
 constexpr uint32_t COMPACT1_ITEMS = 16;
 
 template <typename Pred>

@@ -100,7 +100,12 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--recall", type=float, default=0.97)
     ap.add_argument("--out", default=MODEL_DIR)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="keep the trained weights in --out and only choose the threshold for --recall on a "
+                         "fresh traced block (seed 7999)")
     a = ap.parse_args(argv)
+    if a.calibrate:
+        return calibrate(a.out, a.recall)
     with Context(0) as ctx:
         log("collecting traced mapping records ...")
         x, y = collect(ctx, a.records)
@@ -128,6 +133,29 @@ def main(argv=None):
                 features=32, layers=[32, 128, 128, 1],
                 data="traced mapping of synthetic blocks, seeds 7001+, disjoint from evaluation")
     with open(os.path.join(a.out, META), "w") as f:
+        json.dump(meta, f, indent=1)
+    log(json.dumps(meta))
+    return 0
+
+
+def calibrate(model_dir, recall):
+    params, _ = load(model_dir)
+    with Context(0) as ctx:
+        ctx.set_trace(every=2, max_records=1 << 24)
+        _, reads = synth_reads(TRAIN_SETS[0][1](7999))
+        blk = ctx.run_block(reads, trace=True)
+        x, y = blk.get("trace_x"), blk.get("trace_y")
+        blk.free()
+        net = Mlp(ctx, params)
+        logit = net.forward(x)
+        net.close()
+    pos = np.sort(logit[y > 0.5])
+    thr = float(pos[int((1 - recall) * len(pos))]) if len(pos) else 0.0
+    with open(os.path.join(model_dir, META)) as f:
+        meta = json.load(f)
+    meta.update(threshold=thr, recall=recall, calib_records=int(len(y)),
+                calib_stop_fraction=float(np.mean(logit < thr)))
+    with open(os.path.join(model_dir, META), "w") as f:
         json.dump(meta, f, indent=1)
     log(json.dumps(meta))
     return 0

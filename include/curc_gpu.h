@@ -259,8 +259,8 @@ int curc_mlp_gradients(curc_mlp *m, const float *d_x, const float *d_y, uint64_t
  * given, receives the batch's mean loss before the step */
 int curc_mlp_train_step(curc_mlp *m, const float *d_x, const float *d_y, uint64_t n, float lr, float *loss);
 /* Learned mapping: with CURC_RUN_LEARNED, the mapping stage checks each
- * unsettled read after 1, 2, 3, 4, 6, 8, 12, 16, 24, 32 and 48 probes of a
- * strand pass; a read whose logit is below `threshold` tries no further
+ * unsettled read after 2, 4, 8, 16 and 32 probes of a strand pass; a
+ * read whose logit is below `threshold` tries no further
  * probes in that pass.  Records stay exact (verification decides); reads
  * may map with more mismatches than the exhaustive loop finds, or not at
  * all.  The network must outlive its use; NULL disables. */
