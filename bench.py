@@ -145,26 +145,37 @@ def learned_run(ctx, a, L, d_ascii, plain_ms, plain_stats):
 
 
 def batched_inputs(n_inputs=64, seed=404):
-    """Many-input workload: genomes of 0.2-10 Mbp, read lengths from
-    {75, 100, 150, 250}, read counts log-uniform in [10k, 500k], 0.3-1%
-    substitutions, 0.02% N."""
+    """Many-input workload: 64 inputs, each from its own genome of 1-50 Mbp
+    (uniform), read lengths from {75, 100, 150, 250}, read counts
+    log-uniform in [10k, 5M], 0.3-1% substitutions, 0.02% N."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n_inputs):
         L = int(rng.choice([75, 100, 150, 250]))
-        n = int(np.exp(rng.uniform(np.log(10_000), np.log(500_000))))
+        n = int(np.exp(rng.uniform(np.log(10_000), np.log(5_000_000))))
         sub = float(rng.uniform(0.003, 0.01))
-        g = int(min(10_000_000, max(200_000, n * L // 20)))
+        g = int(rng.uniform(1_000_000, 50_000_000))
         cfg = ca.synth_config(seed=seed * 1000 + i, genome_len=g, gc=0.45, n_reads=n, L=L, sub_first=sub,
                               sub_last=sub, n_rate=0.0002)
         out.append(ca.synth_reads(cfg)[1])
     return out
 
 
-def batched(ins, reps=2):
-    """The many-input workload one input after another on one stream, then
-    as one batch on concurrent streams; both from host memory."""
+def batched(ins, reps=2, learned=True):
+    """The many-input workload one input after another on one stream (the
+    plain path), then as one batch on concurrent streams, then as one batch
+    with mapping under the trained scorer; all from host memory."""
+    from curc_amd import scorer, train_scorer
+
     gbases = sum(r.size for r in ins) / 1e9
+
+    def mapped(blocks):
+        n = 0
+        for b in blocks:
+            n += len(b.get("map_rec"))
+            b.free()
+        return n
+
     with ca.Context(0) as ctx:
         for b in ctx.batch_run(ins[:4], workers=4):
             b.free()
@@ -175,16 +186,42 @@ def batched(ins, reps=2):
         ctx.device_sync()
         seq = (time.perf_counter() - t) / reps
         t = time.perf_counter()
-        for _ in range(reps):
-            for b in ctx.batch_run(ins, workers=0):
-                b.free()
+        for k in range(reps):
+            blocks = ctx.batch_run(ins, workers=0)
+            if k + 1 < reps:
+                for b in blocks:
+                    b.free()
         ctx.device_sync()
         bat = (time.perf_counter() - t) / reps
-    return dict(inputs=len(ins), gbases=round(gbases, 4), sequential_s=round(seq, 4), batched_s=round(bat, 4),
-                sequential_gbases_s=round(gbases / seq, 4), batched_gbases_s=round(gbases / bat, 4),
-                speedup=round(seq / bat, 3),
-                workload="64 inputs: genomes 0.2-10 Mbp, L in {75,100,150,250}, 10k-500k reads log-uniform, "
-                         "0.3-1% subst, input in host memory")
+        plain_mapped = mapped(blocks)
+        lb = learned_mapped = None
+        if learned:
+            params, thr = train_scorer.load()
+            net = scorer.Mlp(ctx, params)
+            ctx.set_scorer(net, thr)
+            for b in ctx.batch_run(ins[:4], workers=4, learned=True):
+                b.free()
+            t = time.perf_counter()
+            for k in range(reps):
+                blocks = ctx.batch_run(ins, workers=0, learned=True)
+                if k + 1 < reps:
+                    for b in blocks:
+                        b.free()
+            ctx.device_sync()
+            lb = (time.perf_counter() - t) / reps
+            learned_mapped = mapped(blocks)
+            ctx.set_scorer(None)
+            net.close()
+    out = dict(inputs=len(ins), gbases=round(gbases, 4), sequential_s=round(seq, 4), batched_s=round(bat, 4),
+               sequential_gbases_s=round(gbases / seq, 4), batched_gbases_s=round(gbases / bat, 4),
+               speedup=round(seq / bat, 3), mapped_reads=plain_mapped,
+               workload="64 inputs: genomes 1-50 Mbp, L in {75,100,150,250}, 10k-5M reads log-uniform, "
+                        "0.3-1% subst, input in host memory")
+    if lb is not None:
+        out.update(learned_batched_s=round(lb, 4), learned_batched_gbases_s=round(gbases / lb, 4),
+                   learned_batched_vs_plain=round(seq / lb, 3), learned_batched_vs_batched=round(bat / lb, 3),
+                   learned_mapped_reads=learned_mapped)
+    return out
 
 
 def cpu_baseline(n_reads, L):
@@ -298,9 +335,11 @@ def main():
     bat = None
     if not a.no_batch:
         log("many-input batch ...")
-        bat = batched(batched_inputs())
+        bat = batched(batched_inputs(), learned=not a.no_learned)
         log(f"  {bat['inputs']} inputs, {bat['gbases']} Gbases: one by one {bat['sequential_s']} s, "
-            f"batched {bat['batched_s']} s ({bat['speedup']}x)")
+            f"batched {bat['batched_s']} s ({bat['speedup']}x)" +
+            (f", learned+batched {bat['learned_batched_s']} s ({bat['learned_batched_vs_plain']}x)"
+             if "learned_batched_s" in bat else ""))
     cpu = None
     if not a.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample ...")

@@ -112,7 +112,7 @@ def lib():
         "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
         "curc_batch_run": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(i32)]),
         "curc_ref_match": (i32, [vp, vp, u64, vp, u64, i32, i32, u32, ctypes.POINTER(vp)]),
-        "curc_ctx_set_scorer": (i32, [vp, vp, ctypes.c_float]),
+        "curc_ctx_set_scorer": (i32, [vp, vp, ctypes.c_float, ctypes.c_float]),
         "curc_ctx_set_trace": (i32, [vp, u32, u64]),
         "curc_mlp_create": (vp, [vp]),
         "curc_mlp_destroy": (None, [vp]),
@@ -282,12 +282,14 @@ class Context:
         return Block(out.value, has_refmatch=refmatch)
 
     def batch_run(self, inputs, workers=0, keep_stages=False, streams=False, preserve_order=False, paired=False,
-                  refmatch=False):
+                  refmatch=False, learned=False):
         """Runs independent blocks together (curc_batch_run): inputs is a
         list of (n, L) uint8 host arrays (lengths may differ); returns one
-        Block per input, in input order."""
+        Block per input, in input order.  With learned, every input maps
+        under the context's scorer."""
         arrs = [np.ascontiguousarray(r, dtype=np.uint8) for r in inputs]
         flags = RUN_KEEP_STAGES if keep_stages else 0
+        flags |= RUN_LEARNED if learned else 0
         if streams:
             flags |= RUN_STREAMS | (RUN_PRESERVE_ORDER if preserve_order else 0) | (RUN_PAIRED if paired else 0)
         if refmatch:
@@ -326,10 +328,12 @@ class Context:
 
     def set_scorer(self, mlp, threshold=0.0):
         """Scorer network (curc_amd.scorer.Mlp, kept alive by the caller) for
-        runs with learned=True; None disables."""
+        runs with learned=True; None disables.  threshold is one logit
+        threshold, or (for reads with a mapping, for reads without one)."""
         self._scorer = mlp
+        t_map, t_un = (threshold, threshold) if np.isscalar(threshold) else threshold
         self._check(lib().curc_ctx_set_scorer(self._h, mlp._h if mlp is not None else None,
-                                              ctypes.c_float(threshold)), "curc_ctx_set_scorer")
+                                              ctypes.c_float(t_map), ctypes.c_float(t_un)), "curc_ctx_set_scorer")
 
     def set_trace(self, every=16, max_records=1 << 22):
         self._check(lib().curc_ctx_set_trace(self._h, every, max_records), "curc_ctx_set_trace")

@@ -233,10 +233,17 @@ __global__ void __launch_bounds__(256) k_map_range(MapArgs a, MapState s, const 
     }
 }
 
-// Learned mode: a read whose logit is below the threshold ends its pass.
+// Learned mode: a read whose logit is below its threshold ends its pass.
+// A read that already has a mapping (best count <= max_mm) only gives up a
+// lower count by stopping; one without gives up being mapped at all, so it
+// has a threshold of its own.
 __global__ void k_map_stop(MapState s, const uint32_t* __restrict__ act, const uint32_t* __restrict__ d_n,
-                           const float* __restrict__ logit, float threshold) {
-    GRID_STRIDE(q, *d_n) if (logit[q] < threshold) s.stop[act[q]] = 1;
+                           const float* __restrict__ logit, const uint8_t* __restrict__ last, uint32_t max_mm,
+                           float thr_mapped, float thr_unmapped) {
+    GRID_STRIDE(q, *d_n) {
+        const uint32_t i = act[q];
+        if (logit[q] < (last[i] <= max_mm ? thr_mapped : thr_unmapped)) s.stop[i] = 1;
+    }
 }
 
 // Reads that stay in the pass: above the target and not stopped.

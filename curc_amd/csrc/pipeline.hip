@@ -107,7 +107,8 @@ struct curc_ctx {
     // learned mapping (CURC_RUN_LEARNED): the scorer's device arrays
     bool has_scorer = false;
     MlpDev scorer_net{};
-    float scorer_threshold = 0.f;
+    float scorer_threshold = 0.f;           // reads that already have a mapping
+    float scorer_threshold_unmapped = 0.f;  // reads without one
     const MlpDev& scorer_dev() const { return scorer_net; }
     // traced mapping (CURC_RUN_TRACE): sampling of reads and record capacity
     uint32_t trace_every = 16;
@@ -735,7 +736,8 @@ void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* c
             hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(nm)), dim3(256), 0, st, c->scorer_dev(), (const float*)X,
                                nm, (const uint32_t*)nact[cur], logit);
             hipLaunchKernelGGL(k_map_stop, dim3(g), dim3(256), 0, st, ms, (const uint32_t*)act[cur],
-                               (const uint32_t*)nact[cur], (const float*)logit, c->scorer_threshold);
+                               (const uint32_t*)nact[cur], (const float*)logit, (const uint8_t*)a.last,
+                               a.max_mm, c->scorer_threshold, c->scorer_threshold_unmapped);
         }
         compact(nact[cur], PredMapActive{act[cur], a.last, ms.stop}, EmitGather{act[cur], act[1 - cur]},
                 nact[1 - cur], S, S_PRIM, st, nm);
@@ -1710,8 +1712,16 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
     }
     std::vector<curc_ctx*> pool{c};
     for (uint32_t w = 1; w < workers; ++w) {
-        c->workers[w - 1]->timing = c->timing;
-        pool.push_back(c->workers[w - 1]);
+        curc_ctx* wc = c->workers[w - 1];
+        wc->timing = c->timing;
+        // workers share the scorer's device arrays (read-only during mapping)
+        wc->has_scorer = c->has_scorer;
+        wc->scorer_net = c->scorer_net;
+        wc->scorer_threshold = c->scorer_threshold;
+        wc->scorer_threshold_unmapped = c->scorer_threshold_unmapped;
+        wc->trace_every = c->trace_every;
+        wc->trace_cap = c->trace_cap;
+        pool.push_back(wc);
     }
     std::vector<uint32_t> order(n_jobs);
     std::iota(order.begin(), order.end(), 0u);
@@ -1932,11 +1942,12 @@ static_assert(CURC_MLP_NPARAMS == curc::MLP_NPARAMS, "scorer parameter count");
 
 extern "C" {
 
-int curc_ctx_set_scorer(curc_ctx* c, curc_mlp* m, float threshold) {
+int curc_ctx_set_scorer(curc_ctx* c, curc_mlp* m, float threshold_mapped, float threshold_unmapped) {
     if (!c) return CURC_EINVAL;
     c->has_scorer = m != nullptr;
     if (m) c->scorer_net = m->d;
-    c->scorer_threshold = threshold;
+    c->scorer_threshold = threshold_mapped;
+    c->scorer_threshold_unmapped = threshold_unmapped;
     return CURC_OK;
 }
 

@@ -8,8 +8,10 @@ tests evaluate on: at each checkpoint of a strand pass a sampled read
 contributes its 32 features and the label "the rest of the pass lowered the
 read's best mismatch count", as decided by exact verification.  The network
 is trained with the library's own bfloat16 MFMA forward / backward kernels
-and Adam; the stop threshold is the logit that keeps `--recall` of the
-positive validation records above it.
+and Adam.  There are two stop thresholds: for reads that already have a
+mapping, the logit that keeps `--recall` of their positive validation
+records above it; for reads without one (feature 17 set), the logit that
+keeps `--recall-unmapped` of theirs.
 """
 import argparse
 import json
@@ -34,6 +36,13 @@ TRAIN_SETS = (
                                              rc_prob=0.5, sub_first=0.005, sub_last=0.005, n_rate=0.0001)),
     ("short reads", lambda s: synth_config(seed=s, genome_len=2_000_000, gc=0.45, n_reads=400_000, L=100,
                                            sub_first=0.003, sub_last=0.012, n_rate=0.0002)),
+    ("75 bp, uniform 1% errors", lambda s: synth_config(seed=s, genome_len=3_000_000, gc=0.45, n_reads=600_000,
+                                                        L=75, sub_first=0.01, sub_last=0.01, n_rate=0.0002)),
+    ("250 bp, uniform 0.6% errors", lambda s: synth_config(seed=s, genome_len=8_000_000, gc=0.45,
+                                                           n_reads=400_000, L=250, sub_first=0.006,
+                                                           sub_last=0.006, n_rate=0.0002)),
+    ("150 bp, low coverage", lambda s: synth_config(seed=s, genome_len=20_000_000, gc=0.45, n_reads=300_000,
+                                                    L=150, sub_first=0.008, sub_last=0.008, n_rate=0.0002)),
 )
 
 
@@ -98,14 +107,15 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--recall", type=float, default=0.97)
+    ap.add_argument("--recall", type=float, default=0.95)
+    ap.add_argument("--recall-unmapped", type=float, default=0.9999)
     ap.add_argument("--out", default=MODEL_DIR)
     ap.add_argument("--calibrate", action="store_true",
-                    help="keep the trained weights in --out and only choose the threshold for --recall on a "
-                         "fresh traced block (seed 7999)")
+                    help="keep the trained weights in --out and only choose the threshold for --recall on "
+                         "one fresh traced block of each training workload (seeds 7990+)")
     a = ap.parse_args(argv)
     if a.calibrate:
-        return calibrate(a.out, a.recall)
+        return calibrate(a.out, a.recall, a.recall_unmapped)
     with Context(0) as ctx:
         log("collecting traced mapping records ...")
         x, y = collect(ctx, a.records)
@@ -118,16 +128,16 @@ def main(argv=None):
         net, steps, dt = train(ctx, xt, yt, a.epochs, a.batch, a.lr)
         flops = 6.0 * 20865 * len(yt) * a.epochs  # forward + backward per sample
         logit = net.forward(xv)
-        pos = np.sort(logit[yv > 0.5])
-        thr = float(pos[int((1 - a.recall) * len(pos))]) if len(pos) else 0.0
-        stop = float(np.mean(logit < thr))
+        thr, thr_un = thresholds(xv, yv, logit, a.recall, a.recall_unmapped)
+        stop = float(np.mean(logit < np.where(xv[:, UNMAPPED_FEATURE] > 0.5, thr_un, thr)))
         acc = float(np.mean((logit > 0) == (yv > 0.5)))
         lv = float(np.mean(np.logaddexp(0, logit) - yv * logit))
         params = net.get_params()
         net.close()
     os.makedirs(a.out, exist_ok=True)
     params.astype("<f4").tofile(os.path.join(a.out, WEIGHTS))
-    meta = dict(threshold=thr, recall=a.recall, records=int(len(y)), positive=float(y.mean()),
+    meta = dict(threshold=thr, threshold_unmapped=thr_un, recall=a.recall, recall_unmapped=a.recall_unmapped,
+                records=int(len(y)), positive=float(y.mean()),
                 val_loss=lv, val_accuracy=acc, val_stop_fraction=stop, epochs=a.epochs, batch=a.batch, lr=a.lr,
                 steps=steps, train_seconds=round(dt, 3), train_tflops=round(flops / dt / 1e12, 3),
                 features=32, layers=[32, 128, 128, 1],
@@ -138,23 +148,42 @@ def main(argv=None):
     return 0
 
 
-def calibrate(model_dir, recall):
+# feature 17 is 1 for a read with no mapping within max_mismatch yet
+UNMAPPED_FEATURE = 17
+
+
+def thresholds(x, y, logit, recall, recall_unmapped):
+    """(threshold for reads with a mapping, threshold for reads without):
+    each keeps the given fraction of its group's positive records."""
+    out = []
+    un = x[:, UNMAPPED_FEATURE] > 0.5
+    for group, r in ((~un, recall), (un, recall_unmapped)):
+        pos = np.sort(logit[group & (y > 0.5)])
+        out.append(float(pos[int((1 - r) * len(pos))]) if len(pos) else -1e30)
+    return tuple(out)
+
+
+def calibrate(model_dir, recall, recall_unmapped):
     params, _ = load(model_dir)
+    xs, ys = [], []
     with Context(0) as ctx:
-        ctx.set_trace(every=2, max_records=1 << 24)
-        _, reads = synth_reads(TRAIN_SETS[0][1](7999))
-        blk = ctx.run_block(reads, trace=True)
-        x, y = blk.get("trace_x"), blk.get("trace_y")
-        blk.free()
+        ctx.set_trace(every=4, max_records=1 << 24)
+        for k, (name, make) in enumerate(TRAIN_SETS):
+            _, reads = synth_reads(make(7990 + k))
+            blk = ctx.run_block(reads, trace=True)
+            xs.append(blk.get("trace_x"))
+            ys.append(blk.get("trace_y"))
+            blk.free()
+        x, y = np.concatenate(xs), np.concatenate(ys)
         net = Mlp(ctx, params)
         logit = net.forward(x)
         net.close()
-    pos = np.sort(logit[y > 0.5])
-    thr = float(pos[int((1 - recall) * len(pos))]) if len(pos) else 0.0
+    thr, thr_un = thresholds(x, y, logit, recall, recall_unmapped)
     with open(os.path.join(model_dir, META)) as f:
         meta = json.load(f)
-    meta.update(threshold=thr, recall=recall, calib_records=int(len(y)),
-                calib_stop_fraction=float(np.mean(logit < thr)))
+    stop = logit < np.where(x[:, UNMAPPED_FEATURE] > 0.5, thr_un, thr)
+    meta.update(threshold=thr, threshold_unmapped=thr_un, recall=recall, recall_unmapped=recall_unmapped,
+                calib_records=int(len(y)), calib_stop_fraction=float(np.mean(stop)))
     with open(os.path.join(model_dir, META), "w") as f:
         json.dump(meta, f, indent=1)
     log(json.dumps(meta))
@@ -162,11 +191,13 @@ def calibrate(model_dir, recall):
 
 
 def load(model_dir=MODEL_DIR):
-    """(params float32 [20865], threshold) of the trained scorer."""
+    """(params float32 [20865], (threshold for reads with a mapping,
+    threshold for reads without)) of the trained scorer."""
     p = np.fromfile(os.path.join(model_dir, WEIGHTS), dtype="<f4")
     with open(os.path.join(model_dir, META)) as f:
         meta = json.load(f)
-    return p, float(meta["threshold"])
+    thr = float(meta["threshold"])
+    return p, (thr, float(meta.get("threshold_unmapped", thr)))
 
 
 if __name__ == "__main__":

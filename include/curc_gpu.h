@@ -260,11 +260,13 @@ int curc_mlp_gradients(curc_mlp *m, const float *d_x, const float *d_y, uint64_t
 int curc_mlp_train_step(curc_mlp *m, const float *d_x, const float *d_y, uint64_t n, float lr, float *loss);
 /* Learned mapping: with CURC_RUN_LEARNED, the mapping stage checks each
  * unsettled read after 2, 4, 8, 16 and 32 probes of a strand pass; a
- * read whose logit is below `threshold` tries no further
- * probes in that pass.  Records stay exact (verification decides); reads
- * may map with more mismatches than the exhaustive loop finds, or not at
- * all.  The network must outlive its use; NULL disables. */
-int curc_ctx_set_scorer(curc_ctx *ctx, curc_mlp *m, float threshold);
+ * read whose logit is below its threshold tries no further probes in that
+ * pass: `threshold_mapped` for a read that already has a mapping within
+ * max_mismatch, `threshold_unmapped` for one that has none yet.  Records
+ * stay exact (verification decides); reads may map with more mismatches
+ * than the exhaustive loop finds, or not at all.  The network must outlive
+ * its use; NULL disables. */
+int curc_ctx_set_scorer(curc_ctx *ctx, curc_mlp *m, float threshold_mapped, float threshold_unmapped);
 /* Traced mapping (CURC_RUN_TRACE): reads whose slot is a multiple of
  * `every` contribute one record per checkpoint, up to max_records. */
 int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import curc_amd as ca
+from curc_amd import scorer, train_scorer
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -61,3 +62,23 @@ def test_batch_empty_and_single(gpu_ctx):
     (b,) = gpu_ctx.batch_run([r])
     s = gpu_ctx.run_block(r)
     assert np.array_equal(b.get("hq_pg"), s.get("hq_pg"))
+
+
+def test_batch_learned_matches_single_learned_runs(gpu_ctx):
+    """Worker streams map under the calling context's scorer."""
+    ins = inputs(8, 6)
+    params, thr = train_scorer.load()
+    net = scorer.Mlp(gpu_ctx, params)
+    gpu_ctx.set_scorer(net, thr)
+    try:
+        print("learned batch of", [r.shape for r in ins], flush=True)
+        blocks = gpu_ctx.batch_run(ins, workers=4, learned=True)
+        for i, (r, b) in enumerate(zip(ins, blocks)):
+            s = gpu_ctx.run_block(r, learned=True)
+            for f in FIELDS:
+                assert np.array_equal(b.get(f), s.get(f)), f"input {i}: {f} differs from the single learned run"
+            s.free()
+            b.free()
+    finally:
+        gpu_ctx.set_scorer(None)
+        net.close()

@@ -69,3 +69,29 @@ def test_trained_scorer_gains(gpu_ctx):
     print("random network:", best, flush=True)
     if best is not None:
         assert got["mapped"] > best["mapped"] and got["size"] < best["size"]
+
+
+@pytest.mark.parametrize("L,sub,genome", [(75, 0.01, 2_000_000), (250, 0.006, 6_000_000), (100, 0.003, 30_000_000)])
+def test_trained_scorer_other_read_lengths(gpu_ctx, L, sub, genome):
+    """Read lengths, uniform error rates and coverages of the many-input
+    workload: the scorer keeps nearly every mapping and the archive size, and
+    never probes more than the exhaustive loop.  (At 0.8x coverage almost no
+    read maps, and a read that will never map cannot be told early from one
+    that maps late, so there the savings are small.)"""
+    cfg = ca.synth_config(seed=57 + L, genome_len=genome, gc=0.45, n_reads=250_000, L=L, sub_first=sub,
+                          sub_last=sub, n_rate=0.0002)
+    _, reads = ca.synth_reads(cfg)
+    print(f"ablation: L={L}, subst {sub}, genome {genome}", flush=True)
+    plain = run(gpu_ctx, reads, False)
+    params, thr = train_scorer.load()
+    net = scorer.Mlp(gpu_ctx, params)
+    gpu_ctx.set_scorer(net, thr)
+    try:
+        got = run(gpu_ctx, reads, True)
+    finally:
+        gpu_ctx.set_scorer(None)
+        net.close()
+    print(plain, got, flush=True)
+    assert got["probes"] <= plain["probes"]
+    assert got["mapped"] >= 0.99 * plain["mapped"]
+    assert got["size"] <= 1.01 * plain["size"]
