@@ -14,6 +14,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -45,12 +46,42 @@ def parse():
     p.add_argument("--no-bits-per-base", action="store_true")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--no-learned", action="store_true")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                   help="only run three device-to-device copies of BYTES (a kernel of known traffic for "
+                        "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
 
 
-def dominant_roofline(stats, steps):
+def copy_probe(nbytes):
+    import torch
+
+    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copied {nbytes} bytes three times")
+
+
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "mapping_traffic_100Mbp_20M_reads.md")
+
+
+def measured_traffic():
+    """HBM bytes per mapping launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None if absent)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            m = re.search(r"^traffic_bytes_per_launch = (\d+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def dominant_roofline(stats, steps, default_workload=True):
     """Roofline entry for the stage with the largest event time whose region
-    brackets single kernel launches."""
+    brackets single kernel launches.  traffic (rocprof HBM bytes per launch)
+    is given for the default workload only, which the profile covers."""
     single = [s for s in stats if s["name"] in ("mapping",)]
     if not single:
         return None
@@ -59,7 +90,7 @@ def dominant_roofline(stats, steps):
     per_launch_bytes = s["alg_bytes"] / max(1, s["launches"])
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     return dict(kernel=s["name"], bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 5), traffic=measured_traffic() if default_workload else None,
                 avg_launch_ms=round(per_launch_ms, 4), alg_bytes_per_launch=per_launch_bytes)
 
 
@@ -250,6 +281,9 @@ def cpu_baseline(n_reads, L):
 
 def main():
     a = parse()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
               f"(blocks shard across GPUs as independent jobs; see DESIGN.md)", file=sys.stderr)
@@ -297,7 +331,7 @@ def main():
     for s in sorted(stats, key=lambda s: -s["ms"]):
         log(f"  {s['name']:<14} {s['ms'] / a.steps:9.2f} ms  ({s['launches'] // max(1, a.steps)} regions/step)")
     log("mapping work per step (both strand passes): " + ", ".join(f"{k[4:]} {v}" for k, v in work.items()))
-    roof = dominant_roofline(stats, a.steps)
+    roof = dominant_roofline(stats, a.steps, default_workload=(a.reads, a.genome) == (20_000_000, 100_000_000))
     learned = None
     if not a.no_learned:
         learned = learned_run(ctx, a, L, d_ascii, ms, stats)
