@@ -34,6 +34,11 @@ __device__ __forceinline__ uint64_t ld64(const uint64_t* p) {
     return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
+// The same for 32 bits (u32 arrays read at arbitrary indexes).
+__device__ __forceinline__ uint32_t ld32(const uint32_t* p) {
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // The same for one byte (byte arrays read at arbitrary offsets).
 __device__ __forceinline__ uint8_t ld8(const uint8_t* p) {
     return __hip_atomic_load(const_cast<uint8_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);

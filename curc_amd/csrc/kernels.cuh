@@ -668,6 +668,29 @@ __global__ void k_map_hash(const uint64_t* __restrict__ pg, uint64_t start, uint
     }
 }
 
+// Bucket-inline index: bucket h is the 64-byte slot bk[16h .. 16h+15]
+// holding its capped size in word 0 and its positions in words 1..12, so a
+// probe reads one line instead of the CSR offsets' line plus the
+// positions' line.  Used while H * 64 bytes stays within MAP_SLOT_MAX_H.
+constexpr uint32_t MAP_SLOT = 16;
+constexpr uint64_t MAP_SLOT_MAX_H = 1ull << 28;
+
+__global__ void k_map_slot_counts(const uint32_t* __restrict__ krf, uint64_t H, uint32_t* __restrict__ bk) {
+    GRID_STRIDE(b, H) bk[b * MAP_SLOT] = min(krf[b + 1] - krf[b], MAP_BUCKET);
+}
+
+// The same order as k_map_fill: the first min(n,12) occurrences by ascending
+// i, stored in descending i.
+__global__ void k_map_slot_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                uint64_t count, const uint32_t* __restrict__ krf, uint32_t* __restrict__ bk) {
+    GRID_STRIDE(t, count) {
+        const uint32_t h = keys[t];
+        const uint32_t q = (uint32_t)t - krf[h];
+        const uint32_t c = min(krf[h + 1] - krf[h], MAP_BUCKET);
+        if (q < c) bk[(uint64_t)h * MAP_SLOT + 1 + (c - 1 - q)] = vals[t];
+    }
+}
+
 // Capped bucket sizes: min(run length, 12), entries [0, H+2).
 __global__ void k_map_caps(const uint32_t* __restrict__ krf, uint64_t H, uint32_t* __restrict__ caps) {
     GRID_STRIDE(b, H + 2) {
@@ -737,7 +760,22 @@ struct MapArgs {
     // [5] candidates of reads left above 1 mismatch by this pass
     unsigned long long* work;
     uint32_t max_mm = 0;       // floor(L / 5) (used by the probe-range kernels)
+    const uint32_t* bk = nullptr;  // bucket-inline index (kr / pos unused when set)
 };
+
+// Bucket h of the mapping index: its capped size and its positions.
+struct MapBucket {
+    uint32_t n;
+    const uint32_t* p;
+};
+__device__ __forceinline__ MapBucket map_bucket(const MapArgs& a, uint32_t h) {
+    if (a.bk) {
+        const uint32_t* s = a.bk + (uint64_t)h * MAP_SLOT;
+        return MapBucket{ld32(s), s + 1};
+    }
+    const uint32_t b = a.kr[h], e = a.kr[h + 1];
+    return MapBucket{min(e - b, MAP_BUCKET), a.pos + b};
+}
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
     uint64_t s = v;
@@ -761,6 +799,42 @@ struct SeenCache {
     }
 };
 
+// One N-read (ASCII, compared base by base) through the k_map loop.
+template <bool RC>
+__device__ __forceinline__ void map_nread(const MapArgs& a, uint64_t i, uint32_t& last, uint64_t& w,
+                                          SeenCache& seen, uint32_t& n_probe, uint32_t& n_cand) {
+    const uint8_t* ra = a.nascii + (i - a.n_un) * a.L;
+    for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
+        ++n_probe;
+        uint32_t h = (uint32_t)(murmur64(kmer_ascii(ra + j, MAP_K)) & a.mask);
+        const MapBucket B = map_bucket(a, h);
+        if (B.n == 0) continue;
+        const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
+        for (uint32_t p = 0; p < B.n; ++p) {
+            uint64_t rp = a.start + (uint64_t)ld32(B.p + p) * MAP_REF_STEP;
+            if (rp < ls) continue;
+            uint64_t mp = rp - ls;
+            if (mp + a.L > a.pg_len) continue;
+            if (seen.check_insert(mp)) continue;
+            ++n_cand;
+            uint32_t c = 0;
+            for (uint32_t o = 0; o < a.L; ++o) {
+                uint8_t rb = base_char(base2(a.pg, mp + o));
+                uint8_t qb = RC ? rev_base_ascii(ra[a.L - 1 - o]) : ra[o];
+                if (rb != qb) {
+                    c++;
+                    if (c >= last) break;
+                }
+            }
+            if (c < last) {
+                w = 1ull | (RC ? 2ull : 0ull) | (mp << 2);
+                last = c;
+                if (last <= MAP_TARGET) return;
+            }
+        }
+    }
+}
+
 // compress.cu:411-521: one thread per read; candidates in bucket order,
 // a strictly better mismatch count replaces the record, stop at <= 1.
 template <bool RC>
@@ -779,12 +853,11 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
             for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
                 ++n_probe;
                 uint32_t h = (uint32_t)(murmur64(xw(rd, j, MAP_K)) & a.mask);
-                uint32_t b = a.kr[h], e = a.kr[h + 1];
-                if (b == e) continue;
-                e = min(b + MAP_BUCKET, e);
+                const MapBucket B = map_bucket(a, h);
+                if (B.n == 0) continue;
                 const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
-                for (uint32_t p = b; p < e; ++p) {
-                    uint64_t rp = a.start + (uint64_t)a.pos[p] * MAP_REF_STEP;
+                for (uint32_t p = 0; p < B.n; ++p) {
+                    uint64_t rp = a.start + (uint64_t)ld32(B.p + p) * MAP_REF_STEP;
                     if (rp < ls) continue;
                     uint64_t mp = rp - ls;
                     if (mp + a.L > a.pg_len) continue;
@@ -799,37 +872,7 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
                 }
             }
         } else {
-            const uint8_t* ra = a.nascii + (i - a.n_un) * a.L;
-            for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
-                ++n_probe;
-                uint32_t h = (uint32_t)(murmur64(kmer_ascii(ra + j, MAP_K)) & a.mask);
-                uint32_t b = a.kr[h], e = a.kr[h + 1];
-                if (b == e) continue;
-                e = min(b + MAP_BUCKET, e);
-                const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
-                for (uint32_t p = b; p < e; ++p) {
-                    uint64_t rp = a.start + (uint64_t)a.pos[p] * MAP_REF_STEP;
-                    if (rp < ls) continue;
-                    uint64_t mp = rp - ls;
-                    if (mp + a.L > a.pg_len) continue;
-                    if (seen.check_insert(mp)) continue;
-                    ++n_cand;
-                    uint32_t c = 0;
-                    for (uint32_t o = 0; o < a.L; ++o) {
-                        uint8_t rb = base_char(base2(a.pg, mp + o));
-                        uint8_t qb = RC ? rev_base_ascii(ra[a.L - 1 - o]) : ra[o];
-                        if (rb != qb) {
-                            c++;
-                            if (c >= last) break;
-                        }
-                    }
-                    if (c < last) {
-                        w = 1ull | (RC ? 2ull : 0ull) | (mp << 2);
-                        last = c;
-                        if (last <= MAP_TARGET) goto done;
-                    }
-                }
-            }
+            map_nread<RC>(a, i, last, w, seen, n_probe, n_cand);
         }
     done:
         a.last[i] = (uint8_t)last;
@@ -847,6 +890,154 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
     wave_add_u64(&a.work[3], n_first);
     wave_add_u64(&a.work[4], n_open_probe);
     wave_add_u64(&a.work[5], n_open_cand);
+}
+
+// ---- k_map with the read in registers (bucket-inline index only) ----------
+// The same loop, candidate order, seen-cache and early exits as k_map, so
+// the same records and counters; what changes is how often a thread waits
+// on memory.  The read's words (and, on the RC strand, the chunks of its
+// reverse complement) are loaded once into registers; the bucket heads of
+// MAP_GROUP probes are requested together; a candidate's pseudogenome
+// window is requested in one go (W + 1 independent 8-byte loads) instead
+// of chunk by chunk.  WM is the register capacity in words (W <= WM).
+constexpr uint32_t MAP_GROUP = 4;
+
+// a[i] for a register array and a run-time i (0 if i >= WM), by selects.
+template <uint32_t WM>
+__device__ __forceinline__ uint64_t pick(const uint64_t (&a)[WM], uint32_t i) {
+    uint64_t v = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < WM; ++k) v = (i == k) ? a[k] : v;
+    return v;
+}
+
+// n (1..32) bases from base `off` of a read held in registers (as xw).
+template <uint32_t WM>
+__device__ __forceinline__ uint64_t xw_pick(const uint64_t (&w)[WM], uint32_t off, uint32_t n) {
+    const uint32_t o1 = off >> 5;
+    const uint32_t o2 = (off & 31u) << 1;
+    const uint64_t left = pick(w, o1) << o2;
+    if ((64u - o2) >= (n << 1)) return left >> ((32u - n) << 1);
+    const uint64_t right = (pick(w, o1 + 1) >> (62u - o2)) >> 2;
+    return (left | right) >> ((32u - n) << 1);
+}
+
+template <bool RC, uint32_t WM>
+__global__ void __launch_bounds__(256) k_map_fast(MapArgs a) {
+    uint32_t n_act = 0, n_probe = 0, n_cand = 0, n_first = 0, n_open_probe = 0, n_open_cand = 0;
+    const uint32_t L = a.L, W = a.W;
+    const uint32_t np = (L - MAP_K) / MAP_READ_STEP + 1;
+    GRID_STRIDE(i, a.n_un + a.n_N) {
+        uint32_t last = a.last[i];
+        if (last <= MAP_TARGET) continue;
+        ++n_act;
+        const uint32_t probe0 = n_probe, cand0 = n_cand;
+        uint64_t w = a.word[i];
+        SeenCache seen;
+        if (i < a.n_un) {
+            const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * W;
+            uint64_t r[WM], q[WM];
+#pragma unroll
+            for (uint32_t k = 0; k < WM; ++k) r[k] = k < W ? ld64(rd + k) : 0ull;
+            // query chunk k: bases [32k, 32k + n_k) of the read or of its
+            // reverse complement, right-aligned (the chunking of fwd_mm / rc_mm)
+#pragma unroll
+            for (uint32_t k = 0; k < WM; ++k) {
+                q[k] = 0;
+                if (32 * k < L) {
+                    const uint32_t n = min(32u, L - 32 * k);
+                    q[k] = RC ? revcomp(xw_pick(r, L - 32 * k - n, n), n) : xw_pick(r, 32 * k, n);
+                }
+            }
+            for (uint32_t t0 = 0; t0 < np; t0 += MAP_GROUP) {
+                uint4 hd[MAP_GROUP];
+                uint32_t hh[MAP_GROUP];
+#pragma unroll
+                for (uint32_t u = 0; u < MAP_GROUP; ++u) {
+                    hd[u] = make_uint4(0, 0, 0, 0);
+                    hh[u] = 0;
+                    if (t0 + u < np) {
+                        hh[u] = (uint32_t)(murmur64(xw_pick(r, MAP_READ_STEP * (t0 + u), MAP_K)) & a.mask);
+                        hd[u] = *reinterpret_cast<const uint4*>(a.bk + (uint64_t)hh[u] * MAP_SLOT);
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < MAP_GROUP; ++u) {
+                    const uint32_t t = t0 + u;
+                    if (t >= np) break;
+                    ++n_probe;
+                    const uint32_t nb = hd[u].x;
+                    const uint32_t j = MAP_READ_STEP * t;
+                    const uint64_t ls = RC ? (uint64_t)(L - MAP_K - j) : (uint64_t)j;
+                    for (uint32_t p = 0; p < nb; ++p) {
+                        const uint32_t pv = p == 0   ? hd[u].y
+                                            : p == 1 ? hd[u].z
+                                            : p == 2 ? hd[u].w
+                                                     : ld32(a.bk + (uint64_t)hh[u] * MAP_SLOT + 1 + p);
+                        const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
+                        if (rp < ls) continue;
+                        const uint64_t mp = rp - ls;
+                        if (mp + L > a.pg_len) continue;
+                        if (seen.check_insert(mp)) continue;
+                        ++n_cand;
+                        // the window's W + 1 words, requested together
+                        const uint64_t* g0 = a.pg + (mp >> 5);
+                        uint64_t g[WM + 1];
+#pragma unroll
+                        for (uint32_t k = 0; k <= WM; ++k) g[k] = k <= W ? ld64(g0 + k) : 0ull;
+                        const uint32_t sh = (uint32_t)(mp & 31u) << 1;
+                        uint32_t c = 0;
+                        const uint32_t bound = last - 1;
+#pragma unroll
+                        for (uint32_t k = 0; k < WM; ++k) {
+                            if (32 * k < L && c <= bound) {
+                                const uint32_t n = min(32u, L - 32 * k);
+                                const uint64_t win = sh ? (g[k] << sh) | (g[k + 1] >> (64u - sh)) : g[k];
+                                c += mm_words(win >> ((32u - n) << 1), q[k]);
+                            }
+                        }
+                        if (c < last) {
+                            w = (RC ? 2ull : 0ull) | (mp << 2);
+                            last = c;
+                            if (last <= MAP_TARGET) goto done;
+                        }
+                    }
+                }
+            }
+        } else {
+            map_nread<RC>(a, i, last, w, seen, n_probe, n_cand);
+        }
+    done:
+        a.last[i] = (uint8_t)last;
+        a.word[i] = w;
+        if (last <= MAP_TARGET) {
+            n_first += (n_probe - probe0 == 1) ? 1u : 0u;
+        } else {
+            n_open_probe += n_probe - probe0;
+            n_open_cand += n_cand - cand0;
+        }
+    }
+    wave_add_u64(&a.work[0], n_act);
+    wave_add_u64(&a.work[1], n_probe);
+    wave_add_u64(&a.work[2], n_cand);
+    wave_add_u64(&a.work[3], n_first);
+    wave_add_u64(&a.work[4], n_open_probe);
+    wave_add_u64(&a.work[5], n_open_cand);
+}
+
+// Alignment records in the 16-byte layout of compress.cu:68-77, built on the
+// device so that one copy delivers them: read_id = id_map[rid[i]] (or
+// rid[i]), word = (v[i] << shift) | flag.
+__global__ void k_rec_aos(const uint32_t* __restrict__ rid, const uint64_t* __restrict__ v,
+                          const uint32_t* __restrict__ id_map, uint64_t n, uint32_t shift, uint64_t flag,
+                          curc_alignment* __restrict__ out) {
+    GRID_STRIDE(i, n) {
+        curc_alignment r;
+        r.read_id = id_map ? id_map[rid[i]] : rid[i];
+        r.pad = 0;
+        r.word = (v[i] << shift) | flag;
+        out[i] = r;
+    }
 }
 
 __global__ void k_map_init(uint8_t* __restrict__ last, uint64_t* __restrict__ word, uint64_t n_un, uint64_t n,

@@ -131,11 +131,184 @@ __device__ __forceinline__ void map_features(const MapArgs& a, const MapCounters
     for (uint32_t k = 0; k < MAP_NFEAT; ++k) o[k] = f[k];
 }
 
+// Bucket statistics of one probe.
+__device__ __forceinline__ void range_bucket(MapCounters& c, uint32_t bs) {
+    c.blast = bs;
+    c.bsum += bs;
+    if (bs == MAP_BUCKET) ++c.full;
+    if (bs == 0) ++c.empty;
+}
+
+// One candidate position mp after its mismatch count cm (exact when below
+// `last`, else the partial sum at which verification stopped); returns true
+// when the read settles (<= 1 mismatch).
+__device__ __forceinline__ bool range_candidate(MapCounters& c, uint32_t& last, uint64_t& w, uint64_t mp,
+                                                uint32_t cm, bool isN, bool rc, uint32_t t) {
+    if (cm <= last + 2) ++c.near;
+    if (cm < last) {
+        c.second = last;
+        w = (isN ? 1ull : 0ull) | (rc ? 2ull : 0ull) | (mp << 2);
+        last = cm;
+        c.tbest = t;
+        ++c.impr;
+        c.votes = 1;
+        return last <= MAP_TARGET;
+    }
+    if (cm < c.second) c.second = cm;
+    return false;
+}
+
+// A read's loop state between ranges.
+struct RangeState {
+    SeenCache seen;
+    MapCounters c;
+    __device__ void load(const MapState& s, uint64_t i) {
+        seen.s0 = s.seen[i];
+        seen.s1 = s.seen[s.n + i];
+        seen.s2 = s.seen[2 * s.n + i];
+        seen.s3 = s.seen[3 * s.n + i];
+        c.load(s, i);
+    }
+    __device__ void store(const MapState& s, uint64_t i) const {
+        s.seen[i] = seen.s0;
+        s.seen[s.n + i] = seen.s1;
+        s.seen[2 * s.n + i] = seen.s2;
+        s.seen[3 * s.n + i] = seen.s3;
+        c.store(s, i);
+    }
+};
+
+// Probes [t0, t1) of one read, generic form: words through memory, N-reads
+// base by base.  Returns with `last` / `w` / st updated.
+template <bool RC>
+__device__ __forceinline__ void range_read_generic(const MapArgs& a, uint64_t i, uint32_t t0, uint32_t t1,
+                                                   uint32_t& last, uint64_t& w, RangeState& st,
+                                                   uint32_t& n_probe, uint32_t& n_cand) {
+    const bool isN = i >= a.n_un;
+    MapCounters& c = st.c;
+    const uint64_t* rd = isN ? nullptr : a.R + (uint64_t)a.un_ids[i] * a.W;
+    const uint8_t* ra = isN ? a.nascii + (i - a.n_un) * a.L : nullptr;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t j = MAP_READ_STEP * t;
+        if (j + MAP_K > a.L) break;
+        ++n_probe;
+        const uint64_t km = isN ? kmer_ascii(ra + j, MAP_K) : xw(rd, j, MAP_K);
+        const uint32_t h = (uint32_t)(murmur64(km) & a.mask);
+        const MapBucket B = map_bucket(a, h);
+        range_bucket(c, B.n);
+        const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
+        for (uint32_t p = 0; p < B.n; ++p) {
+            const uint64_t rp = a.start + (uint64_t)ld32(B.p + p) * MAP_REF_STEP;
+            if (rp < ls) continue;
+            const uint64_t mp = rp - ls;
+            if (mp + a.L > a.pg_len) continue;
+            if (last <= a.max_mm && mp == (w >> 2)) ++c.votes;
+            if (st.seen.check_insert(mp)) continue;
+            ++n_cand;
+            ++c.cands;
+            uint32_t cm;
+            if (!isN) {
+                cm = RC ? rc_mm(a.pg, mp, rd, a.L, last - 1) : fwd_mm(a.pg, mp, rd, a.L, last - 1);
+            } else {
+                cm = 0;
+                for (uint32_t o = 0; o < a.L; ++o) {
+                    const uint8_t rb = base_char(base2(a.pg, mp + o));
+                    const uint8_t qb = RC ? rev_base_ascii(ra[a.L - 1 - o]) : ra[o];
+                    if (rb != qb) {
+                        cm++;
+                        if (cm >= last) break;
+                    }
+                }
+            }
+            if (range_candidate(c, last, w, mp, cm, isN, RC, t)) return;
+        }
+    }
+}
+
+// Probes [t0, t1) of one non-N read with the bucket-inline index and the
+// read in registers (as k_map_fast): the same candidates, counts and
+// partial sums as range_read_generic.
+template <bool RC, uint32_t WM>
+__device__ __forceinline__ void range_read_fast(const MapArgs& a, uint64_t i, uint32_t t0, uint32_t t1,
+                                                uint32_t& last, uint64_t& w, RangeState& st,
+                                                uint32_t& n_probe, uint32_t& n_cand) {
+    MapCounters& c = st.c;
+    const uint32_t L = a.L, W = a.W;
+    const uint32_t np = (L - MAP_K) / MAP_READ_STEP + 1;
+    t1 = min(t1, np);
+    const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * W;
+    uint64_t r[WM], q[WM];
+#pragma unroll
+    for (uint32_t k = 0; k < WM; ++k) r[k] = k < W ? ld64(rd + k) : 0ull;
+#pragma unroll
+    for (uint32_t k = 0; k < WM; ++k) {
+        q[k] = 0;
+        if (32 * k < L) {
+            const uint32_t n = min(32u, L - 32 * k);
+            q[k] = RC ? revcomp(xw_pick(r, L - 32 * k - n, n), n) : xw_pick(r, 32 * k, n);
+        }
+    }
+    for (uint32_t tg = t0; tg < t1; tg += MAP_GROUP) {
+        uint4 hd[MAP_GROUP];
+        uint32_t hh[MAP_GROUP];
+#pragma unroll
+        for (uint32_t u = 0; u < MAP_GROUP; ++u) {
+            hd[u] = make_uint4(0, 0, 0, 0);
+            hh[u] = 0;
+            if (tg + u < t1) {
+                hh[u] = (uint32_t)(murmur64(xw_pick(r, MAP_READ_STEP * (tg + u), MAP_K)) & a.mask);
+                hd[u] = *reinterpret_cast<const uint4*>(a.bk + (uint64_t)hh[u] * MAP_SLOT);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < MAP_GROUP; ++u) {
+            const uint32_t t = tg + u;
+            if (t >= t1) break;
+            ++n_probe;
+            const uint32_t nb = hd[u].x;
+            range_bucket(c, nb);
+            const uint32_t j = MAP_READ_STEP * t;
+            const uint64_t ls = RC ? (uint64_t)(L - MAP_K - j) : (uint64_t)j;
+            for (uint32_t p = 0; p < nb; ++p) {
+                const uint32_t pv = p == 0   ? hd[u].y
+                                    : p == 1 ? hd[u].z
+                                    : p == 2 ? hd[u].w
+                                             : ld32(a.bk + (uint64_t)hh[u] * MAP_SLOT + 1 + p);
+                const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
+                if (rp < ls) continue;
+                const uint64_t mp = rp - ls;
+                if (mp + L > a.pg_len) continue;
+                if (last <= a.max_mm && mp == (w >> 2)) ++c.votes;
+                if (st.seen.check_insert(mp)) continue;
+                ++n_cand;
+                ++c.cands;
+                const uint64_t* g0 = a.pg + (mp >> 5);
+                uint64_t g[WM + 1];
+#pragma unroll
+                for (uint32_t k = 0; k <= WM; ++k) g[k] = k <= W ? ld64(g0 + k) : 0ull;
+                const uint32_t sh = (uint32_t)(mp & 31u) << 1;
+                uint32_t cm = 0;
+                const uint32_t bound = last - 1;
+#pragma unroll
+                for (uint32_t k = 0; k < WM; ++k) {
+                    if (32 * k < L && cm <= bound) {
+                        const uint32_t n = min(32u, L - 32 * k);
+                        const uint64_t win = sh ? (g[k] << sh) | (g[k + 1] >> (64u - sh)) : g[k];
+                        cm += mm_words(win >> ((32u - n) << 1), q[k]);
+                    }
+                }
+                if (range_candidate(c, last, w, mp, cm, false, RC, t)) return;
+            }
+        }
+    }
+}
+
 // Probes [t0, t1) (probe t reads the k-mer at base 2t) of every read in the
-// active list, continuing k_map's loop from the saved state.
+// active list, continuing k_map's loop from the saved state.  WM > 0 takes
+// the register form for non-N reads (bucket-inline index, W <= WM).
 // With x, each read of the list also writes its features after the range
 // (row q of x, t1 probes done); reads already settled write a row too.
-template <bool RC>
+template <bool RC, uint32_t WM>
 __global__ void __launch_bounds__(256) k_map_range(MapArgs a, MapState s, const uint32_t* __restrict__ act,
                                                    const uint32_t* __restrict__ d_n, uint32_t t0, uint32_t t1,
                                                    float* __restrict__ x) {
@@ -144,88 +317,26 @@ __global__ void __launch_bounds__(256) k_map_range(MapArgs a, MapState s, const 
     GRID_STRIDE(q, n_act) {
         const uint64_t i = act[q];
         uint32_t last = a.last[i];
+        const bool isN = i >= a.n_un;
         if (last <= MAP_TARGET) {
             if (x) {
                 MapCounters c;
                 c.load(s, i);
-                map_features(a, c, last, i >= a.n_un, RC, t1, t0, x + q * MAP_NFEAT);
+                map_features(a, c, last, isN, RC, t1, t0, x + q * MAP_NFEAT);
             }
             continue;
         }
-        const bool isN = i >= a.n_un;
         uint64_t w = a.word[i];
-        SeenCache seen;
-        seen.s0 = s.seen[i];
-        seen.s1 = s.seen[s.n + i];
-        seen.s2 = s.seen[2 * s.n + i];
-        seen.s3 = s.seen[3 * s.n + i];
-        MapCounters c;
-        c.load(s, i);
-        const uint64_t* rd = isN ? nullptr : a.R + (uint64_t)a.un_ids[i] * a.W;
-        const uint8_t* ra = isN ? a.nascii + (i - a.n_un) * a.L : nullptr;
-        for (uint32_t t = t0; t < t1; ++t) {
-            const uint32_t j = MAP_READ_STEP * t;
-            if (j + MAP_K > a.L) break;
-            ++n_probe;
-            const uint64_t km = isN ? kmer_ascii(ra + j, MAP_K) : xw(rd, j, MAP_K);
-            const uint32_t h = (uint32_t)(murmur64(km) & a.mask);
-            uint32_t b = a.kr[h], e = a.kr[h + 1];
-            const uint32_t bs = min(e - b, MAP_BUCKET);
-            c.blast = bs;
-            c.bsum += bs;
-            if (bs == MAP_BUCKET) ++c.full;
-            if (b == e) {
-                ++c.empty;
-                continue;
-            }
-            e = b + bs;
-            const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
-            for (uint32_t p = b; p < e; ++p) {
-                const uint64_t rp = a.start + (uint64_t)a.pos[p] * MAP_REF_STEP;
-                if (rp < ls) continue;
-                const uint64_t mp = rp - ls;
-                if (mp + a.L > a.pg_len) continue;
-                if (last <= a.max_mm && mp == (w >> 2)) ++c.votes;
-                if (seen.check_insert(mp)) continue;
-                ++n_cand;
-                ++c.cands;
-                uint32_t cm;
-                if (!isN) {
-                    cm = RC ? rc_mm(a.pg, mp, rd, a.L, last - 1) : fwd_mm(a.pg, mp, rd, a.L, last - 1);
-                } else {
-                    cm = 0;
-                    for (uint32_t o = 0; o < a.L; ++o) {
-                        const uint8_t rb = base_char(base2(a.pg, mp + o));
-                        const uint8_t qb = RC ? rev_base_ascii(ra[a.L - 1 - o]) : ra[o];
-                        if (rb != qb) {
-                            cm++;
-                            if (cm >= last) break;
-                        }
-                    }
-                }
-                if (cm <= last + 2) ++c.near;
-                if (cm < last) {
-                    c.second = last;
-                    w = (isN ? 1ull : 0ull) | (RC ? 2ull : 0ull) | (mp << 2);
-                    last = cm;
-                    c.tbest = t;
-                    ++c.impr;
-                    c.votes = 1;
-                    if (last <= MAP_TARGET) goto done;
-                } else if (cm < c.second) {
-                    c.second = cm;
-                }
-            }
-        }
-    done:
+        RangeState st;
+        st.load(s, i);
+        if (WM > 0 && !isN)
+            range_read_fast<RC, (WM > 0 ? WM : 1)>(a, i, t0, t1, last, w, st, n_probe, n_cand);
+        else
+            range_read_generic<RC>(a, i, t0, t1, last, w, st, n_probe, n_cand);
         a.last[i] = (uint8_t)last;
         a.word[i] = w;
-        s.seen[i] = seen.s0;
-        s.seen[s.n + i] = seen.s1;
-        s.seen[2 * s.n + i] = seen.s2;
-        s.seen[3 * s.n + i] = seen.s3;
-        c.store(s, i);
-        if (x) map_features(a, c, last, isN, RC, t1, t0, x + q * MAP_NFEAT);
+        st.store(s, i);
+        if (x) map_features(a, st.c, last, isN, RC, t1, t0, x + q * MAP_NFEAT);
     }
     if (a.work) {
         wave_add_u64(&a.work[1], n_probe);

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <chrono>
 #include <numeric>
+#include <cstdlib>
 #include <thread>
 #include <cstring>
 #include <string>
@@ -20,6 +21,7 @@
 #include "mapphase.cuh"
 #include "mlp.cuh"
 #include "refmatch.cuh"
+#include "host_pool.h"
 #include "streams.cuh"
 
 using namespace curc;
@@ -34,8 +36,8 @@ enum Slot {
     S_PG_HQ, S_PG_LQ, S_PGIDS, S_PGPOS, S_PGNB, S_SORTED,
     S_A_RANK, S_A_NXT, S_A_PRV, S_A_OFF, S_A_ST0, S_A_ST1, S_A_MEMB, S_A_CKEY, S_A_CLEN, S_A_CCNT,
     S_A_HK0, S_A_HK1, S_A_HV0, S_A_HV1, S_A_OLEN, S_A_OCNT, S_A_CPOS, S_A_CREC,
-    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_LAST, S_WORD,
-    S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N,
+    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_LAST, S_WORD,
+    S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N, S_RECAOS,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
     S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG, S_POOL,
     S_LQIDS, S_LQPOS, S_LQNB, S_NLQIDS, S_NLQPOS, S_NLQNB,
@@ -104,6 +106,9 @@ struct curc_ctx {
     std::string err;
     bool timing = false;
     std::vector<curc_ctx*> workers;  // extra stream/arena pairs of curc_batch_run
+    // mapping index layout: CSR instead of 64-byte bucket slots (environment
+    // CURC_MAP_INDEX=csr at context creation; the layout a huge index uses)
+    bool map_csr = false;
     // learned mapping (CURC_RUN_LEARNED): the scorer's device arrays
     bool has_scorer = false;
     MlpDev scorer_net{};
@@ -189,8 +194,9 @@ uint32_t read_cnt(curc_ctx* c, const uint32_t* p) {
     return v;
 }
 
-template <typename T>
-void d2h(curc_ctx* c, std::vector<T>& out, const T* src, uint64_t n) {
+template <typename V, typename T>
+void d2h(curc_ctx* c, V& out, const T* src, uint64_t n) {
+    static_assert(sizeof(typename V::value_type) == sizeof(T), "element size");
     out.resize(n);
     if (n) CURC_HIP(hipMemcpyAsync(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, c->st));
 }
@@ -214,44 +220,44 @@ struct AsmOut {
 struct curc_block {
     uint32_t L = 0, W = 0;
     uint64_t n_reads = 0;
-    std::vector<uint64_t> packed;
-    std::vector<uint32_t> nonN, Nids, sorted;
-    std::vector<uint32_t> next_dedup, prev_dedup, next_ov, prev_ov, next_pr, prev_pr;
-    std::vector<uint16_t> off_ov;
-    std::vector<uint32_t> next, prev;
-    std::vector<uint16_t> offset;
-    std::vector<uint32_t> hq, unmatched;
-    std::vector<uint64_t> hq_pg2, lq_pg2;
+    hvec<uint64_t> packed;
+    hvec<uint32_t> nonN, Nids, sorted;
+    hvec<uint32_t> next_dedup, prev_dedup, next_ov, prev_ov, next_pr, prev_pr;
+    hvec<uint16_t> off_ov;
+    hvec<uint32_t> next, prev;
+    hvec<uint16_t> offset;
+    hvec<uint32_t> hq, unmatched;
+    hvec<uint64_t> hq_pg2, lq_pg2;
     uint64_t hq_len = 0, lq_len = 0;
     std::string hq_text, lq_text;
     bool hq_text_ok = false, lq_text_ok = false;
-    std::vector<curc_alignment> hq_rec, map_rec, lq_rec;
-    std::vector<uint8_t> map_mis;
-    std::vector<uint32_t> unmapped, unmapped_N;
-    std::vector<uint32_t> lq_next, lq_prev;
-    std::vector<uint16_t> lq_off;
-    std::vector<char> Nascii;
+    hvec<curc_alignment> hq_rec, map_rec, lq_rec;
+    hvec<uint8_t> map_mis;
+    hvec<uint32_t> unmapped, unmapped_N;
+    hvec<uint32_t> lq_next, lq_prev;
+    hvec<uint16_t> lq_off;
+    hvec<char> Nascii;
     std::string nlq_text;
-    std::vector<curc_alignment> nlq_rec;
+    hvec<curc_alignment> nlq_rec;
     // pre-entropy streams (CURC_RUN_STREAMS)
-    std::vector<char> s_strand;
-    std::vector<uint16_t> s_read_off, s_lq_off, s_nlq_off, s_mis_off;
-    std::vector<uint8_t> s_mis_count, s_mis_base, s_mis_table;
-    std::vector<uint32_t> s_pe_order;
-    std::vector<uint64_t> s_id_pos, s_mis_off_start;
-    std::vector<curc_alignment> s_sorted;
+    hvec<char> s_strand;
+    hvec<uint16_t> s_read_off, s_lq_off, s_nlq_off, s_mis_off;
+    hvec<uint8_t> s_mis_count, s_mis_base, s_mis_table;
+    hvec<uint32_t> s_pe_order;
+    hvec<uint64_t> s_id_pos, s_mis_off_start;
+    hvec<curc_alignment> s_sorted;
     // traced mapping (CURC_RUN_TRACE): scorer features and labels
-    std::vector<float> trace_x, trace_y;
+    hvec<float> trace_x, trace_y;
     // pseudogenome self-match (CURC_RUN_REFMATCH): parts HQ, LQ, N-read LQ
     struct RmPart {
         std::string text, map_off, map_len;
-        std::vector<uint64_t> matches, found;  // (src, len, dest) triples, with KEEP_STAGES
+        hvec<uint64_t> matches, found;  // (src, len, dest) triples, with KEEP_STAGES
     } rm[3];
 };
 
 namespace {
 
-std::string decode_pg(const std::vector<uint64_t>& pg2, uint64_t len) {
+std::string decode_pg(const hvec<uint64_t>& pg2, uint64_t len) {
     static const char D[4] = {'A', 'C', 'G', 'T'};
     std::string s(len, 'A');
     for (uint64_t i = 0; i < len; ++i) s[i] = D[(pg2[i >> 5] >> (2 * (31 - (i & 31)))) & 3];
@@ -358,18 +364,14 @@ AsmOut assemble(curc_ctx* c, const uint32_t* V, uint32_t n, uint64_t id_space, c
     return out;
 }
 
-void records_to_host(curc_ctx* c, const AsmOut& a, const uint32_t* id_map, std::vector<curc_alignment>& rec) {
-    std::vector<uint32_t> ids;
-    std::vector<uint64_t> pos;
-    d2h(c, ids, a.rid, a.n);
-    d2h(c, pos, a.pos, a.n);
-    CURC_HIP(hipStreamSynchronize(c->st));
-    rec.resize(a.n);
-    for (uint32_t i = 0; i < a.n; ++i) {
-        rec[i].read_id = id_map ? id_map[ids[i]] : ids[i];
-        rec[i].pad = 0;
-        rec[i].word = pos[i] << 2;
-    }
+// Records of an assembly in host memory (word = pos << 2 | flag; read ids
+// through the device array id_map when given).
+void records_to_host(curc_ctx* c, const AsmOut& a, const uint32_t* id_map, uint64_t flag, hvec<curc_alignment>& rec) {
+    curc_alignment* d = c->S.as<curc_alignment>(S_RECAOS, (uint64_t)a.n + 1);
+    if (a.n)
+        hipLaunchKernelGGL(k_rec_aos, dim3(grid_for(a.n)), dim3(256), 0, c->st, (const uint32_t*)a.rid,
+                           (const uint64_t*)a.pos, id_map, (uint64_t)a.n, 2u, flag, d);
+    d2h(c, rec, (const curc_alignment*)d, a.n);
 }
 
 // Exact all-offset matching (a8/a11 over 2-bit reads, compress.cu:1003-1174;
@@ -484,6 +486,50 @@ RmIndex rm_index(curc_ctx* c, const uint64_t* src, uint64_t src_len) {
     return ix;
 }
 
+// k_map_fast for the read's word count W <= 8 (register capacity 3, 4, 5 or 8).
+void launch_map_fast(bool rc, uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
+#define CURC_MAP_FAST(WM)                                                            \
+    do {                                                                             \
+        if (rc)                                                                      \
+            hipLaunchKernelGGL((k_map_fast<true, WM>), grid, dim3(256), 0, st, a);   \
+        else                                                                         \
+            hipLaunchKernelGGL((k_map_fast<false, WM>), grid, dim3(256), 0, st, a);  \
+    } while (0)
+    if (W <= 3)
+        CURC_MAP_FAST(3);
+    else if (W == 4)
+        CURC_MAP_FAST(4);
+    else if (W == 5)
+        CURC_MAP_FAST(5);
+    else
+        CURC_MAP_FAST(8);
+#undef CURC_MAP_FAST
+}
+
+// k_map_range in register form for W <= 8 with the bucket-inline index
+// (wbk = W; 0 for the CSR index), else in generic form.
+void launch_map_range(bool rc, uint32_t wbk, dim3 grid, hipStream_t st, const MapArgs& a, const MapState& ms,
+                      const uint32_t* act, const uint32_t* nact, uint32_t t0, uint32_t t1, float* x) {
+#define CURC_MAP_RANGE(WM)                                                                                   \
+    do {                                                                                                     \
+        if (rc)                                                                                              \
+            hipLaunchKernelGGL((k_map_range<true, WM>), grid, dim3(256), 0, st, a, ms, act, nact, t0, t1, x);  \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_map_range<false, WM>), grid, dim3(256), 0, st, a, ms, act, nact, t0, t1, x); \
+    } while (0)
+    if (wbk == 0 || wbk > 8)
+        CURC_MAP_RANGE(0);
+    else if (wbk <= 3)
+        CURC_MAP_RANGE(3);
+    else if (wbk == 4)
+        CURC_MAP_RANGE(4);
+    else if (wbk == 5)
+        CURC_MAP_RANGE(5);
+    else
+        CURC_MAP_RANGE(8);
+#undef CURC_MAP_RANGE
+}
+
 template <typename T>
 T read_one(curc_ctx* c, const T* p) {
     T v{};
@@ -492,7 +538,7 @@ T read_one(curc_ctx* c, const T* p) {
     return v;
 }
 
-void put_triples(curc_ctx* c, std::vector<uint64_t>& out, const uint64_t* s, const uint64_t* l, const uint64_t* d,
+void put_triples(curc_ctx* c, hvec<uint64_t>& out, const uint64_t* s, const uint64_t* l, const uint64_t* d,
                  uint64_t n) {
     std::vector<uint64_t> a, b, e;
     d2h(c, a, s, n);
@@ -718,14 +764,7 @@ void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* c
         const bool last_range = t1 >= nprobe;
         // the range kernel writes each listed read's features at its end
         float* Xr = last_range ? nullptr : X;
-        if (t1 > t0) {
-            if (rc)
-                hipLaunchKernelGGL(k_map_range<true>, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
-                                   (const uint32_t*)nact[cur], t0, t1, Xr);
-            else
-                hipLaunchKernelGGL(k_map_range<false>, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[cur],
-                                   (const uint32_t*)nact[cur], t0, t1, Xr);
-        }
+        if (t1 > t0) launch_map_range(rc, a.bk ? a.W : 0, dim3(g), st, a, ms, act[cur], nact[cur], t0, t1, Xr);
         if (last_range) break;
         if (trace)
             hipLaunchKernelGGL(k_map_trace_emit, dim3(g), dim3(256), 0, st, (const uint32_t*)act[cur],
@@ -1133,7 +1172,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const AsmOut& a = hq_asm;
         B->hq_len = a.len;
         hq_pg = build_pg(c, packed, W, a, S_PG_HQ);
-        records_to_host(c, a, nullptr, B->hq_rec);
+        records_to_host(c, a, nullptr, 0, B->hq_rec);
         d2h(c, B->hq_pg2, hq_pg, (a.len + 31) / 32 + 1);
         CURC_HIP(hipStreamSynchronize(st));
     }
@@ -1174,8 +1213,11 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 for (int pi = 0; pi < np; ++pi) {
                     const uint64_t start = parts[pi][0], count = parts[pi][1];
                     const uint32_t H = hash_size(count);
-                    uint32_t* kr = S.as<uint32_t>(S_MKR, (uint64_t)H + 2);
-                    uint32_t* pos = S.as<uint32_t>(S_MPOS, count + 2);
+                    // buckets inline in 64-byte slots up to 2^28 of them (16 GiB), CSR beyond
+                    const bool inline_bk = H <= MAP_SLOT_MAX_H && !c->map_csr;
+                    uint32_t* kr = inline_bk ? nullptr : S.as<uint32_t>(S_MKR, (uint64_t)H + 2);
+                    uint32_t* pos = inline_bk ? nullptr : S.as<uint32_t>(S_MPOS, count + 2);
+                    uint32_t* bk = inline_bk ? S.as<uint32_t>(S_MBK, (uint64_t)H * MAP_SLOT) : nullptr;
                     {
                         Region r(c, "map_index");
                         uint32_t* k0 = S.as<uint32_t>(S_MK0, count + 1);
@@ -1189,13 +1231,22 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         radix_sort_pairs<uint32_t>(k0, v0, k1, v1, count, 0, log2u(H), S, S_PRIM, st);
                         hipLaunchKernelGGL(k_gap_fill, dim3(grid_for(count + 1)), dim3(256), 0, st,
                                            (const uint32_t*)k0, count, (uint64_t)H + 1, krf);
-                        hipLaunchKernelGGL(k_map_caps, dim3(grid_for((uint64_t)H + 2)), dim3(256), 0, st,
-                                           (const uint32_t*)krf, (uint64_t)H, kr);
-                        scan_excl(kr, kr, (uint64_t)H + 2, S, S_PRIM, st);
-                        if (count)
-                            hipLaunchKernelGGL(k_map_fill, dim3(grid_for(count)), dim3(256), 0, st,
-                                               (const uint32_t*)k0, (const uint32_t*)v0, count,
-                                               (const uint32_t*)krf, (const uint32_t*)kr, pos);
+                        if (inline_bk) {
+                            hipLaunchKernelGGL(k_map_slot_counts, dim3(grid_for(H)), dim3(256), 0, st,
+                                               (const uint32_t*)krf, (uint64_t)H, bk);
+                            if (count)
+                                hipLaunchKernelGGL(k_map_slot_fill, dim3(grid_for(count)), dim3(256), 0, st,
+                                                   (const uint32_t*)k0, (const uint32_t*)v0, count,
+                                                   (const uint32_t*)krf, bk);
+                        } else {
+                            hipLaunchKernelGGL(k_map_caps, dim3(grid_for((uint64_t)H + 2)), dim3(256), 0, st,
+                                               (const uint32_t*)krf, (uint64_t)H, kr);
+                            scan_excl(kr, kr, (uint64_t)H + 2, S, S_PRIM, st);
+                            if (count)
+                                hipLaunchKernelGGL(k_map_fill, dim3(grid_for(count)), dim3(256), 0, st,
+                                                   (const uint32_t*)k0, (const uint32_t*)v0, count,
+                                                   (const uint32_t*)krf, (const uint32_t*)kr, pos);
+                        }
                         c->add_bytes("map_index", (double)count * (16 + 8) + 8.0 * H, (double)count);
                     }
                     unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
@@ -1203,16 +1254,19 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     if (phased) {
                         Region r(c, learned ? "mapping_learned" : "mapping_traced");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
-                                  Nascii, nN, last, word, work, P.max_mm};
+                                  Nascii, nN, last, word, work, P.max_mm, bk};
                         map_phased(c, a, rc, nm, cnt, learned, trace);
                     } else {
                         Region r(c, "mapping");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
-                                  Nascii, nN, last, word, work};
-                        if (rc)
-                            hipLaunchKernelGGL(k_map<true>, dim3(grid_for(nm, 1u << 20)), dim3(256), 0, st, a);
+                                  Nascii, nN, last, word, work, P.max_mm, bk};
+                        const dim3 grid(grid_for(nm, 1u << 20));
+                        if (bk && W <= 8)
+                            launch_map_fast(rc, W, grid, st, a);
+                        else if (rc)
+                            hipLaunchKernelGGL(k_map<true>, grid, dim3(256), 0, st, a);
                         else
-                            hipLaunchKernelGGL(k_map<false>, dim3(grid_for(nm, 1u << 20)), dim3(256), 0, st, a);
+                            hipLaunchKernelGGL(k_map<false>, grid, dim3(256), 0, st, a);
                     }
                     if (c->timing) {
                         // algorithmic bytes of one launch: each active read's words, its
@@ -1252,10 +1306,11 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const uint32_t nmap = read_cnt(c, cnt + C_NMAP);
         const uint32_t nun = read_cnt(c, cnt + C_NUNMAP);
         const uint32_t nunN = read_cnt(c, cnt + C_NUNMAPN);
-        std::vector<uint32_t> hid;
-        std::vector<uint64_t> hw;
-        d2h(c, hid, rid, nmap);
-        d2h(c, hw, rw, nmap);
+        curc_alignment* drec = S.as<curc_alignment>(S_RECAOS, (uint64_t)nmap + 1);
+        if (nmap)
+            hipLaunchKernelGGL(k_rec_aos, dim3(grid_for(nmap)), dim3(256), 0, st, (const uint32_t*)rid,
+                               (const uint64_t*)rw, (const uint32_t*)nullptr, (uint64_t)nmap, 0u, 0ull, drec);
+        d2h(c, B->map_rec, (const curc_alignment*)drec, nmap);
         d2h(c, B->map_mis, mis, nmap);
         d2h(c, B->unmapped, um, nun);
         d2h(c, B->unmapped_N, umN, nunN);
@@ -1265,12 +1320,6 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             d2h(c, B->trace_x, (const float*)S.as<float>(S_TRX, c->trace_cap * MAP_NFEAT), nr * MAP_NFEAT);
             d2h(c, B->trace_y, (const float*)S.as<float>(S_TRY, c->trace_cap), nr);
             CURC_HIP(hipStreamSynchronize(st));
-        }
-        B->map_rec.resize(nmap);
-        for (uint32_t i = 0; i < nmap; ++i) {
-            B->map_rec[i].read_id = hid[i];
-            B->map_rec[i].pad = 0;
-            B->map_rec[i].word = hw[i];
         }
     }
     if (stop <= CURC_STAGE_MAP) return;
@@ -1304,7 +1353,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const AsmOut& a = lq_asm;
         B->lq_len = a.len;
         uint64_t* lpg = build_pg(c, lqp, W, a, S_PG_LQ);
-        records_to_host(c, a, B->unmapped.data(), B->lq_rec);
+        records_to_host(c, a, um, 0, B->lq_rec);
         d2h(c, B->lq_pg2, lpg, (a.len + 31) / 32 + 1);
         CURC_HIP(hipStreamSynchronize(st));
     }
@@ -1363,8 +1412,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                                (uint64_t)a.n, npg);
         B->nlq_text.resize(a.len);
         if (a.len) CURC_HIP(hipMemcpyAsync(&B->nlq_text[0], npg, a.len, hipMemcpyDeviceToHost, st));
-        records_to_host(c, a, nullptr, B->nlq_rec);
-        for (auto& rr : B->nlq_rec) rr.word |= 1;  // is_contain_N
+        records_to_host(c, a, nullptr, 1, B->nlq_rec);  // is_contain_N
         CURC_HIP(hipStreamSynchronize(st));
     }
 
@@ -1433,6 +1481,8 @@ curc_ctx* curc_ctx_create(int device) {
     if (hipSetDevice(device) != hipSuccess) return nullptr;
     curc_ctx* c = new curc_ctx();
     c->device = device;
+    const char* mi = std::getenv("CURC_MAP_INDEX");
+    c->map_csr = mi && std::string(mi) == "csr";
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return nullptr;
@@ -1463,6 +1513,8 @@ int curc_ctx_trim(curc_ctx* c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
     c->S.release();
+    for (curc_ctx* w : c->workers) curc_ctx_trim(w);
+    HostPool::get().trim();
     return CURC_OK;
 }
 
@@ -1714,6 +1766,7 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
     for (uint32_t w = 1; w < workers; ++w) {
         curc_ctx* wc = c->workers[w - 1];
         wc->timing = c->timing;
+        wc->map_csr = c->map_csr;
         // workers share the scorer's device arrays (read-only during mapping)
         wc->has_scorer = c->has_scorer;
         wc->scorer_net = c->scorer_net;

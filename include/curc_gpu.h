@@ -58,7 +58,9 @@ int curc_device_count(int *count);
 curc_ctx *curc_ctx_create(int device);
 void curc_ctx_destroy(curc_ctx *ctx);
 const char *curc_ctx_last_error(curc_ctx *ctx);
-/* Drops cached device scratch (it is otherwise kept for the next block). */
+/* Drops cached device scratch (it is otherwise kept for the next block) and
+ * returns the pinned host chunks that no live block holds.  Large block
+ * results live in pinned host memory that is recycled across blocks. */
 int curc_ctx_trim(curc_ctx *ctx);
 
 /* Memory helpers; replace gpu_malloc_host / gpu_malloc / gpu_free_host /

@@ -3,6 +3,8 @@
 Every comparison is exact (integer/byte work).  Intermediate link arrays are
 compared stage by stage, so a failure names the first stage that diverges.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -55,7 +57,7 @@ def test_config_small(gpu_ctx):
     run_both(gpu_ctx, ca.synth_reads(ca.config_small())[1])
 
 
-@pytest.mark.parametrize("L", [75, 100, 250])
+@pytest.mark.parametrize("L", [75, 100, 250, 300])
 def test_read_lengths(gpu_ctx, L):
     cfg = ca.synth_config(seed=200 + L, genome_len=40_000, gc=0.48, n_reads=6_000, L=L,
                           sub_first=0.002, sub_last=0.01, n_rate=0.0003)
@@ -111,3 +113,21 @@ def test_determinism(gpu_ctx):
     b = gpu_ctx.run_block(reads)
     for f in ("next", "prev", "offset", "hq_pg", "map_rec", "map_mis", "lq_pg", "lq_rec"):
         assert np.array_equal(a.get(f), b.get(f)), f
+
+
+def test_mapping_with_csr_index():
+    """The CSR mapping index (taken when the bucket slots would pass 16 GiB;
+    forced here by CURC_MAP_INDEX=csr) gives the same results."""
+    cfg = ca.synth_config(seed=303, genome_len=60_000, gc=0.45, n_reads=9_000, L=150,
+                          sub_first=0.002, sub_last=0.012, n_rate=0.0005)
+    reads = ca.synth_reads(cfg)[1]
+    old = os.environ.get("CURC_MAP_INDEX")
+    os.environ["CURC_MAP_INDEX"] = "csr"
+    try:
+        with ca.Context(0) as ctx:
+            run_both(ctx, reads)
+    finally:
+        if old is None:
+            del os.environ["CURC_MAP_INDEX"]
+        else:
+            os.environ["CURC_MAP_INDEX"] = old
