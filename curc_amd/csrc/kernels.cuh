@@ -159,9 +159,20 @@ struct EmitGatherKey {
 };
 
 // -------------------------------------------------------- prefix index ---
+// Bucket of each prefix's k-mer; with filt, also the k-mer filter bit
+// (hash bits 32..63 under fmask): a suffix whose k-mer bit is clear has no
+// prefix with that k-mer, hence none equal to it, at any offset.
 __global__ void k_prefix_hash(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids, uint64_t n,
-                              uint32_t W, uint32_t kip, uint32_t k, uint32_t mask, uint32_t* __restrict__ hk) {
-    GRID_STRIDE(i, n) hk[i] = (uint32_t)(murmur64(xw(R + (uint64_t)ids[i] * W, kip, k)) & mask);
+                              uint32_t W, uint32_t kip, uint32_t k, uint32_t mask, uint32_t* __restrict__ hk,
+                              uint32_t* __restrict__ filt, uint32_t fmask) {
+    GRID_STRIDE(i, n) {
+        const uint64_t m = murmur64(xw(R + (uint64_t)ids[i] * W, kip, k));
+        hk[i] = (uint32_t)(m & mask);
+        if (filt) {
+            const uint32_t f = (uint32_t)(m >> 32) & fmask;
+            atomicOr(&filt[f >> 5], 1u << (f & 31u));
+        }
+    }
 }
 
 // CSR offsets from sorted keys: kr[b] = first index with key >= b, for
@@ -295,9 +306,43 @@ struct PredFlagClear {
     __device__ bool operator()(uint32_t i) const { return flags[i] == 0; }
 };
 
+// Prefix-index bucket slots: one 64-byte line per hash bucket holding the
+// bucket's size and start in the hash-ordered prefix arrays, and the ids
+// and first words of its first PSLOT_N entries, so that a probe of a small
+// bucket reads one line instead of the CSR offsets plus the arrays.
+constexpr uint32_t PSLOT_N = 4;
+struct alignas(64) PrefixSlot {
+    uint32_t cnt, lo;
+    uint32_t id[PSLOT_N];
+    uint64_t key[PSLOT_N];
+    uint32_t pad[2];
+};
+static_assert(sizeof(PrefixSlot) == 64, "prefix slot size");
+constexpr uint64_t PSLOT_MAX_H = 1ull << 28;
+
+__global__ void k_prefix_slots(const uint32_t* __restrict__ kr, const uint32_t* __restrict__ pl,
+                               const uint64_t* __restrict__ pk, uint64_t H, PrefixSlot* __restrict__ slots) {
+    GRID_STRIDE(h, H) {
+        PrefixSlot sl;
+        sl.lo = kr[h];
+        sl.cnt = kr[h + 1] - sl.lo;
+#pragma unroll
+        for (uint32_t k = 0; k < PSLOT_N; ++k) {
+            sl.id[k] = k < sl.cnt ? pl[sl.lo + k] : INV;
+            sl.key[k] = k < sl.cnt ? pk[sl.lo + k] : 0ull;
+        }
+        sl.pad[0] = sl.pad[1] = 0;
+        slots[h] = sl;
+    }
+}
+
 // a6 probe (compress.cu:1474-1528): hash the k-mer at kip+off, binary
 // search the bucket, take the first free non-self candidate if it matches.
-__global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t* __restrict__ kr, uint32_t mask,
+// With slots, buckets of up to PSLOT_N entries are searched in registers
+// (the same lower bound and the same first-free scan).
+__global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t* __restrict__ kr,
+                                                  const PrefixSlot* __restrict__ slots,
+                                                  const uint32_t* __restrict__ filt, uint32_t fmask, uint32_t mask,
                                                   uint32_t k, uint32_t kip, uint32_t off) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
     const uint32_t ns = *cs.d_ns;
@@ -307,9 +352,63 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
         cs.slinked[i] = 0;
         uint32_t s = cs.sl[i];
         const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-        uint32_t h = (uint32_t)(murmur64(xw(rs, kip + off, k)) & mask);
-        uint32_t lo = kr[h], e = kr[h + 1];
+        const uint64_t m = murmur64(xw(rs, kip + off, k));
+        if (filt) {
+            // no prefix has this k-mer: no candidate (exact, no false negatives)
+            const uint32_t f = (uint32_t)(m >> 32) & fmask;
+            if (((filt[f >> 5] >> (f & 31u)) & 1u) == 0) {
+                cs.choice[i] = INV;
+                continue;
+            }
+        }
+        uint32_t h = (uint32_t)(m & mask);
         uint64_t sx0 = xw(rs, off, n1);
+        uint32_t lo, e;
+        if (slots) {
+            const PrefixSlot sl = slots[h];
+            lo = sl.lo;
+            e = lo + sl.cnt;
+            if (sl.cnt <= PSLOT_N) {
+                // lower bound: first entry not below s[off:L]
+                uint32_t b = sl.cnt;
+                bool found = false;
+#pragma unroll
+                for (uint32_t t = 0; t < PSLOT_N; ++t) {
+                    if (!found && t < sl.cnt &&
+                        cmp_ps<Enc2>(cs.R, cs.W, sl.id[t], sl.key[t], rs, sx0, off, len) >= 0) {
+                        b = t;
+                        found = true;
+                    }
+                }
+                // first entry from there that is neither s nor taken decides;
+                // entries are sorted, so past the equal run nothing can match
+                // and the taken flags need not be read
+                uint32_t ch = INV;
+                bool done = false;
+#pragma unroll
+                for (uint32_t t = 0; t < PSLOT_N; ++t) {
+                    if (!done && t >= b && t < sl.cnt) {
+                        const uint32_t p = sl.id[t];
+                        if (cmp_ps<Enc2>(cs.R, cs.W, p, sl.key[t], rs, sx0, off, len) != 0) {
+                            done = true;
+                        } else if (p != s && cs.prev[p] == INV) {
+                            ch = lo + t;
+                            done = true;
+                        }
+                    }
+                }
+                cs.choice[i] = ch;
+                cs.grp[i] = lo + b;
+                if (ch != INV) {
+                    uint32_t old = atomicCAS(&cs.owner[ch], INV, (uint32_t)i);
+                    if (old != INV) cs.contended[lo + b] = 1;
+                }
+                continue;
+            }
+        } else {
+            lo = kr[h];
+            e = kr[h + 1];
+        }
         uint32_t cnt = e - lo;
         while (cnt > 0) {
             uint32_t step = cnt >> 1, it = lo + step;
@@ -320,11 +419,14 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
                 cnt = step;
             }
         }
+        // the first entry that is neither s nor taken decides; past the run of
+        // entries equal to s[off:L] nothing can match (sorted bucket)
         uint32_t ch = INV;
         for (uint32_t x = lo; x < e; ++x) {
             uint32_t p = cs.pl[x];
+            if (cmp_ps<Enc2>(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) != 0) break;
             if (p == s || cs.prev[p] != INV) continue;
-            if (cmp_ps<Enc2>(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) == 0) ch = x;
+            ch = x;
             break;
         }
         cs.choice[i] = ch;

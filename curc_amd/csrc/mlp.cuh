@@ -173,29 +173,102 @@ __device__ __forceinline__ void mlp_stage_weights(const MlpDev& net, MlpLds& L) 
     }
 }
 
-// Inference over n samples (x: float32 [n][32]); writes logits.  Blocks of
-// 4 waves, one 32-sample tile per wave per step.
-__global__ void __launch_bounds__(256) k_mlp_infer(MlpDev net, const float* __restrict__ x, uint64_t n_host,
-                                                   const uint32_t* __restrict__ d_n, float* __restrict__ logit) {
-    __shared__ MlpLds L;
+// Inference over n samples (x: float32 [n][32]); writes logits.
+// Workgroups of MLP_INF_WAVES waves share one LDS copy of the bf16 weights;
+// each wave keeps only its tile's hidden layer in LDS (8 KB, reused for
+// the logit partials), and takes its tile's inputs straight from global
+// memory into the MFMA operand registers.  136 KB of LDS per workgroup,
+// one workgroup per CU, MLP_INF_WAVES waves per CU.
+constexpr uint32_t MLP_INF_WAVES = 12;
+
+struct alignas(16) MlpInferLds {
+    alignas(16) uint16_t w1t[MLP_H * MLP_IN];
+    alignas(16) uint16_t w2t[MLP_H * MLP_H];
+    alignas(16) uint16_t h1[MLP_INF_WAVES][MLP_TILE * MLP_H];
+};
+static_assert(sizeof(float) * MLP_TILE * 33 <= sizeof(uint16_t) * MLP_TILE * MLP_H, "logit partials fit h1");
+
+inline uint32_t mlp_infer_grid(uint64_t n) {
+    const uint64_t tiles = (n + MLP_TILE - 1) / MLP_TILE;
+    const uint64_t wg = (tiles + MLP_INF_WAVES - 1) / MLP_INF_WAVES;
+    return (uint32_t)(wg < 1 ? 1 : (wg > 512 ? 512 : wg));
+}
+
+// A operand of layer 1 for this lane: features 8*(lane/32) + k0 + 0..7 of
+// sample s0 + lane % 32 (zeros past n).
+__device__ __forceinline__ bf16x8 mlp_load_x(const float* __restrict__ x, uint64_t n, uint64_t s0, uint32_t k0) {
+    const uint32_t lane = __lane_id();
+    const uint64_t smp = s0 + (lane & 31u);
+    bf16x8 a;
+    if (smp < n) {
+        const float4* q = reinterpret_cast<const float4*>(x + smp * MLP_IN + k0 + 8u * (lane >> 5));
+        const float4 u = q[0], v = q[1];
+        const uint16_t b[8] = {f2bf(u.x), f2bf(u.y), f2bf(u.z), f2bf(u.w), f2bf(v.x), f2bf(v.y), f2bf(v.z), f2bf(v.w)};
+        uint4 p;
+        p.x = (uint32_t)b[0] | ((uint32_t)b[1] << 16);
+        p.y = (uint32_t)b[2] | ((uint32_t)b[3] << 16);
+        p.z = (uint32_t)b[4] | ((uint32_t)b[5] << 16);
+        p.w = (uint32_t)b[6] | ((uint32_t)b[7] << 16);
+        a = __builtin_bit_cast(bf16x8, p);
+    } else {
+        a = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+    }
+    return a;
+}
+
+__global__ void __launch_bounds__(64 * MLP_INF_WAVES) k_mlp_infer(MlpDev net, const float* __restrict__ x,
+                                                                   uint64_t n_host, const uint32_t* __restrict__ d_n,
+                                                                   float* __restrict__ logit) {
+    __shared__ MlpInferLds L;
     const uint64_t n = d_n ? (uint64_t)*d_n : n_host;
-    mlp_stage_weights(net, L);
+    for (uint32_t i = threadIdx.x; i < MLP_H * MLP_IN; i += blockDim.x) L.w1t[i] = net.w1t[i];
+    for (uint32_t i = threadIdx.x; i < MLP_H * MLP_H; i += blockDim.x) L.w2t[i] = net.w2t[i];
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+    const uint32_t lr = lane & 31u, lk = 8u * (lane >> 5);
+    const float* b1 = net.p + MLP_OFF_B1;
+    const float* b2 = net.p + MLP_OFF_B2;
+    const float* w3 = net.p + MLP_OFF_W3;
     const float b3 = net.p[MLP_OFF_B3];
+    uint16_t* h1 = L.h1[w];
     const uint64_t ntile = (n + MLP_TILE - 1) / MLP_TILE;
-    for (uint64_t t = (uint64_t)blockIdx.x * 4 + w; t < ntile; t += (uint64_t)gridDim.x * 4) {
+    for (uint64_t t = (uint64_t)blockIdx.x * MLP_INF_WAVES + w; t < ntile; t += (uint64_t)gridDim.x * MLP_INF_WAVES) {
         const uint64_t s0 = t * MLP_TILE;
-        for (uint32_t q = lane; q < MLP_TILE * MLP_IN; q += 64) {
-            const uint64_t smp = s0 + q / MLP_IN;
-            L.x[w][q] = smp < n ? f2bf(x[smp * MLP_IN + q % MLP_IN]) : (uint16_t)0;
+        const bf16x8 a0 = mlp_load_x(x, n, s0, 0), a1 = mlp_load_x(x, n, s0, 16);
+        // layer 1: [32 x 32] . [32 x 128] -> relu -> h1 (bf16, LDS)
+#pragma unroll
+        for (uint32_t nb = 0; nb < 4; ++nb) {
+            floatx16 acc = {};
+            acc = mfma(a0, load8(L.w1t + (nb * 32 + lr) * MLP_IN + 0 + lk), acc);
+            acc = mfma(a1, load8(L.w1t + (nb * 32 + lr) * MLP_IN + 16 + lk), acc);
+            const uint32_t col = nb * 32 + lr;
+            const float bias = b1[col];
+#pragma unroll
+            for (uint32_t r = 0; r < 16; ++r) {
+                const float z = acc[r] + bias;
+                h1[acc_row(r, lane) * MLP_H + col] = f2bf(z > 0.f ? z : 0.f);
+            }
         }
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
+        // layer 2: [32 x 128] . [128 x 128]
         floatx16 acc2[4];
-        mlp_tile_forward(L.x[w], L.w1t, L.w2t, L.b1, L.b2, L.h1[w], acc2);
-        const float s = mlp_tile_logit(acc2, L.w3, b3, L.part[w]);
-        if (lane < 32 && s0 + lane < n) logit[s0 + lane] = s;
+#pragma unroll
+        for (uint32_t nb = 0; nb < 4; ++nb) {
+            floatx16 acc = {};
+#pragma unroll
+            for (uint32_t k0 = 0; k0 < MLP_H; k0 += 16)
+                acc = mfma(load8(h1 + lr * MLP_H + k0 + lk), load8(L.w2t + (nb * 32 + lr) * MLP_H + k0 + lk), acc);
+            const float bias = b2[nb * 32 + lr];
+#pragma unroll
+            for (uint32_t r = 0; r < 16; ++r) acc[r] += bias;
+            acc2[nb] = acc;
+        }
+        // every lane's reads of h1 are done before the partials overwrite it
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        const float sc = mlp_tile_logit(acc2, w3, b3, reinterpret_cast<float*>(h1));
+        if (lane < 32 && s0 + lane < n) logit[s0 + lane] = sc;
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
     }

@@ -36,7 +36,7 @@ enum Slot {
     S_PG_HQ, S_PG_LQ, S_PGIDS, S_PGPOS, S_PGNB, S_SORTED,
     S_A_RANK, S_A_NXT, S_A_PRV, S_A_OFF, S_A_ST0, S_A_ST1, S_A_MEMB, S_A_CKEY, S_A_CLEN, S_A_CCNT,
     S_A_HK0, S_A_HK1, S_A_HV0, S_A_HV1, S_A_OLEN, S_A_OCNT, S_A_CPOS, S_A_CREC,
-    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_LAST, S_WORD,
+    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_PSLOT, S_PFILT, S_LAST, S_WORD,
     S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N, S_RECAOS,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
     S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG, S_POOL,
@@ -106,9 +106,10 @@ struct curc_ctx {
     std::string err;
     bool timing = false;
     std::vector<curc_ctx*> workers;  // extra stream/arena pairs of curc_batch_run
-    // mapping index layout: CSR instead of 64-byte bucket slots (environment
-    // CURC_MAP_INDEX=csr at context creation; the layout a huge index uses)
-    bool map_csr = false;
+    // hash index layout: CSR arrays only, no 64-byte bucket slots, for the
+    // prefix and the mapping index (environment CURC_INDEX=csr at context
+    // creation; the layout an index of more than 2^28 buckets uses)
+    bool index_csr = false;
     // learned mapping (CURC_RUN_LEARNED): the scorer's device arrays
     bool has_scorer = false;
     MlpDev scorer_net{};
@@ -772,8 +773,8 @@ void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* c
                                S.as<float>(S_TRX, c->trace_cap * MAP_NFEAT), S.as<uint8_t>(S_TRLAST, c->trace_cap),
                                S.as<uint32_t>(S_TRSLOT, c->trace_cap), tcount, c->trace_cap);
         if (learned) {
-            hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(nm)), dim3(256), 0, st, c->scorer_dev(), (const float*)X,
-                               nm, (const uint32_t*)nact[cur], logit);
+            hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_infer_grid(nm)), dim3(64 * MLP_INF_WAVES), 0, st,
+                               c->scorer_dev(), (const float*)X, nm, (const uint32_t*)nact[cur], logit);
             hipLaunchKernelGGL(k_map_stop, dim3(g), dim3(256), 0, st, ms, (const uint32_t*)act[cur],
                                (const uint32_t*)nact[cur], (const float*)logit, (const uint8_t*)a.last,
                                a.max_mm, c->scorer_threshold, c->scorer_threshold_unmapped);
@@ -1050,20 +1051,34 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const uint32_t mask = H - 1;
         uint32_t* kr = S.as<uint32_t>(S_KR, (uint64_t)H + 2);
         uint64_t* vk = S.as<uint64_t>(S_VK, (uint64_t)npre + 1);
+        // bucket slots up to 2^28 buckets (16 GiB), the CSR arrays alone beyond
+        PrefixSlot* pslots =
+            (H <= PSLOT_MAX_H && !c->index_csr) ? S.as<PrefixSlot>(S_PSLOT, (uint64_t)H) : nullptr;
+        // prefix k-mer filter: >= 32 bits per prefix (2^20..2^32 bits), so a
+        // probe without an equal k-mer usually stops at one bit
+        uint64_t fbits = 1ull << 20;
+        while (fbits < 32ull * npre && fbits < (1ull << 32)) fbits <<= 1;
+        const uint32_t fmask = (uint32_t)(fbits - 1);
+        const uint64_t fwords = fbits / 32;
+        uint32_t* pfilt = S.as<uint32_t>(S_PFILT, fwords);
         {
             Region r(c, "prefix_index");
             uint32_t* hk0 = S.as<uint32_t>(S_MK0, (uint64_t)npre + 1);
             uint32_t* hk1 = S.as<uint32_t>(S_MK1, (uint64_t)npre + 1);
             uint32_t* vb = S.as<uint32_t>(S_MV1, (uint64_t)npre + 1);
             uint32_t* va_ = va;
+            fill<uint32_t>(c, pfilt, fwords, 0u);
             hipLaunchKernelGGL(k_prefix_hash, dim3(grid_for(npre)), dim3(256), 0, st, (const uint64_t*)packed,
-                               (const uint32_t*)va, (uint64_t)npre, W, P.kip, P.k, mask, hk0);
+                               (const uint32_t*)va, (uint64_t)npre, W, P.kip, P.k, mask, hk0, pfilt, fmask);
             radix_sort_pairs<uint32_t>(hk0, va_, hk1, vb, npre, 0, log2u(H), S, S_PRIM, st);
             if (va_ != va) CURC_HIP(hipMemcpyAsync(va, va_, (uint64_t)npre * 4, hipMemcpyDeviceToDevice, st));
             hipLaunchKernelGGL(k_gap_fill, dim3(grid_for((uint64_t)npre + 1)), dim3(256), 0, st,
                                (const uint32_t*)hk0, (uint64_t)npre, (uint64_t)H + 1, kr);
             hipLaunchKernelGGL(k_gather_key0, dim3(grid_for(npre)), dim3(256), 0, st, (const uint64_t*)packed,
                                (const uint32_t*)va, (uint64_t)npre, W, vk);
+            if (pslots)
+                hipLaunchKernelGGL(k_prefix_slots, dim3(grid_for(H)), dim3(256), 0, st, (const uint32_t*)kr,
+                                   (const uint32_t*)va, (const uint64_t*)vk, (uint64_t)H, pslots);
             c->add_bytes("prefix_index", (double)npre * (16 + 8 + 16) + 8.0 * H, (double)npre);
         }
         // ---- a6: greedy overlap loop (compress.cu:1473-1532)
@@ -1090,8 +1105,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 ClaimState cs{packed, W, L, sl[cur], ns[cur], va, vk, cnt + C_NPRE, next, prev, off,
                               choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                               slinked, nullptr};
-                hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr, mask, P.k,
-                                   P.kip, o);
+                hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr,
+                                   (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask, P.k, P.kip, o);
                 hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
                 hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
                 compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
@@ -1214,7 +1229,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     const uint64_t start = parts[pi][0], count = parts[pi][1];
                     const uint32_t H = hash_size(count);
                     // buckets inline in 64-byte slots up to 2^28 of them (16 GiB), CSR beyond
-                    const bool inline_bk = H <= MAP_SLOT_MAX_H && !c->map_csr;
+                    const bool inline_bk = H <= MAP_SLOT_MAX_H && !c->index_csr;
                     uint32_t* kr = inline_bk ? nullptr : S.as<uint32_t>(S_MKR, (uint64_t)H + 2);
                     uint32_t* pos = inline_bk ? nullptr : S.as<uint32_t>(S_MPOS, count + 2);
                     uint32_t* bk = inline_bk ? S.as<uint32_t>(S_MBK, (uint64_t)H * MAP_SLOT) : nullptr;
@@ -1481,8 +1496,8 @@ curc_ctx* curc_ctx_create(int device) {
     if (hipSetDevice(device) != hipSuccess) return nullptr;
     curc_ctx* c = new curc_ctx();
     c->device = device;
-    const char* mi = std::getenv("CURC_MAP_INDEX");
-    c->map_csr = mi && std::string(mi) == "csr";
+    const char* ix = std::getenv("CURC_INDEX");
+    c->index_csr = ix && std::string(ix) == "csr";
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return nullptr;
@@ -1766,7 +1781,7 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
     for (uint32_t w = 1; w < workers; ++w) {
         curc_ctx* wc = c->workers[w - 1];
         wc->timing = c->timing;
-        wc->map_csr = c->map_csr;
+        wc->index_csr = c->index_csr;
         // workers share the scorer's device arrays (read-only during mapping)
         wc->has_scorer = c->has_scorer;
         wc->scorer_net = c->scorer_net;
@@ -1934,7 +1949,7 @@ int curc_mlp_forward(curc_mlp* m, const float* d_x, uint64_t n, float* d_logit) 
     if (!m || (n && (!d_x || !d_logit))) return CURC_EINVAL;
     if (n == 0) return CURC_OK;
     try {
-        hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_grid(n)), dim3(256), 0, m->c->st, m->d, d_x, n,
+        hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_infer_grid(n)), dim3(64 * MLP_INF_WAVES), 0, m->c->st, m->d, d_x, n,
                            (const uint32_t*)nullptr, d_logit);
         CURC_HIP(hipGetLastError());
         CURC_HIP(hipStreamSynchronize(m->c->st));

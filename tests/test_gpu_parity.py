@@ -115,19 +115,19 @@ def test_determinism(gpu_ctx):
         assert np.array_equal(a.get(f), b.get(f)), f
 
 
-def test_mapping_with_csr_index():
-    """The CSR mapping index (taken when the bucket slots would pass 16 GiB;
-    forced here by CURC_MAP_INDEX=csr) gives the same results."""
+def test_csr_hash_indexes():
+    """The CSR prefix and mapping indexes (taken when bucket slots would pass
+    16 GiB; forced here by CURC_INDEX=csr) give the same results."""
     cfg = ca.synth_config(seed=303, genome_len=60_000, gc=0.45, n_reads=9_000, L=150,
                           sub_first=0.002, sub_last=0.012, n_rate=0.0005)
     reads = ca.synth_reads(cfg)[1]
-    old = os.environ.get("CURC_MAP_INDEX")
-    os.environ["CURC_MAP_INDEX"] = "csr"
+    old = os.environ.get("CURC_INDEX")
+    os.environ["CURC_INDEX"] = "csr"
     try:
         with ca.Context(0) as ctx:
             run_both(ctx, reads)
     finally:
         if old is None:
-            del os.environ["CURC_MAP_INDEX"]
+            del os.environ["CURC_INDEX"]
         else:
-            os.environ["CURC_MAP_INDEX"] = old
+            os.environ["CURC_INDEX"] = old
