@@ -340,19 +340,46 @@ __global__ void k_prefix_slots(const uint32_t* __restrict__ kr, const uint32_t* 
 // search the bucket, take the first free non-self candidate if it matches.
 // With slots, buckets of up to PSLOT_N entries are searched in registers
 // (the same lower bound and the same first-free scan).
+// n bases from base `off` of suffix i, its words held word-major
+// (word w of suffix i at sw[w * stride + i]): neighbouring threads read
+// neighbouring words.
+__device__ __forceinline__ uint64_t xw_soa(const uint64_t* __restrict__ sw, uint64_t stride, uint64_t i,
+                                           uint32_t off, uint32_t n) {
+    const uint32_t o1 = off >> 5;
+    const uint32_t o2 = (off & 31u) << 1;
+    const uint64_t left = sw[o1 * stride + i] << o2;
+    if ((64u - o2) >= (n << 1)) return left >> ((32u - n) << 1);
+    const uint64_t right = (sw[(o1 + 1) * stride + i] >> (62u - o2)) >> 2;
+    return (left | right) >> ((32u - n) << 1);
+}
+
+// Word-major copy of the suffix list's reads.
+__global__ void k_gather_soa(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids, uint64_t n,
+                             uint32_t W, uint64_t* __restrict__ sw) {
+    GRID_STRIDE(t, n * W) {
+        const uint64_t w = t / n, i = t - w * n;
+        sw[t] = R[(uint64_t)ids[i] * W + w];
+    }
+}
+
+// The suffix list is not compacted between offsets: a suffix linked at an
+// earlier offset keeps its slinked flag and is skipped, and the suffixes'
+// words come from the word-major copy sw (stride = list length).
 __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t* __restrict__ kr,
                                                   const PrefixSlot* __restrict__ slots,
                                                   const uint32_t* __restrict__ filt, uint32_t fmask, uint32_t mask,
-                                                  uint32_t k, uint32_t kip, uint32_t off) {
+                                                  const uint64_t* __restrict__ sw, uint32_t k, uint32_t kip,
+                                                  uint32_t off) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
     const uint32_t ns = *cs.d_ns;
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(32u, len);
     GRID_STRIDE(i, ns) {
-        cs.slinked[i] = 0;
-        uint32_t s = cs.sl[i];
-        const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-        const uint64_t m = murmur64(xw(rs, kip + off, k));
+        if (cs.slinked[i]) {
+            cs.choice[i] = INV;
+            continue;
+        }
+        const uint64_t m = murmur64(xw_soa(sw, ns, i, kip + off, k));
         if (filt) {
             // no prefix has this k-mer: no candidate (exact, no false negatives)
             const uint32_t f = (uint32_t)(m >> 32) & fmask;
@@ -361,8 +388,10 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
                 continue;
             }
         }
+        const uint32_t s = cs.sl[i];
+        const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
         uint32_t h = (uint32_t)(m & mask);
-        uint64_t sx0 = xw(rs, off, n1);
+        uint64_t sx0 = xw_soa(sw, ns, i, off, n1);
         uint32_t lo, e;
         if (slots) {
             const PrefixSlot sl = slots[h];

@@ -36,7 +36,7 @@ enum Slot {
     S_PG_HQ, S_PG_LQ, S_PGIDS, S_PGPOS, S_PGNB, S_SORTED,
     S_A_RANK, S_A_NXT, S_A_PRV, S_A_OFF, S_A_ST0, S_A_ST1, S_A_MEMB, S_A_CKEY, S_A_CLEN, S_A_CCNT,
     S_A_HK0, S_A_HK1, S_A_HV0, S_A_HV1, S_A_OLEN, S_A_OCNT, S_A_CPOS, S_A_CREC,
-    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_PSLOT, S_PFILT, S_LAST, S_WORD,
+    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_PSLOT, S_PFILT, S_SUFW, S_LAST, S_WORD,
     S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N, S_RECAOS,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
     S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG, S_POOL,
@@ -1084,7 +1084,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         // ---- a6: greedy overlap loop (compress.cu:1473-1532)
         {
             Region r(c, "overlap");
-            uint32_t* sl[2] = {sl0, S.as<uint32_t>(S_SL1, (uint64_t)nsuf + 1)};
+            uint32_t* sl[1] = {sl0};
             uint32_t* choice = S.as<uint32_t>(S_CHOICE, (uint64_t)nsuf + 1);
             uint32_t* grp = S.as<uint32_t>(S_GRP, (uint64_t)nsuf + 1);
             uint32_t* memb = S.as<uint32_t>(S_MEMB, (uint64_t)nsuf + 1);
@@ -1096,22 +1096,24 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             fill<uint32_t>(c, owner, (uint64_t)npre + 1, INV);
             fill<uint32_t>(c, head, (uint64_t)npre + 1, INV);
             fill<uint32_t>(c, cont, (uint64_t)npre + 1, 0u);
-            uint32_t* ns[2] = {cnt + C_NS0, cnt + C_NS1};
-            CURC_HIP(hipMemcpyAsync(ns[0], cnt + C_NSUF, 4, hipMemcpyDeviceToDevice, st));
+            // one static suffix list for all offsets (its length is nsuf, the
+            // host value of cnt[C_NSUF]); linked suffixes are skipped by flag
             const uint32_t g = grid_for(nsuf, 4096);
-            int cur = 0;
             uint8_t* slinked = S.as<uint8_t>(S_SLINK, (uint64_t)nsuf + 1);
+            fill<uint8_t>(c, slinked, (uint64_t)nsuf + 1, (uint8_t)0);
+            uint64_t* sw = S.as<uint64_t>(S_SUFW, (uint64_t)nsuf * W + 1);
+            if (nsuf)
+                hipLaunchKernelGGL(k_gather_soa, dim3(grid_for((uint64_t)nsuf * W)), dim3(256), 0, st,
+                                   (const uint64_t*)packed, (const uint32_t*)sl[0], (uint64_t)nsuf, W, sw);
             for (uint32_t o = 1; o < P.max_off; ++o) {
-                ClaimState cs{packed, W, L, sl[cur], ns[cur], va, vk, cnt + C_NPRE, next, prev, off,
+                ClaimState cs{packed, W, L, sl[0], cnt + C_NSUF, va, vk, cnt + C_NPRE, next, prev, off,
                               choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                               slinked, nullptr};
                 hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr,
-                                   (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask, P.k, P.kip, o);
+                                   (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
+                                   (const uint64_t*)sw, P.k, P.kip, o);
                 hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
                 hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
-                compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
-                        st, nsuf);
-                cur = 1 - cur;
             }
         }
     }
