@@ -37,7 +37,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--reads", type=int, default=20_000_000)
     p.add_argument("--genome", type=int, default=100_000_000)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -323,7 +323,8 @@ def main():
     ctx.set_timing(False)
     stats = ctx.stats()
     work = {s["name"]: int(s["items"] / a.steps) for s in stats if s["name"].startswith("map_") and s["ms"] == 0}
-    stats = [s for s in stats if not (s["name"].startswith("map_") and s["ms"] == 0)]
+    graphs = {s["name"][6:]: int(s["items"]) for s in stats if s["name"].startswith("graph_")}
+    stats = [s for s in stats if s["ms"] > 0 or not s["name"].startswith(("map_", "graph_"))]
     ms = dt * 1e3 / a.steps
     gbases = a.reads * L / 1e9
     value = gbases / (ms / 1e3)
@@ -397,6 +398,7 @@ def main():
                    "reads": a.reads, "read_len": L, "genome_bp": a.genome},
         "stages_ms": {s["name"]: round(s["ms"] / a.steps, 3) for s in stats},
         "mapping_work": work,
+        "offset_loops": graphs,
         "roofline": roof,
         "cpu_baseline": cpu,
         "bits_per_base": bpb,

@@ -289,6 +289,7 @@ struct ClaimState {
     uint32_t* res_n;
     uint32_t* pool;         // member lists of contended groups, carved per leader
     uint32_t* pool_n;
+    uint64_t* kpool;        // a8: first reversed-prefix word of each pooled member (or null)
     uint8_t* slinked;       // per suffix index: linked at this offset
     uint8_t* plinked;       // per prefix position: linked at this offset (a8 only)
     __device__ __forceinline__ void link(uint32_t si, uint32_t px, uint32_t off) const {
@@ -553,6 +554,50 @@ __device__ __forceinline__ bool member_before(const ClaimState& cs, uint32_t u, 
     return u < v;
 }
 
+// First comparison word of cmp_rev_prefix: the n = min(BPW, off) bases
+// before `off`, reversed.
+template <class E>
+__device__ __forceinline__ uint64_t rev_prefix_key(const uint64_t* __restrict__ R, uint32_t W, uint32_t s,
+                                                   uint32_t off) {
+    const uint32_t n = min(E::BPW, off);
+    return E::reversed(E::word(R + (uint64_t)s * W, off - n, n), n);
+}
+
+// Members a[0..m) of a large a8 group in cmp_rev_prefix order, with each
+// member's first comparison word loaded once into k[0..m) (heapsort on the
+// pairs; the reads are touched again only to break ties of that word).
+template <class E>
+__device__ void sort_members_keyed(const ClaimState& cs, uint32_t* a, uint64_t* k, uint32_t m, uint32_t off) {
+    for (uint32_t i = 0; i < m; ++i) k[i] = rev_prefix_key<E>(cs.R, cs.W, cs.sl[a[i]], off);
+    auto before = [&](uint32_t x, uint32_t y) {
+        if (k[x] != k[y]) return k[x] < k[y];
+        return cmp_rev_prefix<E>(cs.R, cs.W, cs.sl[a[x]], cs.sl[a[y]], off) < 0;
+    };
+    auto swap = [&](uint32_t x, uint32_t y) {
+        const uint32_t t = a[x];
+        a[x] = a[y];
+        a[y] = t;
+        const uint64_t u = k[x];
+        k[x] = k[y];
+        k[y] = u;
+    };
+    auto sift = [&](uint32_t root, uint32_t end) {
+        for (;;) {
+            uint32_t c = 2 * root + 1;
+            if (c >= end) return;
+            if (c + 1 < end && before(c, c + 1)) ++c;
+            if (!before(root, c)) return;
+            swap(root, c);
+            root = c;
+        }
+    };
+    for (uint32_t i = m / 2; i-- > 0;) sift(i, m);
+    for (uint32_t end = m; end-- > 1;) {
+        swap(0, end);
+        sift(0, end);
+    }
+}
+
 template <bool FULLMATCH, class E>
 __device__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint32_t off) {
     if (m <= 16) {
@@ -606,10 +651,14 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
         if (cs.head[g] != lead) continue;
         uint32_t m = 0;
         for (uint32_t u = lead; u != INV; u = cs.memb[u]) ++m;
-        uint32_t* arr = cs.pool + atomicAdd(cs.pool_n, m);
+        const uint32_t base = atomicAdd(cs.pool_n, m);
+        uint32_t* arr = cs.pool + base;
         m = 0;
         for (uint32_t u = lead; u != INV; u = cs.memb[u]) arr[m++] = u;
-        sort_members<FULLMATCH, E>(cs, arr, m, off);
+        if (FULLMATCH && cs.kpool && m > 16)
+            sort_members_keyed<E>(cs, arr, cs.kpool + base, m, off);
+        else
+            sort_members<FULLMATCH, E>(cs, arr, m, off);
         auto next_free = [&](uint32_t x) {
             while (x < np && cs.prev[cs.pl[x]] != INV) ++x;
             return x;

@@ -21,6 +21,7 @@
 #include "mapphase.cuh"
 #include "mlp.cuh"
 #include "refmatch.cuh"
+#include "graph_cache.h"
 #include "host_pool.h"
 #include "streams.cuh"
 
@@ -36,7 +37,7 @@ enum Slot {
     S_PG_HQ, S_PG_LQ, S_PGIDS, S_PGPOS, S_PGNB, S_SORTED,
     S_A_RANK, S_A_NXT, S_A_PRV, S_A_OFF, S_A_ST0, S_A_ST1, S_A_MEMB, S_A_CKEY, S_A_CLEN, S_A_CCNT,
     S_A_HK0, S_A_HK1, S_A_HV0, S_A_HV1, S_A_OLEN, S_A_OCNT, S_A_CPOS, S_A_CREC,
-    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_PSLOT, S_PFILT, S_SUFW, S_LAST, S_WORD,
+    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_PSLOT, S_PFILT, S_SUFW, S_KPOOL, S_LAST, S_WORD,
     S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N, S_RECAOS,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
     S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG, S_POOL,
@@ -110,6 +111,10 @@ struct curc_ctx {
     // prefix and the mapping index (environment CURC_INDEX=csr at context
     // creation; the layout an index of more than 2^28 buckets uses)
     bool index_csr = false;
+    // offset loops replayed as HIP graphs: environment CURC_GRAPHS=1 at
+    // context creation turns this on (measured about 1% slower than plain
+    // stream launches on the 20M-read block, so off by default)
+    GraphCache graphs;
     // learned mapping (CURC_RUN_LEARNED): the scorer's device arrays
     bool has_scorer = false;
     MlpDev scorer_net{};
@@ -238,7 +243,7 @@ struct curc_block {
     hvec<uint32_t> lq_next, lq_prev;
     hvec<uint16_t> lq_off;
     hvec<char> Nascii;
-    std::string nlq_text;
+    hvec<char> nlq_text;
     hvec<curc_alignment> nlq_rec;
     // pre-entropy streams (CURC_RUN_STREAMS)
     hvec<char> s_strand;
@@ -251,7 +256,7 @@ struct curc_block {
     hvec<float> trace_x, trace_y;
     // pseudogenome self-match (CURC_RUN_REFMATCH): parts HQ, LQ, N-read LQ
     struct RmPart {
-        std::string text, map_off, map_len;
+        hvec<char> text, map_off, map_len;
         hvec<uint64_t> matches, found;  // (src, len, dest) triples, with KEEP_STAGES
     } rm[3];
 };
@@ -389,10 +394,19 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     // short lists (<= FM_SHORT) go without block ranges, searching the whole
     // prefix list, which saves the fill / histogram / scan launches
     constexpr uint32_t FM_SHORT = 1u << 12;
+    if (n_max == 0) return;
+    // bounds are rounded up to a power of two (>= FM_SHORT) so that blocks
+    // of similar size share buffer sizes, grids and hence captured graphs;
+    // the kernels read the true list lengths from device counters
+    const uint32_t n_in = n_max;  // length of the caller's lists d_pre / d_suf
+    {
+        uint32_t b = FM_SHORT;
+        while (b < n_max) b <<= 1;
+        n_max = b;
+    }
     uint32_t MAX_BP = 1;
     while (MAX_BP < ((BITS == 2) ? 10u : 6u) && (1ull << (BITS * MAX_BP)) < n_max) ++MAX_BP;
     const bool ranged = n_max > FM_SHORT;
-    if (n_max == 0) return;
     Scratch& S = c->S;
     uint32_t* pl[2] = {S.as<uint32_t>(S_PL0, n_max), S.as<uint32_t>(S_PL1, n_max)};
     uint64_t* pk[2] = {S.as<uint64_t>(S_PK0, n_max), S.as<uint64_t>(S_PK1, n_max)};
@@ -402,6 +416,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     uint32_t* memb = S.as<uint32_t>(S_MEMB, n_max);
     uint32_t* resl = S.as<uint32_t>(S_RESL, n_max);
     uint32_t* pool = S.as<uint32_t>(S_POOL, n_max);
+    uint64_t* kpool = S.as<uint64_t>(S_KPOOL, n_max);
     uint8_t* slinked = S.as<uint8_t>(S_SLINK, n_max);
     uint8_t* plinked = S.as<uint8_t>(S_PLINK, n_max);
     uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)n_max + 1);
@@ -415,36 +430,50 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     uint32_t* ns[2] = {cnt + C_NS0, cnt + C_NS1};
     CURC_HIP(hipMemcpyAsync(np[0], cnt + C_PRE, 4, hipMemcpyDeviceToDevice, c->st));
     CURC_HIP(hipMemcpyAsync(ns[0], cnt + C_SUF, 4, hipMemcpyDeviceToDevice, c->st));
-    CURC_HIP(hipMemcpyAsync(pl[0], d_pre, (uint64_t)n_max * 4, hipMemcpyDeviceToDevice, c->st));
-    CURC_HIP(hipMemcpyAsync(sl[0], d_suf, (uint64_t)n_max * 4, hipMemcpyDeviceToDevice, c->st));
+    CURC_HIP(hipMemcpyAsync(pl[0], d_pre, (uint64_t)n_in * 4, hipMemcpyDeviceToDevice, c->st));
+    CURC_HIP(hipMemcpyAsync(sl[0], d_suf, (uint64_t)n_in * 4, hipMemcpyDeviceToDevice, c->st));
     hipLaunchKernelGGL(k_gather_key0_dev, dim3(grid_for(n_max)), dim3(256), 0, c->st, R, (const uint32_t*)pl[0],
                        (const uint32_t*)np[0], W, pk[0]);
     const uint32_t g = grid_for(n_max, 4096);
-    int cur = 0;
     if (ranged) fill<uint32_t>(c, rng, (1ull << (BITS * std::min(MAX_BP, L - 1))) + 1, 0u);
-    for (uint32_t off = 1; off < L; ++off) {
-        const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
-        const uint64_t nb = 1ull << (BITS * bp);
-        if (ranged) {
-            hipLaunchKernelGGL(k_fm_block_hist<E>, dim3(grid_for(n_max, 4096)), dim3(256), 0, c->st,
-                               (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng, plinked);
-            scan_excl(rng, rng, nb + 1, S, S_PRIM, c->st);
+    auto loop = [&]() {
+        int cur = 0;
+        for (uint32_t off = 1; off < L; ++off) {
+            const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
+            const uint64_t nb = 1ull << (BITS * bp);
+            if (ranged) {
+                hipLaunchKernelGGL(k_fm_block_hist<E>, dim3(grid_for(n_max, 4096)), dim3(256), 0, c->st,
+                                   (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng, plinked);
+                scan_excl(rng, rng, nb + 1, S, S_PRIM, c->st);
+            }
+            ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
+                          choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
+                          kpool, slinked, plinked};
+            hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs,
+                               ranged ? (const uint32_t*)rng : nullptr, bp, off);
+            // the ranges are rebuilt next round over at most nb + 1 values
+            hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, ranged ? rng : nullptr,
+                               ranged ? nb + 1 : 0ull);
+            hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
+            compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
+                    c->st, n_max);
+            compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
+                    np[1 - cur], S, S_PRIM, c->st, n_max);
+            cur = 1 - cur;
         }
-        ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
-                      choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
-                      slinked, plinked};
-        hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs, ranged ? (const uint32_t*)rng : nullptr,
-                           bp, off);
-        // the ranges are rebuilt next round over at most nb + 1 values
-        hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, ranged ? rng : nullptr,
-                           ranged ? nb + 1 : 0ull);
-        hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
-        compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
-                c->st, n_max);
-        compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
-                np[1 - cur], S, S_PRIM, c->st, n_max);
-        cur = 1 - cur;
-    }
+    };
+    GraphKey key;
+    key.v[0] = 1;  // the exact-matching loop
+    key.v[1] = E::BPW;
+    key.v[2] = W;
+    key.v[3] = L;
+    key.v[4] = n_max;
+    key.v[5] = (uint64_t)R;
+    key.v[6] = (uint64_t)next;
+    key.v[7] = (uint64_t)prev;
+    key.v[8] = (uint64_t)offset;
+    key.v[9] = (uint64_t)cnt;
+    c->graphs.run(key, S, c->st, loop);
 }
 
 // Record sort and stream builders (a13; compress.cu:1730-2044), on the
@@ -909,11 +938,12 @@ void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOu
         hipLaunchKernelGGL(k_lq_streams, dim3(grid_for(nlq_asm.n)), dim3(256), 0, st, (const uint32_t*)nlq_asm.rid,
                            (const uint64_t*)nlq_asm.pos, nlq_asm.n, (const uint32_t*)src.Nids, L, preserve, paired,
                            B->hq_len + B->lq_len, nlqoff, o.pe_order + n + lq_asm.n, o.id_pos);
-    // to host
-    std::vector<uint32_t> rid;
-    std::vector<uint64_t> rword;
-    d2h(c, rid, o.rid, n);
-    d2h(c, rword, o.rword, n);
+    // to host (the sorted records in their 16-byte layout, built on the device)
+    curc_alignment* drec = S.as<curc_alignment>(S_RECAOS, (uint64_t)n + 1);
+    if (n)
+        hipLaunchKernelGGL(k_rec_aos, dim3(grid_for(n)), dim3(256), 0, st, (const uint32_t*)o.rid,
+                           (const uint64_t*)o.rword, (const uint32_t*)nullptr, (uint64_t)n, 0u, 0ull, drec);
+    d2h(c, B->s_sorted, (const curc_alignment*)drec, n);
     d2h(c, B->s_strand, (const char*)o.strand, n);
     d2h(c, B->s_mis_count, o.mis_count, n);
     d2h(c, B->s_mis_base, symb, total_mis);
@@ -927,12 +957,6 @@ void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOu
         d2h(c, B->s_id_pos, o.id_pos, N);
     }
     CURC_HIP(hipStreamSynchronize(st));
-    B->s_sorted.resize(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        B->s_sorted[i].read_id = rid[i];
-        B->s_sorted[i].pad = 0;
-        B->s_sorted[i].word = rword[i];
-    }
 }
 
 void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop, uint32_t flags,
@@ -1105,16 +1129,32 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             if (nsuf)
                 hipLaunchKernelGGL(k_gather_soa, dim3(grid_for((uint64_t)nsuf * W)), dim3(256), 0, st,
                                    (const uint64_t*)packed, (const uint32_t*)sl[0], (uint64_t)nsuf, W, sw);
-            for (uint32_t o = 1; o < P.max_off; ++o) {
-                ClaimState cs{packed, W, L, sl[0], cnt + C_NSUF, va, vk, cnt + C_NPRE, next, prev, off,
-                              choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
-                              slinked, nullptr};
-                hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr,
-                                   (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
-                                   (const uint64_t*)sw, P.k, P.kip, o);
-                hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
-                hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
-            }
+            auto loop = [&]() {
+                for (uint32_t o = 1; o < P.max_off; ++o) {
+                    ClaimState cs{packed, W, L, sl[0], cnt + C_NSUF, va, vk, cnt + C_NPRE, next, prev, off,
+                                  choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
+                                  nullptr, slinked, nullptr};
+                    hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr,
+                                       (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
+                                       (const uint64_t*)sw, P.k, P.kip, o);
+                    hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
+                    hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
+                }
+            };
+            GraphKey key;
+            key.v[0] = 2;  // the overlap loop
+            key.v[1] = W;
+            key.v[2] = L;
+            key.v[3] = nsuf;
+            key.v[4] = mask;
+            key.v[5] = fmask;
+            key.v[6] = pslots != nullptr;
+            key.v[7] = (uint64_t)packed;
+            key.v[8] = (uint64_t)next;
+            key.v[9] = (uint64_t)prev;
+            key.v[10] = (uint64_t)off;
+            key.v[11] = (uint64_t)cnt;
+            c->graphs.run(key, S, st, loop);
         }
     }
     if (keep) {
@@ -1500,6 +1540,8 @@ curc_ctx* curc_ctx_create(int device) {
     c->device = device;
     const char* ix = std::getenv("CURC_INDEX");
     c->index_csr = ix && std::string(ix) == "csr";
+    const char* gr = std::getenv("CURC_GRAPHS");
+    c->graphs.enabled = gr && std::string(gr) == "1";
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return nullptr;
@@ -1513,6 +1555,7 @@ void curc_ctx_destroy(curc_ctx* c) {
     c->workers.clear();
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
+    c->graphs.clear();
     c->S.release();
     for (auto& p : c->pending) {
         hipEventDestroy(p.a);
@@ -1529,6 +1572,7 @@ int curc_ctx_trim(curc_ctx* c) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
+    c->graphs.clear();
     c->S.release();
     for (curc_ctx* w : c->workers) curc_ctx_trim(w);
     HostPool::get().trim();
@@ -1739,20 +1783,32 @@ int curc_ctx_set_timing(curc_ctx* c, int on) {
 
 int curc_ctx_stats(curc_ctx* c, curc_stage_stat* out, int max) {
     if (!c) return CURC_EINVAL;
-    int n = (int)c->stats.size();
+    std::vector<StageStat> all = c->stats;
+    // offset loops run eagerly / captured into a graph / replayed
+    const std::pair<const char*, uint64_t> g[3] = {{"graph_eager", c->graphs.eager},
+                                                   {"graph_captures", c->graphs.captures},
+                                                   {"graph_replays", c->graphs.replays}};
+    for (const auto& kv : g) {
+        StageStat st{kv.first};
+        st.items = (double)kv.second;
+        all.push_back(st);
+    }
+    int n = (int)all.size();
     for (int i = 0; i < n && i < max; ++i) {
         memset(&out[i], 0, sizeof(out[i]));
-        strncpy(out[i].name, c->stats[i].name.c_str(), sizeof(out[i].name) - 1);
-        out[i].ms = c->stats[i].ms;
-        out[i].launches = c->stats[i].launches;
-        out[i].alg_bytes = c->stats[i].alg_bytes;
-        out[i].items = c->stats[i].items;
+        strncpy(out[i].name, all[i].name.c_str(), sizeof(out[i].name) - 1);
+        out[i].ms = all[i].ms;
+        out[i].launches = all[i].launches;
+        out[i].alg_bytes = all[i].alg_bytes;
+        out[i].items = all[i].items;
     }
     return n;
 }
 
 void curc_ctx_reset_stats(curc_ctx* c) {
-    if (c) c->stats.clear();
+    if (!c) return;
+    c->stats.clear();
+    c->graphs.eager = c->graphs.captures = c->graphs.replays = 0;
 }
 
 }  // extern "C"
@@ -1784,6 +1840,7 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
         curc_ctx* wc = c->workers[w - 1];
         wc->timing = c->timing;
         wc->index_csr = c->index_csr;
+        wc->graphs.enabled = c->graphs.enabled;
         // workers share the scorer's device arrays (read-only during mapping)
         wc->has_scorer = c->has_scorer;
         wc->scorer_net = c->scorer_net;
@@ -1829,6 +1886,10 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
             c->stats[id].items += st.items;
         }
         wc->stats.clear();
+        c->graphs.eager += wc->graphs.eager;
+        c->graphs.captures += wc->graphs.captures;
+        c->graphs.replays += wc->graphs.replays;
+        wc->graphs.eager = wc->graphs.captures = wc->graphs.replays = 0;
     }
     return first;
 }

@@ -23,14 +23,18 @@ namespace curc {
 
 // Growable device scratch owned by a context: get(slot, bytes) returns a
 // buffer of at least `bytes`, reallocating only when it must grow.
+// `gen` counts reallocations: work captured into a graph stays valid while
+// it is unchanged.
 struct Scratch {
     static constexpr int SLOTS = 200;
     void* ptr[SLOTS] = {};
     size_t cap[SLOTS] = {};
     size_t total = 0;
+    uint64_t gen = 0;
     void* get(int slot, size_t bytes) {
         if (bytes == 0) bytes = 16;
         if (cap[slot] < bytes) {
+            ++gen;
             if (ptr[slot]) CURC_HIP(hipFree(ptr[slot]));
             total -= cap[slot];
             size_t c = bytes + bytes / 8 + 256;
@@ -45,6 +49,7 @@ struct Scratch {
         return static_cast<T*>(get(slot, count * sizeof(T)));
     }
     void release() {
+        ++gen;
         for (int i = 0; i < SLOTS; ++i) {
             if (ptr[i]) hipFree(ptr[i]);
             ptr[i] = nullptr;
