@@ -113,6 +113,59 @@ __global__ void k_gather_word(const uint64_t* __restrict__ R, const uint32_t* __
     GRID_STRIDE(i, n) keys[i] = R[(uint64_t)ids[i] * W + w];
 }
 
+// Tie refinement of the dedup sort.  After a stable sort of all reads by
+// their first word, only reads in runs of equal first words need the other
+// words: their positions are compacted, the subset is sorted by the
+// remaining words (least significant first) and then by run number, all
+// stably, and written back into the same positions.  The order equals a
+// full lexicographic sort with ties by ascending id.
+struct PredTied {
+    const uint64_t* key;  // first words in sorted order
+    uint64_t n;
+    __device__ bool operator()(uint32_t i) const {
+        const uint64_t k = key[i];
+        return (i > 0 && key[i - 1] == k) || (i + 1 < n && key[i + 1] == k);
+    }
+};
+
+// Subset entry j: its read id, its index (the value the subset sorts
+// carry) and 1 when it starts a run.
+__global__ void k_tie_setup(const uint32_t* __restrict__ pos, const uint32_t* __restrict__ d_m,
+                            const uint64_t* __restrict__ key, const uint32_t* __restrict__ ids,
+                            uint32_t* __restrict__ sid, uint32_t* __restrict__ perm, uint32_t* __restrict__ head) {
+    const uint32_t m = *d_m;
+    GRID_STRIDE(j, m) {
+        const uint32_t p = pos[j];
+        sid[j] = ids[p];
+        perm[j] = (uint32_t)j;
+        head[j] = (j == 0 || key[pos[j - 1]] != key[p]) ? 1u : 0u;
+    }
+}
+
+// Run number of entry j from the exclusive scan of the run-start flags.
+__global__ void k_tie_runid(const uint32_t* __restrict__ d_m, const uint32_t* __restrict__ head,
+                            const uint32_t* __restrict__ excl, uint32_t* __restrict__ run) {
+    const uint32_t m = *d_m;
+    GRID_STRIDE(j, m) run[j] = excl[j] + head[j] - 1u;
+}
+
+// Word w of the read behind each permuted subset entry.
+__global__ void k_gather_word_perm(const uint64_t* __restrict__ R, const uint32_t* __restrict__ sid,
+                                   const uint32_t* __restrict__ perm, uint64_t m, uint32_t W, uint32_t w,
+                                   uint64_t* __restrict__ keys) {
+    GRID_STRIDE(q, m) keys[q] = R[(uint64_t)sid[perm[q]] * W + w];
+}
+
+__global__ void k_gather_u32_perm(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint64_t m,
+                                  uint32_t* __restrict__ keys) {
+    GRID_STRIDE(q, m) keys[q] = src[perm[q]];
+}
+
+__global__ void k_tie_scatter(const uint32_t* __restrict__ pos, const uint32_t* __restrict__ sid,
+                              const uint32_t* __restrict__ perm, uint64_t m, uint32_t* __restrict__ ids) {
+    GRID_STRIDE(q, m) ids[pos[q]] = sid[perm[q]];
+}
+
 // compress.cu:1409-1420: adjacent equal reads in sorted order are linked.
 __global__ void k_dedup_link(const uint64_t* __restrict__ R, const uint32_t* __restrict__ sorted, uint64_t n,
                              uint32_t W, uint32_t* __restrict__ next, uint32_t* __restrict__ prev,

@@ -37,7 +37,8 @@ enum Slot {
     S_PG_HQ, S_PG_LQ, S_PGIDS, S_PGPOS, S_PGNB, S_SORTED,
     S_A_RANK, S_A_NXT, S_A_PRV, S_A_OFF, S_A_ST0, S_A_ST1, S_A_MEMB, S_A_CKEY, S_A_CLEN, S_A_CCNT,
     S_A_HK0, S_A_HK1, S_A_HV0, S_A_HV1, S_A_OLEN, S_A_OCNT, S_A_CPOS, S_A_CREC,
-    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_PSLOT, S_PFILT, S_SUFW, S_KPOOL, S_LAST, S_WORD,
+    S_MK0, S_MK1, S_MV0, S_MV1, S_KRF, S_MKR, S_MPOS, S_MBK, S_PSLOT, S_PFILT, S_SUFW, S_KPOOL, S_TPOS, S_TSID, S_TP0, S_TP1,
+    S_THEAD, S_TRUN, S_TK0, S_TK1, S_LAST, S_WORD,
     S_MREC_ID, S_MREC_W, S_MIS, S_UNMAPPED, S_UNMAPPED_N, S_RECAOS,
     S_LQPACKED, S_LQNEXT, S_LQPREV, S_LQOFF,
     S_N3, S_NNEXT, S_NPREV, S_NOFF, S_NSORT, S_NPG, S_POOL,
@@ -51,7 +52,7 @@ enum Slot {
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
     S_LAST_NAMED,
-    S_PRIM = 176  // primitives use slots 176.. (scan recursion, sort hist, compaction)
+    S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
 
 // Device counters (u32) in slot S_CNT.
@@ -59,7 +60,7 @@ static_assert(S_LAST_NAMED <= S_PRIM, "scratch slots overlap");
 
 enum Cnt {
     C_N, C_NONN, C_NN, C_NPRE, C_NSUF, C_NS0, C_NS1, C_RESN, C_NP0, C_NP1,
-    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_RM, C_MA0, C_MA1,
+    C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_RM, C_MA0, C_MA1, C_TIE,
     C_COUNT = 32
 };
 
@@ -1026,14 +1027,54 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint32_t* v0 = S.as<uint32_t>(S_V0, (uint64_t)n + 1);
         uint32_t* v1 = S.as<uint32_t>(S_V1, (uint64_t)n + 1);
         CURC_HIP(hipMemcpyAsync(v0, nonN, (uint64_t)n * 4, hipMemcpyDeviceToDevice, st));
-        // LSD over words, least significant word first; each word sorted in
-        // 8-bit digits (stable), so ties keep ascending id.
-        for (int w = (int)W - 1; w >= 0; --w) {
-            uint32_t nbases = (w == (int)W - 1) ? (L - 32 * (W - 1)) : 32;
-            uint32_t begin = ((64 - 2 * nbases) / 8) * 8;
-            hipLaunchKernelGGL(k_gather_word, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)packed,
-                               (const uint32_t*)v0, (uint64_t)n, W, (uint32_t)w, k0);
-            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, begin, 64, S, S_PRIM, st);
+        // word w holds bases [32w, 32w + 32) (fewer in the last word); only
+        // its top 2 * nbases bits carry data, in 8-bit digits from `begin`
+        auto begin_bit = [&](uint32_t w) {
+            const uint32_t nbases = (w == W - 1) ? (L - 32 * (W - 1)) : 32;
+            return ((64 - 2 * nbases) / 8) * 8;
+        };
+        // every read by its first word (stable, so ties keep ascending id)
+        hipLaunchKernelGGL(k_gather_word, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)packed,
+                           (const uint32_t*)v0, (uint64_t)n, W, 0u, k0);
+        radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, begin_bit(0), 64, S, S_PRIM, st);
+        if (W > 1 && n > 1) {
+            // reads in runs of equal first words, by the remaining words
+            uint32_t* tpos = S.as<uint32_t>(S_TPOS, (uint64_t)n + 1);
+            set_cnt(c, cnt + C_TMP, n);
+            compact(cnt + C_TMP, PredTied{k0, n}, EmitIota{tpos}, cnt + C_TIE, S, S_PRIM, st, n);
+            const uint32_t m = read_cnt(c, cnt + C_TIE);
+            if (m > 0) {
+                uint32_t* sid = S.as<uint32_t>(S_TSID, (uint64_t)m + 1);
+                uint32_t* p0 = S.as<uint32_t>(S_TP0, (uint64_t)m + 1);
+                uint32_t* p1 = S.as<uint32_t>(S_TP1, (uint64_t)m + 1);
+                uint32_t* head = S.as<uint32_t>(S_THEAD, (uint64_t)m + 1);
+                uint32_t* run = S.as<uint32_t>(S_TRUN, (uint64_t)m + 1);
+                uint64_t* tk0 = S.as<uint64_t>(S_TK0, (uint64_t)m + 1);
+                uint64_t* tk1 = S.as<uint64_t>(S_TK1, (uint64_t)m + 1);
+                const dim3 gm(grid_for(m));
+                hipLaunchKernelGGL(k_tie_setup, gm, dim3(256), 0, st, (const uint32_t*)tpos,
+                                   (const uint32_t*)(cnt + C_TIE), (const uint64_t*)k0, (const uint32_t*)v0, sid, p0,
+                                   head);
+                scan_excl(head, run, m, S, S_PRIM, st);
+                hipLaunchKernelGGL(k_tie_runid, gm, dim3(256), 0, st, (const uint32_t*)(cnt + C_TIE),
+                                   (const uint32_t*)head, (const uint32_t*)run, run);
+                uint32_t* perm = p0;
+                uint32_t* permt = p1;
+                for (uint32_t w = W - 1; w >= 1; --w) {
+                    hipLaunchKernelGGL(k_gather_word_perm, gm, dim3(256), 0, st, (const uint64_t*)packed,
+                                       (const uint32_t*)sid, (const uint32_t*)perm, (uint64_t)m, W, w, tk0);
+                    radix_sort_pairs<uint64_t>(tk0, perm, tk1, permt, m, begin_bit(w), 64, S, S_PRIM, st);
+                }
+                // then by run number (stable): runs back in position order
+                uint32_t* rk0 = reinterpret_cast<uint32_t*>(tk0);
+                uint32_t* rk1 = reinterpret_cast<uint32_t*>(tk1);
+                hipLaunchKernelGGL(k_gather_u32_perm, gm, dim3(256), 0, st, (const uint32_t*)run,
+                                   (const uint32_t*)perm, (uint64_t)m, rk0);
+                radix_sort_pairs<uint32_t>(rk0, perm, rk1, permt, m, 0, std::max(8u, 8 * ((log2u(m) + 7) / 8)),
+                                           S, S_PRIM, st);
+                hipLaunchKernelGGL(k_tie_scatter, gm, dim3(256), 0, st, (const uint32_t*)tpos, (const uint32_t*)sid,
+                                   (const uint32_t*)perm, (uint64_t)m, v0);
+            }
         }
         sorted = v0;
         if (n > 1)

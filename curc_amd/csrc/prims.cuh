@@ -26,7 +26,7 @@ namespace curc {
 // `gen` counts reallocations: work captured into a graph stays valid while
 // it is unchanged.
 struct Scratch {
-    static constexpr int SLOTS = 200;
+    static constexpr int SLOTS = 256;
     void* ptr[SLOTS] = {};
     size_t cap[SLOTS] = {};
     size_t total = 0;
