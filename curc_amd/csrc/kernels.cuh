@@ -908,19 +908,27 @@ __global__ void k_map_hash(const uint64_t* __restrict__ pg, uint64_t start, uint
 constexpr uint32_t MAP_SLOT = 16;
 constexpr uint64_t MAP_SLOT_MAX_H = 1ull << 28;
 
-__global__ void k_map_slot_counts(const uint32_t* __restrict__ krf, uint64_t H, uint32_t* __restrict__ bk) {
-    GRID_STRIDE(b, H) bk[b * MAP_SLOT] = min(krf[b + 1] - krf[b], MAP_BUCKET);
-}
-
-// The same order as k_map_fill: the first min(n,12) occurrences by ascending
-// i, stored in descending i.
-__global__ void k_map_slot_fill(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                uint64_t count, const uint32_t* __restrict__ krf, uint32_t* __restrict__ bk) {
-    GRID_STRIDE(t, count) {
-        const uint32_t h = keys[t];
-        const uint32_t q = (uint32_t)t - krf[h];
-        const uint32_t c = min(krf[h + 1] - krf[h], MAP_BUCKET);
-        if (q < c) bk[(uint64_t)h * MAP_SLOT + 1 + (c - 1 - q)] = vals[t];
+// One thread per bucket writes its slot: the capped size and the first
+// min(n,12) occurrences (ascending i, as sorted) in descending order, the
+// order of k_map_fill.  Words past the bucket's positions are not written:
+// readers never look past the size.  16-byte stores, the first always, the
+// others only when the bucket reaches them.
+__global__ void k_map_slots(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ krf, uint64_t H,
+                            uint32_t* __restrict__ bk) {
+    GRID_STRIDE(h, H) {
+        const uint32_t lo = krf[h];
+        const uint32_t c = min(krf[h + 1] - lo, MAP_BUCKET);
+        uint32_t w[MAP_SLOT];
+        w[0] = c;
+#pragma unroll
+        for (uint32_t p = 0; p < MAP_BUCKET; ++p) w[1 + p] = p < c ? vals[lo + c - 1 - p] : 0u;
+#pragma unroll
+        for (uint32_t p = 1 + MAP_BUCKET; p < MAP_SLOT; ++p) w[p] = 0u;
+        uint4* d = reinterpret_cast<uint4*>(bk + h * MAP_SLOT);
+        d[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        if (c > 3) d[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        if (c > 7) d[2] = make_uint4(w[8], w[9], w[10], w[11]);
+        if (c > 11) d[3] = make_uint4(w[12], w[13], w[14], w[15]);
     }
 }
 

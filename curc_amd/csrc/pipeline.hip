@@ -1330,12 +1330,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         hipLaunchKernelGGL(k_gap_fill, dim3(grid_for(count + 1)), dim3(256), 0, st,
                                            (const uint32_t*)k0, count, (uint64_t)H + 1, krf);
                         if (inline_bk) {
-                            hipLaunchKernelGGL(k_map_slot_counts, dim3(grid_for(H)), dim3(256), 0, st,
+                            hipLaunchKernelGGL(k_map_slots, dim3(grid_for(H)), dim3(256), 0, st, (const uint32_t*)v0,
                                                (const uint32_t*)krf, (uint64_t)H, bk);
-                            if (count)
-                                hipLaunchKernelGGL(k_map_slot_fill, dim3(grid_for(count)), dim3(256), 0, st,
-                                                   (const uint32_t*)k0, (const uint32_t*)v0, count,
-                                                   (const uint32_t*)krf, bk);
                         } else {
                             hipLaunchKernelGGL(k_map_caps, dim3(grid_for((uint64_t)H + 2)), dim3(256), 0, st,
                                                (const uint32_t*)krf, (uint64_t)H, kr);
