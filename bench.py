@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--no-bits-per-base", action="store_true")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--no-learned", action="store_true")
+    p.add_argument("--no-tune", action="store_true",
+                   help="keep the default launch shapes instead of autotuning them on the block first")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run three device-to-device copies of BYTES (a kernel of known traffic for "
                         "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -207,9 +209,12 @@ def batched(ins, reps=2, learned=True):
             b.free()
         return n
 
+    from curc_amd import autotune
+
     with ca.Context(0) as ctx:
         for b in ctx.batch_run(ins[:4], workers=4):
             b.free()
+        workers, wtimes = autotune.tune_workers(ctx, ins[::4], log=log)
         t = time.perf_counter()
         for _ in range(reps):
             for r in ins:
@@ -245,7 +250,8 @@ def batched(ins, reps=2, learned=True):
             net.close()
     out = dict(inputs=len(ins), gbases=round(gbases, 4), sequential_s=round(seq, 4), batched_s=round(bat, 4),
                sequential_gbases_s=round(gbases / seq, 4), batched_gbases_s=round(gbases / bat, 4),
-               speedup=round(seq / bat, 3), mapped_reads=plain_mapped,
+               speedup=round(seq / bat, 3), mapped_reads=plain_mapped, workers=workers,
+               workers_tuning_s=wtimes,
                workload="64 inputs: genomes 1-50 Mbp, L in {75,100,150,250}, 10k-5M reads log-uniform, "
                         "0.3-1% subst, input in host memory")
     if lb is not None:
@@ -303,6 +309,16 @@ def main():
     def step():
         b = ctx.run_block(None, L=L, device_ptr=d_ascii, n_reads=a.reads)
         return b
+
+    tuning = None
+    if not a.no_tune:
+        # launch shapes only (results do not depend on them), chosen on this
+        # block before the timed region
+        from curc_amd import autotune
+
+        t0 = time.perf_counter()
+        best, table = autotune.tune_block(ctx, lambda: step().free(), reps=2, log=log)
+        tuning = dict(best, seconds=round(time.perf_counter() - t0, 1), candidates_ms=table)
 
     for i in range(a.warmup):
         t = time.perf_counter()
@@ -399,6 +415,7 @@ def main():
         "stages_ms": {s["name"]: round(s["ms"] / a.steps, 3) for s in stats},
         "mapping_work": work,
         "offset_loops": graphs,
+        "tuning": tuning,
         "roofline": roof,
         "cpu_baseline": cpu,
         "bits_per_base": bpb,

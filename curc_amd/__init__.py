@@ -65,6 +65,10 @@ class Job(ctypes.Structure):
                 ("flags", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("ov_grid", "fm_grid", "map_grid", "radix_items", "workers")]
+
+
 class StageStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("launches", ctypes.c_uint64),
                 ("alg_bytes", ctypes.c_double), ("items", ctypes.c_double)]
@@ -114,6 +118,8 @@ def lib():
         "curc_ref_match": (i32, [vp, vp, u64, vp, u64, i32, i32, u32, ctypes.POINTER(vp)]),
         "curc_ctx_set_scorer": (i32, [vp, vp, ctypes.c_float, ctypes.c_float]),
         "curc_ctx_set_trace": (i32, [vp, u32, u64]),
+        "curc_ctx_get_tuning": (i32, [vp, ctypes.POINTER(Tuning)]),
+        "curc_ctx_set_tuning": (i32, [vp, ctypes.POINTER(Tuning)]),
         "curc_mlp_create": (vp, [vp]),
         "curc_mlp_destroy": (None, [vp]),
         "curc_mlp_set_params": (i32, [vp, vp, u32]),
@@ -334,6 +340,20 @@ class Context:
         t_map, t_un = (threshold, threshold) if np.isscalar(threshold) else threshold
         self._check(lib().curc_ctx_set_scorer(self._h, mlp._h if mlp is not None else None,
                                               ctypes.c_float(t_map), ctypes.c_float(t_un)), "curc_ctx_set_scorer")
+
+    def get_tuning(self):
+        """Launch shapes in use (dict of curc_tuning fields)."""
+        t = Tuning()
+        self._check(lib().curc_ctx_get_tuning(self._h, ctypes.byref(t)), "curc_ctx_get_tuning")
+        return {n: getattr(t, n) for n, _ in Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        """Sets launch shapes (curc_tuning fields; the others keep their
+        values).  They change speed only, never results."""
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = Tuning(**cur)
+        self._check(lib().curc_ctx_set_tuning(self._h, ctypes.byref(t)), "curc_ctx_set_tuning")
 
     def set_trace(self, every=16, max_records=1 << 22):
         self._check(lib().curc_ctx_set_trace(self._h, every, max_records), "curc_ctx_set_trace")

@@ -101,8 +101,20 @@ struct PendingTimer {
 
 }  // namespace
 
+// Launch shapes that change speed only, never results (set by
+// curc_ctx_set_tuning / the autotuner; CURC_TUNE="key=value,..." at context
+// creation overrides them).
+struct Tuning {
+    uint32_t ov_grid = 4096;     // workgroups (cap) of the overlap probe and claim finalisation
+    uint32_t fm_grid = 4096;     // workgroups (cap) of the exact-matching probe and claim finalisation
+    uint32_t map_grid = 1u << 20;  // workgroups (cap) of the mapping kernels
+    uint32_t radix_items = 16;   // keys per thread of a radix-sort tile (8, 16, 32)
+    uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
+};
+
 struct curc_ctx {
     int device = 0;
+    Tuning tune;
     hipStream_t st = nullptr;
     Scratch S;
     std::string err;
@@ -435,7 +447,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     CURC_HIP(hipMemcpyAsync(sl[0], d_suf, (uint64_t)n_in * 4, hipMemcpyDeviceToDevice, c->st));
     hipLaunchKernelGGL(k_gather_key0_dev, dim3(grid_for(n_max)), dim3(256), 0, c->st, R, (const uint32_t*)pl[0],
                        (const uint32_t*)np[0], W, pk[0]);
-    const uint32_t g = grid_for(n_max, 4096);
+    const uint32_t g = grid_for(n_max, c->tune.fm_grid);
     if (ranged) fill<uint32_t>(c, rng, (1ull << (BITS * std::min(MAX_BP, L - 1))) + 1, 0u);
     auto loop = [&]() {
         int cur = 0;
@@ -559,6 +571,30 @@ void launch_map_range(bool rc, uint32_t wbk, dim3 grid, hipStream_t st, const Ma
     else
         CURC_MAP_RANGE(8);
 #undef CURC_MAP_RANGE
+}
+
+// "key=value,key=value" over the Tuning fields (unknown keys ignored).
+void parse_tuning(const char* text, Tuning& t) {
+    std::string s(text);
+    size_t i = 0;
+    while (i < s.size()) {
+        size_t j = s.find(',', i);
+        if (j == std::string::npos) j = s.size();
+        const std::string kv = s.substr(i, j - i);
+        const size_t e = kv.find('=');
+        if (e != std::string::npos) {
+            const std::string k = kv.substr(0, e);
+            const uint32_t v = (uint32_t)std::strtoul(kv.c_str() + e + 1, nullptr, 10);
+            if (v > 0) {
+                if (k == "ov_grid") t.ov_grid = v;
+                else if (k == "fm_grid") t.fm_grid = v;
+                else if (k == "map_grid") t.map_grid = v;
+                else if (k == "radix_items" && (v == 8 || v == 16 || v == 32)) t.radix_items = v;
+                else if (k == "workers" && v <= CURC_BATCH_MAX_WORKERS) t.workers = v;
+            }
+        }
+        i = j + 1;
+    }
 }
 
 template <typename T>
@@ -779,7 +815,7 @@ void map_phased(curc_ctx* c, const MapArgs& a, bool rc, uint64_t nm, uint32_t* c
     set_cnt(c, cnt + C_TMP, (uint32_t)nm);
     compact(cnt + C_TMP, PredNeedsMap{a.last}, EmitIota{act[0]}, nact[0], S, S_PRIM, st, nm);
     MapState ms{S.as<uint64_t>(S_MSEEN, 4 * nm), S.as<uint32_t>(S_MCNT, 4 * nm), S.as<uint8_t>(S_MSTOP, nm), nm};
-    const uint32_t g = grid_for(nm, 1u << 20);
+    const uint32_t g = grid_for(nm, c->tune.map_grid);
     hipLaunchKernelGGL(k_map_state_reset, dim3(g), dim3(256), 0, st, a, ms, (const uint32_t*)act[0],
                        (const uint32_t*)nact[0]);
     float* X = (learned || trace) ? S.as<float>(S_MX, nm * MAP_NFEAT) : nullptr;
@@ -1163,7 +1199,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             fill<uint32_t>(c, cont, (uint64_t)npre + 1, 0u);
             // one static suffix list for all offsets (its length is nsuf, the
             // host value of cnt[C_NSUF]); linked suffixes are skipped by flag
-            const uint32_t g = grid_for(nsuf, 4096);
+            const uint32_t g = grid_for(nsuf, c->tune.ov_grid);
             uint8_t* slinked = S.as<uint8_t>(S_SLINK, (uint64_t)nsuf + 1);
             fill<uint8_t>(c, slinked, (uint64_t)nsuf + 1, (uint8_t)0);
             uint64_t* sw = S.as<uint64_t>(S_SUFW, (uint64_t)nsuf * W + 1);
@@ -1354,7 +1390,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         Region r(c, "mapping");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
                                   Nascii, nN, last, word, work, P.max_mm, bk};
-                        const dim3 grid(grid_for(nm, 1u << 20));
+                        const dim3 grid(grid_for(nm, c->tune.map_grid));
                         if (bk && W <= 8)
                             launch_map_fast(rc, W, grid, st, a);
                         else if (rc)
@@ -1577,6 +1613,7 @@ curc_ctx* curc_ctx_create(int device) {
     c->device = device;
     const char* ix = std::getenv("CURC_INDEX");
     c->index_csr = ix && std::string(ix) == "csr";
+    if (const char* t = std::getenv("CURC_TUNE")) parse_tuning(t, c->tune);
     const char* gr = std::getenv("CURC_GRAPHS");
     c->graphs.enabled = gr && std::string(gr) == "1";
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
@@ -1699,6 +1736,7 @@ int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L,
     curc_block* B = new curc_block();
     try {
         CURC_HIP(hipSetDevice(c->device));
+        radix_items() = c->tune.radix_items;
         run_block(c, ascii, n_reads, L, stop_after <= 0 ? CURC_STAGE_ALL : stop_after, flags, B);
         CURC_HIP(hipGetLastError());
         CURC_HIP(hipStreamSynchronize(c->st));
@@ -1865,7 +1903,7 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
         if (status) status[j] = CURC_OK;
     }
     if (n_jobs == 0) return CURC_OK;
-    if (workers == 0) workers = CURC_BATCH_MAX_WORKERS;
+    if (workers == 0) workers = c->tune.workers;
     workers = std::min<uint32_t>(std::min<uint32_t>(workers, CURC_BATCH_MAX_WORKERS), n_jobs);
     while (c->workers.size() + 1 < workers) {
         curc_ctx* w = curc_ctx_create(c->device);
@@ -1877,6 +1915,7 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
         curc_ctx* wc = c->workers[w - 1];
         wc->timing = c->timing;
         wc->index_csr = c->index_csr;
+        wc->tune = c->tune;
         wc->graphs.enabled = c->graphs.enabled;
         // workers share the scorer's device arrays (read-only during mapping)
         wc->has_scorer = c->has_scorer;
@@ -2126,6 +2165,29 @@ int curc_ctx_set_trace(curc_ctx* c, uint32_t every, uint64_t max_records) {
     return CURC_OK;
 }
 
+int curc_ctx_get_tuning(curc_ctx* c, curc_tuning* t) {
+    if (!c || !t) return CURC_EINVAL;
+    t->ov_grid = c->tune.ov_grid;
+    t->fm_grid = c->tune.fm_grid;
+    t->map_grid = c->tune.map_grid;
+    t->radix_items = c->tune.radix_items;
+    t->workers = c->tune.workers;
+    return CURC_OK;
+}
+
+int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
+    if (!c || !t) return CURC_EINVAL;
+    if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
+        (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32))
+        return fail(c, CURC_EINVAL, "tuning value out of range");
+    c->tune.ov_grid = t->ov_grid;
+    c->tune.fm_grid = t->fm_grid;
+    c->tune.map_grid = t->map_grid;
+    c->tune.radix_items = t->radix_items;
+    c->tune.workers = t->workers;
+    return CURC_OK;
+}
+
 }  // extern "C"
 
 extern "C" {
@@ -2140,6 +2202,7 @@ int curc_ref_match(curc_ctx* c, const char* src, uint64_t src_len, const char* d
     curc_block* B = new curc_block();
     try {
         CURC_HIP(hipSetDevice(c->device));
+        radix_items() = c->tune.radix_items;
         Scratch& S = c->S;
         hipStream_t st = c->st;
         uint32_t* cnt = S.as<uint32_t>(S_CNT, C_COUNT);

@@ -195,18 +195,22 @@ inline void scan_excl(const uint32_t* in, uint32_t* out, uint64_t n, Scratch& S,
 }
 
 // ----------------------------------------------------------- radix sort ----
-constexpr uint32_t RADIX_ITEMS = 16;
-constexpr uint32_t RADIX_TILE = 256 * RADIX_ITEMS;
+// A tile is 256 threads x ITEMS keys; the tile size is a launch shape
+// (results do not depend on it), chosen per thread by radix_items().
+inline uint32_t& radix_items() {
+    static thread_local uint32_t items = 16;
+    return items;
+}
 
-template <typename K>
+template <typename K, uint32_t ITEMS>
 __global__ void __launch_bounds__(256) k_radix_hist(const K* __restrict__ keys, uint64_t n, uint32_t shift,
                                                     uint32_t* __restrict__ hist, uint32_t tiles) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    uint64_t base = (uint64_t)blockIdx.x * RADIX_TILE;
+    uint64_t base = (uint64_t)blockIdx.x * (256u * ITEMS);
 #pragma unroll 4
-    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+    for (uint32_t r = 0; r < ITEMS; ++r) {
         uint64_t i = base + r * 256 + threadIdx.x;
         if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
     }
@@ -217,7 +221,7 @@ __global__ void __launch_bounds__(256) k_radix_hist(const K* __restrict__ keys, 
 // Stable scatter: within a tile, element order is (round, wave, lane); each
 // wave ranks its lanes by digit with 8 ballots, waves are ordered through
 // per-wave digit counts in LDS.
-template <typename K>
+template <typename K, uint32_t ITEMS>
 __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                        uint64_t n, uint32_t shift,
@@ -228,9 +232,9 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin
     run[t] = offs[(uint64_t)t * tiles + blockIdx.x];
     wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * RADIX_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * (256u * ITEMS);
     const uint64_t lt = lanemask_lt();
-    for (uint32_t r = 0; r < RADIX_ITEMS; ++r) {
+    for (uint32_t r = 0; r < ITEMS; ++r) {
         uint64_t i = base + r * 256 + t;
         bool valid = i < n;
         K k = valid ? kin[i] : (K)0;
@@ -262,19 +266,30 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin
 // Sorts n (key, value) pairs by key bits [begin_bit, end_bit) in 8-bit
 // digits, stably.  Ping-pongs between (k, v) and (kt, vt); on return k and v
 // point at the sorted data.
+template <typename K, uint32_t ITEMS>
+void radix_sort_pairs_t(K*& k, uint32_t*& v, K*& kt, uint32_t*& vt, uint64_t n, uint32_t begin_bit,
+                        uint32_t end_bit, Scratch& S, int slot, hipStream_t st) {
+    constexpr uint32_t TILE = 256u * ITEMS;
+    uint32_t tiles = (uint32_t)((n + TILE - 1) / TILE);
+    uint32_t* hist = S.as<uint32_t>(slot, (size_t)256 * tiles);
+    for (uint32_t sh = begin_bit; sh < end_bit; sh += 8) {
+        hipLaunchKernelGGL((k_radix_hist<K, ITEMS>), dim3(tiles), dim3(256), 0, st, (const K*)k, n, sh, hist, tiles);
+        scan_excl(hist, hist, (uint64_t)256 * tiles, S, slot + 1, st);
+        hipLaunchKernelGGL((k_radix_scatter<K, ITEMS>), dim3(tiles), dim3(256), 0, st, (const K*)k,
+                           (const uint32_t*)v, kt, vt, n, sh, (const uint32_t*)hist, tiles);
+        K* tk = k; k = kt; kt = tk;
+        uint32_t* tv = v; v = vt; vt = tv;
+    }
+}
+
 template <typename K>
 void radix_sort_pairs(K*& k, uint32_t*& v, K*& kt, uint32_t*& vt, uint64_t n, uint32_t begin_bit,
                       uint32_t end_bit, Scratch& S, int slot, hipStream_t st) {
     if (n <= 1) return;
-    uint32_t tiles = (uint32_t)((n + RADIX_TILE - 1) / RADIX_TILE);
-    uint32_t* hist = S.as<uint32_t>(slot, (size_t)256 * tiles);
-    for (uint32_t sh = begin_bit; sh < end_bit; sh += 8) {
-        hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(256), 0, st, (const K*)k, n, sh, hist, tiles);
-        scan_excl(hist, hist, (uint64_t)256 * tiles, S, slot + 1, st);
-        hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(256), 0, st, (const K*)k, (const uint32_t*)v,
-                           kt, vt, n, sh, (const uint32_t*)hist, tiles);
-        K* tk = k; k = kt; kt = tk;
-        uint32_t* tv = v; v = vt; vt = tv;
+    switch (radix_items()) {
+        case 8: radix_sort_pairs_t<K, 8>(k, v, kt, vt, n, begin_bit, end_bit, S, slot, st); break;
+        case 32: radix_sort_pairs_t<K, 32>(k, v, kt, vt, n, begin_bit, end_bit, S, slot, st); break;
+        default: radix_sort_pairs_t<K, 16>(k, v, kt, vt, n, begin_bit, end_bit, S, slot, st); break;
     }
 }
 

@@ -273,6 +273,22 @@ int curc_ctx_set_scorer(curc_ctx *ctx, curc_mlp *m, float threshold_mapped, floa
  * `every` contribute one record per checkpoint, up to max_records. */
 int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);
 
+/* ---- launch shapes --------------------------------------------------------
+ * No counterpart in the reference.  Knobs that change speed only, never
+ * results: workgroup caps of the overlap probe, the exact-matching probe and
+ * the mapping kernels, keys per thread of a radix-sort tile, and the
+ * streams curc_batch_run uses when its caller passes 0.  curc_amd/autotune.py
+ * times candidate values on a sample block and sets the fastest. */
+typedef struct curc_tuning {
+    uint32_t ov_grid;     /* >= 1 (default 4096) */
+    uint32_t fm_grid;     /* >= 1 (default 4096) */
+    uint32_t map_grid;    /* >= 1 (default 2^20) */
+    uint32_t radix_items; /* 8, 16 or 32 (default 16) */
+    uint32_t workers;     /* 1 .. CURC_BATCH_MAX_WORKERS (default 4) */
+} curc_tuning;
+int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
+int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);
+
 #ifdef __cplusplus
 }
 #endif

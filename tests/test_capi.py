@@ -110,3 +110,17 @@ def test_scorer_parameter_layout_matches_reference():
     hdr = open(os.path.join(ROOT, "include", "curc_gpu.h")).read()
     n = int(re.search(r"#define CURC_MLP_NPARAMS (\d+)u", hdr).group(1))
     assert n == scorer.N_PARAMS == mlp.N_PARAMS == scorer.init_params(0).size
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    from curc_amd import autotune
+
+    assert autotune.size_class(1) == 1 and autotune.size_class(5) == 8 and autotune.size_class(1 << 20) == 1 << 20
+    p = str(tmp_path / "t.json")
+    assert autotune.load(p, 150, 1000) is None
+    t = dict(ov_grid=4096, fm_grid=2048, map_grid=32768, radix_items=16, workers=3)
+    autotune.save(p, 150, 1000, t)
+    autotune.save(p, 100, 1000, dict(t, workers=1))
+    assert autotune.load(p, 150, 1024) == t
+    assert autotune.load(p, 150, 1025) is None
+    assert autotune.load(p, 100, 900)["workers"] == 1
