@@ -104,11 +104,12 @@ struct PendingTimer {
 // Launch shapes that change speed only, never results (set by
 // curc_ctx_set_tuning / the autotuner; CURC_TUNE="key=value,..." at context
 // creation overrides them).
+// Defaults: the autotuner's picks on a 20M x 150 bp block (MI355X).
 struct Tuning {
-    uint32_t ov_grid = 4096;     // workgroups (cap) of the overlap probe and claim finalisation
+    uint32_t ov_grid = 16384;    // workgroups (cap) of the overlap probe and claim finalisation
     uint32_t fm_grid = 4096;     // workgroups (cap) of the exact-matching probe and claim finalisation
-    uint32_t map_grid = 1u << 20;  // workgroups (cap) of the mapping kernels
-    uint32_t radix_items = 16;   // keys per thread of a radix-sort tile (8, 16, 32)
+    uint32_t map_grid = 32768;   // workgroups (cap) of the mapping kernels
+    uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
 };
 

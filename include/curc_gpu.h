@@ -280,10 +280,10 @@ int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);
  * streams curc_batch_run uses when its caller passes 0.  curc_amd/autotune.py
  * times candidate values on a sample block and sets the fastest. */
 typedef struct curc_tuning {
-    uint32_t ov_grid;     /* >= 1 (default 4096) */
+    uint32_t ov_grid;     /* >= 1 (default 16384) */
     uint32_t fm_grid;     /* >= 1 (default 4096) */
-    uint32_t map_grid;    /* >= 1 (default 2^20) */
-    uint32_t radix_items; /* 8, 16 or 32 (default 16) */
+    uint32_t map_grid;    /* >= 1 (default 32768) */
+    uint32_t radix_items; /* 8, 16 or 32 (default 8) */
     uint32_t workers;     /* 1 .. CURC_BATCH_MAX_WORKERS (default 4) */
 } curc_tuning;
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
