@@ -16,23 +16,26 @@ import time
 from . import Context, archive, fastq
 
 
-def compress(inputs, output_name, block_ratio=1.0, preserve_order=False, working_dir=".", device=0, log=print):
+def compress(inputs, output_name, block_ratio=1.0, preserve_order=False, working_dir=".", device=0, log=print,
+             io_chunk=fastq.IO_CHUNK):
+    """All blocks of the input go to the GPU together (curc_batch_run: one
+    stream per block, up to four at a time), where the reference runs one
+    block task per block (preprocess.cpp:205-213)."""
     if not 1 <= len(inputs) <= 2:
         raise ValueError("one or two FASTQ inputs")
     paired = len(inputs) == 2
     out_path = os.path.join(working_dir, output_name + ".curc")
     blocks = []
-    L = None
-    total = 0
     t0 = time.perf_counter()
+    parts = list(fastq.read_blocks(inputs[0], inputs[1] if paired else None, block_ratio, io_chunk))
+    L = parts[0].shape[1] if parts else None
+    total = sum(len(r) for r in parts)
     with Context(device) as ctx:
-        for reads in fastq.read_blocks(inputs[0], inputs[1] if paired else None, block_ratio):
-            L = reads.shape[1]
-            blk = ctx.run_block(reads, streams=True, preserve_order=preserve_order, paired=paired, refmatch=True)
+        done = ctx.batch_run(parts, streams=True, preserve_order=preserve_order, paired=paired, refmatch=True)
+        for i, (reads, blk) in enumerate(zip(parts, done)):
             blocks.append(archive.BlockStreams.from_block(blk, len(reads), L, preserve_order, paired))
             blk.free()
-            total += len(reads)
-            log(f"block {len(blocks) - 1}: {len(reads)} reads")
+            log(f"block {i}: {len(reads)} reads")
     archive.write_archive(out_path, blocks, preserve_order, paired, L or 0)
     size = os.path.getsize(out_path)
     log(f"reads count : {total}; blocks : {len(blocks)}; archive size : {size} bytes; "

@@ -66,8 +66,10 @@ def _chunks(path, chunk):
         pos = end
 
 
-def read_blocks(path1, path2=None, block_ratio=1.0):
-    """Yields the blocks of a FASTQ input as (n, L) uint8 arrays."""
+def read_blocks(path1, path2=None, block_ratio=1.0, io_chunk=IO_CHUNK):
+    """Yields the blocks of a FASTQ input as (n, L) uint8 arrays.  io_chunk
+    is the reference's 50 MB read granularity (smaller values cut small
+    inputs into several blocks)."""
     if not 0 < block_ratio <= 1:
         raise ValueError("block_ratio must be in (0, 1]")  # main.cpp:111-115
     paired = path2 is not None
@@ -75,7 +77,7 @@ def read_blocks(path1, path2=None, block_ratio=1.0):
     if paired and os.path.getsize(path2) != size1:
         raise FastqError("paired-end file size isn't equal")  # preprocess.cpp:83
     block_bytes = (2 if paired else 1) * size1 * block_ratio
-    chunk = IO_CHUNK // 2 if paired else IO_CHUNK
+    chunk = io_chunk // 2 if paired else io_chunk
     it1 = _chunks(path1, chunk)
     it2 = _chunks(path2, chunk) if paired else None
     L = None
