@@ -66,7 +66,8 @@ class Job(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_uint32) for n in ("ov_grid", "fm_grid", "map_grid", "radix_items", "workers")]
+    _fields_ = [(n, ctypes.c_uint32) for n in ("ov_grid", "fm_grid", "map_grid", "radix_items", "workers",
+                                               "filter_bits")]
 
 
 class StageStat(ctypes.Structure):

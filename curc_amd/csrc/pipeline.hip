@@ -110,6 +110,7 @@ struct Tuning {
     uint32_t fm_grid = 4096;     // workgroups (cap) of the exact-matching probe and claim finalisation
     uint32_t map_grid = 32768;   // workgroups (cap) of the mapping kernels
     uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
+    uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
 };
 
@@ -591,6 +592,7 @@ void parse_tuning(const char* text, Tuning& t) {
                 else if (k == "fm_grid") t.fm_grid = v;
                 else if (k == "map_grid") t.map_grid = v;
                 else if (k == "radix_items" && (v == 8 || v == 16 || v == 32)) t.radix_items = v;
+                else if (k == "filter_bits" && v >= 8 && v <= 128 && (v & (v - 1)) == 0) t.filter_bits = v;
                 else if (k == "workers" && v <= CURC_BATCH_MAX_WORKERS) t.workers = v;
             }
         }
@@ -1156,10 +1158,10 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         // bucket slots up to 2^28 buckets (16 GiB), the CSR arrays alone beyond
         PrefixSlot* pslots =
             (H <= PSLOT_MAX_H && !c->index_csr) ? S.as<PrefixSlot>(S_PSLOT, (uint64_t)H) : nullptr;
-        // prefix k-mer filter: >= 32 bits per prefix (2^20..2^32 bits), so a
-        // probe without an equal k-mer usually stops at one bit
+        // prefix k-mer filter: >= filter_bits bits per prefix (2^20..2^32
+        // bits), so a probe without an equal k-mer usually stops at one bit
         uint64_t fbits = 1ull << 20;
-        while (fbits < 32ull * npre && fbits < (1ull << 32)) fbits <<= 1;
+        while (fbits < (uint64_t)c->tune.filter_bits * npre && fbits < (1ull << 32)) fbits <<= 1;
         const uint32_t fmask = (uint32_t)(fbits - 1);
         const uint64_t fwords = fbits / 32;
         uint32_t* pfilt = S.as<uint32_t>(S_PFILT, fwords);
@@ -2173,19 +2175,22 @@ int curc_ctx_get_tuning(curc_ctx* c, curc_tuning* t) {
     t->map_grid = c->tune.map_grid;
     t->radix_items = c->tune.radix_items;
     t->workers = c->tune.workers;
+    t->filter_bits = c->tune.filter_bits;
     return CURC_OK;
 }
 
 int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!c || !t) return CURC_EINVAL;
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
-        (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32))
+        (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
+        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)))
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;
     c->tune.fm_grid = t->fm_grid;
     c->tune.map_grid = t->map_grid;
     c->tune.radix_items = t->radix_items;
     c->tune.workers = t->workers;
+    c->tune.filter_bits = t->filter_bits;
     return CURC_OK;
 }
 

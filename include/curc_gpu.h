@@ -276,8 +276,9 @@ int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);
 /* ---- launch shapes --------------------------------------------------------
  * No counterpart in the reference.  Knobs that change speed only, never
  * results: workgroup caps of the overlap probe, the exact-matching probe and
- * the mapping kernels, keys per thread of a radix-sort tile, and the
- * streams curc_batch_run uses when its caller passes 0.  curc_amd/autotune.py
+ * the mapping kernels, keys per thread of a radix-sort tile, the streams
+ * curc_batch_run uses when its caller passes 0, and the size of the
+ * overlap stage's prefix k-mer filter.  curc_amd/autotune.py
  * times candidate values on a sample block and sets the fastest. */
 typedef struct curc_tuning {
     uint32_t ov_grid;     /* >= 1 (default 16384) */
@@ -285,6 +286,7 @@ typedef struct curc_tuning {
     uint32_t map_grid;    /* >= 1 (default 32768) */
     uint32_t radix_items; /* 8, 16 or 32 (default 8) */
     uint32_t workers;     /* 1 .. CURC_BATCH_MAX_WORKERS (default 4) */
+    uint32_t filter_bits; /* overlap k-mer filter bits per prefix: 8 .. 128, a power of two (default 16) */
 } curc_tuning;
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
 int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);
