@@ -2,8 +2,9 @@
 
 The knobs change speed only, never results: workgroup caps of the overlap
 probe, the exact-matching probe and the mapping kernels, keys per thread of
-a radix-sort tile and the bits per prefix of the overlap k-mer filter for
-one block; the number of streams for a batch.  `tune_block` times whole blocks of a sample input under candidate
+a radix-sort tile, the bits per prefix of the overlap k-mer filter and the
+list bound of the fused exact-matching loop for one block; the number of
+streams for a batch.  `tune_block` times whole blocks of a sample input under candidate
 values, one knob at a time with the others fixed (coordinate descent,
 best of `reps` runs per candidate), and leaves the fastest setting on the
 context.  `tune_workers` does the same for curc_batch_run's stream count
@@ -20,6 +21,7 @@ BLOCK_KNOBS = (
     ("fm_grid", (2048, 4096, 16384)),
     ("radix_items", (8, 16, 32)),
     ("filter_bits", (16, 32, 64)),
+    ("fm_fused", (0, 1024, 4096)),
 )
 
 

@@ -525,8 +525,8 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
 // bases) bounds the search in the lexicographic prefix list; first equal
 // non-self prefix.
 template <class E>
-__global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t* __restrict__ rng, uint32_t bp,
-                                                  uint32_t off) {
+__device__ __forceinline__ void fm_probe_body(const ClaimState& cs, const uint32_t* __restrict__ rng, uint32_t bp,
+                                              uint32_t off) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
     const uint32_t ns = *cs.d_ns;
     const uint32_t len = cs.L - off;
@@ -575,12 +575,18 @@ __global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t*
     }
 }
 
+template <class E>
+__global__ void __launch_bounds__(256) k_fm_probe(ClaimState cs, const uint32_t* __restrict__ rng, uint32_t bp,
+                                                  uint32_t off) {
+    fm_probe_body<E>(cs, rng, bp, off);
+}
+
 // Uncontended groups link at once; members of contended groups are queued
 // for their leader.
 // zero_p / zero_n: a buffer the next round needs cleared (the block ranges
 // of the exact-matching loop), folded in here to save a launch.
-__global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t off, uint32_t* zero_p = nullptr,
-                                                        uint64_t zero_n = 0) {
+__device__ __forceinline__ void claim_finalize_body(const ClaimState& cs, uint32_t off, uint32_t* zero_p,
+                                                    uint64_t zero_n) {
     GRID_STRIDE(z, zero_n) zero_p[z] = 0;
     const uint32_t ns = *cs.d_ns;
     GRID_STRIDE(i, ns) {
@@ -596,6 +602,11 @@ __global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t 
             cs.link((uint32_t)i, ch, off);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_claim_finalize(ClaimState cs, uint32_t off, uint32_t* zero_p = nullptr,
+                                                        uint64_t zero_n = 0) {
+    claim_finalize_body(cs, off, zero_p, zero_n);
 }
 
 // Suffix order inside a group: a6 keeps the suffix list order (compress.cu:
@@ -693,7 +704,7 @@ __device__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint
 // free part of the run is always {P[a]} followed by P[b..], so one pass
 // suffices.
 template <bool FULLMATCH, class E>
-__global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t off) {
+__device__ __forceinline__ void claim_resolve_body(const ClaimState& cs, uint32_t off) {
     const uint32_t nq = *cs.res_n;
     const uint32_t np = *cs.d_np;
     const uint32_t len = cs.L - off;
@@ -736,6 +747,11 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
         cs.head[g] = INV;
         cs.contended[g] = 0;
     }
+}
+
+template <bool FULLMATCH, class E>
+__global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t off) {
+    claim_resolve_body<FULLMATCH, E>(cs, off);
 }
 
 // Histogram of the prefix list's block values (its first bp bases); an
@@ -787,6 +803,36 @@ struct EmitPre {
         pk2[j] = pk[i];
     }
 };
+
+// The whole exact-matching offset loop for short lists in one launch of one
+// workgroup: per offset the probe, the claim finalisation and resolution
+// and the two stable compactions run as phases separated by workgroup
+// barriers (the same device code as the per-offset kernels, whose
+// grid-stride loops cover the lists with this workgroup's threads).  c0 / c1
+// are the claim states of the ping-pong list pairs; the loop ends early once
+// either list is empty.
+template <class E>
+__global__ void __launch_bounds__(1024) k_fm_fused(ClaimState c0, ClaimState c1, uint32_t L) {
+    int cur = 0;
+    for (uint32_t off = 1; off < L; ++off) {
+        const ClaimState& cs = cur ? c1 : c0;
+        const ClaimState& ct = cur ? c0 : c1;
+        fm_probe_body<E>(cs, nullptr, 0, off);
+        __syncthreads();
+        claim_finalize_body(cs, off, nullptr, 0);
+        __syncthreads();
+        claim_resolve_body<true, E>(cs, off);
+        __syncthreads();
+        compact_small_body(cs.d_ns, PredFlagClear{cs.slinked}, EmitGather{cs.sl, const_cast<uint32_t*>(ct.sl)},
+                           const_cast<uint32_t*>(ct.d_ns));
+        compact_small_body(cs.d_np, PredFlagClear{cs.plinked},
+                           EmitPre{cs.pl, cs.pk, const_cast<uint32_t*>(ct.pl), const_cast<uint64_t*>(ct.pk)},
+                           const_cast<uint32_t*>(ct.d_np));
+        __syncthreads();
+        cur ^= 1;
+        if (*ct.d_ns == 0 || *ct.d_np == 0) break;
+    }
+}
 
 // ----------------------------------------------------------------- prune ---
 // compress.cu:1539-1564

@@ -111,6 +111,7 @@ struct Tuning {
     uint32_t map_grid = 32768;   // workgroups (cap) of the mapping kernels
     uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
     uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
+    uint32_t fm_fused = 4096;    // exact-matching lists up to this bound run in one fused launch (0: never)
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
 };
 
@@ -409,6 +410,8 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     // short lists (<= FM_SHORT) go without block ranges, searching the whole
     // prefix list, which saves the fill / histogram / scan launches
     constexpr uint32_t FM_SHORT = 1u << 12;
+    // lists up to this bound run the fused single-workgroup loop
+    const uint32_t FM_FUSED = c->tune.fm_fused;
     if (n_max == 0) return;
     // bounds are rounded up to a power of two (>= FM_SHORT) so that blocks
     // of similar size share buffer sizes, grids and hence captured graphs;
@@ -450,6 +453,20 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     hipLaunchKernelGGL(k_gather_key0_dev, dim3(grid_for(n_max)), dim3(256), 0, c->st, R, (const uint32_t*)pl[0],
                        (const uint32_t*)np[0], W, pk[0]);
     const uint32_t g = grid_for(n_max, c->tune.fm_grid);
+    if (!ranged && n_max <= FM_FUSED) {
+        // short lists: the whole offset loop in one single-workgroup launch
+        ClaimState c0{R, W, L, sl[0], ns[0], pl[0], pk[0], np[0], next, prev, offset,
+                      choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
+                      kpool, slinked, plinked};
+        ClaimState c1 = c0;
+        c1.sl = sl[1];
+        c1.d_ns = ns[1];
+        c1.pl = pl[1];
+        c1.pk = pk[1];
+        c1.d_np = np[1];
+        hipLaunchKernelGGL((k_fm_fused<E>), dim3(1), dim3(1024), 0, c->st, c0, c1, L);
+        return;
+    }
     if (ranged) fill<uint32_t>(c, rng, (1ull << (BITS * std::min(MAX_BP, L - 1))) + 1, 0u);
     auto loop = [&]() {
         int cur = 0;
@@ -587,6 +604,7 @@ void parse_tuning(const char* text, Tuning& t) {
         if (e != std::string::npos) {
             const std::string k = kv.substr(0, e);
             const uint32_t v = (uint32_t)std::strtoul(kv.c_str() + e + 1, nullptr, 10);
+            if (k == "fm_fused") t.fm_fused = v;
             if (v > 0) {
                 if (k == "ov_grid") t.ov_grid = v;
                 else if (k == "fm_grid") t.fm_grid = v;
@@ -1156,8 +1174,10 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint32_t* kr = S.as<uint32_t>(S_KR, (uint64_t)H + 2);
         uint64_t* vk = S.as<uint64_t>(S_VK, (uint64_t)npre + 1);
         // bucket slots up to 2^28 buckets (16 GiB), the CSR arrays alone beyond
-        PrefixSlot* pslots =
-            (H <= PSLOT_MAX_H && !c->index_csr) ? S.as<PrefixSlot>(S_PSLOT, (uint64_t)H) : nullptr;
+        // and for sparse tables (fewer than H / 8 prefixes)
+        PrefixSlot* pslots = (H <= PSLOT_MAX_H && !c->index_csr && (uint64_t)npre * 8 >= H)
+                                 ? S.as<PrefixSlot>(S_PSLOT, (uint64_t)H)
+                                 : nullptr;
         // prefix k-mer filter: >= filter_bits bits per prefix (2^20..2^32
         // bits), so a probe without an equal k-mer usually stops at one bit
         uint64_t fbits = 1ull << 20;
@@ -1351,7 +1371,9 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     const uint64_t start = parts[pi][0], count = parts[pi][1];
                     const uint32_t H = hash_size(count);
                     // buckets inline in 64-byte slots up to 2^28 of them (16 GiB), CSR beyond
-                    const bool inline_bk = H <= MAP_SLOT_MAX_H && !c->index_csr;
+                    // and for sparse tables (fewer than H / 8 entries: the reference's
+                    // minimum of 2^24 buckets would cost 1 GiB of slots for a small block)
+                    const bool inline_bk = H <= MAP_SLOT_MAX_H && !c->index_csr && count * 8 >= H;
                     uint32_t* kr = inline_bk ? nullptr : S.as<uint32_t>(S_MKR, (uint64_t)H + 2);
                     uint32_t* pos = inline_bk ? nullptr : S.as<uint32_t>(S_MPOS, count + 2);
                     uint32_t* bk = inline_bk ? S.as<uint32_t>(S_MBK, (uint64_t)H * MAP_SLOT) : nullptr;
@@ -2176,6 +2198,7 @@ int curc_ctx_get_tuning(curc_ctx* c, curc_tuning* t) {
     t->radix_items = c->tune.radix_items;
     t->workers = c->tune.workers;
     t->filter_bits = c->tune.filter_bits;
+    t->fm_fused = c->tune.fm_fused;
     return CURC_OK;
 }
 
@@ -2191,6 +2214,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     c->tune.radix_items = t->radix_items;
     c->tune.workers = t->workers;
     c->tune.filter_bits = t->filter_bits;
+    c->tune.fm_fused = t->fm_fused;
     return CURC_OK;
 }
 

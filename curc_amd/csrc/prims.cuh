@@ -357,8 +357,8 @@ __global__ void __launch_bounds__(256) k_compact_scatter(const uint32_t* __restr
 // One workgroup of 1024 threads; each round covers 16 consecutive items per
 // thread.  Correct for any n, fast for short lists.
 template <typename Pred, typename Emit>
-__global__ void __launch_bounds__(1024) k_compact_small(const uint32_t* __restrict__ d_n, Pred pred, Emit emit,
-                                                        uint32_t* __restrict__ d_out_n) {
+__device__ __forceinline__ void compact_small_body(const uint32_t* __restrict__ d_n, Pred pred, Emit emit,
+                                                   uint32_t* __restrict__ d_out_n) {
     __shared__ uint32_t lds[16];
     const uint32_t n = *d_n;
     const uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
@@ -392,6 +392,12 @@ __global__ void __launch_bounds__(1024) k_compact_small(const uint32_t* __restri
         o += tot;
     }
     if (t == 0) *d_out_n = o;
+}
+
+template <typename Pred, typename Emit>
+__global__ void __launch_bounds__(1024) k_compact_small(const uint32_t* __restrict__ d_n, Pred pred, Emit emit,
+                                                        uint32_t* __restrict__ d_out_n) {
+    compact_small_body(d_n, pred, emit, d_out_n);
 }
 
 // Stable compaction: emit(i, j) is called for the j-th i in [0, *d_n) with

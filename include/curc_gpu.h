@@ -277,8 +277,9 @@ int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);
  * No counterpart in the reference.  Knobs that change speed only, never
  * results: workgroup caps of the overlap probe, the exact-matching probe and
  * the mapping kernels, keys per thread of a radix-sort tile, the streams
- * curc_batch_run uses when its caller passes 0, and the size of the
- * overlap stage's prefix k-mer filter.  curc_amd/autotune.py
+ * curc_batch_run uses when its caller passes 0, the size of the overlap
+ * stage's prefix k-mer filter, and the list bound below which an
+ * exact-matching loop runs fused in one launch.  curc_amd/autotune.py
  * times candidate values on a sample block and sets the fastest. */
 typedef struct curc_tuning {
     uint32_t ov_grid;     /* >= 1 (default 16384) */
@@ -287,6 +288,8 @@ typedef struct curc_tuning {
     uint32_t radix_items; /* 8, 16 or 32 (default 8) */
     uint32_t workers;     /* 1 .. CURC_BATCH_MAX_WORKERS (default 4) */
     uint32_t filter_bits; /* overlap k-mer filter bits per prefix: 8 .. 128, a power of two (default 16) */
+    uint32_t fm_fused;    /* exact-matching lists up to this bound run their offset loop in one
+                             single-workgroup launch; 0: never (default 4096) */
 } curc_tuning;
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
 int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);

@@ -118,7 +118,7 @@ def test_autotune_cache_round_trip(tmp_path):
     assert autotune.size_class(1) == 1 and autotune.size_class(5) == 8 and autotune.size_class(1 << 20) == 1 << 20
     p = str(tmp_path / "t.json")
     assert autotune.load(p, 150, 1000) is None
-    t = dict(ov_grid=4096, fm_grid=2048, map_grid=32768, radix_items=16, workers=3, filter_bits=32)
+    t = dict(ov_grid=4096, fm_grid=2048, map_grid=32768, radix_items=16, workers=3, filter_bits=32, fm_fused=4096)
     autotune.save(p, 150, 1000, t)
     autotune.save(p, 100, 1000, dict(t, workers=1))
     assert autotune.load(p, 150, 1024) == t

@@ -21,10 +21,10 @@ def reads_of(seed, n, L, genome):
 
 
 TUNINGS = [
-    dict(ov_grid=1, fm_grid=1, map_grid=1, radix_items=8, filter_bits=8),
-    dict(ov_grid=7, fm_grid=3, map_grid=5, radix_items=32, filter_bits=128),
-    dict(ov_grid=16384, fm_grid=16384, map_grid=32768, radix_items=16),
-    dict(ov_grid=4096, fm_grid=4096, map_grid=1 << 20, radix_items=16),
+    dict(ov_grid=1, fm_grid=1, map_grid=1, radix_items=8, filter_bits=8, fm_fused=0),
+    dict(ov_grid=7, fm_grid=3, map_grid=5, radix_items=32, filter_bits=128, fm_fused=1 << 20),
+    dict(ov_grid=16384, fm_grid=16384, map_grid=32768, radix_items=16, fm_fused=4096),
+    dict(ov_grid=4096, fm_grid=4096, map_grid=1 << 20, radix_items=16, fm_fused=0),
 ]
 
 
