@@ -723,23 +723,41 @@ __device__ __forceinline__ void claim_resolve_body(const ClaimState& cs, uint32_
             sort_members_keyed<E>(cs, arr, cs.kpool + base, m, off);
         else
             sort_members<FULLMATCH, E>(cs, arr, m, off);
+        // All members share s[off:L], and the prefixes equal to it form one
+        // run [g, e) of the sorted list (one bucket's sorted run in a6), which
+        // starts at g.  Its end is found once, by galloping, instead of one
+        // comparison per linked member.
+        const uint32_t s0 = cs.sl[arr[0]];
+        const uint64_t* rs0 = cs.R + (uint64_t)s0 * cs.W;
+        const uint64_t sx00 = E::word(rs0, off, n1);
+        auto equal = [&](uint32_t x) { return cmp_ps<E>(cs.R, cs.W, cs.pl[x], cs.pk[x], rs0, sx00, off, len) == 0; };
+        uint32_t known = g, e = g + 1;  // [g, known] equal; e: first candidate past it
+        uint64_t step = 1;
+        while (e < np && equal(e)) {
+            known = e;
+            step <<= 1;
+            e = ((uint64_t)(np - g) > step) ? g + (uint32_t)step : np;
+        }
+        while (e - known > 1) {
+            const uint32_t mid = known + (e - known) / 2;
+            if (equal(mid))
+                known = mid;
+            else
+                e = mid;
+        }
         auto next_free = [&](uint32_t x) {
-            while (x < np && cs.prev[cs.pl[x]] != INV) ++x;
+            while (x < e && cs.prev[cs.pl[x]] != INV) ++x;
             return x;
         };
         uint32_t a = next_free(g);
         uint32_t b = next_free(a + 1);
-        for (uint32_t k = 0; k < m && a < np; ++k) {
+        for (uint32_t k = 0; k < m && a < e; ++k) {
             const uint32_t s = cs.sl[arr[k]];
-            const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-            const uint64_t sx0 = E::word(rs, off, n1);
-            const uint32_t pa = cs.pl[a];
-            if (pa != s) {
-                if (cmp_ps<E>(cs.R, cs.W, pa, cs.pk[a], rs, sx0, off, len) != 0) break;  // run exhausted
+            if (cs.pl[a] != s) {
                 cs.link(arr[k], a, off);
                 a = b;
                 b = next_free(b + 1);
-            } else if (b < np && cmp_ps<E>(cs.R, cs.W, cs.pl[b], cs.pk[b], rs, sx0, off, len) == 0) {
+            } else if (b < e) {
                 cs.link(arr[k], b, off);
                 b = next_free(b + 1);
             }
