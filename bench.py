@@ -214,7 +214,7 @@ def batched(ins, reps=2, learned=True):
     with ca.Context(0) as ctx:
         for b in ctx.batch_run(ins[:4], workers=4):
             b.free()
-        workers, wtimes = autotune.tune_workers(ctx, ins[::4], log=log)
+        workers, wtimes = autotune.tune_workers(ctx, ins[::2], reps=2, log=log)
         t = time.perf_counter()
         for _ in range(reps):
             for r in ins:
@@ -349,6 +349,14 @@ def main():
         log(f"  {s['name']:<14} {s['ms'] / a.steps:9.2f} ms  ({s['launches'] // max(1, a.steps)} regions/step)")
     log("mapping work per step (both strand passes): " + ", ".join(f"{k[4:]} {v}" for k, v in work.items()))
     roof = dominant_roofline(stats, a.steps, default_workload=(a.reads, a.genome) == (20_000_000, 100_000_000))
+    if roof:
+        # the measured ceiling beside the vendor peak: a plain copy of 2 GiB
+        copy = ctx.stream_copy_gbs()
+        roof["measured_copy_GBs"] = round(copy, 1)
+        roof["frac_of_measured_copy"] = round(roof["achieved"] / copy, 5)
+        if roof.get("traffic"):
+            roof["traffic_GBs"] = round(roof["traffic"] / (roof["avg_launch_ms"] * 1e-3) / 1e9, 1)
+        log(f"stream copy: {copy:.0f} GB/s measured (vendor peak {HBM_PEAK_GBS:.0f} GB/s)")
     learned = None
     if not a.no_learned:
         learned = learned_run(ctx, a, L, d_ascii, ms, stats)

@@ -112,6 +112,7 @@ def lib():
         "curc_memcpy_h2d": (i32, [vp, vp, vp, u64]),
         "curc_memcpy_d2h": (i32, [vp, vp, vp, u64]),
         "curc_sync": (i32, [vp]),
+        "curc_stream_copy": (i32, [vp, vp, vp, u64, i32, ctypes.POINTER(ctypes.c_float)]),
         "curc_device_sync": (i32, [vp]),
         "curc_pack": (i32, [vp, vp, u64, u32, vp, vp]),
         "curc_block_run": (i32, [vp, vp, u64, u32, i32, u32, ctypes.POINTER(vp)]),
@@ -355,6 +356,19 @@ class Context:
         cur.update(kw)
         t = Tuning(**cur)
         self._check(lib().curc_ctx_set_tuning(self._h, ctypes.byref(t)), "curc_ctx_set_tuning")
+
+    def stream_copy_gbs(self, nbytes=1 << 31, reps=5):
+        """Measured device copy bandwidth (read + write bytes per second, GB/s)
+        of a plain 16-byte-word copy kernel over nbytes."""
+        src = self.dev_alloc(nbytes)
+        dst = self.dev_alloc(nbytes)
+        try:
+            ms = ctypes.c_float()
+            self._check(lib().curc_stream_copy(self._h, dst, src, nbytes, reps, ctypes.byref(ms)), "curc_stream_copy")
+            return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+        finally:
+            self.dev_free(src)
+            self.dev_free(dst)
 
     def set_trace(self, every=16, max_records=1 << 22):
         self._check(lib().curc_ctx_set_trace(self._h, every, max_records), "curc_ctx_set_trace")

@@ -19,6 +19,13 @@ namespace curc {
 #define GRID_STRIDE(i, n) \
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
 
+// Plain device copy in 16-byte words (the measured bandwidth ceiling the
+// benchmark reports beside the vendor peak).
+__global__ void __launch_bounds__(256) k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     uint64_t n) {
+    GRID_STRIDE(i, n) dst[i] = src[i];
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, uint64_t n, T v) {
     GRID_STRIDE(i, n) p[i] = v;

@@ -1715,6 +1715,31 @@ int curc_memcpy_d2h(curc_ctx* c, void* d, const void* s, uint64_t bytes) {
         return fail(c, CURC_EHIP, "device to host copy failed");
     return CURC_OK;
 }
+int curc_stream_copy(curc_ctx* c, void* dst, const void* src, uint64_t bytes, int reps, float* ms) {
+    if (!c || !dst || !src || reps < 1 || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return CURC_EINVAL;
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        const uint64_t n = bytes / 16;
+        const dim3 grid(grid_for(n, 1u << 16));
+        hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, c->st, (const uint4*)src, (uint4*)dst, n);  // warm
+        hipEvent_t a = c->ev(), b = c->ev();
+        CURC_HIP(hipEventRecord(a, c->st));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, c->st, (const uint4*)src, (uint4*)dst, n);
+        CURC_HIP(hipEventRecord(b, c->st));
+        CURC_HIP(hipEventSynchronize(b));
+        float t = 0;
+        CURC_HIP(hipEventElapsedTime(&t, a, b));
+        c->event_pool.push_back(a);
+        c->event_pool.push_back(b);
+        if (ms) *ms = t / (float)reps;
+    } catch (const std::exception& e) {
+        return fail(c, CURC_EHIP, e.what());
+    }
+    return CURC_OK;
+}
+
 int curc_sync(curc_ctx* c) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);

@@ -71,6 +71,11 @@ int curc_dev_alloc(curc_ctx *ctx, void **ptr, uint64_t bytes);
 int curc_dev_free(curc_ctx *ctx, void *ptr);
 int curc_memcpy_h2d(curc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int curc_memcpy_d2h(curc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+/* No counterpart in the reference.  Copies `bytes` (a multiple of 16, both
+ * pointers 16-byte aligned, device memory) `reps` times with a plain
+ * 16-byte-word copy kernel and returns the mean time per copy in *ms: the
+ * measured bandwidth ceiling beside the vendor peak. */
+int curc_stream_copy(curc_ctx *ctx, void *dst, const void *src, uint64_t bytes, int reps, float *ms);
 int curc_sync(curc_ctx *ctx);
 /* Waits for all work on the context's device (hipDeviceSynchronize). */
 int curc_device_sync(curc_ctx *ctx);

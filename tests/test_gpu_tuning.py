@@ -61,3 +61,25 @@ def test_autotune_sets_a_candidate():
         pick, times = autotune.tune_workers(ctx, [reads_of(941 + i, 3_000, 100, 30_000) for i in range(4)],
                                             values=(1, 2))
         assert ctx.get_tuning()["workers"] == pick
+
+
+def test_stream_copy_ceiling(gpu_ctx):
+    """The copy kernel behind the measured bandwidth ceiling moves the data
+    and reports a plausible rate; misaligned sizes are refused."""
+    n = 1 << 24
+    src = gpu_ctx.dev_alloc(n)
+    dst = gpu_ctx.dev_alloc(n)
+    try:
+        data = np.random.default_rng(3).integers(0, 256, n, dtype=np.uint8)
+        gpu_ctx.h2d(src, data)
+        ms = ca.ctypes.c_float()
+        assert ca.lib().curc_stream_copy(gpu_ctx._h, dst, src, n, 2, ca.ctypes.byref(ms)) == ca.CURC_OK
+        out = np.empty(n, dtype=np.uint8)
+        gpu_ctx.d2h(out, dst)
+        assert np.array_equal(out, data)
+        assert ms.value > 0
+        assert ca.lib().curc_stream_copy(gpu_ctx._h, dst, src, n - 8, 1, ca.ctypes.byref(ms)) != ca.CURC_OK
+    finally:
+        gpu_ctx.dev_free(src)
+        gpu_ctx.dev_free(dst)
+    assert gpu_ctx.stream_copy_gbs(1 << 28, 2) > 500
