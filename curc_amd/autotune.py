@@ -3,7 +3,8 @@
 The knobs change speed only, never results: workgroup caps of the overlap
 probe, the exact-matching probe and the mapping kernels, keys per thread of
 a radix-sort tile, the bits per prefix of the overlap k-mer filter and the
-list bound of the fused exact-matching loop for one block; the number of
+list bound of the fused exact-matching loop and the form of the overlap
+probe for one block; the number of
 streams for a batch.  `tune_block` times whole blocks of a sample input under candidate
 values, one knob at a time with the others fixed (coordinate descent,
 best of `reps` runs per candidate), and leaves the fastest setting on the
@@ -17,7 +18,8 @@ import time
 
 BLOCK_KNOBS = (
     ("map_grid", (4096, 8192, 16384, 32768, 65536, 1 << 20)),
-    ("ov_grid", (2048, 4096, 16384)),
+    ("ov_queue", (0, 1)),
+    ("ov_grid", (1024, 2048, 4096, 16384)),
     ("fm_grid", (2048, 4096, 16384)),
     ("radix_items", (8, 16, 32)),
     ("filter_bits", (16, 32, 64)),

@@ -423,6 +423,93 @@ __global__ void k_gather_soa(const uint64_t* __restrict__ R, const uint32_t* __r
     }
 }
 
+// Candidate step of the a6 probe for suffix index i whose k-mer hashed to
+// bucket h: lower bound of s[off:L] in the bucket, then the first entry of
+// the equal run that is neither s nor taken.  Sets *grp to the run start
+// (the group id) and returns the chosen prefix position or INV.  With
+// slots, buckets of up to PSLOT_N entries are searched in registers (the
+// same lower bound and the same first-free scan).
+__device__ __forceinline__ uint32_t ov_candidate(const ClaimState& cs, const uint32_t* __restrict__ kr,
+                                                 const PrefixSlot* __restrict__ slots,
+                                                 const uint64_t* __restrict__ sw, uint32_t ns, uint32_t i,
+                                                 uint32_t h, uint32_t off, uint32_t* grp) {
+    const uint32_t len = cs.L - off;
+    const uint32_t n1 = min(32u, len);
+    const uint32_t s = cs.sl[i];
+    const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
+    const uint64_t sx0 = xw_soa(sw, ns, i, off, n1);
+    uint32_t lo, e;
+    if (slots) {
+        const PrefixSlot sl = slots[h];
+        lo = sl.lo;
+        e = lo + sl.cnt;
+        if (sl.cnt <= PSLOT_N) {
+            // lower bound: first entry not below s[off:L]
+            uint32_t b = sl.cnt;
+            bool found = false;
+#pragma unroll
+            for (uint32_t t = 0; t < PSLOT_N; ++t) {
+                if (!found && t < sl.cnt && cmp_ps<Enc2>(cs.R, cs.W, sl.id[t], sl.key[t], rs, sx0, off, len) >= 0) {
+                    b = t;
+                    found = true;
+                }
+            }
+            // first entry from there that is neither s nor taken decides;
+            // entries are sorted, so past the equal run nothing can match
+            // and the taken flags need not be read
+            uint32_t ch = INV;
+            bool done = false;
+#pragma unroll
+            for (uint32_t t = 0; t < PSLOT_N; ++t) {
+                if (!done && t >= b && t < sl.cnt) {
+                    const uint32_t p = sl.id[t];
+                    if (cmp_ps<Enc2>(cs.R, cs.W, p, sl.key[t], rs, sx0, off, len) != 0) {
+                        done = true;
+                    } else if (p != s && cs.prev[p] == INV) {
+                        ch = lo + t;
+                        done = true;
+                    }
+                }
+            }
+            *grp = lo + b;
+            return ch;
+        }
+    } else {
+        lo = kr[h];
+        e = kr[h + 1];
+    }
+    uint32_t cnt = e - lo;
+    while (cnt > 0) {
+        uint32_t step = cnt >> 1, it = lo + step;
+        if (cmp_ps<Enc2>(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
+            lo = it + 1;
+            cnt -= step + 1;
+        } else {
+            cnt = step;
+        }
+    }
+    // the first entry that is neither s nor taken decides; past the run of
+    // entries equal to s[off:L] nothing can match (sorted bucket)
+    uint32_t ch = INV;
+    for (uint32_t x = lo; x < e; ++x) {
+        uint32_t p = cs.pl[x];
+        if (cmp_ps<Enc2>(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) != 0) break;
+        if (p == s || cs.prev[p] != INV) continue;
+        ch = x;
+        break;
+    }
+    *grp = lo;
+    return ch;
+}
+
+// k-mer filter verdict: a clear bit proves that no prefix has this k-mer
+// (exact, no false negatives).
+__device__ __forceinline__ bool ov_filter_pass(const uint32_t* __restrict__ filt, uint32_t fmask, uint64_t m) {
+    if (!filt) return true;
+    const uint32_t f = (uint32_t)(m >> 32) & fmask;
+    return ((filt[f >> 5] >> (f & 31u)) & 1u) != 0;
+}
+
 // The suffix list is not compacted between offsets: a suffix linked at an
 // earlier offset keeps its slinked flag and is skipped, and the suffixes'
 // words come from the word-major copy sw (stride = list length).
@@ -433,97 +520,147 @@ __global__ void __launch_bounds__(256) k_ov_probe(ClaimState cs, const uint32_t*
                                                   uint32_t off) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
     const uint32_t ns = *cs.d_ns;
-    const uint32_t len = cs.L - off;
-    const uint32_t n1 = min(32u, len);
     GRID_STRIDE(i, ns) {
         if (cs.slinked[i]) {
             cs.choice[i] = INV;
             continue;
         }
         const uint64_t m = murmur64(xw_soa(sw, ns, i, kip + off, k));
-        if (filt) {
-            // no prefix has this k-mer: no candidate (exact, no false negatives)
-            const uint32_t f = (uint32_t)(m >> 32) & fmask;
-            if (((filt[f >> 5] >> (f & 31u)) & 1u) == 0) {
-                cs.choice[i] = INV;
-                continue;
-            }
+        if (!ov_filter_pass(filt, fmask, m)) {
+            cs.choice[i] = INV;
+            continue;
         }
-        const uint32_t s = cs.sl[i];
-        const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-        uint32_t h = (uint32_t)(m & mask);
-        uint64_t sx0 = xw_soa(sw, ns, i, off, n1);
-        uint32_t lo, e;
-        if (slots) {
-            const PrefixSlot sl = slots[h];
-            lo = sl.lo;
-            e = lo + sl.cnt;
-            if (sl.cnt <= PSLOT_N) {
-                // lower bound: first entry not below s[off:L]
-                uint32_t b = sl.cnt;
-                bool found = false;
-#pragma unroll
-                for (uint32_t t = 0; t < PSLOT_N; ++t) {
-                    if (!found && t < sl.cnt &&
-                        cmp_ps<Enc2>(cs.R, cs.W, sl.id[t], sl.key[t], rs, sx0, off, len) >= 0) {
-                        b = t;
-                        found = true;
-                    }
-                }
-                // first entry from there that is neither s nor taken decides;
-                // entries are sorted, so past the equal run nothing can match
-                // and the taken flags need not be read
-                uint32_t ch = INV;
-                bool done = false;
-#pragma unroll
-                for (uint32_t t = 0; t < PSLOT_N; ++t) {
-                    if (!done && t >= b && t < sl.cnt) {
-                        const uint32_t p = sl.id[t];
-                        if (cmp_ps<Enc2>(cs.R, cs.W, p, sl.key[t], rs, sx0, off, len) != 0) {
-                            done = true;
-                        } else if (p != s && cs.prev[p] == INV) {
-                            ch = lo + t;
-                            done = true;
-                        }
-                    }
-                }
-                cs.choice[i] = ch;
-                cs.grp[i] = lo + b;
-                if (ch != INV) {
-                    uint32_t old = atomicCAS(&cs.owner[ch], INV, (uint32_t)i);
-                    if (old != INV) cs.contended[lo + b] = 1;
-                }
-                continue;
-            }
-        } else {
-            lo = kr[h];
-            e = kr[h + 1];
-        }
-        uint32_t cnt = e - lo;
-        while (cnt > 0) {
-            uint32_t step = cnt >> 1, it = lo + step;
-            if (cmp_ps<Enc2>(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
-                lo = it + 1;
-                cnt -= step + 1;
-            } else {
-                cnt = step;
-            }
-        }
-        // the first entry that is neither s nor taken decides; past the run of
-        // entries equal to s[off:L] nothing can match (sorted bucket)
-        uint32_t ch = INV;
-        for (uint32_t x = lo; x < e; ++x) {
-            uint32_t p = cs.pl[x];
-            if (cmp_ps<Enc2>(cs.R, cs.W, p, cs.pk[x], rs, sx0, off, len) != 0) break;
-            if (p == s || cs.prev[p] != INV) continue;
-            ch = x;
-            break;
-        }
+        uint32_t g;
+        const uint32_t ch = ov_candidate(cs, kr, slots, sw, ns, (uint32_t)i, (uint32_t)(m & mask), off, &g);
         cs.choice[i] = ch;
-        cs.grp[i] = lo;
+        cs.grp[i] = g;
         if (ch != INV) {
             uint32_t old = atomicCAS(&cs.owner[ch], INV, (uint32_t)i);
-            if (old != INV) cs.contended[lo] = 1;
+            if (old != INV) cs.contended[g] = 1;
+        }
+    }
+}
+
+// Queued form of the a6 probe.  About one suffix in fifteen passes the
+// k-mer filter, so in the plain form nearly every wave carries a few lanes
+// through the candidate step's chain of dependent loads (slot, read words,
+// taken flag, owner).  Here workgroup b owns the contiguous chunk
+// [b*per, b*per + per) of the suffix list: it filters OVQ_TILE suffixes at
+// a time (OVQ_ITEMS per thread, their loads issued together), queues the
+// passing ones in an LDS ring, and runs the candidate step on full batches
+// of 256, one per thread.  Suffixes that propose are written to the chunk's
+// own segment of cand (cand[b*per ...], ncand[b] of them), which is all the
+// finalisation needs; choice and grp are written for proposers only.  The
+// proposals are the same as the plain form's: each depends on the suffix
+// alone, and the claim outcome does not depend on proposal order.
+constexpr uint32_t OVQ_ITEMS = 4;
+constexpr uint32_t OVQ_TILE = 256 * OVQ_ITEMS;
+constexpr uint32_t OVQ_CAP = 2048;  // ring: one tile plus a partial batch fit
+static_assert(OVQ_CAP >= OVQ_TILE + 512 && (OVQ_CAP & (OVQ_CAP - 1)) == 0, "overlap queue capacity");
+
+__global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_t* __restrict__ kr,
+                                                    const PrefixSlot* __restrict__ slots,
+                                                    const uint32_t* __restrict__ filt, uint32_t fmask, uint32_t mask,
+                                                    const uint64_t* __restrict__ sw, uint32_t k, uint32_t kip,
+                                                    uint32_t off, uint32_t per, uint32_t* __restrict__ cand,
+                                                    uint32_t* __restrict__ ncand) {
+    __shared__ uint32_t qi[OVQ_CAP];
+    __shared__ uint32_t qh[OVQ_CAP];
+    __shared__ uint32_t qtail, nout;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
+    const uint32_t ns = *cs.d_ns;
+    const uint64_t beg64 = (uint64_t)blockIdx.x * per;
+    const uint32_t beg = (uint32_t)min<uint64_t>(beg64, ns);
+    const uint32_t end = (uint32_t)min<uint64_t>(beg64 + per, ns);
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+    if (threadIdx.x == 0) qtail = nout = 0;
+    __syncthreads();
+
+    // candidate step for queue entry q (valid: the thread holds an entry)
+    auto candidate = [&](bool valid, uint32_t q) {
+        uint32_t ch = INV, i = 0;
+        if (valid) {
+            i = qi[q & (OVQ_CAP - 1)];
+            const uint32_t h = qh[q & (OVQ_CAP - 1)];
+            uint32_t g;
+            ch = ov_candidate(cs, kr, slots, sw, ns, i, h, off, &g);
+            if (ch != INV) {
+                cs.choice[i] = ch;
+                cs.grp[i] = g;
+                uint32_t old = atomicCAS(&cs.owner[ch], INV, i);
+                if (old != INV) cs.contended[g] = 1;
+            }
+        }
+        const uint64_t bal = __ballot(ch != INV);
+        if (bal) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&nout, (uint32_t)__popcll(bal));
+            base = __shfl(base, 0);
+            if (ch != INV) cand[beg + base + (uint32_t)__popcll(bal & lt)] = i;
+        }
+    };
+
+    uint32_t head = 0;
+    for (uint32_t base = beg; base < end; base += OVQ_TILE) {
+        uint32_t idx[OVQ_ITEMS];
+        bool live[OVQ_ITEMS];
+        uint64_t kw[OVQ_ITEMS];
+#pragma unroll
+        for (uint32_t j = 0; j < OVQ_ITEMS; ++j) {
+            idx[j] = base + j * 256 + threadIdx.x;
+            live[j] = idx[j] < end && cs.slinked[idx[j]] == 0;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < OVQ_ITEMS; ++j) kw[j] = live[j] ? xw_soa(sw, ns, idx[j], kip + off, k) : 0ull;
+#pragma unroll
+        for (uint32_t j = 0; j < OVQ_ITEMS; ++j) {
+            const uint64_t m = murmur64(kw[j]);
+            const bool pass = live[j] && ov_filter_pass(filt, fmask, m);
+            const uint64_t bal = __ballot(pass);
+            if (bal) {
+                uint32_t wb = 0;
+                if (lane == 0) wb = atomicAdd(&qtail, (uint32_t)__popcll(bal));
+                wb = __shfl(wb, 0);
+                if (pass) {
+                    const uint32_t q = (wb + (uint32_t)__popcll(bal & lt)) & (OVQ_CAP - 1);
+                    qi[q] = idx[j];
+                    qh[q] = (uint32_t)(m & mask);
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t t = qtail;
+        while (t - head >= 256) {
+            candidate(true, head + threadIdx.x);
+            head += 256;
+        }
+        __syncthreads();
+    }
+    const uint32_t t = qtail;
+    if (t > head) candidate(head + threadIdx.x < t, head + threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) ncand[blockIdx.x] = nout;
+}
+
+// Finalisation over the proposers k_ov_probe_q listed (same rules as
+// claim_finalize_body): workgroup b walks its own segment.
+__global__ void __launch_bounds__(256) k_ov_finalize_q(ClaimState cs, uint32_t off, uint32_t per,
+                                                       const uint32_t* __restrict__ cand,
+                                                       const uint32_t* __restrict__ ncand) {
+    const uint32_t nc = ncand[blockIdx.x];
+    const uint64_t beg = (uint64_t)blockIdx.x * per;
+    for (uint32_t x = threadIdx.x; x < nc; x += blockDim.x) {
+        const uint32_t i = cand[beg + x];
+        const uint32_t ch = cs.choice[i];
+        cs.owner[ch] = INV;
+        const uint32_t g = cs.grp[i];
+        if (cs.contended[g]) {
+            cs.memb[i] = atomicExch(&cs.head[g], i);
+            const uint32_t q = atomicAdd(cs.res_n, 1u);
+            cs.res_list[q] = i;
+        } else {
+            cs.link(i, ch, off);
         }
     }
 }

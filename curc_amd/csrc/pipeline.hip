@@ -51,6 +51,7 @@ enum Slot {
     S_RM_T, S_RM_TMAX, S_RM_J0, S_RM_J1, S_RM_MARK, S_RM_AS, S_RM_AL, S_RM_AD, S_RM_ASTART, S_RM_AEND,
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
+    S_OVCAND, S_OVNC,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -112,6 +113,7 @@ struct Tuning {
     uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
     uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
     uint32_t fm_fused = 4096;    // exact-matching lists up to this bound run in one fused launch (0: never)
+    uint32_t ov_queue = 1;       // overlap probe queued in LDS (k_ov_probe_q) or one suffix per thread
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
 };
 
@@ -605,6 +607,7 @@ void parse_tuning(const char* text, Tuning& t) {
             const std::string k = kv.substr(0, e);
             const uint32_t v = (uint32_t)std::strtoul(kv.c_str() + e + 1, nullptr, 10);
             if (k == "fm_fused") t.fm_fused = v;
+            if (k == "ov_queue" && v <= 1) t.ov_queue = v;
             if (v > 0) {
                 if (k == "ov_grid") t.ov_grid = v;
                 else if (k == "fm_grid") t.fm_grid = v;
@@ -1229,15 +1232,36 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             if (nsuf)
                 hipLaunchKernelGGL(k_gather_soa, dim3(grid_for((uint64_t)nsuf * W)), dim3(256), 0, st,
                                    (const uint64_t*)packed, (const uint32_t*)sl[0], (uint64_t)nsuf, W, sw);
+            // queued probe: workgroup b owns suffixes [b*per, b*per + per),
+            // per a whole number of filter tiles, and lists its proposers in
+            // that segment of qcand
+            const bool queued = c->tune.ov_queue != 0;
+            uint32_t per = 0, gq = 0;
+            uint32_t *qcand = nullptr, *qn = nullptr;
+            if (queued) {
+                const uint64_t want = ((uint64_t)nsuf + c->tune.ov_grid - 1) / c->tune.ov_grid;
+                per = (uint32_t)std::max<uint64_t>(OVQ_TILE, (want + OVQ_TILE - 1) / OVQ_TILE * OVQ_TILE);
+                gq = (uint32_t)std::max<uint64_t>(1, ((uint64_t)nsuf + per - 1) / per);
+                qcand = S.as<uint32_t>(S_OVCAND, (uint64_t)nsuf + 1);
+                qn = S.as<uint32_t>(S_OVNC, (uint64_t)gq);
+            }
             auto loop = [&]() {
                 for (uint32_t o = 1; o < P.max_off; ++o) {
                     ClaimState cs{packed, W, L, sl[0], cnt + C_NSUF, va, vk, cnt + C_NPRE, next, prev, off,
                                   choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                                   nullptr, slinked, nullptr};
-                    hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr,
-                                       (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
-                                       (const uint64_t*)sw, P.k, P.kip, o);
-                    hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
+                    if (queued) {
+                        hipLaunchKernelGGL(k_ov_probe_q, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
+                                           (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
+                                           (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn);
+                        hipLaunchKernelGGL(k_ov_finalize_q, dim3(gq), dim3(256), 0, st, cs, o, per,
+                                           (const uint32_t*)qcand, (const uint32_t*)qn);
+                    } else {
+                        hipLaunchKernelGGL(k_ov_probe, dim3(g), dim3(256), 0, st, cs, (const uint32_t*)kr,
+                                           (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
+                                           (const uint64_t*)sw, P.k, P.kip, o);
+                        hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
+                    }
                     hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
                 }
             };
@@ -1254,6 +1278,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             key.v[9] = (uint64_t)prev;
             key.v[10] = (uint64_t)off;
             key.v[11] = (uint64_t)cnt;
+            key.v[12] = ((uint64_t)queued << 32) | (queued ? per : g);
             c->graphs.run(key, S, st, loop);
         }
     }
@@ -2224,6 +2249,7 @@ int curc_ctx_get_tuning(curc_ctx* c, curc_tuning* t) {
     t->workers = c->tune.workers;
     t->filter_bits = c->tune.filter_bits;
     t->fm_fused = c->tune.fm_fused;
+    t->ov_queue = c->tune.ov_queue;
     return CURC_OK;
 }
 
@@ -2231,7 +2257,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!c || !t) return CURC_EINVAL;
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
         (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
-        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)))
+        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 1)
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;
     c->tune.fm_grid = t->fm_grid;
@@ -2240,6 +2266,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     c->tune.workers = t->workers;
     c->tune.filter_bits = t->filter_bits;
     c->tune.fm_fused = t->fm_fused;
+    c->tune.ov_queue = t->ov_queue;
     return CURC_OK;
 }
 

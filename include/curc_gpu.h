@@ -283,8 +283,9 @@ int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);
  * results: workgroup caps of the overlap probe, the exact-matching probe and
  * the mapping kernels, keys per thread of a radix-sort tile, the streams
  * curc_batch_run uses when its caller passes 0, the size of the overlap
- * stage's prefix k-mer filter, and the list bound below which an
- * exact-matching loop runs fused in one launch.  curc_amd/autotune.py
+ * stage's prefix k-mer filter, the list bound below which an
+ * exact-matching loop runs fused in one launch, and the form of the overlap
+ * probe.  curc_amd/autotune.py
  * times candidate values on a sample block and sets the fastest. */
 typedef struct curc_tuning {
     uint32_t ov_grid;     /* >= 1 (default 16384) */
@@ -295,6 +296,9 @@ typedef struct curc_tuning {
     uint32_t filter_bits; /* overlap k-mer filter bits per prefix: 8 .. 128, a power of two (default 16) */
     uint32_t fm_fused;    /* exact-matching lists up to this bound run their offset loop in one
                              single-workgroup launch; 0: never (default 4096) */
+    uint32_t ov_queue;    /* overlap probe: 1 queues the suffixes that pass the k-mer filter in
+                             LDS and searches their buckets in full batches; 0: one suffix per
+                             thread throughout (default 1) */
 } curc_tuning;
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
 int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);
