@@ -121,11 +121,27 @@ def test_csr_hash_indexes():
     cfg = ca.synth_config(seed=303, genome_len=60_000, gc=0.45, n_reads=9_000, L=150,
                           sub_first=0.002, sub_last=0.012, n_rate=0.0005)
     reads = ca.synth_reads(cfg)[1]
+
+    def work(ctx):
+        # mapping work counters: the slot index runs the register-resident
+        # kernel, the CSR index the generic one; both walk the same probes
+        # and verify the same candidates
+        ctx.reset_stats()
+        ctx.set_timing(True)
+        ctx.run_block(reads).free()
+        ctx.set_timing(False)
+        st = {s["name"]: s["items"] for s in ctx.stats()}
+        return {k: st[k] for k in ("map_active", "map_probes", "map_cands", "map_first_probe", "map_open_probes",
+                                   "map_open_cands")}
+
+    with ca.Context(0) as ctx:
+        fast = work(ctx)
     old = os.environ.get("CURC_INDEX")
     os.environ["CURC_INDEX"] = "csr"
     try:
         with ca.Context(0) as ctx:
             run_both(ctx, reads)
+            assert work(ctx) == fast
     finally:
         if old is None:
             del os.environ["CURC_INDEX"]
