@@ -916,31 +916,36 @@ __global__ void __launch_bounds__(256) k_claim_resolve(ClaimState cs, uint32_t o
     claim_resolve_body<FULLMATCH, E>(cs, off);
 }
 
-// Histogram of the prefix list's block values (its first bp bases); an
-// exclusive scan of it gives the block ranges.  The list is sorted, so equal
-// blocks are adjacent: each wave adds one count per run it holds.
+// Block ranges of the sorted prefix list in one launch: rng[b] is the first
+// list position whose block value (first bp bases) is >= b, for b in
+// [0, nb], with rng[nb] = np.  Position i writes the values in
+// (block(i-1), block(i)] (position np: up to nb), so every entry is written
+// exactly once and no histogram, scan or clearing is needed.  Gaps of 64
+// or more values are written by the whole wave.  Also clears the prefix
+// flags of the list (as the histogram kernel does).
 template <class E>
-__global__ void __launch_bounds__(256) k_fm_block_hist(const uint64_t* __restrict__ pk,
-                                                       const uint32_t* __restrict__ d_np, uint32_t bp,
-                                                       uint32_t* __restrict__ hist, uint8_t* __restrict__ plinked) {
+__global__ void __launch_bounds__(256) k_fm_ranges(const uint64_t* __restrict__ pk, const uint32_t* __restrict__ d_np,
+                                                   uint32_t bp, uint32_t nb, uint32_t* __restrict__ rng,
+                                                   uint8_t* __restrict__ plinked) {
     const uint32_t np = *d_np;
     const uint32_t lane = __lane_id();
-    const uint64_t lt = lanemask_lt();
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < np; base += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base <= np; base += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = base + threadIdx.x;
-        const bool valid = i < np;
-        if (valid) plinked[i] = 0;
-        const uint64_t key = valid ? E::head(pk[i], bp) : ~0ull;
-        uint64_t pkey = __shfl_up(key, 1, 64);
-        const bool start = valid && (lane == 0 || pkey != key);
-        const uint64_t starts = __ballot(start);
-        const uint64_t vmask = __ballot(valid);
-        if (start) {
-            // lanes of this run: from this lane up to the next start (or the last valid lane)
-            const uint64_t after = starts & ~(lt | (1ull << lane));
-            const uint64_t upto = after ? (after & (~after + 1)) - 1 : ~0ull;  // lanes below the next start
-            const uint32_t count = (uint32_t)__popcll(vmask & upto & ~lt);
-            atomicAdd(&hist[key], count);
+        uint32_t lo = 1, hi = 0;  // empty unless this position starts a gap
+        if (i <= np) {
+            if (i < np) plinked[i] = 0;
+            lo = (i == 0) ? 0u : (uint32_t)E::head(pk[i - 1], bp) + 1u;
+            hi = (i == np) ? nb : (uint32_t)E::head(pk[i], bp);
+        }
+        const bool have = i <= np && lo <= hi;
+        if (have && hi - lo < 64)
+            for (uint32_t b = lo; b <= hi; ++b) rng[b] = (uint32_t)i;
+        uint64_t wide = __ballot(have && hi - lo >= 64);
+        while (wide) {
+            const int l = __ffsll((long long)wide) - 1;
+            wide &= wide - 1;
+            const uint32_t wlo = __shfl(lo, l), whi = __shfl(hi, l), v = __shfl((uint32_t)i, l);
+            for (uint32_t b = wlo + lane; b <= whi; b += 64) rng[b] = v;
         }
     }
 }

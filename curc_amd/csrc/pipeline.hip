@@ -51,7 +51,7 @@ enum Slot {
     S_RM_T, S_RM_TMAX, S_RM_J0, S_RM_J1, S_RM_MARK, S_RM_AS, S_RM_AL, S_RM_AD, S_RM_ASTART, S_RM_AEND,
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
-    S_OVCAND, S_OVNC,
+    S_OVCAND, S_OVNC, S_LB,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -469,30 +469,43 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
         hipLaunchKernelGGL((k_fm_fused<E>), dim3(1), dim3(1024), 0, c->st, c0, c1, L);
         return;
     }
-    if (ranged) fill<uint32_t>(c, rng, (1ull << (BITS * std::min(MAX_BP, L - 1))) + 1, 0u);
+    // both lists compacted in one single-pass launch (up to COMPACT2_MAX_TILES
+    // tiles), else by two three-launch compactions
+    // (its status words are cleared by each offset's finalisation)
+    LookbackState* lb = nullptr;
+    uint32_t* lb_status = nullptr;
+    uint64_t lb_words = 0;
+    if (2 * compact2_tiles(n_max) <= COMPACT2_MAX_TILES) {
+        lb = S.as<LookbackState>(S_LB, 1);
+        fill<uint32_t>(c, (uint32_t*)lb, sizeof(LookbackState) / 4, 0u);
+        lb_status = (uint32_t*)lb->status;
+        lb_words = compact2_words(n_max, n_max);
+    }
     auto loop = [&]() {
         int cur = 0;
         for (uint32_t off = 1; off < L; ++off) {
             const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
-            const uint64_t nb = 1ull << (BITS * bp);
-            if (ranged) {
-                hipLaunchKernelGGL(k_fm_block_hist<E>, dim3(grid_for(n_max, 4096)), dim3(256), 0, c->st,
-                                   (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, rng, plinked);
-                scan_excl(rng, rng, nb + 1, S, S_PRIM, c->st);
-            }
+            const uint32_t nb = 1u << (BITS * bp);
+            if (ranged)
+                hipLaunchKernelGGL(k_fm_ranges<E>, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, c->st,
+                                   (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, nb, rng, plinked);
             ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
                           choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                           kpool, slinked, plinked};
             hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs,
                                ranged ? (const uint32_t*)rng : nullptr, bp, off);
-            // the ranges are rebuilt next round over at most nb + 1 values
-            hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, ranged ? rng : nullptr,
-                               ranged ? nb + 1 : 0ull);
+            hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, lb_status, lb_words);
             hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
-            compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
-                    c->st, n_max);
-            compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
-                    np[1 - cur], S, S_PRIM, c->st, n_max);
+            if (lb) {
+                compact2(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], n_max,
+                         np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
+                         np[1 - cur], n_max, lb, c->st);
+            } else {
+                compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
+                        S_PRIM, c->st, n_max);
+                compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
+                        np[1 - cur], S, S_PRIM, c->st, n_max);
+            }
             cur = 1 - cur;
         }
     };
@@ -508,6 +521,8 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     key.v[8] = (uint64_t)offset;
     key.v[9] = (uint64_t)cnt;
     c->graphs.run(key, S, c->st, loop);
+    if (lb && read_cnt(c, &lb->err) != 0)
+        throw std::runtime_error("exact matching: list compaction look-back did not complete");
 }
 
 // Record sort and stream builders (a13; compress.cu:1730-2044), on the

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -398,6 +399,144 @@ template <typename Pred, typename Emit>
 __global__ void __launch_bounds__(1024) k_compact_small(const uint32_t* __restrict__ d_n, Pred pred, Emit emit,
                                                         uint32_t* __restrict__ d_out_n) {
     compact_small_body(d_n, pred, emit, d_out_n);
+}
+
+// ------------------------------------------- two lists in one launch ----
+// Stable compaction of two lists in a single launch (decoupled look-back).
+// The grid holds tilesA tiles of list A then tilesB tiles of list B, each
+// tile COMPACT2_TILE items; workgroup b processes tile b.  Each tile
+// publishes its count (AGG) at once and its inclusive prefix (INC) after
+// looking back over its predecessors.  Workgroups are dispatched in
+// increasing id order on each XCD, so a wait cannot close a cycle: the
+// lowest undispatched id waits only for slots held by lower ids, whose own
+// waits end.  Status words hold state and value together, so one atomic
+// load sees a consistent pair; nothing else is published through them, so
+// the loads and stores are relaxed (no cache invalidation while a wave
+// waits).  The status words must be zero at launch: the caller clears them
+// between launches (on the same stream) and the launch leaves them set.
+// The look-back wait is bounded; a wait that runs out sets st->err and the
+// launch's output is then invalid (the caller checks).
+constexpr uint32_t COMPACT2_ITEMS = 16;
+constexpr uint32_t COMPACT2_TILE = 256 * COMPACT2_ITEMS;
+constexpr uint32_t COMPACT2_MAX_TILES = 4096;  // per launch (both lists)
+constexpr uint64_t LB_AGG = 1ull << 32, LB_INC = 2ull << 32;
+constexpr uint32_t LB_SPIN_LIMIT = 1u << 26;
+
+struct LookbackState {
+    uint32_t err, pad[3];
+    uint64_t status[COMPACT2_MAX_TILES];
+};
+
+template <typename Pred, typename Emit>
+__device__ __forceinline__ void compact2_tile(const uint32_t* __restrict__ d_n, Pred pred, Emit emit,
+                                              uint32_t* __restrict__ d_out_n, uint64_t* __restrict__ status,
+                                              uint32_t tile, uint32_t* __restrict__ err, uint32_t* lds) {
+    const uint32_t n = *d_n;
+    const uint64_t first = (uint64_t)tile * COMPACT2_TILE;
+    if (tile > 0 && first >= n) return;  // past the list: no later tile needs it
+    // round r covers items first + r*256 + [0, 256): reads and writes are
+    // coalesced, and an item's rank in the tile follows (round, wave, lane)
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    uint64_t bal[COMPACT2_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < COMPACT2_ITEMS; ++r) {
+        const uint64_t i = first + r * 256 + threadIdx.x;
+        bal[r] = __ballot(i < n && pred((uint32_t)i));
+        if (lane == 0) lds[8 + r * 4 + w] = (uint32_t)__popcll(bal[r]);
+    }
+    __syncthreads();
+    uint32_t agg = 0;
+    if (threadIdx.x < 64) {
+        // exclusive offsets of the 64 (round, wave) groups, and the tile total
+        const uint32_t v = lds[8 + threadIdx.x];
+        const uint32_t inc = wave_incl_scan(v);
+        lds[8 + 64 + threadIdx.x] = inc - v;
+        agg = __shfl(inc, 63);
+    }
+    if (threadIdx.x < 64) {
+        // wave 0 looks back 64 predecessors at a time; lane l reads tile
+        // top - l, and the nearest inclusive prefix ends the walk
+        uint32_t excl = 0;
+        if (tile == 0) {
+            if (lane == 0) __hip_atomic_store(&status[0], LB_INC | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0) __hip_atomic_store(&status[tile], LB_AGG | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t top = (int64_t)tile - 1;
+            for (;;) {
+                const int64_t j = top - (int64_t)lane;
+                uint64_t v = LB_INC;  // before tile 0: an inclusive prefix of 0
+                if (j >= 0) {
+                    uint32_t spins = 0;
+                    while (((v = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) ==
+                           0) {
+                        if (++spins == LB_SPIN_LIMIT) {
+                            atomicExch(err, 1u);
+                            v = LB_INC;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                const uint64_t inc = __ballot((v >> 32) == 2);
+                const uint32_t stop = inc ? (uint32_t)(__ffsll((long long)inc) - 1) : 63u;
+                const uint32_t part = wave_incl_scan(lane <= stop ? (uint32_t)v : 0u);
+                excl += __shfl(part, 63);
+                if (inc) break;
+                top -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], LB_INC | (uint64_t)(excl + agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            lds[4] = excl;
+            // the tile holding the list's last item (or tile 0 of an empty list) reports the total
+            if (first + COMPACT2_TILE >= n) *d_out_n = excl + agg;
+        }
+    }
+    __syncthreads();
+    const uint32_t base = lds[4];
+#pragma unroll
+    for (uint32_t r = 0; r < COMPACT2_ITEMS; ++r) {
+        if ((bal[r] >> lane) & 1ull)
+            emit((uint32_t)(first + r * 256 + threadIdx.x),
+                 base + lds[8 + 64 + r * 4 + w] + (uint32_t)__popcll(bal[r] & lt));
+    }
+}
+
+template <typename PredA, typename EmitA, typename PredB, typename EmitB>
+__global__ void __launch_bounds__(256) k_compact2(const uint32_t* __restrict__ na, PredA pa, EmitA ea,
+                                                  uint32_t* __restrict__ outa, uint32_t tiles_a,
+                                                  const uint32_t* __restrict__ nb, PredB pb, EmitB eb,
+                                                  uint32_t* __restrict__ outb, LookbackState* __restrict__ st) {
+    __shared__ uint32_t lds[8 + 128];
+    const uint32_t t = blockIdx.x;
+    if (t < tiles_a)
+        compact2_tile(na, pa, ea, outa, st->status, t, &st->err, lds);
+    else
+        compact2_tile(nb, pb, eb, outb, st->status + tiles_a, t - tiles_a, &st->err, lds);
+}
+
+// Compacts list A and list B (host-known bounds bound_a, bound_b) in one
+// launch.  st->status[0, compact2_words(bound_a, bound_b) / 2) must be zero
+// at launch; the launch leaves it set.
+inline uint64_t compact2_tiles(uint64_t bound) {
+    return std::max<uint64_t>(1, (bound + COMPACT2_TILE - 1) / COMPACT2_TILE);
+}
+// 32-bit words of status the launch sets
+inline uint64_t compact2_words(uint64_t bound_a, uint64_t bound_b) {
+    return 2 * (compact2_tiles(bound_a) + compact2_tiles(bound_b));
+}
+
+template <typename PredA, typename EmitA, typename PredB, typename EmitB>
+void compact2(const uint32_t* na, PredA pa, EmitA ea, uint32_t* outa, uint64_t bound_a, const uint32_t* nb,
+              PredB pb, EmitB eb, uint32_t* outb, uint64_t bound_b, LookbackState* st, hipStream_t s) {
+    const uint32_t ta = (uint32_t)compact2_tiles(bound_a);
+    const uint32_t tb = (uint32_t)compact2_tiles(bound_b);
+    if ((uint64_t)ta + tb > COMPACT2_MAX_TILES) throw std::length_error("compact2: lists too long");
+    hipLaunchKernelGGL((k_compact2<PredA, EmitA, PredB, EmitB>), dim3(ta + tb), dim3(256), 0, s, na, pa, ea, outa,
+                       ta, nb, pb, eb, outb, st);
 }
 
 // Stable compaction: emit(i, j) is called for the j-th i in [0, *d_n) with
