@@ -847,12 +847,130 @@ __device__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint
 // merge join with self-avoid rotation, compress.cu:1089-1160, for a8).  The
 // free part of the run is always {P[a]} followed by P[b..], so one pass
 // suffices.
-template <bool FULLMATCH, class E>
-__device__ __forceinline__ void claim_resolve_body(const ClaimState& cs, uint32_t off) {
-    const uint32_t nq = *cs.res_n;
+// The leader's walk over a sorted group arr[0..m) whose prefixes start at
+// run position g.  All members share s[off:L], and the prefixes equal to it
+// form one run [g, e) of the sorted list (one bucket's sorted run in a6),
+// which starts at g.  Its end is found once, by galloping, instead of one
+// comparison per linked member.
+template <class E>
+__device__ __forceinline__ void claim_assign(const ClaimState& cs, const uint32_t* arr, uint32_t m, uint32_t g,
+                                             uint32_t off) {
     const uint32_t np = *cs.d_np;
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(E::BPW, len);
+    const uint32_t s0 = cs.sl[arr[0]];
+    const uint64_t* rs0 = cs.R + (uint64_t)s0 * cs.W;
+    const uint64_t sx00 = E::word(rs0, off, n1);
+    auto equal = [&](uint32_t x) { return cmp_ps<E>(cs.R, cs.W, cs.pl[x], cs.pk[x], rs0, sx00, off, len) == 0; };
+    uint32_t known = g, e = g + 1;  // [g, known] equal; e: first candidate past it
+    uint64_t step = 1;
+    while (e < np && equal(e)) {
+        known = e;
+        step <<= 1;
+        e = ((uint64_t)(np - g) > step) ? g + (uint32_t)step : np;
+    }
+    while (e - known > 1) {
+        const uint32_t mid = known + (e - known) / 2;
+        if (equal(mid))
+            known = mid;
+        else
+            e = mid;
+    }
+    auto next_free = [&](uint32_t x) {
+        while (x < e && cs.prev[cs.pl[x]] != INV) ++x;
+        return x;
+    };
+    uint32_t a = next_free(g);
+    uint32_t b = next_free(a + 1);
+    for (uint32_t k = 0; k < m && a < e; ++k) {
+        const uint32_t s = cs.sl[arr[k]];
+        if (cs.pl[a] != s) {
+            cs.link(arr[k], a, off);
+            a = b;
+            b = next_free(b + 1);
+        } else if (b < e) {
+            cs.link(arr[k], b, off);
+            b = next_free(b + 1);
+        }
+    }
+    cs.head[g] = INV;
+    cs.contended[g] = 0;
+}
+
+// a8 groups of RESOLVE_COOP_MIN < m <= RESOLVE_COOP_MAX members are sorted
+// by the whole workgroup (bitonic sort in LDS on the same order as
+// sort_members_keyed: first comparison word, then the reads) after its
+// grid-stride pass, instead of by the leader alone; up to RESOLVE_COOP_LIST
+// per workgroup and call, the rest as before.
+constexpr uint32_t RESOLVE_COOP_MIN = 64;
+constexpr uint32_t RESOLVE_COOP_MAX = 4096;
+constexpr uint32_t RESOLVE_COOP_LIST = 32;
+struct ResolveCoop {
+    uint64_t key[RESOLVE_COOP_MAX];
+    uint32_t mem[RESOLVE_COOP_MAX];
+    uint32_t lead[RESOLVE_COOP_LIST], base[RESOLVE_COOP_LIST], m[RESOLVE_COOP_LIST];
+    uint32_t n;
+};
+
+template <class E>
+__device__ void resolve_coop(const ClaimState& cs, ResolveCoop& co, uint32_t off) {
+    __syncthreads();
+    const uint32_t nb = min(co.n, RESOLVE_COOP_LIST);
+    for (uint32_t gi = 0; gi < nb; ++gi) {
+        const uint32_t m = co.m[gi];
+        const uint32_t* arr = cs.pool + co.base[gi];
+        uint32_t P = 1;
+        while (P < m) P <<= 1;
+        for (uint32_t x = threadIdx.x; x < P; x += blockDim.x) {
+            if (x < m) {
+                const uint32_t u = arr[x];
+                co.mem[x] = u;
+                co.key[x] = rev_prefix_key<E>(cs.R, cs.W, cs.sl[u], off);
+            } else {
+                co.mem[x] = INV;  // padding sorts last
+                co.key[x] = ~0ull;
+            }
+        }
+        __syncthreads();
+        auto before = [&](uint32_t x, uint32_t y) -> bool {
+            const uint32_t ux = co.mem[x], uy = co.mem[y];
+            if (ux == INV) return false;
+            if (uy == INV) return true;
+            const uint64_t kx = co.key[x], ky = co.key[y];
+            if (kx != ky) return kx < ky;
+            return cmp_rev_prefix<E>(cs.R, cs.W, cs.sl[ux], cs.sl[uy], off) < 0;
+        };
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t x = threadIdx.x; x < P; x += blockDim.x) {
+                    const uint32_t y = x ^ j;
+                    if (y > x && (((x & k) == 0) ? before(y, x) : before(x, y))) {
+                        const uint32_t tm = co.mem[x];
+                        co.mem[x] = co.mem[y];
+                        co.mem[y] = tm;
+                        const uint64_t tk = co.key[x];
+                        co.key[x] = co.key[y];
+                        co.key[y] = tk;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) claim_assign<E>(cs, co.mem, m, cs.grp[co.lead[gi]], off);
+        __syncthreads();
+    }
+}
+
+template <bool FULLMATCH, class E>
+__device__ __forceinline__ void claim_resolve_body(const ClaimState& cs, uint32_t off) {
+    const uint32_t nq = *cs.res_n;
+    ResolveCoop* co = nullptr;
+    if constexpr (FULLMATCH) {
+        __shared__ ResolveCoop coop;
+        co = &coop;
+        if (threadIdx.x == 0) co->n = 0;
+        __syncthreads();
+    }
     GRID_STRIDE(q, nq) {
         uint32_t lead = cs.res_list[q];
         uint32_t g = cs.grp[lead];
@@ -863,51 +981,23 @@ __device__ __forceinline__ void claim_resolve_body(const ClaimState& cs, uint32_
         uint32_t* arr = cs.pool + base;
         m = 0;
         for (uint32_t u = lead; u != INV; u = cs.memb[u]) arr[m++] = u;
+        if (FULLMATCH && cs.kpool && m > RESOLVE_COOP_MIN && m <= RESOLVE_COOP_MAX) {
+            const uint32_t slot = atomicAdd(&co->n, 1u);
+            if (slot < RESOLVE_COOP_LIST) {
+                co->lead[slot] = lead;
+                co->base[slot] = base;
+                co->m[slot] = m;
+                continue;
+            }
+        }
         if (FULLMATCH && cs.kpool && m > 16)
             sort_members_keyed<E>(cs, arr, cs.kpool + base, m, off);
         else
             sort_members<FULLMATCH, E>(cs, arr, m, off);
-        // All members share s[off:L], and the prefixes equal to it form one
-        // run [g, e) of the sorted list (one bucket's sorted run in a6), which
-        // starts at g.  Its end is found once, by galloping, instead of one
-        // comparison per linked member.
-        const uint32_t s0 = cs.sl[arr[0]];
-        const uint64_t* rs0 = cs.R + (uint64_t)s0 * cs.W;
-        const uint64_t sx00 = E::word(rs0, off, n1);
-        auto equal = [&](uint32_t x) { return cmp_ps<E>(cs.R, cs.W, cs.pl[x], cs.pk[x], rs0, sx00, off, len) == 0; };
-        uint32_t known = g, e = g + 1;  // [g, known] equal; e: first candidate past it
-        uint64_t step = 1;
-        while (e < np && equal(e)) {
-            known = e;
-            step <<= 1;
-            e = ((uint64_t)(np - g) > step) ? g + (uint32_t)step : np;
-        }
-        while (e - known > 1) {
-            const uint32_t mid = known + (e - known) / 2;
-            if (equal(mid))
-                known = mid;
-            else
-                e = mid;
-        }
-        auto next_free = [&](uint32_t x) {
-            while (x < e && cs.prev[cs.pl[x]] != INV) ++x;
-            return x;
-        };
-        uint32_t a = next_free(g);
-        uint32_t b = next_free(a + 1);
-        for (uint32_t k = 0; k < m && a < e; ++k) {
-            const uint32_t s = cs.sl[arr[k]];
-            if (cs.pl[a] != s) {
-                cs.link(arr[k], a, off);
-                a = b;
-                b = next_free(b + 1);
-            } else if (b < e) {
-                cs.link(arr[k], b, off);
-                b = next_free(b + 1);
-            }
-        }
-        cs.head[g] = INV;
-        cs.contended[g] = 0;
+        claim_assign<E>(cs, arr, m, g, off);
+    }
+    if constexpr (FULLMATCH) {
+        if (cs.kpool) resolve_coop<E>(cs, *co, off);
     }
 }
 
