@@ -222,15 +222,47 @@ struct EmitGatherKey {
 // Bucket of each prefix's k-mer; with filt, also the k-mer filter bit
 // (hash bits 32..63 under fmask): a suffix whose k-mer bit is clear has no
 // prefix with that k-mer, hence none equal to it, at any offset.
+// Minimizer-blocked prefix k-mer filter.  A k-mer's two filter bits
+// (ov_fbits) lie in the 1024-bit block chosen by its
+// minimizer, the least ov_mhash over its OVM-base substrings.  Neighbouring
+// k-mers of a read usually share their minimizer, so the k-mers of one
+// suffix at all offsets touch a few 128-byte blocks.  Exact like the plain
+// filter: a clear bit proves that no prefix has the k-mer.
+constexpr uint32_t OVM = 16;
+constexpr uint32_t OVB_WORDS = 32;  // 1024-bit blocks
+// order of the m-mers for minimizers: a bijection of 32-bit values
+__device__ __forceinline__ uint32_t ov_mhash(uint32_t v) { return v * 0x9E3779B1u; }
+// the k-mer's two bit positions in its block: bits 0..9 and 10..19
+__device__ __forceinline__ uint32_t ov_fbits(uint64_t K) { return (uint32_t)((K * 0xC2B2AE3D27D4EB4Full) >> 44); }
+__device__ __forceinline__ uint32_t ov_block(uint32_t mn, uint32_t bmask) {
+    return (uint32_t)(((uint64_t)(mn ^ 0x5bd1e995u) * 0xD6E8FEB86659FD93ull) >> 40) & bmask;
+}
+// minimizer of the k-base k-mer K (2k bits, first base highest)
+__device__ __forceinline__ uint32_t ov_minimizer(uint64_t K, uint32_t k) {
+    uint32_t mn = 0xFFFFFFFFu;
+    for (uint32_t t = 0; t + OVM <= k; ++t) mn = min(mn, ov_mhash((uint32_t)(K >> (2 * (k - OVM - t)))));
+    return mn;
+}
+
+// Bucket hash of each prefix's k-mer at kip, and its bits in the plain
+// filter (filt) or the minimizer-blocked one (mfilt, bmask + 1 blocks).
 __global__ void k_prefix_hash(const uint64_t* __restrict__ R, const uint32_t* __restrict__ ids, uint64_t n,
                               uint32_t W, uint32_t kip, uint32_t k, uint32_t mask, uint32_t* __restrict__ hk,
-                              uint32_t* __restrict__ filt, uint32_t fmask) {
+                              uint32_t* __restrict__ filt, uint32_t fmask, uint32_t* __restrict__ mfilt,
+                              uint32_t bmask) {
     GRID_STRIDE(i, n) {
-        const uint64_t m = murmur64(xw(R + (uint64_t)ids[i] * W, kip, k));
+        const uint64_t K = xw(R + (uint64_t)ids[i] * W, kip, k);
+        const uint64_t m = murmur64(K);
         hk[i] = (uint32_t)(m & mask);
         if (filt) {
             const uint32_t f = (uint32_t)(m >> 32) & fmask;
             atomicOr(&filt[f >> 5], 1u << (f & 31u));
+        }
+        if (mfilt) {
+            uint32_t* fb = mfilt + (uint64_t)ov_block(ov_minimizer(K, k), bmask) * OVB_WORDS;
+            const uint32_t f = ov_fbits(K), b1 = f & 1023u, b2 = f >> 10;
+            atomicOr(&fb[b1 >> 5], 1u << (b1 & 31u));
+            atomicOr(&fb[b2 >> 5], 1u << (b2 & 31u));
         }
     }
 }
@@ -558,12 +590,17 @@ constexpr uint32_t OVQ_TILE = 256 * OVQ_ITEMS;
 constexpr uint32_t OVQ_CAP = 2048;  // ring: one tile plus a partial batch fit
 static_assert(OVQ_CAP >= OVQ_TILE + 512 && (OVQ_CAP & (OVQ_CAP - 1)) == 0, "overlap queue capacity");
 
+// VERD: the filter verdicts were computed for all offsets beforehand
+// (k_ov_verdicts; bit i of vb[(off - 1) * nw + i / 64]), so the filter pass
+// reads one bit per suffix and the candidate step hashes the k-mer itself.
+template <bool VERD>
 __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_t* __restrict__ kr,
                                                     const PrefixSlot* __restrict__ slots,
                                                     const uint32_t* __restrict__ filt, uint32_t fmask, uint32_t mask,
                                                     const uint64_t* __restrict__ sw, uint32_t k, uint32_t kip,
                                                     uint32_t off, uint32_t per, uint32_t* __restrict__ cand,
-                                                    uint32_t* __restrict__ ncand) {
+                                                    uint32_t* __restrict__ ncand, const uint64_t* __restrict__ vb,
+                                                    uint64_t nw) {
     __shared__ uint32_t qi[OVQ_CAP];
     __shared__ uint32_t qh[OVQ_CAP];
     __shared__ uint32_t qtail, nout;
@@ -582,7 +619,8 @@ __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_
         uint32_t ch = INV, i = 0;
         if (valid) {
             i = qi[q & (OVQ_CAP - 1)];
-            const uint32_t h = qh[q & (OVQ_CAP - 1)];
+            const uint32_t h = VERD ? (uint32_t)(murmur64(xw_soa(sw, ns, i, kip + off, k)) & mask)
+                                    : qh[q & (OVQ_CAP - 1)];
             uint32_t g;
             ch = ov_candidate(cs, kr, slots, sw, ns, i, h, off, &g);
             if (ch != INV) {
@@ -609,14 +647,20 @@ __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_
 #pragma unroll
         for (uint32_t j = 0; j < OVQ_ITEMS; ++j) {
             idx[j] = base + j * 256 + threadIdx.x;
-            live[j] = idx[j] < end && cs.slinked[idx[j]] == 0;
+            if (VERD)
+                live[j] = idx[j] < end && ((vb[(uint64_t)(off - 1) * nw + (idx[j] >> 6)] >> (idx[j] & 63u)) & 1ull) &&
+                          cs.slinked[idx[j]] == 0;
+            else
+                live[j] = idx[j] < end && cs.slinked[idx[j]] == 0;
+        }
+        if (!VERD) {
+#pragma unroll
+            for (uint32_t j = 0; j < OVQ_ITEMS; ++j) kw[j] = live[j] ? xw_soa(sw, ns, idx[j], kip + off, k) : 0ull;
         }
 #pragma unroll
-        for (uint32_t j = 0; j < OVQ_ITEMS; ++j) kw[j] = live[j] ? xw_soa(sw, ns, idx[j], kip + off, k) : 0ull;
-#pragma unroll
         for (uint32_t j = 0; j < OVQ_ITEMS; ++j) {
-            const uint64_t m = murmur64(kw[j]);
-            const bool pass = live[j] && ov_filter_pass(filt, fmask, m);
+            const uint64_t m = VERD ? 0ull : murmur64(kw[j]);
+            const bool pass = live[j] && (VERD || ov_filter_pass(filt, fmask, m));
             const uint64_t bal = __ballot(pass);
             if (bal) {
                 uint32_t wb = 0;
@@ -625,7 +669,7 @@ __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_
                 if (pass) {
                     const uint32_t q = (wb + (uint32_t)__popcll(bal & lt)) & (OVQ_CAP - 1);
                     qi[q] = idx[j];
-                    qh[q] = (uint32_t)(m & mask);
+                    if (!VERD) qh[q] = (uint32_t)(m & mask);
                 }
             }
         }
@@ -1464,6 +1508,96 @@ __device__ __forceinline__ uint64_t xw_pick(const uint64_t (&w)[WM], uint32_t of
     if ((64u - o2) >= (n << 1)) return left >> ((32u - n) << 1);
     const uint64_t right = (pick(w, o1 + 1) >> (62u - o2)) >> 2;
     return (left | right) >> ((32u - n) << 1);
+}
+
+// Overlap filter verdicts for every offset in one pass (minimizer-blocked
+// filter): bit i of vb[(off - 1) * nw + i / 64] is set when the k-mer of
+// suffix i at kip + off has both of its bits set, for off in [1, max_off).
+// Thread i holds its read's words (from the word-major copy sw), and every
+// lane does the same work per offset, so lanes do not diverge.  For k = 32
+// (K32) offsets go in chunks of NM = 17, the m-mers per k-mer: with the 64
+// bases from the chunk's first k-mer in two words, each k-mer and m-mer of
+// the chunk is a fixed shift of them, and the least m-mer hash of each
+// window is min(suffix minimum over the chunk's m-mers from its start,
+// prefix minimum over the next chunk's m-mers up to its end).  For other k
+// the minimizer is recomputed from each k-mer.  The filter block of the
+// current minimizer is kept in the thread's own LDS slot and reloaded only
+// when the minimizer changes: between two offsets of one thread the other
+// resident threads touch too many blocks for the caches to keep it.  Each
+// 64-suffix word is written by its wave with one ballot per offset.
+template <uint32_t WM, bool K32>
+__global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict__ sw, uint32_t ns, uint32_t W,
+                                                     uint32_t k, uint32_t kip, uint32_t max_off,
+                                                     const uint32_t* __restrict__ mfilt, uint32_t bmask,
+                                                     uint64_t* __restrict__ vb, uint64_t nw) {
+    __shared__ uint32_t blocks[256 * (OVB_WORDS + 1)];  // stride 33: spreads banks
+    uint32_t* mine = blocks + threadIdx.x * (OVB_WORDS + 1);
+    uint32_t held = 0xFFFFFFFFu;  // block in the LDS slot
+    auto verdict = [&](uint64_t K, uint32_t mv, bool valid, uint64_t i, uint32_t off) {
+        const uint32_t blk = ov_block(mv, bmask);
+        if (blk != held) {
+            const uint4* src = reinterpret_cast<const uint4*>(mfilt + (uint64_t)blk * OVB_WORDS);
+#pragma unroll
+            for (uint32_t q = 0; q < OVB_WORDS / 4; ++q) {
+                const uint4 v = src[q];
+                mine[4 * q] = v.x;
+                mine[4 * q + 1] = v.y;
+                mine[4 * q + 2] = v.z;
+                mine[4 * q + 3] = v.w;
+            }
+            held = blk;
+        }
+        const uint32_t f = ov_fbits(K), b1 = f & 1023u, b2 = f >> 10;
+        const bool pass = valid && ((mine[b1 >> 5] >> (b1 & 31u)) & (mine[b2 >> 5] >> (b2 & 31u)) & 1u) != 0;
+        const uint64_t bal = __ballot(pass);
+        // a wave past the list's end (lane 0 beyond it) has no word to write
+        if (__lane_id() == 0 && valid) vb[(uint64_t)(off - 1) * nw + (i >> 6)] = bal;
+    };
+    for (uint64_t base = (uint64_t)blockIdx.x * 256; base < ns; base += (uint64_t)gridDim.x * 256) {
+        const uint64_t i = base + threadIdx.x;
+        const bool valid = i < ns;
+        uint64_t r[WM];
+#pragma unroll
+        for (uint32_t w = 0; w < WM; ++w) r[w] = (valid && w < W) ? sw[(uint64_t)w * ns + i] : 0ull;
+        if constexpr (K32) {
+            constexpr uint32_t NM = 32 - OVM + 1;
+            uint32_t hc[NM];  // m-mer hashes of the current chunk (positions x0 .. x0 + NM - 1)
+            for (uint32_t o0 = 1; o0 < max_off; o0 += NM) {
+                const uint32_t x0 = kip + o0;
+                const uint64_t A = xw_pick(r, x0, 32), B = xw_pick(r, x0 + 32, 32);
+                // the 32 bases from base t of A:B (t <= 33)
+                auto win = [&](uint32_t t) -> uint64_t {
+                    if (t == 0) return A;
+                    if (t < 32) return (A << (2 * t)) | (B >> (64 - 2 * t));
+                    return B << (2 * (t - 32));
+                };
+                if (o0 == 1) {
+#pragma unroll
+                    for (uint32_t t = 0; t < NM; ++t) hc[t] = ov_mhash((uint32_t)(win(t) >> 32));
+                }
+                uint32_t hn[NM];  // the next chunk's
+#pragma unroll
+                for (uint32_t t = 0; t < NM; ++t) hn[t] = ov_mhash((uint32_t)(win(NM + t) >> 32));
+#pragma unroll
+                for (uint32_t t = NM - 1; t-- > 0;) hc[t] = min(hc[t], hc[t + 1]);  // suffix minima
+                uint32_t pm = 0xFFFFFFFFu;  // prefix minimum of hn[0 .. j)
+#pragma unroll
+                for (uint32_t j = 0; j < NM; ++j) {
+                    const uint32_t off = o0 + j;
+                    if (off >= max_off) break;
+                    verdict(win(j), min(hc[j], pm), valid, i, off);
+                    pm = min(pm, hn[j]);
+                }
+#pragma unroll
+                for (uint32_t t = 0; t < NM; ++t) hc[t] = hn[t];
+            }
+        } else {
+            for (uint32_t off = 1; off < max_off; ++off) {
+                const uint64_t K = xw_pick(r, kip + off, k);
+                verdict(K, ov_minimizer(K, k), valid, i, off);
+            }
+        }
+    }
 }
 
 template <bool RC, uint32_t WM>

@@ -51,7 +51,7 @@ enum Slot {
     S_RM_T, S_RM_TMAX, S_RM_J0, S_RM_J1, S_RM_MARK, S_RM_AS, S_RM_AL, S_RM_AD, S_RM_ASTART, S_RM_AEND,
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
-    S_OVCAND, S_OVNC, S_LB,
+    S_OVCAND, S_OVNC, S_LB, S_OVVB,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -113,7 +113,7 @@ struct Tuning {
     uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
     uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
     uint32_t fm_fused = 4096;    // exact-matching lists up to this bound run in one fused launch (0: never)
-    uint32_t ov_queue = 1;       // overlap probe queued in LDS (k_ov_probe_q) or one suffix per thread
+    uint32_t ov_queue = 2;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
 };
 
@@ -622,7 +622,7 @@ void parse_tuning(const char* text, Tuning& t) {
             const std::string k = kv.substr(0, e);
             const uint32_t v = (uint32_t)std::strtoul(kv.c_str() + e + 1, nullptr, 10);
             if (k == "fm_fused") t.fm_fused = v;
-            if (k == "ov_queue" && v <= 1) t.ov_queue = v;
+            if (k == "ov_queue" && v <= 2) t.ov_queue = v;
             if (v > 0) {
                 if (k == "ov_grid") t.ov_grid = v;
                 else if (k == "fm_grid") t.fm_grid = v;
@@ -1197,12 +1197,17 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                                  ? S.as<PrefixSlot>(S_PSLOT, (uint64_t)H)
                                  : nullptr;
         // prefix k-mer filter: >= filter_bits bits per prefix (2^20..2^32
-        // bits), so a probe without an equal k-mer usually stops at one bit
+        // bits), so a probe without an equal k-mer usually stops at one bit.
+        // With verdicts (overlap probe form 2: reads of at most 256 bases,
+        // k >= OVM + 4) the filter is minimizer-blocked and consulted once
+        // per suffix for all offsets; otherwise it is the plain bitmap.
         uint64_t fbits = 1ull << 20;
         while (fbits < (uint64_t)c->tune.filter_bits * npre && fbits < (1ull << 32)) fbits <<= 1;
         const uint32_t fmask = (uint32_t)(fbits - 1);
         const uint64_t fwords = fbits / 32;
         uint32_t* pfilt = S.as<uint32_t>(S_PFILT, fwords);
+        const bool verd = c->tune.ov_queue == 2 && W <= 8 && P.k >= OVM + 4;
+        const uint32_t bmask = (uint32_t)(fbits / (32 * OVB_WORDS) - 1);
         {
             Region r(c, "prefix_index");
             uint32_t* hk0 = S.as<uint32_t>(S_MK0, (uint64_t)npre + 1);
@@ -1211,7 +1216,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             uint32_t* va_ = va;
             fill<uint32_t>(c, pfilt, fwords, 0u);
             hipLaunchKernelGGL(k_prefix_hash, dim3(grid_for(npre)), dim3(256), 0, st, (const uint64_t*)packed,
-                               (const uint32_t*)va, (uint64_t)npre, W, P.kip, P.k, mask, hk0, pfilt, fmask);
+                               (const uint32_t*)va, (uint64_t)npre, W, P.kip, P.k, mask, hk0,
+                               verd ? nullptr : pfilt, fmask, verd ? pfilt : nullptr, bmask);
             radix_sort_pairs<uint32_t>(hk0, va_, hk1, vb, npre, 0, log2u(H), S, S_PRIM, st);
             if (va_ != va) CURC_HIP(hipMemcpyAsync(va, va_, (uint64_t)npre * 4, hipMemcpyDeviceToDevice, st));
             hipLaunchKernelGGL(k_gap_fill, dim3(grid_for((uint64_t)npre + 1)), dim3(256), 0, st,
@@ -1247,6 +1253,31 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             if (nsuf)
                 hipLaunchKernelGGL(k_gather_soa, dim3(grid_for((uint64_t)nsuf * W)), dim3(256), 0, st,
                                    (const uint64_t*)packed, (const uint32_t*)sl[0], (uint64_t)nsuf, W, sw);
+            // filter verdicts of every offset, one 64-suffix word per offset
+            const uint64_t vnw = ((uint64_t)nsuf + 63) / 64;
+            uint64_t* vb = nullptr;
+            if (verd && nsuf && P.max_off > 1) {
+                vb = S.as<uint64_t>(S_OVVB, (uint64_t)(P.max_off - 1) * vnw);
+                const dim3 vg(grid_for(nsuf, 65536));
+#define CURC_OV_VERDICTS(WM)                                                                           \
+    do {                                                                                               \
+        if (P.k == 32)                                                                                 \
+            hipLaunchKernelGGL((k_ov_verdicts<WM, true>), vg, dim3(256), 0, st, (const uint64_t*)sw, nsuf, W, \
+                               P.k, P.kip, P.max_off, (const uint32_t*)pfilt, bmask, vb, vnw);         \
+        else                                                                                           \
+            hipLaunchKernelGGL((k_ov_verdicts<WM, false>), vg, dim3(256), 0, st, (const uint64_t*)sw, nsuf, W, \
+                               P.k, P.kip, P.max_off, (const uint32_t*)pfilt, bmask, vb, vnw);         \
+    } while (0)
+                if (W <= 3)
+                    CURC_OV_VERDICTS(3);
+                else if (W <= 4)
+                    CURC_OV_VERDICTS(4);
+                else if (W <= 5)
+                    CURC_OV_VERDICTS(5);
+                else
+                    CURC_OV_VERDICTS(8);
+#undef CURC_OV_VERDICTS
+            }
             // queued probe: workgroup b owns suffixes [b*per, b*per + per),
             // per a whole number of filter tiles, and lists its proposers in
             // that segment of qcand
@@ -1265,10 +1296,18 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     ClaimState cs{packed, W, L, sl[0], cnt + C_NSUF, va, vk, cnt + C_NPRE, next, prev, off,
                                   choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                                   nullptr, slinked, nullptr};
-                    if (queued) {
-                        hipLaunchKernelGGL(k_ov_probe_q, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
+                    if (queued && vb) {
+                        hipLaunchKernelGGL(k_ov_probe_q<true>, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
+                                           (const PrefixSlot*)pslots, (const uint32_t*)nullptr, fmask, mask,
+                                           (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn,
+                                           (const uint64_t*)vb, vnw);
+                        hipLaunchKernelGGL(k_ov_finalize_q, dim3(gq), dim3(256), 0, st, cs, o, per,
+                                           (const uint32_t*)qcand, (const uint32_t*)qn);
+                    } else if (queued) {
+                        hipLaunchKernelGGL(k_ov_probe_q<false>, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
                                            (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
-                                           (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn);
+                                           (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn,
+                                           (const uint64_t*)nullptr, (uint64_t)0);
                         hipLaunchKernelGGL(k_ov_finalize_q, dim3(gq), dim3(256), 0, st, cs, o, per,
                                            (const uint32_t*)qcand, (const uint32_t*)qn);
                     } else {
@@ -1294,6 +1333,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             key.v[10] = (uint64_t)off;
             key.v[11] = (uint64_t)cnt;
             key.v[12] = ((uint64_t)queued << 32) | (queued ? per : g);
+            key.v[13] = vb != nullptr;
             c->graphs.run(key, S, st, loop);
         }
     }
@@ -2272,7 +2312,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!c || !t) return CURC_EINVAL;
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
         (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
-        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 1)
+        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2)
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;
     c->tune.fm_grid = t->fm_grid;

@@ -297,8 +297,10 @@ typedef struct curc_tuning {
     uint32_t fm_fused;    /* exact-matching lists up to this bound run their offset loop in one
                              single-workgroup launch; 0: never (default 4096) */
     uint32_t ov_queue;    /* overlap probe: 1 queues the suffixes that pass the k-mer filter in
-                             LDS and searches their buckets in full batches; 0: one suffix per
-                             thread throughout (default 1) */
+                             LDS and searches their buckets in full batches; 2 as 1, with the
+                             filter minimizer-blocked and its verdicts for all offsets computed in
+                             one pass beforehand (reads of at most 256 bases; others use 1);
+                             0: one suffix per thread throughout (default 2) */
 } curc_tuning;
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
 int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);

@@ -64,6 +64,18 @@ def test_read_lengths(gpu_ctx, L):
     run_both(gpu_ctx, ca.synth_reads(cfg)[1])
 
 
+@pytest.mark.parametrize("L", [60, 75, 150, 250])
+def test_overlap_filter_verdicts(L):
+    """Overlap probe form 2: minimizer-blocked k-mer filter, verdicts for all
+    offsets computed in one pass.  Same links as the oracle."""
+    cfg = ca.synth_config(seed=600 + L, genome_len=40_000, gc=0.46, repeat_frac=0.2, rep_min=200, rep_max=900,
+                          rep_families=4, div_min=0.0, div_max=0.02, n_reads=7_000, L=L,
+                          sub_first=0.002, sub_last=0.01, n_rate=0.0003)
+    with ca.Context(0) as ctx:
+        ctx.set_tuning(ov_queue=2)
+        run_both(ctx, ca.synth_reads(cfg)[1])
+
+
 def test_high_coverage_duplicates(gpu_ctx):
     """Tiny genome at ~1500x: many exact duplicates and contended claims."""
     cfg = ca.synth_config(seed=7, genome_len=2_000, gc=0.5, n_reads=20_000, L=100,

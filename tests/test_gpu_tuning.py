@@ -27,6 +27,8 @@ TUNINGS = [
     dict(ov_grid=4096, fm_grid=4096, map_grid=1 << 20, radix_items=16, fm_fused=0, ov_queue=1),
     dict(ov_grid=3, filter_bits=16, ov_queue=1),
     dict(ov_grid=1 << 20, filter_bits=16, ov_queue=1),
+    dict(ov_grid=16384, filter_bits=8, ov_queue=2),
+    dict(ov_grid=5, filter_bits=64, ov_queue=2),
 ]
 
 
@@ -47,7 +49,7 @@ def test_results_do_not_depend_on_tuning(L):
 def test_invalid_tuning_is_refused(gpu_ctx):
     before = gpu_ctx.get_tuning()
     for bad in (dict(radix_items=12), dict(map_grid=0), dict(workers=5), dict(filter_bits=24), dict(filter_bits=4),
-                dict(ov_queue=2)):
+                dict(ov_queue=3)):
         with pytest.raises(ca.CurcError):
             gpu_ctx.set_tuning(**bad)
     assert gpu_ctx.get_tuning() == before
