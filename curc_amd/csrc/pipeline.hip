@@ -51,7 +51,7 @@ enum Slot {
     S_RM_T, S_RM_TMAX, S_RM_J0, S_RM_J1, S_RM_MARK, S_RM_AS, S_RM_AL, S_RM_AD, S_RM_ASTART, S_RM_AEND,
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
-    S_OVCAND, S_OVNC, S_LB, S_OVVB,
+    S_OVCAND, S_OVNC, S_LB, S_OVVB, S_RECHQ, S_RECMAP,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -121,6 +121,11 @@ struct curc_ctx {
     int device = 0;
     Tuning tune;
     hipStream_t st = nullptr;
+    // copies of results to pinned host memory, ordered after the compute
+    // stream's work by the events cev, so that later stages run meanwhile;
+    // joined before a block is returned (curc_block_run)
+    hipStream_t cst = nullptr;
+    hipEvent_t cev[2] = {nullptr, nullptr};
     Scratch S;
     std::string err;
     bool timing = false;
@@ -223,6 +228,21 @@ void d2h(curc_ctx* c, V& out, const T* src, uint64_t n) {
     static_assert(sizeof(typename V::value_type) == sizeof(T), "element size");
     out.resize(n);
     if (n) CURC_HIP(hipMemcpyAsync(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, c->st));
+}
+
+// d2h on the copy stream, after the compute stream's work so far (event k),
+// when the destination is pinned; else on the compute stream.
+template <typename V, typename T>
+void d2h_side(curc_ctx* c, V& out, const T* src, uint64_t n, int k) {
+    static_assert(sizeof(typename V::value_type) == sizeof(T), "element size");
+    if (n * sizeof(T) < HostPool::POOL_MIN_BYTES) {
+        d2h(c, out, src, n);
+        return;
+    }
+    out.resize(n);
+    CURC_HIP(hipEventRecord(c->cev[k], c->st));
+    CURC_HIP(hipStreamWaitEvent(c->cst, c->cev[k], 0));
+    CURC_HIP(hipMemcpyAsync(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, c->cst));
 }
 
 template <typename T>
@@ -1409,9 +1429,14 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const AsmOut& a = hq_asm;
         B->hq_len = a.len;
         hq_pg = build_pg(c, packed, W, a, S_PG_HQ);
-        records_to_host(c, a, nullptr, 0, B->hq_rec);
-        d2h(c, B->hq_pg2, hq_pg, (a.len + 31) / 32 + 1);
-        CURC_HIP(hipStreamSynchronize(st));
+        // records and text go to the host while mapping runs (own slots:
+        // nothing later in the block writes them)
+        curc_alignment* drec = S.as<curc_alignment>(S_RECHQ, (uint64_t)a.n + 1);
+        if (a.n)
+            hipLaunchKernelGGL(k_rec_aos, dim3(grid_for(a.n)), dim3(256), 0, st, (const uint32_t*)a.rid,
+                               (const uint64_t*)a.pos, (const uint32_t*)nullptr, (uint64_t)a.n, 2u, 0ull, drec);
+        d2h_side(c, B->hq_rec, (const curc_alignment*)drec, a.n, 0);
+        d2h_side(c, B->hq_pg2, (const uint64_t*)hq_pg, (a.len + 31) / 32 + 1, 0);
     }
     if (stop <= CURC_STAGE_ASSEMBLY) return;
 
@@ -1541,15 +1566,16 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const uint32_t nmap = read_cnt(c, cnt + C_NMAP);
         const uint32_t nun = read_cnt(c, cnt + C_NUNMAP);
         const uint32_t nunN = read_cnt(c, cnt + C_NUNMAPN);
-        curc_alignment* drec = S.as<curc_alignment>(S_RECAOS, (uint64_t)nmap + 1);
+        // results go to the host while the LQ stages run (the later stages
+        // only read mis, um and umN, at sizes that keep their allocations)
+        curc_alignment* drec = S.as<curc_alignment>(S_RECMAP, (uint64_t)nmap + 1);
         if (nmap)
             hipLaunchKernelGGL(k_rec_aos, dim3(grid_for(nmap)), dim3(256), 0, st, (const uint32_t*)rid,
                                (const uint64_t*)rw, (const uint32_t*)nullptr, (uint64_t)nmap, 0u, 0ull, drec);
-        d2h(c, B->map_rec, (const curc_alignment*)drec, nmap);
-        d2h(c, B->map_mis, mis, nmap);
-        d2h(c, B->unmapped, um, nun);
-        d2h(c, B->unmapped_N, umN, nunN);
-        CURC_HIP(hipStreamSynchronize(st));
+        d2h_side(c, B->map_rec, (const curc_alignment*)drec, nmap, 1);
+        d2h_side(c, B->map_mis, (const uint8_t*)mis, nmap, 1);
+        d2h_side(c, B->unmapped, (const uint32_t*)um, nun, 1);
+        d2h_side(c, B->unmapped_N, (const uint32_t*)umN, nunN, 1);
         if (trace) {
             const uint64_t nr = std::min<uint64_t>(read_one(c, S.as<unsigned long long>(S_TRCNT, 1)), c->trace_cap);
             d2h(c, B->trace_x, (const float*)S.as<float>(S_TRX, c->trace_cap * MAP_NFEAT), nr * MAP_NFEAT);
@@ -1721,7 +1747,14 @@ curc_ctx* curc_ctx_create(int device) {
     if (const char* t = std::getenv("CURC_TUNE")) parse_tuning(t, c->tune);
     const char* gr = std::getenv("CURC_GRAPHS");
     c->graphs.enabled = gr && std::string(gr) == "1";
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->cev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->cev[1], hipEventDisableTiming) != hipSuccess) {
+        if (c->cev[1]) hipEventDestroy(c->cev[1]);
+        if (c->cev[0]) hipEventDestroy(c->cev[0]);
+        if (c->cst) hipStreamDestroy(c->cst);
+        if (c->st) hipStreamDestroy(c->st);
         delete c;
         return nullptr;
     }
@@ -1734,6 +1767,7 @@ void curc_ctx_destroy(curc_ctx* c) {
     c->workers.clear();
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
+    hipStreamSynchronize(c->cst);
     c->graphs.clear();
     c->S.release();
     for (auto& p : c->pending) {
@@ -1741,6 +1775,9 @@ void curc_ctx_destroy(curc_ctx* c) {
         hipEventDestroy(p.b);
     }
     for (auto e : c->event_pool) hipEventDestroy(e);
+    hipEventDestroy(c->cev[0]);
+    hipEventDestroy(c->cev[1]);
+    hipStreamDestroy(c->cst);
     hipStreamDestroy(c->st);
     delete c;
 }
@@ -1751,6 +1788,7 @@ int curc_ctx_trim(curc_ctx* c) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
+    hipStreamSynchronize(c->cst);
     c->graphs.clear();
     c->S.release();
     for (curc_ctx* w : c->workers) curc_ctx_trim(w);
@@ -1870,6 +1908,7 @@ int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L,
         run_block(c, ascii, n_reads, L, stop_after <= 0 ? CURC_STAGE_ALL : stop_after, flags, B);
         CURC_HIP(hipGetLastError());
         CURC_HIP(hipStreamSynchronize(c->st));
+        CURC_HIP(hipStreamSynchronize(c->cst));
         c->harvest();
         if (flags & CURC_RUN_KEEP_STAGES) {
             // expose the packed reads and N-read text for checking
@@ -1880,13 +1919,16 @@ int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L,
             B->Nascii.assign(na.begin(), na.end());
         }
     } catch (const std::length_error& e) {
-        delete B;
+        // copies into B's arrays may be in flight: join them before freeing
         hipStreamSynchronize(c->st);
+        hipStreamSynchronize(c->cst);
+        delete B;
         c->pending.clear();
         return fail(c, CURC_ELIMIT, e.what());
     } catch (const std::exception& e) {
-        delete B;
         hipStreamSynchronize(c->st);
+        hipStreamSynchronize(c->cst);
+        delete B;
         c->pending.clear();
         std::string m = e.what();
         return fail(c, m.find("out of memory") != std::string::npos ? CURC_ENOMEM : CURC_EHIP, m);
@@ -2366,13 +2408,16 @@ int curc_ref_match(curc_ctx* c, const char* src, uint64_t src_len, const char* d
         CURC_HIP(hipStreamSynchronize(st));
         c->harvest();
     } catch (const std::length_error& e) {
-        delete B;
+        // copies into B's arrays may be in flight: join them before freeing
         hipStreamSynchronize(c->st);
+        hipStreamSynchronize(c->cst);
+        delete B;
         c->pending.clear();
         return fail(c, CURC_ELIMIT, e.what());
     } catch (const std::exception& e) {
-        delete B;
         hipStreamSynchronize(c->st);
+        hipStreamSynchronize(c->cst);
+        delete B;
         c->pending.clear();
         return fail(c, CURC_EHIP, e.what());
     }
