@@ -1572,10 +1572,13 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         if (nmap)
             hipLaunchKernelGGL(k_rec_aos, dim3(grid_for(nmap)), dim3(256), 0, st, (const uint32_t*)rid,
                                (const uint64_t*)rw, (const uint32_t*)nullptr, (uint64_t)nmap, 0u, 0ull, drec);
+        // small arrays are pageable and copy synchronously on the compute
+        // stream: issue them before the large copies, whose DMA they would
+        // otherwise wait behind
+        d2h_side(c, B->unmapped_N, (const uint32_t*)umN, nunN, 1);
+        d2h_side(c, B->unmapped, (const uint32_t*)um, nun, 1);
         d2h_side(c, B->map_rec, (const curc_alignment*)drec, nmap, 1);
         d2h_side(c, B->map_mis, (const uint8_t*)mis, nmap, 1);
-        d2h_side(c, B->unmapped, (const uint32_t*)um, nun, 1);
-        d2h_side(c, B->unmapped_N, (const uint32_t*)umN, nunN, 1);
         if (trace) {
             const uint64_t nr = std::min<uint64_t>(read_one(c, S.as<unsigned long long>(S_TRCNT, 1)), c->trace_cap);
             d2h(c, B->trace_x, (const float*)S.as<float>(S_TRX, c->trace_cap * MAP_NFEAT), nr * MAP_NFEAT);
