@@ -67,7 +67,7 @@ class Job(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("ov_grid", "fm_grid", "map_grid", "radix_items", "workers",
-                                               "filter_bits", "fm_fused", "ov_queue")]
+                                               "filter_bits", "fm_fused", "ov_queue", "map_both")]
 
 
 class StageStat(ctypes.Structure):

@@ -1254,6 +1254,9 @@ __global__ void k_map_hash(const uint64_t* __restrict__ pg, uint64_t start, uint
 // positions' line.  Used while H * 64 bytes stays within MAP_SLOT_MAX_H.
 constexpr uint32_t MAP_SLOT = 16;
 constexpr uint64_t MAP_SLOT_MAX_H = 1ull << 28;
+// Both strands' slots of bucket h in one 128-byte line: forward at word 0,
+// reverse complement at word MAP_SLOT (k_map_both).
+constexpr uint32_t MAP_SLOT2 = 2 * MAP_SLOT;
 
 // One thread per bucket writes its slot: the capped size and the first
 // min(n,12) occurrences (ascending i, as sorted) in descending order, the
@@ -1261,7 +1264,7 @@ constexpr uint64_t MAP_SLOT_MAX_H = 1ull << 28;
 // readers never look past the size.  16-byte stores, the first always, the
 // others only when the bucket reaches them.
 __global__ void k_map_slots(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ krf, uint64_t H,
-                            uint32_t* __restrict__ bk) {
+                            uint32_t* __restrict__ bk, uint32_t stride = MAP_SLOT, uint32_t off = 0) {
     GRID_STRIDE(h, H) {
         const uint32_t lo = krf[h];
         const uint32_t c = min(krf[h + 1] - lo, MAP_BUCKET);
@@ -1271,11 +1274,40 @@ __global__ void k_map_slots(const uint32_t* __restrict__ vals, const uint32_t* _
         for (uint32_t p = 0; p < MAP_BUCKET; ++p) w[1 + p] = p < c ? vals[lo + c - 1 - p] : 0u;
 #pragma unroll
         for (uint32_t p = 1 + MAP_BUCKET; p < MAP_SLOT; ++p) w[p] = 0u;
-        uint4* d = reinterpret_cast<uint4*>(bk + h * MAP_SLOT);
+        uint4* d = reinterpret_cast<uint4*>(bk + h * stride + off);
         d[0] = make_uint4(w[0], w[1], w[2], w[3]);
         if (c > 3) d[1] = make_uint4(w[4], w[5], w[6], w[7]);
         if (c > 7) d[2] = make_uint4(w[8], w[9], w[10], w[11]);
         if (c > 11) d[3] = make_uint4(w[12], w[13], w[14], w[15]);
+    }
+}
+
+// Both halves of the two-strand table (MAP_SLOT2) in one pass: bucket h's
+// forward slot from (vals0, krf0) and reverse-complement slot from (vals1,
+// krf1), each as k_map_slots writes it, so each 128-byte line is written
+// by one thread.
+__global__ void k_map_slots2(const uint32_t* __restrict__ vals0, const uint32_t* __restrict__ krf0,
+                             const uint32_t* __restrict__ vals1, const uint32_t* __restrict__ krf1, uint64_t H,
+                             uint32_t* __restrict__ bk) {
+    GRID_STRIDE(h, H) {
+#pragma unroll
+        for (uint32_t sd = 0; sd < 2; ++sd) {
+            const uint32_t* vals = sd ? vals1 : vals0;
+            const uint32_t* krf = sd ? krf1 : krf0;
+            const uint32_t lo = krf[h];
+            const uint32_t c = min(krf[h + 1] - lo, MAP_BUCKET);
+            uint32_t w[MAP_SLOT];
+            w[0] = c;
+#pragma unroll
+            for (uint32_t p = 0; p < MAP_BUCKET; ++p) w[1 + p] = p < c ? vals[lo + c - 1 - p] : 0u;
+#pragma unroll
+            for (uint32_t p = 1 + MAP_BUCKET; p < MAP_SLOT; ++p) w[p] = 0u;
+            uint4* d = reinterpret_cast<uint4*>(bk + h * MAP_SLOT2 + sd * MAP_SLOT);
+            d[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            if (c > 3) d[1] = make_uint4(w[4], w[5], w[6], w[7]);
+            if (c > 7) d[2] = make_uint4(w[8], w[9], w[10], w[11]);
+            if (c > 11) d[3] = make_uint4(w[12], w[13], w[14], w[15]);
+        }
     }
 }
 
@@ -1349,6 +1381,11 @@ struct MapArgs {
     unsigned long long* work;
     uint32_t max_mm = 0;       // floor(L / 5) (used by the probe-range kernels)
     const uint32_t* bk = nullptr;  // bucket-inline index (kr / pos unused when set)
+    // slot h of the strand's index starts at word h * bk_stride + bk_off of
+    // bk: one strand per table (MAP_SLOT, 0), or both strands in one table
+    // (MAP_SLOT2, MAP_SLOT * strand)
+    uint32_t bk_stride = 16;
+    uint32_t bk_off = 0;
 };
 
 // Bucket h of the mapping index: its capped size and its positions.
@@ -1358,7 +1395,7 @@ struct MapBucket {
 };
 __device__ __forceinline__ MapBucket map_bucket(const MapArgs& a, uint32_t h) {
     if (a.bk) {
-        const uint32_t* s = a.bk + (uint64_t)h * MAP_SLOT;
+        const uint32_t* s = a.bk + (uint64_t)h * a.bk_stride + a.bk_off;
         return MapBucket{ld32(s), s + 1};
     }
     const uint32_t b = a.kr[h], e = a.kr[h + 1];
@@ -1694,6 +1731,222 @@ __global__ void __launch_bounds__(256) k_map_fast(MapArgs a) {
             n_open_probe += n_probe - probe0;
             n_open_cand += n_cand - cand0;
         }
+    }
+    wave_add_u64(&a.work[0], n_act);
+    wave_add_u64(&a.work[1], n_probe);
+    wave_add_u64(&a.work[2], n_cand);
+    wave_add_u64(&a.work[3], n_first);
+    wave_add_u64(&a.work[4], n_open_probe);
+    wave_add_u64(&a.work[5], n_open_cand);
+}
+
+// Both strand passes of a read in one loop over its probes, on the
+// two-strand table (MAP_SLOT2): each probe fetches one line that holds the
+// bucket's forward and reverse-complement slots, where the separate passes
+// fetch two.  The result and the work counters are those of the forward
+// pass followed by the reverse-complement pass (k_map_fast<false> then
+// <true>):
+//  - the forward state is exactly the forward pass's; when it reaches
+//    MAP_TARGET the read is done and the reverse pass never counts;
+//  - the reverse pass runs alongside with its own best Rb, accepting c <
+//    min(Rb, F) where F is the forward best so far.  Since F only falls and
+//    the sequential pass starts from the final F, a candidate with c >= F
+//    can never be the reverse winner; the reverse winner is the first
+//    candidate reaching the least such c, as in the sequential pass, and it
+//    replaces the forward result when it is below the final F.  Its partial
+//    sums stop at min(Rb, F) - 1 >= 1, so counts at or below MAP_TARGET are
+//    exact and it stops at the same candidate as the sequential pass;
+//  - each pass keeps its own SeenCache, and the reverse work is added to
+//    the counters only when the forward pass ends above MAP_TARGET.
+template <uint32_t WM>
+__global__ void __launch_bounds__(256, 5) k_map_both(MapArgs a) {
+    constexpr uint32_t G = 1;  // probes whose slot heads are requested together
+    uint32_t n_act = 0, n_probe = 0, n_cand = 0, n_first = 0, n_open_probe = 0, n_open_cand = 0;
+    const uint32_t L = a.L, W = a.W;
+    const uint32_t np = (L - MAP_K) / MAP_READ_STEP + 1;
+    GRID_STRIDE(i, a.n_un + a.n_N) {
+        uint32_t last = a.last[i];
+        if (last <= MAP_TARGET) continue;
+        uint64_t w = a.word[i];
+        if (i >= a.n_un) {
+            // N-reads: the two passes one after the other (as the two launches)
+            MapArgs ar = a;
+            ar.bk_off = MAP_SLOT;
+            for (int rc = 0; rc < 2; ++rc) {
+                if (last <= MAP_TARGET) break;
+                ++n_act;
+                const uint32_t probe0 = n_probe, cand0 = n_cand;
+                SeenCache seen;
+                if (rc)
+                    map_nread<true>(ar, i, last, w, seen, n_probe, n_cand);
+                else
+                    map_nread<false>(a, i, last, w, seen, n_probe, n_cand);
+                if (last <= MAP_TARGET) {
+                    n_first += (n_probe - probe0 == 1) ? 1u : 0u;
+                } else {
+                    n_open_probe += n_probe - probe0;
+                    n_open_cand += n_cand - cand0;
+                }
+            }
+            a.last[i] = (uint8_t)last;
+            a.word[i] = w;
+            continue;
+        }
+        uint64_t qf[WM], qr[WM];
+        {
+            const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * W;
+            uint64_t r[WM];
+#pragma unroll
+            for (uint32_t k = 0; k < WM; ++k) r[k] = k < W ? ld64(rd + k) : 0ull;
+#pragma unroll
+            for (uint32_t k = 0; k < WM; ++k) {
+                qf[k] = qr[k] = 0;
+                if (32 * k < L) {
+                    const uint32_t n = min(32u, L - 32 * k);
+                    qf[k] = xw_pick(r, 32 * k, n);
+                    qr[k] = revcomp(xw_pick(r, L - 32 * k - n, n), n);
+                }
+            }
+        }
+        // the read's words again from qf: all full except the last chunk,
+        // which qf holds right-aligned
+        const uint32_t nlast = L - 32 * (W - 1);
+        auto rword = [&](uint32_t k) -> uint64_t {
+            const uint64_t v = pick(qf, k);
+            return (k == W - 1 && nlast < 32) ? v << (2 * (32 - nlast)) : v;
+        };
+        auto kmer = [&](uint32_t off) -> uint64_t {  // MAP_K bases from base off (as xw_pick)
+            const uint32_t o1 = off >> 5;
+            const uint32_t o2 = (off & 31u) << 1;
+            const uint64_t left = rword(o1) << o2;
+            if ((64u - o2) >= (MAP_K << 1)) return left >> ((32u - MAP_K) << 1);
+            const uint64_t right = (rword(o1 + 1) >> (62u - o2)) >> 2;
+            return (left | right) >> ((32u - MAP_K) << 1);
+        };
+        // mismatches of query q against the window at mp, stopping once above bound
+        auto count = [&](const uint64_t (&q)[WM], uint64_t mp, uint32_t bound) -> uint32_t {
+            const uint64_t* g0 = a.pg + (mp >> 5);
+            uint64_t g[WM + 1];
+#pragma unroll
+            for (uint32_t k = 0; k <= WM; ++k) g[k] = k <= W ? ld64(g0 + k) : 0ull;
+            const uint32_t sh = (uint32_t)(mp & 31u) << 1;
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < WM; ++k) {
+                if (32 * k < L && c <= bound) {
+                    const uint32_t n = min(32u, L - 32 * k);
+                    const uint64_t win = sh ? (g[k] << sh) | (g[k + 1] >> (64u - sh)) : g[k];
+                    c += mm_words(win >> ((32u - n) << 1), q[k]);
+                }
+            }
+            return c;
+        };
+        uint32_t F = last, Rb = last;  // forward best, reverse best (below F when it counts)
+        uint64_t wF = w, wR = w;
+        bool doneR = false;
+        uint32_t pF = 0, cF = 0, pR = 0, cR = 0;  // work of each pass
+        SeenCache seenF, seenR;
+        for (uint32_t t0 = 0; t0 < np; t0 += G) {
+            uint4 hf[G], hr[G];
+            uint32_t hh[G];
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) {
+                hf[u] = hr[u] = make_uint4(0, 0, 0, 0);
+                hh[u] = 0;
+                if (t0 + u < np) {
+                    hh[u] = (uint32_t)(murmur64(kmer(MAP_READ_STEP * (t0 + u))) & a.mask);
+                    const uint32_t* sl = a.bk + (uint64_t)hh[u] * MAP_SLOT2;
+                    hf[u] = *reinterpret_cast<const uint4*>(sl);
+                    if (!doneR) hr[u] = *reinterpret_cast<const uint4*>(sl + MAP_SLOT);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) {
+                const uint32_t t = t0 + u;
+                if (t >= np) break;
+                const uint32_t j = MAP_READ_STEP * t;
+                const uint32_t* sl = a.bk + (uint64_t)hh[u] * MAP_SLOT2;
+                // forward pass, probe t
+                ++pF;
+                {
+                    const uint32_t nb = hf[u].x;
+                    const uint64_t ls = (uint64_t)j;
+                    for (uint32_t p = 0; p < nb; ++p) {
+                        const uint32_t pv = p == 0 ? hf[u].y : p == 1 ? hf[u].z : p == 2 ? hf[u].w : ld32(sl + 1 + p);
+                        const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
+                        if (rp < ls) continue;
+                        const uint64_t mp = rp - ls;
+                        if (mp + L > a.pg_len) continue;
+                        if (seenF.check_insert(mp)) continue;
+                        ++cF;
+                        const uint32_t c = count(qf, mp, F - 1);
+                        if (c < F) {
+                            wF = mp << 2;
+                            F = c;
+                            if (F <= MAP_TARGET) goto fwd_done;
+                        }
+                    }
+                }
+                // reverse pass, probe t
+                if (!doneR) {
+                    ++pR;
+                    const uint32_t nb = hr[u].x;
+                    const uint64_t ls = (uint64_t)(L - MAP_K - j);
+                    for (uint32_t p = 0; p < nb; ++p) {
+                        const uint32_t pv = p == 0   ? hr[u].y
+                                            : p == 1 ? hr[u].z
+                                            : p == 2 ? hr[u].w
+                                                     : ld32(sl + MAP_SLOT + 1 + p);
+                        const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
+                        if (rp < ls) continue;
+                        const uint64_t mp = rp - ls;
+                        if (mp + L > a.pg_len) continue;
+                        if (seenR.check_insert(mp)) continue;
+                        ++cR;
+                        const uint32_t lim = min(Rb, F);
+                        const uint32_t c = count(qr, mp, lim - 1);
+                        if (c < lim) {
+                            wR = 2ull | (mp << 2);
+                            Rb = c;
+                            if (Rb <= MAP_TARGET) {
+                                doneR = true;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // forward pass ended above MAP_TARGET: the reverse pass counts
+        n_act += 2;
+        n_probe += pF + pR;
+        n_cand += cF + cR;
+        n_open_probe += pF;
+        n_open_cand += cF;
+        if (doneR) {
+            n_first += (pR == 1) ? 1u : 0u;
+        } else {
+            n_open_probe += pR;
+            n_open_cand += cR;
+        }
+        if (Rb < F) {
+            last = Rb;
+            w = wR;
+        } else {
+            last = F;
+            w = wF;
+        }
+        a.last[i] = (uint8_t)last;
+        a.word[i] = w;
+        continue;
+    fwd_done:
+        // forward pass reached MAP_TARGET: its result; no reverse pass
+        ++n_act;
+        n_probe += pF;
+        n_cand += cF;
+        n_first += (pF == 1) ? 1u : 0u;
+        a.last[i] = (uint8_t)F;
+        a.word[i] = wF;
     }
     wave_add_u64(&a.work[0], n_act);
     wave_add_u64(&a.work[1], n_probe);

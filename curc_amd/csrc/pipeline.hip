@@ -51,7 +51,7 @@ enum Slot {
     S_RM_T, S_RM_TMAX, S_RM_J0, S_RM_J1, S_RM_MARK, S_RM_AS, S_RM_AL, S_RM_AD, S_RM_ASTART, S_RM_AEND,
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
-    S_OVCAND, S_OVNC, S_LB, S_OVVB, S_RECHQ, S_RECMAP,
+    S_OVCAND, S_OVNC, S_LB, S_OVVB, S_RECHQ, S_RECMAP, S_MK2, S_MK3, S_MV2, S_MV3, S_KRF2,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -113,6 +113,7 @@ struct Tuning {
     uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
     uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
     uint32_t fm_fused = 4096;    // exact-matching lists up to this bound run in one fused launch (0: never)
+    uint32_t map_both = 1;       // both strand passes in one launch on a two-strand index (1)
     uint32_t ov_queue = 2;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
 };
@@ -585,6 +586,18 @@ RmIndex rm_index(curc_ctx* c, const uint64_t* src, uint64_t src_len) {
     return ix;
 }
 
+// k_map_both for the read's word count W <= 8 (register capacity 3, 4, 5 or 8).
+void launch_map_both(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
+    if (W <= 3)
+        hipLaunchKernelGGL(k_map_both<3>, grid, dim3(256), 0, st, a);
+    else if (W <= 4)
+        hipLaunchKernelGGL(k_map_both<4>, grid, dim3(256), 0, st, a);
+    else if (W <= 5)
+        hipLaunchKernelGGL(k_map_both<5>, grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_map_both<8>, grid, dim3(256), 0, st, a);
+}
+
 // k_map_fast for the read's word count W <= 8 (register capacity 3, 4, 5 or 8).
 void launch_map_fast(bool rc, uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
 #define CURC_MAP_FAST(WM)                                                            \
@@ -643,6 +656,7 @@ void parse_tuning(const char* text, Tuning& t) {
             const uint32_t v = (uint32_t)std::strtoul(kv.c_str() + e + 1, nullptr, 10);
             if (k == "fm_fused") t.fm_fused = v;
             if (k == "ov_queue" && v <= 2) t.ov_queue = v;
+            if (k == "map_both" && v <= 1) t.map_both = v;
             if (v > 0) {
                 if (k == "ov_grid") t.ov_grid = v;
                 else if (k == "fm_grid") t.fm_grid = v;
@@ -1456,7 +1470,65 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         if (nm) hipLaunchKernelGGL(k_map_init, dim3(grid_for(nm)), dim3(256), 0, st, last, word, (uint64_t)n_unm, nm, P.max_mm);
         uint64_t index_count = 0;
         if (ref_len + 1 >= MAP_K) index_count = (ref_len - MAP_K + 1 + MAP_REF_STEP - 1) / MAP_REF_STEP;
-        if (L >= MAP_K && nm > 0) {
+        // both strand passes in one launch (k_map_both) on an index holding
+        // both strands' slots of a bucket in one 128-byte line: the plain
+        // mode, one index part, bucket-inline slots, reads of <= 256 bases
+        const uint32_t H1 = index_count ? hash_size(index_count) : 0;
+        const bool both = c->tune.map_both && !phased && L >= MAP_K && nm > 0 && W <= 8 && !c->index_csr &&
+                          index_count > 0 && index_count <= (1ull << 30) && H1 <= MAP_SLOT_MAX_H &&
+                          index_count * 8 >= H1;
+        if (both) {
+            const uint64_t count = index_count;
+            const uint32_t H = H1;
+            uint32_t* bk = S.as<uint32_t>(S_MBK, (uint64_t)H * MAP_SLOT2);
+            {
+                Region r(c, "map_index");
+                // each strand's (hash, position) pairs sorted, then both slots
+                // of every bucket written together
+                const int sk[2][5] = {{S_MK0, S_MK1, S_MV0, S_MV1, S_KRF}, {S_MK2, S_MK3, S_MV2, S_MV3, S_KRF2}};
+                const uint32_t* sv[2];
+                const uint32_t* sr[2];
+                for (int strand = 0; strand < 2; ++strand) {
+                    uint32_t* k0 = S.as<uint32_t>(sk[strand][0], count + 1);
+                    uint32_t* k1 = S.as<uint32_t>(sk[strand][1], count + 1);
+                    uint32_t* v0 = S.as<uint32_t>(sk[strand][2], count + 1);
+                    uint32_t* v1 = S.as<uint32_t>(sk[strand][3], count + 1);
+                    uint32_t* krf = S.as<uint32_t>(sk[strand][4], (uint64_t)H + 2);
+                    hipLaunchKernelGGL(k_map_hash, dim3(grid_for(count)), dim3(256), 0, st, (const uint64_t*)hq_pg,
+                                       (uint64_t)0, count, strand == 1, H - 1, k0, v0);
+                    radix_sort_pairs<uint32_t>(k0, v0, k1, v1, count, 0, log2u(H), S, S_PRIM, st);
+                    hipLaunchKernelGGL(k_gap_fill, dim3(grid_for(count + 1)), dim3(256), 0, st,
+                                       (const uint32_t*)k0, count, (uint64_t)H + 1, krf);
+                    sv[strand] = v0;
+                    sr[strand] = krf;
+                }
+                hipLaunchKernelGGL(k_map_slots2, dim3(grid_for(H)), dim3(256), 0, st, sv[0], sr[0], sv[1], sr[1],
+                                   (uint64_t)H, bk);
+                c->add_bytes("map_index", 2.0 * ((double)count * (16 + 8) + 8.0 * H), 2.0 * (double)count);
+            }
+            unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
+            fill<unsigned long long>(c, work, 8, 0ull);
+            {
+                Region r(c, "mapping");
+                MapArgs a{hq_pg, ref_len, nullptr, nullptr, H - 1, 0, packed, W, L, unm, n_unm,
+                          Nascii, nN, last, word, work, P.max_mm, bk};
+                a.bk_stride = MAP_SLOT2;
+                a.bk_off = 0;
+                launch_map_both(W, dim3(grid_for(nm, c->tune.map_grid)), st, a);
+            }
+            if (c->timing) {
+                // as for the separate passes, over both strands
+                unsigned long long wk[8];
+                CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
+                CURC_HIP(hipStreamSynchronize(st));
+                const double P8 = 8.0 * W;
+                c->add_bytes("mapping", (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
+                             (double)wk[2]);
+                const char* names[6] = {"map_active", "map_probes", "map_cands", "map_first_probe",
+                                        "map_open_probes", "map_open_cands"};
+                for (int q = 0; q < 6; ++q) c->add_bytes(names[q], 0.0, (double)wk[q]);
+            }
+        } else if (L >= MAP_K && nm > 0) {
             for (int strand = 0; strand < 2; ++strand) {
                 const bool rc = strand == 1;
                 uint64_t parts[2][2];
@@ -2350,6 +2422,7 @@ int curc_ctx_get_tuning(curc_ctx* c, curc_tuning* t) {
     t->filter_bits = c->tune.filter_bits;
     t->fm_fused = c->tune.fm_fused;
     t->ov_queue = c->tune.ov_queue;
+    t->map_both = c->tune.map_both;
     return CURC_OK;
 }
 
@@ -2357,7 +2430,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!c || !t) return CURC_EINVAL;
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
         (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
-        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2)
+        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2 || t->map_both > 1)
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;
     c->tune.fm_grid = t->fm_grid;
@@ -2367,6 +2440,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     c->tune.filter_bits = t->filter_bits;
     c->tune.fm_fused = t->fm_fused;
     c->tune.ov_queue = t->ov_queue;
+    c->tune.map_both = t->map_both;
     return CURC_OK;
 }
 

@@ -301,6 +301,9 @@ typedef struct curc_tuning {
                              filter minimizer-blocked and its verdicts for all offsets computed in
                              one pass beforehand (reads of at most 256 bases; others use 1);
                              0: one suffix per thread throughout (default 2) */
+    uint32_t map_both;    /* 1: both strand passes of the mapping in one launch, on an index that
+                             holds a bucket's forward and reverse-complement slots in one line;
+                             0: one launch and one index per strand (default 1) */
 } curc_tuning;
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
 int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);

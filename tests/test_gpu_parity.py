@@ -76,6 +76,45 @@ def test_overlap_filter_verdicts(L):
         run_both(ctx, ca.synth_reads(cfg)[1])
 
 
+@pytest.mark.parametrize("case", ["plain", "repeats"])
+def test_mapping_both_strands_one_launch(case):
+    """Tuning map_both=1 (the default): both strand passes in one launch on the two-strand
+    index (taken by blocks whose pseudogenome fills bucket-inline slots, so
+    6 and 9 Mbp genomes at 30x here).  Same results as the oracle (OpenMP build),
+    and the same work counters as the forward pass followed by the
+    reverse-complement pass."""
+    if case == "plain":
+        cfg = ca.config_scale(seed=81, n_reads=1_200_000, genome_len=6_000_000)
+    else:
+        cfg = ca.synth_config(seed=82, genome_len=9_000_000, gc=0.45, repeat_frac=0.3, rep_min=300, rep_max=3000,
+                              rep_families=12, div_min=0.0, div_max=0.03, n_reads=1_800_000, L=150,
+                              sub_first=0.002, sub_last=0.02, n_rate=0.001)
+    reads = ca.synth_reads(cfg)[1]
+    print(f"case {case}: {reads.shape}", flush=True)
+    names = ("map_active", "map_probes", "map_cands", "map_first_probe", "map_open_probes", "map_open_cands")
+
+    def work(ctx):
+        ctx.reset_stats()
+        ctx.set_timing(True)
+        blk = ctx.run_block(reads)
+        ctx.set_timing(False)
+        st = {s["name"]: s for s in ctx.stats()}
+        return blk, {k: st[k]["items"] for k in names}, st
+
+    with ca.Context(0) as ctx:
+        ctx.set_tuning(map_both=0)
+        b0, separate, st0 = work(ctx)
+        ctx.set_tuning(map_both=1)
+        b1, together, st1 = work(ctx)
+        assert st1["mapping"]["launches"] == 1 and st0["mapping"]["launches"] == 2, "the combined path did not run"
+        assert together == separate
+        o = orc.OracleBlock(reads, stop="all", kind="omp")
+        compare(b1, o, ("hq_pg", "hq_rec", "map_rec", "map_mis", "unmapped", "unmapped_N", "lq_pg", "lq_rec",
+                        "nlq_pg", "nlq_rec"))
+        b0.free()
+        b1.free()
+
+
 def test_high_coverage_duplicates(gpu_ctx):
     """Tiny genome at ~1500x: many exact duplicates and contended claims."""
     cfg = ca.synth_config(seed=7, genome_len=2_000, gc=0.5, n_reads=20_000, L=100,
