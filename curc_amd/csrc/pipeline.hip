@@ -124,9 +124,13 @@ struct curc_ctx {
     hipStream_t st = nullptr;
     // copies of results to pinned host memory, ordered after the compute
     // stream's work by the events cev, so that later stages run meanwhile;
-    // joined before a block is returned (curc_block_run)
+    // joined before a block is returned (curc_block_run).  Created on first
+    // use, and not used by the contexts of a batch: their streams already
+    // overlap one another, and more streams than hardware queues would make
+    // unrelated streams share a queue (and its order)
     hipStream_t cst = nullptr;
     hipEvent_t cev[2] = {nullptr, nullptr};
+    bool side_copies = true;
     Scratch S;
     std::string err;
     bool timing = false;
@@ -233,10 +237,20 @@ void d2h(curc_ctx* c, V& out, const T* src, uint64_t n) {
 
 // d2h on the copy stream, after the compute stream's work so far (event k),
 // when the destination is pinned; else on the compute stream.
+void join_side(curc_ctx* c) {
+    if (c->cst) CURC_HIP(hipStreamSynchronize(c->cst));
+}
+
 template <typename V, typename T>
 void d2h_side(curc_ctx* c, V& out, const T* src, uint64_t n, int k) {
     static_assert(sizeof(typename V::value_type) == sizeof(T), "element size");
-    if (n * sizeof(T) < HostPool::POOL_MIN_BYTES) {
+    if (!c->side_copies || n * sizeof(T) < HostPool::POOL_MIN_BYTES) {
+        d2h(c, out, src, n);
+        return;
+    }
+    if (!c->cst && hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking) != hipSuccess) {
+        c->cst = nullptr;
+        c->side_copies = false;
         d2h(c, out, src, n);
         return;
     }
@@ -1823,12 +1837,10 @@ curc_ctx* curc_ctx_create(int device) {
     const char* gr = std::getenv("CURC_GRAPHS");
     c->graphs.enabled = gr && std::string(gr) == "1";
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->cev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->cev[1], hipEventDisableTiming) != hipSuccess) {
         if (c->cev[1]) hipEventDestroy(c->cev[1]);
         if (c->cev[0]) hipEventDestroy(c->cev[0]);
-        if (c->cst) hipStreamDestroy(c->cst);
         if (c->st) hipStreamDestroy(c->st);
         delete c;
         return nullptr;
@@ -1842,7 +1854,7 @@ void curc_ctx_destroy(curc_ctx* c) {
     c->workers.clear();
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
-    hipStreamSynchronize(c->cst);
+    if (c->cst) hipStreamSynchronize(c->cst);
     c->graphs.clear();
     c->S.release();
     for (auto& p : c->pending) {
@@ -1852,7 +1864,7 @@ void curc_ctx_destroy(curc_ctx* c) {
     for (auto e : c->event_pool) hipEventDestroy(e);
     hipEventDestroy(c->cev[0]);
     hipEventDestroy(c->cev[1]);
-    hipStreamDestroy(c->cst);
+    if (c->cst) hipStreamDestroy(c->cst);
     hipStreamDestroy(c->st);
     delete c;
 }
@@ -1863,7 +1875,7 @@ int curc_ctx_trim(curc_ctx* c) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
-    hipStreamSynchronize(c->cst);
+    if (c->cst) hipStreamSynchronize(c->cst);
     c->graphs.clear();
     c->S.release();
     for (curc_ctx* w : c->workers) curc_ctx_trim(w);
@@ -1983,7 +1995,7 @@ int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L,
         run_block(c, ascii, n_reads, L, stop_after <= 0 ? CURC_STAGE_ALL : stop_after, flags, B);
         CURC_HIP(hipGetLastError());
         CURC_HIP(hipStreamSynchronize(c->st));
-        CURC_HIP(hipStreamSynchronize(c->cst));
+        join_side(c);
         c->harvest();
         if (flags & CURC_RUN_KEEP_STAGES) {
             // expose the packed reads and N-read text for checking
@@ -1996,13 +2008,13 @@ int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L,
     } catch (const std::length_error& e) {
         // copies into B's arrays may be in flight: join them before freeing
         hipStreamSynchronize(c->st);
-        hipStreamSynchronize(c->cst);
+        if (c->cst) hipStreamSynchronize(c->cst);
         delete B;
         c->pending.clear();
         return fail(c, CURC_ELIMIT, e.what());
     } catch (const std::exception& e) {
         hipStreamSynchronize(c->st);
-        hipStreamSynchronize(c->cst);
+        if (c->cst) hipStreamSynchronize(c->cst);
         delete B;
         c->pending.clear();
         std::string m = e.what();
@@ -2155,8 +2167,19 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
     while (c->workers.size() + 1 < workers) {
         curc_ctx* w = curc_ctx_create(c->device);
         if (!w) return fail(c, CURC_EHIP, "could not create a worker stream");
+        w->side_copies = false;
         c->workers.push_back(w);
     }
+    // the batch's contexts copy results on their own streams (one stream
+    // per context); the caller's copy stream is released for the batch
+    hipSetDevice(c->device);
+    if (c->cst) {
+        hipStreamSynchronize(c->cst);
+        hipStreamDestroy(c->cst);
+        c->cst = nullptr;
+    }
+    const bool side = c->side_copies;
+    c->side_copies = false;
     std::vector<curc_ctx*> pool{c};
     for (uint32_t w = 1; w < workers; ++w) {
         curc_ctx* wc = c->workers[w - 1];
@@ -2214,6 +2237,7 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
         c->graphs.replays += wc->graphs.replays;
         wc->graphs.eager = wc->graphs.captures = wc->graphs.replays = 0;
     }
+    c->side_copies = side;
     return first;
 }
 
@@ -2487,13 +2511,13 @@ int curc_ref_match(curc_ctx* c, const char* src, uint64_t src_len, const char* d
     } catch (const std::length_error& e) {
         // copies into B's arrays may be in flight: join them before freeing
         hipStreamSynchronize(c->st);
-        hipStreamSynchronize(c->cst);
+        if (c->cst) hipStreamSynchronize(c->cst);
         delete B;
         c->pending.clear();
         return fail(c, CURC_ELIMIT, e.what());
     } catch (const std::exception& e) {
         hipStreamSynchronize(c->st);
-        hipStreamSynchronize(c->cst);
+        if (c->cst) hipStreamSynchronize(c->cst);
         delete B;
         c->pending.clear();
         return fail(c, CURC_EHIP, e.what());
