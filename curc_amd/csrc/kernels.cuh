@@ -1284,29 +1284,29 @@ __global__ void k_map_slots(const uint32_t* __restrict__ vals, const uint32_t* _
 
 // Both halves of the two-strand table (MAP_SLOT2) in one pass: bucket h's
 // forward slot from (vals0, krf0) and reverse-complement slot from (vals1,
-// krf1), each as k_map_slots writes it, so each 128-byte line is written
-// by one thread.
+// krf1), each as k_map_slots writes it (the head always, further 16-byte
+// pieces as far as the size reaches).  Eight consecutive lanes write the
+// eight pieces of one bucket's line, so a wave's writes fall in eight
+// consecutive lines.
 __global__ void k_map_slots2(const uint32_t* __restrict__ vals0, const uint32_t* __restrict__ krf0,
                              const uint32_t* __restrict__ vals1, const uint32_t* __restrict__ krf1, uint64_t H,
                              uint32_t* __restrict__ bk) {
-    GRID_STRIDE(h, H) {
+    GRID_STRIDE(t, H * 8) {
+        const uint64_t h = t >> 3;
+        const uint32_t sd = (uint32_t)(t >> 2) & 1u, q = (uint32_t)t & 3u;  // strand, piece of its slot
+        const uint32_t* vals = sd ? vals1 : vals0;
+        const uint32_t* krf = sd ? krf1 : krf0;
+        const uint32_t lo = krf[h];
+        const uint32_t c = min(krf[h + 1] - lo, MAP_BUCKET);
+        if (q == 0 || c > 4 * q - 1) {
+            // slot word 0: the size; word 1 + p: position p, in descending order
+            uint32_t w[4];
 #pragma unroll
-        for (uint32_t sd = 0; sd < 2; ++sd) {
-            const uint32_t* vals = sd ? vals1 : vals0;
-            const uint32_t* krf = sd ? krf1 : krf0;
-            const uint32_t lo = krf[h];
-            const uint32_t c = min(krf[h + 1] - lo, MAP_BUCKET);
-            uint32_t w[MAP_SLOT];
-            w[0] = c;
-#pragma unroll
-            for (uint32_t p = 0; p < MAP_BUCKET; ++p) w[1 + p] = p < c ? vals[lo + c - 1 - p] : 0u;
-#pragma unroll
-            for (uint32_t p = 1 + MAP_BUCKET; p < MAP_SLOT; ++p) w[p] = 0u;
-            uint4* d = reinterpret_cast<uint4*>(bk + h * MAP_SLOT2 + sd * MAP_SLOT);
-            d[0] = make_uint4(w[0], w[1], w[2], w[3]);
-            if (c > 3) d[1] = make_uint4(w[4], w[5], w[6], w[7]);
-            if (c > 7) d[2] = make_uint4(w[8], w[9], w[10], w[11]);
-            if (c > 11) d[3] = make_uint4(w[12], w[13], w[14], w[15]);
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t word = 4 * q + k;
+                w[k] = word == 0 ? c : (word - 1 < c ? vals[lo + c - word] : 0u);
+            }
+            *reinterpret_cast<uint4*>(bk + h * MAP_SLOT2 + sd * MAP_SLOT + 4 * q) = make_uint4(w[0], w[1], w[2], w[3]);
         }
     }
 }

@@ -1516,8 +1516,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     sv[strand] = v0;
                     sr[strand] = krf;
                 }
-                hipLaunchKernelGGL(k_map_slots2, dim3(grid_for(H)), dim3(256), 0, st, sv[0], sr[0], sv[1], sr[1],
-                                   (uint64_t)H, bk);
+                hipLaunchKernelGGL(k_map_slots2, dim3(grid_for((uint64_t)H * 8, 1u << 20)), dim3(256), 0, st, sv[0],
+                                   sr[0], sv[1], sr[1], (uint64_t)H, bk);
                 c->add_bytes("map_index", 2.0 * ((double)count * (16 + 8) + 8.0 * H), 2.0 * (double)count);
             }
             unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
