@@ -1264,21 +1264,24 @@ constexpr uint32_t MAP_SLOT2 = 2 * MAP_SLOT;
 // readers never look past the size.  16-byte stores, the first always, the
 // others only when the bucket reaches them.
 __global__ void k_map_slots(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ krf, uint64_t H,
-                            uint32_t* __restrict__ bk, uint32_t stride = MAP_SLOT, uint32_t off = 0) {
-    GRID_STRIDE(h, H) {
+                            uint32_t* __restrict__ bk) {
+    // four consecutive lanes write the four 16-byte pieces of one slot: the
+    // head always, the others as far as the size reaches
+    GRID_STRIDE(t, H * 4) {
+        const uint64_t h = t >> 2;
+        const uint32_t q = (uint32_t)t & 3u;
         const uint32_t lo = krf[h];
         const uint32_t c = min(krf[h + 1] - lo, MAP_BUCKET);
-        uint32_t w[MAP_SLOT];
-        w[0] = c;
+        if (q == 0 || c > 4 * q - 1) {
+            // slot word 0: the size; word 1 + p: position p, in descending order
+            uint32_t w[4];
 #pragma unroll
-        for (uint32_t p = 0; p < MAP_BUCKET; ++p) w[1 + p] = p < c ? vals[lo + c - 1 - p] : 0u;
-#pragma unroll
-        for (uint32_t p = 1 + MAP_BUCKET; p < MAP_SLOT; ++p) w[p] = 0u;
-        uint4* d = reinterpret_cast<uint4*>(bk + h * stride + off);
-        d[0] = make_uint4(w[0], w[1], w[2], w[3]);
-        if (c > 3) d[1] = make_uint4(w[4], w[5], w[6], w[7]);
-        if (c > 7) d[2] = make_uint4(w[8], w[9], w[10], w[11]);
-        if (c > 11) d[3] = make_uint4(w[12], w[13], w[14], w[15]);
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t word = 4 * q + k;
+                w[k] = word == 0 ? c : (word - 1 < c ? vals[lo + c - word] : 0u);
+            }
+            *reinterpret_cast<uint4*>(bk + h * MAP_SLOT + 4 * q) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
     }
 }
 

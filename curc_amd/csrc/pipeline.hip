@@ -1582,8 +1582,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         hipLaunchKernelGGL(k_gap_fill, dim3(grid_for(count + 1)), dim3(256), 0, st,
                                            (const uint32_t*)k0, count, (uint64_t)H + 1, krf);
                         if (inline_bk) {
-                            hipLaunchKernelGGL(k_map_slots, dim3(grid_for(H)), dim3(256), 0, st, (const uint32_t*)v0,
-                                               (const uint32_t*)krf, (uint64_t)H, bk);
+                            hipLaunchKernelGGL(k_map_slots, dim3(grid_for((uint64_t)H * 4, 1u << 20)), dim3(256), 0,
+                                               st, (const uint32_t*)v0, (const uint32_t*)krf, (uint64_t)H, bk);
                         } else {
                             hipLaunchKernelGGL(k_map_caps, dim3(grid_for((uint64_t)H + 2)), dim3(256), 0, st,
                                                (const uint32_t*)krf, (uint64_t)H, kr);
