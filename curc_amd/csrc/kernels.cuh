@@ -1743,6 +1743,20 @@ __global__ void __launch_bounds__(256) k_map_fast(MapArgs a) {
     wave_add_u64(&a.work[5], n_open_cand);
 }
 
+// SeenCache over 32-bit positions (pseudogenomes shorter than 2^32 bases).
+struct SeenCache32 {
+    uint32_t s0 = ~0u, s1 = ~0u, s2 = ~0u, s3 = ~0u;
+    __device__ __forceinline__ bool check_insert(uint64_t mp64) {
+        const uint32_t mp = (uint32_t)mp64;
+        if (mp == s0 || mp == s1 || mp == s2 || mp == s3) return true;
+        s3 = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = mp;
+        return false;
+    }
+};
+
 // Both strand passes of a read in one loop over its probes, on the
 // two-strand table (MAP_SLOT2): each probe fetches one line that holds the
 // bucket's forward and reverse-complement slots, where the separate passes
@@ -1848,7 +1862,7 @@ __global__ void __launch_bounds__(256, 5) k_map_both(MapArgs a) {
         uint64_t wF = w, wR = w;
         bool doneR = false;
         uint32_t pF = 0, cF = 0, pR = 0, cR = 0;  // work of each pass
-        SeenCache seenF, seenR;
+        SeenCache32 seenF, seenR;  // positions < pg_len < 2^32
         for (uint32_t t0 = 0; t0 < np; t0 += G) {
             uint4 hf[G], hr[G];
             uint32_t hh[G];

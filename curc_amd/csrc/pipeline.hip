@@ -1490,7 +1490,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const uint32_t H1 = index_count ? hash_size(index_count) : 0;
         const bool both = c->tune.map_both && !phased && L >= MAP_K && nm > 0 && W <= 8 && !c->index_csr &&
                           index_count > 0 && index_count <= (1ull << 30) && H1 <= MAP_SLOT_MAX_H &&
-                          index_count * 8 >= H1;
+                          index_count * 8 >= H1 && ref_len < (1ull << 32);  // 32-bit positions
         if (both) {
             const uint64_t count = index_count;
             const uint32_t H = H1;
