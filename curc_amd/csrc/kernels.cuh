@@ -223,16 +223,17 @@ struct EmitGatherKey {
 // (hash bits 32..63 under fmask): a suffix whose k-mer bit is clear has no
 // prefix with that k-mer, hence none equal to it, at any offset.
 // Minimizer-blocked prefix k-mer filter.  A k-mer's two filter bits
-// (ov_fbits) lie in the 1024-bit block chosen by its
+// (ov_fbits) lie in the 512-bit block chosen by its
 // minimizer, the least ov_mhash over its OVM-base substrings.  Neighbouring
 // k-mers of a read usually share their minimizer, so the k-mers of one
-// suffix at all offsets touch a few 128-byte blocks.  Exact like the plain
+// suffix at all offsets touch a few 64-byte blocks.  Exact like the plain
 // filter: a clear bit proves that no prefix has the k-mer.
 constexpr uint32_t OVM = 16;
-constexpr uint32_t OVB_WORDS = 32;  // 1024-bit blocks
+constexpr uint32_t OVB_WORDS = 16;  // 512-bit blocks
+constexpr uint32_t OVB_MASK = OVB_WORDS * 32 - 1;
 // order of the m-mers for minimizers: a bijection of 32-bit values
 __device__ __forceinline__ uint32_t ov_mhash(uint32_t v) { return v * 0x9E3779B1u; }
-// the k-mer's two bit positions in its block: bits 0..9 and 10..19
+// the k-mer's two bit positions in its block: from bits 0..9 and 10..19
 __device__ __forceinline__ uint32_t ov_fbits(uint64_t K) { return (uint32_t)((K * 0xC2B2AE3D27D4EB4Full) >> 44); }
 __device__ __forceinline__ uint32_t ov_block(uint32_t mn, uint32_t bmask) {
     return (uint32_t)(((uint64_t)(mn ^ 0x5bd1e995u) * 0xD6E8FEB86659FD93ull) >> 40) & bmask;
@@ -260,7 +261,7 @@ __global__ void k_prefix_hash(const uint64_t* __restrict__ R, const uint32_t* __
         }
         if (mfilt) {
             uint32_t* fb = mfilt + (uint64_t)ov_block(ov_minimizer(K, k), bmask) * OVB_WORDS;
-            const uint32_t f = ov_fbits(K), b1 = f & 1023u, b2 = f >> 10;
+            const uint32_t f = ov_fbits(K), b1 = f & OVB_MASK, b2 = (f >> 10) & OVB_MASK;
             atomicOr(&fb[b1 >> 5], 1u << (b1 & 31u));
             atomicOr(&fb[b2 >> 5], 1u << (b2 & 31u));
         }
@@ -1570,7 +1571,7 @@ __global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict_
                                                      uint32_t k, uint32_t kip, uint32_t max_off,
                                                      const uint32_t* __restrict__ mfilt, uint32_t bmask,
                                                      uint64_t* __restrict__ vb, uint64_t nw) {
-    __shared__ uint32_t blocks[256 * (OVB_WORDS + 1)];  // stride 33: spreads banks
+    __shared__ uint32_t blocks[256 * (OVB_WORDS + 1)];  // odd stride: spreads banks
     uint32_t* mine = blocks + threadIdx.x * (OVB_WORDS + 1);
     uint32_t held = 0xFFFFFFFFu;  // block in the LDS slot
     auto verdict = [&](uint64_t K, uint32_t mv, bool valid, uint64_t i, uint32_t off) {
@@ -1587,7 +1588,7 @@ __global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict_
             }
             held = blk;
         }
-        const uint32_t f = ov_fbits(K), b1 = f & 1023u, b2 = f >> 10;
+        const uint32_t f = ov_fbits(K), b1 = f & OVB_MASK, b2 = (f >> 10) & OVB_MASK;
         const bool pass = valid && ((mine[b1 >> 5] >> (b1 & 31u)) & (mine[b2 >> 5] >> (b2 & 31u)) & 1u) != 0;
         const uint64_t bal = __ballot(pass);
         // a wave past the list's end (lane 0 beyond it) has no word to write
