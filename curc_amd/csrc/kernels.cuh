@@ -38,54 +38,72 @@ __global__ void k_iota(uint32_t* __restrict__ p, uint64_t n) {
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 // ------------------------------------------------------------------ pack ---
-// One block packs 64 reads: the 64*L ASCII bytes are staged in LDS with
-// coalesced dword loads, each thread then packs one 32-base word, and the
-// 64*W packed words leave through LDS again as one contiguous store.
-constexpr uint32_t PACK_READS = 64;
-
-template <bool ALIGNED4>
-__global__ void __launch_bounds__(256) k_pack(const uint8_t* __restrict__ ascii, uint64_t n, uint32_t L,
-                                              uint32_t W, uint64_t* __restrict__ packed,
-                                              uint8_t* __restrict__ isN) {
-    extern __shared__ uint8_t smem[];
-    uint8_t* chars = smem;                                            // 64*L bytes, padded to 8
-    uint64_t* words = (uint64_t*)(smem + ((PACK_READS * L + 15) & ~15u)); // 64*W words
-    __shared__ uint32_t nflag[PACK_READS];
-    const uint64_t r0 = (uint64_t)blockIdx.x * PACK_READS;
-    const uint32_t nr = (uint32_t)min<uint64_t>(PACK_READS, n - r0);
-    const uint32_t nbytes = nr * L;
-    const uint8_t* src = ascii + r0 * L;
-    if (threadIdx.x < PACK_READS) nflag[threadIdx.x] = 0;
-    if (ALIGNED4) {
-        const uint32_t nw = nbytes >> 2;
-        const uint32_t* s4 = (const uint32_t*)src;
-        uint32_t* d4 = (uint32_t*)chars;
-        for (uint32_t i = threadIdx.x; i < nw; i += 256) d4[i] = s4[i];
-        for (uint32_t i = (nw << 2) + threadIdx.x; i < nbytes; i += 256) chars[i] = src[i];
-    } else {
-        for (uint32_t i = threadIdx.x; i < nbytes; i += 256) chars[i] = src[i];
+// The 32 bytes from c (nb <= 32 of them wanted) as eight little-endian
+// dwords, read as aligned dwords; a dword reaching outside [begin, end) is
+// read byte by byte.  Bytes past nb are left as loaded (or zero).
+__device__ __forceinline__ void load_bytes32(const uint8_t* c, uint32_t nb, const uint8_t* begin, const uint8_t* end,
+                                             uint32_t (&a)[8]) {
+    const uintptr_t p = (uintptr_t)c;
+    const uint32_t sh = (uint32_t)(p & 3u);
+    const uint8_t* base = c - sh;
+    uint32_t d[9];
+#pragma unroll
+    for (uint32_t k = 0; k < 9; ++k) {
+        const uint8_t* q = base + 4 * k;
+        d[k] = 0u;
+        if (4 * k < sh + nb) {
+            if (q >= begin && q + 4 <= end) {
+                d[k] = *reinterpret_cast<const uint32_t*>(q);
+            } else {
+#pragma unroll
+                for (uint32_t b = 0; b < 4; ++b)
+                    if (q + b >= begin && q + b < end) d[k] |= (uint32_t)q[b] << (8 * b);
+            }
+        }
     }
-    __syncthreads();
-    const uint32_t tasks = nr * W;
-    for (uint32_t t = threadIdx.x; t < tasks; t += 256) {
-        uint32_t r = t / W, w = t - r * W;
-        uint32_t b0 = w * 32;
-        uint32_t nb = min(32u, L - b0);
-        const uint8_t* c = chars + r * L + b0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) a[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], sh);
+}
+
+// ASCII reads to 2-bit words, one thread per packed word (enc_base & 3 per
+// base, first base in the top bits).  A read holding an uppercase 'N' gets
+// isN set (isN cleared beforehand; the flag byte is set with an atomic OR
+// on its dword, which few reads need).
+__global__ void __launch_bounds__(256) k_pack_w(const uint8_t* __restrict__ ascii, uint64_t n, uint32_t L, uint32_t W,
+                                                uint64_t* __restrict__ packed, uint8_t* __restrict__ isN) {
+    const uint8_t* end = ascii + n * L;
+    GRID_STRIDE(t, n * W) {
+        const uint64_t r = t / W;
+        const uint32_t w = (uint32_t)(t - r * W);
+        const uint32_t b0 = 32u * w;
+        const uint32_t nb = min(32u, L - b0);
+        uint32_t a[8];
+        load_bytes32(ascii + r * L + b0, nb, ascii, end, a);
         uint64_t v = 0;
         uint32_t hasN = 0;
-        for (uint32_t j = 0; j < nb; ++j) {
-            uint8_t ch = c[j];
-            v |= (uint64_t)(enc_base(ch) & 3u) << (62u - 2u * j);
-            hasN |= (ch == 'N');
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            // 'N' among this dword's wanted bytes (zero-byte test; others forced non-zero)
+            const uint32_t valid = 4 * k < nb ? min(4u, nb - 4 * k) : 0u;
+            const uint32_t keep = valid >= 4 ? 0xFFFFFFFFu : ((1u << (8 * valid)) - 1u);
+            const uint32_t x = (a[k] ^ 0x4E4E4E4Eu) | ~keep;
+            hasN |= (x - 0x01010101u) & ~x & 0x80808080u;
+#pragma unroll
+            for (uint32_t b = 0; b < 4; ++b) {
+                const uint32_t j = 4 * k + b;
+                if (j < nb) {
+                    const uint32_t u = (a[k] >> (8 * b)) & 0xDFu;
+                    const uint32_t code = (u == 'C' ? 1u : 0u) | (u == 'G' ? 2u : 0u) | (u == 'T' ? 3u : 0u);
+                    v |= (uint64_t)code << (62u - 2u * j);
+                }
+            }
         }
-        words[t] = v;
-        if (hasN) atomicOr(&nflag[r], 1u);
+        packed[t] = v;
+        if (isN && hasN) {
+            const uintptr_t fp = (uintptr_t)(isN + r);
+            atomicOr(reinterpret_cast<uint32_t*>(fp & ~(uintptr_t)3), 1u << (8 * (fp & 3)));
+        }
     }
-    __syncthreads();
-    uint64_t* dst = packed + r0 * W;
-    for (uint32_t t = threadIdx.x; t < tasks; t += 256) dst[t] = words[t];
-    if (isN && threadIdx.x < nr) isN[r0 + threadIdx.x] = (uint8_t)nflag[threadIdx.x];
 }
 
 struct PredIsN {

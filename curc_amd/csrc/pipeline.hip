@@ -1115,14 +1115,11 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     {
         Region r(c, "pack");
         fill<uint64_t>(c, packed + N * W, 1, 0ull);
-        size_t lds = ((PACK_READS * L + 15) & ~15u) + PACK_READS * W * 8;
-        uint32_t grid = (uint32_t)((N + PACK_READS - 1) / PACK_READS);
-        if (grid) {
-            if (((uintptr_t)d_ascii & 3) == 0)
-                hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), lds, st, d_ascii, N, L, W, packed, isN);
-            else
-                hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), lds, st, d_ascii, N, L, W, packed, isN);
-        }
+        // one thread per packed word; isN is set only for reads with an 'N'
+        fill<uint8_t>(c, isN, N + 1, (uint8_t)0);
+        if (N)
+            hipLaunchKernelGGL(k_pack_w, dim3(grid_for(N * W, 1u << 20)), dim3(256), 0, st, d_ascii, N, L, W, packed,
+                               isN);
         set_cnt(c, cnt + C_N, (uint32_t)N);
         compact(cnt + C_N, PredIsN{isN, false}, EmitIndex{nonN}, cnt + C_NONN, S, S_PRIM, st, N);
         compact(cnt + C_N, PredIsN{isN, true}, EmitIndex{Nids}, cnt + C_NN, S, S_PRIM, st, N);
@@ -1962,15 +1959,10 @@ int curc_pack(curc_ctx* c, const char* d_ascii, uint64_t n, uint32_t L, uint64_t
     try {
         CURC_HIP(hipSetDevice(c->device));
         const uint32_t W = (L + 31) / 32;
-        size_t lds = ((PACK_READS * L + 15) & ~15u) + PACK_READS * W * 8;
-        uint32_t grid = (uint32_t)((n + PACK_READS - 1) / PACK_READS);
-        if (grid) {
-            if (((uintptr_t)d_ascii & 3) == 0)
-                hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), lds, c->st, (const uint8_t*)d_ascii, n, L,
-                                   W, d_packed, d_isN);
-            else
-                hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), lds, c->st, (const uint8_t*)d_ascii, n,
-                                   L, W, d_packed, d_isN);
+        if (n) {
+            if (d_isN) fill<uint8_t>(c, d_isN, n, (uint8_t)0);
+            hipLaunchKernelGGL(k_pack_w, dim3(grid_for(n * W, 1u << 20)), dim3(256), 0, c->st,
+                               (const uint8_t*)d_ascii, n, L, W, d_packed, d_isN);
             CURC_HIP(hipGetLastError());
         }
         CURC_HIP(hipStreamSynchronize(c->st));
