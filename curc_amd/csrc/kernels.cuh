@@ -488,7 +488,8 @@ __device__ __forceinline__ uint32_t ov_candidate(const ClaimState& cs, const uin
     const uint32_t n1 = min(32u, len);
     const uint32_t s = cs.sl[i];
     const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
-    const uint64_t sx0 = xw_soa(sw, ns, i, off, n1);
+    // the suffix's words: from the word-major copy when there is one, else its packed row
+    const uint64_t sx0 = sw ? xw_soa(sw, ns, i, off, n1) : xw(rs, off, n1);
     uint32_t lo, e;
     if (slots) {
         const PrefixSlot sl = slots[h];
@@ -611,7 +612,9 @@ static_assert(OVQ_CAP >= OVQ_TILE + 512 && (OVQ_CAP & (OVQ_CAP - 1)) == 0, "over
 
 // VERD: the filter verdicts were computed for all offsets beforehand
 // (k_ov_verdicts; bit i of vb[(off - 1) * nw + i / 64]), so the filter pass
-// reads one bit per suffix and the candidate step hashes the k-mer itself.
+// reads one bit per suffix and the candidate step hashes the k-mer itself,
+// from the suffix's packed row (sw is null: only the few suffixes that pass
+// read words, and they read that row anyway).
 template <bool VERD>
 __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_t* __restrict__ kr,
                                                     const PrefixSlot* __restrict__ slots,
@@ -638,7 +641,7 @@ __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_
         uint32_t ch = INV, i = 0;
         if (valid) {
             i = qi[q & (OVQ_CAP - 1)];
-            const uint32_t h = VERD ? (uint32_t)(murmur64(xw_soa(sw, ns, i, kip + off, k)) & mask)
+            const uint32_t h = VERD ? (uint32_t)(murmur64(xw(cs.R + (uint64_t)cs.sl[i] * cs.W, kip + off, k)) & mask)
                                     : qh[q & (OVQ_CAP - 1)];
             uint32_t g;
             ch = ov_candidate(cs, kr, slots, sw, ns, i, h, off, &g);
@@ -1572,8 +1575,9 @@ __device__ __forceinline__ uint64_t xw_pick(const uint64_t (&w)[WM], uint32_t of
 // Overlap filter verdicts for every offset in one pass (minimizer-blocked
 // filter): bit i of vb[(off - 1) * nw + i / 64] is set when the k-mer of
 // suffix i at kip + off has both of its bits set, for off in [1, max_off).
-// Thread i holds its read's words (from the word-major copy sw), and every
-// lane does the same work per offset, so lanes do not diverge.  For k = 32
+// Thread i holds its read's words (its row of the packed reads R, through
+// the suffix list sl), and every lane does the same work per offset, so
+// lanes do not diverge.  For k = 32
 // (K32) offsets go in chunks of NM = 17, the m-mers per k-mer: with the 64
 // bases from the chunk's first k-mer in two words, each k-mer and m-mer of
 // the chunk is a fixed shift of them, and the least m-mer hash of each
@@ -1585,10 +1589,10 @@ __device__ __forceinline__ uint64_t xw_pick(const uint64_t (&w)[WM], uint32_t of
 // resident threads touch too many blocks for the caches to keep it.  Each
 // 64-suffix word is written by its wave with one ballot per offset.
 template <uint32_t WM, bool K32>
-__global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict__ sw, uint32_t ns, uint32_t W,
-                                                     uint32_t k, uint32_t kip, uint32_t max_off,
-                                                     const uint32_t* __restrict__ mfilt, uint32_t bmask,
-                                                     uint64_t* __restrict__ vb, uint64_t nw) {
+__global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict__ R, const uint32_t* __restrict__ sl,
+                                                     uint32_t ns, uint32_t W, uint32_t k, uint32_t kip,
+                                                     uint32_t max_off, const uint32_t* __restrict__ mfilt,
+                                                     uint32_t bmask, uint64_t* __restrict__ vb, uint64_t nw) {
     __shared__ uint32_t blocks[256 * (OVB_WORDS + 1)];  // odd stride: spreads banks
     uint32_t* mine = blocks + threadIdx.x * (OVB_WORDS + 1);
     uint32_t held = 0xFFFFFFFFu;  // block in the LDS slot
@@ -1616,8 +1620,9 @@ __global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict_
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < ns;
         uint64_t r[WM];
+        const uint64_t* rd = R + (uint64_t)(valid ? sl[i] : 0u) * W;
 #pragma unroll
-        for (uint32_t w = 0; w < WM; ++w) r[w] = (valid && w < W) ? sw[(uint64_t)w * ns + i] : 0ull;
+        for (uint32_t w = 0; w < WM; ++w) r[w] = (valid && w < W) ? rd[w] : 0ull;
         if constexpr (K32) {
             constexpr uint32_t NM = 32 - OVM + 1;
             uint32_t hc[NM];  // m-mer hashes of the current chunk (positions x0 .. x0 + NM - 1)

@@ -1294,10 +1294,15 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             const uint32_t g = grid_for(nsuf, c->tune.ov_grid);
             uint8_t* slinked = S.as<uint8_t>(S_SLINK, (uint64_t)nsuf + 1);
             fill<uint8_t>(c, slinked, (uint64_t)nsuf + 1, (uint8_t)0);
-            uint64_t* sw = S.as<uint64_t>(S_SUFW, (uint64_t)nsuf * W + 1);
-            if (nsuf)
-                hipLaunchKernelGGL(k_gather_soa, dim3(grid_for((uint64_t)nsuf * W)), dim3(256), 0, st,
-                                   (const uint64_t*)packed, (const uint32_t*)sl[0], (uint64_t)nsuf, W, sw);
+            // word-major copy of the suffixes' words for the probe forms that
+            // read a word of every suffix at every offset (0 and 1)
+            uint64_t* sw = nullptr;
+            if (!verd) {
+                sw = S.as<uint64_t>(S_SUFW, (uint64_t)nsuf * W + 1);
+                if (nsuf)
+                    hipLaunchKernelGGL(k_gather_soa, dim3(grid_for((uint64_t)nsuf * W)), dim3(256), 0, st,
+                                       (const uint64_t*)packed, (const uint32_t*)sl[0], (uint64_t)nsuf, W, sw);
+            }
             // filter verdicts of every offset, one 64-suffix word per offset
             const uint64_t vnw = ((uint64_t)nsuf + 63) / 64;
             uint64_t* vb = nullptr;
@@ -1307,11 +1312,13 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
 #define CURC_OV_VERDICTS(WM)                                                                           \
     do {                                                                                               \
         if (P.k == 32)                                                                                 \
-            hipLaunchKernelGGL((k_ov_verdicts<WM, true>), vg, dim3(256), 0, st, (const uint64_t*)sw, nsuf, W, \
-                               P.k, P.kip, P.max_off, (const uint32_t*)pfilt, bmask, vb, vnw);         \
+            hipLaunchKernelGGL((k_ov_verdicts<WM, true>), vg, dim3(256), 0, st, (const uint64_t*)packed,      \
+                               (const uint32_t*)sl[0], nsuf, W, P.k, P.kip, P.max_off, (const uint32_t*)pfilt, \
+                               bmask, vb, vnw);                                                        \
         else                                                                                           \
-            hipLaunchKernelGGL((k_ov_verdicts<WM, false>), vg, dim3(256), 0, st, (const uint64_t*)sw, nsuf, W, \
-                               P.k, P.kip, P.max_off, (const uint32_t*)pfilt, bmask, vb, vnw);         \
+            hipLaunchKernelGGL((k_ov_verdicts<WM, false>), vg, dim3(256), 0, st, (const uint64_t*)packed,     \
+                               (const uint32_t*)sl[0], nsuf, W, P.k, P.kip, P.max_off, (const uint32_t*)pfilt, \
+                               bmask, vb, vnw);                                                        \
     } while (0)
                 if (W <= 3)
                     CURC_OV_VERDICTS(3);
