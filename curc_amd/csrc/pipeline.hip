@@ -10,6 +10,7 @@
 #include <chrono>
 #include <numeric>
 #include <cstdlib>
+#include <exception>
 #include <thread>
 #include <cstring>
 #include <string>
@@ -131,6 +132,10 @@ struct curc_ctx {
     hipStream_t cst = nullptr;
     hipEvent_t cev[2] = {nullptr, nullptr};
     bool side_copies = true;
+    // context (own stream, arena and counters) that runs the N-read stage
+    // beside the LQ stage of a single block; created on first use
+    curc_ctx* helper = nullptr;
+    hipEvent_t nev = nullptr;  // end of mapping on st, for the helper to wait on
     Scratch S;
     std::string err;
     bool timing = false;
@@ -1083,6 +1088,98 @@ void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOu
     CURC_HIP(hipStreamSynchronize(st));
 }
 
+// a12: N-read LQ pseudogenome (compress.cu:1724-1728, 1176-1347).  Same
+// exact matching over the N-reads in the 3-bit ACGNT encoding; ids are N
+// indices, and the visit order is the sorted unmapped list.  Runs on context
+// x (its arena, counters and stream): the block's own, or a helper that then
+// first waits for `wait` (the end of mapping on the block's stream).  Nascii
+// and umN are the N-reads' ASCII and their unmapped list in device memory.
+struct NlqOut {
+    AsmOut a;
+    uint8_t* npg = nullptr;
+};
+NlqOut nlq_stage(curc_ctx* x, hipEvent_t wait, const uint8_t* Nascii, uint64_t nN, const uint32_t* umN, uint32_t n_nlq,
+                 uint32_t L, curc_block* B) {
+    // ---- a12: N-read LQ pseudogenome (compress.cu:1724-1728, 1176-1347).
+    // Same exact matching over the N-reads in the 3-bit ACGNT encoding; ids
+    // are N indices, and the visit order is the sorted unmapped list.
+    NlqOut out;
+    Scratch& S = x->S;
+    hipStream_t st = x->st;
+    if (wait) CURC_HIP(hipStreamWaitEvent(st, wait, 0));
+    uint32_t* cnt = S.as<uint32_t>(S_CNT, C_COUNT);
+    if (wait) fill<uint32_t>(x, cnt, C_COUNT, 0u);  // a helper's own counters
+    {
+        Region r(x, "nlq");
+        const uint32_t W3 = (L + 20) / 21;
+        uint64_t* p3 = S.as<uint64_t>(S_N3, (uint64_t)nN * W3 + 1);
+        hipLaunchKernelGGL(k_pack3, dim3(grid_for((uint64_t)nN * W3)), dim3(256), 0, st, (const uint8_t*)Nascii,
+                           (uint64_t)nN, L, W3, p3);
+        fill<uint64_t>(x, p3 + (uint64_t)nN * W3, 1, 0ull);
+        uint32_t* nnext = S.as<uint32_t>(S_NNEXT, nN);
+        uint32_t* nprev = S.as<uint32_t>(S_NPREV, nN);
+        uint16_t* noff = S.as<uint16_t>(S_NOFF, nN);
+        fill<uint32_t>(x, nnext, nN, INV);
+        fill<uint32_t>(x, nprev, nN, INV);
+        fill<uint16_t>(x, noff, nN, INVOFF);
+        // compress.cu:1179-1193: sort by content (ties by N index), link equal reads
+        uint64_t* k0 = S.as<uint64_t>(S_K0, (uint64_t)n_nlq + 1);
+        uint64_t* k1 = S.as<uint64_t>(S_K1, (uint64_t)n_nlq + 1);
+        uint32_t* v0 = S.as<uint32_t>(S_V0, (uint64_t)n_nlq + 1);
+        uint32_t* v1 = S.as<uint32_t>(S_V1, (uint64_t)n_nlq + 1);
+        CURC_HIP(hipMemcpyAsync(v0, umN, (uint64_t)n_nlq * 4, hipMemcpyDeviceToDevice, st));
+        for (int w = (int)W3 - 1; w >= 0; --w) {
+            uint32_t nd = (w == (int)W3 - 1) ? (L - 21 * (W3 - 1)) : 21;
+            uint32_t begin = ((63 - 3 * nd) / 8) * 8;
+            hipLaunchKernelGGL(k_gather_word, dim3(grid_for(n_nlq)), dim3(256), 0, st, (const uint64_t*)p3,
+                               (const uint32_t*)v0, (uint64_t)n_nlq, W3, (uint32_t)w, k0);
+            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n_nlq, begin, 64, S, S_PRIM, st);
+        }
+        uint32_t* nsorted = S.as<uint32_t>(S_NSORT, n_nlq);
+        CURC_HIP(hipMemcpyAsync(nsorted, v0, (uint64_t)n_nlq * 4, hipMemcpyDeviceToDevice, st));
+        if (n_nlq > 1)
+            hipLaunchKernelGGL(k_dedup_link, dim3(grid_for(n_nlq)), dim3(256), 0, st, (const uint64_t*)p3,
+                               (const uint32_t*)nsorted, (uint64_t)n_nlq, W3, nnext, nprev, noff);
+        uint32_t* pre = S.as<uint32_t>(S_PRE, n_nlq);
+        uint32_t* suf = S.as<uint32_t>(S_SUF, n_nlq);
+        set_cnt(x, cnt + C_TMP, n_nlq);
+        compact(cnt + C_TMP, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE, S, S_PRIM, st,
+                n_nlq);
+        compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st,
+                n_nlq);
+        full_match<Enc3>(x, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff);
+        out.a = assemble(x, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt, S_NLQIDS);
+        const AsmOut& a = out.a;
+        uint8_t* npg = S.as<uint8_t>(S_NPG, a.len + 16);
+        out.npg = npg;
+        if (a.n)
+            hipLaunchKernelGGL(k_pg_write_ascii, dim3(grid_for(a.n)), dim3(256), 0, st, (const uint8_t*)Nascii, L,
+                               (const uint32_t*)a.rid, (const uint64_t*)a.pos, (const uint32_t*)a.nb,
+                               (uint64_t)a.n, npg);
+        B->nlq_text.resize(a.len);
+        if (a.len) CURC_HIP(hipMemcpyAsync(&B->nlq_text[0], npg, a.len, hipMemcpyDeviceToHost, st));
+        records_to_host(x, a, nullptr, 1, B->nlq_rec);  // is_contain_N
+    }
+    CURC_HIP(hipStreamSynchronize(st));
+    return out;
+
+}
+
+// The helper context of c (settings copied from c), or null if it cannot be made.
+curc_ctx* helper_ctx(curc_ctx* c) {
+    if (!c->helper) {
+        c->helper = curc_ctx_create(c->device);
+        if (!c->helper) return nullptr;
+        c->helper->side_copies = false;
+    }
+    curc_ctx* h = c->helper;
+    h->timing = c->timing;
+    h->index_csr = c->index_csr;
+    h->tune = c->tune;
+    h->graphs.enabled = c->graphs.enabled;
+    return h;
+}
+
 void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop, uint32_t flags,
                curc_block* B) {
     curc_params P;
@@ -1678,6 +1775,38 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     }
     if (stop <= CURC_STAGE_MAP) return;
 
+    // ---- a12 beside a11: in a single-block run the N-read stage runs on a
+    // helper context (own stream and arena) in a thread of its own, after
+    // the end of mapping, while this thread runs the LQ stage
+    const uint32_t n_nlq = (uint32_t)B->unmapped_N.size();
+    const uint32_t* umN_dev = S.as<uint32_t>(S_UNMAPPED_N, (uint64_t)nN + 1);
+    NlqOut nlq;
+    bool nlq_side = false;
+    std::exception_ptr nlq_err;
+    curc_ctx* hx = nullptr;
+    struct Joiner {
+        std::thread t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } nlq_thread;
+    if (n_nlq && stop > CURC_STAGE_LQ && c->side_copies && c->nev && (hx = helper_ctx(c)) != nullptr) {
+        CURC_HIP(hipEventRecord(c->nev, st));
+        nlq_side = true;
+        const uint32_t items = c->tune.radix_items;
+        nlq_thread.t = std::thread([&, items] {
+            try {
+                CURC_HIP(hipSetDevice(hx->device));
+                radix_items() = items;
+                nlq = nlq_stage(hx, c->nev, Nascii, nN, umN_dev, n_nlq, L, B);
+            } catch (...) {
+                nlq_err = std::current_exception();
+                // no copy into the block's arrays may outlive an error path
+                hipStreamSynchronize(hx->st);
+            }
+        });
+    }
+
     // ---- a11: LQ pseudogenome (compress.cu:1649-1722)
     const uint32_t n_lq = (uint32_t)B->unmapped.size();
     if (n_lq) {
@@ -1711,64 +1840,26 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->lq_pg2, lpg, (a.len + 31) / 32 + 1);
         CURC_HIP(hipStreamSynchronize(st));
     }
+    if (nlq_side) {
+        nlq_thread.t.join();
+        if (nlq_err) std::rethrow_exception(nlq_err);
+        // the helper's stage times go to this context
+        CURC_HIP(hipStreamSynchronize(hx->st));
+        hx->harvest();
+        for (const StageStat& sst : hx->stats) {
+            const int id = c->stat_id(sst.name.c_str());
+            c->stats[id].ms += sst.ms;
+            c->stats[id].launches += sst.launches;
+            c->stats[id].alg_bytes += sst.alg_bytes;
+            c->stats[id].items += sst.items;
+        }
+        hx->stats.clear();
+    }
     if (stop <= CURC_STAGE_LQ) return;
 
-    // ---- a12: N-read LQ pseudogenome (compress.cu:1724-1728, 1176-1347).
-    // Same exact matching over the N-reads in the 3-bit ACGNT encoding; ids
-    // are N indices, and the visit order is the sorted unmapped list.
-    const uint32_t n_nlq = (uint32_t)B->unmapped_N.size();
-    if (n_nlq) {
-        Region r(c, "nlq");
-        const uint32_t W3 = (L + 20) / 21;
-        uint64_t* p3 = S.as<uint64_t>(S_N3, (uint64_t)nN * W3 + 1);
-        hipLaunchKernelGGL(k_pack3, dim3(grid_for((uint64_t)nN * W3)), dim3(256), 0, st, (const uint8_t*)Nascii,
-                           (uint64_t)nN, L, W3, p3);
-        fill<uint64_t>(c, p3 + (uint64_t)nN * W3, 1, 0ull);
-        uint32_t* nnext = S.as<uint32_t>(S_NNEXT, nN);
-        uint32_t* nprev = S.as<uint32_t>(S_NPREV, nN);
-        uint16_t* noff = S.as<uint16_t>(S_NOFF, nN);
-        fill<uint32_t>(c, nnext, nN, INV);
-        fill<uint32_t>(c, nprev, nN, INV);
-        fill<uint16_t>(c, noff, nN, INVOFF);
-        // compress.cu:1179-1193: sort by content (ties by N index), link equal reads
-        uint64_t* k0 = S.as<uint64_t>(S_K0, (uint64_t)n_nlq + 1);
-        uint64_t* k1 = S.as<uint64_t>(S_K1, (uint64_t)n_nlq + 1);
-        uint32_t* v0 = S.as<uint32_t>(S_V0, (uint64_t)n_nlq + 1);
-        uint32_t* v1 = S.as<uint32_t>(S_V1, (uint64_t)n_nlq + 1);
-        CURC_HIP(hipMemcpyAsync(v0, S.as<uint32_t>(S_UNMAPPED_N, n_nlq), (uint64_t)n_nlq * 4,
-                                hipMemcpyDeviceToDevice, st));
-        for (int w = (int)W3 - 1; w >= 0; --w) {
-            uint32_t nd = (w == (int)W3 - 1) ? (L - 21 * (W3 - 1)) : 21;
-            uint32_t begin = ((63 - 3 * nd) / 8) * 8;
-            hipLaunchKernelGGL(k_gather_word, dim3(grid_for(n_nlq)), dim3(256), 0, st, (const uint64_t*)p3,
-                               (const uint32_t*)v0, (uint64_t)n_nlq, W3, (uint32_t)w, k0);
-            radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n_nlq, begin, 64, S, S_PRIM, st);
-        }
-        uint32_t* nsorted = S.as<uint32_t>(S_NSORT, n_nlq);
-        CURC_HIP(hipMemcpyAsync(nsorted, v0, (uint64_t)n_nlq * 4, hipMemcpyDeviceToDevice, st));
-        if (n_nlq > 1)
-            hipLaunchKernelGGL(k_dedup_link, dim3(grid_for(n_nlq)), dim3(256), 0, st, (const uint64_t*)p3,
-                               (const uint32_t*)nsorted, (uint64_t)n_nlq, W3, nnext, nprev, noff);
-        uint32_t* pre = S.as<uint32_t>(S_PRE, n_nlq);
-        uint32_t* suf = S.as<uint32_t>(S_SUF, n_nlq);
-        set_cnt(c, cnt + C_TMP, n_nlq);
-        compact(cnt + C_TMP, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE, S, S_PRIM, st,
-                n_nlq);
-        compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st,
-                n_nlq);
-        full_match<Enc3>(c, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff);
-        nlq_asm = assemble(c, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt, S_NLQIDS);
-        const AsmOut& a = nlq_asm;
-        uint8_t* npg = S.as<uint8_t>(S_NPG, a.len + 16);
-        if (a.n)
-            hipLaunchKernelGGL(k_pg_write_ascii, dim3(grid_for(a.n)), dim3(256), 0, st, (const uint8_t*)Nascii, L,
-                               (const uint32_t*)a.rid, (const uint64_t*)a.pos, (const uint32_t*)a.nb,
-                               (uint64_t)a.n, npg);
-        B->nlq_text.resize(a.len);
-        if (a.len) CURC_HIP(hipMemcpyAsync(&B->nlq_text[0], npg, a.len, hipMemcpyDeviceToHost, st));
-        records_to_host(c, a, nullptr, 1, B->nlq_rec);  // is_contain_N
-        CURC_HIP(hipStreamSynchronize(st));
-    }
+    // ---- a12 ran beside a11 (or runs now, on this context)
+    if (n_nlq && !nlq_side) nlq = nlq_stage(c, nullptr, Nascii, nN, umN_dev, n_nlq, L, B);
+    nlq_asm = nlq.a;
 
     // ---- a13: record sort + stream builders (compress.cu:1730-2044)
     if (flags & CURC_RUN_STREAMS) {
@@ -1789,7 +1880,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             rm_match(c, ix, hq2, hq_len, S.as<uint64_t>(S_PG_LQ, (B->lq_len + 31) / 32 + 1), nullptr, B->lq_len,
                      false, keepm, cnt, B->rm[1]);
         if (!B->nlq_text.empty())
-            rm_match(c, ix, hq2, hq_len, nullptr, S.as<uint8_t>(S_NPG, B->nlq_text.size() + 16),
+            rm_match(c, ix, hq2, hq_len, nullptr, nlq.npg ? nlq.npg : S.as<uint8_t>(S_NPG, B->nlq_text.size() + 16),
                      B->nlq_text.size(), false, keepm, cnt, B->rm[2]);
         rm_match(c, ix, hq2, hq_len, hq2, nullptr, hq_len, true, keepm, cnt, B->rm[0]);
     }
@@ -1842,7 +1933,9 @@ curc_ctx* curc_ctx_create(int device) {
     c->graphs.enabled = gr && std::string(gr) == "1";
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->cev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->cev[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->cev[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->nev, hipEventDisableTiming) != hipSuccess) {
+        if (c->nev) hipEventDestroy(c->nev);
         if (c->cev[1]) hipEventDestroy(c->cev[1]);
         if (c->cev[0]) hipEventDestroy(c->cev[0]);
         if (c->st) hipStreamDestroy(c->st);
@@ -1856,6 +1949,8 @@ void curc_ctx_destroy(curc_ctx* c) {
     if (!c) return;
     for (curc_ctx* w : c->workers) curc_ctx_destroy(w);
     c->workers.clear();
+    if (c->helper) curc_ctx_destroy(c->helper);
+    c->helper = nullptr;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->st);
     if (c->cst) hipStreamSynchronize(c->cst);
@@ -1868,6 +1963,7 @@ void curc_ctx_destroy(curc_ctx* c) {
     for (auto e : c->event_pool) hipEventDestroy(e);
     hipEventDestroy(c->cev[0]);
     hipEventDestroy(c->cev[1]);
+    hipEventDestroy(c->nev);
     if (c->cst) hipStreamDestroy(c->cst);
     hipStreamDestroy(c->st);
     delete c;
@@ -1883,6 +1979,7 @@ int curc_ctx_trim(curc_ctx* c) {
     c->graphs.clear();
     c->S.release();
     for (curc_ctx* w : c->workers) curc_ctx_trim(w);
+    if (c->helper) curc_ctx_trim(c->helper);
     HostPool::get().trim();
     return CURC_OK;
 }
