@@ -1323,7 +1323,7 @@ __global__ void k_map_slots2(const uint32_t* __restrict__ vals0, const uint32_t*
         const uint32_t* krf = sd ? krf1 : krf0;
         const uint32_t lo = krf[h];
         const uint32_t c = min(krf[h + 1] - lo, MAP_BUCKET);
-        if (q == 0 || c > 4 * q - 1) {
+        {
             // slot word 0: the size; word 1 + p: position p, in descending order
             uint32_t w[4];
 #pragma unroll
@@ -1864,16 +1864,27 @@ __global__ void __launch_bounds__(256, 5) k_map_both(MapArgs a) {
             const uint64_t right = (rword(o1 + 1) >> (62u - o2)) >> 2;
             return (left | right) >> ((32u - MAP_K) << 1);
         };
-        // mismatches of query q against the window at mp, stopping once above bound
+        // mismatches of query q against the window at mp, stopping once above
+        // bound.  The first 32 bases come from two words (one 128-byte line
+        // in 15 of 16 cases); the rest of the window is fetched only when
+        // they stay within the bound, which hash-bucket neighbours rarely do.
         auto count = [&](const uint64_t (&q)[WM], uint64_t mp, uint32_t bound) -> uint32_t {
             const uint64_t* g0 = a.pg + (mp >> 5);
             uint64_t g[WM + 1];
-#pragma unroll
-            for (uint32_t k = 0; k <= WM; ++k) g[k] = k <= W ? ld64(g0 + k) : 0ull;
+            g[0] = ld64(g0);
+            g[1] = ld64(g0 + 1);  // W >= 1
             const uint32_t sh = (uint32_t)(mp & 31u) << 1;
-            uint32_t c = 0;
+            uint32_t c;
+            {
+                const uint32_t n = min(32u, L);
+                const uint64_t win = sh ? (g[0] << sh) | (g[1] >> (64u - sh)) : g[0];
+                c = mm_words(win >> ((32u - n) << 1), q[0]);
+            }
+            if (c > bound || L <= 32) return c;
 #pragma unroll
-            for (uint32_t k = 0; k < WM; ++k) {
+            for (uint32_t k = 2; k <= WM; ++k) g[k] = k <= W ? ld64(g0 + k) : 0ull;
+#pragma unroll
+            for (uint32_t k = 1; k < WM; ++k) {
                 if (32 * k < L && c <= bound) {
                     const uint32_t n = min(32u, L - 32 * k);
                     const uint64_t win = sh ? (g[k] << sh) | (g[k + 1] >> (64u - sh)) : g[k];

@@ -221,19 +221,38 @@ __global__ void __launch_bounds__(256) k_radix_hist(const K* __restrict__ keys, 
 
 // Stable scatter: within a tile, element order is (round, wave, lane); each
 // wave ranks its lanes by digit with 8 ballots, waves are ordered through
-// per-wave digit counts in LDS.
+// per-wave digit counts in LDS.  The tile is first sorted by digit in LDS
+// (each key at its digit's tile-local start plus its rank), then written out
+// in that order, so consecutive threads write consecutive addresses of a
+// digit's run instead of one key per run and round.
 template <typename K, uint32_t ITEMS>
 __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                        uint64_t n, uint32_t shift,
                                                        const uint32_t* __restrict__ offs, uint32_t tiles) {
-    __shared__ uint32_t run[256];
+    constexpr uint32_t TILE = 256u * ITEMS;
+    __shared__ K sk[TILE];
+    __shared__ uint32_t sv[TILE];
+    __shared__ uint32_t run[256];    // global start of this tile's run of digit d
+    __shared__ uint32_t lbase[256];  // tile-local start of digit d
+    __shared__ uint32_t lrun[256];   // keys of digit d placed so far
     __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t scan4[4];
     const uint32_t t = threadIdx.x, w = t >> 6;
-    run[t] = offs[(uint64_t)t * tiles + blockIdx.x];
+    {
+        // offs is the exclusive scan of the (digit, tile) counts in digit-major
+        // order, so this tile's count of digit t is the step to the next entry
+        const uint64_t idx = (uint64_t)t * tiles + blockIdx.x;
+        const uint32_t o = offs[idx];
+        const uint32_t o1 = idx + 1 < 256ull * tiles ? offs[idx + 1] : (uint32_t)n;
+        run[t] = o;
+        uint32_t tot;
+        lbase[t] = block_excl_scan256(o1 - o, scan4, &tot);
+    }
+    lrun[t] = 0;
     wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * (256u * ITEMS);
+    const uint64_t base = (uint64_t)blockIdx.x * TILE;
     const uint64_t lt = lanemask_lt();
     for (uint32_t r = 0; r < ITEMS; ++r) {
         uint64_t i = base + r * 256 + t;
@@ -252,15 +271,23 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin
         if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = run[d] + rank;
+            uint32_t pos = lbase[d] + lrun[d] + rank;
             for (uint32_t q = 0; q < w; ++q) pos += wcnt[q][d];
-            kout[pos] = k;
-            vout[pos] = v;
+            sk[pos] = k;
+            sv[pos] = v;
         }
         __syncthreads();
-        run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        lrun[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
         wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
         __syncthreads();
+    }
+    const uint32_t tile_n = n - base < TILE ? (uint32_t)(n - base) : TILE;
+    for (uint32_t j = t; j < tile_n; j += 256) {
+        const K k = sk[j];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint32_t pos = run[d] + (j - lbase[d]);
+        kout[pos] = k;
+        vout[pos] = sv[j];
     }
 }
 
