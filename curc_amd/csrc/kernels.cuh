@@ -1800,7 +1800,7 @@ struct SeenCache32 {
 //  - each pass keeps its own SeenCache, and the reverse work is added to
 //    the counters only when the forward pass ends above MAP_TARGET.
 template <uint32_t WM>
-__global__ void __launch_bounds__(256, 5) k_map_both(MapArgs a) {
+__global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
     constexpr uint32_t G = 1;  // probes whose slot heads are requested together
     uint32_t n_act = 0, n_probe = 0, n_cand = 0, n_first = 0, n_open_probe = 0, n_open_cand = 0;
     const uint32_t L = a.L, W = a.W;
