@@ -219,12 +219,14 @@ __global__ void __launch_bounds__(256) k_radix_hist(const K* __restrict__ keys, 
     hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: within a tile, element order is (round, wave, lane); each
-// wave ranks its lanes by digit with 8 ballots, waves are ordered through
-// per-wave digit counts in LDS.  The tile is first sorted by digit in LDS
-// (each key at its digit's tile-local start plus its rank), then written out
-// in that order, so consecutive threads write consecutive addresses of a
-// digit's run instead of one key per run and round.
+// Stable scatter.  Wave w of a tile owns its contiguous quarter (64 x ITEMS
+// keys, 64 per round) and ranks them against wave-private digit counts in
+// LDS (8 ballots per round find each lane's peers), without block barriers
+// between rounds; one prefix over the four waves' counts then orders the
+// waves, so within a digit the order is (wave, round, lane), the input order.
+// The tile is sorted by digit in LDS (each key at its digit's tile-local
+// start plus its rank) and written out in that order, so consecutive threads
+// write consecutive addresses of a digit's run.
 template <typename K, uint32_t ITEMS>
 __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        K* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -235,10 +237,10 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin
     __shared__ uint32_t sv[TILE];
     __shared__ uint32_t run[256];    // global start of this tile's run of digit d
     __shared__ uint32_t lbase[256];  // tile-local start of digit d
-    __shared__ uint32_t lrun[256];   // keys of digit d placed so far
-    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t wc[4][256];  // per-wave digit counts, then each wave's start within the digit
     __shared__ uint32_t scan4[4];
-    const uint32_t t = threadIdx.x, w = t >> 6;
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63u;
+    wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
     {
         // offs is the exclusive scan of the (digit, tile) counts in digit-major
         // order, so this tile's count of digit t is the step to the next entry
@@ -247,41 +249,63 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const K* __restrict__ kin
         const uint32_t o1 = idx + 1 < 256ull * tiles ? offs[idx + 1] : (uint32_t)n;
         run[t] = o;
         uint32_t tot;
-        lbase[t] = block_excl_scan256(o1 - o, scan4, &tot);
+        lbase[t] = block_excl_scan256(o1 - o, scan4, &tot);  // its barriers also publish wc's zeros
     }
-    lrun[t] = 0;
-    wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)w * (64u * ITEMS);
     const uint64_t lt = lanemask_lt();
+    K key[ITEMS];
+    uint32_t val[ITEMS], rank[ITEMS];
+#pragma unroll
     for (uint32_t r = 0; r < ITEMS; ++r) {
-        uint64_t i = base + r * 256 + t;
-        bool valid = i < n;
-        K k = valid ? kin[i] : (K)0;
-        uint32_t v = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint64_t i = base + r * 64 + lane;
+        const bool valid = i < n;
+        key[r] = valid ? kin[i] : (K)0;
+        val[r] = valid ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < ITEMS; ++r) {
+        const bool valid = base + r * 64 + lane < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (uint32_t b = 0; b < 8; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = __ballot(bit);
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = lbase[d] + lrun[d] + rank;
-            for (uint32_t q = 0; q < w; ++q) pos += wcnt[q][d];
-            sk[pos] = k;
-            sv[pos] = v;
-        }
-        __syncthreads();
-        lrun[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
-        wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
-        __syncthreads();
+        const uint32_t below = (uint32_t)__popcll(peers & lt);
+        const uint32_t prior = valid ? wc[w][d] : 0u;
+        rank[r] = prior + below;
+        // every lane of the wave has read its count before the first lane of
+        // each digit group adds the group (LDS operations of a wave are
+        // performed in order)
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) wc[w][d] = prior + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
     }
-    const uint32_t tile_n = n - base < TILE ? (uint32_t)(n - base) : TILE;
+    __syncthreads();
+    {
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t c = wc[q][t];
+            wc[q][t] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < ITEMS; ++r) {
+        if (base + r * 64 + lane < n) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+            const uint32_t pos = lbase[d] + wc[w][d] + rank[r];
+            sk[pos] = key[r];
+            sv[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint64_t tbase = (uint64_t)blockIdx.x * TILE;
+    const uint32_t tile_n = n - tbase < TILE ? (uint32_t)(n - tbase) : TILE;
     for (uint32_t j = t; j < tile_n; j += 256) {
         const K k = sk[j];
         const uint32_t d = (uint32_t)(k >> shift) & 255u;
