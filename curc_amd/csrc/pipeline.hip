@@ -535,7 +535,8 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
             hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs,
                                ranged ? (const uint32_t*)rng : nullptr, bp, off);
             hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, lb_status, lb_words);
-            hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(256), dim3(256), 0, c->st, cs, off);
+            // three workgroups per CU (the cooperative sort's LDS allows three)
+            hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(768), dim3(256), 0, c->st, cs, off);
             if (lb) {
                 compact2(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], n_max,
                          np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
@@ -1465,7 +1466,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                                            (const uint64_t*)sw, P.k, P.kip, o);
                         hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
                     }
-                    hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(256), dim3(256), 0, st, cs, o);
+                    hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(1024), dim3(256), 0, st, cs, o);
                 }
             };
             GraphKey key;
