@@ -1309,10 +1309,10 @@ __global__ void k_map_slots(const uint32_t* __restrict__ vals, const uint32_t* _
 
 // Both halves of the two-strand table (MAP_SLOT2) in one pass: bucket h's
 // forward slot from (vals0, krf0) and reverse-complement slot from (vals1,
-// krf1), each as k_map_slots writes it (the head always, further 16-byte
-// pieces as far as the size reaches).  Eight consecutive lanes write the
-// eight pieces of one bucket's line, so a wave's writes fall in eight
-// consecutive lines.
+// krf1), laid out as k_map_slots lays them out.  Eight consecutive lanes
+// write the eight 16-byte pieces of one bucket's line, all of them (zeros
+// past the sizes), so a wave writes eight whole consecutive lines; partial
+// lines cost more at HBM than the zeros do.
 __global__ void k_map_slots2(const uint32_t* __restrict__ vals0, const uint32_t* __restrict__ krf0,
                              const uint32_t* __restrict__ vals1, const uint32_t* __restrict__ krf1, uint64_t H,
                              uint32_t* __restrict__ bk) {
