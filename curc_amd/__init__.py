@@ -65,6 +65,14 @@ class Job(ctypes.Structure):
                 ("flags", ctypes.c_uint32)]
 
 
+class FastqJob(ctypes.Structure):
+    _fields_ = [("fq1", ctypes.c_void_p), ("bytes1", ctypes.c_uint64), ("fq2", ctypes.c_void_p),
+                ("bytes2", ctypes.c_uint64), ("L", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+BLOCK_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int)
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("ov_grid", "fm_grid", "map_grid", "radix_items", "workers",
                                                "filter_bits", "fm_fused", "ov_queue", "map_both")]
@@ -129,6 +137,11 @@ def lib():
         "curc_mlp_forward": (i32, [vp, vp, u64, vp]),
         "curc_mlp_gradients": (i32, [vp, vp, vp, u64, vp]),
         "curc_mlp_train_step": (i32, [vp, vp, vp, u64, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+        "curc_fastq_block_run": (i32, [vp, ctypes.POINTER(FastqJob), i32, ctypes.POINTER(vp)]),
+        "curc_fastq_batch_run": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(i32)]),
+        "curc_fastq_stream_run": (i32, [vp, vp, u32, BLOCK_CB, vp]),
+        "curc_fastq_pack": (i32, [vp, vp, u64, u32, u64, vp, vp, vp, ctypes.POINTER(u64)]),
+        "curc_block_reads": (u64, [vp]),
         "curc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "curc_block_free": (None, [vp]),
         "curc_ctx_set_timing": (i32, [vp, i32]),
@@ -210,6 +223,11 @@ class Block:
         self._h = ctypes.c_void_p(handle)
         self.has_refmatch = has_refmatch
 
+    @property
+    def n_reads(self):
+        """Reads in the block (the parsed record count for FASTQ input)."""
+        return int(lib().curc_block_reads(self._h))
+
     def mis_off(self, m):
         """Mismatch-offset stream for reads with m mismatches."""
         return self._field(MIS_OFF_BASE + m, np.uint16)
@@ -247,6 +265,54 @@ class Block:
             self.free()
         except Exception:
             pass
+
+
+class HostBuffer:
+    """Pinned host memory (curc_host_alloc) viewed as a uint8 array."""
+
+    def __init__(self, ctx, nbytes):
+        self._ctx = ctx
+        p = ctypes.c_void_p()
+        ctx._check(lib().curc_host_alloc(ctx._h, ctypes.byref(p), max(1, nbytes)), "curc_host_alloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+        self.array = np.frombuffer((ctypes.c_char * max(1, nbytes)).from_address(self.ptr), dtype=np.uint8,
+                                   count=nbytes)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().curc_host_free(self._ctx._h, ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _host_ptr(buf):
+    """(address, length, keep-alive) of FASTQ bytes given as bytes, a uint8
+    array or a HostBuffer."""
+    if buf is None:
+        return None, 0, None
+    if isinstance(buf, HostBuffer):
+        return buf.ptr, buf.nbytes, buf
+    a = np.frombuffer(buf, dtype=np.uint8) if isinstance(buf, (bytes, bytearray, memoryview)) else \
+        np.ascontiguousarray(buf, dtype=np.uint8)
+    return (a.ctypes.data if a.size else None), a.size, a
+
+
+def _run_flags(keep_stages=False, streams=False, preserve_order=False, paired=False, refmatch=False, trace=False,
+               learned=False):
+    flags = RUN_KEEP_STAGES if keep_stages else 0
+    flags |= (RUN_TRACE if trace else 0) | (RUN_LEARNED if learned else 0)
+    if streams:
+        flags |= RUN_STREAMS | (RUN_PRESERVE_ORDER if preserve_order else 0) | (RUN_PAIRED if paired else 0)
+    if refmatch:
+        flags |= RUN_REFMATCH
+    return flags
 
 
 class Context:
@@ -316,6 +382,100 @@ class Context:
                     b.free()
             self._check(rc, "curc_batch_run")
         return blocks
+
+    def _fastq_job(self, fq1, fq2, L, flags, on_device=False, nbytes=None):
+        keep = []
+        if on_device:
+            a1, n1 = fq1, nbytes[0]
+            a2, n2 = (fq2, nbytes[1]) if fq2 is not None else (None, 0)
+            flags |= RUN_INPUT_ON_DEVICE
+        else:
+            a1, n1, k1 = _host_ptr(fq1)
+            a2, n2, k2 = _host_ptr(fq2)
+            keep += [k1, k2]
+        if fq2 is not None:
+            flags |= RUN_PAIRED
+        return FastqJob(a1, n1, a2, n2, L, flags), keep
+
+    def fastq_block(self, fq1, fq2=None, L=None, stop_after=STAGE_ALL, on_device=False, nbytes=None, **kw):
+        """One block from FASTQ bytes (curc_fastq_block_run): fq1 / fq2 are
+        the block's bytes of file 1 / file 2 (bytes, uint8 arrays or
+        HostBuffers), or device pointers with on_device=True and
+        nbytes=(bytes1, bytes2).  L is the input's read length.  Keywords as
+        run_block (keep_stages, streams, preserve_order, refmatch, ...)."""
+        if L is None:
+            raise ValueError("L (the input's read length) is required")
+        job, keep = self._fastq_job(fq1, fq2, L, _run_flags(**kw), on_device, nbytes)
+        out = ctypes.c_void_p()
+        rc = lib().curc_fastq_block_run(self._h, ctypes.byref(job), stop_after, ctypes.byref(out))
+        del keep
+        self._check(rc, "curc_fastq_block_run")
+        return Block(out.value, has_refmatch=bool(kw.get("refmatch")))
+
+    def fastq_batch(self, inputs, L, workers=0, **kw):
+        """Blocks from FASTQ bytes together (curc_fastq_batch_run); inputs is
+        a list of fq1 or (fq1, fq2) host buffers.  Returns one Block each."""
+        flags = _run_flags(**kw)
+        n = len(inputs)
+        jobs = (FastqJob * max(1, n))()
+        keep = []
+        for i, x in enumerate(inputs):
+            f1, f2 = (x if isinstance(x, tuple) else (x, None))
+            jobs[i], k = self._fastq_job(f1, f2, L, flags)
+            keep += k
+        out = (ctypes.c_void_p * max(1, n))()
+        status = (ctypes.c_int32 * max(1, n))()
+        rc = lib().curc_fastq_batch_run(self._h, jobs, n, workers, out, status)
+        blocks = [Block(out[i], has_refmatch=bool(kw.get("refmatch"))) if out[i] else None for i in range(n)]
+        if rc != CURC_OK:
+            for b in blocks:
+                if b is not None:
+                    b.free()
+            self._check(rc, "curc_fastq_batch_run")
+        return blocks
+
+    def fastq_stream(self, inputs, L, on_block, **kw):
+        """Blocks from FASTQ host buffers in order, each next block's bytes
+        copied to the device while the current one runs
+        (curc_fastq_stream_run).  on_block(i, block) is called for each
+        block in order and owns it (block.free())."""
+        flags = _run_flags(**kw)
+        n = len(inputs)
+        jobs = (FastqJob * max(1, n))()
+        keep = []
+        for i, x in enumerate(inputs):
+            f1, f2 = (x if isinstance(x, tuple) else (x, None))
+            jobs[i], k = self._fastq_job(f1, f2, L, flags)
+            keep += k
+        errors = []
+
+        def cb(_user, i, handle, status):
+            if status != CURC_OK or not handle:
+                return
+            try:
+                on_block(int(i), Block(handle, has_refmatch=bool(kw.get("refmatch"))))
+            except Exception as e:  # raised again after the run
+                errors.append(e)
+
+        fn = BLOCK_CB(cb)
+        rc = lib().curc_fastq_stream_run(self._h, jobs, n, fn, None)
+        del keep
+        self._check(rc, "curc_fastq_stream_run")
+        if errors:
+            raise errors[0]
+
+    def fastq_pack(self, d_fq, nbytes, L, max_reads, d_packed, d_isN, d_seq_off):
+        """Record parse + N flag + 2-bit pack of device FASTQ bytes
+        (curc_fastq_pack); returns the record count."""
+        n = ctypes.c_uint64()
+        self._check(lib().curc_fastq_pack(self._h, ctypes.c_void_p(d_fq), nbytes, L, max_reads,
+                                          ctypes.c_void_p(d_packed), ctypes.c_void_p(d_isN) if d_isN else None,
+                                          ctypes.c_void_p(d_seq_off), ctypes.byref(n)), "curc_fastq_pack")
+        return int(n.value)
+
+    def host_buffer(self, nbytes):
+        """Pinned host memory of nbytes (HostBuffer)."""
+        return HostBuffer(self, nbytes)
 
     def ref_match(self, src, dest, dest_contain_n=False, dest_is_ref=False, keep_stages=False):
         """RefMatcher of dest against src (bytes); returns dict(text,

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <chrono>
 #include <numeric>
 #include <cstdlib>
@@ -18,6 +20,7 @@
 
 #include "../../include/curc_gpu.h"
 #include "assembly.cuh"
+#include "fastq.cuh"
 #include "kernels.cuh"
 #include "mapphase.cuh"
 #include "mlp.cuh"
@@ -53,6 +56,7 @@ enum Slot {
     S_RM_LB, S_RM_GAP, S_RM_MOFF, S_RM_MLEN, S_RM_TEXT, S_RM_WORK,
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
     S_OVCAND, S_OVNC, S_LB, S_OVVB, S_RECHQ, S_RECMAP, S_MK2, S_MK3, S_MV2, S_MV3, S_KRF2,
+    S_FQ0, S_FQ1, S_FQSEQ, S_FQTILE,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -63,6 +67,7 @@ static_assert(S_LAST_NAMED <= S_PRIM, "scratch slots overlap");
 enum Cnt {
     C_N, C_NONN, C_NN, C_NPRE, C_NSUF, C_NS0, C_NS1, C_RESN, C_NP0, C_NP1,
     C_HQ, C_UNM, C_PRE, C_SUF, C_NMAP, C_NUNMAP, C_NUNMAPN, C_TMP, C_ASM, C_POOL, C_RM, C_MA0, C_MA1, C_TIE,
+    C_FQERR,
     C_COUNT = 32
 };
 
@@ -131,6 +136,9 @@ struct curc_ctx {
     // unrelated streams share a queue (and its order)
     hipStream_t cst = nullptr;
     hipEvent_t cev[2] = {nullptr, nullptr};
+    // uploads of the next FASTQ block while the current one runs
+    // (curc_fastq_stream_run); created on first use
+    hipStream_t ust = nullptr;
     bool side_copies = true;
     // context (own stream, arena and counters) that runs the N-read stage
     // beside the LQ stage of a single block; created on first use
@@ -1181,8 +1189,97 @@ curc_ctx* helper_ctx(curc_ctx* c) {
     return h;
 }
 
+// Malformed input (reported as CURC_EINVAL).
+struct InputError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// FASTQ bytes of one block in device memory: file 1, and file 2 of a
+// paired block (reads interleave as 2k / 2k+1).  When `ready` is set the
+// stream waits on it before the first read of the bytes; `done`, when set,
+// is recorded after the last one.
+struct FqInput {
+    const uint8_t* fq[2] = {nullptr, nullptr};
+    uint64_t bytes[2] = {0, 0};
+    hipEvent_t ready = nullptr;
+    hipEvent_t done = nullptr;
+};
+
+// Record starts of a block's FASTQ bytes (preprocess.cpp:147-193): returns
+// the read count and leaves the start of each read's sequence line (byte
+// offset in its file) in *seq.  Scratch: tile counts in `tile_slot`, the
+// starts in `seq_slot`.  Throws InputError on malformed input.
+uint64_t fq_parse(curc_ctx* c, const FqInput& in, uint64_t** seq, int tile_slot, int seq_slot, uint32_t* flags,
+                  uint64_t* tile_buf = nullptr, uint64_t tile_cap = 0, uint64_t* seq_buf = nullptr,
+                  uint64_t seq_cap = 0) {
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    const int nf = in.fq[1] ? 2 : 1;
+    uint64_t tiles[2] = {0, 0}, off[2] = {0, 0};
+    uint64_t total_tiles = 0;
+    for (int f = 0; f < nf; ++f) {
+        if (((uintptr_t)in.fq[f] & 15) != 0) throw InputError("FASTQ bytes must be 16-byte aligned on the device");
+        tiles[f] = std::max<uint64_t>(1, (in.bytes[f] + FQ_TILE - 1) / FQ_TILE);
+        off[f] = total_tiles;
+        total_tiles += tiles[f] + 1;  // + the file's total
+    }
+    if (tile_buf && total_tiles > tile_cap) throw std::length_error("FASTQ block larger than the scratch buffer");
+    uint64_t* tc = tile_buf ? tile_buf : S.as<uint64_t>(tile_slot, total_tiles);
+    set_cnt(c, flags, 0u);
+    for (int f = 0; f < nf; ++f) {
+        if (in.bytes[f]) hipLaunchKernelGGL(k_fq_ends, dim3(1), dim3(1), 0, st, in.fq[f], in.bytes[f], flags);
+        hipLaunchKernelGGL(k_fq_count, dim3((uint32_t)tiles[f]), dim3(256), 0, st, in.fq[f], in.bytes[f], tc + off[f]);
+        hipLaunchKernelGGL(k_fq_scan, dim3(1), dim3(1024), 0, st, tc + off[f], tiles[f], tc + off[f] + tiles[f]);
+    }
+    uint64_t lines[2] = {0, 0};
+    uint32_t fl = 0;
+    for (int f = 0; f < nf; ++f)
+        CURC_HIP(hipMemcpyAsync(&lines[f], tc + off[f] + tiles[f], 8, hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipMemcpyAsync(&fl, flags, 4, hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipStreamSynchronize(st));
+    if (fl & 1u) throw InputError("FASTQ block does not start with '@'");
+    for (int f = 0; f < nf; ++f)
+        if ((fl & 2u) || lines[f] % 4) throw InputError("truncated FASTQ record");
+    const uint64_t n1 = lines[0] / 4;
+    if (nf == 2 && lines[1] / 4 != n1) throw InputError("paired-end files hold different record counts");
+    const uint64_t N = n1 * nf;
+    if (N >= (uint64_t)INV) throw std::length_error("a block holds at most 2^32-2 reads");
+    if (seq_buf && N > seq_cap) throw std::length_error("more FASTQ records than the caller's arrays hold");
+    uint64_t* sq = seq_buf ? seq_buf : S.as<uint64_t>(seq_slot, N + 1);
+    for (int f = 0; f < nf; ++f)
+        if (in.bytes[f])
+            hipLaunchKernelGGL(k_fq_lines, dim3((uint32_t)tiles[f]), dim3(256), 0, st, in.fq[f], in.bytes[f],
+                               (const uint64_t*)(tc + off[f]), sq, (uint32_t)nf, (uint32_t)f);
+    *seq = sq;
+    return N;
+}
+
+// Parse + pack of one file's FASTQ bytes on the device into the caller's
+// arrays (packed: max_reads * W words, isN optional, seq: max_reads starts);
+// counters in cnt (device, C_COUNT words); tile counts in tile_buf when
+// given (capacity tile_cap words), else in scratch.  Returns the read count.
+uint64_t fq_pack(curc_ctx* c, const FqInput& in, uint32_t L, uint32_t W, uint64_t max_reads, uint64_t* packed,
+                 uint8_t* isN, uint64_t* seq, uint32_t* cnt, uint64_t* tile_buf, uint64_t tile_cap) {
+    hipStream_t st = c->st;
+    fill<uint32_t>(c, cnt, C_COUNT, 0u);
+    if (in.bytes[0] == 0) return 0;
+    uint64_t* sq = nullptr;
+    const uint64_t N = fq_parse(c, in, &sq, S_FQTILE, S_FQSEQ, cnt + C_FQERR, tile_buf, tile_cap, seq, max_reads);
+    if (N) {
+        if (isN) fill<uint8_t>(c, isN, N, (uint8_t)0);
+        const FqReads fr{in.fq[0], in.fq[0], in.bytes[0], 0, sq, 0u};
+        hipLaunchKernelGGL(k_pack_fq, dim3(grid_for(N * W, 1u << 20)), dim3(256), 0, st, fr, N, L, W, packed, isN,
+                           cnt + C_FQERR);
+    }
+    uint32_t e = 0;
+    CURC_HIP(hipMemcpyAsync(&e, cnt + C_FQERR, 4, hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipStreamSynchronize(st));
+    if (e) throw InputError("read length isn't fix");  // preprocess.cpp:154
+    return N;
+}
+
 void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop, uint32_t flags,
-               curc_block* B) {
+               curc_block* B, const FqInput* fq = nullptr) {
     curc_params P;
     curc_make_params(L, &P);
     const uint32_t W = P.W;
@@ -1191,19 +1288,33 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     const bool keep = (flags & CURC_RUN_KEEP_STAGES) != 0;
     B->L = L;
     B->W = W;
-    B->n_reads = N;
     uint32_t* cnt = S.as<uint32_t>(S_CNT, C_COUNT);
     AsmOut hq_asm, lq_asm, nlq_asm;
     fill<uint32_t>(c, cnt, C_COUNT, 0u);
 
-    // ---- a1/a2: N split + pack (preprocess.cpp:147-199, compress.cu:2151-2180)
-    const uint8_t* d_ascii;
-    if (flags & CURC_RUN_INPUT_ON_DEVICE) {
+    // ---- a1/a2: record parse, N split + pack (preprocess.cpp:147-199, compress.cu:2151-2180)
+    const uint8_t* d_ascii = nullptr;
+    FqReads fr{};
+    if (fq) {
+        Region r(c, "parse");
+        if (fq->ready) CURC_HIP(hipStreamWaitEvent(st, fq->ready, 0));
+        uint64_t* seq = nullptr;
+        N = fq_parse(c, *fq, &seq, S_FQTILE, S_FQSEQ, cnt + C_FQERR);
+        fr = FqReads{fq->fq[0], fq->fq[1] ? fq->fq[1] : fq->fq[0], fq->bytes[0], fq->bytes[1], seq,
+                     fq->fq[1] ? 1u : 0u};
+        c->add_bytes("parse", 2.0 * (fq->bytes[0] + fq->bytes[1]) + 8.0 * N, (double)N);
+    } else if (flags & CURC_RUN_INPUT_ON_DEVICE) {
         d_ascii = (const uint8_t*)ascii;
     } else {
         uint8_t* a = S.as<uint8_t>(S_ASCII, N * L + 16);
         h2d(c, a, (const uint8_t*)ascii, N * L);
         d_ascii = a;
+    }
+    B->n_reads = N;
+    if (fq && N == 0) {
+        // no records: an empty block
+        if (fq->done) CURC_HIP(hipEventRecord(fq->done, st));
+        return;
     }
     uint64_t* packed = S.as<uint64_t>(S_PACKED, N * W + 1);
     uint8_t* isN = S.as<uint8_t>(S_ISN, N + 1);
@@ -1215,7 +1326,10 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         fill<uint64_t>(c, packed + N * W, 1, 0ull);
         // one thread per packed word; isN is set only for reads with an 'N'
         fill<uint8_t>(c, isN, N + 1, (uint8_t)0);
-        if (N)
+        if (N && fq)
+            hipLaunchKernelGGL(k_pack_fq, dim3(grid_for(N * W, 1u << 20)), dim3(256), 0, st, fr, N, L, W, packed,
+                               isN, cnt + C_FQERR);
+        else if (N)
             hipLaunchKernelGGL(k_pack_w, dim3(grid_for(N * W, 1u << 20)), dim3(256), 0, st, d_ascii, N, L, W, packed,
                                isN);
         set_cnt(c, cnt + C_N, (uint32_t)N);
@@ -1223,12 +1337,22 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         compact(cnt + C_N, PredIsN{isN, true}, EmitIndex{Nids}, cnt + C_NN, S, S_PRIM, st, N);
         c->add_bytes("pack", (double)N * (L + 8.0 * W) + 4.0 * N, (double)N);
     }
-    n = read_cnt(c, cnt + C_NONN);
-    nN = read_cnt(c, cnt + C_NN);
+    {
+        uint32_t h[C_COUNT];
+        CURC_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+        CURC_HIP(hipStreamSynchronize(st));
+        if (fq && h[C_FQERR]) throw InputError("read length isn't fix");  // preprocess.cpp:154
+        n = h[C_NONN];
+        nN = h[C_NN];
+    }
     uint8_t* Nascii = S.as<uint8_t>(S_NASCII, (uint64_t)nN * L + 16);
-    if (nN)
+    if (nN && fq)
+        hipLaunchKernelGGL(k_gather_ascii_fq, dim3(grid_for((uint64_t)nN * L)), dim3(256), 0, st, fr,
+                           (const uint32_t*)Nids, (uint64_t)nN, L, Nascii);
+    else if (nN)
         hipLaunchKernelGGL(k_gather_ascii, dim3(grid_for((uint64_t)nN * L)), dim3(256), 0, st, d_ascii,
                            (const uint32_t*)Nids, (uint64_t)nN, L, Nascii);
+    if (fq && fq->done) CURC_HIP(hipEventRecord(fq->done, st));
 
     // ---- a4: dedup (compress.cu:1396-1420)
     uint32_t* next = S.as<uint32_t>(S_NEXT, N + 1);
@@ -1892,6 +2016,195 @@ int fail(curc_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
+// One block from ASCII reads (ascii, n_reads) or from FASTQ bytes on the
+// device (fq); *out receives the block.
+int block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L, int stop_after, uint32_t flags,
+              const FqInput* fq, curc_block** out) {
+    curc_params P;
+    if (curc_make_params(L, &P) != CURC_OK) return fail(c, CURC_EINVAL, "unsupported read length");
+    curc_block* B = new curc_block();
+    bool done_recorded = false;
+    auto finish_input = [&]() {
+        // the input's consumer event is recorded on every path, so an uploader waiting on it proceeds
+        if (fq && fq->done && !done_recorded) hipEventRecord(fq->done, c->st);
+        done_recorded = true;
+    };
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        radix_items() = c->tune.radix_items;
+        run_block(c, ascii, n_reads, L, stop_after <= 0 ? CURC_STAGE_ALL : stop_after, flags, B, fq);
+        done_recorded = true;
+        CURC_HIP(hipGetLastError());
+        CURC_HIP(hipStreamSynchronize(c->st));
+        join_side(c);
+        c->harvest();
+        if (flags & CURC_RUN_KEEP_STAGES) {
+            // expose the packed reads and N-read text for checking
+            const uint64_t N = B->n_reads;
+            d2h(c, B->packed, c->S.as<uint64_t>(S_PACKED, N * P.W + 1), N * P.W);
+            std::vector<uint8_t> na;
+            d2h(c, na, c->S.as<uint8_t>(S_NASCII, (uint64_t)B->Nids.size() * L + 16), (uint64_t)B->Nids.size() * L);
+            CURC_HIP(hipStreamSynchronize(c->st));
+            B->Nascii.assign(na.begin(), na.end());
+        }
+    } catch (const std::exception& e) {
+        // copies into B's arrays may be in flight: join them before freeing
+        finish_input();
+        hipStreamSynchronize(c->st);
+        if (c->cst) hipStreamSynchronize(c->cst);
+        delete B;
+        c->pending.clear();
+        std::string m = e.what();
+        int code = CURC_EHIP;
+        if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
+        else if (dynamic_cast<const std::length_error*>(&e)) code = CURC_ELIMIT;
+        else if (m.find("out of memory") != std::string::npos) code = CURC_ENOMEM;
+        return fail(c, code, m);
+    }
+    *out = B;
+    return CURC_OK;
+}
+
+// Runs n_jobs jobs on up to `workers` contexts (c and its worker contexts,
+// each with its own stream and scratch arena), heaviest first by `weight`
+// (stable), each by run(ctx, job, &out[job]).  Worker errors and stage
+// times land on c.
+template <class Fn>
+int run_on_workers(curc_ctx* c, uint32_t n_jobs, uint32_t workers, const std::vector<uint64_t>& weight,
+                   curc_block** out, int* status, Fn run) {
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        out[j] = nullptr;
+        if (status) status[j] = CURC_OK;
+    }
+    if (n_jobs == 0) return CURC_OK;
+    if (workers == 0) workers = c->tune.workers;
+    workers = std::min<uint32_t>(std::min<uint32_t>(workers, CURC_BATCH_MAX_WORKERS), n_jobs);
+    while (c->workers.size() + 1 < workers) {
+        curc_ctx* w = curc_ctx_create(c->device);
+        if (!w) return fail(c, CURC_EHIP, "could not create a worker stream");
+        w->side_copies = false;
+        c->workers.push_back(w);
+    }
+    // the batch's contexts copy results on their own streams (one stream
+    // per context); the caller's copy stream is released for the batch
+    hipSetDevice(c->device);
+    if (c->cst) {
+        hipStreamSynchronize(c->cst);
+        hipStreamDestroy(c->cst);
+        c->cst = nullptr;
+    }
+    const bool side = c->side_copies;
+    c->side_copies = false;
+    std::vector<curc_ctx*> pool{c};
+    for (uint32_t w = 1; w < workers; ++w) {
+        curc_ctx* wc = c->workers[w - 1];
+        wc->timing = c->timing;
+        wc->index_csr = c->index_csr;
+        wc->tune = c->tune;
+        wc->graphs.enabled = c->graphs.enabled;
+        // workers share the scorer's device arrays (read-only during mapping)
+        wc->has_scorer = c->has_scorer;
+        wc->scorer_net = c->scorer_net;
+        wc->scorer_threshold = c->scorer_threshold;
+        wc->scorer_threshold_unmapped = c->scorer_threshold_unmapped;
+        wc->trace_every = c->trace_every;
+        wc->trace_cap = c->trace_cap;
+        pool.push_back(wc);
+    }
+    std::vector<uint32_t> order(n_jobs);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return weight[a] > weight[b]; });
+    std::atomic<uint32_t> next{0};
+    std::vector<int> rc(n_jobs, CURC_OK);
+    auto work = [&](curc_ctx* wc) {
+        hipSetDevice(wc->device);
+        for (uint32_t k = next++; k < n_jobs; k = next++) {
+            const uint32_t j = order[k];
+            rc[j] = run(wc, j, &out[j]);
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t w = 1; w < workers; ++w) th.emplace_back(work, pool[w]);
+    work(c);
+    for (auto& t : th) t.join();
+    int first = CURC_OK;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        if (status) status[j] = rc[j];
+        if (rc[j] != CURC_OK && first == CURC_OK) first = rc[j];
+    }
+    for (uint32_t w = 1; w < workers; ++w) {
+        curc_ctx* wc = pool[w];
+        if (first != CURC_OK && c->err.empty()) c->err = wc->err;
+        for (const StageStat& st : wc->stats) {
+            const int id = c->stat_id(st.name.c_str());
+            c->stats[id].ms += st.ms;
+            c->stats[id].launches += st.launches;
+            c->stats[id].alg_bytes += st.alg_bytes;
+            c->stats[id].items += st.items;
+        }
+        wc->stats.clear();
+        c->graphs.eager += wc->graphs.eager;
+        c->graphs.captures += wc->graphs.captures;
+        c->graphs.replays += wc->graphs.replays;
+        wc->graphs.eager = wc->graphs.captures = wc->graphs.replays = 0;
+    }
+    c->side_copies = side;
+    return first;
+}
+
+// Copies a FASTQ job's bytes into the device buffer `dev` (file 2 at the
+// next 256-byte boundary after file 1) on stream s; returns the layout.
+FqInput fq_upload(const curc_fastq_job& j, uint8_t* dev, hipStream_t s) {
+    FqInput in;
+    const uint64_t off2 = (j.bytes1 + 255) & ~255ull;
+    in.fq[0] = dev;
+    in.bytes[0] = j.bytes1;
+    if (j.bytes1) CURC_HIP(hipMemcpyAsync(dev, j.fq1, j.bytes1, hipMemcpyHostToDevice, s));
+    if (j.fq2) {
+        in.fq[1] = dev + off2;
+        in.bytes[1] = j.bytes2;
+        if (j.bytes2) CURC_HIP(hipMemcpyAsync(dev + off2, j.fq2, j.bytes2, hipMemcpyHostToDevice, s));
+    }
+    return in;
+}
+
+uint64_t fq_dev_bytes(const curc_fastq_job& j) {
+    return ((j.bytes1 + 255) & ~255ull) + (j.fq2 ? j.bytes2 : 0) + 256;
+}
+
+int fq_check_job(curc_ctx* c, const curc_fastq_job* j) {
+    if (!j) return fail(c, CURC_EINVAL, "null job");
+    if ((j->bytes1 && !j->fq1) || (j->fq2 == nullptr && j->bytes2)) return fail(c, CURC_EINVAL, "null input");
+    curc_params P;
+    if (curc_make_params(j->L, &P) != CURC_OK) return fail(c, CURC_EINVAL, "unsupported read length");
+    return CURC_OK;
+}
+
+uint32_t fq_flags(const curc_fastq_job& j) { return j.flags | (j.fq2 ? CURC_RUN_PAIRED : 0u); }
+
+// One FASTQ job: bytes already on the device, or copied into scratch slot
+// `slot` first on the context's stream.
+int fq_block_run(curc_ctx* c, const curc_fastq_job& j, int stop_after, int slot, curc_block** out) {
+    int rc = fq_check_job(c, &j);
+    if (rc != CURC_OK) return rc;
+    FqInput in;
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        if (j.flags & CURC_RUN_INPUT_ON_DEVICE) {
+            in.fq[0] = (const uint8_t*)j.fq1;
+            in.bytes[0] = j.bytes1;
+            in.fq[1] = (const uint8_t*)j.fq2;
+            in.bytes[1] = j.fq2 ? j.bytes2 : 0;
+        } else {
+            in = fq_upload(j, c->S.as<uint8_t>(slot, fq_dev_bytes(j)), c->st);
+        }
+    } catch (const std::exception& e) {
+        hipStreamSynchronize(c->st);
+        return fail(c, CURC_EHIP, e.what());
+    }
+    return block_run(c, nullptr, 0, j.L, stop_after, fq_flags(j), &in, out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1966,6 +2279,10 @@ void curc_ctx_destroy(curc_ctx* c) {
     hipEventDestroy(c->cev[1]);
     hipEventDestroy(c->nev);
     if (c->cst) hipStreamDestroy(c->cst);
+    if (c->ust) {
+        hipStreamSynchronize(c->ust);
+        hipStreamDestroy(c->ust);
+    }
     hipStreamDestroy(c->st);
     delete c;
 }
@@ -2081,44 +2398,9 @@ int curc_block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L,
                    curc_block** out) {
     if (!c || !out) return CURC_EINVAL;
     *out = nullptr;
-    curc_params P;
-    if (curc_make_params(L, &P) != CURC_OK) return fail(c, CURC_EINVAL, "unsupported read length");
     if (n_reads >= (uint64_t)INV) return fail(c, CURC_ELIMIT, "a block holds at most 2^32-2 reads");
     if (n_reads && !ascii) return fail(c, CURC_EINVAL, "null input");
-    curc_block* B = new curc_block();
-    try {
-        CURC_HIP(hipSetDevice(c->device));
-        radix_items() = c->tune.radix_items;
-        run_block(c, ascii, n_reads, L, stop_after <= 0 ? CURC_STAGE_ALL : stop_after, flags, B);
-        CURC_HIP(hipGetLastError());
-        CURC_HIP(hipStreamSynchronize(c->st));
-        join_side(c);
-        c->harvest();
-        if (flags & CURC_RUN_KEEP_STAGES) {
-            // expose the packed reads and N-read text for checking
-            d2h(c, B->packed, c->S.as<uint64_t>(S_PACKED, n_reads * P.W + 1), n_reads * P.W);
-            std::vector<uint8_t> na;
-            d2h(c, na, c->S.as<uint8_t>(S_NASCII, (uint64_t)B->Nids.size() * L + 16), (uint64_t)B->Nids.size() * L);
-            CURC_HIP(hipStreamSynchronize(c->st));
-            B->Nascii.assign(na.begin(), na.end());
-        }
-    } catch (const std::length_error& e) {
-        // copies into B's arrays may be in flight: join them before freeing
-        hipStreamSynchronize(c->st);
-        if (c->cst) hipStreamSynchronize(c->cst);
-        delete B;
-        c->pending.clear();
-        return fail(c, CURC_ELIMIT, e.what());
-    } catch (const std::exception& e) {
-        hipStreamSynchronize(c->st);
-        if (c->cst) hipStreamSynchronize(c->cst);
-        delete B;
-        c->pending.clear();
-        std::string m = e.what();
-        return fail(c, m.find("out of memory") != std::string::npos ? CURC_ENOMEM : CURC_EHIP, m);
-    }
-    *out = B;
-    return CURC_OK;
+    return block_run(c, ascii, n_reads, L, stop_after, flags, nullptr, out);
 }
 
 uint64_t curc_block_get(curc_block* B, int field, const void** ptr) {
@@ -2254,88 +2536,163 @@ extern "C" {
 int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t workers, curc_block** out,
                    int* status) {
     if (!c || (!jobs && n_jobs) || (!out && n_jobs)) return CURC_EINVAL;
+    std::vector<uint64_t> w(n_jobs);
+    for (uint32_t j = 0; j < n_jobs; ++j) w[j] = jobs[j].n_reads * (uint64_t)jobs[j].L;
+    return run_on_workers(c, n_jobs, workers, w, out, status, [&](curc_ctx* wc, uint32_t j, curc_block** o) {
+        return curc_block_run(wc, jobs[j].ascii, jobs[j].n_reads, jobs[j].L, CURC_STAGE_ALL, jobs[j].flags, o);
+    });
+}
+
+// FASTQ input (preprocess.cpp:66-243): the record loop runs on the device.
+int curc_fastq_block_run(curc_ctx* c, const curc_fastq_job* job, int stop_after, curc_block** out) {
+    if (!c || !out) return CURC_EINVAL;
+    *out = nullptr;
+    return fq_block_run(c, *job, stop_after, S_FQ0, out);
+}
+
+int curc_fastq_batch_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jobs, uint32_t workers,
+                         curc_block** out, int* status) {
+    if (!c || (!jobs && n_jobs) || (!out && n_jobs)) return CURC_EINVAL;
+    std::vector<uint64_t> w(n_jobs);
+    for (uint32_t j = 0; j < n_jobs; ++j) w[j] = jobs[j].bytes1 + (jobs[j].fq2 ? jobs[j].bytes2 : 0);
+    return run_on_workers(c, n_jobs, workers, w, out, status, [&](curc_ctx* wc, uint32_t j, curc_block** o) {
+        return fq_block_run(wc, jobs[j], CURC_STAGE_ALL, S_FQ0, o);
+    });
+}
+
+// Blocks in order on the context's stream while a host thread copies the
+// next block's bytes on the upload stream into the other of two device
+// buffers (the reference parses the next chunks while block_compress runs,
+// preprocess.cpp:91-243).  A copy into a buffer waits for the end of the
+// parse of the block that used it before.
+int curc_fastq_stream_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jobs, curc_block_cb cb, void* user) {
+    if (!c || (!jobs && n_jobs) || !cb) return CURC_EINVAL;
     for (uint32_t j = 0; j < n_jobs; ++j) {
-        out[j] = nullptr;
-        if (status) status[j] = CURC_OK;
+        const int rc = fq_check_job(c, &jobs[j]);
+        if (rc != CURC_OK) return rc;
+        if (jobs[j].flags & CURC_RUN_INPUT_ON_DEVICE)
+            return fail(c, CURC_EINVAL, "curc_fastq_stream_run copies host bytes (no CURC_RUN_INPUT_ON_DEVICE)");
     }
     if (n_jobs == 0) return CURC_OK;
-    if (workers == 0) workers = c->tune.workers;
-    workers = std::min<uint32_t>(std::min<uint32_t>(workers, CURC_BATCH_MAX_WORKERS), n_jobs);
-    while (c->workers.size() + 1 < workers) {
-        curc_ctx* w = curc_ctx_create(c->device);
-        if (!w) return fail(c, CURC_EHIP, "could not create a worker stream");
-        w->side_copies = false;
-        c->workers.push_back(w);
-    }
-    // the batch's contexts copy results on their own streams (one stream
-    // per context); the caller's copy stream is released for the batch
-    hipSetDevice(c->device);
-    if (c->cst) {
-        hipStreamSynchronize(c->cst);
-        hipStreamDestroy(c->cst);
-        c->cst = nullptr;
-    }
-    const bool side = c->side_copies;
-    c->side_copies = false;
-    std::vector<curc_ctx*> pool{c};
-    for (uint32_t w = 1; w < workers; ++w) {
-        curc_ctx* wc = c->workers[w - 1];
-        wc->timing = c->timing;
-        wc->index_csr = c->index_csr;
-        wc->tune = c->tune;
-        wc->graphs.enabled = c->graphs.enabled;
-        // workers share the scorer's device arrays (read-only during mapping)
-        wc->has_scorer = c->has_scorer;
-        wc->scorer_net = c->scorer_net;
-        wc->scorer_threshold = c->scorer_threshold;
-        wc->scorer_threshold_unmapped = c->scorer_threshold_unmapped;
-        wc->trace_every = c->trace_every;
-        wc->trace_cap = c->trace_cap;
-        pool.push_back(wc);
-    }
-    std::vector<uint32_t> order(n_jobs);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return jobs[a].n_reads * (uint64_t)jobs[a].L > jobs[b].n_reads * (uint64_t)jobs[b].L;
-    });
-    std::atomic<uint32_t> next{0};
-    std::vector<int> rc(n_jobs, CURC_OK);
-    auto work = [&](curc_ctx* wc) {
-        hipSetDevice(wc->device);
-        for (uint32_t k = next++; k < n_jobs; k = next++) {
-            const uint32_t j = order[k];
-            rc[j] = curc_block_run(wc, jobs[j].ascii, jobs[j].n_reads, jobs[j].L, CURC_STAGE_ALL, jobs[j].flags,
-                                   &out[j]);
+    uint8_t* buf[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> up(n_jobs, nullptr), used(n_jobs, nullptr);
+    auto release = [&]() {
+        for (uint32_t i = 0; i < n_jobs; ++i) {
+            if (up[i]) hipEventDestroy(up[i]);
+            if (used[i]) hipEventDestroy(used[i]);
         }
     };
-    std::vector<std::thread> th;
-    for (uint32_t w = 1; w < workers; ++w) th.emplace_back(work, pool[w]);
-    work(c);
-    for (auto& t : th) t.join();
-    int first = CURC_OK;
-    for (uint32_t j = 0; j < n_jobs; ++j) {
-        if (status) status[j] = rc[j];
-        if (rc[j] != CURC_OK && first == CURC_OK) first = rc[j];
-    }
-    // worker errors and stage times land on the calling context
-    for (uint32_t w = 1; w < workers; ++w) {
-        curc_ctx* wc = pool[w];
-        if (first != CURC_OK && c->err.empty()) c->err = wc->err;
-        for (const StageStat& st : wc->stats) {
-            const int id = c->stat_id(st.name.c_str());
-            c->stats[id].ms += st.ms;
-            c->stats[id].launches += st.launches;
-            c->stats[id].alg_bytes += st.alg_bytes;
-            c->stats[id].items += st.items;
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        uint64_t mx = 0;
+        for (uint32_t j = 0; j < n_jobs; ++j) mx = std::max(mx, fq_dev_bytes(jobs[j]));
+        buf[0] = c->S.as<uint8_t>(S_FQ0, mx);
+        buf[1] = c->S.as<uint8_t>(S_FQ1, mx);
+        if (!c->ust) CURC_HIP(hipStreamCreateWithFlags(&c->ust, hipStreamNonBlocking));
+        for (uint32_t i = 0; i < n_jobs; ++i) {
+            CURC_HIP(hipEventCreateWithFlags(&up[i], hipEventDisableTiming));
+            CURC_HIP(hipEventCreateWithFlags(&used[i], hipEventDisableTiming));
         }
-        wc->stats.clear();
-        c->graphs.eager += wc->graphs.eager;
-        c->graphs.captures += wc->graphs.captures;
-        c->graphs.replays += wc->graphs.replays;
-        wc->graphs.eager = wc->graphs.captures = wc->graphs.replays = 0;
+    } catch (const std::exception& e) {
+        release();
+        return fail(c, CURC_EHIP, e.what());
     }
-    c->side_copies = side;
+    std::mutex m;
+    std::condition_variable cv;
+    std::vector<int> uploaded(n_jobs, 0);  // 1: copies enqueued, -1: failed
+    std::vector<char> consumed(n_jobs, 0); // the block's `used` event is recorded
+    std::vector<FqInput> inputs(n_jobs);
+    bool stop = false;
+    std::string up_err;
+    std::thread uploader([&]() {
+        hipSetDevice(c->device);
+        for (uint32_t i = 0; i < n_jobs; ++i) {
+            if (i >= 2) {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return consumed[i - 2] || stop; });
+                if (stop) return;
+            }
+            int ok = 1;
+            try {
+                if (i >= 2) CURC_HIP(hipStreamWaitEvent(c->ust, used[i - 2], 0));
+                inputs[i] = fq_upload(jobs[i], buf[i & 1], c->ust);
+                CURC_HIP(hipEventRecord(up[i], c->ust));
+            } catch (const std::exception& e) {
+                ok = -1;
+                up_err = e.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(m);
+                uploaded[i] = ok;
+            }
+            cv.notify_all();
+            if (ok < 0) return;
+        }
+    });
+    int first = CURC_OK;
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        int state;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return uploaded[i] != 0; });
+            state = uploaded[i];
+        }
+        curc_block* B = nullptr;
+        int rc;
+        if (state < 0) {
+            rc = fail(c, CURC_EHIP, up_err);
+        } else {
+            FqInput in = inputs[i];
+            in.ready = up[i];
+            in.done = used[i];
+            rc = block_run(c, nullptr, 0, jobs[i].L, CURC_STAGE_ALL, fq_flags(jobs[i]), &in, &B);
+        }
+        {
+            std::lock_guard<std::mutex> lk(m);
+            consumed[i] = 1;
+            if (rc != CURC_OK) stop = true;
+        }
+        cv.notify_all();
+        cb(user, i, B, rc);
+        if (rc != CURC_OK) {
+            first = rc;
+            break;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(m);
+        stop = true;
+    }
+    cv.notify_all();
+    uploader.join();
+    hipStreamSynchronize(c->ust);
+    release();
     return first;
+}
+
+uint64_t curc_block_reads(const curc_block* B) { return B ? B->n_reads : 0; }
+
+int curc_fastq_pack(curc_ctx* c, const char* d_fq, uint64_t bytes, uint32_t L, uint64_t max_reads,
+                    uint64_t* d_packed, uint8_t* d_isN, uint64_t* d_seq_off, uint64_t* n_reads) {
+    if (!c || !n_reads || (bytes && (!d_fq || !d_packed || !d_seq_off))) return CURC_EINVAL;
+    *n_reads = 0;
+    curc_params P;
+    if (curc_make_params(L, &P) != CURC_OK) return fail(c, CURC_EINVAL, "unsupported read length");
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        FqInput in;
+        in.fq[0] = (const uint8_t*)d_fq;
+        in.bytes[0] = bytes;
+        uint32_t* cnt = c->S.as<uint32_t>(S_CNT, C_COUNT);
+        *n_reads = fq_pack(c, in, L, P.W, max_reads, d_packed, d_isN, d_seq_off, cnt, nullptr, 0);
+    } catch (const std::exception& e) {
+        hipStreamSynchronize(c->st);
+        int code = CURC_EHIP;
+        if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
+        else if (dynamic_cast<const std::length_error*>(&e)) code = CURC_ELIMIT;
+        return fail(c, code, e.what());
+    }
+    return CURC_OK;
 }
 
 }  // extern "C"

@@ -1,120 +1,210 @@
-"""FASTQ reading and block cutting (preprocess.cpp:66-319).
+"""FASTQ input: the reference's chunking and block cut (preprocess.cpp:66-243).
 
-Reads are taken in I/O chunks of 50 MB (25 MB per file when paired); a block
-closes after the chunk that takes the block past block_ratio of the input
-bytes (twice the file size when paired), so blocks hold whole chunks.  Block-
-global read ids follow record order; paired reads interleave as 2k (file 1)
-and 2k+1 (file 2).  Only the sequence line of each record is kept, as in
-CURC, which compresses the DNA stream alone.
+The reference reads an input in I/O chunks of 50 MB (25 MB per file when
+paired), memory-mapped one at a time.  Every chunk but the last is cut back
+to the start of a record by a backward scan (preprocess.cpp:114-138), and a
+block closes after the chunk that takes it past block_ratio of the input
+bytes (twice the file size when paired, :205-207).  So a block is a run of
+whole chunks, which here is one byte range [start, end) of the file (the
+same range of both files when paired: file 2 is mapped at file 1's offsets,
+:98-101).  block_ranges() computes those ranges touching only the first
+line and the tail of each chunk; the records themselves are parsed on the
+GPU (curc_fastq_* in libcurc_gpu) or, for the ASCII interface, by
+sequences() below.
+
+Block-global read ids follow record order; paired reads interleave as 2k
+(file 1) and 2k+1 (file 2).  Only the sequence line of each record is kept,
+as in CURC, which compresses the DNA stream alone.
 """
+import mmap
 import os
 
 import numpy as np
 
 IO_CHUNK = 50 * 1000 * 1000
+HEADER_PREFIX = b"synth."
 
 
 class FastqError(ValueError):
     pass
 
 
-def _sequences(buf: bytes, L=None):
-    """Sequence lines of the FASTQ records in buf -> (n, L) uint8."""
-    a = np.frombuffer(buf, dtype=np.uint8)
+def _cut(buf, start, size):
+    """Backward scan of preprocess.cpp:114-138 over the chunk
+    buf[start:start+size] (not the input's last chunk): returns the chunk's
+    new size.  Candidates are positions e (scanning down from size minus the
+    first line's length) with a newline before and '@' at e; each candidate's
+    bytes are compared with the chunk's first line, and the candidate is
+    taken as soon as more than five comparisons in a row were equal.  The
+    run counter is reset only by an unequal byte, not between candidates,
+    exactly as in the reference.  Positions without a candidate leave the
+    counter alone, so only candidates are visited (rfind)."""
+    if buf[start:start + 1] != b"@":
+        raise FastqError("io_buffer doesn't start with @")  # preprocess.cpp:116
+    nl = buf.find(b"\n", start, start + size)
+    if nl < 0:
+        raise FastqError("no line end in an I/O chunk")
+    first_len = nl - start
+    first = buf[start:start + first_len]
+    e = size - first_len
+    equal = 0
+    while e > 0:
+        # the next candidate at or below e: '\n' at e-1 and '@' at e
+        k = buf.rfind(b"\n@", start, start + e + 1)
+        if k < 0:
+            break
+        e = k + 1 - start
+        cand = buf[start + e:start + e + first_len]
+        for j in range(first_len):
+            if first[j] == cand[j]:
+                equal += 1
+                if equal > 5:
+                    return e
+            else:
+                equal = 0
+        e -= 1
+    # the reference keeps scanning with an empty chunk and never advances
+    raise FastqError("no record start found in an I/O chunk (records longer than the chunk, "
+                     "or headers without a common prefix of six bytes)")
+
+
+def block_ranges(path1, path2=None, block_ratio=1.0, io_chunk=IO_CHUNK):
+    """Byte ranges [(start, end), ...] of the blocks of an input, following
+    process() (preprocess.cpp:72-243): chunks of io_chunk bytes (half of it
+    per file when paired), each but the last cut back by _cut, blocks closed
+    after the chunk that takes the block's bytes past
+    block_bytes = fastq_bytes * block_ratio (both doubled when paired)."""
+    if not 0 < block_ratio <= 1:
+        raise ValueError("block_ratio must be in (0, 1]")  # main.cpp:111-115
+    paired = path2 is not None
+    size = os.path.getsize(path1)
+    if paired and os.path.getsize(path2) != size:
+        raise FastqError("paired-end file size isn't equal")  # preprocess.cpp:83
+    if size == 0:
+        return []
+    block_bytes = int((2.0 if paired else 1.0) * float(size) * block_ratio)
+    chunk = io_chunk // 2 if paired else io_chunk
+    out = []
+    with open(path1, "rb") as f:
+        buf = mmap.mmap(f.fileno(), 0, access=mmap.ACCESS_READ)
+        try:
+            total = last = 0
+            while total < size:
+                start = total
+                n = min(chunk, size - total)
+                total += n
+                if total < size:
+                    cut = _cut(buf, start, n)
+                    total -= n - cut
+                blk = total - last
+                if paired:
+                    blk *= 2
+                if blk > block_bytes:
+                    out.append((last, total))
+                    last = total
+            if total - last > 0:
+                out.append((last, total))
+        finally:
+            buf.close()
+    return out
+
+
+def read_length(path):
+    """Sequence length of the first record (compress(), preprocess.cpp:281-292)."""
+    with open(path, "rb") as f:
+        f.readline()
+        return len(f.readline().rstrip(b"\n"))
+
+
+def sequences(buf, L=None):
+    """Sequence lines of the FASTQ records in buf (bytes-like or uint8
+    array) -> (n, L) uint8, following the record loop of preprocess.cpp:147-
+    193 (four lines per record, the second one kept)."""
+    a = np.frombuffer(buf, dtype=np.uint8) if not isinstance(buf, np.ndarray) else buf
+    if len(a) == 0:
+        return np.zeros((0, L or 0), dtype=np.uint8)
+    if a[0] != 64:
+        raise FastqError("FASTQ block does not start with '@'")
+    if a[-1] != 10:
+        raise FastqError("truncated FASTQ record")
     nl = np.flatnonzero(a == 10)
     if len(nl) % 4:
         raise FastqError("truncated FASTQ record")
-    starts = np.concatenate([[0], nl[:-1] + 1])
-    if len(starts) and buf[0:1] != b"@":
-        raise FastqError("FASTQ record does not start with '@'")
-    s = starts[1::4]
+    s = nl[0::4] + 1
     e = nl[1::4]
     lens = e - s
-    if len(lens) == 0:
-        return np.zeros((0, L or 0), dtype=np.uint8)
     if L is None:
         L = int(lens[0])
     if np.any(lens != L):
         raise FastqError("read length isn't fix")  # preprocess.cpp:154
-    if L > 512:
-        raise FastqError("read length must be at most 512")  # preprocess.cpp:315-318
-    idx = s[:, None] + np.arange(L)[None, :]
-    return a[idx]
+    if L == 0 or L > 512:
+        raise FastqError("read length must be 1..512")  # preprocess.cpp:315-318
+    return a[s[:, None] + np.arange(L)[None, :]]
 
 
-def _chunks(path, chunk):
-    """Byte chunks of a FASTQ file, each ending at a record boundary."""
+def read_range(path, start, end):
+    """Bytes [start, end) of a file."""
     with open(path, "rb") as f:
-        data = f.read()
-    n = len(data)
-    pos = 0
-    while pos < n:
-        end = min(n, pos + chunk)
-        if end < n:
-            # back up to the last complete record (four newlines)
-            cut = end
-            k = data.rfind(b"\n@", pos, cut)
-            while k != -1:
-                # a record start is a line starting with '@' whose line index is a multiple of 4
-                if data.count(b"\n", pos, k + 1) % 4 == 0:
-                    end = k + 1
-                    break
-                k = data.rfind(b"\n@", pos, k)
-            if k == -1:
-                end = n
-        yield data[pos:end]
-        pos = end
+        f.seek(start)
+        return f.read(end - start)
 
 
 def read_blocks(path1, path2=None, block_ratio=1.0, io_chunk=IO_CHUNK):
-    """Yields the blocks of a FASTQ input as (n, L) uint8 arrays.  io_chunk
-    is the reference's 50 MB read granularity (smaller values cut small
-    inputs into several blocks)."""
-    if not 0 < block_ratio <= 1:
-        raise ValueError("block_ratio must be in (0, 1]")  # main.cpp:111-115
-    paired = path2 is not None
-    size1 = os.path.getsize(path1)
-    if paired and os.path.getsize(path2) != size1:
-        raise FastqError("paired-end file size isn't equal")  # preprocess.cpp:83
-    block_bytes = (2 if paired else 1) * size1 * block_ratio
-    chunk = io_chunk // 2 if paired else io_chunk
-    it1 = _chunks(path1, chunk)
-    it2 = _chunks(path2, chunk) if paired else None
-    L = None
-    cur, cur_bytes = [], 0
-    for c1 in it1:
-        r1 = _sequences(c1, L)
-        L = r1.shape[1] if L is None else L
-        if paired:
-            c2 = next(it2)
-            r2 = _sequences(c2, L)
-            if len(r2) != len(r1):
-                raise FastqError("paired files have different record counts in a chunk")
-            r = np.empty((2 * len(r1), L), dtype=np.uint8)
-            r[0::2], r[1::2] = r1, r2
-            cur_bytes += len(c1) + len(c2)
-        else:
-            r = r1
-            cur_bytes += len(c1)
-        cur.append(r)
-        if cur_bytes > block_bytes:
-            yield np.concatenate(cur)
-            cur, cur_bytes = [], 0
-    if cur:
-        yield np.concatenate(cur)
+    """Yields the blocks of a FASTQ input as (n, L) uint8 arrays (paired
+    reads interleaved), cut as block_ranges() cuts them."""
+    L = read_length(path1)
+    for s, e in block_ranges(path1, path2, block_ratio, io_chunk):
+        r1 = sequences(read_range(path1, s, e), L)
+        if path2 is None:
+            yield r1
+            continue
+        r2 = sequences(read_range(path2, s, e), L)
+        if len(r2) != len(r1):
+            raise FastqError("paired-end files hold different record counts in a block")
+        r = np.empty((2 * len(r1), L), dtype=np.uint8)
+        r[0::2], r[1::2] = r1, r2
+        yield r
 
 
-def write_fastq(path, reads, name="r"):
-    """Writes reads ((n, L) uint8) as FASTQ with constant qualities."""
+def fastq_bytes(reads, first_id=0, mate=None, prefix=HEADER_PREFIX):
+    """FASTQ records of reads ((n, L) uint8) as one uint8 array: header
+    '@<prefix><id, 10 digits>' (with '/1' or '/2' for a mate), the sequence,
+    '+', and constant qualities 'I'.  Every record has the same length, and
+    headers share their first bytes, which is what the reference's chunk cut
+    looks for (preprocess.cpp:119-130)."""
+    reads = np.ascontiguousarray(reads, dtype=np.uint8)
     n, L = reads.shape
-    qual = b"I" * L
+    head = b"@" + prefix
+    tail = b"" if mate is None else b"/%d" % mate
+    hl = len(head) + 10 + len(tail) + 1
+    rec = hl + L + 3 + L + 1
+    out = np.empty((n, rec), dtype=np.uint8)
+    out[:, :len(head)] = np.frombuffer(head, dtype=np.uint8)
+    ids = np.arange(first_id, first_id + n, dtype=np.int64)
+    for d in range(10):
+        out[:, len(head) + 9 - d] = 48 + (ids // 10 ** d) % 10
+    if tail:
+        out[:, len(head) + 10:hl - 1] = np.frombuffer(tail, dtype=np.uint8)
+    out[:, hl - 1] = 10
+    out[:, hl:hl + L] = reads
+    out[:, hl + L:hl + L + 3] = np.frombuffer(b"\n+\n", dtype=np.uint8)
+    out[:, hl + L + 3:rec - 1] = 73
+    out[:, rec - 1] = 10
+    return out.reshape(-1)
+
+
+def write_fastq(path, reads, mate=None):
+    """Writes reads ((n, L) uint8) as FASTQ (fastq_bytes records)."""
     with open(path, "wb") as f:
-        for i in range(n):
-            f.write(b"@%s%d\n" % (name.encode(), i))
-            f.write(reads[i].tobytes())
-            f.write(b"\n+\n")
-            f.write(qual)
-            f.write(b"\n")
+        step = 1 << 20
+        for i in range(0, len(reads), step):
+            f.write(fastq_bytes(reads[i:i + step], first_id=i, mate=mate).tobytes())
+
+
+def write_paired_fastq(path1, path2, reads):
+    """Interleaved pairs (reads 2k, 2k+1) as two FASTQ files of equal size."""
+    write_fastq(path1, reads[0::2], mate=1)
+    write_fastq(path2, reads[1::2], mate=2)
 
 
 def write_seq(path, reads):

@@ -244,6 +244,59 @@ typedef struct curc_job {
 int curc_batch_run(curc_ctx *ctx, const curc_job *jobs, uint32_t n_jobs, uint32_t workers, curc_block **out,
                    int *status);
 
+/* ---- FASTQ input -----------------------------------------------------------
+ * The record loop of the reference's host parser (preprocess.cpp:147-199,
+ * with the read length of compress(), :281-292) runs on the device: a
+ * block's FASTQ bytes are copied to HBM as they are and parsed there (four
+ * lines per record, the sequence line kept, 'N' reads split off, 2-bit
+ * packed).  A block is a run of whole records; where blocks end is the
+ * caller's chunking (preprocess.cpp:91-138, 200-243; curc_amd/fastq.py).
+ * Malformed input (a sequence line of another length than L: "read length
+ * isn't fix", :154; a block not starting with '@' or ending mid-record;
+ * paired files with different record counts) fails with CURC_EINVAL.  Block
+ * ids follow record order; a paired block (fq2 given) interleaves file 1's
+ * record k as read 2k and file 2's as 2k+1 (:156-192), and its streams are
+ * built as paired. */
+typedef struct curc_fastq_job {
+    const char *fq1;  /* file 1's bytes of the block (host memory; device with CURC_RUN_INPUT_ON_DEVICE,
+                         then 16-byte aligned) */
+    uint64_t bytes1;
+    const char *fq2;  /* file 2's bytes of a paired block, else NULL */
+    uint64_t bytes2;
+    uint32_t L;       /* read length of the input */
+    uint32_t flags;   /* CURC_RUN_* */
+} curc_fastq_job;
+
+/* One block from FASTQ bytes; replaces the per-chunk parse, encode and
+ * block_compress calls of process() (preprocess.cpp:147-213) for one block.
+ * Host bytes are copied to the device first (pinned memory copies fastest). */
+int curc_fastq_block_run(curc_ctx *ctx, const curc_fastq_job *job, int stop_after, curc_block **out);
+/* Blocks from FASTQ bytes on up to CURC_BATCH_MAX_WORKERS streams, as
+ * curc_batch_run (each worker copies its block's bytes, then runs it). */
+int curc_fastq_batch_run(curc_ctx *ctx, const curc_fastq_job *jobs, uint32_t n_jobs, uint32_t workers,
+                         curc_block **out, int *status);
+/* Blocks from FASTQ bytes in host memory, in order on the context's stream,
+ * while the next block's bytes are copied to the device beside it (the
+ * reference parses the next chunks while block_compress runs,
+ * preprocess.cpp:91-243).  Each finished block goes to cb on the calling
+ * thread, in job order, with its status; cb owns it (curc_block_free).
+ * Stops at the first failing block (its cb gets a NULL block) and returns
+ * its code. */
+typedef void (*curc_block_cb)(void *user, uint32_t job, curc_block *block, int status);
+int curc_fastq_stream_run(curc_ctx *ctx, const curc_fastq_job *jobs, uint32_t n_jobs, curc_block_cb cb,
+                          void *user);
+/* Reads in a block (the parsed record count for FASTQ input). */
+uint64_t curc_block_reads(const curc_block *b);
+/* Record parse + N flag + 2-bit pack of single-end FASTQ bytes on the device
+ * (d_fq 16-byte aligned): the parse part of process() and encode
+ * (preprocess.cpp:147-199, compress.cu:2151-2180) for one buffer.  Writes
+ * d_packed (max_reads * ceil(L/32) words), d_isN (optional; max_reads bytes
+ * rounded up to a multiple of 4) and d_seq_off (max_reads u64: where each
+ * read's sequence line starts in d_fq); *n_reads receives the record count.
+ * CURC_ELIMIT when the bytes hold more than max_reads records. */
+int curc_fastq_pack(curc_ctx *ctx, const char *d_fq, uint64_t bytes, uint32_t L, uint64_t max_reads,
+                    uint64_t *d_packed, uint8_t *d_isN, uint64_t *d_seq_off, uint64_t *n_reads);
+
 /* ---- learned candidate scorer ----------------------------------------------
  * No counterpart in the reference.  A 32 -> 128 -> 128 -> 1 network (ReLU,
  * bfloat16 matrix products on the MFMA units, float32 accumulation and

@@ -695,6 +695,58 @@ void orc_pack(const char *ascii, uint64_t n, uint32_t L, uint64_t *packed) {
     pack_reads(ascii, n, L, (L + 31) / 32, packed);
 }
 
+// The record loop of process() (preprocess.cpp:147-193) over one block's
+// bytes: per record, skip the header line, take the sequence line (its
+// length must be L, :152-154), skip the '+' and quality lines; a paired
+// block reads file 2's record in step with file 1's, ids 2k / 2k+1
+// (:156-192).  The loop runs over file 1's bytes as the reference's does;
+// a file 2 that runs out of records first is reported (-3) where the
+// reference would read past its buffer.  Reads land in `out` in id order
+// (cap_reads of them at most).  Returns 0, -1 (read length isn't fix),
+// -2 (truncated record) or -3 (record counts differ) / -4 (cap exceeded).
+int orc_parse_fastq(const char *fq1, uint64_t bytes1, const char *fq2, uint64_t bytes2, uint32_t L, char *out,
+                    uint64_t cap_reads, uint64_t *n_reads) {
+    uint64_t i = 0, j = 0, n = 0;
+    const bool paired = fq2 != nullptr;
+    auto line_end = [](const char *b, uint64_t len, uint64_t at) -> uint64_t {
+        const void *p = at < len ? std::memchr(b + at, '\n', len - at) : nullptr;
+        return p ? (uint64_t)((const char *)p - b) : len;
+    };
+    *n_reads = 0;
+    while (i < bytes1) {
+        const uint64_t h = line_end(fq1, bytes1, i);
+        const uint64_t seq = h + 1;
+        const uint64_t e = line_end(fq1, bytes1, seq);
+        if (e >= bytes1) return -2;
+        if (e - seq != L) return -1;
+        if (n >= cap_reads) return -4;
+        std::memcpy(out + n * L, fq1 + seq, L);
+        ++n;
+        const uint64_t plus = line_end(fq1, bytes1, e + 1);
+        const uint64_t qual = plus < bytes1 ? line_end(fq1, bytes1, plus + 1) : bytes1;
+        if (qual >= bytes1) return -2;
+        i = qual + 1;
+        if (paired) {
+            if (j >= bytes2) return -3;
+            const uint64_t h2 = line_end(fq2, bytes2, j);
+            const uint64_t seq2 = h2 + 1;
+            const uint64_t e2 = line_end(fq2, bytes2, seq2);
+            if (e2 >= bytes2) return -2;
+            if (e2 - seq2 != L) return -1;
+            if (n >= cap_reads) return -4;
+            std::memcpy(out + n * L, fq2 + seq2, L);
+            ++n;
+            const uint64_t plus2 = line_end(fq2, bytes2, e2 + 1);
+            const uint64_t qual2 = plus2 < bytes2 ? line_end(fq2, bytes2, plus2 + 1) : bytes2;
+            if (qual2 >= bytes2) return -2;
+            j = qual2 + 1;
+        }
+    }
+    if (paired && j < bytes2) return -3;
+    *n_reads = n;
+    return 0;
+}
+
 orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int stop_after) {
     orc_params P;
     if (orc_make_params(L, &P) != 0) return nullptr;

@@ -122,6 +122,14 @@ int orc_rc_mismatch(const uint64_t *ref, uint64_t ref_off, const uint64_t *read,
                     uint32_t L, uint32_t bound);
 void orc_pack(const char *ascii, uint64_t n, uint32_t L, uint64_t *packed);
 
+/* Record loop of preprocess.cpp:147-193 over one block's FASTQ bytes (fq2
+ * for paired input, ids 2k / 2k+1): reads in id order into out (cap_reads
+ * reads of L bytes at most), count in *n_reads.  0 on success; -1 a
+ * sequence line of another length than L, -2 a truncated record, -3 paired
+ * files with different record counts, -4 more than cap_reads reads. */
+int orc_parse_fastq(const char *fq1, uint64_t bytes1, const char *fq2, uint64_t bytes2, uint32_t L, char *out,
+                    uint64_t cap_reads, uint64_t *n_reads);
+
 /* Whole-block run over n_reads ASCII reads of length L (block-global ids
  * 0..n_reads-1 in input order).  Returns NULL on invalid input. */
 orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int stop_after);

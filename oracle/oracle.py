@@ -63,6 +63,7 @@ def lib(kind="st"):
         "orc_forward_mismatch": (i32, [vp, u64, vp, u32, u32]),
         "orc_rc_mismatch": (i32, [vp, u64, vp, u32, u32]),
         "orc_pack": (None, [vp, u64, u32, vp]),
+        "orc_parse_fastq": (i32, [vp, u64, vp, u64, u32, vp, u64, ctypes.POINTER(u64)]),
         "orc_block_run": (vp, [vp, u64, u32, i32]),
         "orc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "orc_block_free": (None, [vp]),
@@ -199,3 +200,23 @@ def pack(reads):
     out = np.zeros(n * W + 1, dtype=np.uint64)
     lib().orc_pack(reads.ctypes.data, n, L, out.ctypes.data)
     return out[: n * W].reshape(n, W)
+
+
+PARSE_ERRORS = {-1: "read length isn't fix", -2: "truncated FASTQ record",
+                -3: "paired-end files hold different record counts", -4: "too many records"}
+
+
+def parse_fastq(fq1, L, fq2=None, kind="st"):
+    """Reads of one block's FASTQ bytes (preprocess.cpp:147-193) as an
+    (n, L) uint8 array in id order (paired: 2k from fq1, 2k+1 from fq2)."""
+    a1 = np.frombuffer(fq1, dtype=np.uint8) if not isinstance(fq1, np.ndarray) else fq1
+    a2 = None if fq2 is None else (np.frombuffer(fq2, dtype=np.uint8) if not isinstance(fq2, np.ndarray) else fq2)
+    cap = (len(a1) + (0 if a2 is None else len(a2))) // (L + 5) + 1
+    out = np.empty((cap, L), dtype=np.uint8)
+    n = ctypes.c_uint64()
+    rc = lib(kind).orc_parse_fastq(a1.ctypes.data if len(a1) else None, len(a1),
+                                   None if a2 is None else (a2.ctypes.data if len(a2) else None),
+                                   0 if a2 is None else len(a2), L, out.ctypes.data, cap, ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(PARSE_ERRORS.get(rc, f"parse error {rc}"))
+    return out[: n.value]
