@@ -1296,8 +1296,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     const uint8_t* d_ascii = nullptr;
     FqReads fr{};
     if (fq) {
-        Region r(c, "parse");
         if (fq->ready) CURC_HIP(hipStreamWaitEvent(st, fq->ready, 0));
+        Region r(c, "parse");
         uint64_t* seq = nullptr;
         N = fq_parse(c, *fq, &seq, S_FQTILE, S_FQSEQ, cnt + C_FQERR);
         fr = FqReads{fq->fq[0], fq->fq[1] ? fq->fq[1] : fq->fq[0], fq->bytes[0], fq->bytes[1], seq,

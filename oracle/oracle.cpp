@@ -13,10 +13,22 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+#include <parallel/algorithm>
 #define ORC_PARALLEL_FOR _Pragma("omp parallel for schedule(dynamic, 1024)")
 #else
 #define ORC_PARALLEL_FOR
 #endif
+
+// Stable sort: libstdc++'s parallel multiway mergesort in the OpenMP build
+// (a stable sort has one result, so both builds agree byte for byte).
+template <class It, class Cmp>
+void orc_stable_sort(It a, It b, Cmp cmp) {
+#ifdef _OPENMP
+    __gnu_parallel::stable_sort(a, b, cmp);
+#else
+    std::stable_sort(a, b, cmp);
+#endif
+}
 
 namespace {
 
@@ -778,7 +790,7 @@ orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int st
     uint32_t *next = B->next.data(), *prev = B->prev.data();
     uint16_t *offset = B->offset.data();
     B->sorted = B->nonN;
-    std::stable_sort(B->sorted.begin(), B->sorted.end(), [&](uint32_t a, uint32_t b) {
+    orc_stable_sort(B->sorted.begin(), B->sorted.end(), [&](uint32_t a, uint32_t b) {
         for (uint32_t w = 0; w < W; ++w) {
             uint64_t x = R[(uint64_t)a * W + w], y = R[(uint64_t)b * W + w];
             if (x != y) return x < y;
@@ -811,7 +823,7 @@ orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int st
             hk[i] = (uint32_t)(murmur(xw(R, (uint64_t)value[i] * W, P.kip, P.k)) & mask);
         std::vector<uint32_t> ord(value.size());
         std::iota(ord.begin(), ord.end(), 0u);
-        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return hk[a] < hk[b]; });
+        orc_stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return hk[a] < hk[b]; });
         std::vector<uint32_t> va(value.size());
         std::vector<uint32_t> kr((size_t)H + 2, 0);
         for (size_t i = 0; i < ord.size(); ++i) {
@@ -826,10 +838,22 @@ orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int st
         }
 
         // a6: greedy overlap loop (compress.cu:1473-1532), suffixes in list
-        // order, claims visible to later suffixes at once.
+        // order, claims visible to later suffixes at once.  The scan of
+        // :1502-1527 takes the first candidate of the bucket from the lower
+        // bound on that is neither the suffix itself nor taken, and links it
+        // if it equals s[off:L].  A bucket is in lexicographic order, so the
+        // equal candidates are one run [first, eq) from the lower bound, and
+        // that outcome is: the first free non-self member of the run, if any.
+        // The run does not depend on the claims, so it is found for all
+        // suffixes first (in parallel in the OpenMP build), then the claims
+        // are made in list order.
+        std::vector<uint32_t> run_lo(suffix.size()), run_hi(suffix.size());
         for (uint32_t off = 1; off < P.max_off; ++off) {
             const uint32_t cl = L - off;
-            for (uint32_t s : suffix) {
+            const int64_t ns = (int64_t)suffix.size();
+            ORC_PARALLEL_FOR
+            for (int64_t i = 0; i < ns; ++i) {
+                const uint32_t s = suffix[i];
                 uint64_t km = xw(R, (uint64_t)s * W, P.kip + off, P.k);
                 uint32_t h = (uint32_t)(murmur(km) & mask);
                 uint32_t first = kr[h], end = kr[h + 1];
@@ -844,14 +868,19 @@ orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int st
                         count = step;
                     }
                 }
-                for (uint32_t x = first; x < end; ++x) {
+                uint32_t eq = first;
+                while (eq < end && cmp_bases(R, W, va[eq], 0, s, off, cl) == 0) ++eq;
+                run_lo[i] = first;
+                run_hi[i] = eq;
+            }
+            for (int64_t i = 0; i < ns; ++i) {
+                const uint32_t s = suffix[i];
+                for (uint32_t x = run_lo[i]; x < run_hi[i]; ++x) {
                     uint32_t p = va[x];
                     if (p == s || prev[p] != INV) continue;
-                    if (cmp_bases(R, W, p, 0, s, off, cl) == 0) {
-                        prev[p] = s;
-                        next[s] = p;
-                        offset[s] = (uint16_t)off;
-                    }
+                    prev[p] = s;
+                    next[s] = p;
+                    offset[s] = (uint16_t)off;
                     break;
                 }
             }
