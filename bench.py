@@ -356,6 +356,50 @@ def cpu_baseline(fq, L, gpu_block, sample_reads, full=True):
     return out
 
 
+def row_hashes(a):
+    """A 64-bit hash per row of a uint8 matrix (for multiset comparisons)."""
+    n, L = a.shape
+    w = (L + 7) // 8
+    pad = np.zeros((n, 8 * w), dtype=np.uint8)
+    pad[:, :L] = a
+    words = pad.view(np.uint64)
+    h = np.full(n, 0x9E3779B97F4A7C15, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        for k in range(w):
+            h = (h ^ words[:, k]) * np.uint64(0xBF58476D1CE4E5B9)
+            h ^= h >> np.uint64(31)
+    return h
+
+
+def decode_run(ctx, blk, n_reads, L, hb):
+    """curc -d's read rebuild of the same block on the GPU, from its streams
+    (order not preserved), twice; the second run is timed.  The rebuilt
+    reads must be the input's reads as a multiset."""
+    from curc_amd import archive
+
+    bs = archive.BlockStreams.from_block(blk, n_reads, L, False, False)
+    ctx.decode(bs, L, False, False)
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    ctx.device_sync()
+    t = time.perf_counter()
+    out = ctx.decode(bs, L, False, False)
+    ms = (time.perf_counter() - t) * 1e3
+    ctx.set_timing(False)
+    st = {s["name"]: s["ms"] for s in ctx.stats() if s["ms"] > 0}
+    kern = st.get("decode", 0.0)
+    inp = fastq.sequences(hb.array, L)
+    lossless = bool(np.array_equal(np.sort(row_hashes(out)), np.sort(row_hashes(inp))))
+    gb = n_reads * L / 1e9
+    res = dict(kernel_ms=round(kern, 3), gbases_s=round(gb / (kern / 1e3), 3) if kern else None,
+               host_to_host_ms=round(ms, 3), host_to_host_gbases_s=round(gb / (ms / 1e3), 3),
+               h2d_ms=round(st.get("decode_h2d", 0.0), 3), d2h_ms=round(st.get("decode_d2h", 0.0), 3),
+               reads_match_input=lossless)
+    log(f"GPU read rebuild (curc -d): kernels {kern:.1f} ms ({res['gbases_s']} Gbases/s), host to host "
+        f"{ms:.1f} ms; reads == input (multiset): {lossless}")
+    return res
+
+
 def make_fastq(ctx, cfg):
     """The workload's reads as FASTQ records in pinned host memory."""
     _, reads = ca.synth_reads(cfg)
@@ -497,6 +541,7 @@ def main():
                 "pg_bases_after_refmatch": {p: int(len(b.get(f"rm_{p}_text"))) for p in ("hq", "lq", "nlq")}}
         log(f"block with streams + self-match: {full_ms:.1f} ms; " +
             ", ".join(f"{k} {v:.1f}" for k, v in full["stages_ms"].items() if k in ("streams", "refmatch")))
+        full["decode"] = decode_run(ctx, b, a.reads, L, hb)
         b.free()
     ctx.dev_free(d_fq)
     cpu = None
@@ -556,6 +601,7 @@ def main():
         "stage_roofline": stage_roof,
         "cpu_baseline": cpu,
         "bits_per_base": bpb,
+        "decode_gbases_s": (full or {}).get("decode", {}).get("gbases_s"),
         "full_block": full,
         "batched": bat,
         "learned": learned,

@@ -70,6 +70,18 @@ class FastqJob(ctypes.Structure):
                 ("bytes2", ctypes.c_uint64), ("L", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
+class DecodeJob(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("n_reads", ctypes.c_uint64),
+                ("hq", ctypes.c_void_p), ("hq_len", ctypes.c_uint64), ("lq", ctypes.c_void_p),
+                ("lq_len", ctypes.c_uint64), ("nlq", ctypes.c_void_p), ("nlq_len", ctypes.c_uint64),
+                ("n_rec", ctypes.c_uint64), ("strand", ctypes.c_void_p), ("mis_count", ctypes.c_void_p),
+                ("mis_base", ctypes.c_void_p), ("n_mis", ctypes.c_uint64), ("mis_table", ctypes.c_void_p),
+                ("max_mm", ctypes.c_uint32), ("mis_off", ctypes.c_void_p), ("mis_off_len", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("lq_off", ctypes.c_void_p), ("n_lq", ctypes.c_uint64),
+                ("nlq_off", ctypes.c_void_p), ("n_nlq", ctypes.c_uint64), ("pe_slot", ctypes.c_void_p),
+                ("id_pos", ctypes.c_void_p)]
+
+
 BLOCK_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int)
 
 
@@ -142,6 +154,7 @@ def lib():
         "curc_fastq_stream_run": (i32, [vp, vp, u32, BLOCK_CB, vp]),
         "curc_fastq_pack": (i32, [vp, vp, u64, u32, u64, vp, vp, vp, ctypes.POINTER(u64)]),
         "curc_block_reads": (u64, [vp]),
+        "curc_decode_block": (i32, [vp, ctypes.POINTER(DecodeJob), vp, vp]),
         "curc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "curc_block_free": (None, [vp]),
         "curc_ctx_set_timing": (i32, [vp, i32]),
@@ -472,6 +485,57 @@ class Context:
                                           ctypes.c_void_p(d_packed), ctypes.c_void_p(d_isN) if d_isN else None,
                                           ctypes.c_void_p(d_seq_off), ctypes.byref(n)), "curc_fastq_pack")
         return int(n.value)
+
+    def decode(self, bs, L, preserve, paired, pe_slot=None):
+        """Read reconstruction of one block on the GPU (curc_decode_block)
+        from its streams (archive.BlockStreams, texts after the '%'
+        expansion).  Returns the reads as an (n, L) uint8 array in output
+        order (paired: file 1's line k is row 2k, file 2's row 2k+1)."""
+        from . import archive
+
+        keep = []
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(np.frombuffer(x, dtype=dt) if isinstance(x, (bytes, bytearray)) else x,
+                                     dtype=dt)
+            keep.append(a)
+            return a
+
+        def ptr(a):
+            return a.ctypes.data if a.size else None
+
+        n = int(bs.reads_count)
+        hq, lq, nlq = arr(bs.hq_text, np.uint8), arr(bs.lq_text, np.uint8), arr(bs.nlq_text, np.uint8)
+        strand, cnt = arr(bs.strand, np.uint8), arr(bs.mis_count, np.uint8)
+        base, tab = arr(bs.mis_base, np.uint8), arr(bs.mis_table, np.uint8)
+        max_mm = int(bs.max_mm)
+        offs = [arr(bs.mis_off.get(m, np.zeros(0, np.uint16)), np.uint16) for m in range(1, max_mm + 1)]
+        off_ptrs = (ctypes.c_void_p * max(1, max_mm))(*[ptr(o) for o in offs])
+        off_lens = np.array([len(o) for o in offs] or [0], dtype=np.uint64)
+        j = DecodeJob(L=L, flags=(RUN_PRESERVE_ORDER if preserve else 0) | (RUN_PAIRED if paired else 0),
+                      n_reads=n, hq=ptr(hq), hq_len=hq.size, lq=ptr(lq), lq_len=lq.size, nlq=ptr(nlq),
+                      nlq_len=nlq.size, n_rec=cnt.size, strand=ptr(strand), mis_count=ptr(cnt), mis_base=ptr(base),
+                      n_mis=base.size, mis_table=ptr(tab), max_mm=max_mm,
+                      mis_off=ctypes.cast(off_ptrs, ctypes.c_void_p), mis_off_len=off_lens.ctypes.data)
+        if preserve:
+            ip = arr(bs.id_pos, np.uint64)
+            j.id_pos = ptr(ip)
+        else:
+            ro, lo, no = arr(bs.read_off, np.uint16), arr(bs.lq_off, np.uint16), arr(bs.nlq_off, np.uint16)
+            j.read_off, j.lq_off, j.n_lq, j.nlq_off, j.n_nlq = ptr(ro), ptr(lo), lo.size, ptr(no), no.size
+            if paired:
+                ps = arr(pe_slot if pe_slot is not None else archive.pe_slots(bs, n), np.uint32)
+                j.pe_slot = ptr(ps)
+        lines = n // 2 if paired else n
+        o1 = np.empty((lines, L + 1), dtype=np.uint8)
+        o2 = np.empty((lines, L + 1), dtype=np.uint8) if paired else None
+        self._check(lib().curc_decode_block(self._h, ctypes.byref(j), o1.ctypes.data,
+                                            o2.ctypes.data if paired else None), "curc_decode_block")
+        if not paired:
+            return o1[:, :L]
+        out = np.empty((n, L), dtype=np.uint8)
+        out[0::2], out[1::2] = o1[:, :L], o2[:, :L]
+        return out
 
     def host_buffer(self, nbytes):
         """Pinned host memory of nbytes (HostBuffer)."""

@@ -199,6 +199,31 @@ def _pe_order_decode(delta, second, n):
     return out
 
 
+def pe_slots_from_order(pe_order):
+    """Output slots of a paired block whose order is not preserved, from the
+    global ids in emission order: pairs in the order their first read is
+    emitted, each as (file 1 mate, file 2 mate); the value is the emission
+    index (the same order as _pe_order_decode, computed without a loop)."""
+    pe_order = np.asarray(pe_order, dtype=np.int64)
+    n = len(pe_order)
+    idx = np.empty(n, dtype=np.int64)
+    idx[pe_order] = np.arange(n)
+    e1, e2 = idx[0::2], idx[1::2]
+    order = np.argsort(np.minimum(e1, e2), kind="stable")
+    out = np.empty(n, dtype=np.int64)
+    out[0::2], out[1::2] = e1[order], e2[order]
+    return out
+
+
+def pe_slots(bs, n):
+    """Output slots of a paired block (see pe_slots_from_order): from the
+    emission order when the streams carry it, else decoded from the
+    archive's pair stream."""
+    if getattr(bs, "pe_order", None) is not None and len(bs.pe_order):
+        return pe_slots_from_order(bs.pe_order)
+    return _pe_order_decode(bs.pe_pairs[0], bs.pe_pairs[1], n)
+
+
 def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired: bool):
     out.write(struct.pack("<I", bs.reads_count))
     out.write(bs.mis_table)

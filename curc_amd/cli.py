@@ -27,15 +27,32 @@ def compress(inputs, output_name, block_ratio=1.0, preserve_order=False, working
     out_path = os.path.join(working_dir, output_name + ".curc")
     blocks = []
     t0 = time.perf_counter()
-    parts = list(fastq.read_blocks(inputs[0], inputs[1] if paired else None, block_ratio, io_chunk))
-    L = parts[0].shape[1] if parts else None
-    total = sum(len(r) for r in parts)
+    L = fastq.read_length(inputs[0])
+    ranges = fastq.block_ranges(inputs[0], inputs[1] if paired else None, block_ratio, io_chunk)
+    total = 0
     with Context(device) as ctx:
-        done = ctx.batch_run(parts, streams=True, preserve_order=preserve_order, paired=paired, refmatch=True)
-        for i, (reads, blk) in enumerate(zip(parts, done)):
-            blocks.append(archive.BlockStreams.from_block(blk, len(reads), L, preserve_order, paired))
+        # each block's FASTQ bytes in pinned host memory, parsed on the GPU
+        jobs = []
+        for s, e in ranges:
+            bufs = []
+            for path in inputs:
+                hb = ctx.host_buffer(e - s)
+                with open(path, "rb") as f:
+                    f.seek(s)
+                    if f.readinto(memoryview(hb.array)) != e - s:
+                        raise fastq.FastqError("short read of " + path)
+                bufs.append(hb)
+            jobs.append(tuple(bufs) if paired else bufs[0])
+        done = ctx.fastq_batch(jobs, L, streams=True, preserve_order=preserve_order, paired=paired, refmatch=True)
+        for i, blk in enumerate(done):
+            n = blk.n_reads
+            total += n
+            blocks.append(archive.BlockStreams.from_block(blk, n, L, preserve_order, paired))
             blk.free()
-            log(f"block {i}: {len(reads)} reads")
+            log(f"block {i}: {n} reads")
+        for j in jobs:
+            for hb in (j if isinstance(j, tuple) else (j,)):
+                hb.free()
     archive.write_archive(out_path, blocks, preserve_order, paired, L or 0)
     size = os.path.getsize(out_path)
     log(f"reads count : {total}; blocks : {len(blocks)}; archive size : {size} bytes; "
@@ -43,13 +60,16 @@ def compress(inputs, output_name, block_ratio=1.0, preserve_order=False, working
     return out_path
 
 
-def decompress(archive_path, output_prefix=None, working_dir="."):
+def decompress(archive_path, output_prefix=None, working_dir=".", device=0):
+    """Entropy decoding and the '%' expansion run on the host; each block's
+    reads are rebuilt on the GPU (curc_decode_block)."""
     preserve, paired, L, blocks = archive.read_archive(archive_path)
     if output_prefix is None:
         output_prefix = os.path.splitext(os.path.basename(archive_path))[0]
     prefix = os.path.join(working_dir, output_prefix)
     import numpy as np
-    parts = [archive.decode_block(b, L, preserve, paired) for b in blocks]
+    with Context(device) as ctx:
+        parts = [ctx.decode(b, L, preserve, paired) for b in blocks]
     reads = np.concatenate(parts) if parts else np.zeros((0, L), np.uint8)
     if paired:
         fastq.write_seq(prefix + "_1.seq", reads[0::2])

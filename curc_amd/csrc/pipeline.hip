@@ -20,6 +20,7 @@
 
 #include "../../include/curc_gpu.h"
 #include "assembly.cuh"
+#include "decode.cuh"
 #include "fastq.cuh"
 #include "kernels.cuh"
 #include "mapphase.cuh"
@@ -57,6 +58,9 @@ enum Slot {
     S_MACT0, S_MACT1, S_MSEEN, S_MCNT, S_MSTOP, S_MX, S_MLOGIT, S_TRX, S_TRY, S_TRLAST, S_TRSLOT, S_TRCNT,
     S_OVCAND, S_OVNC, S_LB, S_OVVB, S_RECHQ, S_RECMAP, S_MK2, S_MK3, S_MV2, S_MV3, S_KRF2,
     S_FQ0, S_FQ1, S_FQSEQ, S_FQTILE,
+    S_D_TEXT, S_D_STRAND, S_D_CNT, S_D_BASE, S_D_STREAMS, S_D_SSTART, S_D_TAB, S_D_ROFF, S_D_LQ, S_D_NLQ,
+    S_D_PE, S_D_IDPOS, S_D_PV, S_D_PX, S_D_HQV, S_D_HQX, S_D_BV, S_D_BX, S_D_K0, S_D_K1, S_D_V0, S_D_V1,
+    S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -2207,6 +2211,163 @@ int fq_block_run(curc_ctx* c, const curc_fastq_job& j, int stop_after, int slot,
 
 }  // namespace
 
+namespace {
+
+// Device copy of a host array (scratch slot `slot`), or nullptr when empty.
+template <typename T>
+T* dec_upload(curc_ctx* c, int slot, const T* src, uint64_t n) {
+    T* d = c->S.as<T>(slot, n + 1);
+    if (n) h2d(c, d, src, n);
+    return d;
+}
+
+// Read reconstruction of one block (decompress.cu:909-1005, :472-577) from
+// host streams into host output lines.
+void decode_run(curc_ctx* c, const curc_decode_job& j, char* out1, char* out2) {
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    const bool preserve = (j.flags & CURC_RUN_PRESERVE_ORDER) != 0;
+    const bool paired = (j.flags & CURC_RUN_PAIRED) != 0;
+    const uint32_t L = j.L, LL = L + 1;
+    const uint64_t n = j.n_reads;
+    if (paired && (n & 1)) throw InputError("a paired block holds an even number of reads");
+    if (!preserve && j.n_rec + j.n_lq + j.n_nlq != n) throw InputError("record counts do not add up to the reads");
+    uint64_t n_mis = 0;
+    for (uint64_t r = 0; r < j.n_rec; ++r) n_mis += j.mis_count[r];
+    if (n_mis != j.n_mis) throw InputError("mismatch symbols do not match the mismatch counts");
+    uint32_t* cnt = S.as<uint32_t>(S_CNT, C_COUNT);
+    fill<uint32_t>(c, cnt, C_COUNT, 0u);
+    DecArgs a{};
+    a.L = L;
+    a.paired = paired ? 1u : 0u;
+    a.n_reads = n;
+    a.hq_len = j.hq_len;
+    a.lq_len = j.lq_len;
+    a.nlq_len = j.nlq_len;
+    a.n_mis = n_mis;
+    a.err = cnt + C_FQERR;
+    {
+        Region r(c, "decode_h2d");
+        uint8_t* text = S.as<uint8_t>(S_D_TEXT, j.hq_len + j.lq_len + j.nlq_len + 1);
+        if (j.hq_len) h2d(c, text, (const uint8_t*)j.hq, j.hq_len);
+        if (j.lq_len) h2d(c, text + j.hq_len, (const uint8_t*)j.lq, j.lq_len);
+        if (j.nlq_len) h2d(c, text + j.hq_len + j.lq_len, (const uint8_t*)j.nlq, j.nlq_len);
+        a.text = text;
+        a.strand = (const char*)dec_upload(c, S_D_STRAND, (const uint8_t*)j.strand, j.n_rec);
+        a.mis_count = dec_upload(c, S_D_CNT, j.mis_count, j.n_rec);
+        a.mis_base = dec_upload(c, S_D_BASE, j.mis_base, n_mis);
+        a.table = dec_upload(c, S_D_TAB, j.mis_table, 16);
+        if (preserve) {
+            a.id_pos = dec_upload(c, S_D_IDPOS, j.id_pos, n);
+        } else {
+            a.pe_slot = paired ? dec_upload(c, S_D_PE, j.pe_slot, n) : nullptr;
+        }
+    }
+    // offset streams, one after another; sstart[m] = first u16 of stream m
+    std::vector<uint64_t> sstart(j.max_mm + 2, 0);
+    for (uint32_t m = 1; m <= j.max_mm; ++m) sstart[m + 1] = sstart[m] + j.mis_off_len[m - 1];
+    uint16_t* streams = S.as<uint16_t>(S_D_STREAMS, sstart[j.max_mm + 1] + 1);
+    {
+        Region r(c, "decode_h2d");
+        for (uint32_t m = 1; m <= j.max_mm; ++m)
+            if (j.mis_off_len[m - 1]) h2d(c, streams + sstart[m], j.mis_off[m - 1], j.mis_off_len[m - 1]);
+    }
+    uint64_t* d_sstart = dec_upload(c, S_D_SSTART, sstart.data(), sstart.size());
+    const uint16_t* read_off = nullptr;
+    const uint16_t* lq_off = nullptr;
+    const uint16_t* nlq_off = nullptr;
+    if (!preserve) {
+        Region r(c, "decode_h2d");
+        read_off = dec_upload(c, S_D_ROFF, j.read_off, j.n_rec);
+        lq_off = dec_upload(c, S_D_LQ, j.lq_off, j.n_lq);
+        nlq_off = dec_upload(c, S_D_NLQ, j.nlq_off, j.n_nlq);
+    }
+    uint8_t* o1 = S.as<uint8_t>(S_D_OUT1, (paired ? n / 2 : n) * LL + 1);
+    uint8_t* o2 = paired ? S.as<uint8_t>(S_D_OUT2, n / 2 * LL + 1) : nullptr;
+    a.out1 = o1;
+    a.out2 = o2;
+    uint64_t hq_reads = 0;
+    {
+        Region r(c, "decode");
+        // positions (order not preserved) or HQ record indexes (preserved)
+        if (!preserve) {
+            uint64_t* pv = S.as<uint64_t>(S_D_PV, n + 1);
+            uint64_t* px = S.as<uint64_t>(S_D_PX, n + 1);
+            if (n)
+                hipLaunchKernelGGL(k_dec_pos_values, dim3(grid_for(n)), dim3(256), 0, st, read_off, j.n_rec, lq_off,
+                                   j.n_lq, nlq_off, j.n_nlq, L, pv);
+            scan_excl_t<uint64_t>(pv, px, j.n_rec, S, S_PRIM, st);
+            scan_excl_t<uint64_t>(pv + j.n_rec, px + j.n_rec, j.n_lq, S, S_PRIM, st);
+            scan_excl_t<uint64_t>(pv + j.n_rec + j.n_lq, px + j.n_rec + j.n_lq, j.n_nlq, S, S_PRIM, st);
+            a.pos_v = pv;
+            a.pos_ex = px;
+            a.nh = j.n_rec;
+            a.nl = j.n_lq;
+        } else {
+            uint64_t* hv = S.as<uint64_t>(S_D_HQV, n + 1);
+            uint64_t* hx = S.as<uint64_t>(S_D_HQX, n + 1);
+            if (n) hipLaunchKernelGGL(k_dec_hq_mask, dim3(grid_for(n)), dim3(256), 0, st, a.id_pos, n, j.hq_len, hv);
+            scan_excl_t<uint64_t>(hv, hx, n, S, S_PRIM, st);
+            a.hq_idx = hx;
+            uint64_t last[2] = {0, 0};
+            if (n) {
+                CURC_HIP(hipMemcpyAsync(&last[0], hx + n - 1, 8, hipMemcpyDeviceToHost, st));
+                CURC_HIP(hipMemcpyAsync(&last[1], hv + n - 1, 8, hipMemcpyDeviceToHost, st));
+                CURC_HIP(hipStreamSynchronize(st));
+            }
+            hq_reads = last[0] + last[1];
+            if (hq_reads != j.n_rec) throw InputError("reads placed in the HQ text do not match the records");
+        }
+        // first mismatch of each record
+        uint64_t* bv = S.as<uint64_t>(S_D_BV, j.n_rec + 1);
+        uint64_t* bx = S.as<uint64_t>(S_D_BX, j.n_rec + 1);
+        if (j.n_rec)
+            hipLaunchKernelGGL(k_dec_u8_to_u64, dim3(grid_for(j.n_rec)), dim3(256), 0, st, a.mis_count, j.n_rec, bv);
+        scan_excl_t<uint64_t>(bv, bx, j.n_rec, S, S_PRIM, st);
+        a.bidx = bx;
+        // mismatch offsets in record order
+        uint16_t* flat = S.as<uint16_t>(S_D_FLAT, n_mis + 1);
+        if (j.n_rec && n_mis) {
+            uint32_t* k0 = S.as<uint32_t>(S_D_K0, j.n_rec + 1);
+            uint32_t* k1 = S.as<uint32_t>(S_D_K1, j.n_rec + 1);
+            uint32_t* v0 = S.as<uint32_t>(S_D_V0, j.n_rec + 1);
+            uint32_t* v1 = S.as<uint32_t>(S_D_V1, j.n_rec + 1);
+            uint64_t* gs = S.as<uint64_t>(S_D_GS, 257);
+            hipLaunchKernelGGL(k_dec_count_keys, dim3(grid_for(j.n_rec)), dim3(256), 0, st, a.mis_count, j.n_rec, k0,
+                               v0);
+            radix_sort_pairs<uint32_t>(k0, v0, k1, v1, j.n_rec, 0, 8, S, S_PRIM, st);
+            hipLaunchKernelGGL(k_dec_group_start, dim3(grid_for(j.n_rec)), dim3(256), 0, st, (const uint32_t*)k0,
+                               j.n_rec, gs);
+            hipLaunchKernelGGL(k_dec_offsets, dim3(grid_for(j.n_rec)), dim3(256), 0, st, (const uint32_t*)k0,
+                               (const uint32_t*)v0, j.n_rec, (const uint64_t*)gs, (const uint16_t*)streams,
+                               (const uint64_t*)d_sstart, (const uint64_t*)bx, L, j.max_mm, flat, a.err);
+        }
+        a.flat_off = flat;
+        const uint64_t wgs = (n + DEC_LINES - 1) / DEC_LINES;
+        if (wgs) {
+            if (preserve)
+                hipLaunchKernelGGL(k_dec_reads<true>, dim3((uint32_t)wgs), dim3(256), DEC_LINES * LL, st, a);
+            else
+                hipLaunchKernelGGL(k_dec_reads<false>, dim3((uint32_t)wgs), dim3(256), DEC_LINES * LL, st, a);
+        }
+        c->add_bytes("decode", (double)n * (L + LL) + (double)j.n_rec * 18 + (double)n_mis * 5,
+                     (double)n);
+    }
+    uint32_t e = 0;
+    CURC_HIP(hipMemcpyAsync(&e, a.err, 4, hipMemcpyDeviceToHost, st));
+    CURC_HIP(hipStreamSynchronize(st));
+    if (e) throw InputError("corrupt streams (offsets or positions out of range)");
+    {
+        Region r(c, "decode_d2h");
+        const uint64_t lines1 = paired ? n / 2 : n;
+        if (lines1) CURC_HIP(hipMemcpyAsync(out1, o1, lines1 * LL, hipMemcpyDeviceToHost, st));
+        if (paired && lines1) CURC_HIP(hipMemcpyAsync(out2, o2, lines1 * LL, hipMemcpyDeviceToHost, st));
+    }
+    CURC_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
 extern "C" {
 
 int curc_make_params(uint32_t L, curc_params* p) {
@@ -2690,6 +2851,28 @@ int curc_fastq_pack(curc_ctx* c, const char* d_fq, uint64_t bytes, uint32_t L, u
         int code = CURC_EHIP;
         if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
         else if (dynamic_cast<const std::length_error*>(&e)) code = CURC_ELIMIT;
+        return fail(c, code, e.what());
+    }
+    return CURC_OK;
+}
+
+int curc_decode_block(curc_ctx* c, const curc_decode_job* j, char* out1, char* out2) {
+    if (!c || !j || (j->n_reads && !out1)) return CURC_EINVAL;
+    if (j->L == 0 || j->L > 512) return fail(c, CURC_EINVAL, "unsupported read length");
+    if ((j->flags & CURC_RUN_PAIRED) && j->n_reads && !out2) return fail(c, CURC_EINVAL, "paired output needs out2");
+    if (j->flags & CURC_RUN_INPUT_ON_DEVICE) return fail(c, CURC_EINVAL, "decode inputs are host arrays");
+    if (j->max_mm > 255 || (j->max_mm && (!j->mis_off || !j->mis_off_len)))
+        return fail(c, CURC_EINVAL, "mismatch-offset streams missing");
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        decode_run(c, *j, out1, out2);
+        c->harvest();
+    } catch (const std::exception& e) {
+        hipStreamSynchronize(c->st);
+        c->pending.clear();
+        int code = CURC_EHIP;
+        if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
+        else if (std::string(e.what()).find("out of memory") != std::string::npos) code = CURC_ENOMEM;
         return fail(c, code, e.what());
     }
     return CURC_OK;

@@ -297,6 +297,51 @@ uint64_t curc_block_reads(const curc_block *b);
 int curc_fastq_pack(curc_ctx *ctx, const char *d_fq, uint64_t bytes, uint32_t L, uint64_t max_reads,
                     uint64_t *d_packed, uint8_t *d_isN, uint64_t *d_seq_off, uint64_t *n_reads);
 
+/* ---- decompression: read reconstruction -----------------------------------
+ * The read rebuild of curc -d (decompress_block, decompress.cu:909-994, and
+ * preserve_order_process, :472-577) on the device, from a block's decoded
+ * streams (after the entropy decoders and the '%' expansion of the texts,
+ * recoverRef, :418-444, which stay on the host).  Every read is a slice of
+ * the HQ, LQ or N-read LQ text; a read of the HQ text gets its mismatch
+ * symbols (decoder rows for contexts A, T, C, G) at its decoded offsets and,
+ * on strand '1', is reverse complemented.  Output lines are L bases and a
+ * newline: order not preserved, HQ/mapped records in stream order, then LQ,
+ * then N-LQ (pairs regrouped by pe_slot when paired); order preserved, by
+ * read id.  Paired output goes to out1 / out2 (n_reads / 2 lines each),
+ * single-end output to out1 (n_reads lines).  All pointers are host memory.
+ * Inconsistent streams (counts that do not add up, offsets or positions out
+ * of range) fail with CURC_EINVAL. */
+typedef struct curc_decode_job {
+    uint32_t L;
+    uint32_t flags;             /* CURC_RUN_PRESERVE_ORDER, CURC_RUN_PAIRED */
+    uint64_t n_reads;           /* reads_count of the block */
+    const char *hq;             /* texts after the '%' expansion */
+    uint64_t hq_len;
+    const char *lq;
+    uint64_t lq_len;
+    const char *nlq;
+    uint64_t nlq_len;
+    uint64_t n_rec;             /* HQ and mapped records */
+    const char *strand;         /* n_rec '0'/'1' */
+    const uint8_t *mis_count;   /* n_rec */
+    const uint8_t *mis_base;    /* n_mis recoded mismatch symbols (0..3) */
+    uint64_t n_mis;
+    const uint8_t *mis_table;   /* 16 bytes */
+    uint32_t max_mm;            /* floor(L / 5) */
+    const uint16_t *const *mis_off; /* stream for m mismatches at [m - 1], m = 1..max_mm */
+    const uint64_t *mis_off_len;    /* u16 values in each */
+    /* order not preserved */
+    const uint16_t *read_off;   /* n_rec position deltas */
+    const uint16_t *lq_off;     /* n_lq, L - delta */
+    uint64_t n_lq;
+    const uint16_t *nlq_off;    /* n_nlq, L - delta */
+    uint64_t n_nlq;
+    const uint32_t *pe_slot;    /* paired: stream index of the read of each output slot (2k file 1, 2k+1 file 2) */
+    /* order preserved */
+    const uint64_t *id_pos;     /* n_reads joined-text positions, by read id */
+} curc_decode_job;
+int curc_decode_block(curc_ctx *ctx, const curc_decode_job *job, char *out1, char *out2);
+
 /* ---- learned candidate scorer ----------------------------------------------
  * No counterpart in the reference.  A 32 -> 128 -> 128 -> 1 network (ReLU,
  * bfloat16 matrix products on the MFMA units, float32 accumulation and

@@ -128,3 +128,14 @@ def test_paired_order_covers_all_reads():
     o = orc.OracleBlock(reads, stop="all")
     o.streams(False, True)
     assert sorted(o.get("s_pe_order").tolist()) == list(range(len(reads)))
+
+
+def test_pe_slots_match_the_pair_stream():
+    """The vectorised output slots of a paired block equal the sequential
+    decode of the archive's pair stream."""
+    from curc_amd import archive
+
+    rng = np.random.default_rng(3)
+    order = rng.permutation(2000)
+    d, s = archive._pe_order_encode(order)
+    assert np.array_equal(archive.pe_slots_from_order(order), archive._pe_order_decode(d, s, 2000))
