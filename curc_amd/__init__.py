@@ -153,6 +153,7 @@ def lib():
         "curc_fastq_batch_run": (i32, [vp, vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(i32)]),
         "curc_fastq_stream_run": (i32, [vp, vp, u32, BLOCK_CB, vp]),
         "curc_fastq_pack": (i32, [vp, vp, u64, u32, u64, vp, vp, vp, ctypes.POINTER(u64)]),
+        "curc_fastq_pack_static": (i32, [vp, vp, u64, u32, u64, vp, vp, vp, ctypes.POINTER(u64)]),
         "curc_block_reads": (u64, [vp]),
         "curc_decode_block": (i32, [vp, ctypes.POINTER(DecodeJob), vp, vp]),
         "curc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
@@ -536,6 +537,17 @@ class Context:
         out = np.empty((n, L), dtype=np.uint8)
         out[0::2], out[1::2] = o1[:, :L], o2[:, :L]
         return out
+
+    def fastq_pack_static(self, d_fq, nbytes, L, max_reads, d_packed, d_isN, d_seq_off):
+        """As fastq_pack, through the entry that allocates nothing
+        (curc_fastq_pack_static)."""
+        n = ctypes.c_uint64()
+        self._check(lib().curc_fastq_pack_static(self._h, ctypes.c_void_p(d_fq), nbytes, L, max_reads,
+                                                 ctypes.c_void_p(d_packed),
+                                                 ctypes.c_void_p(d_isN) if d_isN else None,
+                                                 ctypes.c_void_p(d_seq_off), ctypes.byref(n)),
+                    "curc_fastq_pack_static")
+        return int(n.value)
 
     def host_buffer(self, nbytes):
         """Pinned host memory of nbytes (HostBuffer)."""

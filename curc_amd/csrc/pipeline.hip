@@ -32,6 +32,31 @@
 
 using namespace curc;
 
+// Fixed device scratch of curc_fastq_pack_static, for callers that cannot
+// allocate: counters in the first 256 bytes, then the parse's tile counts.
+// Each GPU holds its own copy for the life of the process; its address on
+// a device is looked up once (static_scratch_init) and used from then on.
+static constexpr uint64_t STATIC_SCRATCH_WORDS = 1ull << 20;  // 8 MiB
+static constexpr uint64_t STATIC_CNT_WORDS = 32;               // 256 bytes of u32 counters
+static __device__ __align__(256) uint64_t scratch[STATIC_SCRATCH_WORDS];
+
+static constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static uint64_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Resolves `scratch` on the current device (once per device); returns the
+// device address, or null when the lookup fails.
+static uint64_t* static_scratch_init(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = static_cast<uint64_t*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
 namespace {
 
 enum Slot {
@@ -74,6 +99,9 @@ enum Cnt {
     C_FQERR,
     C_COUNT = 32
 };
+static_assert(C_COUNT * 4 <= STATIC_CNT_WORDS * 8, "counters exceed the fixed scratch's counter area");
+static_assert(CURC_FASTQ_STATIC_MAX_BYTES / FQ_TILE + 2 <= STATIC_SCRATCH_WORDS - STATIC_CNT_WORDS,
+              "tile counts of the largest accepted input exceed the fixed scratch");
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap = 8192) {
     uint64_t g = (n + 255) / 256;
@@ -2399,6 +2427,8 @@ curc_ctx* curc_ctx_create(int device) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return nullptr;
     if (hipSetDevice(device) != hipSuccess) return nullptr;
+    // the fixed scratch of curc_fastq_pack_static on this device
+    if (!static_scratch_init(device)) return nullptr;
     curc_ctx* c = new curc_ctx();
     c->device = device;
     const char* ix = std::getenv("CURC_INDEX");
@@ -2846,6 +2876,38 @@ int curc_fastq_pack(curc_ctx* c, const char* d_fq, uint64_t bytes, uint32_t L, u
         in.bytes[0] = bytes;
         uint32_t* cnt = c->S.as<uint32_t>(S_CNT, C_COUNT);
         *n_reads = fq_pack(c, in, L, P.W, max_reads, d_packed, d_isN, d_seq_off, cnt, nullptr, 0);
+    } catch (const std::exception& e) {
+        hipStreamSynchronize(c->st);
+        int code = CURC_EHIP;
+        if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
+        else if (dynamic_cast<const std::length_error*>(&e)) code = CURC_ELIMIT;
+        return fail(c, code, e.what());
+    }
+    return CURC_OK;
+}
+
+// curc_fastq_pack without allocation: counters and tile counts live in the
+// fixed device array `scratch` (through its per-device address), so no
+// device or host memory is allocated.  Calls that overlap on one GPU share
+// that array and race.
+int curc_fastq_pack_static(curc_ctx* c, const char* d_fq, uint64_t bytes, uint32_t L, uint64_t max_reads,
+                           uint64_t* d_packed, uint8_t* d_isN, uint64_t* d_seq_off, uint64_t* n_reads) {
+    if (!c || !n_reads || (bytes && (!d_fq || !d_packed || !d_seq_off))) return CURC_EINVAL;
+    *n_reads = 0;
+    curc_params P;
+    if (curc_make_params(L, &P) != CURC_OK) return fail(c, CURC_EINVAL, "unsupported read length");
+    if (bytes > CURC_FASTQ_STATIC_MAX_BYTES)
+        return fail(c, CURC_ELIMIT, "FASTQ bytes beyond the fixed scratch maximum (CURC_FASTQ_STATIC_MAX_BYTES)");
+    uint64_t* base = static_scratch_init(c->device);  // resolved when the context was created
+    if (!base) return fail(c, CURC_EHIP, "fixed scratch not resolved on this device");
+    try {
+        CURC_HIP(hipSetDevice(c->device));
+        FqInput in;
+        in.fq[0] = (const uint8_t*)d_fq;
+        in.bytes[0] = bytes;
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(base);
+        *n_reads = fq_pack(c, in, L, P.W, max_reads, d_packed, d_isN, d_seq_off, cnt, base + STATIC_CNT_WORDS,
+                           STATIC_SCRATCH_WORDS - STATIC_CNT_WORDS);
     } catch (const std::exception& e) {
         hipStreamSynchronize(c->st);
         int code = CURC_EHIP;

@@ -296,6 +296,16 @@ uint64_t curc_block_reads(const curc_block *b);
  * CURC_ELIMIT when the bytes hold more than max_reads records. */
 int curc_fastq_pack(curc_ctx *ctx, const char *d_fq, uint64_t bytes, uint32_t L, uint64_t max_reads,
                     uint64_t *d_packed, uint8_t *d_isN, uint64_t *d_seq_off, uint64_t *n_reads);
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  curc_fastq_pack with the same arguments and results, but
+ * allocating nothing: its counters and tile counts live in a fixed 8 MiB
+ * device array that each GPU holds until the process ends.  Inputs of more
+ * than CURC_FASTQ_STATIC_MAX_BYTES bytes fail with CURC_ELIMIT.  Calls that
+ * overlap on one GPU share that array and race: serialise them. */
+#define CURC_FASTQ_STATIC_MAX_BYTES 16000000000ull
+int curc_fastq_pack_static(curc_ctx *ctx, const char *d_fq, uint64_t bytes, uint32_t L, uint64_t max_reads,
+                           uint64_t *d_packed, uint8_t *d_isN, uint64_t *d_seq_off, uint64_t *n_reads);
 
 /* ---- decompression: read reconstruction -----------------------------------
  * The read rebuild of curc -d (decompress_block, decompress.cu:909-994, and
