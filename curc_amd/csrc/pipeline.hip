@@ -158,6 +158,7 @@ struct Tuning {
 
 struct curc_ctx {
     int device = 0;
+    uint32_t cus = 256;  // compute units of the device (hipDeviceProp_t::multiProcessorCount)
     Tuning tune;
     hipStream_t st = nullptr;
     // copies of results to pinned host memory, ordered after the compute
@@ -576,7 +577,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
                                ranged ? (const uint32_t*)rng : nullptr, bp, off);
             hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, lb_status, lb_words);
             // three workgroups per CU (the cooperative sort's LDS allows three)
-            hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(768), dim3(256), 0, c->st, cs, off);
+            hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(3 * c->cus), dim3(256), 0, c->st, cs, off);
             if (lb) {
                 compact2(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], n_max,
                          np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
@@ -1622,7 +1623,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                                            (const uint64_t*)sw, P.k, P.kip, o);
                         hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, st, cs, o);
                     }
-                    hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(1024), dim3(256), 0, st, cs, o);
+                    hipLaunchKernelGGL((k_claim_resolve<false, Enc2>), dim3(4 * c->cus), dim3(256), 0, st, cs, o);
                 }
             };
             GraphKey key;
@@ -2431,6 +2432,11 @@ curc_ctx* curc_ctx_create(int device) {
     if (!static_scratch_init(device)) return nullptr;
     curc_ctx* c = new curc_ctx();
     c->device = device;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->cus = (uint32_t)prop.multiProcessorCount;
+    }
     const char* ix = std::getenv("CURC_INDEX");
     c->index_csr = ix && std::string(ix) == "csr";
     if (const char* t = std::getenv("CURC_TUNE")) parse_tuning(t, c->tune);

@@ -136,8 +136,11 @@ def block_case(n, L, glen, seed, err=0.01, nrate=0.0):
     return reads, orc.OracleBlock(to_array(reads), stop="all")
 
 
+# the last three take the k = 32 branch (L >= 89) that every benchmark read length uses:
+# k-mer index position 3 (L = 100), 21 (L = 150), and 2 at L = 96
 CASES = [(120, 40, 300, 1, 0.01, 0.0), (200, 36, 150, 2, 0.0, 0.0), (150, 50, 400, 3, 0.02, 0.01),
-         (90, 45, 90, 4, 0.0, 0.0), (160, 64, 700, 5, 0.005, 0.002)]
+         (90, 45, 90, 4, 0.0, 0.0), (160, 64, 700, 5, 0.005, 0.002), (150, 96, 400, 6, 0.005, 0.002),
+         (150, 100, 500, 7, 0.01, 0.0), (150, 150, 600, 8, 0.005, 0.002)]
 
 
 @pytest.mark.parametrize("case", CASES)
