@@ -164,6 +164,8 @@ def lib():
         "curc_synth_default": (None, [ctypes.POINTER(SynthCfg)]),
         "curc_synth_genome": (i32, [ctypes.POINTER(SynthCfg), vp]),
         "curc_synth_reads": (i32, [ctypes.POINTER(SynthCfg), vp, vp]),
+        "curc_synth_fastq_record_len": (u64, [u32, ctypes.c_char_p, i32]),
+        "curc_synth_fastq": (i32, [vp, u64, u32, u64, u64, ctypes.c_char_p, i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -214,6 +216,30 @@ def synth_reads(cfg, genome=None):
     if rc != CURC_OK:
         raise CurcError(f"curc_synth_reads failed ({rc})")
     return genome, out
+
+
+def synth_fastq(reads, out=None, first_id=0, mate=0, stride_reads=1, prefix=b"synth."):
+    """FASTQ records of reads ((n, L) uint8; every stride_reads-th row, so
+    stride_reads=2 writes one mate of interleaved pairs) in the layout of
+    fastq.fastq_bytes, written in parallel into out (a uint8 array, e.g. a
+    pinned HostBuffer's) or a new array; returns the filled array."""
+    reads = np.ascontiguousarray(reads, dtype=np.uint8)
+    total, L = reads.shape
+    n = (total + stride_reads - 1) // stride_reads if total else 0
+    rec = lib().curc_synth_fastq_record_len(L, prefix, mate)
+    if out is None:
+        out = np.empty(n * rec, dtype=np.uint8)
+    if len(out) < n * rec:
+        raise ValueError("output buffer too small")
+    rc = lib().curc_synth_fastq(reads.ctypes.data if n else None, n, L, L * stride_reads, first_id, prefix, mate,
+                                out.ctypes.data if n else None)
+    if rc != CURC_OK:
+        raise CurcError(f"curc_synth_fastq failed ({rc})")
+    return out[: n * rec]
+
+
+def synth_fastq_record_len(L, mate=0, prefix=b"synth."):
+    return int(lib().curc_synth_fastq_record_len(L, prefix, mate))
 
 
 # workload presets (sizes as the reference's headline configurations)

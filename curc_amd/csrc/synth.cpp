@@ -198,4 +198,46 @@ int curc_synth_reads(const curc_synth_cfg* c, const char* g, char* out) {
     return CURC_OK;
 }
 
+// FASTQ records of n reads (read i at reads + i * stride) with the layout of
+// curc_amd.fastq.fastq_bytes: "@<prefix><id, 10 digits>[/mate]\n", the
+// sequence, "\n+\n", L quality bytes 'I', "\n".  Records have a fixed
+// length, so each is written at its own offset (in parallel).
+uint64_t curc_synth_fastq_record_len(uint32_t L, const char* prefix, int mate) {
+    return 1 + strlen(prefix) + 10 + (mate > 0 ? 2 : 0) + 1 + L + 3 + L + 1;
+}
+
+int curc_synth_fastq(const char* reads, uint64_t n, uint32_t L, uint64_t stride, uint64_t first_id, const char* prefix,
+                     int mate, char* out) {
+    if ((!reads && n) || (!out && n) || !prefix || L == 0 || stride < L || mate < 0 || mate > 9) return CURC_EINVAL;
+    const uint64_t plen = strlen(prefix);
+    const uint64_t rec = curc_synth_fastq_record_len(L, prefix, mate);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        char* o = out + (uint64_t)i * rec;
+        *o++ = '@';
+        memcpy(o, prefix, plen);
+        o += plen;
+        uint64_t id = first_id + (uint64_t)i;
+        for (int d = 9; d >= 0; --d) {
+            o[d] = (char)('0' + id % 10);
+            id /= 10;
+        }
+        o += 10;
+        if (mate > 0) {
+            *o++ = '/';
+            *o++ = (char)('0' + mate);
+        }
+        *o++ = '\n';
+        memcpy(o, reads + (uint64_t)i * stride, L);
+        o += L;
+        *o++ = '\n';
+        *o++ = '+';
+        *o++ = '\n';
+        memset(o, 'I', L);
+        o += L;
+        *o = '\n';
+    }
+    return CURC_OK;
+}
+
 }  // extern "C"

@@ -29,32 +29,47 @@ class FastqError(ValueError):
     pass
 
 
-def _cut(buf, start, size):
-    """Backward scan of preprocess.cpp:114-138 over the chunk
-    buf[start:start+size] (not the input's last chunk): returns the chunk's
-    new size.  Candidates are positions e (scanning down from size minus the
-    first line's length) with a newline before and '@' at e; each candidate's
-    bytes are compared with the chunk's first line, and the candidate is
-    taken as soon as more than five comparisons in a row were equal.  The
-    run counter is reset only by an unequal byte, not between candidates,
-    exactly as in the reference.  Positions without a candidate leave the
-    counter alone, so only candidates are visited (rfind)."""
-    if buf[start:start + 1] != b"@":
+_WINDOW = 1 << 16
+
+
+def _cut(get, start, size):
+    """Backward scan of preprocess.cpp:114-138 over the chunk [start,
+    start+size) of an input (not its last chunk); get(a, b) returns the
+    input's bytes [a, b).  Returns the chunk's new size.  Candidates are
+    positions e (scanning down from size minus the first line's length) with
+    a newline before and '@' at e; each candidate's bytes are compared with
+    the chunk's first line, and the candidate is taken as soon as more than
+    five comparisons in a row were equal.  The run counter is reset only by
+    an unequal byte, not between candidates, exactly as in the reference.
+    Positions without a candidate leave the counter alone, so only candidates
+    are visited (found with rfind in windows read backwards)."""
+    head, first_len, n = b"", -1, 256
+    while first_len < 0 and len(head) < size:
+        head = get(start, start + min(size, n))
+        first_len = head.find(b"\n")
+        n *= 4
+    if head[:1] != b"@":
         raise FastqError("io_buffer doesn't start with @")  # preprocess.cpp:116
-    nl = buf.find(b"\n", start, start + size)
-    if nl < 0:
+    if first_len < 0:
         raise FastqError("no line end in an I/O chunk")
-    first_len = nl - start
-    first = buf[start:start + first_len]
+    first = head[:first_len]
     e = size - first_len
     equal = 0
+    win_lo, win = None, b""
     while e > 0:
-        # the next candidate at or below e: '\n' at e-1 and '@' at e
-        k = buf.rfind(b"\n@", start, start + e + 1)
+        # the largest candidate at or below e: '\n' at e-1, '@' at e
+        if win_lo is None or e - 1 < win_lo:
+            win_lo = max(0, e + 1 - _WINDOW)
+            win = get(start + win_lo, start + e + 1)
+        k = win.rfind(b"\n@", 0, e + 1 - win_lo)
         if k < 0:
-            break
-        e = k + 1 - start
-        cand = buf[start + e:start + e + first_len]
+            if win_lo == 0:
+                break
+            e = win_lo  # a pair straddling the window's start is candidate e = win_lo
+            win_lo = None
+            continue
+        e = win_lo + k + 1
+        cand = get(start + e, start + e + first_len)
         for j in range(first_len):
             if first[j] == cand[j]:
                 equal += 1
@@ -68,45 +83,75 @@ def _cut(buf, start, size):
                      "or headers without a common prefix of six bytes)")
 
 
-def block_ranges(path1, path2=None, block_ratio=1.0, io_chunk=IO_CHUNK):
-    """Byte ranges [(start, end), ...] of the blocks of an input, following
-    process() (preprocess.cpp:72-243): chunks of io_chunk bytes (half of it
-    per file when paired), each but the last cut back by _cut, blocks closed
-    after the chunk that takes the block's bytes past
-    block_bytes = fastq_bytes * block_ratio (both doubled when paired)."""
+class _Input:
+    """A FASTQ input: a file path (memory-mapped) or a buffer in memory
+    (bytes, bytearray or uint8 array)."""
+
+    def __init__(self, src):
+        self._f = self._mm = None
+        if isinstance(src, (str, os.PathLike)):
+            self.size = os.path.getsize(src)
+            if self.size:
+                self._f = open(src, "rb")
+                self._mm = mmap.mmap(self._f.fileno(), 0, access=mmap.ACCESS_READ)
+            self._buf = self._mm
+        else:
+            self._buf = memoryview(src).cast("B") if not isinstance(src, np.ndarray) else src
+            self.size = len(self._buf)
+
+    def get(self, a, b):
+        return bytes(self._buf[a:b])
+
+    def close(self):
+        if self._mm is not None:
+            self._mm.close()
+        if self._f is not None:
+            self._f.close()
+
+
+def block_ranges(src1, src2=None, block_ratio=1.0, io_chunk=IO_CHUNK):
+    """Byte ranges [(start, end), ...] of the blocks of an input (file paths
+    or in-memory buffers), following process() (preprocess.cpp:72-243):
+    chunks of io_chunk bytes (half of it per file when paired), each but the
+    last cut back by _cut, blocks closed after the chunk that takes the
+    block's bytes past block_bytes = fastq_bytes * block_ratio (both doubled
+    when paired).  File 2 of a paired input is cut at file 1's offsets."""
     if not 0 < block_ratio <= 1:
         raise ValueError("block_ratio must be in (0, 1]")  # main.cpp:111-115
-    paired = path2 is not None
-    size = os.path.getsize(path1)
-    if paired and os.path.getsize(path2) != size:
-        raise FastqError("paired-end file size isn't equal")  # preprocess.cpp:83
-    if size == 0:
-        return []
-    block_bytes = int((2.0 if paired else 1.0) * float(size) * block_ratio)
-    chunk = io_chunk // 2 if paired else io_chunk
-    out = []
-    with open(path1, "rb") as f:
-        buf = mmap.mmap(f.fileno(), 0, access=mmap.ACCESS_READ)
-        try:
-            total = last = 0
-            while total < size:
-                start = total
-                n = min(chunk, size - total)
-                total += n
-                if total < size:
-                    cut = _cut(buf, start, n)
-                    total -= n - cut
-                blk = total - last
-                if paired:
-                    blk *= 2
-                if blk > block_bytes:
-                    out.append((last, total))
-                    last = total
-            if total - last > 0:
+    paired = src2 is not None
+    a = _Input(src1)
+    try:
+        size = a.size
+        if paired:
+            b = _Input(src2)
+            size2 = b.size
+            b.close()
+            if size2 != size:
+                raise FastqError("paired-end file size isn't equal")  # preprocess.cpp:83
+        if size == 0:
+            return []
+        block_bytes = int((2.0 if paired else 1.0) * float(size) * block_ratio)
+        chunk = io_chunk // 2 if paired else io_chunk
+        out = []
+        total = last = 0
+        while total < size:
+            start = total
+            n = min(chunk, size - total)
+            total += n
+            if total < size:
+                cut = _cut(a.get, start, n)
+                total -= n - cut
+            blk = total - last
+            if paired:
+                blk *= 2
+            if blk > block_bytes:
                 out.append((last, total))
-        finally:
-            buf.close()
-    return out
+                last = total
+        if total - last > 0:
+            out.append((last, total))
+        return out
+    finally:
+        a.close()
 
 
 def read_length(path):

@@ -216,6 +216,13 @@ typedef struct curc_synth_cfg {
 void curc_synth_default(curc_synth_cfg *cfg);
 int curc_synth_genome(const curc_synth_cfg *cfg, char *genome);
 int curc_synth_reads(const curc_synth_cfg *cfg, const char *genome, char *reads);
+/* FASTQ records of n reads (read i at reads + i * stride): header
+ * "@<prefix><first_id + i, 10 digits>" (with "/<mate>" when mate > 0), the
+ * sequence, "+", L quality bytes 'I'; every record is
+ * curc_synth_fastq_record_len bytes. */
+uint64_t curc_synth_fastq_record_len(uint32_t L, const char *prefix, int mate);
+int curc_synth_fastq(const char *reads, uint64_t n, uint32_t L, uint64_t stride, uint64_t first_id,
+                     const char *prefix, int mate, char *out);
 
 /* RefMatcher on its own (compress.cu:630-1001): RefMatcher::init on src
  * (the HQ pseudogenome text) and one RefMatcher::match of dest (a text with

@@ -211,7 +211,8 @@ def parse_fastq(fq1, L, fq2=None, kind="st"):
     (n, L) uint8 array in id order (paired: 2k from fq1, 2k+1 from fq2)."""
     a1 = np.frombuffer(fq1, dtype=np.uint8) if not isinstance(fq1, np.ndarray) else fq1
     a2 = None if fq2 is None else (np.frombuffer(fq2, dtype=np.uint8) if not isinstance(fq2, np.ndarray) else fq2)
-    cap = (len(a1) + (0 if a2 is None else len(a2))) // (L + 5) + 1
+    # a record takes at least L + 4 bytes (empty header and quality lines)
+    cap = (len(a1) + (0 if a2 is None else len(a2))) // (L + 4) + 1
     out = np.empty((cap, L), dtype=np.uint8)
     n = ctypes.c_uint64()
     rc = lib(kind).orc_parse_fastq(a1.ctypes.data if len(a1) else None, len(a1),

@@ -153,3 +153,33 @@ def test_no_cut_is_an_error(tmp_path):
     p.write_bytes(d)
     with pytest.raises(fastq.FastqError, match="no record start"):
         fastq.block_ranges(str(p), None, 1.0, 500)
+
+
+@pytest.mark.parametrize("window", [7, 23, 64, 300])
+def test_block_cut_with_small_search_windows(tmp_path, monkeypatch, window):
+    """The backward scan reads the chunk in windows; with windows shorter than
+    a record the search crosses many window starts, and the result must still
+    be the reference loop's."""
+    monkeypatch.setattr(fastq, "_WINDOW", window)
+    L = 60
+    r = random_reads(80, L, seed=window)
+    d = records(r, lambda k: b"synth.%d extra=%d/1" % (k, 3 * k))
+    check_against_reference(d, None, 0.3, 1900, tmp_path)
+    d = records(r, lambda k: b"ab.%d%06d/1" % (1 + k % 9, k))
+    check_against_reference(d, None, 0.3, 1200, tmp_path)
+
+
+def test_block_ranges_on_buffers_equal_files(tmp_path):
+    L = 50
+    r1 = random_reads(70, L, seed=12)
+    r2 = random_reads(70, L, seed=13)
+    d1 = records(r1, lambda k: b"synth.%d/1" % k)
+    d2 = records(r2, lambda k: b"synth.%d/2" % k)
+    p1, p2 = tmp_path / "b_1.fastq", tmp_path / "b_2.fastq"
+    p1.write_bytes(d1)
+    p2.write_bytes(d2)
+    want = fastq.block_ranges(str(p1), str(p2), 0.25, 1700)
+    assert fastq.block_ranges(d1, d2, 0.25, 1700) == want
+    assert fastq.block_ranges(np.frombuffer(d1, dtype=np.uint8), np.frombuffer(d2, dtype=np.uint8), 0.25,
+                              1700) == want
+    assert fastq.block_ranges(b"", None) == []
