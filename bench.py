@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-full-block", action="store_true")
     p.add_argument("--no-bits-per-base", action="store_true")
     p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--no-repeat-pe", action="store_true",
+                   help="skip the 100M-read repeat-heavy paired-end configuration")
     p.add_argument("--no-learned", action="store_true")
     p.add_argument("--no-tune", action="store_true",
                    help="keep the default launch shapes instead of autotuning them on the block first")
@@ -273,6 +275,7 @@ def batched(ins, reps=2, learned=True):
             learned_mapped = mapped(blocks)
             ctx.set_scorer(None)
             net.close()
+    out_bits = batch_bits(ins, learned)
     out = dict(inputs=len(ins), gbases=round(gbases, 4), sequential_s=round(seq, 4), batched_s=round(bat, 4),
                sequential_gbases_s=round(gbases / seq, 4), batched_gbases_s=round(gbases / bat, 4),
                speedup=round(seq / bat, 3), mapped_reads=plain_mapped, workers=workers,
@@ -282,7 +285,43 @@ def batched(ins, reps=2, learned=True):
     if lb is not None:
         out.update(learned_batched_s=round(lb, 4), learned_batched_gbases_s=round(gbases / lb, 4),
                    learned_batched_vs_plain=round(seq / lb, 3), learned_batched_vs_batched=round(bat / lb, 3),
-                   learned_mapped_reads=learned_mapped)
+                   scorer_gain_alone=round(bat / lb, 3), learned_mapped_reads=learned_mapped)
+    out["bits_per_base"] = out_bits
+    return out
+
+
+def batch_bits(ins, learned=True):
+    """bits/base over the many-input workload (one .curc archive per input,
+    summed), plain and learned, from batches with streams and self-match."""
+    from curc_amd import archive, scorer, train_scorer
+
+    t = time.perf_counter()
+    bases = sum(r.size for r in ins)
+
+    def total(ctx, lrn):
+        size = 0
+        blocks = ctx.batch_run(ins, workers=0, streams=True, refmatch=True, learned=lrn)
+        for r, b in zip(ins, blocks):
+            bs = archive.BlockStreams.from_block(b, len(r), r.shape[1], False, False)
+            b.free()
+            size += archive.archive_bytes([bs], False, False, r.shape[1])
+        return size
+
+    with ca.Context(0) as ctx:
+        plain = total(ctx, False)
+        out = dict(plain=round(8.0 * plain / bases, 5), plain_archive_bytes=plain)
+        if learned:
+            params, thr = train_scorer.load()
+            net = scorer.Mlp(ctx, params)
+            ctx.set_scorer(net, thr)
+            lsize = total(ctx, True)
+            ctx.set_scorer(None)
+            net.close()
+            out.update(learned=round(8.0 * lsize / bases, 5), learned_archive_bytes=lsize,
+                       learned_vs_plain=round(lsize / plain, 5))
+    out["seconds"] = round(time.perf_counter() - t, 1)
+    log(f"  bits/base over the {len(ins)} inputs: plain {out['plain']}" +
+        (f", learned {out['learned']} ({out['learned_vs_plain']}x)" if learned else "") + f" ({out['seconds']} s)")
     return out
 
 
@@ -398,6 +437,98 @@ def decode_run(ctx, blk, n_reads, L, hb):
     log(f"GPU read rebuild (curc -d): kernels {kern:.1f} ms ({res['gbases_s']} Gbases/s), host to host "
         f"{ms:.1f} ms; reads == input (multiset): {lossless}")
     return res
+
+
+def repeat_pe_run(reps=2):
+    """The repeat-heavy paired-end configuration at its stated size (500 Mbp,
+    45% repeats, 100M x 100 bp pairs, order preserved, block_ratio 0.25 at
+    the 50 MB chunk granularity -> 4 uneven blocks), as one batch from FASTQ
+    bytes in pinned host memory (curc_fastq_batch_run, 4 streams); then each
+    block alone, and the 4 blocks streamed in order on one stream."""
+    from curc_amd import workloads
+
+    t0 = time.perf_counter()
+    _, reads = ca.synth_reads(workloads.config_repeat_pe())
+    L = reads.shape[1]
+    n = len(reads)
+    with ca.Context(0) as ctx:
+        f1, f2 = workloads.pinned_fastq(ctx, reads, True)
+        del reads
+        ranges = fastq.block_ranges(f1.array, f2.array, 0.25)
+        jobs = [(f1.array[s:e], f2.array[s:e]) for s, e in ranges]
+        log(f"repeat-heavy PE input: {n} reads, 2 x {f1.nbytes / 1e9:.2f} GB FASTQ, {len(ranges)} blocks, "
+            f"built in {time.perf_counter() - t0:.1f} s")
+        kw = dict(streams=True, preserve_order=True, paired=True)
+        for b in ctx.fastq_batch(jobs, L, workers=4, **kw):  # warm-up (scratch, pinned result chunks)
+            b.free()
+        times = []
+        counts = None
+        for _ in range(reps):
+            ctx.device_sync()
+            t = time.perf_counter()
+            blocks = ctx.fastq_batch(jobs, L, workers=4, **kw)
+            ctx.device_sync()
+            times.append(time.perf_counter() - t)
+            counts = [b.n_reads for b in blocks]
+            for b in blocks:
+                b.free()
+        single = []
+        for j in jobs:
+            ctx.device_sync()
+            t = time.perf_counter()
+            ctx.fastq_block(j[0], j[1], L=L, **kw).free()
+            single.append(round((time.perf_counter() - t) * 1e3, 1))
+        ctx.device_sync()
+        t = time.perf_counter()
+        ctx.fastq_stream(jobs, L, lambda i, b: b.free(), **kw)
+        stream_s = time.perf_counter() - t
+        f1.free()
+        f2.free()
+    gb = n * L / 1e9
+    bs = min(times)
+    out = dict(workload="500 Mbp genome, 45% repeats (300 bp-6 kbp, 5-15% divergence), 2 x 100 bp pairs, "
+                        "fragment ~N(400,50), 20x, 0.5% subst, preserve_order, block_ratio 0.25, "
+                        "from pinned FASTQ bytes, streams built",
+               reads=n, gbases=round(gb, 3), blocks=len(ranges), block_reads=counts,
+               batch_s=round(bs, 4), gbases_s=round(gb / bs, 3), batch_runs_s=[round(x, 4) for x in times],
+               block_alone_ms=single, stream_in_order_s=round(stream_s, 4),
+               stream_in_order_gbases_s=round(gb / stream_s, 3))
+    log(f"  batch of {len(ranges)} blocks: {bs:.3f} s ({gb / bs:.2f} Gbases/s); blocks alone {single} ms; "
+        f"in order on one stream {stream_s:.3f} s")
+    return out
+
+
+def block_bits(ctx, plain_blk, run, n_reads, L, learned=True):
+    """bits/base of the block's .curc archive (GPU streams + self-match,
+    xz-coded sub-files), plain and with mapping under the trained scorer, with
+    the learned archive's reads rebuilt on the GPU and checked against the
+    plain one's as a multiset."""
+    from curc_amd import archive, scorer, train_scorer
+
+    t = time.perf_counter()
+    bs = archive.BlockStreams.from_block(plain_blk, n_reads, L, False, False)
+    size = archive.archive_bytes([bs], False, False, L)
+    out = dict(plain=round(8.0 * size / (n_reads * L), 5), plain_archive_bytes=size)
+    if learned:
+        params, thr = train_scorer.load()
+        net = scorer.Mlp(ctx, params)
+        ctx.set_scorer(net, thr)
+        lb = run(streams=True, refmatch=True, learned=True)
+        ctx.set_scorer(None)
+        net.close()
+        lbs = archive.BlockStreams.from_block(lb, n_reads, L, False, False)
+        lsize = archive.archive_bytes([lbs], False, False, L)
+        a = ctx.decode(lbs, L, False, False)
+        b = ctx.decode(bs, L, False, False)
+        same = bool(np.array_equal(np.sort(row_hashes(a)), np.sort(row_hashes(b))))
+        lb.free()
+        out.update(learned=round(8.0 * lsize / (n_reads * L), 5), learned_archive_bytes=lsize,
+                   learned_vs_plain=round(lsize / size, 5), learned_lossless=same)
+    out["seconds"] = round(time.perf_counter() - t, 1)
+    log(f"  bits/base of this block: plain {out['plain']}" +
+        (f", learned {out['learned']} ({out['learned_vs_plain']}x, lossless {out['learned_lossless']})"
+         if learned else "") + f" ({out['seconds']} s)")
+    return out
 
 
 def make_fastq(ctx, cfg):
@@ -542,6 +673,8 @@ def main():
         log(f"block with streams + self-match: {full_ms:.1f} ms; " +
             ", ".join(f"{k} {v:.1f}" for k, v in full["stages_ms"].items() if k in ("streams", "refmatch")))
         full["decode"] = decode_run(ctx, b, a.reads, L, hb)
+        if not a.no_bits_per_base:
+            full["bits_per_base"] = block_bits(ctx, b, run_dev, a.reads, L, learned=not a.no_learned)
         b.free()
     ctx.dev_free(d_fq)
     cpu = None
@@ -571,6 +704,10 @@ def main():
             f"batched {bat['batched_s']} s ({bat['speedup']}x)" +
             (f", learned+batched {bat['learned_batched_s']} s ({bat['learned_batched_vs_plain']}x)"
              if "learned_batched_s" in bat else ""))
+    c3 = None
+    if not a.no_repeat_pe:
+        log("repeat-heavy paired-end configuration ...")
+        c3 = repeat_pe_run()
     out = {
         "metric": "Gbases/s (pseudogenome build+mapping, 1 GPU)",
         "value": round(value, 4),
@@ -604,6 +741,7 @@ def main():
         "decode_gbases_s": (full or {}).get("decode", {}).get("gbases_s"),
         "full_block": full,
         "batched": bat,
+        "c3": c3,
         "learned": learned,
     }
     print(json.dumps(out))

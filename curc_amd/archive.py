@@ -15,8 +15,9 @@ decompress.cu:760-1060):
                  unmap_n_ref_off.map, unmap_n_ref_len.map, mismatch_count, mismatch_base,
                u8 max_mismatch, mismatch_off_1 .. mismatch_off_max (each u64 size + payload)
 
-The pre-entropy streams are CURC's.  Each sub-file payload here is an xz
-(LZMA2) stream; the pseudogenome text is first packed to 2 bits per base
+The pre-entropy streams are CURC's.  Each sub-file payload here is a run of
+xz (LZMA2) streams over 8 MiB chunks (u32 count, then u64 size + stream per
+chunk), coded in parallel; the pseudogenome text is first packed to 2 bits per base
 with an exception list for other symbols.  The reference's coders (PgRC's
 PPMd/LZMA framing, VarLenDNA + fast-lzma2, bsc, a range coder) are
 third-party code this package does not carry, so archives are not
@@ -24,6 +25,7 @@ byte-compatible with CURC's; the streams they hold are.
 """
 import io
 import lzma
+import os
 import struct
 
 import numpy as np
@@ -42,16 +44,81 @@ MAP_NAMES = ("ref_off", "ref_len", "unmap_ref_off", "unmap_ref_len", "unmap_n_re
 
 
 # ------------------------------------------------------------------ coders ---
-def xz(data: bytes) -> bytes:
+# A sub-file payload is split into chunks of XZ_CHUNK bytes, each its own xz
+# stream: u32 chunk count, then u64 size + xz bytes per chunk.  The chunks of
+# a block are (de)compressed on a thread pool (the lzma module releases the
+# GIL), so archive time scales with cores; XZ_CHUNK matches the dictionary of
+# preset 6, so chunking costs little ratio.
+XZ_CHUNK = 8 << 20
+_POOL = None
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+
+        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    return _POOL
+
+
+def _xz1(data: bytes) -> bytes:
     return lzma.compress(data, format=lzma.FORMAT_XZ, preset=XZ_PRESET)
 
 
-def unxz(data: bytes) -> bytes:
+def _unxz1(data: bytes) -> bytes:
     return lzma.decompress(data, format=lzma.FORMAT_XZ)
+
+
+def xz_many(payloads):
+    """Chunked xz containers of several payloads, compressed in parallel."""
+    pieces = [[p[i:i + XZ_CHUNK] for i in range(0, len(p), XZ_CHUNK)] for p in payloads]
+    flat = [c for ps in pieces for c in ps]
+    comp = list(_pool().map(_xz1, flat))
+    out, k = [], 0
+    for ps in pieces:
+        parts = [struct.pack("<I", len(ps))]
+        for c in comp[k:k + len(ps)]:
+            parts += [struct.pack("<Q", len(c)), c]
+        k += len(ps)
+        out.append(b"".join(parts))
+    return out
+
+
+def unxz_many(containers):
+    """Inverse of xz_many."""
+    spans = []
+    for data in containers:
+        (n,) = struct.unpack_from("<I", data, 0)
+        off, cs = 4, []
+        for _ in range(n):
+            (ln,) = struct.unpack_from("<Q", data, off)
+            cs.append(data[off + 8:off + 8 + ln])
+            off += 8 + ln
+        spans.append(cs)
+    flat = list(_pool().map(_unxz1, [c for cs in spans for c in cs]))
+    out, k = [], 0
+    for cs in spans:
+        out.append(b"".join(flat[k:k + len(cs)]))
+        k += len(cs)
+    return out
+
+
+def xz(data: bytes) -> bytes:
+    return xz_many([data])[0]
+
+
+def unxz(data: bytes) -> bytes:
+    return unxz_many([data])[0]
 
 
 def pack_dna(text: bytes) -> bytes:
     """2-bit ACGT packing with (position, byte) exceptions, then xz."""
+    return xz(pack_dna_raw(text))
+
+
+def pack_dna_raw(text: bytes) -> bytes:
+    """2-bit ACGT packing with (position, byte) exceptions."""
     a = np.frombuffer(text, dtype=np.uint8)
     code = np.full(256, 255, dtype=np.uint8)
     code[[65, 67, 71, 84]] = [0, 1, 2, 3]
@@ -62,12 +129,14 @@ def pack_dna(text: bytes) -> bytes:
     c4 = np.concatenate([c, np.zeros(pad, dtype=np.uint8)]).reshape(-1, 4)
     packed = (c4[:, 0] << 6) | (c4[:, 1] << 4) | (c4[:, 2] << 2) | c4[:, 3]
     head = struct.pack("<QQ", len(a), len(exc))
-    body = head + exc.astype("<u8").tobytes() + a[exc].tobytes() + packed.astype(np.uint8).tobytes()
-    return xz(body)
+    return head + exc.astype("<u8").tobytes() + a[exc].tobytes() + packed.astype(np.uint8).tobytes()
 
 
 def unpack_dna(blob: bytes) -> bytes:
-    body = unxz(blob)
+    return unpack_dna_raw(unxz(blob))
+
+
+def unpack_dna_raw(body: bytes) -> bytes:
     n, ne = struct.unpack_from("<QQ", body, 0)
     off = 16
     exc = np.frombuffer(body, dtype="<u8", count=ne, offset=off)
@@ -234,29 +303,36 @@ def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired
     texts = bs.rm_texts if bs.rm_texts is not None else (bs.hq_text, bs.lq_text, bs.nlq_text)
     out.write(struct.pack("<QQQ", *(len(t) for t in texts)))
 
+    # raw sub-file payloads in file order; all are compressed together
+    head = []
+    if preserve:
+        dt = "<u4" if joined <= 0xFFFFFFFF else "<u8"
+        head.append(np.asarray(bs.id_pos).astype(dt).tobytes())
+    head.append(bytes(bs.strand))
+    head.append(pack_dna_raw(b"".join(texts)))
+    if not preserve:
+        head.append(np.asarray(bs.read_off, dtype="<u2").tobytes())
+        head.append(np.asarray(bs.nlq_off, dtype="<u2").tobytes())
+        head.append(np.asarray(bs.lq_off, dtype="<u2").tobytes())
+        if paired:
+            d, s = _pe_order_encode(bs.pe_order)
+            head.append(struct.pack("<Q", len(d)) + d + s)
+    for k in MAP_NAMES:
+        head.append(bytes(bs.maps[k]))
+    head.append(np.asarray(bs.mis_count, dtype=np.uint8).tobytes())
+    head.append(np.asarray(bs.mis_base, dtype=np.uint8).tobytes())
+    tail = [np.asarray(bs.mis_off.get(m, np.zeros(0)), dtype="<u2").tobytes() for m in range(1, bs.max_mm + 1)]
+    comp = xz_many(head + tail)
+
     def sub(payload: bytes):
         out.write(struct.pack("<Q", len(payload)))
         out.write(payload)
 
-    if preserve:
-        dt = "<u4" if joined <= 0xFFFFFFFF else "<u8"
-        sub(xz(np.asarray(bs.id_pos).astype(dt).tobytes()))
-    sub(xz(bs.strand))
-    sub(pack_dna(b"".join(texts)))
-    if not preserve:
-        sub(xz(np.asarray(bs.read_off, dtype="<u2").tobytes()))
-        sub(xz(np.asarray(bs.nlq_off, dtype="<u2").tobytes()))
-        sub(xz(np.asarray(bs.lq_off, dtype="<u2").tobytes()))
-        if paired:
-            d, s = _pe_order_encode(bs.pe_order)
-            sub(xz(struct.pack("<Q", len(d)) + d + s))
-    for k in MAP_NAMES:
-        sub(xz(bs.maps[k]))
-    sub(xz(np.asarray(bs.mis_count, dtype=np.uint8).tobytes()))
-    sub(xz(np.asarray(bs.mis_base, dtype=np.uint8).tobytes()))
+    for c in comp[:len(head)]:
+        sub(c)
     out.write(struct.pack("<B", bs.max_mm))
-    for m in range(1, bs.max_mm + 1):
-        sub(xz(np.asarray(bs.mis_off.get(m, np.zeros(0)), dtype="<u2").tobytes()))
+    for c in comp[len(head):]:
+        sub(c)
 
 
 def read_block(inp: io.BufferedIOBase, preserve: bool, paired: bool, L: int) -> BlockStreams:
@@ -274,29 +350,33 @@ def read_block(inp: io.BufferedIOBase, preserve: bool, paired: bool, L: int) -> 
         (n,) = rd("<Q")
         return inp.read(n)
 
+    n_head = (1 if preserve else 0) + 2 + ((3 + (1 if paired else 0)) if not preserve else 0) + len(MAP_NAMES) + 2
+    raw = [sub() for _ in range(n_head)]
+    (max_mm,) = rd("<B")
+    raw += [sub() for _ in range(max_mm)]
+    dec = iter(unxz_many(raw))
     id_pos = None
     if preserve:
         dt = "<u4" if join_std else "<u8"
-        id_pos = np.frombuffer(unxz(sub()), dtype=dt).astype(np.uint64)
-    strand = unxz(sub())
-    text = unpack_dna(sub())
+        id_pos = np.frombuffer(next(dec), dtype=dt).astype(np.uint64)
+    strand = next(dec)
+    text = unpack_dna_raw(next(dec))
     read_off = lq_off = nlq_off = pe_order = None
     pe_pairs = None
     if not preserve:
-        read_off = np.frombuffer(unxz(sub()), dtype="<u2")
-        nlq_off = np.frombuffer(unxz(sub()), dtype="<u2")
-        lq_off = np.frombuffer(unxz(sub()), dtype="<u2")
+        read_off = np.frombuffer(next(dec), dtype="<u2")
+        nlq_off = np.frombuffer(next(dec), dtype="<u2")
+        lq_off = np.frombuffer(next(dec), dtype="<u2")
         if paired:
-            raw = unxz(sub())
-            (nd,) = struct.unpack_from("<Q", raw, 0)
-            pe_pairs = (raw[8:8 + nd], raw[8 + nd:])
+            pr = next(dec)
+            (nd,) = struct.unpack_from("<Q", pr, 0)
+            pe_pairs = (pr[8:8 + nd], pr[8 + nd:])
     maps = {}
     for k in MAP_NAMES:
-        maps[k] = unxz(sub())
-    mis_count = np.frombuffer(unxz(sub()), dtype=np.uint8)
-    mis_base = np.frombuffer(unxz(sub()), dtype=np.uint8)
-    (max_mm,) = rd("<B")
-    mis_off = {m: np.frombuffer(unxz(sub()), dtype="<u2") for m in range(1, max_mm + 1)}
+        maps[k] = next(dec)
+    mis_count = np.frombuffer(next(dec), dtype=np.uint8)
+    mis_base = np.frombuffer(next(dec), dtype=np.uint8)
+    mis_off = {m: np.frombuffer(next(dec), dtype="<u2") for m in range(1, max_mm + 1)}
     # decompress.cpp:602-608: the HQ text first (its references point into
     # itself), then the LQ and N-read LQ texts against it
     rm = (text[:hq_len], text[hq_len:hq_len + lq_len], text[hq_len + lq_len:])
@@ -410,3 +490,12 @@ def decode_block(bs: BlockStreams, L: int, preserve: bool, paired: bool):
     out[hq_ids] = _apply_records(bs.hq_text, id_pos[hq_ids], bs.strand, bs.mis_count, bs.mis_base, bs.mis_off,
                                  bs.mis_table, L)
     return out
+
+
+def archive_bytes(blocks, preserve, paired, L):
+    """Size of the .curc archive of the given BlockStreams (written in memory)."""
+    buf = io.BytesIO()
+    buf.write(struct.pack("<BBHB", int(preserve), int(paired), L, len(blocks)))
+    for bs in blocks:
+        write_block(buf, bs, preserve, paired)
+    return buf.tell()

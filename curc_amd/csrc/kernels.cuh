@@ -403,6 +403,7 @@ struct ClaimState {
     uint64_t* kpool;        // a8: first reversed-prefix word of each pooled member (or null)
     uint8_t* slinked;       // per suffix index: linked at this offset
     uint8_t* plinked;       // per prefix position: linked at this offset (a8 only)
+    uint64_t* work;         // a8: suffix-list lengths summed over offsets (or null)
     __device__ __forceinline__ void link(uint32_t si, uint32_t px, uint32_t off) const {
         const uint32_t s = sl[si], p = pl[px];
         next[s] = p;
@@ -622,7 +623,7 @@ __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_
                                                     const uint64_t* __restrict__ sw, uint32_t k, uint32_t kip,
                                                     uint32_t off, uint32_t per, uint32_t* __restrict__ cand,
                                                     uint32_t* __restrict__ ncand, const uint64_t* __restrict__ vb,
-                                                    uint64_t nw) {
+                                                    uint64_t nw, uint32_t* __restrict__ passed) {
     __shared__ uint32_t qi[OVQ_CAP];
     __shared__ uint32_t qh[OVQ_CAP];
     __shared__ uint32_t qtail, nout;
@@ -706,7 +707,34 @@ __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_
     const uint32_t t = qtail;
     if (t > head) candidate(head + threadIdx.x < t, head + threadIdx.x);
     __syncthreads();
-    if (threadIdx.x == 0) ncand[blockIdx.x] = nout;
+    if (threadIdx.x == 0) {
+        ncand[blockIdx.x] = nout;
+        // suffixes of this workgroup's segment that searched their bucket, summed over offsets
+        if (passed) passed[blockIdx.x] += t;
+    }
+}
+
+// Overlap probes of a static suffix list: a suffix linked at offset o was
+// probed at offsets 1..o, an unlinked one at every offset below max_off.
+__global__ void __launch_bounds__(256) k_ov_probe_count(const uint32_t* __restrict__ sl, uint32_t ns,
+                                                        const uint16_t* __restrict__ off, uint32_t max_off,
+                                                        uint64_t* __restrict__ out) {
+    uint64_t acc = 0;
+    GRID_STRIDE(i, ns) {
+        const uint16_t o = off[sl[i]];
+        acc += o == INVOFF ? (uint64_t)(max_off - 1) : (uint64_t)o;
+    }
+    // 64-bit totals per wave, one atomic per wave
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((unsigned long long*)out, (unsigned long long)acc);
+}
+
+__global__ void __launch_bounds__(256) k_sum_u32(const uint32_t* __restrict__ a, uint32_t n,
+                                                 uint64_t* __restrict__ out) {
+    uint64_t acc = 0;
+    GRID_STRIDE(i, n) acc += a[i];
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((unsigned long long*)out, (unsigned long long)acc);
 }
 
 // Finalisation over the proposers k_ov_probe_q listed (same rules as
@@ -737,8 +765,14 @@ __global__ void __launch_bounds__(256) k_ov_finalize_q(ClaimState cs, uint32_t o
 template <class E>
 __device__ __forceinline__ void fm_probe_body(const ClaimState& cs, const uint32_t* __restrict__ rng, uint32_t bp,
                                               uint32_t off) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
     const uint32_t ns = *cs.d_ns;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *cs.res_n = *cs.pool_n = 0;
+        if (cs.work) {
+            cs.work[0] += ns;
+            if (off == 1) cs.work[1] = ns;
+        }
+    }
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(E::BPW, len);
     const uint32_t np = *cs.d_np;

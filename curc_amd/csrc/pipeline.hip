@@ -85,7 +85,7 @@ enum Slot {
     S_FQ0, S_FQ1, S_FQSEQ, S_FQTILE,
     S_D_TEXT, S_D_STRAND, S_D_CNT, S_D_BASE, S_D_STREAMS, S_D_SSTART, S_D_TAB, S_D_ROFF, S_D_LQ, S_D_NLQ,
     S_D_PE, S_D_IDPOS, S_D_PV, S_D_PX, S_D_HQV, S_D_HQX, S_D_BV, S_D_BX, S_D_K0, S_D_K1, S_D_V0, S_D_V1,
-    S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2,
+    S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2, S_OVPASS, S_WORK,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -172,6 +172,9 @@ struct curc_ctx {
     // uploads of the next FASTQ block while the current one runs
     // (curc_fastq_stream_run); created on first use
     hipStream_t ust = nullptr;
+    // overlap probes and bucket searches of the last block (copied from the
+    // device after the loop; outlives the block's stack frame)
+    uint64_t ov_work[2] = {0, 0};
     bool side_copies = true;
     // context (own stream, arena and counters) that runs the N-read stage
     // beside the LQ stage of a single block; created on first use
@@ -487,7 +490,7 @@ void records_to_host(curc_ctx* c, const AsmOut& a, const uint32_t* id_map, uint6
 template <class E>
 void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const uint32_t* d_pre,
                 const uint32_t* d_suf, uint32_t* cnt, uint32_t n_max, uint32_t* next, uint32_t* prev,
-                uint16_t* offset) {
+                uint16_t* offset, const char* stage) {
     constexpr uint32_t BITS = (E::BPW == 32) ? 2 : 3;
     // block width: about one block per list entry, at most 2^20 block values;
     // short lists (<= FM_SHORT) go without block ranges, searching the whole
@@ -527,6 +530,9 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     fill<uint32_t>(c, owner, (uint64_t)n_max + 1, INV);
     fill<uint32_t>(c, head, (uint64_t)n_max + 1, INV);
     fill<uint32_t>(c, cont, (uint64_t)n_max + 1, 0u);
+    // work: suffix-list lengths summed over offsets, the first offset's length
+    uint64_t* work = S.as<uint64_t>(S_WORK, 4);
+    fill<uint64_t>(c, work, 4, 0ull);
     uint32_t* np[2] = {cnt + C_NP0, cnt + C_NP1};
     uint32_t* ns[2] = {cnt + C_NS0, cnt + C_NS1};
     CURC_HIP(hipMemcpyAsync(np[0], cnt + C_PRE, 4, hipMemcpyDeviceToDevice, c->st));
@@ -540,7 +546,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
         // short lists: the whole offset loop in one single-workgroup launch
         ClaimState c0{R, W, L, sl[0], ns[0], pl[0], pk[0], np[0], next, prev, offset,
                       choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
-                      kpool, slinked, plinked};
+                      kpool, slinked, plinked, work};
         ClaimState c1 = c0;
         c1.sl = sl[1];
         c1.d_ns = ns[1];
@@ -572,7 +578,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
                                    (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, nb, rng, plinked);
             ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
                           choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
-                          kpool, slinked, plinked};
+                          kpool, slinked, plinked, work};
             hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs,
                                ranged ? (const uint32_t*)rng : nullptr, bp, off);
             hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, lb_status, lb_words);
@@ -603,8 +609,18 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     key.v[8] = (uint64_t)offset;
     key.v[9] = (uint64_t)cnt;
     c->graphs.run(key, S, c->st, loop);
-    if (lb && read_cnt(c, &lb->err) != 0)
-        throw std::runtime_error("exact matching: list compaction look-back did not complete");
+    uint64_t w[2] = {0, 0};
+    uint32_t fin = 0, lb_err = 0;
+    CURC_HIP(hipMemcpyAsync(w, work, 16, hipMemcpyDeviceToHost, c->st));
+    CURC_HIP(hipMemcpyAsync(&fin, ns[(L - 1) & 1], 4, hipMemcpyDeviceToHost, c->st));
+    if (lb) CURC_HIP(hipMemcpyAsync(&lb_err, &lb->err, 4, hipMemcpyDeviceToHost, c->st));
+    CURC_HIP(hipStreamSynchronize(c->st));
+    if (lb_err != 0) throw std::runtime_error("exact matching: list compaction look-back did not complete");
+    // suffix-list reads per offset (the read's words and its id, first word) plus one
+    // prefix read per link
+    const double P8 = 8.0 * W;
+    c->add_bytes(stage, (double)w[0] * (P8 + 8) + (double)(w[1] - std::min<uint64_t>(w[1], fin)) * (4 + P8),
+                 (double)w[0]);
 }
 
 // Record sort and stream builders (a13; compress.cu:1730-2044), on the
@@ -1189,7 +1205,7 @@ NlqOut nlq_stage(curc_ctx* x, hipEvent_t wait, const uint8_t* Nascii, uint64_t n
                 n_nlq);
         compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st,
                 n_nlq);
-        full_match<Enc3>(x, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff);
+        full_match<Enc3>(x, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff, "nlq");
         out.a = assemble(x, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt, S_NLQIDS);
         const AsmOut& a = out.a;
         uint8_t* npg = S.as<uint8_t>(S_NPG, a.len + 16);
@@ -1479,6 +1495,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         return;
     }
 
+    bool ov_counted = false;
     // ---- a5: prefix k-mer index (compress.cu:1427-1471)
     uint32_t* va = S.as<uint32_t>(S_VA, (uint64_t)n + 1);
     uint32_t* sl0 = S.as<uint32_t>(S_SL0, (uint64_t)n + 1);
@@ -1598,6 +1615,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 qcand = S.as<uint32_t>(S_OVCAND, (uint64_t)nsuf + 1);
                 qn = S.as<uint32_t>(S_OVNC, (uint64_t)gq);
             }
+            uint32_t* passed = queued ? S.as<uint32_t>(S_OVPASS, (uint64_t)gq) : nullptr;
+            if (passed) fill<uint32_t>(c, passed, gq, 0u);
             auto loop = [&]() {
                 for (uint32_t o = 1; o < P.max_off; ++o) {
                     ClaimState cs{packed, W, L, sl[0], cnt + C_NSUF, va, vk, cnt + C_NPRE, next, prev, off,
@@ -1607,14 +1626,14 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         hipLaunchKernelGGL(k_ov_probe_q<true>, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
                                            (const PrefixSlot*)pslots, (const uint32_t*)nullptr, fmask, mask,
                                            (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn,
-                                           (const uint64_t*)vb, vnw);
+                                           (const uint64_t*)vb, vnw, passed);
                         hipLaunchKernelGGL(k_ov_finalize_q, dim3(gq), dim3(256), 0, st, cs, o, per,
                                            (const uint32_t*)qcand, (const uint32_t*)qn);
                     } else if (queued) {
                         hipLaunchKernelGGL(k_ov_probe_q<false>, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
                                            (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
                                            (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn,
-                                           (const uint64_t*)nullptr, (uint64_t)0);
+                                           (const uint64_t*)nullptr, (uint64_t)0, passed);
                         hipLaunchKernelGGL(k_ov_finalize_q, dim3(gq), dim3(256), 0, st, cs, o, per,
                                            (const uint32_t*)qcand, (const uint32_t*)qn);
                     } else {
@@ -1642,6 +1661,19 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             key.v[12] = ((uint64_t)queued << 32) | (queued ? per : g);
             key.v[13] = vb != nullptr;
             c->graphs.run(key, S, st, loop);
+            // work of the loop: probes (suffix x offset) and bucket searches
+            // (probes that passed the k-mer filter); read back with the
+            // partition's counts
+            uint64_t* ovw = S.as<uint64_t>(S_WORK, 4);
+            fill<uint64_t>(c, ovw, 2, 0ull);
+            if (nsuf)
+                hipLaunchKernelGGL(k_ov_probe_count, dim3(grid_for(nsuf)), dim3(256), 0, st, (const uint32_t*)sl[0],
+                                   nsuf, (const uint16_t*)off, P.max_off, ovw);
+            if (passed)
+                hipLaunchKernelGGL(k_sum_u32, dim3(grid_for(gq)), dim3(256), 0, st, (const uint32_t*)passed, gq,
+                                   ovw + 1);
+            CURC_HIP(hipMemcpyAsync(c->ov_work, ovw, 16, hipMemcpyDeviceToHost, st));
+            ov_counted = true;
         }
     }
     if (keep) {
@@ -1674,6 +1706,9 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     }
     n_hq = read_cnt(c, cnt + C_HQ);
     n_unm = read_cnt(c, cnt + C_UNM);
+    if (ov_counted)  // per probe: the suffix's k-mer, the bucket's bounds; per search: a prefix's id and words
+        c->add_bytes("overlap", (double)c->ov_work[0] * 32 + (double)c->ov_work[1] * (4 + 8.0 * W),
+                     (double)c->ov_work[0]);
     if (keep) {
         d2h(c, B->next_pr, next, N);
         d2h(c, B->prev_pr, prev, N);
@@ -1698,7 +1733,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         set_cnt(c, cnt + C_TMP, n_hq);
         compact(cnt + C_TMP, PredLinkState{hq, prev}, EmitGather{hq, pre}, cnt + C_PRE, S, S_PRIM, st, n_hq);
         compact(cnt + C_TMP, PredLinkState{hq, next}, EmitGather{hq, suf}, cnt + C_SUF, S, S_PRIM, st, n_hq);
-        full_match<Enc2>(c, packed, W, L, pre, suf, cnt, n_hq, next, prev, off);
+        full_match<Enc2>(c, packed, W, L, pre, suf, cnt, n_hq, next, prev, off, "full_match");
     }
     if (keep || stop <= CURC_STAGE_FULLMATCH) {
         d2h(c, B->next, next, N);
@@ -1984,7 +2019,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         hipLaunchKernelGGL(k_iota, dim3(grid_for(n_lq)), dim3(256), 0, st, ids, (uint64_t)n_lq);
         set_cnt(c, cnt + C_PRE, n_lq);
         set_cnt(c, cnt + C_SUF, n_lq);
-        full_match<Enc2>(c, lqp, W, L, ids, ids, cnt, n_lq, lnext, lprev, loff);
+        full_match<Enc2>(c, lqp, W, L, ids, ids, cnt, n_lq, lnext, lprev, loff, "lq");
         if (keep) {
             d2h(c, B->lq_next, lnext, n_lq);
             d2h(c, B->lq_prev, lprev, n_lq);
