@@ -299,13 +299,13 @@ def batch_bits(ins, learned=True):
     bases = sum(r.size for r in ins)
 
     def total(ctx, lrn):
-        size = 0
         blocks = ctx.batch_run(ins, workers=0, streams=True, refmatch=True, learned=lrn)
+        arch = []
         for r, b in zip(ins, blocks):
-            bs = archive.BlockStreams.from_block(b, len(r), r.shape[1], False, False)
+            arch.append(([archive.BlockStreams.from_block(b, len(r), r.shape[1], False, False)], False, False,
+                         r.shape[1]))
             b.free()
-            size += archive.archive_bytes([bs], False, False, r.shape[1])
-        return size
+        return sum(archive.archive_sizes(arch))
 
     with ca.Context(0) as ctx:
         plain = total(ctx, False)

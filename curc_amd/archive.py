@@ -293,17 +293,16 @@ def pe_slots(bs, n):
     return _pe_order_decode(bs.pe_pairs[0], bs.pe_pairs[1], n)
 
 
-def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired: bool):
-    out.write(struct.pack("<I", bs.reads_count))
-    out.write(bs.mis_table)
+def block_parts(bs: BlockStreams, preserve: bool, paired: bool):
+    """A block's fixed header bytes and its raw sub-file payloads, in file
+    order (the u8 max_mm goes after the first len(head) of them)."""
+    hdr = [struct.pack("<I", bs.reads_count), bytes(bs.mis_table)]
     joined = len(bs.hq_text) + len(bs.lq_text) + len(bs.nlq_text)
     if preserve:
-        out.write(struct.pack("<B", 1 if joined <= 0xFFFFFFFF else 0))
-    out.write(struct.pack("<B", 1 if len(bs.hq_text) <= 0xFFFFFFFF else 0))
+        hdr.append(struct.pack("<B", 1 if joined <= 0xFFFFFFFF else 0))
+    hdr.append(struct.pack("<B", 1 if len(bs.hq_text) <= 0xFFFFFFFF else 0))
     texts = bs.rm_texts if bs.rm_texts is not None else (bs.hq_text, bs.lq_text, bs.nlq_text)
-    out.write(struct.pack("<QQQ", *(len(t) for t in texts)))
-
-    # raw sub-file payloads in file order; all are compressed together
+    hdr.append(struct.pack("<QQQ", *(len(t) for t in texts)))
     head = []
     if preserve:
         dt = "<u4" if joined <= 0xFFFFFFFF else "<u8"
@@ -322,17 +321,24 @@ def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired
     head.append(np.asarray(bs.mis_count, dtype=np.uint8).tobytes())
     head.append(np.asarray(bs.mis_base, dtype=np.uint8).tobytes())
     tail = [np.asarray(bs.mis_off.get(m, np.zeros(0)), dtype="<u2").tobytes() for m in range(1, bs.max_mm + 1)]
-    comp = xz_many(head + tail)
+    return b"".join(hdr), head, tail, bs.max_mm
 
-    def sub(payload: bytes):
-        out.write(struct.pack("<Q", len(payload)))
-        out.write(payload)
 
-    for c in comp[:len(head)]:
-        sub(c)
-    out.write(struct.pack("<B", bs.max_mm))
-    for c in comp[len(head):]:
-        sub(c)
+def _write_parts(out, hdr, comp_head, comp_tail, max_mm):
+    out.write(hdr)
+    for c in comp_head:
+        out.write(struct.pack("<Q", len(c)))
+        out.write(c)
+    out.write(struct.pack("<B", max_mm))
+    for c in comp_tail:
+        out.write(struct.pack("<Q", len(c)))
+        out.write(c)
+
+
+def write_block(out: io.BufferedIOBase, bs: BlockStreams, preserve: bool, paired: bool):
+    hdr, head, tail, max_mm = block_parts(bs, preserve, paired)
+    comp = xz_many(head + tail)  # all sub-files of the block compressed together
+    _write_parts(out, hdr, comp[:len(head)], comp[len(head):], max_mm)
 
 
 def read_block(inp: io.BufferedIOBase, preserve: bool, paired: bool, L: int) -> BlockStreams:
@@ -493,9 +499,22 @@ def decode_block(bs: BlockStreams, L: int, preserve: bool, paired: bool):
 
 
 def archive_bytes(blocks, preserve, paired, L):
-    """Size of the .curc archive of the given BlockStreams (written in memory)."""
-    buf = io.BytesIO()
-    buf.write(struct.pack("<BBHB", int(preserve), int(paired), L, len(blocks)))
-    for bs in blocks:
-        write_block(buf, bs, preserve, paired)
-    return buf.tell()
+    """Size of the .curc archive of the given BlockStreams."""
+    return archive_sizes([(blocks, preserve, paired, L)])[0]
+
+
+def archive_sizes(archives):
+    """Sizes of several .curc archives, each given as (blocks, preserve,
+    paired, L), with all their sub-files compressed in one parallel pass."""
+    parts = [[block_parts(bs, pr, pa) for bs in blocks] for blocks, pr, pa, _ in archives]
+    payloads = [p for arch in parts for (_, head, tail, _) in arch for p in head + tail]
+    comp = iter(xz_many(payloads))
+    sizes = []
+    for arch in parts:
+        size = 5  # u8 preserve, u8 paired, u16 L, u8 block count
+        for hdr, head, tail, _ in arch:
+            size += len(hdr) + 1
+            for _ in range(len(head) + len(tail)):
+                size += 8 + len(next(comp))
+        sizes.append(size)
+    return sizes
