@@ -155,7 +155,7 @@ def bits_per_base(learned=False):
                 workload="1 Mbp uniform genome, 133,333 x 150 bp SE, 0.5% subst, 0.01% N, order not preserved")
 
 
-def learned_run(ctx, a, L, run, plain_ms, plain_stats):
+def learned_run(ctx, a, L, run, plain_ms, plain_stats, cands):
     """The same block (FASTQ bytes resident in HBM) with mapping under the
     trained scorer; run(**flags) runs one block."""
     from curc_amd import scorer, train_scorer
@@ -191,17 +191,48 @@ def learned_run(ctx, a, L, run, plain_ms, plain_stats):
     work = {s["name"]: int(s["items"] / a.steps) for s in raw if s["name"].startswith("map_") and s["ms"] == 0}
     log("  learned stages: " + ", ".join(f"{k} {v:.1f}" for k, v in st.items()))
     log("  learned mapping work: " + ", ".join(f"{k[4:]} {v}" for k, v in work.items()))
+    rate = mlp_rate(ctx, net)
     ctx.set_scorer(None)
     net.close()
     pm = sum(s["ms"] for s in plain_stats if s["name"] == "mapping") / a.steps
+    per_cand = dict(mlp_samples_per_s=round(rate, 1), mlp_tflops=round(rate * MLP_FLOP / 1e12, 2),
+                    candidates_per_block=int(cands), scoring_all_candidates_ms=round(cands / rate * 1e3, 2),
+                    plain_mapping_ms=round(pm, 3))
+    log(f"  network inference: {rate / 1e9:.2f} G samples/s ({rate * MLP_FLOP / 1e12:.1f} TFLOP/s); scoring every one "
+        f"of the block's {int(cands)} candidates would take {cands / rate * 1e3:.1f} ms against {pm:.1f} ms for the "
+        f"whole plain mapping")
     log(f"learned mapping: {ms:.1f} ms per block (plain {plain_ms:.1f}); mapping {st.get('mapping_learned', 0):.1f} ms "
         f"(plain {pm:.1f}); mapped {out['mapped']} (plain {plain_out['mapped']}), mismatches {out['mismatches']} "
         f"(plain {plain_out['mismatches']})")
     return dict(ms_per_step=round(ms, 3), gbases_s=round(a.reads * L / 1e9 / (ms / 1e3), 4),
                 speedup_vs_plain=round(plain_ms / ms, 4), mapping_ms=st.get("mapping_learned"),
                 mapping_ms_plain=round(pm, 3), threshold=thr, outcome=out, outcome_plain=plain_out,
-                stages_ms=st, mapping_work=work,
+                stages_ms=st, mapping_work=work, per_candidate_scoring=per_cand,
                 input="FASTQ bytes resident in HBM (plain_ms is the same input)")
+
+
+MLP_FLOP = 2 * (32 * 128 + 128 * 128 + 128)  # multiply-adds of one 32 -> 128 -> 128 -> 1 forward, x2
+
+
+def mlp_rate(ctx, net, n=1 << 23, reps=3):
+    """Forward throughput of the scorer network (samples/s) on n feature rows
+    resident in HBM."""
+    from curc_amd import scorer
+
+    x = np.random.default_rng(0).random((n, 32), dtype=np.float32) * 4
+    dx = scorer.DeviceArray(ctx, x)
+    do = scorer.DeviceArray(ctx, np.zeros(n, np.float32))
+    try:
+        net.forward_dev(dx.ptr, n, do.ptr)
+        ctx.device_sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            net.forward_dev(dx.ptr, n, do.ptr)
+        ctx.device_sync()
+        return n * reps / (time.perf_counter() - t)
+    finally:
+        dx.free()
+        do.free()
 
 
 def batched_inputs(n_inputs=64, seed=404):
@@ -653,7 +684,7 @@ def main():
 
     learned = None
     if not a.no_learned:
-        learned = learned_run(ctx, a, L, run_dev, ms_hbm, stats)
+        learned = learned_run(ctx, a, L, run_dev, ms_hbm, stats, work.get("map_cands", 0))
     full = None
     if not a.no_full_block:
         # the same block once more with the stream builders and the

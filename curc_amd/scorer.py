@@ -90,6 +90,11 @@ class Mlp:
         out.free()
         return res
 
+    def forward_dev(self, dx_ptr, n, dout_ptr):
+        """Logits of n device feature rows (float32 [n, 32]) into device memory."""
+        self._check(lib().curc_mlp_forward(self._h, ctypes.c_void_p(dx_ptr), n, ctypes.c_void_p(dout_ptr)),
+                    "curc_mlp_forward")
+
     def gradients(self, x, y):
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, N_FEATURES)
         dx, dy = DeviceArray(self.ctx, x), DeviceArray(self.ctx, np.asarray(y, np.float32))
