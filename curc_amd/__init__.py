@@ -87,7 +87,8 @@ BLOCK_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_voi
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("ov_grid", "fm_grid", "map_grid", "radix_items", "workers",
-                                               "filter_bits", "fm_fused", "ov_queue", "map_both")]
+                                               "filter_bits", "fm_fused", "ov_queue", "map_both",
+                                               "fm_static")]
 
 
 class StageStat(ctypes.Structure):

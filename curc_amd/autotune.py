@@ -3,8 +3,9 @@
 The knobs change speed only, never results: workgroup caps of the overlap
 probe, the exact-matching probe and the mapping kernels, keys per thread of
 a radix-sort tile, the bits per prefix of the overlap k-mer filter and the
-list bound of the fused exact-matching loop and the form of the overlap
-probe for one block; the number of
+list bound of the fused exact-matching loop, the form of the overlap
+probe and whether exact matching keeps its lists in place, for one block;
+the number of
 streams for a batch.  `tune_block` times whole blocks of a sample input under candidate
 values, one knob at a time with the others fixed (coordinate descent,
 best of `reps` runs per candidate), and leaves the fastest setting on the
@@ -25,6 +26,7 @@ BLOCK_KNOBS = (
     ("radix_items", (8, 16, 32)),
     ("filter_bits", (16, 32, 64)),
     ("fm_fused", (0, 1024, 4096)),
+    ("fm_static", (0, 1)),
 )
 
 

@@ -404,6 +404,7 @@ struct ClaimState {
     uint8_t* slinked;       // per suffix index: linked at this offset
     uint8_t* plinked;       // per prefix position: linked at this offset (a8 only)
     uint64_t* work;         // a8: suffix-list lengths summed over offsets (or null)
+    uint32_t* nf;           // a8 over lists kept in place: prefix skip pointers (see fm_find), or null
     __device__ __forceinline__ void link(uint32_t si, uint32_t px, uint32_t off) const {
         const uint32_t s = sl[si], p = pl[px];
         next[s] = p;
@@ -411,6 +412,7 @@ struct ClaimState {
         offset[s] = (uint16_t)off;
         slinked[si] = 1;
         if (plinked) plinked[px] = 1;
+        if (nf) nf[px] = px + 1;
     }
 };
 
@@ -875,7 +877,7 @@ __device__ __forceinline__ uint64_t rev_prefix_key(const uint64_t* __restrict__ 
 // member's first comparison word loaded once into k[0..m) (heapsort on the
 // pairs; the reads are touched again only to break ties of that word).
 template <class E>
-__device__ void sort_members_keyed(const ClaimState& cs, uint32_t* a, uint64_t* k, uint32_t m, uint32_t off) {
+__device__ __forceinline__ void sort_members_keyed(const ClaimState& cs, uint32_t* a, uint64_t* k, uint32_t m, uint32_t off) {
     for (uint32_t i = 0; i < m; ++i) k[i] = rev_prefix_key<E>(cs.R, cs.W, cs.sl[a[i]], off);
     auto before = [&](uint32_t x, uint32_t y) {
         if (k[x] != k[y]) return k[x] < k[y];
@@ -907,7 +909,7 @@ __device__ void sort_members_keyed(const ClaimState& cs, uint32_t* a, uint64_t* 
 }
 
 template <bool FULLMATCH, class E>
-__device__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint32_t off) {
+__device__ __forceinline__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint32_t off) {
     if (m <= 16) {
         for (uint32_t i = 1; i < m; ++i) {
             uint32_t x = a[i], j = i;
@@ -941,6 +943,23 @@ __device__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint
     }
 }
 
+// First free prefix position >= x of a list kept in place (np: none).
+// nf[x] == x for a free position; linking x sets nf[x] = x + 1, so every
+// position in [x, nf[x]) is linked, and the walk shortens the path it takes
+// (nf[x] = nf[nf[x]]).  Values only grow and each one is a valid bound at
+// any time, so walks racing with each other or with links elsewhere stay
+// correct: a stale value only makes a walk longer.
+__device__ __forceinline__ uint32_t fm_find(uint32_t* __restrict__ nf, uint32_t x, uint32_t np) {
+    while (x < np) {
+        const uint32_t y = nf[x];
+        if (y == x) return x;
+        const uint32_t z = (y < np) ? nf[y] : y;
+        if (z != y) nf[x] = z;
+        x = z;
+    }
+    return np;
+}
+
 // One leader per contended group replays it as the reference's sequential
 // order would: members sorted by suffix order, each taking the first free
 // non-self prefix of the group's run (compress.cu:1500-1527 for a6; the
@@ -952,13 +971,12 @@ __device__ void sort_members(const ClaimState& cs, uint32_t* a, uint32_t m, uint
 // form one run [g, e) of the sorted list (one bucket's sorted run in a6),
 // which starts at g.  Its end is found once, by galloping, instead of one
 // comparison per linked member.
-template <class E>
-__device__ __forceinline__ void claim_assign(const ClaimState& cs, const uint32_t* arr, uint32_t m, uint32_t g,
-                                             uint32_t off) {
+template <class E, typename At>
+__device__ __forceinline__ void claim_assign_at(const ClaimState& cs, At arr, uint32_t m, uint32_t g, uint32_t off) {
     const uint32_t np = *cs.d_np;
     const uint32_t len = cs.L - off;
     const uint32_t n1 = min(E::BPW, len);
-    const uint32_t s0 = cs.sl[arr[0]];
+    const uint32_t s0 = cs.sl[arr(0)];
     const uint64_t* rs0 = cs.R + (uint64_t)s0 * cs.W;
     const uint64_t sx00 = E::word(rs0, off, n1);
     auto equal = [&](uint32_t x) { return cmp_ps<E>(cs.R, cs.W, cs.pl[x], cs.pk[x], rs0, sx00, off, len) == 0; };
@@ -976,25 +994,35 @@ __device__ __forceinline__ void claim_assign(const ClaimState& cs, const uint32_
         else
             e = mid;
     }
-    auto next_free = [&](uint32_t x) {
+    auto next_free = [&](uint32_t x) -> uint32_t {
+        if (cs.nf) {
+            const uint32_t f = fm_find(cs.nf, x, np);
+            return f < e ? f : e;
+        }
         while (x < e && cs.prev[cs.pl[x]] != INV) ++x;
         return x;
     };
     uint32_t a = next_free(g);
     uint32_t b = next_free(a + 1);
     for (uint32_t k = 0; k < m && a < e; ++k) {
-        const uint32_t s = cs.sl[arr[k]];
+        const uint32_t s = cs.sl[arr(k)];
         if (cs.pl[a] != s) {
-            cs.link(arr[k], a, off);
+            cs.link(arr(k), a, off);
             a = b;
             b = next_free(b + 1);
         } else if (b < e) {
-            cs.link(arr[k], b, off);
+            cs.link(arr(k), b, off);
             b = next_free(b + 1);
         }
     }
     cs.head[g] = INV;
-    cs.contended[g] = 0;
+    if (cs.contended) cs.contended[g] = 0;
+}
+
+template <class E>
+__device__ __forceinline__ void claim_assign(const ClaimState& cs, const uint32_t* arr, uint32_t m, uint32_t g,
+                                             uint32_t off) {
+    claim_assign_at<E>(cs, [&](uint32_t k) { return arr[k]; }, m, g, off);
 }
 
 // a8 groups of RESOLVE_COOP_MIN < m <= RESOLVE_COOP_MAX members are sorted
@@ -1002,7 +1030,7 @@ __device__ __forceinline__ void claim_assign(const ClaimState& cs, const uint32_
 // sort_members_keyed: first comparison word, then the reads) after its
 // grid-stride pass, instead of by the leader alone; up to RESOLVE_COOP_LIST
 // per workgroup and call, the rest as before.
-constexpr uint32_t RESOLVE_COOP_MIN = 64;
+constexpr uint32_t RESOLVE_COOP_MIN = 16;
 constexpr uint32_t RESOLVE_COOP_MAX = 4096;
 constexpr uint32_t RESOLVE_COOP_LIST = 32;
 struct ResolveCoop {
@@ -1013,7 +1041,7 @@ struct ResolveCoop {
 };
 
 template <class E>
-__device__ void resolve_coop(const ClaimState& cs, ResolveCoop& co, uint32_t off) {
+__device__ __forceinline__ void resolve_coop(const ClaimState& cs, ResolveCoop& co, uint32_t off) {
     __syncthreads();
     const uint32_t nb = min(co.n, RESOLVE_COOP_LIST);
     for (uint32_t gi = 0; gi < nb; ++gi) {
@@ -1123,7 +1151,7 @@ __global__ void __launch_bounds__(256) k_fm_ranges(const uint64_t* __restrict__ 
         const uint64_t i = base + threadIdx.x;
         uint32_t lo = 1, hi = 0;  // empty unless this position starts a gap
         if (i <= np) {
-            if (i < np) plinked[i] = 0;
+            if (i < np && plinked) plinked[i] = 0;
             lo = (i == 0) ? 0u : (uint32_t)E::head(pk[i - 1], bp) + 1u;
             hi = (i == np) ? nb : (uint32_t)E::head(pk[i], bp);
         }
@@ -1188,6 +1216,535 @@ __global__ void __launch_bounds__(1024) k_fm_fused(ClaimState c0, ClaimState c1,
         __syncthreads();
         cur ^= 1;
         if (*ct.d_ns == 0 || *ct.d_np == 0) break;
+    }
+}
+
+// ------------------------------------------------ exact matching, groups ---
+constexpr uint32_t FM_SMALL = 16;  // groups up to this size are sorted in registers
+// The exact-matching loop of k_fm_probe_s / k_fm_resolve_s, over lists kept
+// in place (cs.nf set) or compacted after every offset (cs.nf null).
+//
+// Lists in place: the reference drops linked reads from both lists after
+// every offset (compress.cu:1162-1172); here a linked suffix is skipped by
+// its flag (slinked, kept set) and a linked prefix by the skip pointers nf
+// (fm_find).  The free entries keep their order, so every suffix meets the
+// same free prefixes in the same order as over the compacted lists, and the
+// block ranges of the prefix list change only with the block width.  A
+// suffix whose filter verdict at this offset is clear (vb: no prefix starts
+// with its 32 bases from off) is not searched.
+//
+// Each suffix with an equal free non-self prefix joins its group (the first
+// equal position of the list, grp) and records that prefix (choice): it
+// takes the next rank of the group's count gcnt, goes into the group's row
+// of gmem when the rank is below FM_SMALL (so that the resolution reads a
+// small group with one load), and into the group's member list (head /
+// memb) for larger ones.  Rank 0 sets its lead flag, by which the
+// resolution finds the group.  No counter is shared by all suffixes.
+// zero_p / zero_n: a buffer to clear on the way (the look-back status words
+// of the compaction that follows, between compacted offsets).
+template <class E>
+__global__ void __launch_bounds__(256) k_fm_probe_s(ClaimState cs, const uint32_t* __restrict__ rng, uint32_t bp,
+                                                    uint32_t off, const uint64_t* __restrict__ vb, uint64_t vnw,
+                                                    uint8_t* __restrict__ lead, uint32_t* __restrict__ gcnt,
+                                                    uint32_t* __restrict__ gmem, uint32_t* __restrict__ zero_p,
+                                                    uint64_t zero_n) {
+    const uint32_t ns = *cs.d_ns, np = *cs.d_np;
+    const bool inplace = cs.nf != nullptr;
+    // a buffer the next compaction needs cleared (its look-back status)
+    GRID_STRIDE(z, zero_n) zero_p[z] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *cs.pool_n = 0;
+        if (cs.work) {
+            if (!inplace) cs.work[0] += ns;
+            if (off == 1) cs.work[1] = ns;
+        }
+    }
+    // compacted lists without block ranges: clear the prefix flags here
+    if (!inplace && !rng) GRID_STRIDE(i, np) cs.plinked[i] = 0;
+    const uint32_t len = cs.L - off;
+    const uint32_t n1 = min(E::BPW, len);
+    const uint64_t* vrow = vb ? vb + (uint64_t)(off - 1) * vnw : nullptr;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < ns; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        bool act = i < ns;
+        if (act) {
+            // compacted: every listed suffix is unlinked (its flag is of the last offset)
+            if (inplace)
+                act = cs.slinked[i] == 0;
+            else
+                cs.slinked[i] = 0;
+        }
+        const bool pr = act && (!vrow || ((vrow[i >> 6] >> (i & 63u)) & 1ull));
+        if (!pr) continue;
+        const uint32_t s = cs.sl[i];
+        const uint64_t* rs = cs.R + (uint64_t)s * cs.W;
+        uint32_t lo = 0, e = np;
+        if (rng) {
+            const uint32_t b = (uint32_t)E::word(rs, off, bp);
+            lo = rng[b];
+            e = rng[b + 1];
+        }
+        const uint64_t sx0 = E::word(rs, off, n1);
+        uint32_t cnt = e - lo;
+        while (cnt > 0) {
+            const uint32_t step = cnt >> 1, it = lo + step;
+            if (cmp_ps<E>(cs.R, cs.W, cs.pl[it], cs.pk[it], rs, sx0, off, len) < 0) {
+                lo = it + 1;
+                cnt -= step + 1;
+            } else {
+                cnt = step;
+            }
+        }
+        // lo: the first position >= s[off:L] (the group id when equal)
+        const uint32_t a = inplace ? fm_find(cs.nf, lo, np) : lo;
+        uint32_t ch = INV;
+        if (a < e && cmp_ps<E>(cs.R, cs.W, cs.pl[a], cs.pk[a], rs, sx0, off, len) == 0) {
+            if (cs.pl[a] != s) {
+                ch = a;
+            } else {
+                const uint32_t b2 = inplace ? fm_find(cs.nf, a + 1, np) : a + 1;
+                if (b2 < e && cmp_ps<E>(cs.R, cs.W, cs.pl[b2], cs.pk[b2], rs, sx0, off, len) == 0) ch = b2;
+            }
+        }
+        if (ch == INV) continue;
+        cs.choice[i] = ch;
+        cs.grp[i] = lo;
+        const uint32_t r = atomicAdd(&gcnt[lo], 1u);
+        if (r < FM_SMALL) gmem[(uint64_t)lo * FM_SMALL + r] = (uint32_t)i;
+        cs.memb[i] = atomicExch(&cs.head[lo], (uint32_t)i);
+        if (r == 0) lead[i] = 1;
+    }
+}
+
+// Work of the offsets [1, Ls) of k_fm_probe_s, counted once afterwards: a
+// suffix linked at offset o was considered at offsets 1..o, an unlinked one
+// at 1..Ls-1; it was searched at those of them with a set verdict bit, and
+// at every one from vmax on (no verdicts).  Adds the totals to out[0]
+// (considered) and out[2] (searched), one atomic per workgroup.
+__global__ void __launch_bounds__(256) k_fm_count_work(const uint32_t* __restrict__ sl, const uint32_t* __restrict__ d_ns,
+                                                       const uint32_t* __restrict__ next,
+                                                       const uint16_t* __restrict__ offset, uint32_t Ls,
+                                                       const uint64_t* __restrict__ vb, uint64_t vnw, uint32_t vmax,
+                                                       uint64_t* __restrict__ out) {
+    __shared__ uint64_t part[2][4];
+    const uint32_t ns = *d_ns;
+    uint64_t act = 0, srch = 0;
+    GRID_STRIDE(i, ns) {
+        const uint32_t s = sl[i];
+        const uint32_t o = next[s] != INV ? (uint32_t)offset[s] : Ls - 1;
+        act += o;
+        uint32_t nv = 0;  // offsets 1 .. min(o, vmax - 1) with a verdict
+        if (vb) {
+            const uint32_t top = min(o, vmax - 1);
+            for (uint32_t f = 1; f <= top; ++f) nv += (uint32_t)((vb[(uint64_t)(f - 1) * vnw + (i >> 6)] >> (i & 63)) & 1ull);
+            srch += nv + (o > top ? o - top : 0u);
+        } else {
+            srch += o;
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        act += __shfl_down(act, d, 64);
+        srch += __shfl_down(srch, d, 64);
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][w] = act;
+        part[1][w] = srch;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t a = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        const uint64_t b = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+        if (a) atomicAdd((unsigned long long*)&out[0], (unsigned long long)a);
+        if (b) atomicAdd((unsigned long long*)&out[2], (unsigned long long)b);
+    }
+}
+
+// cs.link with the suffix and prefix read ids already known.
+__device__ __forceinline__ void fm_link(const ClaimState& cs, uint32_t si, uint32_t s, uint32_t px, uint32_t p,
+                                        uint32_t off) {
+    cs.next[s] = p;
+    cs.prev[p] = s;
+    cs.offset[s] = (uint16_t)off;
+    cs.slinked[si] = 1;
+    if (cs.plinked) cs.plinked[px] = 1;
+    if (cs.nf) cs.nf[px] = px + 1;
+}
+
+// Member order of a group (the order of member_before / cmp_rev_prefix):
+// first reversed-prefix word, then the reads; INV entries last.
+template <class E>
+__device__ __forceinline__ bool fm_before(const ClaimState& cs, uint32_t ux, uint64_t kx, uint32_t sx, uint32_t uy,
+                                          uint64_t ky, uint32_t sy, uint32_t off) {
+    if (ux == INV) return false;
+    if (uy == INV) return true;
+    if (kx != ky) return kx < ky;
+    return cmp_rev_prefix<E>(cs.R, cs.W, sx, sy, off) < 0;
+}
+
+// The walk of claim_assign over a run whose positions are all free (lists
+// compacted after every offset: the run [g, e) holds no linked prefix, and
+// only this group links into it), with the members' reads sid[0..m) in
+// sorted order and the run's first positions known: pos(j) is the prefix
+// read at g + j, or INV past the run, for j <= m + 1 (the walk never looks
+// further: b grows by one per linked member).  a / b are the two free
+// positions the walk holds (claim_assign's a and b).
+template <typename SidAt, typename MemAt, typename PosAt>
+__device__ __forceinline__ void fm_walk_free(const ClaimState& cs, uint32_t m, uint32_t g, uint32_t off, SidAt sid,
+                                             MemAt mem, PosAt pos) {
+    uint32_t a = 0, b = 1;
+    for (uint32_t k = 0; k < m; ++k) {
+        const uint32_t pa = pos(a);
+        if (pa == INV) break;
+        const uint32_t s = sid(k);
+        if (pa != s) {
+            fm_link(cs, mem(k), s, g + a, pa, off);
+            a = b;
+            b = b + 1;
+        } else {
+            const uint32_t pb = pos(b);
+            if (pb != INV) {
+                fm_link(cs, mem(k), s, g + b, pb, off);
+                b = b + 1;
+            }
+        }
+    }
+}
+
+constexpr uint32_t FM_GATHER_MAX = 1u << 15;  // compacted lists up to this length: large groups gathered by scan
+
+// The members' group marks: grp[i] is INV for every suffix position except
+// those matched at the current offset (set by k_fm_probe_s); resolving a
+// group clears its members' marks, so that a workgroup can gather a
+// group's members by scanning the marks.
+
+// A group walked by one thread over the pool: the members in list order
+// into the pool, sorted (keyed heapsort), walked, and their marks cleared.
+template <class E>
+__device__ __forceinline__ void fm_serial_group(const ClaimState& cs, uint32_t g, uint32_t off) {
+    const uint32_t u0 = cs.head[g];
+    uint32_t m = 0;
+    for (uint32_t u = u0; u != INV; u = cs.memb[u]) ++m;
+    const uint32_t base = atomicAdd(cs.pool_n, m);
+    uint32_t* arr = cs.pool + base;
+    m = 0;
+    for (uint32_t u = u0; u != INV; u = cs.memb[u]) arr[m++] = u;
+    sort_members_keyed<E>(cs, arr, cs.kpool + base, m, off);
+    claim_assign<E>(cs, arr, m, g, off);
+    for (uint32_t k = 0; k < m; ++k) cs.grp[arr[k]] = INV;
+}
+
+// A group of 2..FM_SMALL members (mem[0..m), suffix positions; the rest
+// INV): sorted by a bitonic network in registers, then walked.
+template <class E>
+__device__ __forceinline__ void fm_small_group(const ClaimState& cs, uint32_t (&mem)[FM_SMALL], uint32_t m, uint32_t g,
+                                               uint32_t off) {
+    uint64_t key[FM_SMALL];
+    uint32_t sid[FM_SMALL];
+#pragma unroll
+    for (uint32_t k = 0; k < FM_SMALL; ++k) {
+        sid[k] = mem[k] != INV ? cs.sl[mem[k]] : INV;
+        key[k] = mem[k] != INV ? rev_prefix_key<E>(cs.R, cs.W, sid[k], off) : ~0ull;
+    }
+#pragma unroll
+    for (uint32_t k = 2; k <= FM_SMALL; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (uint32_t x = 0; x < FM_SMALL; ++x) {
+                const uint32_t y = x ^ j;
+                if (y > x) {
+                    // ascending where x & k is clear: y must not come before x
+                    const bool up = (x & k) == 0;
+                    const bool sw = up ? fm_before<E>(cs, mem[y], key[y], sid[y], mem[x], key[x], sid[x], off)
+                                       : fm_before<E>(cs, mem[x], key[x], sid[x], mem[y], key[y], sid[y], off);
+                    if (sw) {
+                        const uint32_t tm = mem[x], ts = sid[x];
+                        const uint64_t tk = key[x];
+                        mem[x] = mem[y];
+                        sid[x] = sid[y];
+                        key[x] = key[y];
+                        mem[y] = tm;
+                        sid[y] = ts;
+                        key[y] = tk;
+                    }
+                }
+            }
+        }
+    }
+    if (cs.nf) {
+        // lists in place (free positions by skip pointer): the member list
+        // relinked in sorted order, and claim_assign walks it
+#pragma unroll
+        for (uint32_t k = 0; k + 1 < FM_SMALL; ++k)
+            if (k + 1 < m) cs.memb[mem[k]] = mem[k + 1];
+        uint32_t at_k = 0, at_u = mem[0];
+        claim_assign_at<E>(
+            cs,
+            [&](uint32_t k) {
+                while (at_k < k) {
+                    at_u = cs.memb[at_u];
+                    ++at_k;
+                }
+                return at_u;
+            },
+            m, g, off);
+    } else {
+        // compacted lists: fm_walk_free's walk over the run's first m + 2
+        // positions, with the members in registers
+        const uint32_t np = *cs.d_np, len = cs.L - off, n1 = min(E::BPW, len);
+        const uint64_t* rs0 = cs.R + (uint64_t)sid[0] * cs.W;
+        const uint64_t sx00 = E::word(rs0, off, n1);
+        // the comparisons are independent (their loads overlap); the run is
+        // the positions before the first unequal one
+        uint32_t pr[FM_SMALL + 2];
+#pragma unroll
+        for (uint32_t j = 0; j < FM_SMALL + 2; ++j) {
+            const uint32_t x = g + j;
+            const bool eq = j <= m + 1 && x < np && cmp_ps<E>(cs.R, cs.W, cs.pl[x], cs.pk[x], rs0, sx00, off, len) == 0;
+            pr[j] = eq ? cs.pl[x] : INV;
+        }
+        bool in = true;
+#pragma unroll
+        for (uint32_t j = 0; j < FM_SMALL + 2; ++j) {
+            in = in && pr[j] != INV;
+            if (!in) pr[j] = INV;
+        }
+        auto pick = [&](uint32_t j) {
+            uint32_t v = INV;
+#pragma unroll
+            for (uint32_t t = 0; t < FM_SMALL + 2; ++t) v = (t == j) ? pr[t] : v;
+            return v;
+        };
+        uint32_t a = 0, b = 1;
+        bool done = false;
+#pragma unroll
+        for (uint32_t k = 0; k < FM_SMALL; ++k) {
+            if (k < m && !done) {
+                const uint32_t pa = pick(a);
+                if (pa == INV) {
+                    done = true;
+                } else if (pa != sid[k]) {
+                    fm_link(cs, mem[k], sid[k], g + a, pa, off);
+                    a = b;
+                    b = b + 1;
+                } else {
+                    const uint32_t pb = pick(b);
+                    if (pb != INV) {
+                        fm_link(cs, mem[k], sid[k], g + b, pb, off);
+                        b = b + 1;
+                    }
+                }
+            }
+        }
+        cs.head[g] = INV;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < FM_SMALL; ++k)
+        if (k < m) cs.grp[mem[k]] = INV;
+}
+
+// Groups of more than FM_SMALL members, up to FM_COOP_MAX, are sorted by
+// the whole workgroup in LDS (bitonic) after its scan; the walk then reads
+// the members' reads and the run's prefixes from LDS.  A group's members
+// come from the pool (base, m) or, for compacted lists of at most
+// FM_GATHER_MAX suffixes (base INV), from a scan of the group marks, which
+// costs less than following the member list one load at a time.
+constexpr uint32_t FM_COOP_MAX = RESOLVE_COOP_MAX - 2;
+struct FmCoop {
+    uint64_t key[RESOLVE_COOP_MAX];  // sort keys; then the members' reads (u32) and the run's prefixes (u32)
+    uint32_t mem[RESOLVE_COOP_MAX];
+    uint32_t g[RESOLVE_COOP_LIST], base[RESOLVE_COOP_LIST], m[RESOLVE_COOP_LIST];
+    uint32_t n, cnt;
+};
+
+template <class E>
+__device__ __forceinline__ void fm_coop_groups(const ClaimState& cs, FmCoop& co, uint32_t off) {
+    __syncthreads();
+    const uint32_t nb = min(co.n, RESOLVE_COOP_LIST);
+    const uint32_t ns = *cs.d_ns;
+    uint32_t* sidl = reinterpret_cast<uint32_t*>(co.key);
+    uint32_t* posl = sidl + RESOLVE_COOP_MAX;
+    for (uint32_t gi = 0; gi < nb; ++gi) {
+        const uint32_t g = co.g[gi];
+        uint32_t m;
+        if (co.base[gi] == INV) {
+            if (threadIdx.x == 0) co.cnt = 0;
+            __syncthreads();
+            for (uint32_t x = threadIdx.x; x < ns; x += blockDim.x) {
+                if (cs.grp[x] == g) {
+                    const uint32_t q = atomicAdd(&co.cnt, 1u);
+                    if (q < FM_COOP_MAX) co.mem[q] = x;
+                }
+            }
+            __syncthreads();
+            m = co.cnt;
+            if (m > FM_COOP_MAX) {
+                if (threadIdx.x == 0) fm_serial_group<E>(cs, g, off);
+                __syncthreads();
+                continue;
+            }
+        } else {
+            m = co.m[gi];
+            const uint32_t* arr = cs.pool + co.base[gi];
+            for (uint32_t x = threadIdx.x; x < m; x += blockDim.x) co.mem[x] = arr[x];
+            __syncthreads();
+        }
+        uint32_t P = 1;
+        while (P < m) P <<= 1;
+        for (uint32_t x = threadIdx.x; x < P; x += blockDim.x) {
+            if (x < m) {
+                co.key[x] = rev_prefix_key<E>(cs.R, cs.W, cs.sl[co.mem[x]], off);
+            } else {
+                co.mem[x] = INV;  // padding sorts last
+                co.key[x] = ~0ull;
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t x = threadIdx.x; x < P; x += blockDim.x) {
+                    const uint32_t y = x ^ j;
+                    if (y > x) {
+                        const uint32_t ux = co.mem[x], uy = co.mem[y];
+                        const uint64_t kx = co.key[x], ky = co.key[y];
+                        // the reads are loaded only to break a tie of the keys
+                        const bool tie = ux != INV && uy != INV && kx == ky;
+                        const uint32_t sx = tie ? cs.sl[ux] : INV, sy = tie ? cs.sl[uy] : INV;
+                        const bool sw = ((x & k) == 0) ? fm_before<E>(cs, uy, ky, sy, ux, kx, sx, off)
+                                                       : fm_before<E>(cs, ux, kx, sx, uy, ky, sy, off);
+                        if (sw) {
+                            co.mem[x] = uy;
+                            co.mem[y] = ux;
+                            co.key[x] = ky;
+                            co.key[y] = kx;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (cs.nf) {
+            if (threadIdx.x == 0) claim_assign<E>(cs, co.mem, m, g, off);
+        } else {
+            // the members' reads, and the run's prefixes at positions 0 .. m + 1
+            const uint32_t np = *cs.d_np, len = cs.L - off, n1 = min(E::BPW, len);
+            const uint64_t* rs0 = cs.R + (uint64_t)cs.sl[co.mem[0]] * cs.W;
+            const uint64_t sx00 = E::word(rs0, off, n1);
+            __syncthreads();  // the keys are no longer read: their space holds sidl / posl
+            for (uint32_t x = threadIdx.x; x < m + 2; x += blockDim.x) {
+                if (x < m) sidl[x] = cs.sl[co.mem[x]];
+                const uint32_t p = g + x;
+                posl[x] =
+                    (p < np && cmp_ps<E>(cs.R, cs.W, cs.pl[p], cs.pk[p], rs0, sx00, off, len) == 0) ? cs.pl[p] : INV;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                // the run is [g, e) of the sorted list: posl is INV from e - g on
+                fm_walk_free(
+                    cs, m, g, off, [&](uint32_t k) { return sidl[k]; }, [&](uint32_t k) { return co.mem[k]; },
+                    [&](uint32_t j) { return posl[j]; });
+                cs.head[g] = INV;
+            }
+        }
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < m; x += blockDim.x) cs.grp[co.mem[x]] = INV;
+        __syncthreads();
+    }
+}
+
+// Resolution of the groups k_fm_probe_s formed, found by their lead flags
+// (cleared here).  A lone member links to its choice (claim_assign's walk
+// gives the same prefix for one member); groups of up to FM_SMALL members
+// are sorted in registers by the thread that found them, larger ones by
+// the workgroup (up to FM_COOP_MAX and RESOLVE_COOP_LIST per workgroup;
+// beyond, by that thread, as in claim_resolve_body).  Their members are
+// walked in sorted order (claim_assign; fm_walk_free when the lists are
+// compacted).
+template <class E>
+__global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t off, uint8_t* __restrict__ lead,
+                                                      uint32_t* __restrict__ gcnt, const uint32_t* __restrict__ gmem) {
+    __shared__ FmCoop co;
+    if (threadIdx.x == 0) co.n = 0;
+    __syncthreads();
+    const uint32_t ns = *cs.d_ns;
+    auto group = [&](uint32_t i) {
+        const uint32_t g = cs.grp[i];
+        uint32_t m = gcnt[g];
+        gcnt[g] = 0;
+        if (m <= FM_SMALL) {
+            uint32_t mem[FM_SMALL];
+            const uint4* row = reinterpret_cast<const uint4*>(gmem + (uint64_t)g * FM_SMALL);
+#pragma unroll
+            for (uint32_t q = 0; q < FM_SMALL / 4; ++q) {
+                const uint4 v = row[q];
+                mem[4 * q] = 4 * q < m ? v.x : INV;
+                mem[4 * q + 1] = 4 * q + 1 < m ? v.y : INV;
+                mem[4 * q + 2] = 4 * q + 2 < m ? v.z : INV;
+                mem[4 * q + 3] = 4 * q + 3 < m ? v.w : INV;
+            }
+            const uint32_t u0 = mem[0];
+            if (m == 1) {
+                const uint32_t ch = cs.choice[u0];
+                fm_link(cs, u0, cs.sl[u0], ch, cs.pl[ch], off);
+                cs.grp[u0] = INV;
+                cs.head[g] = INV;
+            } else {
+                fm_small_group<E>(cs, mem, m, g, off);
+            }
+            return;
+        }
+        const uint32_t u0 = cs.head[g];
+        uint32_t u = u0;
+        if (!cs.nf && ns <= FM_GATHER_MAX) {
+            const uint32_t slot = atomicAdd(&co.n, 1u);
+            if (slot < RESOLVE_COOP_LIST) {
+                co.g[slot] = g;
+                co.base[slot] = INV;
+                return;
+            }
+        } else {
+            if (m <= FM_COOP_MAX) {
+                const uint32_t slot = atomicAdd(&co.n, 1u);
+                if (slot < RESOLVE_COOP_LIST) {
+                    const uint32_t base = atomicAdd(cs.pool_n, m);
+                    uint32_t* arr = cs.pool + base;
+                    m = 0;
+                    for (u = u0; u != INV; u = cs.memb[u]) arr[m++] = u;
+                    co.g[slot] = g;
+                    co.base[slot] = base;
+                    co.m[slot] = m;
+                    return;
+                }
+            }
+        }
+        fm_serial_group<E>(cs, g, off);
+    };
+    // each workgroup scans its own run of positions, so that the groups of
+    // a short list still spread over all workgroups and threads
+    const uint32_t per = (ns + gridDim.x - 1) / gridDim.x;
+    const uint64_t beg = (uint64_t)blockIdx.x * per;
+    const uint64_t end = min<uint64_t>(beg + per, ns);
+    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        if (!lead[i]) continue;
+        lead[i] = 0;
+        group((uint32_t)i);
+    }
+    fm_coop_groups<E>(cs, co, off);
+}
+
+// Minimizer-blocked filter of the prefixes' first 32 bases (the blocks and
+// bits of k_prefix_hash), over a list whose length is a device counter.
+__global__ void k_fm_prefix_filter(const uint64_t* __restrict__ R, const uint32_t* __restrict__ pl,
+                                   const uint32_t* __restrict__ d_np, uint32_t W, uint32_t* __restrict__ mfilt,
+                                   uint32_t bmask) {
+    const uint32_t np = *d_np;
+    GRID_STRIDE(i, np) {
+        const uint64_t K = xw(R + (uint64_t)pl[i] * W, 0, 32);
+        uint32_t* fb = mfilt + (uint64_t)ov_block(ov_minimizer(K, 32), bmask) * OVB_WORDS;
+        const uint32_t f = ov_fbits(K), b1 = f & OVB_MASK, b2 = (f >> 10) & OVB_MASK;
+        atomicOr(&fb[b1 >> 5], 1u << (b1 & 31u));
+        atomicOr(&fb[b2 >> 5], 1u << (b2 & 31u));
     }
 }
 
@@ -1609,6 +2166,8 @@ __device__ __forceinline__ uint64_t xw_pick(const uint64_t (&w)[WM], uint32_t of
 // Overlap filter verdicts for every offset in one pass (minimizer-blocked
 // filter): bit i of vb[(off - 1) * nw + i / 64] is set when the k-mer of
 // suffix i at kip + off has both of its bits set, for off in [1, max_off).
+// Also the exact-matching verdicts (kip 0, k 32: the suffix's 32 bases from
+// off against the prefixes' first 32 bases).
 // Thread i holds its read's words (its row of the packed reads R, through
 // the suffix list sl), and every lane does the same work per offset, so
 // lanes do not diverge.  For k = 32
@@ -1626,7 +2185,9 @@ template <uint32_t WM, bool K32>
 __global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict__ R, const uint32_t* __restrict__ sl,
                                                      uint32_t ns, uint32_t W, uint32_t k, uint32_t kip,
                                                      uint32_t max_off, const uint32_t* __restrict__ mfilt,
-                                                     uint32_t bmask, uint64_t* __restrict__ vb, uint64_t nw) {
+                                                     uint32_t bmask, uint64_t* __restrict__ vb, uint64_t nw,
+                                                     const uint32_t* __restrict__ d_ns) {
+    if (d_ns) ns = *d_ns;  // the list length as a device counter (ns is then its bound)
     __shared__ uint32_t blocks[256 * (OVB_WORDS + 1)];  // odd stride: spreads banks
     uint32_t* mine = blocks + threadIdx.x * (OVB_WORDS + 1);
     uint32_t held = 0xFFFFFFFFu;  // block in the LDS slot

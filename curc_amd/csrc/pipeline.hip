@@ -85,7 +85,7 @@ enum Slot {
     S_FQ0, S_FQ1, S_FQSEQ, S_FQTILE,
     S_D_TEXT, S_D_STRAND, S_D_CNT, S_D_BASE, S_D_STREAMS, S_D_SSTART, S_D_TAB, S_D_ROFF, S_D_LQ, S_D_NLQ,
     S_D_PE, S_D_IDPOS, S_D_PV, S_D_PX, S_D_HQV, S_D_HQX, S_D_BV, S_D_BX, S_D_K0, S_D_K1, S_D_V0, S_D_V1,
-    S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2, S_OVPASS, S_WORK,
+    S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2, S_OVPASS, S_WORK, S_FMNF, S_FMVB, S_FMFILT, S_FMLEAD, S_FMGC, S_FMGM,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -154,6 +154,7 @@ struct Tuning {
     uint32_t map_both = 1;       // both strand passes in one launch on a two-strand index (1)
     uint32_t ov_queue = 2;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
+    uint32_t fm_static = 1;      // exact matching over lists kept in place (1) or compacted per offset (0)
 };
 
 struct curc_ctx {
@@ -482,6 +483,85 @@ void records_to_host(curc_ctx* c, const AsmOut& a, const uint32_t* id_map, uint6
     d2h(c, rec, (const curc_alignment*)d, a.n);
 }
 
+// Exact-matching verdicts (k_ov_verdicts with kip 0, k 32) for reads of W
+// words: its register copy of a read is sized at compile time.
+inline void fm_verdicts(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const uint32_t* sl,
+                        const uint32_t* d_ns, uint32_t bound, const uint32_t* filt, uint32_t bmask, uint64_t* vb,
+                        uint64_t vnw) {
+    const dim3 vg(grid_for(bound, 65536));
+#define CURC_FM_VERDICTS(WM)                                                                                     \
+    hipLaunchKernelGGL((k_ov_verdicts<WM, true>), vg, dim3(256), 0, c->st, R, sl, bound, W, 32u, 0u, L - 31, filt, \
+                       bmask, vb, vnw, d_ns)
+    if (W <= 2) CURC_FM_VERDICTS(2);
+    else if (W <= 3) CURC_FM_VERDICTS(3);
+    else if (W <= 4) CURC_FM_VERDICTS(4);
+    else if (W <= 5) CURC_FM_VERDICTS(5);
+    else if (W <= 6) CURC_FM_VERDICTS(6);
+    else if (W <= 7) CURC_FM_VERDICTS(7);
+    else CURC_FM_VERDICTS(8);
+#undef CURC_FM_VERDICTS
+}
+
+// Offsets [1, Ls) of the exact-matching loop over lists kept in place
+// (k_fm_probe_s / k_fm_resolve_s): two launches per offset, and the block
+// ranges of the prefix list once per block width.  For 2-bit reads of up to
+// 8 words the suffixes are screened by the prefix filter at every offset
+// that leaves at least 32 bases.  Links set the flags slinked / plinked, so
+// that the lists can be compacted afterwards; the work counts of the phase
+// are added to work[0] (suffix-offset pairs) and work[2] (searches), and
+// work[1] gets the suffix count.
+template <class E>
+void fm_static_phase(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, uint32_t Ls, uint32_t n_max,
+                     bool ranged, uint32_t MAX_BP, uint32_t BITS, uint32_t* cnt, uint32_t* next, uint32_t* prev,
+                     uint16_t* offset, uint32_t* pl, uint64_t* pk, uint32_t* sl, uint32_t* d_np, uint32_t* d_ns,
+                     uint32_t* choice, uint32_t* grp, uint32_t* memb, uint32_t* resl, uint32_t* pool,
+                     uint64_t* kpool, uint8_t* slinked, uint8_t* plinked, uint8_t* lead, uint32_t* gcnt,
+                     uint32_t* gmem, uint32_t* head, uint64_t* work, uint32_t g) {
+    Scratch& S = c->S;
+    hipStream_t st = c->st;
+    uint32_t* rng = S.as<uint32_t>(S_RNG, (1u << 20) + 1);
+    uint32_t* nf = S.as<uint32_t>(S_FMNF, n_max);
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n_max)), dim3(256), 0, st, nf, (uint64_t)n_max);
+    fill<uint8_t>(c, slinked, n_max, (uint8_t)0);
+    fill<uint8_t>(c, plinked, n_max, (uint8_t)0);
+    // prefix filter and verdicts (offsets 1 .. L-32)
+    const bool verd = E::BPW == 32 && W <= 8 && L > 32;
+    uint64_t* vb = nullptr;
+    const uint64_t vnw = ((uint64_t)n_max + 63) / 64;
+    const uint32_t vmax = verd ? L - 31 : 1;  // verdicts for off < vmax
+    if (verd) {
+        uint64_t fbits = 1ull << 20;
+        while (fbits < (uint64_t)c->tune.filter_bits * n_max && fbits < (1ull << 32)) fbits <<= 1;
+        const uint32_t bmask = (uint32_t)(fbits / (32 * OVB_WORDS) - 1);
+        uint32_t* filt = S.as<uint32_t>(S_FMFILT, fbits / 32);
+        fill<uint32_t>(c, filt, fbits / 32, 0u);
+        hipLaunchKernelGGL(k_fm_prefix_filter, dim3(grid_for(n_max)), dim3(256), 0, st, R, (const uint32_t*)pl,
+                           (const uint32_t*)d_np, W, filt, bmask);
+        vb = S.as<uint64_t>(S_FMVB, (uint64_t)(vmax - 1) * vnw);
+        fm_verdicts(c, R, W, L, sl, d_ns, n_max, filt, bmask, vb, vnw);
+    }
+    uint32_t last_bp = 0;
+    for (uint32_t off = 1; off < Ls; ++off) {
+        const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
+        if (ranged && bp != last_bp) {
+            hipLaunchKernelGGL(k_fm_ranges<E>, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, st,
+                               (const uint64_t*)pk, (const uint32_t*)d_np, bp, 1u << (BITS * bp), rng,
+                               (uint8_t*)nullptr);
+            last_bp = bp;
+        }
+        ClaimState cs{R, W, L, sl, d_ns, pl, pk, d_np, next, prev, offset, choice, grp, nullptr, nullptr,
+                      head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL, kpool, slinked, plinked, work, nf};
+        hipLaunchKernelGGL(k_fm_probe_s<E>, dim3(g), dim3(256), 0, st, cs, ranged ? (const uint32_t*)rng : nullptr, bp,
+                           off, (vb && off < vmax) ? (const uint64_t*)vb : nullptr, vnw, lead, gcnt, gmem,
+                           (uint32_t*)nullptr, 0ull);
+        hipLaunchKernelGGL((k_fm_resolve_s<E>), dim3(3 * c->cus), dim3(256), 0, st, cs, off, lead, gcnt,
+                           (const uint32_t*)gmem);
+    }
+    hipLaunchKernelGGL(k_fm_count_work, dim3(grid_for(n_max, 1024)), dim3(256), 0, st, (const uint32_t*)sl,
+                       (const uint32_t*)d_ns, (const uint32_t*)next, (const uint16_t*)offset, Ls,
+                       (const uint64_t*)vb, vnw, vmax, work);
+}
+
 // Exact all-offset matching (a8/a11 over 2-bit reads, compress.cu:1003-1174;
 // a12 over 3-bit N-reads, compress.cu:1176-1347).  Initial prefix/suffix
 // lists are lexicographic; their lengths are the device counters
@@ -523,6 +603,13 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     uint64_t* kpool = S.as<uint64_t>(S_KPOOL, n_max);
     uint8_t* slinked = S.as<uint8_t>(S_SLINK, n_max);
     uint8_t* plinked = S.as<uint8_t>(S_PLINK, n_max);
+    uint8_t* lead = S.as<uint8_t>(S_FMLEAD, n_max);
+    fill<uint8_t>(c, lead, n_max, (uint8_t)0);
+    // per group (prefix position): member count and the first FM_SMALL members
+    uint32_t* gcnt = S.as<uint32_t>(S_FMGC, n_max);
+    uint32_t* gmem = S.as<uint32_t>(S_FMGM, (uint64_t)n_max * FM_SMALL);
+    fill<uint32_t>(c, gcnt, n_max, 0u);
+    fill<uint32_t>(c, grp, n_max, INV);  // group marks (see k_fm_resolve_s)
     uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)n_max + 1);
     uint32_t* cont = S.as<uint32_t>(S_CONT, (uint64_t)n_max + 1);
     uint32_t* head = S.as<uint32_t>(S_HEAD, (uint64_t)n_max + 1);
@@ -530,9 +617,9 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     fill<uint32_t>(c, owner, (uint64_t)n_max + 1, INV);
     fill<uint32_t>(c, head, (uint64_t)n_max + 1, INV);
     fill<uint32_t>(c, cont, (uint64_t)n_max + 1, 0u);
-    // work: suffix-list lengths summed over offsets, the first offset's length
-    uint64_t* work = S.as<uint64_t>(S_WORK, 4);
-    fill<uint64_t>(c, work, 4, 0ull);
+    // work counts (see the byte count at the end)
+    uint64_t* work = S.as<uint64_t>(S_WORK, 8);
+    fill<uint64_t>(c, work, 8, 0ull);
     uint32_t* np[2] = {cnt + C_NP0, cnt + C_NP1};
     uint32_t* ns[2] = {cnt + C_NS0, cnt + C_NS1};
     CURC_HIP(hipMemcpyAsync(np[0], cnt + C_PRE, 4, hipMemcpyDeviceToDevice, c->st));
@@ -556,6 +643,12 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
         hipLaunchKernelGGL((k_fm_fused<E>), dim3(1), dim3(1024), 0, c->st, c0, c1, L);
         return;
     }
+    // Offsets before the last FM_TAIL run over the lists in place (few reads
+    // link per offset there); the lists are then compacted once, and the
+    // tail, where most of the remaining reads link in large groups, runs
+    // with both lists compacted after every offset.
+    constexpr uint32_t FM_TAIL = 16;
+    const uint32_t Ls = (c->tune.fm_static && L > FM_TAIL + 1) ? L - FM_TAIL : 1;
     // both lists compacted in one single-pass launch (up to COMPACT2_MAX_TILES
     // tiles), else by two three-launch compactions
     // (its status words are cleared by each offset's finalisation)
@@ -568,32 +661,45 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
         lb_status = (uint32_t*)lb->status;
         lb_words = compact2_words(n_max, n_max);
     }
+    auto compact_lists = [&](int cur) {
+        if (lb) {
+            compact2(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], n_max, np[cur],
+                     PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]}, np[1 - cur],
+                     n_max, lb, c->st);
+        } else {
+            compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S, S_PRIM,
+                    c->st, n_max);
+            compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
+                    np[1 - cur], S, S_PRIM, c->st, n_max);
+        }
+    };
+    // the compacted tail's counts go to work[4] (suffix-list lengths) and
+    // work[5] (the first offset's length when there is no in-place phase)
+    const int cur0 = Ls > 1 ? 1 : 0;
     auto loop = [&]() {
-        int cur = 0;
-        for (uint32_t off = 1; off < L; ++off) {
+        if (Ls > 1) {
+            fm_static_phase<E>(c, R, W, L, Ls, n_max, ranged, MAX_BP, BITS, cnt, next, prev, offset, pl[0], pk[0],
+                               sl[0], np[0], ns[0], choice, grp, memb, resl, pool, kpool, slinked, plinked, lead,
+                               gcnt, gmem, head, work, g);
+            compact_lists(0);
+        }
+        int cur = cur0;
+        for (uint32_t off = Ls; off < L; ++off) {
             const uint32_t bp = (L - off >= MAX_BP) ? MAX_BP : (L - off);
             const uint32_t nb = 1u << (BITS * bp);
             if (ranged)
                 hipLaunchKernelGGL(k_fm_ranges<E>, dim3(grid_for((uint64_t)n_max + 1, 4096)), dim3(256), 0, c->st,
                                    (const uint64_t*)pk[cur], (const uint32_t*)np[cur], bp, nb, rng, plinked);
             ClaimState cs{R, W, L, sl[cur], ns[cur], pl[cur], pk[cur], np[cur], next, prev, offset,
-                          choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
-                          kpool, slinked, plinked, work};
-            hipLaunchKernelGGL(k_fm_probe<E>, dim3(g), dim3(256), 0, c->st, cs,
-                               ranged ? (const uint32_t*)rng : nullptr, bp, off);
-            hipLaunchKernelGGL(k_claim_finalize, dim3(g), dim3(256), 0, c->st, cs, off, lb_status, lb_words);
+                          choice, grp, nullptr, nullptr, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
+                          kpool, slinked, plinked, work + 4};
+            hipLaunchKernelGGL(k_fm_probe_s<E>, dim3(g), dim3(256), 0, c->st, cs,
+                               ranged ? (const uint32_t*)rng : nullptr, bp, off, (const uint64_t*)nullptr, 0ull, lead,
+                               gcnt, gmem, lb_status, lb_words);
             // three workgroups per CU (the cooperative sort's LDS allows three)
-            hipLaunchKernelGGL((k_claim_resolve<true, E>), dim3(3 * c->cus), dim3(256), 0, c->st, cs, off);
-            if (lb) {
-                compact2(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], n_max,
-                         np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
-                         np[1 - cur], n_max, lb, c->st);
-            } else {
-                compact(ns[cur], PredFlagClear{slinked}, EmitGather{sl[cur], sl[1 - cur]}, ns[1 - cur], S,
-                        S_PRIM, c->st, n_max);
-                compact(np[cur], PredFlagClear{plinked}, EmitPre{pl[cur], pk[cur], pl[1 - cur], pk[1 - cur]},
-                        np[1 - cur], S, S_PRIM, c->st, n_max);
-            }
+            hipLaunchKernelGGL((k_fm_resolve_s<E>), dim3(3 * c->cus), dim3(256), 0, c->st, cs, off, lead, gcnt,
+                               (const uint32_t*)gmem);
+            compact_lists(cur);
             cur = 1 - cur;
         }
     };
@@ -608,19 +714,30 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     key.v[7] = (uint64_t)prev;
     key.v[8] = (uint64_t)offset;
     key.v[9] = (uint64_t)cnt;
+    key.v[10] = Ls;
+    key.v[11] = g;
+    key.v[12] = c->cus;
     c->graphs.run(key, S, c->st, loop);
-    uint64_t w[2] = {0, 0};
+    uint64_t w[8] = {};
     uint32_t fin = 0, lb_err = 0;
-    CURC_HIP(hipMemcpyAsync(w, work, 16, hipMemcpyDeviceToHost, c->st));
-    CURC_HIP(hipMemcpyAsync(&fin, ns[(L - 1) & 1], 4, hipMemcpyDeviceToHost, c->st));
+    CURC_HIP(hipMemcpyAsync(w, work, sizeof(w), hipMemcpyDeviceToHost, c->st));
+    CURC_HIP(hipMemcpyAsync(&fin, ns[cur0 ^ ((L - Ls) & 1)], 4, hipMemcpyDeviceToHost, c->st));
     if (lb) CURC_HIP(hipMemcpyAsync(&lb_err, &lb->err, 4, hipMemcpyDeviceToHost, c->st));
     CURC_HIP(hipStreamSynchronize(c->st));
     if (lb_err != 0) throw std::runtime_error("exact matching: list compaction look-back did not complete");
-    // suffix-list reads per offset (the read's words and its id, first word) plus one
-    // prefix read per link
+    // In place: per offset the suffix flags (a byte each) and, where there
+    // are verdicts, their row (a bit each); per search the suffix's read and
+    // block range.  Compacted: per offset and listed suffix its read and id.
+    // Per link the prefix read.
     const double P8 = 8.0 * W;
-    c->add_bytes(stage, (double)w[0] * (P8 + 8) + (double)(w[1] - std::min<uint64_t>(w[1], fin)) * (4 + P8),
-                 (double)w[0]);
+    const uint64_t ns0 = std::max(w[1], w[5]);
+    double bytes = (double)w[4] * (P8 + 8) + (double)(ns0 - std::min<uint64_t>(ns0, fin)) * (4 + P8);
+    if (Ls > 1) {
+        const bool verd = E::BPW == 32 && W <= 8 && L > 32;
+        const uint32_t vrows = verd ? std::min(L - 31, Ls) - 1 : 0;
+        bytes += (double)(Ls - 1) * ns0 + (double)vrows * ns0 / 8.0 + (double)w[2] * (P8 + 8);
+    }
+    c->add_bytes(stage, bytes, (double)(w[0] + w[4]));
 }
 
 // Record sort and stream builders (a13; compress.cu:1730-2044), on the
@@ -734,6 +851,7 @@ void parse_tuning(const char* text, Tuning& t) {
             if (k == "fm_fused") t.fm_fused = v;
             if (k == "ov_queue" && v <= 2) t.ov_queue = v;
             if (k == "map_both" && v <= 1) t.map_both = v;
+            if (k == "fm_static" && v <= 1) t.fm_static = v;
             if (v > 0) {
                 if (k == "ov_grid") t.ov_grid = v;
                 else if (k == "fm_grid") t.fm_grid = v;
@@ -1586,11 +1704,11 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         if (P.k == 32)                                                                                 \
             hipLaunchKernelGGL((k_ov_verdicts<WM, true>), vg, dim3(256), 0, st, (const uint64_t*)packed,      \
                                (const uint32_t*)sl[0], nsuf, W, P.k, P.kip, P.max_off, (const uint32_t*)pfilt, \
-                               bmask, vb, vnw);                                                        \
+                               bmask, vb, vnw, (const uint32_t*)nullptr);                              \
         else                                                                                           \
             hipLaunchKernelGGL((k_ov_verdicts<WM, false>), vg, dim3(256), 0, st, (const uint64_t*)packed,     \
                                (const uint32_t*)sl[0], nsuf, W, P.k, P.kip, P.max_off, (const uint32_t*)pfilt, \
-                               bmask, vb, vnw);                                                        \
+                               bmask, vb, vnw, (const uint32_t*)nullptr);                              \
     } while (0)
                 if (W <= 3)
                     CURC_OV_VERDICTS(3);
@@ -3187,6 +3305,7 @@ int curc_ctx_get_tuning(curc_ctx* c, curc_tuning* t) {
     t->fm_fused = c->tune.fm_fused;
     t->ov_queue = c->tune.ov_queue;
     t->map_both = c->tune.map_both;
+    t->fm_static = c->tune.fm_static;
     return CURC_OK;
 }
 
@@ -3194,7 +3313,8 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!c || !t) return CURC_EINVAL;
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
         (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
-        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2 || t->map_both > 1)
+        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2 || t->map_both > 1 ||
+        t->fm_static > 1)
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;
     c->tune.fm_grid = t->fm_grid;
@@ -3205,6 +3325,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     c->tune.fm_fused = t->fm_fused;
     c->tune.ov_queue = t->ov_queue;
     c->tune.map_both = t->map_both;
+    c->tune.fm_static = t->fm_static;
     return CURC_OK;
 }
 

@@ -399,8 +399,9 @@ int curc_ctx_set_trace(curc_ctx *ctx, uint32_t every, uint64_t max_records);
  * the mapping kernels, keys per thread of a radix-sort tile, the streams
  * curc_batch_run uses when its caller passes 0, the size of the overlap
  * stage's prefix k-mer filter, the list bound below which an
- * exact-matching loop runs fused in one launch, and the form of the overlap
- * probe.  curc_amd/autotune.py
+ * exact-matching loop runs fused in one launch, the form of the overlap
+ * probe, the mapping index layout and the exact-matching list handling.
+ * curc_amd/autotune.py
  * times candidate values on a sample block and sets the fastest. */
 typedef struct curc_tuning {
     uint32_t ov_grid;     /* >= 1 (default 16384) */
@@ -419,6 +420,9 @@ typedef struct curc_tuning {
     uint32_t map_both;    /* 1: both strand passes of the mapping in one launch, on an index that
                              holds a bucket's forward and reverse-complement slots in one line;
                              0: one launch and one index per strand (default 1) */
+    uint32_t fm_static;   /* 1: exact matching keeps its lists in place (linked reads skipped by
+                             flag and skip pointer, suffixes screened by a prefix filter); 0: both
+                             lists compacted after every offset (default 1) */
 } curc_tuning;
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
 int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);

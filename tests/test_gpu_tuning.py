@@ -21,10 +21,11 @@ def reads_of(seed, n, L, genome):
 
 
 TUNINGS = [
-    dict(ov_grid=1, fm_grid=1, map_grid=1, radix_items=8, filter_bits=8, fm_fused=0, ov_queue=1),
+    dict(ov_grid=1, fm_grid=1, map_grid=1, radix_items=8, filter_bits=8, fm_fused=0, ov_queue=1, fm_static=1),
     dict(ov_grid=7, fm_grid=3, map_grid=5, radix_items=32, filter_bits=128, fm_fused=1 << 20, ov_queue=0),
-    dict(ov_grid=16384, fm_grid=16384, map_grid=32768, radix_items=16, fm_fused=4096, ov_queue=0),
-    dict(ov_grid=4096, fm_grid=4096, map_grid=1 << 20, radix_items=16, fm_fused=0, ov_queue=1),
+    dict(ov_grid=16384, fm_grid=16384, map_grid=32768, radix_items=16, fm_fused=4096, ov_queue=0, fm_static=0),
+    dict(ov_grid=4096, fm_grid=4096, map_grid=1 << 20, radix_items=16, fm_fused=0, ov_queue=1, fm_static=0),
+    dict(fm_grid=5, fm_fused=0, fm_static=1, filter_bits=8),
     dict(ov_grid=3, filter_bits=16, ov_queue=1),
     dict(ov_grid=1 << 20, filter_bits=16, ov_queue=1),
     dict(ov_grid=16384, filter_bits=8, ov_queue=2),
@@ -50,7 +51,7 @@ def test_results_do_not_depend_on_tuning(L):
 def test_invalid_tuning_is_refused(gpu_ctx):
     before = gpu_ctx.get_tuning()
     for bad in (dict(radix_items=12), dict(map_grid=0), dict(workers=5), dict(filter_bits=24), dict(filter_bits=4),
-                dict(ov_queue=3), dict(map_both=2)):
+                dict(ov_queue=3), dict(map_both=2), dict(fm_static=2)):
         with pytest.raises(ca.CurcError):
             gpu_ctx.set_tuning(**bad)
     assert gpu_ctx.get_tuning() == before
