@@ -1927,6 +1927,63 @@ __global__ void k_map_slots2(const uint32_t* __restrict__ vals0, const uint32_t*
     }
 }
 
+// Compact two-strand index: one 32-byte slot per bucket, so that the table
+// of a 2^26-bucket index is 2 GiB instead of 8 GiB of 128-byte lines (random
+// probes into a table of 4 GiB or more miss the address-translation caches
+// and run at about two thirds of the rate of probes into one of 1-2 GiB on
+// MI355X).  Slot word 0: the forward size (bits 0-3) and the
+// reverse-complement size (bits 4-7), each capped at MAP_BUCKET.  A bucket
+// with at most CS_INLINE positions in all keeps them in words 1..7, the
+// forward ones first, each strand in its bucket order (as k_map_slots); a
+// larger one (0.2% of the probes that reach an indexed position on a 100 Mbp
+// genome at 30x) keeps in word 1 the offset of its positions, in the same
+// order, in the overflow array.
+constexpr uint32_t CS_INLINE = 7;
+
+// Overflow sizes: the bucket's positions when they do not fit its slot,
+// else 0; entries [0, H] (entry H = 0, so that an exclusive scan ends with
+// the total).
+__global__ void k_cslot_count(const uint32_t* __restrict__ krf0, const uint32_t* __restrict__ krf1, uint64_t H,
+                              uint32_t* __restrict__ ovc) {
+    GRID_STRIDE(h, H + 1) {
+        uint32_t t = 0;
+        if (h < H) {
+            t = min(krf0[h + 1] - krf0[h], MAP_BUCKET) + min(krf1[h + 1] - krf1[h], MAP_BUCKET);
+            if (t <= CS_INLINE) t = 0;
+        }
+        ovc[h] = t;
+    }
+}
+
+// One thread per bucket writes its slot, tab[2h] and tab[2h+1] (and its
+// overflow positions at ovo[h], the exclusive scan of k_cslot_count).
+__global__ void k_cslot_write(const uint32_t* __restrict__ vals0, const uint32_t* __restrict__ krf0,
+                              const uint32_t* __restrict__ vals1, const uint32_t* __restrict__ krf1, uint64_t H,
+                              const uint32_t* __restrict__ ovo, uint4* __restrict__ tab, uint32_t* __restrict__ ovf) {
+    GRID_STRIDE(h, H) {
+        const uint32_t l0 = krf0[h], c0 = min(krf0[h + 1] - l0, MAP_BUCKET);
+        const uint32_t l1 = krf1[h], c1 = min(krf1[h + 1] - l1, MAP_BUCKET);
+        const uint32_t t = c0 + c1;
+        // position k of the bucket: forward first; the first min(n,12)
+        // occurrences of each strand in descending order
+        auto at = [&](uint32_t k) -> uint32_t {
+            return k < c0 ? vals0[l0 + c0 - 1 - k] : vals1[l1 + c1 - 1 - (k - c0)];
+        };
+        uint32_t w[8] = {c0 | (c1 << 4), 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        if (t <= CS_INLINE) {
+#pragma unroll
+            for (uint32_t k = 0; k < CS_INLINE; ++k)
+                if (k < t) w[1 + k] = at(k);
+        } else {
+            const uint32_t o = ovo[h];
+            w[1] = o;
+            for (uint32_t k = 0; k < t; ++k) ovf[o + k] = at(k);
+        }
+        tab[2 * h] = make_uint4(w[0], w[1], w[2], w[3]);
+        tab[2 * h + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+}
+
 // Capped bucket sizes: min(run length, 12), entries [0, H+2).
 __global__ void k_map_caps(const uint32_t* __restrict__ krf, uint64_t H, uint32_t* __restrict__ caps) {
     GRID_STRIDE(b, H + 2) {
@@ -2002,7 +2059,41 @@ struct MapArgs {
     // (MAP_SLOT2, MAP_SLOT * strand)
     uint32_t bk_stride = 16;
     uint32_t bk_off = 0;
+    // compact two-strand index (k_cslot_write): bk holds one 16-byte slot
+    // per bucket, ovf the positions of buckets that do not fit their slot
+    uint32_t cs = 0;
+    const uint32_t* ovf = nullptr;
 };
+
+// Both strands' slots of a bucket of a two-strand index: the 128-byte line
+// form (MAP_SLOT2: f and r are the slots' first 16 bytes, sl the line) or
+// the compact form (CS: f and r are the two halves of the 32-byte slot, sl
+// the bucket's overflow positions when it has any).
+struct Bk2 {
+    uint4 f, r;
+    const uint32_t* sl;
+    uint32_t nf, nr;  // sizes (nr 0 when the reverse slot was not asked for)
+    uint32_t nt;      // compact form: both sizes together
+};
+// Position p of the forward (rc false) or reverse-complement slot.
+template <bool CS>
+__device__ __forceinline__ uint32_t bk2_pos(const Bk2& b, bool rc, uint32_t p) {
+    if (CS) {
+        const uint32_t k = rc ? b.nf + p : p;
+        if (b.nt > CS_INLINE) return ld32(b.sl + k);
+        switch (k) {
+            case 0: return b.f.y;
+            case 1: return b.f.z;
+            case 2: return b.f.w;
+            case 3: return b.r.x;
+            case 4: return b.r.y;
+            case 5: return b.r.z;
+            default: return b.r.w;
+        }
+    }
+    const uint4& hd = rc ? b.r : b.f;
+    return p == 0 ? hd.y : p == 1 ? hd.z : p == 2 ? hd.w : ld32(b.sl + (rc ? MAP_SLOT : 0u) + 1 + p);
+}
 
 // Bucket h of the mapping index: its capped size and its positions.
 struct MapBucket {
@@ -2016,6 +2107,32 @@ __device__ __forceinline__ MapBucket map_bucket(const MapArgs& a, uint32_t h) {
     }
     const uint32_t b = a.kr[h], e = a.kr[h + 1];
     return MapBucket{min(e - b, MAP_BUCKET), a.pos + b};
+}
+
+// Bucket h of a two-strand index; the reverse-complement slot only when
+// want_r (its size reads 0 otherwise).
+template <bool CS>
+__device__ __forceinline__ Bk2 bk2_fetch(const MapArgs& a, uint32_t h, bool want_r) {
+    Bk2 b;
+    b.r = make_uint4(0, 0, 0, 0);
+    if (CS) {
+        const uint4* sl = reinterpret_cast<const uint4*>(a.bk) + 2 * (uint64_t)h;
+        b.f = sl[0];
+        b.r = sl[1];
+        b.nf = b.f.x & 15u;
+        const uint32_t nr = (b.f.x >> 4) & 15u;
+        b.nt = b.nf + nr;
+        b.nr = want_r ? nr : 0u;
+        b.sl = b.nt > CS_INLINE ? a.ovf + b.f.y : a.ovf;
+        return b;
+    }
+    b.sl = a.bk + (uint64_t)h * MAP_SLOT2;
+    b.f = *reinterpret_cast<const uint4*>(b.sl);
+    if (want_r) b.r = *reinterpret_cast<const uint4*>(b.sl + MAP_SLOT);
+    b.nf = b.f.x;
+    b.nr = b.r.x;
+    b.nt = 0;
+    return b;
 }
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
@@ -2048,11 +2165,19 @@ __device__ __forceinline__ void map_nread(const MapArgs& a, uint64_t i, uint32_t
     for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
         ++n_probe;
         uint32_t h = (uint32_t)(murmur64(kmer_ascii(ra + j, MAP_K)) & a.mask);
-        const MapBucket B = map_bucket(a, h);
+        MapBucket B{0u, nullptr};
+        Bk2 cb{};
+        if (a.cs) {
+            cb = bk2_fetch<true>(a, h, RC);
+            B.n = RC ? cb.nr : cb.nf;
+        } else {
+            B = map_bucket(a, h);
+        }
         if (B.n == 0) continue;
         const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
         for (uint32_t p = 0; p < B.n; ++p) {
-            uint64_t rp = a.start + (uint64_t)ld32(B.p + p) * MAP_REF_STEP;
+            const uint32_t pv = a.cs ? bk2_pos<true>(cb, RC, p) : ld32(B.p + p);
+            uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
             if (rp < ls) continue;
             uint64_t mp = rp - ls;
             if (mp + a.L > a.pg_len) continue;
@@ -2394,7 +2519,7 @@ struct SeenCache32 {
 //    exact and it stops at the same candidate as the sequential pass;
 //  - each pass keeps its own SeenCache, and the reverse work is added to
 //    the counters only when the forward pass ends above MAP_TARGET.
-template <uint32_t WM>
+template <uint32_t WM, bool CS>
 __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
     constexpr uint32_t G = 1;  // probes whose slot heads are requested together
     uint32_t n_act = 0, n_probe = 0, n_cand = 0, n_first = 0, n_open_probe = 0, n_open_cand = 0;
@@ -2496,15 +2621,20 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
         for (uint32_t t0 = 0; t0 < np; t0 += G) {
             uint4 hf[G], hr[G];
             uint32_t hh[G];
+            Bk2 bb[G];  // compact index (CS)
 #pragma unroll
             for (uint32_t u = 0; u < G; ++u) {
                 hf[u] = hr[u] = make_uint4(0, 0, 0, 0);
                 hh[u] = 0;
                 if (t0 + u < np) {
                     hh[u] = (uint32_t)(murmur64(kmer(MAP_READ_STEP * (t0 + u))) & a.mask);
-                    const uint32_t* sl = a.bk + (uint64_t)hh[u] * MAP_SLOT2;
-                    hf[u] = *reinterpret_cast<const uint4*>(sl);
-                    if (!doneR) hr[u] = *reinterpret_cast<const uint4*>(sl + MAP_SLOT);
+                    if constexpr (CS) {
+                        bb[u] = bk2_fetch<true>(a, hh[u], !doneR);
+                    } else {
+                        const uint32_t* sl = a.bk + (uint64_t)hh[u] * MAP_SLOT2;
+                        hf[u] = *reinterpret_cast<const uint4*>(sl);
+                        if (!doneR) hr[u] = *reinterpret_cast<const uint4*>(sl + MAP_SLOT);
+                    }
                 }
             }
 #pragma unroll
@@ -2516,10 +2646,18 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                 // forward pass, probe t
                 ++pF;
                 {
-                    const uint32_t nb = hf[u].x;
+                    uint32_t nb;
+                    if constexpr (CS)
+                        nb = bb[u].nf;
+                    else
+                        nb = hf[u].x;
                     const uint64_t ls = (uint64_t)j;
                     for (uint32_t p = 0; p < nb; ++p) {
-                        const uint32_t pv = p == 0 ? hf[u].y : p == 1 ? hf[u].z : p == 2 ? hf[u].w : ld32(sl + 1 + p);
+                        uint32_t pv;
+                        if constexpr (CS)
+                            pv = bk2_pos<true>(bb[u], false, p);
+                        else
+                            pv = p == 0 ? hf[u].y : p == 1 ? hf[u].z : p == 2 ? hf[u].w : ld32(sl + 1 + p);
                         const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
                         if (rp < ls) continue;
                         const uint64_t mp = rp - ls;
@@ -2537,13 +2675,21 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                 // reverse pass, probe t
                 if (!doneR) {
                     ++pR;
-                    const uint32_t nb = hr[u].x;
+                    uint32_t nb;
+                    if constexpr (CS)
+                        nb = bb[u].nr;
+                    else
+                        nb = hr[u].x;
                     const uint64_t ls = (uint64_t)(L - MAP_K - j);
                     for (uint32_t p = 0; p < nb; ++p) {
-                        const uint32_t pv = p == 0   ? hr[u].y
-                                            : p == 1 ? hr[u].z
-                                            : p == 2 ? hr[u].w
-                                                     : ld32(sl + MAP_SLOT + 1 + p);
+                        uint32_t pv;
+                        if constexpr (CS)
+                            pv = bk2_pos<true>(bb[u], true, p);
+                        else
+                            pv = p == 0   ? hr[u].y
+                                 : p == 1 ? hr[u].z
+                                 : p == 2 ? hr[u].w
+                                          : ld32(sl + MAP_SLOT + 1 + p);
                         const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
                         if (rp < ls) continue;
                         const uint64_t mp = rp - ls;

@@ -151,7 +151,7 @@ struct Tuning {
     uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
     uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
     uint32_t fm_fused = 4096;    // exact-matching lists up to this bound run in one fused launch (0: never)
-    uint32_t map_both = 1;       // both strand passes in one launch on a two-strand index (1)
+    uint32_t map_both = 2;       // both strand passes in one launch on a two-strand index of 128-byte lines (1) or 32-byte slots (2)
     uint32_t ov_queue = 2;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
     uint32_t fm_static = 1;      // exact matching over lists kept in place (1) or compacted per offset (0)
@@ -780,16 +780,24 @@ RmIndex rm_index(curc_ctx* c, const uint64_t* src, uint64_t src_len) {
     return ix;
 }
 
-// k_map_both for the read's word count W <= 8 (register capacity 3, 4, 5 or 8).
-void launch_map_both(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
+// k_map_both for the read's word count W <= 8 (register capacity 3, 4, 5 or
+// 8) and the index form (a.cs: compact slots).
+template <bool CS>
+void launch_map_both_t(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
     if (W <= 3)
-        hipLaunchKernelGGL(k_map_both<3>, grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<3, CS>), grid, dim3(256), 0, st, a);
     else if (W <= 4)
-        hipLaunchKernelGGL(k_map_both<4>, grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<4, CS>), grid, dim3(256), 0, st, a);
     else if (W <= 5)
-        hipLaunchKernelGGL(k_map_both<5>, grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<5, CS>), grid, dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL(k_map_both<8>, grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<8, CS>), grid, dim3(256), 0, st, a);
+}
+void launch_map_both(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
+    if (a.cs)
+        launch_map_both_t<true>(W, grid, st, a);
+    else
+        launch_map_both_t<false>(W, grid, st, a);
 }
 
 // k_map_fast for the read's word count W <= 8 (register capacity 3, 4, 5 or 8).
@@ -850,7 +858,7 @@ void parse_tuning(const char* text, Tuning& t) {
             const uint32_t v = (uint32_t)std::strtoul(kv.c_str() + e + 1, nullptr, 10);
             if (k == "fm_fused") t.fm_fused = v;
             if (k == "ov_queue" && v <= 2) t.ov_queue = v;
-            if (k == "map_both" && v <= 1) t.map_both = v;
+            if (k == "map_both" && v <= 2) t.map_both = v;
             if (k == "fm_static" && v <= 1) t.fm_static = v;
             if (v > 0) {
                 if (k == "ov_grid") t.ov_grid = v;
@@ -1906,7 +1914,11 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         if (both) {
             const uint64_t count = index_count;
             const uint32_t H = H1;
-            uint32_t* bk = S.as<uint32_t>(S_MBK, (uint64_t)H * MAP_SLOT2);
+            // compact slots (tuning map_both 2) or 128-byte lines (1)
+            const bool cs = c->tune.map_both == 2;
+            uint32_t* bk = cs ? reinterpret_cast<uint32_t*>(S.as<uint4>(S_MBK, 2 * (uint64_t)H))
+                              : S.as<uint32_t>(S_MBK, (uint64_t)H * MAP_SLOT2);
+            uint32_t* ovf = nullptr;
             {
                 Region r(c, "map_index");
                 // each strand's (hash, position) pairs sorted, then both slots
@@ -1928,8 +1940,22 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     sv[strand] = v0;
                     sr[strand] = krf;
                 }
-                hipLaunchKernelGGL(k_map_slots2, dim3(grid_for((uint64_t)H * 8, 1u << 20)), dim3(256), 0, st, sv[0],
-                                   sr[0], sv[1], sr[1], (uint64_t)H, bk);
+                if (cs) {
+                    // overflow offsets by a scan of the overflow sizes (the CSR
+                    // slot S_MKR is free on this path), overflow positions in
+                    // S_MPOS, at most all positions of both strands
+                    uint32_t* ovc = S.as<uint32_t>(S_MKR, (uint64_t)H + 2);
+                    ovf = S.as<uint32_t>(S_MPOS, 2 * count + 2);
+                    hipLaunchKernelGGL(k_cslot_count, dim3(grid_for((uint64_t)H + 1)), dim3(256), 0, st, sr[0],
+                                       sr[1], (uint64_t)H, ovc);
+                    scan_excl(ovc, ovc, (uint64_t)H + 1, S, S_PRIM, st);
+                    hipLaunchKernelGGL(k_cslot_write, dim3(grid_for((uint64_t)H)), dim3(256), 0, st, sv[0], sr[0],
+                                       sv[1], sr[1], (uint64_t)H, (const uint32_t*)ovc, reinterpret_cast<uint4*>(bk),
+                                       ovf);
+                } else {
+                    hipLaunchKernelGGL(k_map_slots2, dim3(grid_for((uint64_t)H * 8, 1u << 20)), dim3(256), 0, st,
+                                       sv[0], sr[0], sv[1], sr[1], (uint64_t)H, bk);
+                }
                 c->add_bytes("map_index", 2.0 * ((double)count * (16 + 8) + 8.0 * H), 2.0 * (double)count);
             }
             unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
@@ -1940,6 +1966,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                           Nascii, nN, last, word, work, P.max_mm, bk};
                 a.bk_stride = MAP_SLOT2;
                 a.bk_off = 0;
+                a.cs = cs ? 1u : 0u;
+                a.ovf = ovf;
                 launch_map_both(W, dim3(grid_for(nm, c->tune.map_grid)), st, a);
             }
             if (c->timing) {
@@ -3313,7 +3341,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!c || !t) return CURC_EINVAL;
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
         (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
-        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2 || t->map_both > 1 ||
+        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2 || t->map_both > 2 ||
         t->fm_static > 1)
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;

@@ -418,8 +418,10 @@ typedef struct curc_tuning {
                              one pass beforehand (reads of at most 256 bases; others use 1);
                              0: one suffix per thread throughout (default 2) */
     uint32_t map_both;    /* 1: both strand passes of the mapping in one launch, on an index that
-                             holds a bucket's forward and reverse-complement slots in one line;
-                             0: one launch and one index per strand (default 1) */
+                             holds a bucket's forward and reverse-complement slots in one 128-byte
+                             line; 2: as 1, on an index of one 32-byte slot per bucket (positions
+                             of larger buckets in an overflow array); 0: one launch and one index
+                             per strand (default 2) */
     uint32_t fm_static;   /* 1: exact matching keeps its lists in place (linked reads skipped by
                              flag and skip pointer, suffixes screened by a prefix filter); 0: both
                              lists compacted after every offset (default 1) */

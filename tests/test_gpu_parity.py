@@ -78,9 +78,11 @@ def test_overlap_filter_verdicts(L):
 
 @pytest.mark.parametrize("case", ["plain", "repeats"])
 def test_mapping_both_strands_one_launch(case):
-    """Tuning map_both=1 (the default): both strand passes in one launch on the two-strand
-    index (taken by blocks whose pseudogenome fills bucket-inline slots, so
-    6 and 9 Mbp genomes at 30x here).  Same results as the oracle (OpenMP build),
+    """Tuning map_both=1: both strand passes in one launch on the two-strand index of
+    128-byte lines (taken by blocks whose pseudogenome fills bucket-inline slots,
+    so 6 and 9 Mbp genomes at 30x here); map_both=2 (the default): the same on the
+    compact index of 32-byte slots, whose larger buckets (repeats) keep their
+    positions in the overflow array.  Same results as the oracle (OpenMP build),
     and the same work counters as the forward pass followed by the
     reverse-complement pass."""
     if case == "plain":
@@ -106,13 +108,20 @@ def test_mapping_both_strands_one_launch(case):
         b0, separate, st0 = work(ctx)
         ctx.set_tuning(map_both=1)
         b1, together, st1 = work(ctx)
+        ctx.set_tuning(map_both=2)
+        b2, compact, st2 = work(ctx)
         assert st1["mapping"]["launches"] == 1 and st0["mapping"]["launches"] == 2, "the combined path did not run"
+        assert st2["mapping"]["launches"] == 1
         assert together == separate
+        assert compact == separate
         o = orc.OracleBlock(reads, stop="all", kind="omp")
-        compare(b1, o, ("hq_pg", "hq_rec", "map_rec", "map_mis", "unmapped", "unmapped_N", "lq_pg", "lq_rec",
-                        "nlq_pg", "nlq_rec"))
+        fields = ("hq_pg", "hq_rec", "map_rec", "map_mis", "unmapped", "unmapped_N", "lq_pg", "lq_rec", "nlq_pg",
+                  "nlq_rec")
+        compare(b1, o, fields)
+        compare(b2, o, fields)
         b0.free()
         b1.free()
+        b2.free()
 
 
 def test_high_coverage_duplicates(gpu_ctx):
