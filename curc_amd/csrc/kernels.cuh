@@ -341,6 +341,44 @@ struct Enc3 {
     }
 };
 
+template <uint32_t WM>
+__device__ __forceinline__ uint64_t pick(const uint64_t (&a)[WM], uint32_t i) {
+    uint64_t v = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < WM; ++k) v = (i == k) ? a[k] : v;
+    return v;
+}
+
+// n (1..32) bases from base `off` of a read held in registers (as xw).
+template <uint32_t WM>
+__device__ __forceinline__ uint64_t xw_pick(const uint64_t (&w)[WM], uint32_t off, uint32_t n) {
+    const uint32_t o1 = off >> 5;
+    const uint32_t o2 = (off & 31u) << 1;
+    const uint64_t left = pick(w, o1) << o2;
+    if ((64u - o2) >= (n << 1)) return left >> ((32u - n) << 1);
+    const uint64_t right = (pick(w, o1 + 1) >> (62u - o2)) >> 2;
+    return (left | right) >> ((32u - n) << 1);
+}
+
+// Overlap filter verdicts for every offset in one pass (minimizer-blocked
+// filter): bit i of vb[(off - 1) * nw + i / 64] is set when the k-mer of
+// suffix i at kip + off has both of its bits set, for off in [1, max_off).
+// Also the exact-matching verdicts (kip 0, k 32: the suffix's 32 bases from
+// off against the prefixes' first 32 bases).
+// Thread i holds its read's words (its row of the packed reads R, through
+// the suffix list sl), and every lane does the same work per offset, so
+// lanes do not diverge.  For k = 32
+// (K32) offsets go in chunks of NM = 17, the m-mers per k-mer: with the 64
+// bases from the chunk's first k-mer in two words, each k-mer and m-mer of
+// the chunk is a fixed shift of them, and the least m-mer hash of each
+// window is min(suffix minimum over the chunk's m-mers from its start,
+// prefix minimum over the next chunk's m-mers up to its end).  For other k
+// the minimizer is recomputed from each k-mer.  The filter block of the
+// current minimizer is kept in the thread's own LDS slot and reloaded only
+// when the minimizer changes: between two offsets of one thread the other
+// resident threads touch too many blocks for the caches to keep it.  Each
+// 64-suffix word is written by its wave with one ballot per offset.
+
 // p[0:len) vs s[off:off+len), with p's first word given and s's first
 // min(BPW,len) bases pre-extracted in sx0.
 template <class E>
@@ -557,6 +595,89 @@ __device__ __forceinline__ uint32_t ov_candidate(const ClaimState& cs, const uin
     return ch;
 }
 
+// ov_candidate for the verdict form, with rows held in registers.  The
+// suffix's row is read once (its words requested together) and supplies the
+// k-mer and every chunk of s[off:L); a bucket entry whose first word equals
+// s[off:off+32) has its whole row requested at once and compared in
+// registers.  Comparing from memory costs one more round of dependent loads
+// per further 32-base chunk, and the lower bound and the first-free scan
+// compare the same entry twice.  Same lower bound, same first-free scan,
+// same result as ov_candidate; buckets of more than PSLOT_N entries (and
+// indexes without slots) go to ov_candidate.
+template <uint32_t WM>
+__device__ __forceinline__ uint32_t ov_candidate_r(const ClaimState& cs, const uint32_t* __restrict__ kr,
+                                                   const PrefixSlot* __restrict__ slots, uint32_t ns, uint32_t i,
+                                                   uint32_t k, uint32_t kip, uint32_t mask, uint32_t off,
+                                                   uint32_t* grp) {
+    const uint32_t W = cs.W;
+    const uint32_t len = cs.L - off;
+    const uint32_t s = cs.sl[i];
+    uint64_t r[WM];
+    {
+        const uint64_t* rs = cs.R + (uint64_t)s * W;
+#pragma unroll
+        for (uint32_t w = 0; w < WM; ++w) r[w] = w < W ? ld64(rs + w) : 0ull;
+    }
+    const uint32_t h = (uint32_t)(murmur64(xw_pick(r, kip + off, k)) & mask);
+    if (!slots) return ov_candidate(cs, kr, slots, nullptr, ns, i, h, off, grp);
+    const PrefixSlot sl = slots[h];
+    if (sl.cnt > PSLOT_N) return ov_candidate(cs, kr, slots, nullptr, ns, i, h, off, grp);
+    const uint32_t n1 = min(32u, len);
+    const uint64_t sx0 = xw_pick(r, off, n1);
+    // sign of p[0:len) - s[off:L) for each entry (its first word decides
+    // unless it equals s[off:off+32))
+    int res[PSLOT_N];
+#pragma unroll
+    for (uint32_t t = 0; t < PSLOT_N; ++t) {
+        const uint64_t a = Enc2::head(sl.key[t], n1);
+        res[t] = a == sx0 ? 0 : (a < sx0 ? -1 : 1);
+    }
+    if (len > 32) {
+#pragma unroll
+        for (uint32_t t = 0; t < PSLOT_N; ++t) {
+            if (t < sl.cnt && res[t] == 0) {
+                uint64_t q[WM];
+                const uint64_t* rp = cs.R + (uint64_t)sl.id[t] * W;
+#pragma unroll
+                for (uint32_t w = 0; w < WM; ++w) q[w] = w < W ? ld64(rp + w) : 0ull;
+                for (uint32_t j = 32; j < len; j += 32) {
+                    const uint32_t n = min(32u, len - j);
+                    const uint64_t x = xw_pick(q, j, n), y = xw_pick(r, off + j, n);
+                    if (x != y) {
+                        res[t] = x < y ? -1 : 1;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    // lower bound: the first entry not below s[off:L); from there the first
+    // entry that is neither s nor taken decides, and past the equal run
+    // nothing can match (sorted bucket)
+    uint32_t b = sl.cnt;
+#pragma unroll
+    for (uint32_t t = PSLOT_N; t-- > 0;)
+        if (t < sl.cnt && res[t] >= 0) b = t;
+    uint32_t ch = INV;
+    bool done = false;
+#pragma unroll
+    for (uint32_t t = 0; t < PSLOT_N; ++t) {
+        if (!done && t >= b && t < sl.cnt) {
+            if (res[t] != 0) {
+                done = true;
+            } else {
+                const uint32_t p = sl.id[t];
+                if (p != s && cs.prev[p] == INV) {
+                    ch = sl.lo + t;
+                    done = true;
+                }
+            }
+        }
+    }
+    *grp = sl.lo + b;
+    return ch;
+}
+
 // k-mer filter verdict: a clear bit proves that no prefix has this k-mer
 // (exact, no false negatives).
 __device__ __forceinline__ bool ov_filter_pass(const uint32_t* __restrict__ filt, uint32_t fmask, uint64_t m) {
@@ -615,10 +736,11 @@ static_assert(OVQ_CAP >= OVQ_TILE + 512 && (OVQ_CAP & (OVQ_CAP - 1)) == 0, "over
 
 // VERD: the filter verdicts were computed for all offsets beforehand
 // (k_ov_verdicts; bit i of vb[(off - 1) * nw + i / 64]), so the filter pass
-// reads one bit per suffix and the candidate step hashes the k-mer itself,
-// from the suffix's packed row (sw is null: only the few suffixes that pass
-// read words, and they read that row anyway).
-template <bool VERD>
+// reads one bit per suffix and the candidate step (ov_candidate_r, rows of
+// at most WM words) hashes the k-mer itself, from the suffix's packed row
+// (sw is null: only the few suffixes that pass read words, and they read
+// that row anyway).
+template <bool VERD, uint32_t WM>
 __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_t* __restrict__ kr,
                                                     const PrefixSlot* __restrict__ slots,
                                                     const uint32_t* __restrict__ filt, uint32_t fmask, uint32_t mask,
@@ -644,10 +766,11 @@ __global__ void __launch_bounds__(256) k_ov_probe_q(ClaimState cs, const uint32_
         uint32_t ch = INV, i = 0;
         if (valid) {
             i = qi[q & (OVQ_CAP - 1)];
-            const uint32_t h = VERD ? (uint32_t)(murmur64(xw(cs.R + (uint64_t)cs.sl[i] * cs.W, kip + off, k)) & mask)
-                                    : qh[q & (OVQ_CAP - 1)];
             uint32_t g;
-            ch = ov_candidate(cs, kr, slots, sw, ns, i, h, off, &g);
+            if constexpr (VERD)
+                ch = ov_candidate_r<WM>(cs, kr, slots, ns, i, k, kip, mask, off, &g);
+            else
+                ch = ov_candidate(cs, kr, slots, sw, ns, i, qh[q & (OVQ_CAP - 1)], off, &g);
             if (ch != INV) {
                 cs.choice[i] = ch;
                 cs.grp[i] = g;
@@ -2269,43 +2392,6 @@ __global__ void __launch_bounds__(256) k_map(MapArgs a) {
 constexpr uint32_t MAP_GROUP = 4;
 
 // a[i] for a register array and a run-time i (0 if i >= WM), by selects.
-template <uint32_t WM>
-__device__ __forceinline__ uint64_t pick(const uint64_t (&a)[WM], uint32_t i) {
-    uint64_t v = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < WM; ++k) v = (i == k) ? a[k] : v;
-    return v;
-}
-
-// n (1..32) bases from base `off` of a read held in registers (as xw).
-template <uint32_t WM>
-__device__ __forceinline__ uint64_t xw_pick(const uint64_t (&w)[WM], uint32_t off, uint32_t n) {
-    const uint32_t o1 = off >> 5;
-    const uint32_t o2 = (off & 31u) << 1;
-    const uint64_t left = pick(w, o1) << o2;
-    if ((64u - o2) >= (n << 1)) return left >> ((32u - n) << 1);
-    const uint64_t right = (pick(w, o1 + 1) >> (62u - o2)) >> 2;
-    return (left | right) >> ((32u - n) << 1);
-}
-
-// Overlap filter verdicts for every offset in one pass (minimizer-blocked
-// filter): bit i of vb[(off - 1) * nw + i / 64] is set when the k-mer of
-// suffix i at kip + off has both of its bits set, for off in [1, max_off).
-// Also the exact-matching verdicts (kip 0, k 32: the suffix's 32 bases from
-// off against the prefixes' first 32 bases).
-// Thread i holds its read's words (its row of the packed reads R, through
-// the suffix list sl), and every lane does the same work per offset, so
-// lanes do not diverge.  For k = 32
-// (K32) offsets go in chunks of NM = 17, the m-mers per k-mer: with the 64
-// bases from the chunk's first k-mer in two words, each k-mer and m-mer of
-// the chunk is a fixed shift of them, and the least m-mer hash of each
-// window is min(suffix minimum over the chunk's m-mers from its start,
-// prefix minimum over the next chunk's m-mers up to its end).  For other k
-// the minimizer is recomputed from each k-mer.  The filter block of the
-// current minimizer is kept in the thread's own LDS slot and reloaded only
-// when the minimizer changes: between two offsets of one thread the other
-// resident threads touch too many blocks for the caches to keep it.  Each
-// 64-suffix word is written by its wave with one ballot per offset.
 template <uint32_t WM, bool K32>
 __global__ void __launch_bounds__(256) k_ov_verdicts(const uint64_t* __restrict__ R, const uint32_t* __restrict__ sl,
                                                      uint32_t ns, uint32_t W, uint32_t k, uint32_t kip,

@@ -1749,14 +1749,23 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                                   choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                                   nullptr, slinked, nullptr};
                     if (queued && vb) {
-                        hipLaunchKernelGGL(k_ov_probe_q<true>, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
-                                           (const PrefixSlot*)pslots, (const uint32_t*)nullptr, fmask, mask,
-                                           (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn,
-                                           (const uint64_t*)vb, vnw, passed);
+#define CURC_OV_PROBE_V(WM)                                                                                    \
+    hipLaunchKernelGGL((k_ov_probe_q<true, WM>), dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,            \
+                       (const PrefixSlot*)pslots, (const uint32_t*)nullptr, fmask, mask, (const uint64_t*)sw, P.k, \
+                       P.kip, o, per, qcand, qn, (const uint64_t*)vb, vnw, passed)
+                        if (W <= 3)
+                            CURC_OV_PROBE_V(3);
+                        else if (W <= 4)
+                            CURC_OV_PROBE_V(4);
+                        else if (W <= 5)
+                            CURC_OV_PROBE_V(5);
+                        else
+                            CURC_OV_PROBE_V(8);
+#undef CURC_OV_PROBE_V
                         hipLaunchKernelGGL(k_ov_finalize_q, dim3(gq), dim3(256), 0, st, cs, o, per,
                                            (const uint32_t*)qcand, (const uint32_t*)qn);
                     } else if (queued) {
-                        hipLaunchKernelGGL(k_ov_probe_q<false>, dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
+                        hipLaunchKernelGGL((k_ov_probe_q<false, 1>), dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,
                                            (const PrefixSlot*)pslots, (const uint32_t*)pfilt, fmask, mask,
                                            (const uint64_t*)sw, P.k, P.kip, o, per, qcand, qn,
                                            (const uint64_t*)nullptr, (uint64_t)0, passed);
