@@ -59,6 +59,24 @@ static uint64_t* static_scratch_init(int device) {
 
 namespace {
 
+// Waits for a stream's work by polling it (the block's many short host
+// waits: counts read back between stages).  A blocking wait wakes the host
+// thread tens of microseconds after the GPU has finished, and the GPU idles
+// meanwhile; polling returns within a few.
+#ifndef CURC_SPIN_SYNC
+#define CURC_SPIN_SYNC 1
+#endif
+inline hipError_t stream_wait(hipStream_t st) {
+#if CURC_SPIN_SYNC
+    hipError_t e;
+    while ((e = hipStreamQuery(st)) == hipErrorNotReady) {
+    }
+    return e;
+#else
+    return hipStreamSynchronize(st);
+#endif
+}
+
 enum Slot {
     S_ASCII, S_PACKED, S_ISN, S_NONN, S_NIDS, S_NASCII, S_CNT,
     S_K0, S_K1, S_V0, S_V1, S_NEXT, S_PREV, S_OFF,
@@ -274,7 +292,7 @@ void set_cnt(curc_ctx* c, uint32_t* p, uint32_t v) {
 uint32_t read_cnt(curc_ctx* c, const uint32_t* p) {
     uint32_t v = 0;
     CURC_HIP(hipMemcpyAsync(&v, p, 4, hipMemcpyDeviceToHost, c->st));
-    CURC_HIP(hipStreamSynchronize(c->st));
+    CURC_HIP(stream_wait(c->st));
     return v;
 }
 
@@ -288,7 +306,7 @@ void d2h(curc_ctx* c, V& out, const T* src, uint64_t n) {
 // d2h on the copy stream, after the compute stream's work so far (event k),
 // when the destination is pinned; else on the compute stream.
 void join_side(curc_ctx* c) {
-    if (c->cst) CURC_HIP(hipStreamSynchronize(c->cst));
+    if (c->cst) CURC_HIP(stream_wait(c->cst));
 }
 
 template <typename V, typename T>
@@ -469,7 +487,7 @@ AsmOut assemble(curc_ctx* c, const uint32_t* V, uint32_t n, uint64_t id_space, c
                        (const uint32_t*)nxtR, (const uint16_t*)offR, L, (const uint64_t*)cpos,
                        (const uint32_t*)crec, out.rid, out.pos, out.nb);
     CURC_HIP(hipMemcpyAsync(&out.len, olen + nh, 8, hipMemcpyDeviceToHost, st));
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
     return out;
 }
 
@@ -723,7 +741,7 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     CURC_HIP(hipMemcpyAsync(w, work, sizeof(w), hipMemcpyDeviceToHost, c->st));
     CURC_HIP(hipMemcpyAsync(&fin, ns[cur0 ^ ((L - Ls) & 1)], 4, hipMemcpyDeviceToHost, c->st));
     if (lb) CURC_HIP(hipMemcpyAsync(&lb_err, &lb->err, 4, hipMemcpyDeviceToHost, c->st));
-    CURC_HIP(hipStreamSynchronize(c->st));
+    CURC_HIP(stream_wait(c->st));
     if (lb_err != 0) throw std::runtime_error("exact matching: list compaction look-back did not complete");
     // In place: per offset the suffix flags (a byte each) and, where there
     // are verdicts, their row (a bit each); per search the suffix's read and
@@ -877,7 +895,7 @@ template <typename T>
 T read_one(curc_ctx* c, const T* p) {
     T v{};
     CURC_HIP(hipMemcpyAsync(&v, p, sizeof(T), hipMemcpyDeviceToHost, c->st));
-    CURC_HIP(hipStreamSynchronize(c->st));
+    CURC_HIP(stream_wait(c->st));
     return v;
 }
 
@@ -887,7 +905,7 @@ void put_triples(curc_ctx* c, hvec<uint64_t>& out, const uint64_t* s, const uint
     d2h(c, a, s, n);
     d2h(c, b, l, n);
     d2h(c, e, d, n);
-    CURC_HIP(hipStreamSynchronize(c->st));
+    CURC_HIP(stream_wait(c->st));
     out.resize(3 * n);
     for (uint64_t i = 0; i < n; ++i) {
         out[3 * i] = a[i];
@@ -953,7 +971,7 @@ void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_
         if (c->timing) {
             unsigned long long wk[2];
             CURC_HIP(hipMemcpyAsync(wk, a.work, sizeof(wk), hipMemcpyDeviceToHost, st));
-            CURC_HIP(hipStreamSynchronize(st));
+            CURC_HIP(stream_wait(st));
             // per probe: the 36-mer (9 B of 2-bit text) and two CSR offsets;
             // per verified candidate: its position and both 36-mers; per
             // extended base: a quarter byte of each text
@@ -1070,7 +1088,7 @@ void rm_match(curc_ctx* c, const RmIndex& ix, const uint64_t* src, uint64_t src_
         if (n_acc) CURC_HIP(hipMemcpyAsync(&out.map_off[0], moff, n_acc * w, hipMemcpyDeviceToHost, st));
         CURC_HIP(hipMemcpyAsync(&out.map_len[0], mlen, (uint64_t)len_bytes + 1, hipMemcpyDeviceToHost, st));
     }
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
 }
 
 // ------------------------------------------------- mapping in probe ranges ---
@@ -1191,7 +1209,7 @@ void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOu
     uint32_t total_mis = 0;
     CURC_HIP(hipMemcpyAsync(hgs.data(), gstart, (max_mm + 2) * 4, hipMemcpyDeviceToHost, st));
     CURC_HIP(hipMemcpyAsync(&total_mis, bidx + n, 4, hipMemcpyDeviceToHost, st));
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
     std::vector<uint64_t> mbase(max_mm + 2, 0);
     B->s_mis_off_start.assign(max_mm + 2, 0);
     uint64_t acc = 0;
@@ -1220,7 +1238,7 @@ void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOu
     // destination order (std::sort of four elements is an insertion sort)
     uint32_t hh[20];
     CURC_HIP(hipMemcpyAsync(hh, hist, sizeof(hh), hipMemcpyDeviceToHost, st));
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
     static const char CTX[4] = {'A', 'T', 'C', 'G'};
     static const char DEST[4][4] = {{'C', 'G', 'N', 'T'}, {'A', 'C', 'G', 'N'}, {'A', 'G', 'N', 'T'}, {'A', 'C', 'N', 'T'}};
     auto sym = [](char ch) { return ch == 'A' ? 0 : ch == 'C' ? 1 : ch == 'G' ? 2 : ch == 'T' ? 4 : 3; };
@@ -1269,7 +1287,7 @@ void build_streams(curc_ctx* c, curc_block* B, const AsmOut& hq_asm, const AsmOu
     } else {
         d2h(c, B->s_id_pos, o.id_pos, N);
     }
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
 }
 
 // a12: N-read LQ pseudogenome (compress.cu:1724-1728, 1176-1347).  Same
@@ -1344,7 +1362,7 @@ NlqOut nlq_stage(curc_ctx* x, hipEvent_t wait, const uint8_t* Nascii, uint64_t n
         if (a.len) CURC_HIP(hipMemcpyAsync(&B->nlq_text[0], npg, a.len, hipMemcpyDeviceToHost, st));
         records_to_host(x, a, nullptr, 1, B->nlq_rec);  // is_contain_N
     }
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
     return out;
 
 }
@@ -1411,7 +1429,7 @@ uint64_t fq_parse(curc_ctx* c, const FqInput& in, uint64_t** seq, int tile_slot,
     for (int f = 0; f < nf; ++f)
         CURC_HIP(hipMemcpyAsync(&lines[f], tc + off[f] + tiles[f], 8, hipMemcpyDeviceToHost, st));
     CURC_HIP(hipMemcpyAsync(&fl, flags, 4, hipMemcpyDeviceToHost, st));
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
     if (fl & 1u) throw InputError("FASTQ block does not start with '@'");
     for (int f = 0; f < nf; ++f)
         if ((fl & 2u) || lines[f] % 4) throw InputError("truncated FASTQ record");
@@ -1448,7 +1466,7 @@ uint64_t fq_pack(curc_ctx* c, const FqInput& in, uint32_t L, uint32_t W, uint64_
     }
     uint32_t e = 0;
     CURC_HIP(hipMemcpyAsync(&e, cnt + C_FQERR, 4, hipMemcpyDeviceToHost, st));
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
     if (e) throw InputError("read length isn't fix");  // preprocess.cpp:154
     return N;
 }
@@ -1515,7 +1533,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     {
         uint32_t h[C_COUNT];
         CURC_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
         if (fq && h[C_FQERR]) throw InputError("read length isn't fix");  // preprocess.cpp:154
         n = h[C_NONN];
         nN = h[C_NN];
@@ -1617,7 +1635,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->next, next, N);
         d2h(c, B->prev, prev, N);
         d2h(c, B->offset, off, N);
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
         return;
     }
 
@@ -1820,7 +1838,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->next, next, N);
         d2h(c, B->prev, prev, N);
         d2h(c, B->offset, off, N);
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
         return;
     }
 
@@ -1856,7 +1874,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->next, next, N);
         d2h(c, B->prev, prev, N);
         d2h(c, B->offset, off, N);
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
         return;
     }
 
@@ -1874,7 +1892,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         d2h(c, B->next, next, N);
         d2h(c, B->prev, prev, N);
         d2h(c, B->offset, off, N);
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
     }
     if (stop <= CURC_STAGE_FULLMATCH) return;
 
@@ -1983,7 +2001,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 // as for the separate passes, over both strands
                 unsigned long long wk[8];
                 CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
-                CURC_HIP(hipStreamSynchronize(st));
+                CURC_HIP(stream_wait(st));
                 const double P8 = 8.0 * W;
                 c->add_bytes("mapping", (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
                              (double)wk[2]);
@@ -2070,7 +2088,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         // candidate, its 4 B position plus the P bytes of pseudogenome
                         unsigned long long wk[8];
                         CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
-                        CURC_HIP(hipStreamSynchronize(st));
+                        CURC_HIP(stream_wait(st));
                         const double P8 = 8.0 * W;
                         c->add_bytes(phased ? (learned ? "mapping_learned" : "mapping_traced") : "mapping",
                                      (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
@@ -2118,7 +2136,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             const uint64_t nr = std::min<uint64_t>(read_one(c, S.as<unsigned long long>(S_TRCNT, 1)), c->trace_cap);
             d2h(c, B->trace_x, (const float*)S.as<float>(S_TRX, c->trace_cap * MAP_NFEAT), nr * MAP_NFEAT);
             d2h(c, B->trace_y, (const float*)S.as<float>(S_TRY, c->trace_cap), nr);
-            CURC_HIP(hipStreamSynchronize(st));
+            CURC_HIP(stream_wait(st));
         }
     }
     if (stop <= CURC_STAGE_MAP) return;
@@ -2150,7 +2168,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             } catch (...) {
                 nlq_err = std::current_exception();
                 // no copy into the block's arrays may outlive an error path
-                hipStreamSynchronize(hx->st);
+                stream_wait(hx->st);
             }
         });
     }
@@ -2186,13 +2204,13 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint64_t* lpg = build_pg(c, lqp, W, a, S_PG_LQ);
         records_to_host(c, a, um, 0, B->lq_rec);
         d2h(c, B->lq_pg2, lpg, (a.len + 31) / 32 + 1);
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
     }
     if (nlq_side) {
         nlq_thread.t.join();
         if (nlq_err) std::rethrow_exception(nlq_err);
         // the helper's stage times go to this context
-        CURC_HIP(hipStreamSynchronize(hx->st));
+        CURC_HIP(stream_wait(hx->st));
         hx->harvest();
         for (const StageStat& sst : hx->stats) {
             const int id = c->stat_id(sst.name.c_str());
@@ -2258,7 +2276,7 @@ int block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L, int 
         run_block(c, ascii, n_reads, L, stop_after <= 0 ? CURC_STAGE_ALL : stop_after, flags, B, fq);
         done_recorded = true;
         CURC_HIP(hipGetLastError());
-        CURC_HIP(hipStreamSynchronize(c->st));
+        CURC_HIP(stream_wait(c->st));
         join_side(c);
         c->harvest();
         if (flags & CURC_RUN_KEEP_STAGES) {
@@ -2267,14 +2285,14 @@ int block_run(curc_ctx* c, const char* ascii, uint64_t n_reads, uint32_t L, int 
             d2h(c, B->packed, c->S.as<uint64_t>(S_PACKED, N * P.W + 1), N * P.W);
             std::vector<uint8_t> na;
             d2h(c, na, c->S.as<uint8_t>(S_NASCII, (uint64_t)B->Nids.size() * L + 16), (uint64_t)B->Nids.size() * L);
-            CURC_HIP(hipStreamSynchronize(c->st));
+            CURC_HIP(stream_wait(c->st));
             B->Nascii.assign(na.begin(), na.end());
         }
     } catch (const std::exception& e) {
         // copies into B's arrays may be in flight: join them before freeing
         finish_input();
-        hipStreamSynchronize(c->st);
-        if (c->cst) hipStreamSynchronize(c->cst);
+        stream_wait(c->st);
+        if (c->cst) stream_wait(c->cst);
         delete B;
         c->pending.clear();
         std::string m = e.what();
@@ -2312,7 +2330,7 @@ int run_on_workers(curc_ctx* c, uint32_t n_jobs, uint32_t workers, const std::ve
     // per context); the caller's copy stream is released for the batch
     hipSetDevice(c->device);
     if (c->cst) {
-        hipStreamSynchronize(c->cst);
+        stream_wait(c->cst);
         hipStreamDestroy(c->cst);
         c->cst = nullptr;
     }
@@ -2422,7 +2440,7 @@ int fq_block_run(curc_ctx* c, const curc_fastq_job& j, int stop_after, int slot,
             in = fq_upload(j, c->S.as<uint8_t>(slot, fq_dev_bytes(j)), c->st);
         }
     } catch (const std::exception& e) {
-        hipStreamSynchronize(c->st);
+        stream_wait(c->st);
         return fail(c, CURC_EHIP, e.what());
     }
     return block_run(c, nullptr, 0, j.L, stop_after, fq_flags(j), &in, out);
@@ -2532,7 +2550,7 @@ void decode_run(curc_ctx* c, const curc_decode_job& j, char* out1, char* out2) {
             if (n) {
                 CURC_HIP(hipMemcpyAsync(&last[0], hx + n - 1, 8, hipMemcpyDeviceToHost, st));
                 CURC_HIP(hipMemcpyAsync(&last[1], hv + n - 1, 8, hipMemcpyDeviceToHost, st));
-                CURC_HIP(hipStreamSynchronize(st));
+                CURC_HIP(stream_wait(st));
             }
             hq_reads = last[0] + last[1];
             if (hq_reads != j.n_rec) throw InputError("reads placed in the HQ text do not match the records");
@@ -2574,7 +2592,7 @@ void decode_run(curc_ctx* c, const curc_decode_job& j, char* out1, char* out2) {
     }
     uint32_t e = 0;
     CURC_HIP(hipMemcpyAsync(&e, a.err, 4, hipMemcpyDeviceToHost, st));
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
     if (e) throw InputError("corrupt streams (offsets or positions out of range)");
     {
         Region r(c, "decode_d2h");
@@ -2582,7 +2600,7 @@ void decode_run(curc_ctx* c, const curc_decode_job& j, char* out1, char* out2) {
         if (lines1) CURC_HIP(hipMemcpyAsync(out1, o1, lines1 * LL, hipMemcpyDeviceToHost, st));
         if (paired && lines1) CURC_HIP(hipMemcpyAsync(out2, o2, lines1 * LL, hipMemcpyDeviceToHost, st));
     }
-    CURC_HIP(hipStreamSynchronize(st));
+    CURC_HIP(stream_wait(st));
 }
 
 }  // namespace
@@ -2653,8 +2671,8 @@ void curc_ctx_destroy(curc_ctx* c) {
     if (c->helper) curc_ctx_destroy(c->helper);
     c->helper = nullptr;
     hipSetDevice(c->device);
-    hipStreamSynchronize(c->st);
-    if (c->cst) hipStreamSynchronize(c->cst);
+    stream_wait(c->st);
+    if (c->cst) stream_wait(c->cst);
     c->graphs.clear();
     c->S.release();
     for (auto& p : c->pending) {
@@ -2667,7 +2685,7 @@ void curc_ctx_destroy(curc_ctx* c) {
     hipEventDestroy(c->nev);
     if (c->cst) hipStreamDestroy(c->cst);
     if (c->ust) {
-        hipStreamSynchronize(c->ust);
+        stream_wait(c->ust);
         hipStreamDestroy(c->ust);
     }
     hipStreamDestroy(c->st);
@@ -2679,8 +2697,8 @@ const char* curc_ctx_last_error(curc_ctx* c) { return c ? c->err.c_str() : "null
 int curc_ctx_trim(curc_ctx* c) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);
-    hipStreamSynchronize(c->st);
-    if (c->cst) hipStreamSynchronize(c->cst);
+    stream_wait(c->st);
+    if (c->cst) stream_wait(c->cst);
     c->graphs.clear();
     c->S.release();
     for (curc_ctx* w : c->workers) curc_ctx_trim(w);
@@ -2714,7 +2732,7 @@ int curc_memcpy_h2d(curc_ctx* c, void* d, const void* s, uint64_t bytes) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);
     if (hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, c->st) != hipSuccess ||
-        hipStreamSynchronize(c->st) != hipSuccess)
+        stream_wait(c->st) != hipSuccess)
         return fail(c, CURC_EHIP, "host to device copy failed");
     return CURC_OK;
 }
@@ -2722,7 +2740,7 @@ int curc_memcpy_d2h(curc_ctx* c, void* d, const void* s, uint64_t bytes) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);
     if (hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
-        hipStreamSynchronize(c->st) != hipSuccess)
+        stream_wait(c->st) != hipSuccess)
         return fail(c, CURC_EHIP, "device to host copy failed");
     return CURC_OK;
 }
@@ -2754,7 +2772,7 @@ int curc_stream_copy(curc_ctx* c, void* dst, const void* src, uint64_t bytes, in
 int curc_sync(curc_ctx* c) {
     if (!c) return CURC_EINVAL;
     hipSetDevice(c->device);
-    return hipStreamSynchronize(c->st) == hipSuccess ? CURC_OK : fail(c, CURC_EHIP, "stream sync failed");
+    return stream_wait(c->st) == hipSuccess ? CURC_OK : fail(c, CURC_EHIP, "stream sync failed");
 }
 
 int curc_device_sync(curc_ctx* c) {
@@ -2774,7 +2792,7 @@ int curc_pack(curc_ctx* c, const char* d_ascii, uint64_t n, uint32_t L, uint64_t
                                (const uint8_t*)d_ascii, n, L, W, d_packed, d_isN);
             CURC_HIP(hipGetLastError());
         }
-        CURC_HIP(hipStreamSynchronize(c->st));
+        CURC_HIP(stream_wait(c->st));
     } catch (const std::exception& e) {
         return fail(c, CURC_EHIP, e.what());
     }
@@ -3052,7 +3070,7 @@ int curc_fastq_stream_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jo
     }
     cv.notify_all();
     uploader.join();
-    hipStreamSynchronize(c->ust);
+    stream_wait(c->ust);
     release();
     return first;
 }
@@ -3073,7 +3091,7 @@ int curc_fastq_pack(curc_ctx* c, const char* d_fq, uint64_t bytes, uint32_t L, u
         uint32_t* cnt = c->S.as<uint32_t>(S_CNT, C_COUNT);
         *n_reads = fq_pack(c, in, L, P.W, max_reads, d_packed, d_isN, d_seq_off, cnt, nullptr, 0);
     } catch (const std::exception& e) {
-        hipStreamSynchronize(c->st);
+        stream_wait(c->st);
         int code = CURC_EHIP;
         if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
         else if (dynamic_cast<const std::length_error*>(&e)) code = CURC_ELIMIT;
@@ -3105,7 +3123,7 @@ int curc_fastq_pack_static(curc_ctx* c, const char* d_fq, uint64_t bytes, uint32
         *n_reads = fq_pack(c, in, L, P.W, max_reads, d_packed, d_isN, d_seq_off, cnt, base + STATIC_CNT_WORDS,
                            STATIC_SCRATCH_WORDS - STATIC_CNT_WORDS);
     } catch (const std::exception& e) {
-        hipStreamSynchronize(c->st);
+        stream_wait(c->st);
         int code = CURC_EHIP;
         if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
         else if (dynamic_cast<const std::length_error*>(&e)) code = CURC_ELIMIT;
@@ -3126,7 +3144,7 @@ int curc_decode_block(curc_ctx* c, const curc_decode_job* j, char* out1, char* o
         decode_run(c, *j, out1, out2);
         c->harvest();
     } catch (const std::exception& e) {
-        hipStreamSynchronize(c->st);
+        stream_wait(c->st);
         c->pending.clear();
         int code = CURC_EHIP;
         if (dynamic_cast<const InputError*>(&e)) code = CURC_EINVAL;
@@ -3204,7 +3222,7 @@ curc_mlp* curc_mlp_create(curc_ctx* c) {
         for (float* q : {m->d.p, m->d.m, m->d.v, m->d.g})
             hipLaunchKernelGGL(k_fill<float>, dim3(32), dim3(256), 0, c->st, q, (uint64_t)MLP_NPARAMS, 0.f);
         hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, c->st, m->d);
-        CURC_HIP(hipStreamSynchronize(c->st));
+        CURC_HIP(stream_wait(c->st));
     } catch (const std::exception& e) {
         c->err = e.what();
         curc_mlp_destroy(m);
@@ -3231,7 +3249,7 @@ int curc_mlp_set_params(curc_mlp* m, const float* params, uint32_t n) {
         for (float* q : {m->d.m, m->d.v, m->d.g})
             hipLaunchKernelGGL(k_fill<float>, dim3(32), dim3(256), 0, st, q, (uint64_t)MLP_NPARAMS, 0.f);
         hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, st, m->d);
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
         m->t = 0;
     } catch (const std::exception& e) {
         return fail(m->c, CURC_EHIP, e.what());
@@ -3243,7 +3261,7 @@ int curc_mlp_get_params(curc_mlp* m, float* params, uint32_t n) {
     if (!m || !params || n != MLP_NPARAMS) return CURC_EINVAL;
     try {
         CURC_HIP(hipMemcpyAsync(params, m->d.p, MLP_NPARAMS * 4, hipMemcpyDeviceToHost, m->c->st));
-        CURC_HIP(hipStreamSynchronize(m->c->st));
+        CURC_HIP(stream_wait(m->c->st));
     } catch (const std::exception& e) {
         return fail(m->c, CURC_EHIP, e.what());
     }
@@ -3257,7 +3275,7 @@ int curc_mlp_forward(curc_mlp* m, const float* d_x, uint64_t n, float* d_logit) 
         hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_infer_grid(n)), dim3(64 * MLP_INF_WAVES), 0, m->c->st, m->d, d_x, n,
                            (const uint32_t*)nullptr, d_logit);
         CURC_HIP(hipGetLastError());
-        CURC_HIP(hipStreamSynchronize(m->c->st));
+        CURC_HIP(stream_wait(m->c->st));
     } catch (const std::exception& e) {
         return fail(m->c, CURC_EHIP, e.what());
     }
@@ -3273,7 +3291,7 @@ int curc_mlp_gradients(curc_mlp* m, const float* d_x, const float* d_y, uint64_t
         CURC_HIP(hipMemcpyAsync(grads, m->d.g, MLP_NPARAMS * 4, hipMemcpyDeviceToHost, st));
         hipLaunchKernelGGL(k_fill<float>, dim3(32), dim3(256), 0, st, m->d.g, (uint64_t)MLP_NPARAMS, 0.f);
         CURC_HIP(hipGetLastError());
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
     } catch (const std::exception& e) {
         return fail(m->c, CURC_EHIP, e.what());
     }
@@ -3300,7 +3318,7 @@ int curc_mlp_train_step(curc_mlp* m, const float* d_x, const float* d_y, uint64_
         if (loss) {
             float sum = 0;
             CURC_HIP(hipMemcpyAsync(&sum, m->lsum, 4, hipMemcpyDeviceToHost, st));
-            CURC_HIP(hipStreamSynchronize(st));
+            CURC_HIP(stream_wait(st));
             *loss = sum / (float)n;
         }
     } catch (const std::exception& e) {
@@ -3404,18 +3422,18 @@ int curc_ref_match(curc_ctx* c, const char* src, uint64_t src_len, const char* d
         rm_match(c, ix, src2, src_len, dest2, dest_contain_n ? ddest : nullptr, dest_len, dest_is_ref != 0,
                  (flags & CURC_RUN_KEEP_STAGES) != 0, cnt, B->rm[0]);
         CURC_HIP(hipGetLastError());
-        CURC_HIP(hipStreamSynchronize(st));
+        CURC_HIP(stream_wait(st));
         c->harvest();
     } catch (const std::length_error& e) {
         // copies into B's arrays may be in flight: join them before freeing
-        hipStreamSynchronize(c->st);
-        if (c->cst) hipStreamSynchronize(c->cst);
+        stream_wait(c->st);
+        if (c->cst) stream_wait(c->cst);
         delete B;
         c->pending.clear();
         return fail(c, CURC_ELIMIT, e.what());
     } catch (const std::exception& e) {
-        hipStreamSynchronize(c->st);
-        if (c->cst) hipStreamSynchronize(c->cst);
+        stream_wait(c->st);
+        if (c->cst) stream_wait(c->cst);
         delete B;
         c->pending.clear();
         return fail(c, CURC_EHIP, e.what());
