@@ -81,8 +81,9 @@ def test_mapping_both_strands_one_launch(case):
     """Tuning map_both=1: both strand passes in one launch on the two-strand index of
     128-byte lines (taken by blocks whose pseudogenome fills bucket-inline slots,
     so 6 and 9 Mbp genomes at 30x here); map_both=2 (the default): the same on the
-    compact index of 32-byte slots, whose larger buckets (repeats) keep their
-    positions in the overflow array.  Same results as the oracle (OpenMP build),
+    compact index of 32-byte slots, whose larger buckets keep their positions
+    in the overflow array (the repeat case's index has about 37,500 buckets of
+    more than 7 positions, up to 24).  Same results as the oracle (OpenMP build),
     and the same work counters as the forward pass followed by the
     reverse-complement pass."""
     if case == "plain":
