@@ -2186,7 +2186,17 @@ struct MapArgs {
     // per bucket, ovf the positions of buckets that do not fit their slot
     uint32_t cs = 0;
     const uint32_t* ovf = nullptr;
+    // global ids of the N-reads (their packed rows in R), for k_map_both's
+    // N-read instance
+    const uint32_t* nids = nullptr;
 };
+
+// Mismatches of two 2-bit words, with the bases marked in m (01 in the
+// base's field) counted whatever their codes.
+__device__ __forceinline__ uint32_t mm_words_m(uint64_t a, uint64_t b, uint64_t m) {
+    const uint64_t x = a ^ b;
+    return (uint32_t)__popcll(((x & 0xAAAAAAAAAAAAAAAAULL) >> 1) | (x & 0x5555555555555555ULL) | m);
+}
 
 // Both strands' slots of a bucket of a two-strand index: the 128-byte line
 // form (MAP_SLOT2: f and r are the slots' first 16 bytes, sl the line) or
@@ -2605,43 +2615,36 @@ struct SeenCache32 {
 //    exact and it stops at the same candidate as the sequential pass;
 //  - each pass keeps its own SeenCache, and the reverse work is added to
 //    the counters only when the forward pass ends above MAP_TARGET.
-template <uint32_t WM, bool CS>
+//
+// NR: the instance for the N-reads (indexes [n_un, n_un + n_N)), launched
+// after the one for the other reads.  The reference compares an N-read
+// character by character (compress.cu:411-521 with util.cuh:271-281): a
+// read character that is not one of A, C, G, T (upper case) mismatches
+// every pseudogenome base, and k-mers hash the read packed with such
+// characters as code enc_base & 3 (N as A).  Here the read is that packed
+// row (k_pack_w packs it so), compared word by word like the other reads,
+// and a mask marks its bases outside A, C, G, T, which count as mismatches
+// whatever their codes (in the reverse-complement pass the complement of
+// such a character is again outside A, C, G, T, and the mask is reversed
+// with the read).  So an N-read's counts, results and work counters are
+// those of the character loop, at the cost of the other reads.
+template <uint32_t WM, bool CS, bool NR>
 __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
     constexpr uint32_t G = 1;  // probes whose slot heads are requested together
     uint32_t n_act = 0, n_probe = 0, n_cand = 0, n_first = 0, n_open_probe = 0, n_open_cand = 0;
     const uint32_t L = a.L, W = a.W;
     const uint32_t np = (L - MAP_K) / MAP_READ_STEP + 1;
-    GRID_STRIDE(i, a.n_un + a.n_N) {
+    const uint64_t i1 = NR ? a.n_un + a.n_N : a.n_un;
+    const uint64_t flagN = NR ? 1ull : 0ull;  // is_contain_N bit of the record word
+    for (uint64_t i = (NR ? a.n_un : 0) + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < i1;
+         i += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t last = a.last[i];
         if (last <= MAP_TARGET) continue;
         uint64_t w = a.word[i];
-        if (i >= a.n_un) {
-            // N-reads: the two passes one after the other (as the two launches)
-            MapArgs ar = a;
-            ar.bk_off = MAP_SLOT;
-            for (int rc = 0; rc < 2; ++rc) {
-                if (last <= MAP_TARGET) break;
-                ++n_act;
-                const uint32_t probe0 = n_probe, cand0 = n_cand;
-                SeenCache seen;
-                if (rc)
-                    map_nread<true>(ar, i, last, w, seen, n_probe, n_cand);
-                else
-                    map_nread<false>(a, i, last, w, seen, n_probe, n_cand);
-                if (last <= MAP_TARGET) {
-                    n_first += (n_probe - probe0 == 1) ? 1u : 0u;
-                } else {
-                    n_open_probe += n_probe - probe0;
-                    n_open_cand += n_cand - cand0;
-                }
-            }
-            a.last[i] = (uint8_t)last;
-            a.word[i] = w;
-            continue;
-        }
         uint64_t qf[WM], qr[WM];
+        uint64_t mf[NR ? WM : 1], mr[NR ? WM : 1];  // N-reads: bases outside A, C, G, T
         {
-            const uint64_t* rd = a.R + (uint64_t)a.un_ids[i] * W;
+            const uint64_t* rd = a.R + (uint64_t)(NR ? a.nids[i - a.n_un] : a.un_ids[i]) * W;
             uint64_t r[WM];
 #pragma unroll
             for (uint32_t k = 0; k < WM; ++k) r[k] = k < W ? ld64(rd + k) : 0ull;
@@ -2652,6 +2655,45 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                     const uint32_t n = min(32u, L - 32 * k);
                     qf[k] = xw_pick(r, 32 * k, n);
                     qr[k] = revcomp(xw_pick(r, L - 32 * k - n, n), n);
+                }
+            }
+            mf[0] = mr[0] = 0;
+            if constexpr (NR) {
+                // the mask in the packed layout (01 in the field of a base
+                // outside A, C, G, T), from the read's characters, read as
+                // aligned 4-byte words (the character array is padded by 16
+                // bytes past its end)
+                uint64_t mk[WM];
+#pragma unroll
+                for (uint32_t k = 0; k < WM; ++k) mk[k] = 0;
+                const uint8_t* ra = a.nascii + (i - a.n_un) * L;
+                const uintptr_t a0 = (uintptr_t)ra & ~(uintptr_t)3;
+                const uint32_t lead = (uint32_t)((uintptr_t)ra - a0);
+                for (uint32_t o = 0; o < lead + L; o += 4) {
+                    const uint32_t v = ld32(reinterpret_cast<const uint32_t*>(a0 + o));
+#pragma unroll
+                    for (uint32_t t = 0; t < 4; ++t) {
+                        const int32_t b = (int32_t)(o + t) - (int32_t)lead;  // base index
+                        const uint32_t ch = (v >> (8 * t)) & 0xFFu;
+                        const bool out = ch != 'A' && ch != 'C' && ch != 'G' && ch != 'T';
+                        if (b >= 0 && (uint32_t)b < L && out) {
+                            const uint32_t wb = (uint32_t)b >> 5;
+                            const uint64_t bit = 1ull << (62u - 2u * ((uint32_t)b & 31u));
+#pragma unroll
+                            for (uint32_t k = 0; k < WM; ++k)
+                                if (k == wb) mk[k] |= bit;
+                        }
+                    }
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < WM; ++k) {
+                    mf[k] = mr[k] = 0;
+                    if (32 * k < L) {
+                        const uint32_t n = min(32u, L - 32 * k);
+                        mf[k] = xw_pick(mk, 32 * k, n);
+                        // reversed, not complemented
+                        mr[k] = Enc2::reversed(xw_pick(mk, L - 32 * k - n, n), n);
+                    }
                 }
             }
         }
@@ -2674,7 +2716,8 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
         // bound.  The first 32 bases come from two words (one 128-byte line
         // in 15 of 16 cases); the rest of the window is fetched only when
         // they stay within the bound, which hash-bucket neighbours rarely do.
-        auto count = [&](const uint64_t (&q)[WM], uint64_t mp, uint32_t bound) -> uint32_t {
+        auto count = [&](const uint64_t (&q)[WM], const uint64_t (&m)[NR ? WM : 1], uint64_t mp,
+                         uint32_t bound) -> uint32_t {
             const uint64_t* g0 = a.pg + (mp >> 5);
             uint64_t g[WM + 1];
             g[0] = ld64(g0);
@@ -2684,7 +2727,10 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
             {
                 const uint32_t n = min(32u, L);
                 const uint64_t win = sh ? (g[0] << sh) | (g[1] >> (64u - sh)) : g[0];
-                c = mm_words(win >> ((32u - n) << 1), q[0]);
+                if constexpr (NR)
+                    c = mm_words_m(win >> ((32u - n) << 1), q[0], m[0]);
+                else
+                    c = mm_words(win >> ((32u - n) << 1), q[0]);
             }
             if (c > bound || L <= 32) return c;
 #pragma unroll
@@ -2694,7 +2740,10 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                 if (32 * k < L && c <= bound) {
                     const uint32_t n = min(32u, L - 32 * k);
                     const uint64_t win = sh ? (g[k] << sh) | (g[k + 1] >> (64u - sh)) : g[k];
-                    c += mm_words(win >> ((32u - n) << 1), q[k]);
+                    if constexpr (NR)
+                        c += mm_words_m(win >> ((32u - n) << 1), q[k], m[NR ? k : 0]);
+                    else
+                        c += mm_words(win >> ((32u - n) << 1), q[k]);
                 }
             }
             return c;
@@ -2750,9 +2799,9 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                         if (mp + L > a.pg_len) continue;
                         if (seenF.check_insert(mp)) continue;
                         ++cF;
-                        const uint32_t c = count(qf, mp, F - 1);
+                        const uint32_t c = count(qf, mf, mp, F - 1);
                         if (c < F) {
-                            wF = mp << 2;
+                            wF = flagN | (mp << 2);
                             F = c;
                             if (F <= MAP_TARGET) goto fwd_done;
                         }
@@ -2783,9 +2832,9 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                         if (seenR.check_insert(mp)) continue;
                         ++cR;
                         const uint32_t lim = min(Rb, F);
-                        const uint32_t c = count(qr, mp, lim - 1);
+                        const uint32_t c = count(qr, mr, mp, lim - 1);
                         if (c < lim) {
-                            wR = 2ull | (mp << 2);
+                            wR = flagN | 2ull | (mp << 2);
                             Rb = c;
                             if (Rb <= MAP_TARGET) {
                                 doneR = true;

@@ -800,22 +800,28 @@ RmIndex rm_index(curc_ctx* c, const uint64_t* src, uint64_t src_len) {
 
 // k_map_both for the read's word count W <= 8 (register capacity 3, 4, 5 or
 // 8) and the index form (a.cs: compact slots).
-template <bool CS>
-void launch_map_both_t(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
+template <bool CS, bool NR>
+void launch_map_both_w(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
     if (W <= 3)
-        hipLaunchKernelGGL((k_map_both<3, CS>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<3, CS, NR>), grid, dim3(256), 0, st, a);
     else if (W <= 4)
-        hipLaunchKernelGGL((k_map_both<4, CS>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<4, CS, NR>), grid, dim3(256), 0, st, a);
     else if (W <= 5)
-        hipLaunchKernelGGL((k_map_both<5, CS>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<5, CS, NR>), grid, dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL((k_map_both<8, CS>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_map_both<8, CS, NR>), grid, dim3(256), 0, st, a);
 }
-void launch_map_both(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) {
+// the other reads, then the N-reads (grid capped by cap)
+template <bool CS>
+void launch_map_both_t(uint32_t W, uint32_t cap, hipStream_t st, const MapArgs& a) {
+    if (a.n_un) launch_map_both_w<CS, false>(W, dim3(grid_for(a.n_un, cap)), st, a);
+    if (a.n_N) launch_map_both_w<CS, true>(W, dim3(grid_for(a.n_N, cap)), st, a);
+}
+void launch_map_both(uint32_t W, uint32_t cap, hipStream_t st, const MapArgs& a) {
     if (a.cs)
-        launch_map_both_t<true>(W, grid, st, a);
+        launch_map_both_t<true>(W, cap, st, a);
     else
-        launch_map_both_t<false>(W, grid, st, a);
+        launch_map_both_t<false>(W, cap, st, a);
 }
 
 // k_map_fast for the read's word count W <= 8 (register capacity 3, 4, 5 or 8).
@@ -1995,7 +2001,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 a.bk_off = 0;
                 a.cs = cs ? 1u : 0u;
                 a.ovf = ovf;
-                launch_map_both(W, dim3(grid_for(nm, c->tune.map_grid)), st, a);
+                a.nids = Nids;
+                launch_map_both(W, c->tune.map_grid, st, a);
             }
             if (c->timing) {
                 // as for the separate passes, over both strands
