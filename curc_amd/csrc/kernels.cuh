@@ -2186,10 +2186,32 @@ struct MapArgs {
     // per bucket, ovf the positions of buckets that do not fit their slot
     uint32_t cs = 0;
     const uint32_t* ovf = nullptr;
-    // global ids of the N-reads (their packed rows in R), for k_map_both's
-    // N-read instance
+    // global ids of the N-reads (their packed rows in R) and the masks of
+    // their bases outside A, C, G, T (k_nmask): with them, N-reads are
+    // compared word by word (nread_count, k_map_both's N-read instance)
     const uint32_t* nids = nullptr;
+    const uint64_t* nmask = nullptr;
 };
+
+// Masks of the N-reads' bases outside A, C, G, T (upper case: the
+// characters that mismatch every pseudogenome base in the reference's
+// character compare, compress.cu:411-521), in the packed layout (01 in the
+// base's field): W words per N-read, in N-read order.
+__global__ void k_nmask(const uint8_t* __restrict__ nascii, uint64_t nN, uint32_t L, uint32_t W,
+                        uint64_t* __restrict__ out) {
+    GRID_STRIDE(t, nN * W) {
+        const uint64_t r = t / W;
+        const uint32_t k = (uint32_t)(t - r * W);
+        const uint8_t* ra = nascii + r * L + 32 * k;
+        const uint32_t n = min(32u, L - 32 * k);
+        uint64_t m = 0;
+        for (uint32_t b = 0; b < n; ++b) {
+            const uint8_t ch = ra[b];
+            if (ch != 'A' && ch != 'C' && ch != 'G' && ch != 'T') m |= 1ull << (62u - 2u * b);
+        }
+        out[t] = m;
+    }
+}
 
 // Mismatches of two 2-bit words, with the bases marked in m (01 in the
 // base's field) counted whatever their codes.
@@ -2290,14 +2312,44 @@ struct SeenCache {
     }
 };
 
-// One N-read (ASCII, compared base by base) through the k_map loop.
+// Mismatches of an N-read (packed row rd, mask row mk) against the window at
+// mp, 32 bases at a time, the masked bases counted whatever their codes:
+// the count of the reference's character loop (compress.cu:437-450 with
+// util.cuh:271-281), which stops at `last`, so counts of last or more come
+// back as last.
+template <bool RC>
+__device__ __forceinline__ uint32_t nread_count(const MapArgs& a, const uint64_t* __restrict__ rd,
+                                                const uint64_t* __restrict__ mk, uint64_t mp, uint32_t last) {
+    const uint32_t L = a.L;
+    uint32_t c = 0;
+    for (uint32_t o = 0; o < L && c < last; o += 32) {
+        const uint32_t n = min(32u, L - o);
+        const uint64_t win = xwu(a.pg, mp + o, n);
+        uint64_t q, m;
+        if (RC) {
+            q = revcomp(xw(rd, L - o - n, n), n);
+            m = Enc2::reversed(xw(mk, L - o - n, n), n);
+        } else {
+            q = xw(rd, o, n);
+            m = xw(mk, o, n);
+        }
+        c += mm_words_m(win, q, m);
+    }
+    return min(c, last);
+}
+
+// One N-read through the k_map loop: with the masks (a.nmask) word by word
+// (nread_count), else base by base from its characters.
 template <bool RC>
 __device__ __forceinline__ void map_nread(const MapArgs& a, uint64_t i, uint32_t& last, uint64_t& w,
                                           SeenCache& seen, uint32_t& n_probe, uint32_t& n_cand) {
     const uint8_t* ra = a.nascii + (i - a.n_un) * a.L;
+    const uint64_t* rd = a.nmask ? a.R + (uint64_t)a.nids[i - a.n_un] * a.W : nullptr;
+    const uint64_t* mk = a.nmask ? a.nmask + (i - a.n_un) * a.W : nullptr;
     for (uint32_t j = 0; j + MAP_K <= a.L; j += MAP_READ_STEP) {
         ++n_probe;
-        uint32_t h = (uint32_t)(murmur64(kmer_ascii(ra + j, MAP_K)) & a.mask);
+        // the packed row holds the characters as enc_base & 3, as kmer_ascii
+        uint32_t h = (uint32_t)(murmur64(rd ? xw(rd, j, MAP_K) : kmer_ascii(ra + j, MAP_K)) & a.mask);
         MapBucket B{0u, nullptr};
         Bk2 cb{};
         if (a.cs) {
@@ -2317,12 +2369,16 @@ __device__ __forceinline__ void map_nread(const MapArgs& a, uint64_t i, uint32_t
             if (seen.check_insert(mp)) continue;
             ++n_cand;
             uint32_t c = 0;
-            for (uint32_t o = 0; o < a.L; ++o) {
-                uint8_t rb = base_char(base2(a.pg, mp + o));
-                uint8_t qb = RC ? rev_base_ascii(ra[a.L - 1 - o]) : ra[o];
-                if (rb != qb) {
-                    c++;
-                    if (c >= last) break;
+            if (rd) {
+                c = nread_count<RC>(a, rd, mk, mp, last);
+            } else {
+                for (uint32_t o = 0; o < a.L; ++o) {
+                    uint8_t rb = base_char(base2(a.pg, mp + o));
+                    uint8_t qb = RC ? rev_base_ascii(ra[a.L - 1 - o]) : ra[o];
+                    if (rb != qb) {
+                        c++;
+                        if (c >= last) break;
+                    }
                 }
             }
             if (c < last) {
@@ -2623,10 +2679,10 @@ struct SeenCache32 {
 // every pseudogenome base, and k-mers hash the read packed with such
 // characters as code enc_base & 3 (N as A).  Here the read is that packed
 // row (k_pack_w packs it so), compared word by word like the other reads,
-// and a mask marks its bases outside A, C, G, T, which count as mismatches
-// whatever their codes (in the reverse-complement pass the complement of
-// such a character is again outside A, C, G, T, and the mask is reversed
-// with the read).  So an N-read's counts, results and work counters are
+// and a mask (k_nmask) marks its bases outside A, C, G, T, which count as
+// mismatches whatever their codes (in the reverse-complement pass the
+// complement of such a character is again outside A, C, G, T, and the mask
+// is reversed with the read).  So an N-read's counts, results and work counters are
 // those of the character loop, at the cost of the other reads.
 template <uint32_t WM, bool CS, bool NR>
 __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
@@ -2659,32 +2715,11 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
             }
             mf[0] = mr[0] = 0;
             if constexpr (NR) {
-                // the mask in the packed layout (01 in the field of a base
-                // outside A, C, G, T), from the read's characters, read as
-                // aligned 4-byte words (the character array is padded by 16
-                // bytes past its end)
+                // the read's mask (k_nmask)
                 uint64_t mk[WM];
+                const uint64_t* mrow = a.nmask + (i - a.n_un) * W;
 #pragma unroll
-                for (uint32_t k = 0; k < WM; ++k) mk[k] = 0;
-                const uint8_t* ra = a.nascii + (i - a.n_un) * L;
-                const uintptr_t a0 = (uintptr_t)ra & ~(uintptr_t)3;
-                const uint32_t lead = (uint32_t)((uintptr_t)ra - a0);
-                for (uint32_t o = 0; o < lead + L; o += 4) {
-                    const uint32_t v = ld32(reinterpret_cast<const uint32_t*>(a0 + o));
-#pragma unroll
-                    for (uint32_t t = 0; t < 4; ++t) {
-                        const int32_t b = (int32_t)(o + t) - (int32_t)lead;  // base index
-                        const uint32_t ch = (v >> (8 * t)) & 0xFFu;
-                        const bool out = ch != 'A' && ch != 'C' && ch != 'G' && ch != 'T';
-                        if (b >= 0 && (uint32_t)b < L && out) {
-                            const uint32_t wb = (uint32_t)b >> 5;
-                            const uint64_t bit = 1ull << (62u - 2u * ((uint32_t)b & 31u));
-#pragma unroll
-                            for (uint32_t k = 0; k < WM; ++k)
-                                if (k == wb) mk[k] |= bit;
-                        }
-                    }
-                }
+                for (uint32_t k = 0; k < WM; ++k) mk[k] = k < W ? ld64(mrow + k) : 0ull;
 #pragma unroll
                 for (uint32_t k = 0; k < WM; ++k) {
                     mf[k] = mr[k] = 0;

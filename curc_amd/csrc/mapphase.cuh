@@ -186,13 +186,18 @@ __device__ __forceinline__ void range_read_generic(const MapArgs& a, uint64_t i,
                                                    uint32_t& n_probe, uint32_t& n_cand) {
     const bool isN = i >= a.n_un;
     MapCounters& c = st.c;
-    const uint64_t* rd = isN ? nullptr : a.R + (uint64_t)a.un_ids[i] * a.W;
+    // N-reads: their packed rows and masks when given (word by word), else
+    // their characters
+    const bool nw = isN && a.nmask;
+    const uint64_t* rd = !isN ? a.R + (uint64_t)a.un_ids[i] * a.W
+                              : (nw ? a.R + (uint64_t)a.nids[i - a.n_un] * a.W : nullptr);
+    const uint64_t* mk = nw ? a.nmask + (i - a.n_un) * a.W : nullptr;
     const uint8_t* ra = isN ? a.nascii + (i - a.n_un) * a.L : nullptr;
     for (uint32_t t = t0; t < t1; ++t) {
         const uint32_t j = MAP_READ_STEP * t;
         if (j + MAP_K > a.L) break;
         ++n_probe;
-        const uint64_t km = isN ? kmer_ascii(ra + j, MAP_K) : xw(rd, j, MAP_K);
+        const uint64_t km = (isN && !nw) ? kmer_ascii(ra + j, MAP_K) : xw(rd, j, MAP_K);
         const uint32_t h = (uint32_t)(murmur64(km) & a.mask);
         const MapBucket B = map_bucket(a, h);
         range_bucket(c, B.n);
@@ -209,6 +214,8 @@ __device__ __forceinline__ void range_read_generic(const MapArgs& a, uint64_t i,
             uint32_t cm;
             if (!isN) {
                 cm = RC ? rc_mm(a.pg, mp, rd, a.L, last - 1) : fwd_mm(a.pg, mp, rd, a.L, last - 1);
+            } else if (nw) {
+                cm = nread_count<RC>(a, rd, mk, mp, last);
             } else {
                 cm = 0;
                 for (uint32_t o = 0; o < a.L; ++o) {

@@ -104,6 +104,7 @@ enum Slot {
     S_D_TEXT, S_D_STRAND, S_D_CNT, S_D_BASE, S_D_STREAMS, S_D_SSTART, S_D_TAB, S_D_ROFF, S_D_LQ, S_D_NLQ,
     S_D_PE, S_D_IDPOS, S_D_PV, S_D_PX, S_D_HQV, S_D_HQX, S_D_BV, S_D_BX, S_D_K0, S_D_K1, S_D_V0, S_D_V1,
     S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2, S_OVPASS, S_WORK, S_FMNF, S_FMVB, S_FMFILT, S_FMLEAD, S_FMGC, S_FMGM,
+    S_NMASK,
     S_LAST_NAMED,
     S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
 };
@@ -1937,6 +1938,12 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         if (nm) hipLaunchKernelGGL(k_map_init, dim3(grid_for(nm)), dim3(256), 0, st, last, word, (uint64_t)n_unm, nm, P.max_mm);
         uint64_t index_count = 0;
         if (ref_len + 1 >= MAP_K) index_count = (ref_len - MAP_K + 1 + MAP_REF_STEP - 1) / MAP_REF_STEP;
+        // the N-reads' masks of bases outside A, C, G, T: every mapping kernel
+        // then compares N-reads word by word, like the other reads
+        uint64_t* nmask = S.as<uint64_t>(S_NMASK, (uint64_t)nN * W + 1);
+        if (nN)
+            hipLaunchKernelGGL(k_nmask, dim3(grid_for((uint64_t)nN * W)), dim3(256), 0, st, (const uint8_t*)Nascii,
+                               (uint64_t)nN, L, W, nmask);
         // both strand passes in one launch (k_map_both) on an index holding
         // both strands' slots of a bucket in one 128-byte line: the plain
         // mode, one index part, bucket-inline slots, reads of <= 256 bases
@@ -2002,6 +2009,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 a.cs = cs ? 1u : 0u;
                 a.ovf = ovf;
                 a.nids = Nids;
+                a.nmask = nmask;
                 launch_map_both(W, c->tune.map_grid, st, a);
             }
             if (c->timing) {
@@ -2075,11 +2083,15 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         Region r(c, learned ? "mapping_learned" : "mapping_traced");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
                                   Nascii, nN, last, word, work, P.max_mm, bk};
+                        a.nids = Nids;
+                        a.nmask = nmask;
                         map_phased(c, a, rc, nm, cnt, learned, trace);
                     } else {
                         Region r(c, "mapping");
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
                                   Nascii, nN, last, word, work, P.max_mm, bk};
+                        a.nids = Nids;
+                        a.nmask = nmask;
                         const dim3 grid(grid_for(nm, c->tune.map_grid));
                         if (bk && W <= 8)
                             launch_map_fast(rc, W, grid, st, a);
