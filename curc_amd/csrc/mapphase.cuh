@@ -199,11 +199,20 @@ __device__ __forceinline__ void range_read_generic(const MapArgs& a, uint64_t i,
         ++n_probe;
         const uint64_t km = (isN && !nw) ? kmer_ascii(ra + j, MAP_K) : xw(rd, j, MAP_K);
         const uint32_t h = (uint32_t)(murmur64(km) & a.mask);
-        const MapBucket B = map_bucket(a, h);
+        // the compact two-strand index (a.cs) or one strand's slots / CSR lists
+        Bk2 b2;
+        MapBucket B{0, nullptr};
+        if (a.cs) {
+            b2 = bk2_fetch<true>(a, h, RC);
+            B.n = RC ? b2.nr : b2.nf;
+        } else {
+            B = map_bucket(a, h);
+        }
         range_bucket(c, B.n);
         const uint64_t ls = RC ? (uint64_t)(a.L - MAP_K - j) : (uint64_t)j;
         for (uint32_t p = 0; p < B.n; ++p) {
-            const uint64_t rp = a.start + (uint64_t)ld32(B.p + p) * MAP_REF_STEP;
+            const uint32_t pv = a.cs ? bk2_pos<true>(b2, RC, p) : ld32(B.p + p);
+            const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
             if (rp < ls) continue;
             const uint64_t mp = rp - ls;
             if (mp + a.L > a.pg_len) continue;
@@ -232,10 +241,11 @@ __device__ __forceinline__ void range_read_generic(const MapArgs& a, uint64_t i,
     }
 }
 
-// Probes [t0, t1) of one non-N read with the bucket-inline index and the
+// Probes [t0, t1) of one non-N read with the bucket-inline index (one
+// strand's 64-byte slots, or with CS the compact two-strand index) and the
 // read in registers (as k_map_fast): the same candidates, counts and
 // partial sums as range_read_generic.
-template <bool RC, uint32_t WM>
+template <bool RC, uint32_t WM, bool CS>
 __device__ __forceinline__ void range_read_fast(const MapArgs& a, uint64_t i, uint32_t t0, uint32_t t1,
                                                 uint32_t& last, uint64_t& w, RangeState& st,
                                                 uint32_t& n_probe, uint32_t& n_cand) {
@@ -257,14 +267,23 @@ __device__ __forceinline__ void range_read_fast(const MapArgs& a, uint64_t i, ui
     }
     for (uint32_t tg = t0; tg < t1; tg += MAP_GROUP) {
         uint4 hd[MAP_GROUP];
+        Bk2 hb[MAP_GROUP];
         uint32_t hh[MAP_GROUP];
 #pragma unroll
         for (uint32_t u = 0; u < MAP_GROUP; ++u) {
             hd[u] = make_uint4(0, 0, 0, 0);
+            if constexpr (CS) {
+                hb[u].nf = 0;
+                hb[u].nr = 0;
+                hb[u].nt = 0;
+            }
             hh[u] = 0;
             if (tg + u < t1) {
                 hh[u] = (uint32_t)(murmur64(xw_pick(r, MAP_READ_STEP * (tg + u), MAP_K)) & a.mask);
-                hd[u] = *reinterpret_cast<const uint4*>(a.bk + (uint64_t)hh[u] * MAP_SLOT);
+                if constexpr (CS)
+                    hb[u] = bk2_fetch<true>(a, hh[u], RC);
+                else
+                    hd[u] = *reinterpret_cast<const uint4*>(a.bk + (uint64_t)hh[u] * MAP_SLOT);
             }
         }
 #pragma unroll
@@ -272,15 +291,23 @@ __device__ __forceinline__ void range_read_fast(const MapArgs& a, uint64_t i, ui
             const uint32_t t = tg + u;
             if (t >= t1) break;
             ++n_probe;
-            const uint32_t nb = hd[u].x;
+            uint32_t nb;
+            if constexpr (CS)
+                nb = RC ? hb[u].nr : hb[u].nf;
+            else
+                nb = hd[u].x;
             range_bucket(c, nb);
             const uint32_t j = MAP_READ_STEP * t;
             const uint64_t ls = RC ? (uint64_t)(L - MAP_K - j) : (uint64_t)j;
             for (uint32_t p = 0; p < nb; ++p) {
-                const uint32_t pv = p == 0   ? hd[u].y
-                                    : p == 1 ? hd[u].z
-                                    : p == 2 ? hd[u].w
-                                             : ld32(a.bk + (uint64_t)hh[u] * MAP_SLOT + 1 + p);
+                uint32_t pv;
+                if constexpr (CS)
+                    pv = bk2_pos<true>(hb[u], RC, p);
+                else
+                    pv = p == 0   ? hd[u].y
+                         : p == 1 ? hd[u].z
+                         : p == 2 ? hd[u].w
+                                  : ld32(a.bk + (uint64_t)hh[u] * MAP_SLOT + 1 + p);
                 const uint64_t rp = a.start + (uint64_t)pv * MAP_REF_STEP;
                 if (rp < ls) continue;
                 const uint64_t mp = rp - ls;
@@ -312,10 +339,11 @@ __device__ __forceinline__ void range_read_fast(const MapArgs& a, uint64_t i, ui
 
 // Probes [t0, t1) (probe t reads the k-mer at base 2t) of every read in the
 // active list, continuing k_map's loop from the saved state.  WM > 0 takes
-// the register form for non-N reads (bucket-inline index, W <= WM).
+// the register form for non-N reads (bucket-inline index, W <= WM; CS: the
+// compact two-strand index).
 // With x, each read of the list also writes its features after the range
 // (row q of x, t1 probes done); reads already settled write a row too.
-template <bool RC, uint32_t WM>
+template <bool RC, uint32_t WM, bool CS>
 __global__ void __launch_bounds__(256) k_map_range(MapArgs a, MapState s, const uint32_t* __restrict__ act,
                                                    const uint32_t* __restrict__ d_n, uint32_t t0, uint32_t t1,
                                                    float* __restrict__ x) {
@@ -337,7 +365,7 @@ __global__ void __launch_bounds__(256) k_map_range(MapArgs a, MapState s, const 
         RangeState st;
         st.load(s, i);
         if (WM > 0 && !isN)
-            range_read_fast<RC, (WM > 0 ? WM : 1)>(a, i, t0, t1, last, w, st, n_probe, n_cand);
+            range_read_fast<RC, (WM > 0 ? WM : 1), CS>(a, i, t0, t1, last, w, st, n_probe, n_cand);
         else
             range_read_generic<RC>(a, i, t0, t1, last, w, st, n_probe, n_cand);
         a.last[i] = (uint8_t)last;

@@ -846,18 +846,26 @@ void launch_map_fast(bool rc, uint32_t W, dim3 grid, hipStream_t st, const MapAr
 }
 
 // k_map_range in register form for W <= 8 with the bucket-inline index
-// (wbk = W; 0 for the CSR index), else in generic form.
+// (wbk = W; 0 for the CSR index), on one strand's slots or the compact
+// two-strand index (a.cs), else in generic form.
 void launch_map_range(bool rc, uint32_t wbk, dim3 grid, hipStream_t st, const MapArgs& a, const MapState& ms,
                       const uint32_t* act, const uint32_t* nact, uint32_t t0, uint32_t t1, float* x) {
-#define CURC_MAP_RANGE(WM)                                                                                   \
-    do {                                                                                                     \
-        if (rc)                                                                                              \
-            hipLaunchKernelGGL((k_map_range<true, WM>), grid, dim3(256), 0, st, a, ms, act, nact, t0, t1, x);  \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_map_range<false, WM>), grid, dim3(256), 0, st, a, ms, act, nact, t0, t1, x); \
+#define CURC_MAP_RANGE_CS(WM, CS)                                                                                 \
+    do {                                                                                                          \
+        if (rc)                                                                                                   \
+            hipLaunchKernelGGL((k_map_range<true, WM, CS>), grid, dim3(256), 0, st, a, ms, act, nact, t0, t1, x);  \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_map_range<false, WM, CS>), grid, dim3(256), 0, st, a, ms, act, nact, t0, t1, x); \
+    } while (0)
+#define CURC_MAP_RANGE(WM)               \
+    do {                                 \
+        if (a.cs)                        \
+            CURC_MAP_RANGE_CS(WM, true);  \
+        else                             \
+            CURC_MAP_RANGE_CS(WM, false); \
     } while (0)
     if (wbk == 0 || wbk > 8)
-        CURC_MAP_RANGE(0);
+        CURC_MAP_RANGE_CS(0, false);
     else if (wbk <= 3)
         CURC_MAP_RANGE(3);
     else if (wbk == 4)
@@ -867,6 +875,7 @@ void launch_map_range(bool rc, uint32_t wbk, dim3 grid, hipStream_t st, const Ma
     else
         CURC_MAP_RANGE(8);
 #undef CURC_MAP_RANGE
+#undef CURC_MAP_RANGE_CS
 }
 
 // "key=value,key=value" over the Tuning fields (unknown keys ignored).
@@ -1944,11 +1953,31 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         if (nN)
             hipLaunchKernelGGL(k_nmask, dim3(grid_for((uint64_t)nN * W)), dim3(256), 0, st, (const uint8_t*)Nascii,
                                (uint64_t)nN, L, W, nmask);
-        // both strand passes in one launch (k_map_both) on an index holding
-        // both strands' slots of a bucket in one 128-byte line: the plain
-        // mode, one index part, bucket-inline slots, reads of <= 256 bases
+        // work counters of a mapping pass, when timing (stats entries named
+        // map_*), and its algorithmic bytes: each active read's words, its
+        // state write (1 + 8 B), 16 B per k-mer probe (two CSR offsets and
+        // the hash of a register-resident k-mer) and, per verified
+        // candidate, its 4 B position plus the P bytes of pseudogenome
+        auto map_stats = [&](const char* region, unsigned long long* work) {
+            if (!c->timing) return;
+            unsigned long long wk[8];
+            CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
+            CURC_HIP(stream_wait(st));
+            const double P8 = 8.0 * W;
+            c->add_bytes(region, (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
+                         (double)wk[2]);
+            const char* names[6] = {"map_active", "map_probes", "map_cands", "map_first_probe", "map_open_probes",
+                                    "map_open_cands"};
+            for (int q = 0; q < 6; ++q) c->add_bytes(names[q], 0.0, (double)wk[q]);
+        };
+        const char* phased_region = learned ? "mapping_learned" : "mapping_traced";
+        // both strands' buckets in one index: bucket-inline slots, one index
+        // part, reads of <= 256 bases.  The plain mode runs both strand passes
+        // in one launch (k_map_both); the phased modes (compact slots only)
+        // run their probe ranges strand by strand on it.
         const uint32_t H1 = index_count ? hash_size(index_count) : 0;
-        const bool both = c->tune.map_both && !phased && L >= MAP_K && nm > 0 && W <= 8 && !c->index_csr &&
+        const bool both = c->tune.map_both && (!phased || c->tune.map_both == 2) && L >= MAP_K && nm > 0 && W <= 8 &&
+                          !c->index_csr &&
                           index_count > 0 && index_count <= (1ull << 30) && H1 <= MAP_SLOT_MAX_H &&
                           index_count * 8 >= H1 && ref_len < (1ull << 32);  // 32-bit positions
         if (both) {
@@ -1999,30 +2028,31 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 c->add_bytes("map_index", 2.0 * ((double)count * (16 + 8) + 8.0 * H), 2.0 * (double)count);
             }
             unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
-            fill<unsigned long long>(c, work, 8, 0ull);
-            {
-                Region r(c, "mapping");
-                MapArgs a{hq_pg, ref_len, nullptr, nullptr, H - 1, 0, packed, W, L, unm, n_unm,
-                          Nascii, nN, last, word, work, P.max_mm, bk};
-                a.bk_stride = MAP_SLOT2;
-                a.bk_off = 0;
-                a.cs = cs ? 1u : 0u;
-                a.ovf = ovf;
-                a.nids = Nids;
-                a.nmask = nmask;
-                launch_map_both(W, c->tune.map_grid, st, a);
-            }
-            if (c->timing) {
+            MapArgs a{hq_pg, ref_len, nullptr, nullptr, H - 1, 0, packed, W, L, unm, n_unm,
+                      Nascii, nN, last, word, work, P.max_mm, bk};
+            a.bk_stride = MAP_SLOT2;
+            a.bk_off = 0;
+            a.cs = cs ? 1u : 0u;
+            a.ovf = ovf;
+            a.nids = Nids;
+            a.nmask = nmask;
+            if (phased) {
+                for (int strand = 0; strand < 2; ++strand) {
+                    fill<unsigned long long>(c, work, 8, 0ull);
+                    {
+                        Region r(c, phased_region);
+                        map_phased(c, a, strand == 1, nm, cnt, learned, trace);
+                    }
+                    map_stats(phased_region, work);
+                }
+            } else {
+                fill<unsigned long long>(c, work, 8, 0ull);
+                {
+                    Region r(c, "mapping");
+                    launch_map_both(W, c->tune.map_grid, st, a);
+                }
                 // as for the separate passes, over both strands
-                unsigned long long wk[8];
-                CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
-                CURC_HIP(stream_wait(st));
-                const double P8 = 8.0 * W;
-                c->add_bytes("mapping", (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
-                             (double)wk[2]);
-                const char* names[6] = {"map_active", "map_probes", "map_cands", "map_first_probe",
-                                        "map_open_probes", "map_open_cands"};
-                for (int q = 0; q < 6; ++q) c->add_bytes(names[q], 0.0, (double)wk[q]);
+                map_stats("mapping", work);
             }
         } else if (L >= MAP_K && nm > 0) {
             for (int strand = 0; strand < 2; ++strand) {
@@ -2080,7 +2110,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     unsigned long long* work = S.as<unsigned long long>(S_MWORK, 8);
                     fill<unsigned long long>(c, work, 8, 0ull);
                     if (phased) {
-                        Region r(c, learned ? "mapping_learned" : "mapping_traced");
+                        Region r(c, phased_region);
                         MapArgs a{hq_pg, ref_len, kr, pos, H - 1, start, packed, W, L, unm, n_unm,
                                   Nascii, nN, last, word, work, P.max_mm, bk};
                         a.nids = Nids;
@@ -2100,23 +2130,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                         else
                             hipLaunchKernelGGL(k_map<false>, grid, dim3(256), 0, st, a);
                     }
-                    if (c->timing) {
-                        // algorithmic bytes of one launch: each active read's words, its
-                        // state write (1 + 8 B), 16 B per k-mer probe (two CSR offsets and
-                        // the hash of a register-resident k-mer) and, per verified
-                        // candidate, its 4 B position plus the P bytes of pseudogenome
-                        unsigned long long wk[8];
-                        CURC_HIP(hipMemcpyAsync(wk, work, sizeof(wk), hipMemcpyDeviceToHost, st));
-                        CURC_HIP(stream_wait(st));
-                        const double P8 = 8.0 * W;
-                        c->add_bytes(phased ? (learned ? "mapping_learned" : "mapping_traced") : "mapping",
-                                     (double)wk[0] * (P8 + 9) + (double)wk[1] * 16 + (double)wk[2] * (4 + P8),
-                                     (double)wk[2]);
-                        // work profile of the pass (stats entries named map_*)
-                        const char* names[6] = {"map_active", "map_probes", "map_cands", "map_first_probe",
-                                                "map_open_probes", "map_open_cands"};
-                        for (int q = 0; q < 6; ++q) c->add_bytes(names[q], 0.0, (double)wk[q]);
-                    }
+                    map_stats(phased ? phased_region : "mapping", work);
                 }
             }
         }

@@ -96,3 +96,53 @@ def test_stopping_keeps_records_exact_and_lossless(gpu_ctx, threshold):
     assert sorted(map(bytes, dec)) == sorted(map(bytes, r))
     gpu_ctx.set_scorer(None)
     net.close()
+
+
+def test_ranges_on_the_compact_index():
+    """A block large enough for the bucket-inline index (6 Mbp genome, 1.2M
+    reads): the probe ranges run on the compact two-strand index (tuning
+    map_both 2) or on one strand's 64-byte slots (map_both 0).  Without
+    stops both equal the exhaustive mapping and walk the same probes and
+    candidates; with a stopping scorer both stop the same reads, since the
+    features do not depend on the index layout."""
+    cfg = ca.config_scale(seed=83, n_reads=1_200_000, genome_len=6_000_000)
+    r = ca.synth_reads(cfg)[1]
+
+    def work(ctx, **kw):
+        ctx.reset_stats()
+        ctx.set_timing(True)
+        b = ctx.run_block(r, **kw)
+        ctx.set_timing(False)
+        st = {s["name"]: s for s in ctx.stats()}
+        return b, {k: st[k]["items"] for k in ("map_probes", "map_cands")}
+
+    with ca.Context(0) as ctx:
+        ctx.set_trace(every=64, max_records=1 << 20)
+        print("compact index: plain", flush=True)
+        plain, wp = work(ctx)
+        runs = {}
+        for mb in (2, 0):
+            ctx.set_tuning(map_both=mb)
+            print(f"compact index: trace, map_both {mb}", flush=True)
+            runs[mb] = work(ctx, trace=True)
+        for mb, (b, w) in runs.items():
+            for f in MAP_FIELDS:
+                assert np.array_equal(plain.get(f), b.get(f)), (mb, f)
+            assert w == wp, (mb, w, wp)
+        # trace records come in the order their reads claimed them
+        rows = lambda b: (lambda x: x[np.lexsort(x.T[::-1])])(b.get("trace_x"))
+        assert np.array_equal(rows(runs[2][0]), rows(runs[0][0]))
+        net = scorer.Mlp(ctx, scorer.init_params(5))
+        ctx.set_scorer(net, threshold=0.0)
+        got = {}
+        for mb in (2, 0):
+            ctx.set_tuning(map_both=mb)
+            print(f"compact index: learned, map_both {mb}", flush=True)
+            got[mb] = work(ctx, learned=True)
+        for f in MAP_FIELDS:
+            assert np.array_equal(got[2][0].get(f), got[0][0].get(f)), f
+        assert got[2][1] == got[0][1]
+        assert got[2][1]["map_probes"] <= wp["map_probes"]
+        recount(got[2][0], r)
+        ctx.set_scorer(None)
+        net.close()
