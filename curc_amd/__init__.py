@@ -88,7 +88,7 @@ BLOCK_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_voi
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("ov_grid", "fm_grid", "map_grid", "radix_items", "workers",
                                                "filter_bits", "fm_fused", "ov_queue", "map_both",
-                                               "fm_static")]
+                                               "fm_static", "fq_host")]
 
 
 class StageStat(ctypes.Structure):
@@ -167,6 +167,8 @@ def lib():
         "curc_synth_reads": (i32, [ctypes.POINTER(SynthCfg), vp, vp]),
         "curc_synth_fastq_record_len": (u64, [u32, ctypes.c_char_p, i32]),
         "curc_synth_fastq": (i32, [vp, u64, u32, u64, u64, ctypes.c_char_p, i32, vp]),
+        "curc_fastq_host_cap": (u64, [u64, u32, u32]),
+        "curc_fastq_host_parse": (i32, [vp, u64, u32, u32, vp, u64, ctypes.POINTER(u64), ctypes.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -187,6 +189,27 @@ def device_count():
     n = ctypes.c_int(0)
     lib().curc_device_count(ctypes.byref(n))
     return n.value
+
+
+# ------------------------------------------------------------ host parse ---
+FASTQ_FORMAT_ERRORS = {1: "FASTQ block does not start with '@'", 2: "truncated FASTQ record",
+                       3: "read length isn't fix"}
+
+
+def fastq_host_parse(buf, L, threads=8):
+    """The library's host-side FASTQ record parse (curc_fastq_host_parse)
+    of one file's bytes -> (n, L) uint8 sequences; a format error raises
+    CurcError with the device parser's message."""
+    a = np.frombuffer(buf, dtype=np.uint8) if not isinstance(buf, np.ndarray) else buf
+    cap = int(lib().curc_fastq_host_cap(len(a), L, threads))
+    out = np.empty(max(cap, 1), dtype=np.uint8)
+    n = ctypes.c_uint64(0)
+    kind = ctypes.c_int(0)
+    rc = lib().curc_fastq_host_parse(a.ctypes.data if len(a) else None, len(a), L, threads, out.ctypes.data, cap,
+                                     ctypes.byref(n), ctypes.byref(kind))
+    if rc != CURC_OK:
+        raise CurcError(FASTQ_FORMAT_ERRORS.get(kind.value, f"curc_fastq_host_parse failed ({rc})"))
+    return out[:n.value * L].reshape(n.value, L).copy()
 
 
 # ----------------------------------------------------------------- synth ---

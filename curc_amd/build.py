@@ -18,7 +18,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libcurc_gpu.so")
 
 HIP_SOURCES = ["pipeline.hip"]
-CXX_SOURCES = ["synth.cpp"]
+CXX_SOURCES = ["synth.cpp", "fqhost.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CURC_OFFLOAD_ARCH", "gfx950")

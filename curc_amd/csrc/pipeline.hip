@@ -28,6 +28,7 @@
 #include "refmatch.cuh"
 #include "graph_cache.h"
 #include "host_pool.h"
+#include "fqhost.h"
 #include "streams.cuh"
 
 using namespace curc;
@@ -174,6 +175,7 @@ struct Tuning {
     uint32_t ov_queue = 2;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
     uint32_t fm_static = 1;      // exact matching over lists kept in place (1) or compacted per offset (0)
+    uint32_t fq_host = 8;        // host threads that parse FASTQ records before the copy (0: parse on the device)
 };
 
 struct curc_ctx {
@@ -192,6 +194,20 @@ struct curc_ctx {
     // uploads of the next FASTQ block while the current one runs
     // (curc_fastq_stream_run); created on first use
     hipStream_t ust = nullptr;
+    // pinned host buffers of host-parsed FASTQ sequences (tuning fq_host),
+    // two so that one is filled while the other is copied; grown on demand
+    uint8_t* hstage[2] = {nullptr, nullptr};
+    uint64_t hstage_cap[2] = {0, 0};
+    uint8_t* stage(int k, uint64_t bytes) {
+        if (hstage_cap[k] < bytes) {
+            if (hstage[k]) CURC_HIP(hipHostFree(hstage[k]));
+            hstage[k] = nullptr;
+            hstage_cap[k] = 0;
+            CURC_HIP(hipHostMalloc((void**)&hstage[k], bytes, hipHostMallocDefault));
+            hstage_cap[k] = bytes;
+        }
+        return hstage[k];
+    }
     // overlap probes and bucket searches of the last block (copied from the
     // device after the loop; outlives the block's stack frame)
     uint64_t ov_work[2] = {0, 0};
@@ -894,6 +910,7 @@ void parse_tuning(const char* text, Tuning& t) {
             if (k == "ov_queue" && v <= 2) t.ov_queue = v;
             if (k == "map_both" && v <= 2) t.map_both = v;
             if (k == "fm_static" && v <= 1) t.fm_static = v;
+            if (k == "fq_host" && v <= CURC_FQ_HOST_MAX) t.fq_host = v;
             if (v > 0) {
                 if (k == "ov_grid") t.ov_grid = v;
                 else if (k == "fm_grid") t.fm_grid = v;
@@ -1410,6 +1427,10 @@ struct InputError : std::runtime_error {
 struct FqInput {
     const uint8_t* fq[2] = {nullptr, nullptr};
     uint64_t bytes[2] = {0, 0};
+    // records parsed on the host: their sequences, n_ascii rows of L bytes
+    // in read id order, in device memory (fq then unused)
+    const uint8_t* ascii = nullptr;
+    uint64_t n_ascii = 0;
     hipEvent_t ready = nullptr;
     hipEvent_t done = nullptr;
 };
@@ -1504,7 +1525,13 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     // ---- a1/a2: record parse, N split + pack (preprocess.cpp:147-199, compress.cu:2151-2180)
     const uint8_t* d_ascii = nullptr;
     FqReads fr{};
-    if (fq) {
+    // FASTQ bytes parsed here on the device (else sequences parsed on the host)
+    const bool fqd = fq && !fq->ascii;
+    if (fq && fq->ascii) {
+        if (fq->ready) CURC_HIP(hipStreamWaitEvent(st, fq->ready, 0));
+        d_ascii = fq->ascii;
+        N = fq->n_ascii;
+    } else if (fq) {
         if (fq->ready) CURC_HIP(hipStreamWaitEvent(st, fq->ready, 0));
         Region r(c, "parse");
         uint64_t* seq = nullptr;
@@ -1535,7 +1562,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         fill<uint64_t>(c, packed + N * W, 1, 0ull);
         // one thread per packed word; isN is set only for reads with an 'N'
         fill<uint8_t>(c, isN, N + 1, (uint8_t)0);
-        if (N && fq)
+        if (N && fqd)
             hipLaunchKernelGGL(k_pack_fq, dim3(grid_for(N * W, 1u << 20)), dim3(256), 0, st, fr, N, L, W, packed,
                                isN, cnt + C_FQERR);
         else if (N)
@@ -1550,12 +1577,12 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint32_t h[C_COUNT];
         CURC_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
         CURC_HIP(stream_wait(st));
-        if (fq && h[C_FQERR]) throw InputError("read length isn't fix");  // preprocess.cpp:154
+        if (fqd && h[C_FQERR]) throw InputError("read length isn't fix");  // preprocess.cpp:154
         n = h[C_NONN];
         nN = h[C_NN];
     }
     uint8_t* Nascii = S.as<uint8_t>(S_NASCII, (uint64_t)nN * L + 16);
-    if (nN && fq)
+    if (nN && fqd)
         hipLaunchKernelGGL(k_gather_ascii_fq, dim3(grid_for((uint64_t)nN * L)), dim3(256), 0, st, fr,
                            (const uint32_t*)Nids, (uint64_t)nN, L, Nascii);
     else if (nN)
@@ -2446,6 +2473,77 @@ uint64_t fq_dev_bytes(const curc_fastq_job& j) {
     return ((j.bytes1 + 255) & ~255ull) + (j.fq2 ? j.bytes2 : 0) + 256;
 }
 
+// Device bytes a job needs: its FASTQ bytes, or with host parsing its
+// sequences (at most the FASTQ bytes: a record takes at least L + 4 bytes),
+// twice over for a paired block (each file's rows, then the interleaved rows).
+uint64_t fq_dev_bytes(const curc_fastq_job& j, uint32_t host_threads) {
+    if (!host_threads || !j.fq2) return fq_dev_bytes(j);
+    return 2 * (j.bytes1 + j.bytes2) + 512;
+}
+
+// Host threads for a file of nb bytes: ranges of at least 1 MB each.
+uint32_t fq_host_threads(uint32_t want, uint64_t nb) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, nb >> 20));
+}
+
+// Rows 2k (file 1's record k) and 2k+1 (file 2's) of a paired block.
+__global__ void k_interleave_rows(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b, uint64_t n1,
+                                  uint32_t L, uint8_t* __restrict__ out) {
+    GRID_STRIDE(i, 2 * n1 * L) {
+        const uint64_t r = i / L, j = i - r * L;
+        out[i] = ((r & 1) ? b : a)[(r >> 1) * L + j];
+    }
+}
+
+// Host parse of a job's FASTQ bytes (fq_host_parse, `threads` threads) into
+// the pinned buffer c->stage(which), then copies of the sequences to the
+// device buffer `dev` (fq_dev_bytes(j, threads) bytes) on stream s.  The
+// caller makes sure earlier copies out of that pinned buffer have ended.
+// Format errors throw InputError with the device parser's messages, in its
+// order (preprocess.cpp:147-199).
+FqInput fq_stage_host(curc_ctx* c, const curc_fastq_job& j, uint32_t threads, int which, uint8_t* dev,
+                      hipStream_t s) {
+    const int nf = j.fq2 ? 2 : 1;
+    const uint64_t nb[2] = {j.bytes1, j.fq2 ? j.bytes2 : 0};
+    const uint8_t* src[2] = {(const uint8_t*)j.fq1, (const uint8_t*)j.fq2};
+    uint32_t th[2];
+    uint64_t cap[2] = {0, 0};
+    for (int f = 0; f < nf; ++f) {
+        th[f] = fq_host_threads(threads, nb[f]);
+        cap[f] = fq_host_cap(nb[f], j.L, th[f]);
+    }
+    uint8_t* h = c->stage(which, cap[0] + cap[1] + 64);
+    FqHostParse r[2];
+    for (int f = 0; f < nf; ++f) fq_host_parse(src[f], nb[f], j.L, th[f], h + (f ? cap[0] : 0), &r[f]);
+    for (int f = 0; f < nf; ++f)
+        if (r[f].err == FQH_START) throw InputError("FASTQ block does not start with '@'");
+    for (int f = 0; f < nf; ++f)
+        if (r[f].err == FQH_TRUNC) throw InputError("truncated FASTQ record");
+    if (nf == 2 && r[0].n != r[1].n) throw InputError("paired-end files hold different record counts");
+    for (int f = 0; f < nf; ++f)
+        if (r[f].err == FQH_LEN) throw InputError("read length isn't fix");  // preprocess.cpp:154
+    const uint64_t n1 = r[0].n, N = n1 * nf;
+    if (N >= (uint64_t)INV) throw std::length_error("a block holds at most 2^32-2 reads");
+    FqInput in;
+    in.n_ascii = N;
+    in.ascii = dev;
+    const uint64_t L = j.L;
+    for (int f = 0; f < nf; ++f) {
+        // single-end rows straight to their place; a paired block's files to
+        // the rows after the interleaved ones, then interleaved on the device
+        uint8_t* d = nf == 1 ? dev : dev + (2 + f) * n1 * L;
+        uint64_t at = 0;
+        for (const FqSeg& g : r[f].segs) {
+            CURC_HIP(hipMemcpyAsync(d + at * L, h + (f ? cap[0] : 0) + g.off, g.count * L, hipMemcpyHostToDevice, s));
+            at += g.count;
+        }
+    }
+    if (nf == 2 && n1)
+        hipLaunchKernelGGL(k_interleave_rows, dim3(grid_for(2 * n1 * L)), dim3(256), 0, s,
+                           (const uint8_t*)(dev + 2 * n1 * L), (const uint8_t*)(dev + 3 * n1 * L), n1, j.L, dev);
+    return in;
+}
+
 int fq_check_job(curc_ctx* c, const curc_fastq_job* j) {
     if (!j) return fail(c, CURC_EINVAL, "null job");
     if ((j->bytes1 && !j->fq1) || (j->fq2 == nullptr && j->bytes2)) return fail(c, CURC_EINVAL, "null input");
@@ -2458,7 +2556,8 @@ uint32_t fq_flags(const curc_fastq_job& j) { return j.flags | (j.fq2 ? CURC_RUN_
 
 // One FASTQ job: bytes already on the device, or copied into scratch slot
 // `slot` first on the context's stream.
-int fq_block_run(curc_ctx* c, const curc_fastq_job& j, int stop_after, int slot, curc_block** out) {
+int fq_block_run(curc_ctx* c, const curc_fastq_job& j, int stop_after, int slot, curc_block** out,
+                 uint32_t host_threads) {
     int rc = fq_check_job(c, &j);
     if (rc != CURC_OK) return rc;
     FqInput in;
@@ -2469,12 +2568,16 @@ int fq_block_run(curc_ctx* c, const curc_fastq_job& j, int stop_after, int slot,
             in.bytes[0] = j.bytes1;
             in.fq[1] = (const uint8_t*)j.fq2;
             in.bytes[1] = j.fq2 ? j.bytes2 : 0;
+        } else if (host_threads) {
+            // the previous block's copies out of the pinned buffer ended with
+            // its block (block_run waits for the stream)
+            in = fq_stage_host(c, j, host_threads, 0, c->S.as<uint8_t>(slot, fq_dev_bytes(j, host_threads)), c->st);
         } else {
             in = fq_upload(j, c->S.as<uint8_t>(slot, fq_dev_bytes(j)), c->st);
         }
     } catch (const std::exception& e) {
         stream_wait(c->st);
-        return fail(c, CURC_EHIP, e.what());
+        return fail(c, dynamic_cast<const InputError*>(&e) ? CURC_EINVAL : CURC_EHIP, e.what());
     }
     return block_run(c, nullptr, 0, j.L, stop_after, fq_flags(j), &in, out);
 }
@@ -2721,6 +2824,8 @@ void curc_ctx_destroy(curc_ctx* c) {
         stream_wait(c->ust);
         hipStreamDestroy(c->ust);
     }
+    for (auto& h : c->hstage)
+        if (h) hipHostFree(h);
     hipStreamDestroy(c->st);
     delete c;
 }
@@ -2985,7 +3090,7 @@ int curc_batch_run(curc_ctx* c, const curc_job* jobs, uint32_t n_jobs, uint32_t 
 int curc_fastq_block_run(curc_ctx* c, const curc_fastq_job* job, int stop_after, curc_block** out) {
     if (!c || !out) return CURC_EINVAL;
     *out = nullptr;
-    return fq_block_run(c, *job, stop_after, S_FQ0, out);
+    return fq_block_run(c, *job, stop_after, S_FQ0, out, c->tune.fq_host);
 }
 
 int curc_fastq_batch_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jobs, uint32_t workers,
@@ -2993,8 +3098,11 @@ int curc_fastq_batch_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_job
     if (!c || (!jobs && n_jobs) || (!out && n_jobs)) return CURC_EINVAL;
     std::vector<uint64_t> w(n_jobs);
     for (uint32_t j = 0; j < n_jobs; ++j) w[j] = jobs[j].bytes1 + (jobs[j].fq2 ? jobs[j].bytes2 : 0);
+    // the host parse threads shared among the workers
+    const uint32_t nw = std::max<uint32_t>(1, std::min<uint32_t>(workers ? workers : c->tune.workers, n_jobs));
+    const uint32_t ht = c->tune.fq_host ? std::max<uint32_t>(1, c->tune.fq_host / nw) : 0;
     return run_on_workers(c, n_jobs, workers, w, out, status, [&](curc_ctx* wc, uint32_t j, curc_block** o) {
-        return fq_block_run(wc, jobs[j], CURC_STAGE_ALL, S_FQ0, o);
+        return fq_block_run(wc, jobs[j], CURC_STAGE_ALL, S_FQ0, o, ht);
     });
 }
 
@@ -3012,6 +3120,9 @@ int curc_fastq_stream_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jo
             return fail(c, CURC_EINVAL, "curc_fastq_stream_run copies host bytes (no CURC_RUN_INPUT_ON_DEVICE)");
     }
     if (n_jobs == 0) return CURC_OK;
+    // host parse (tuning fq_host): the uploader parses block i into pinned
+    // buffer i & 1 and copies only its sequences
+    const uint32_t host_threads = c->tune.fq_host;
     uint8_t* buf[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> up(n_jobs, nullptr), used(n_jobs, nullptr);
     auto release = [&]() {
@@ -3023,7 +3134,7 @@ int curc_fastq_stream_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jo
     try {
         CURC_HIP(hipSetDevice(c->device));
         uint64_t mx = 0;
-        for (uint32_t j = 0; j < n_jobs; ++j) mx = std::max(mx, fq_dev_bytes(jobs[j]));
+        for (uint32_t j = 0; j < n_jobs; ++j) mx = std::max(mx, fq_dev_bytes(jobs[j], host_threads));
         buf[0] = c->S.as<uint8_t>(S_FQ0, mx);
         buf[1] = c->S.as<uint8_t>(S_FQ1, mx);
         if (!c->ust) CURC_HIP(hipStreamCreateWithFlags(&c->ust, hipStreamNonBlocking));
@@ -3042,6 +3153,7 @@ int curc_fastq_stream_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jo
     std::vector<FqInput> inputs(n_jobs);
     bool stop = false;
     std::string up_err;
+    int up_code = CURC_EHIP;
     std::thread uploader([&]() {
         hipSetDevice(c->device);
         for (uint32_t i = 0; i < n_jobs; ++i) {
@@ -3052,12 +3164,18 @@ int curc_fastq_stream_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jo
             }
             int ok = 1;
             try {
-                if (i >= 2) CURC_HIP(hipStreamWaitEvent(c->ust, used[i - 2], 0));
-                inputs[i] = fq_upload(jobs[i], buf[i & 1], c->ust);
+                if (i >= 2) {
+                    // pinned buffer i & 1 is free once block i - 2's copies ended
+                    if (host_threads) CURC_HIP(hipEventSynchronize(up[i - 2]));
+                    CURC_HIP(hipStreamWaitEvent(c->ust, used[i - 2], 0));
+                }
+                inputs[i] = host_threads ? fq_stage_host(c, jobs[i], host_threads, (int)(i & 1), buf[i & 1], c->ust)
+                                         : fq_upload(jobs[i], buf[i & 1], c->ust);
                 CURC_HIP(hipEventRecord(up[i], c->ust));
             } catch (const std::exception& e) {
                 ok = -1;
                 up_err = e.what();
+                if (dynamic_cast<const InputError*>(&e)) up_code = CURC_EINVAL;
             }
             {
                 std::lock_guard<std::mutex> lk(m);
@@ -3078,7 +3196,7 @@ int curc_fastq_stream_run(curc_ctx* c, const curc_fastq_job* jobs, uint32_t n_jo
         curc_block* B = nullptr;
         int rc;
         if (state < 0) {
-            rc = fail(c, CURC_EHIP, up_err);
+            rc = fail(c, up_code, up_err);
         } else {
             FqInput in = inputs[i];
             in.ready = up[i];
@@ -3394,6 +3512,7 @@ int curc_ctx_get_tuning(curc_ctx* c, curc_tuning* t) {
     t->ov_queue = c->tune.ov_queue;
     t->map_both = c->tune.map_both;
     t->fm_static = c->tune.fm_static;
+    t->fq_host = c->tune.fq_host;
     return CURC_OK;
 }
 
@@ -3402,7 +3521,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
         (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
         t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2 || t->map_both > 2 ||
-        t->fm_static > 1)
+        t->fm_static > 1 || t->fq_host > CURC_FQ_HOST_MAX)
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;
     c->tune.fm_grid = t->fm_grid;
@@ -3414,6 +3533,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     c->tune.ov_queue = t->ov_queue;
     c->tune.map_both = t->map_both;
     c->tune.fm_static = t->fm_static;
+    c->tune.fq_host = t->fq_host;
     return CURC_OK;
 }
 

@@ -294,6 +294,18 @@ int curc_fastq_stream_run(curc_ctx *ctx, const curc_fastq_job *jobs, uint32_t n_
                           void *user);
 /* Reads in a block (the parsed record count for FASTQ input). */
 uint64_t curc_block_reads(const curc_block *b);
+/* The record loop of preprocess.cpp:147-199 on the host, for one file's
+ * bytes (no device needed): the sequence lines, L bytes each, in record
+ * order, into out (out_bytes >= curc_fastq_host_cap(bytes, L, threads)),
+ * using `threads` threads.  *format_error gets 0, or the first rule the
+ * bytes break, in the device parser's order: 1 they do not start with '@',
+ * 2 they end inside a record, 3 a sequence line does not hold L bytes
+ * (:154); then CURC_EINVAL is returned.  curc_fastq_stream_run and
+ * curc_fastq_block_run parse host bytes this way when the tuning field
+ * fq_host is not 0, and copy only the sequences to the device. */
+uint64_t curc_fastq_host_cap(uint64_t bytes, uint32_t L, uint32_t threads);
+int curc_fastq_host_parse(const char *fq, uint64_t bytes, uint32_t L, uint32_t threads, char *out,
+                          uint64_t out_bytes, uint64_t *n_reads, int *format_error);
 /* Record parse + N flag + 2-bit pack of single-end FASTQ bytes on the device
  * (d_fq 16-byte aligned): the parse part of process() and encode
  * (preprocess.cpp:147-199, compress.cu:2151-2180) for one buffer.  Writes
@@ -425,7 +437,12 @@ typedef struct curc_tuning {
     uint32_t fm_static;   /* 1: exact matching keeps its lists in place (linked reads skipped by
                              flag and skip pointer, suffixes screened by a prefix filter); 0: both
                              lists compacted after every offset (default 1) */
+    uint32_t fq_host;     /* host threads that parse FASTQ records of host bytes before the copy
+                             (curc_fastq_host_parse), so that only the sequences are copied to the
+                             device; 0: the bytes are copied as they are and parsed on the device.
+                             At most CURC_FQ_HOST_MAX (default 8) */
 } curc_tuning;
+#define CURC_FQ_HOST_MAX 64u
 int curc_ctx_get_tuning(curc_ctx *ctx, curc_tuning *t);
 int curc_ctx_set_tuning(curc_ctx *ctx, const curc_tuning *t);
 

@@ -1,7 +1,10 @@
-"""FASTQ input on the device (curc_fastq_*): the record parse, N split and
-pack of preprocess.cpp:147-199 run on the GPU.  Blocks from FASTQ bytes must
-equal the ASCII-read runs of the same records and the oracle (whose parser
-restates the reference loop); malformed input fails with CURC_EINVAL."""
+"""FASTQ input (curc_fastq_*): the record parse of preprocess.cpp:147-199 on
+the GPU (tuning fq_host 0: the bytes copied as they are) or on host threads
+before the copy (fq_host 8: only the sequences copied), then the N split and
+pack on the GPU.  Every test runs in both modes.  Blocks from FASTQ bytes
+must equal the ASCII-read runs of the same records and the oracle (whose
+parser restates the reference loop); malformed input fails with
+CURC_EINVAL."""
 import ctypes
 
 import numpy as np
@@ -15,6 +18,15 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("nonN_ids", "N_ids", "hq_pg", "hq_rec", "map_rec", "map_mis", "unmapped", "unmapped_N", "lq_pg", "lq_rec",
           "nlq_pg", "nlq_rec")
+
+
+@pytest.fixture(params=[0, 8], ids=["device_parse", "host_parse"], autouse=True)
+def parse_mode(request, gpu_ctx):
+    before = gpu_ctx.get_tuning()
+    gpu_ctx.set_tuning(fq_host=request.param)
+    print(f"fq_host {request.param}", flush=True)
+    yield request.param
+    gpu_ctx.set_tuning(**before)
 
 
 def records(reads, mate=None, vary=True):
