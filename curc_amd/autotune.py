@@ -19,7 +19,7 @@ import time
 
 BLOCK_KNOBS = (
     ("map_grid", (4096, 8192, 16384, 32768, 65536, 1 << 20)),
-    ("ov_queue", (0, 1, 2)),
+    ("ov_queue", (0, 1, 2, 3)),
     ("map_both", (0, 1, 2)),
     ("ov_grid", (1024, 2048, 4096, 16384)),
     ("fm_grid", (2048, 4096, 16384)),

@@ -28,7 +28,7 @@
 namespace curc {
 
 struct GraphKey {
-    static constexpr int N = 14;
+    static constexpr int N = 20;
     uint64_t v[N] = {};
     bool operator<(const GraphKey& o) const { return std::lexicographical_compare(v, v + N, o.v, o.v + N); }
 };

@@ -443,6 +443,7 @@ struct ClaimState {
     uint8_t* plinked;       // per prefix position: linked at this offset (a8 only)
     uint64_t* work;         // a8: suffix-list lengths summed over offsets (or null)
     uint32_t* nf;           // a8 over lists kept in place: prefix skip pointers (see fm_find), or null
+    uint64_t* slbits;       // a6 probe by verdict words: bit si set once suffix si is linked, or null
     __device__ __forceinline__ void link(uint32_t si, uint32_t px, uint32_t off) const {
         const uint32_t s = sl[si], p = pl[px];
         next[s] = p;
@@ -451,6 +452,7 @@ struct ClaimState {
         slinked[si] = 1;
         if (plinked) plinked[px] = 1;
         if (nf) nf[px] = px + 1;
+        if (slbits) atomicOr((unsigned long long*)&slbits[si >> 6], 1ull << (si & 63u));
     }
 };
 
@@ -881,6 +883,106 @@ __global__ void __launch_bounds__(256) k_ov_finalize_q(ClaimState cs, uint32_t o
         } else {
             cs.link(i, ch, off);
         }
+    }
+}
+
+// ---- overlap probe by verdict words (tuning ov_queue 3) ------------------
+// As the verdict form of k_ov_probe_q, but each thread takes 64 suffixes at
+// once: the verdict word of the offset AND NOT the word of the linked-suffix
+// bitmap (cs.slbits, set by ClaimState::link) leaves the suffixes that
+// search their bucket, about one in twenty.  Workgroup b owns verdict words
+// [b*wper, b*wper + wper) (suffixes b*wper*64 ...), OVW_TPW threads to a
+// word (64 / OVW_TPW suffixes each), pushes their set bits into the LDS
+// ring at most OVW_TAKE per thread and round, and runs the candidate step
+// on full batches of 256, as k_ov_probe_q.  Its proposers go to its own
+// segment of cand (per = wper * 64 entries), which k_ov_finalize_q walks.
+constexpr uint32_t OVW_TAKE = 4;
+constexpr uint32_t OVW_TPW = 4;
+constexpr uint32_t OVW_TILE = 256 / OVW_TPW;  // words per round of the workgroup
+static_assert(OVQ_CAP >= 256 * OVW_TAKE + 512, "overlap queue capacity");
+
+template <uint32_t WM>
+__global__ void __launch_bounds__(256) k_ov_probe_w(ClaimState cs, const uint32_t* __restrict__ kr,
+                                                    const PrefixSlot* __restrict__ slots, uint32_t k, uint32_t kip,
+                                                    uint32_t mask, uint32_t off, uint32_t wper,
+                                                    const uint64_t* __restrict__ vb, uint64_t nw,
+                                                    uint32_t* __restrict__ cand, uint32_t* __restrict__ ncand,
+                                                    uint32_t* __restrict__ passed) {
+    __shared__ uint32_t qi[OVQ_CAP];
+    __shared__ uint32_t qtail, nout;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cs.res_n = *cs.pool_n = 0;
+    const uint32_t ns = *cs.d_ns;
+    const uint64_t wb = (uint64_t)blockIdx.x * wper;
+    const uint64_t we = min<uint64_t>(wb + wper, nw);
+    const uint64_t beg = wb * 64;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
+    const uint64_t* vrow = vb + (uint64_t)(off - 1) * nw;
+    if (threadIdx.x == 0) qtail = nout = 0;
+    __syncthreads();
+
+    auto candidate = [&](bool valid, uint32_t q) {
+        uint32_t ch = INV, i = 0;
+        if (valid) {
+            i = qi[q & (OVQ_CAP - 1)];
+            uint32_t g;
+            ch = ov_candidate_r<WM>(cs, kr, slots, ns, i, k, kip, mask, off, &g);
+            if (ch != INV) {
+                cs.choice[i] = ch;
+                cs.grp[i] = g;
+                const uint32_t old = atomicCAS(&cs.owner[ch], INV, i);
+                if (old != INV) cs.contended[g] = 1;
+            }
+        }
+        const uint64_t bal = __ballot(ch != INV);
+        if (bal) {
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(&nout, (uint32_t)__popcll(bal));
+            at = __shfl(at, 0);
+            if (ch != INV) cand[beg + at + (uint32_t)__popcll(bal & lt)] = i;
+        }
+    };
+
+    uint32_t head = 0;
+    constexpr uint32_t SB = 64 / OVW_TPW;  // suffixes per thread
+    const uint32_t sub = threadIdx.x % OVW_TPW;
+    for (uint64_t w0 = wb; w0 < we; w0 += OVW_TILE) {
+        const uint64_t t = w0 + threadIdx.x / OVW_TPW;
+        // verdict bits of suffixes past the list's end are clear
+        uint64_t bits = t < we ? (vrow[t] & ~cs.slbits[t]) : 0ull;
+        bits = (bits >> (SB * sub)) & (SB == 64 ? ~0ull : ((1ull << SB) - 1));
+        const uint32_t i0 = (uint32_t)(t * 64 + SB * sub);
+        while (__syncthreads_or(bits != 0)) {
+            const uint32_t take = min((uint32_t)__popcll(bits), OVW_TAKE);
+            uint32_t inc = take;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t u = __shfl_up(inc, d, 64);
+                if (lane >= (uint32_t)d) inc += u;
+            }
+            const uint32_t tot = __shfl(inc, 63);
+            uint32_t qb = 0;
+            if (lane == 0 && tot) qb = atomicAdd(&qtail, tot);
+            uint32_t q = __shfl(qb, 0) + inc - take;
+            for (uint32_t j = 0; j < take; ++j) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(bits);
+                bits &= bits - 1;
+                qi[q++ & (OVQ_CAP - 1)] = i0 + b;
+            }
+            __syncthreads();
+            const uint32_t tl = qtail;
+            while (tl - head >= 256) {
+                candidate(true, head + threadIdx.x);
+                head += 256;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t tl = qtail;
+    if (tl > head) candidate(head + threadIdx.x < tl, head + threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ncand[blockIdx.x] = nout;
+        if (passed) passed[blockIdx.x] += tl;
     }
 }
 

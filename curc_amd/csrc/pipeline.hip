@@ -106,8 +106,9 @@ enum Slot {
     S_D_PE, S_D_IDPOS, S_D_PV, S_D_PX, S_D_HQV, S_D_HQX, S_D_BV, S_D_BX, S_D_K0, S_D_K1, S_D_V0, S_D_V1,
     S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2, S_OVPASS, S_WORK, S_FMNF, S_FMVB, S_FMFILT, S_FMLEAD, S_FMGC, S_FMGM,
     S_NMASK,
+    S_OVLIST,
     S_LAST_NAMED,
-    S_PRIM = 232  // primitives use slots 232.. (scan recursion, sort hist, compaction)
+    S_PRIM = 236  // primitives use slots 236.. (scan recursion, sort hist, compaction)
 };
 
 // Device counters (u32) in slot S_CNT.
@@ -172,7 +173,8 @@ struct Tuning {
     uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
     uint32_t fm_fused = 4096;    // exact-matching lists up to this bound run in one fused launch (0: never)
     uint32_t map_both = 2;       // both strand passes in one launch on a two-strand index of 128-byte lines (1) or 32-byte slots (2)
-    uint32_t ov_queue = 2;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts
+    uint32_t ov_queue = 3;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts,
+                                 // 3 verdict words against a linked-suffix bitmap
     uint32_t workers = 4;        // streams of curc_batch_run when the caller passes 0
     uint32_t fm_static = 1;      // exact matching over lists kept in place (1) or compacted per offset (0)
     uint32_t fq_host = 8;        // host threads that parse FASTQ records before the copy (0: parse on the device)
@@ -907,7 +909,7 @@ void parse_tuning(const char* text, Tuning& t) {
             const std::string k = kv.substr(0, e);
             const uint32_t v = (uint32_t)std::strtoul(kv.c_str() + e + 1, nullptr, 10);
             if (k == "fm_fused") t.fm_fused = v;
-            if (k == "ov_queue" && v <= 2) t.ov_queue = v;
+            if (k == "ov_queue" && v <= 3) t.ov_queue = v;
             if (k == "map_both" && v <= 2) t.map_both = v;
             if (k == "fm_static" && v <= 1) t.fm_static = v;
             if (k == "fq_host" && v <= CURC_FQ_HOST_MAX) t.fq_host = v;
@@ -1710,7 +1712,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         const uint32_t fmask = (uint32_t)(fbits - 1);
         const uint64_t fwords = fbits / 32;
         uint32_t* pfilt = S.as<uint32_t>(S_PFILT, fwords);
-        const bool verd = c->tune.ov_queue == 2 && W <= 8 && P.k >= OVM + 4;
+        const bool verd = c->tune.ov_queue >= 2 && W <= 8 && P.k >= OVM + 4;
         const uint32_t bmask = (uint32_t)(fbits / (32 * OVB_WORDS) - 1);
         {
             Region r(c, "prefix_index");
@@ -1789,10 +1791,26 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     CURC_OV_VERDICTS(8);
 #undef CURC_OV_VERDICTS
             }
+            // probe by verdict words (tuning ov_queue 3): workgroup b owns
+            // verdict words [b*wper, b*wper + wper); linked suffixes are
+            // tracked in the bitmap slbits
+            const bool wscan = c->tune.ov_queue == 3 && vb;
+            uint64_t* slbits = nullptr;
+            uint32_t wper = 0, gw = 0;
+            if (wscan) {
+                slbits = S.as<uint64_t>(S_OVLIST, vnw + 1);
+                fill<uint64_t>(c, slbits, vnw + 1, 0ull);
+                const uint64_t want = (vnw + c->tune.ov_grid - 1) / c->tune.ov_grid;
+                wper = (uint32_t)std::max<uint64_t>(OVW_TILE, (want + OVW_TILE - 1) / OVW_TILE * OVW_TILE);
+                gw = (uint32_t)std::max<uint64_t>(1, (vnw + wper - 1) / wper);
+            }
+            // work counters of the loop (probes, bucket searches)
+            uint64_t* ovw = S.as<uint64_t>(S_WORK, 4);
+            fill<uint64_t>(c, ovw, 2, 0ull);
             // queued probe: workgroup b owns suffixes [b*per, b*per + per),
             // per a whole number of filter tiles, and lists its proposers in
             // that segment of qcand
-            const bool queued = c->tune.ov_queue != 0;
+            const bool queued = c->tune.ov_queue != 0 && !wscan;
             uint32_t per = 0, gq = 0;
             uint32_t *qcand = nullptr, *qn = nullptr;
             if (queued) {
@@ -1804,12 +1822,35 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             }
             uint32_t* passed = queued ? S.as<uint32_t>(S_OVPASS, (uint64_t)gq) : nullptr;
             if (passed) fill<uint32_t>(c, passed, gq, 0u);
+            if (wscan) {
+                qcand = S.as<uint32_t>(S_OVCAND, (uint64_t)gw * wper * 64 + 1);
+                qn = S.as<uint32_t>(S_OVNC, (uint64_t)gw);
+                passed = S.as<uint32_t>(S_OVPASS, (uint64_t)gw);
+                fill<uint32_t>(c, passed, gw, 0u);
+            }
             auto loop = [&]() {
                 for (uint32_t o = 1; o < P.max_off; ++o) {
                     ClaimState cs{packed, W, L, sl[0], cnt + C_NSUF, va, vk, cnt + C_NPRE, next, prev, off,
                                   choice, grp, owner, cont, head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL,
                                   nullptr, slinked, nullptr};
-                    if (queued && vb) {
+                    if (wscan) {
+                        cs.slbits = slbits;
+#define CURC_OV_PROBE_W(WM)                                                                                    \
+    hipLaunchKernelGGL((k_ov_probe_w<WM>), dim3(gw), dim3(256), 0, st, cs, (const uint32_t*)kr,                  \
+                       (const PrefixSlot*)pslots, P.k, P.kip, mask, o, wper, (const uint64_t*)vb, vnw, qcand, qn,   \
+                       passed)
+                        if (W <= 3)
+                            CURC_OV_PROBE_W(3);
+                        else if (W <= 4)
+                            CURC_OV_PROBE_W(4);
+                        else if (W <= 5)
+                            CURC_OV_PROBE_W(5);
+                        else
+                            CURC_OV_PROBE_W(8);
+#undef CURC_OV_PROBE_W
+                        hipLaunchKernelGGL(k_ov_finalize_q, dim3(gw), dim3(256), 0, st, cs, o, wper * 64,
+                                           (const uint32_t*)qcand, (const uint32_t*)qn);
+                    } else if (queued && vb) {
 #define CURC_OV_PROBE_V(WM)                                                                                    \
     hipLaunchKernelGGL((k_ov_probe_q<true, WM>), dim3(gq), dim3(256), 0, st, cs, (const uint32_t*)kr,            \
                        (const PrefixSlot*)pslots, (const uint32_t*)nullptr, fmask, mask, (const uint64_t*)sw, P.k, \
@@ -1855,13 +1896,16 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             key.v[10] = (uint64_t)off;
             key.v[11] = (uint64_t)cnt;
             key.v[12] = ((uint64_t)queued << 32) | (queued ? per : g);
-            key.v[13] = vb != nullptr;
+            key.v[13] = (uint64_t)vb | (wscan ? 1u : 0u);
+            key.v[14] = (uint64_t)slbits;
+            key.v[15] = ((uint64_t)gw << 32) | wper;
+            key.v[16] = (uint64_t)qcand;
+            key.v[17] = (uint64_t)qn;
+            key.v[18] = (uint64_t)passed;
             c->graphs.run(key, S, st, loop);
             // work of the loop: probes (suffix x offset) and bucket searches
             // (probes that passed the k-mer filter); read back with the
             // partition's counts
-            uint64_t* ovw = S.as<uint64_t>(S_WORK, 4);
-            fill<uint64_t>(c, ovw, 2, 0ull);
             if (nsuf)
                 hipLaunchKernelGGL(k_ov_probe_count, dim3(grid_for(nsuf)), dim3(256), 0, st, (const uint32_t*)sl[0],
                                    nsuf, (const uint16_t*)off, P.max_off, ovw);
@@ -3520,7 +3564,7 @@ int curc_ctx_set_tuning(curc_ctx* c, const curc_tuning* t) {
     if (!c || !t) return CURC_EINVAL;
     if (!t->ov_grid || !t->fm_grid || !t->map_grid || !t->workers || t->workers > CURC_BATCH_MAX_WORKERS ||
         (t->radix_items != 8 && t->radix_items != 16 && t->radix_items != 32) || t->filter_bits < 8 ||
-        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 2 || t->map_both > 2 ||
+        t->filter_bits > 128 || (t->filter_bits & (t->filter_bits - 1)) || t->ov_queue > 3 || t->map_both > 2 ||
         t->fm_static > 1 || t->fq_host > CURC_FQ_HOST_MAX)
         return fail(c, CURC_EINVAL, "tuning value out of range");
     c->tune.ov_grid = t->ov_grid;

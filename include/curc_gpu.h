@@ -428,7 +428,10 @@ typedef struct curc_tuning {
                              LDS and searches their buckets in full batches; 2 as 1, with the
                              filter minimizer-blocked and its verdicts for all offsets computed in
                              one pass beforehand (reads of at most 256 bases; others use 1);
-                             0: one suffix per thread throughout (default 2) */
+                             3: as 2, each thread taking a quarter of a 64-suffix verdict word
+                             against a bitmap of the suffixes linked so far, so that only the
+                             suffixes that search their bucket are touched;
+                             0: one suffix per thread throughout (default 3) */
     uint32_t map_both;    /* 1: both strand passes of the mapping in one launch, on an index that
                              holds a bucket's forward and reverse-complement slots in one 128-byte
                              line; 2: as 1, on an index of one 32-byte slot per bucket (positions

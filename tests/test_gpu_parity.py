@@ -64,15 +64,17 @@ def test_read_lengths(gpu_ctx, L):
     run_both(gpu_ctx, ca.synth_reads(cfg)[1])
 
 
+@pytest.mark.parametrize("form", [2, 3])
 @pytest.mark.parametrize("L", [60, 75, 150, 250])
-def test_overlap_filter_verdicts(L):
-    """Overlap probe form 2: minimizer-blocked k-mer filter, verdicts for all
-    offsets computed in one pass.  Same links as the oracle."""
+def test_overlap_filter_verdicts(L, form):
+    """Overlap probe forms 2 and 3: minimizer-blocked k-mer filter, verdicts
+    for all offsets computed in one pass; form 3 turns them into one list per
+    offset of the suffixes that pass.  Same links as the oracle."""
     cfg = ca.synth_config(seed=600 + L, genome_len=40_000, gc=0.46, repeat_frac=0.2, rep_min=200, rep_max=900,
                           rep_families=4, div_min=0.0, div_max=0.02, n_reads=7_000, L=L,
                           sub_first=0.002, sub_last=0.01, n_rate=0.0003)
     with ca.Context(0) as ctx:
-        ctx.set_tuning(ov_queue=2)
+        ctx.set_tuning(ov_queue=form)
         run_both(ctx, ca.synth_reads(cfg)[1])
 
 

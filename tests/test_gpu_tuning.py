@@ -33,6 +33,8 @@ TUNINGS = [
     dict(map_grid=3, map_both=1),
     dict(map_grid=2, map_both=2),
     dict(map_grid=2, map_both=0),
+    dict(ov_grid=3, filter_bits=8, ov_queue=3),
+    dict(ov_grid=1 << 20, filter_bits=128, ov_queue=3, map_both=1),
 ]
 
 
@@ -53,7 +55,7 @@ def test_results_do_not_depend_on_tuning(L):
 def test_invalid_tuning_is_refused(gpu_ctx):
     before = gpu_ctx.get_tuning()
     for bad in (dict(radix_items=12), dict(map_grid=0), dict(workers=5), dict(filter_bits=24), dict(filter_bits=4),
-                dict(ov_queue=3), dict(map_both=3), dict(fm_static=2)):
+                dict(ov_queue=4), dict(map_both=3), dict(fq_host=65), dict(fm_static=2)):
         with pytest.raises(ca.CurcError):
             gpu_ctx.set_tuning(**bad)
     assert gpu_ctx.get_tuning() == before
