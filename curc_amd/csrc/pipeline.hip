@@ -1910,8 +1910,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 hipLaunchKernelGGL(k_ov_probe_count, dim3(grid_for(nsuf)), dim3(256), 0, st, (const uint32_t*)sl[0],
                                    nsuf, (const uint16_t*)off, P.max_off, ovw);
             if (passed)
-                hipLaunchKernelGGL(k_sum_u32, dim3(grid_for(gq)), dim3(256), 0, st, (const uint32_t*)passed, gq,
-                                   ovw + 1);
+                hipLaunchKernelGGL(k_sum_u32, dim3(grid_for(wscan ? gw : gq)), dim3(256), 0, st,
+                                   (const uint32_t*)passed, wscan ? gw : gq, ovw + 1);
             CURC_HIP(hipMemcpyAsync(c->ov_work, ovw, 16, hipMemcpyDeviceToHost, st));
             ov_counted = true;
         }

@@ -78,6 +78,27 @@ def test_overlap_filter_verdicts(L, form):
         run_both(ctx, ca.synth_reads(cfg)[1])
 
 
+def test_overlap_forms_count_the_same_work():
+    """Overlap probe forms 2 and 3 search the same suffixes at every offset
+    (verdict passed, not linked yet): their work counts (probes and bucket
+    searches, in the stage's algorithmic bytes) are equal."""
+    cfg = ca.synth_config(seed=611, genome_len=300_000, gc=0.45, n_reads=60_000, L=150, sub_first=0.002,
+                          sub_last=0.01, n_rate=0.0003)
+    reads = ca.synth_reads(cfg)[1]
+    got = {}
+    with ca.Context(0) as ctx:
+        for form in (2, 3):
+            ctx.set_tuning(ov_queue=form)
+            ctx.reset_stats()
+            ctx.set_timing(True)
+            ctx.run_block(reads).free()
+            ctx.set_timing(False)
+            st = {s["name"]: s for s in ctx.stats()}
+            got[form] = (st["overlap"]["items"], st["overlap"]["alg_bytes"])
+    print(got, flush=True)
+    assert got[2] == got[3] and got[2][1] > 32 * got[2][0]
+
+
 @pytest.mark.parametrize("case", ["plain", "repeats"])
 def test_mapping_both_strands_one_launch(case):
     """Tuning map_both=1: both strand passes in one launch on the two-strand index of
