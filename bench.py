@@ -94,9 +94,10 @@ def measured_traffic():
 
 
 def dominant_roofline(stats, steps, default_workload=True):
-    """Roofline entry for the stage with the largest event time whose region
-    brackets single kernel launches.  traffic (rocprof HBM bytes per launch)
-    is given for the default workload only, which the profile covers."""
+    """Roofline entry for the mapping stage, whose event pair brackets its
+    launches (reads without N, then the N-reads) and nothing else.  traffic
+    (rocprof fetched + written bytes of those launches) is given for the
+    default workload only, which the profile covers."""
     single = [s for s in stats if s["name"] in ("mapping",)]
     if not single:
         return None
