@@ -218,6 +218,8 @@ struct curc_ctx {
     // beside the LQ stage of a single block; created on first use
     curc_ctx* helper = nullptr;
     hipEvent_t nev = nullptr;  // end of mapping on st, for the helper to wait on
+    // start and end of the N-reads' mapping launch on the helper's stream
+    hipEvent_t mev[2] = {nullptr, nullptr};
     Scratch S;
     std::string err;
     bool timing = false;
@@ -832,15 +834,17 @@ void launch_map_both_w(uint32_t W, dim3 grid, hipStream_t st, const MapArgs& a) 
 }
 // the other reads, then the N-reads (grid capped by cap)
 template <bool CS>
-void launch_map_both_t(uint32_t W, uint32_t cap, hipStream_t st, const MapArgs& a) {
+// The N-reads' launch goes to stream st_n (st, or a stream that runs it
+// beside the other reads' launch), first.
+void launch_map_both_t(uint32_t W, uint32_t cap, hipStream_t st, const MapArgs& a, hipStream_t st_n) {
+    if (a.n_N) launch_map_both_w<CS, true>(W, dim3(grid_for(a.n_N, cap)), st_n, a);
     if (a.n_un) launch_map_both_w<CS, false>(W, dim3(grid_for(a.n_un, cap)), st, a);
-    if (a.n_N) launch_map_both_w<CS, true>(W, dim3(grid_for(a.n_N, cap)), st, a);
 }
-void launch_map_both(uint32_t W, uint32_t cap, hipStream_t st, const MapArgs& a) {
+void launch_map_both(uint32_t W, uint32_t cap, hipStream_t st, const MapArgs& a, hipStream_t st_n) {
     if (a.cs)
-        launch_map_both_t<true>(W, cap, st, a);
+        launch_map_both_t<true>(W, cap, st, a, st_n);
     else
-        launch_map_both_t<false>(W, cap, st, a);
+        launch_map_both_t<false>(W, cap, st, a, st_n);
 }
 
 // k_map_fast for the read's word count W <= 8 (register capacity 3, 4, 5 or 8).
@@ -2120,7 +2124,21 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 fill<unsigned long long>(c, work, 8, 0ull);
                 {
                     Region r(c, "mapping");
-                    launch_map_both(W, c->tune.map_grid, st, a);
+                    // the N-reads' launch (a few percent of the reads, each
+                    // probing long) on the helper context's idle stream beside
+                    // the other reads' launch, instead of after it
+                    curc_ctx* hx = (nN && c->side_copies) ? helper_ctx(c) : nullptr;
+                    if (hx) {
+                        for (auto& e : c->mev)
+                            if (!e) CURC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                        CURC_HIP(hipEventRecord(c->mev[0], st));
+                        CURC_HIP(hipStreamWaitEvent(hx->st, c->mev[0], 0));
+                        launch_map_both(W, c->tune.map_grid, st, a, hx->st);
+                        CURC_HIP(hipEventRecord(c->mev[1], hx->st));
+                        CURC_HIP(hipStreamWaitEvent(st, c->mev[1], 0));
+                    } else {
+                        launch_map_both(W, c->tune.map_grid, st, a, st);
+                    }
                 }
                 // as for the separate passes, over both strands
                 map_stats("mapping", work);
@@ -2863,6 +2881,8 @@ void curc_ctx_destroy(curc_ctx* c) {
     hipEventDestroy(c->cev[0]);
     hipEventDestroy(c->cev[1]);
     hipEventDestroy(c->nev);
+    for (auto e : c->mev)
+        if (e) hipEventDestroy(e);
     if (c->cst) hipStreamDestroy(c->cst);
     if (c->ust) {
         stream_wait(c->ust);
