@@ -2786,6 +2786,34 @@ struct SeenCache32 {
 // complement of such a character is again outside A, C, G, T, and the mask
 // is reversed with the read).  So an N-read's counts, results and work counters are
 // those of the character loop, at the cost of the other reads.
+// Early end of a strand pass that cannot change its result.  The pass keeps
+// the first window of least count, and replaces it only with a window of
+// fewer mismatches.  Let B be the count to beat: a window q with at most
+// B - 1 mismatches matches the read exactly in every k-mer window
+// [j, j + MAP_K) that holds none of them.  Probe j (j = 2t) can list q only
+// if q + j (forward; q + L - MAP_K - j on the reverse strand) is a multiple
+// of MAP_REF_STEP, which fixes t mod 5: q's probes are those of one class,
+// 10 bases apart, so one mismatch spoils at most ceil(MAP_K / 10) = 3 of
+// them.  A probe whose bucket holds fewer than MAP_BUCKET positions lists
+// every indexed position of its k-mer's hash.  So once every class has had
+// more than 3 (B - 1) such complete probes, every q that could beat B has
+// already been a candidate (and was counted then, against a bound at least
+// as large): the remaining probes cannot change the pass's result.  clean
+// holds the complete probes per class, 4 bits each.  Only the work counts
+// change (fewer probes and candidates than the exhaustive loop).
+constexpr uint32_t MAP_SETTLE_KILL = (MAP_K + 2 * MAP_REF_STEP - 1) / (2 * MAP_REF_STEP);  // 3
+constexpr uint32_t MAP_SETTLE_MIN = 5 * (MAP_SETTLE_KILL + 1);  // probes before the first class can settle
+static_assert(MAP_READ_STEP == 2 && MAP_REF_STEP == 5, "probe classes assume steps 2 and 5");
+__device__ __forceinline__ bool map_settled(uint32_t clean, uint32_t B) {
+    if (B <= 1) return true;
+    const uint32_t need = MAP_SETTLE_KILL * (B - 1) + 1;
+    if (need > 15) return false;
+    uint32_t m = clean & 15u;
+#pragma unroll
+    for (uint32_t c = 1; c < 5; ++c) m = min(m, (clean >> (4 * c)) & 15u);
+    return m >= need;
+}
+
 template <uint32_t WM, bool CS, bool NR>
 __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
     constexpr uint32_t G = 1;  // probes whose slot heads are requested together
@@ -2887,7 +2915,10 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
         };
         uint32_t F = last, Rb = last;  // forward best, reverse best (below F when it counts)
         uint64_t wF = w, wR = w;
-        bool doneR = false;
+        bool doneR = false, tgtR = false;  // reverse pass over; over because it reached MAP_TARGET
+        bool doneF = false;                // forward pass over above MAP_TARGET (map_settled)
+        uint32_t cleanF = 0, cleanR = 0;   // complete buckets probed per probe class (map_settled)
+        uint32_t cls = 0;                  // class of probe t: t mod 5
         uint32_t pF = 0, cF = 0, pR = 0, cR = 0;  // work of each pass
         SeenCache32 seenF, seenR;  // positions < pg_len < 2^32
         for (uint32_t t0 = 0; t0 < np; t0 += G) {
@@ -2916,13 +2947,14 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                 const uint32_t j = MAP_READ_STEP * t;
                 const uint32_t* sl = a.bk + (uint64_t)hh[u] * MAP_SLOT2;
                 // forward pass, probe t
-                ++pF;
-                {
+                if (!doneF) {
+                    ++pF;
                     uint32_t nb;
                     if constexpr (CS)
                         nb = bb[u].nf;
                     else
                         nb = hf[u].x;
+                    if (nb < MAP_BUCKET) cleanF += 1u << (4 * cls);
                     const uint64_t ls = (uint64_t)j;
                     for (uint32_t p = 0; p < nb; ++p) {
                         uint32_t pv;
@@ -2952,6 +2984,7 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                         nb = bb[u].nr;
                     else
                         nb = hr[u].x;
+                    if (nb < MAP_BUCKET) cleanR += 1u << (4 * cls);
                     const uint64_t ls = (uint64_t)(L - MAP_K - j);
                     for (uint32_t p = 0; p < nb; ++p) {
                         uint32_t pv;
@@ -2974,13 +3007,21 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                             wR = flagN | 2ull | (mp << 2);
                             Rb = c;
                             if (Rb <= MAP_TARGET) {
-                                doneR = true;
+                                doneR = tgtR = true;
                                 break;
                             }
                         }
                     }
                 }
+                cls = cls == 4 ? 0u : cls + 1u;
+                // a pass whose best cannot be beaten by a window it has not
+                // seen ends here (map_settled)
+                if (t + 1 >= MAP_SETTLE_MIN) {
+                    if (!doneF && map_settled(cleanF, F)) doneF = true;
+                    if (!doneR && map_settled(cleanR, min(Rb, F))) doneR = true;
+                }
             }
+            if (doneF && doneR) break;
         }
         // forward pass ended above MAP_TARGET: the reverse pass counts
         n_act += 2;
@@ -2988,7 +3029,7 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
         n_cand += cF + cR;
         n_open_probe += pF;
         n_open_cand += cF;
-        if (doneR) {
+        if (tgtR) {
             n_first += (pR == 1) ? 1u : 0u;
         } else {
             n_open_probe += pR;

@@ -5,6 +5,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -589,10 +590,29 @@ void build_map_index(MapIndex &ix, const uint64_t *ref, uint64_t start, uint64_t
     }
 }
 
-// compress.cu:411-521 (mapping<strand>) for one index half.
+// Not part of the reference: the early end of a strand pass that the
+// library's mapping kernel uses (a pass stops once every class of probes,
+// t mod 5, has had more than 3 (last - 1) probes of buckets holding fewer
+// than MAP_BUCKET positions; see k_map_both).  Off by default; the tests turn
+// it on to show that it never changes a result of the loop below.
+// g_map_settle_kill: the probes of one class a mismatch is taken to spoil
+// (3 is the rule; 0 turns the early end off; smaller values are mutants).
+uint32_t g_map_settle_kill = 0;
+std::atomic<uint64_t> g_map_settled{0};  // passes ended early
+
+bool map_settled(const uint32_t clean[5], uint32_t last) {
+    if (last <= 1) return true;
+    const uint32_t need = g_map_settle_kill * (last - 1) + 1;
+    for (int c = 0; c < 5; ++c)
+        if (clean[c] < need) return false;
+    return true;
+}
+
+// compress.cu:411-521 (mapping<strand>) for one index half.  settle: apply
+// the early end above (one index half only).
 void map_reads(const MapIndex &ix, uint64_t start, bool rc, const uint64_t *ref, uint64_t ref_len,
                uint32_t L, uint32_t W, const uint64_t *gathered, uint64_t n_un, const char *nascii,
-               uint64_t n_N, std::vector<uint8_t> &last, std::vector<orc_rec> &m) {
+               uint64_t n_N, std::vector<uint8_t> &last, std::vector<orc_rec> &m, bool settle = false) {
     uint32_t mask = ix.H - 1;
     ORC_PARALLEL_FOR
     for (int64_t ii = 0; ii < (int64_t)(n_un + n_N); ++ii) {
@@ -601,10 +621,16 @@ void map_reads(const MapIndex &ix, uint64_t start, bool rc, const uint64_t *ref,
         bool isN = i >= n_un;
         const uint64_t *rd = isN ? nullptr : gathered + i * W;
         const char *ra = isN ? nascii + (i - n_un) * L : nullptr;
+        uint32_t clean[5] = {0, 0, 0, 0, 0};
         for (uint32_t j = 0; j + MAP_K <= L; j += MAP_READ_STEP) {
+            if (settle && j / MAP_READ_STEP >= 5 * (g_map_settle_kill + 1) && map_settled(clean, last[i])) {
+                g_map_settled.fetch_add(1, std::memory_order_relaxed);
+                break;
+            }
             uint64_t key = isN ? kmer_ascii(ra + j, MAP_K) : xw(rd, 0, j, MAP_K);
             uint32_t h = (uint32_t)(murmur(key) & mask);
             uint32_t b = ix.kr[h], e = ix.kr[h + 1];
+            if (e - b < MAP_BUCKET) clean[(j / MAP_READ_STEP) % 5]++;
             if (b == e) continue;
             e = std::min(b + MAP_BUCKET, e);
             for (uint32_t p = b; p < e; ++p) {
@@ -670,6 +696,16 @@ struct orc_block {
 };
 
 extern "C" {
+
+// The early end of strand passes in the mapping (see map_settled) for the
+// blocks run after this call: kill = probes of a class one mismatch spoils
+// (3: the library's rule; 0: off, the reference's exhaustive loop).
+void orc_set_map_settle(int kill) {
+    g_map_settle_kill = kill > 0 ? (uint32_t)kill : 0u;
+    g_map_settled.store(0);
+}
+// Strand passes ended early since the last orc_set_map_settle.
+uint64_t orc_map_settled_count() { return g_map_settled.load(); }
 
 int orc_make_params(uint32_t L, orc_params *p) {
     // preprocess.cpp:296-313
@@ -968,7 +1004,8 @@ orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int st
                 } else {
                     MapIndex ix;
                     build_map_index(ix, ref, 0, index_count, rc);
-                    map_reads(ix, 0, rc, ref, ref_len, L, W, g.data(), n_un, B->Nascii.data(), n_N, last, m);
+                    map_reads(ix, 0, rc, ref, ref_len, L, W, g.data(), n_un, B->Nascii.data(), n_N, last, m,
+                              g_map_settle_kill != 0);
                 }
             }
         }

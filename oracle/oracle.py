@@ -67,6 +67,8 @@ def lib(kind="st"):
         "orc_block_run": (vp, [vp, u64, u32, i32]),
         "orc_block_get": (u64, [vp, i32, ctypes.POINTER(vp)]),
         "orc_block_free": (None, [vp]),
+        "orc_set_map_settle": (None, [i32]),
+        "orc_map_settled_count": (u64, []),
         "orc_block_streams": (i32, [vp, i32, i32]),
         "orc_marush36": (u32, [vp]),
         "orc_ref_match": (vp, [vp, u64, vp, u64, i32, i32]),

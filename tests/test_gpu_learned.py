@@ -102,9 +102,10 @@ def test_ranges_on_the_compact_index():
     """A block large enough for the bucket-inline index (6 Mbp genome, 1.2M
     reads): the probe ranges run on the compact two-strand index (tuning
     map_both 2) or on one strand's 64-byte slots (map_both 0).  Without
-    stops both equal the exhaustive mapping and walk the same probes and
-    candidates; with a stopping scorer both stop the same reads, since the
-    features do not depend on the index layout."""
+    stops both equal the plain mapping and walk the same probes and
+    candidates as its exhaustive per-strand form (map_both 0); with a
+    stopping scorer both stop the same reads, since the features do not
+    depend on the index layout."""
     cfg = ca.config_scale(seed=83, n_reads=1_200_000, genome_len=6_000_000)
     r = ca.synth_reads(cfg)[1]
 
@@ -119,7 +120,9 @@ def test_ranges_on_the_compact_index():
     with ca.Context(0) as ctx:
         ctx.set_trace(every=64, max_records=1 << 20)
         print("compact index: plain", flush=True)
-        plain, wp = work(ctx)
+        plain = work(ctx)[0]
+        ctx.set_tuning(map_both=0)
+        wp = work(ctx)[1]
         runs = {}
         for mb in (2, 0):
             ctx.set_tuning(map_both=mb)

@@ -40,7 +40,13 @@ def test_trained_scorer_gains(gpu_ctx):
     cfg = ca.config_scale(seed=31, n_reads=600_000, genome_len=3_000_000)
     _, reads = ca.synth_reads(cfg)
     print("ablation: plain", flush=True)
+    # the method the scorer stops early: the exhaustive loop of each strand
+    # pass (the one-launch plain kernel also ends passes early, by a rule
+    # that never changes a result)
+    before = gpu_ctx.get_tuning()
+    gpu_ctx.set_tuning(map_both=0)
     plain = run(gpu_ctx, reads, False)
+    gpu_ctx.set_tuning(**before)
     params, thr = train_scorer.load()
     net = scorer.Mlp(gpu_ctx, params)
     gpu_ctx.set_scorer(net, thr)
@@ -82,7 +88,10 @@ def test_trained_scorer_other_read_lengths(gpu_ctx, L, sub, genome):
                           sub_last=sub, n_rate=0.0002)
     _, reads = ca.synth_reads(cfg)
     print(f"ablation: L={L}, subst {sub}, genome {genome}", flush=True)
+    before = gpu_ctx.get_tuning()
+    gpu_ctx.set_tuning(map_both=0)  # the exhaustive loop, as above
     plain = run(gpu_ctx, reads, False)
+    gpu_ctx.set_tuning(**before)
     params, thr = train_scorer.load()
     net = scorer.Mlp(gpu_ctx, params)
     gpu_ctx.set_scorer(net, thr)
