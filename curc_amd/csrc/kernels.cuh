@@ -3014,10 +3014,14 @@ __global__ void __launch_bounds__(256, 7) k_map_both(MapArgs a) {
                     }
                 }
                 cls = cls == 4 ? 0u : cls + 1u;
-                // a pass whose best cannot be beaten by a window it has not
-                // seen ends here (map_settled)
+                // a pass whose answer no window it has not seen can change
+                // ends here (map_settled).  Forward: a window must beat F,
+                // and reach the reverse best Rb (a tie goes to the forward
+                // pass) unless it reaches MAP_TARGET, which ends the read
+                // before the reverse pass counts.  Reverse: it must beat
+                // min(Rb, F).
                 if (t + 1 >= MAP_SETTLE_MIN) {
-                    if (!doneF && map_settled(cleanF, F)) doneF = true;
+                    if (!doneF && map_settled(cleanF, min(F, max(Rb + 1, MAP_TARGET + 1)))) doneF = true;
                     if (!doneR && map_settled(cleanR, min(Rb, F))) doneR = true;
                 }
             }

@@ -665,6 +665,112 @@ void map_reads(const MapIndex &ix, uint64_t start, bool rc, const uint64_t *ref,
     }
 }
 
+// Not part of the reference: the two strand passes of a read interleaved
+// probe by probe, as the library's k_map_both runs them (the reverse pass
+// accepts a count below min(its best, the forward best so far) and wins when
+// below the final forward best; a forward pass reaching MAP_TARGET ends the
+// read), with the early end of each pass when g_map_settle_kill > 0: the
+// forward pass with B = min(forward best, max(reverse best + 1,
+// MAP_TARGET + 1)) (a forward window that does not reach the reverse best
+// cannot change the answer, unless it reaches MAP_TARGET: then the forward
+// pass ends the read and the reverse pass never counts), the reverse pass
+// with B = min(reverse best, forward best).  The tests compare it with
+// map_reads over both strands.  g_map_both_tie: a mutant that drops the + 1.
+bool g_map_both = false;
+bool g_map_both_tie = false;
+
+void map_reads_both(const MapIndex &ixf, const MapIndex &ixr, const uint64_t *ref, uint64_t ref_len, uint32_t L,
+                    uint32_t W, const uint64_t *gathered, uint64_t n_un, const char *nascii, uint64_t n_N,
+                    std::vector<uint8_t> &last, std::vector<orc_rec> &m) {
+    const uint32_t np = (L - MAP_K) / MAP_READ_STEP + 1;
+    const uint32_t kill = g_map_settle_kill;
+    ORC_PARALLEL_FOR
+    for (int64_t ii = 0; ii < (int64_t)(n_un + n_N); ++ii) {
+        const uint64_t i = (uint64_t)ii;
+        if (last[i] <= MAP_TARGET) continue;
+        const bool isN = i >= n_un;
+        const uint64_t *rd = isN ? nullptr : gathered + i * W;
+        const char *ra = isN ? nascii + (i - n_un) * L : nullptr;
+        auto count = [&](bool rc, uint64_t mp, uint32_t bound) -> uint32_t {
+            if (!isN) return (uint32_t)(rc ? rc_mm(ref, mp, rd, L, bound) : fwd_mm(ref, mp, rd, L, bound));
+            uint32_t c = 0;
+            for (uint32_t o = 0; o < L; ++o) {
+                char rb = base_at_u(ref, mp + o);
+                char qb = rc ? rev_base(ra[L - 1 - o]) : ra[o];
+                if (rb != qb && ++c > bound) break;
+            }
+            return c;
+        };
+        uint32_t F = last[i], Rb = last[i];
+        uint64_t wF = m[i].word, wR = m[i].word;
+        bool doneF = false, doneR = false, fwdT = false;
+        uint32_t cleanF[5] = {0, 0, 0, 0, 0}, cleanR[5] = {0, 0, 0, 0, 0};
+        for (uint32_t t = 0; t < np && !(doneF && doneR); ++t) {
+            const uint32_t j = MAP_READ_STEP * t;
+            const uint64_t key = isN ? kmer_ascii(ra + j, MAP_K) : xw(rd, 0, j, MAP_K);
+            const uint32_t hv = (uint32_t)murmur(key);
+            if (!doneF) {
+                const uint32_t h = hv & (ixf.H - 1), b = ixf.kr[h], e = std::min(b + MAP_BUCKET, ixf.kr[h + 1]);
+                if (ixf.kr[h + 1] - b < MAP_BUCKET) cleanF[t % 5]++;
+                for (uint32_t p = b; p < e && !fwdT; ++p) {
+                    const uint64_t rpos = (uint64_t)ixf.pos[p] * MAP_REF_STEP;
+                    if (rpos < j) continue;
+                    const uint64_t mp = rpos - j;
+                    if (mp + L > ref_len) continue;
+                    const uint32_t c = count(false, mp, F - 1);
+                    if (c < F) {
+                        F = c;
+                        wF = rec_word(isN ? 1 : 0, 0, mp);
+                        if (F <= MAP_TARGET) fwdT = true;
+                    }
+                }
+                if (fwdT) break;
+            }
+            if (!doneR) {
+                const uint32_t h = hv & (ixr.H - 1), b = ixr.kr[h], e = std::min(b + MAP_BUCKET, ixr.kr[h + 1]);
+                if (ixr.kr[h + 1] - b < MAP_BUCKET) cleanR[t % 5]++;
+                const uint64_t ls = L - MAP_K - j;
+                for (uint32_t p = b; p < e; ++p) {
+                    const uint64_t rpos = (uint64_t)ixr.pos[p] * MAP_REF_STEP;
+                    if (rpos < ls) continue;
+                    const uint64_t mp = rpos - ls;
+                    if (mp + L > ref_len) continue;
+                    const uint32_t lim = std::min(Rb, F);
+                    const uint32_t c = count(true, mp, lim - 1);
+                    if (c < lim) {
+                        Rb = c;
+                        wR = rec_word(isN ? 1 : 0, 1, mp);
+                        if (Rb <= MAP_TARGET) {
+                            doneR = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (kill) {
+                const uint32_t bf = std::min(F, std::max(g_map_both_tie ? Rb : Rb + 1, MAP_TARGET + 1));
+                const uint32_t br = std::min(Rb, F);
+                const bool late = t + 1 >= 5 * (kill + 1);
+                if (!doneF && (bf <= 1 || (late && map_settled(cleanF, bf)))) {
+                    doneF = true;
+                    g_map_settled.fetch_add(1, std::memory_order_relaxed);
+                }
+                if (!doneR && (br <= 1 || (late && map_settled(cleanR, br)))) {
+                    doneR = true;
+                    g_map_settled.fetch_add(1, std::memory_order_relaxed);
+                }
+            }
+        }
+        if (!fwdT && Rb < F) {
+            last[i] = (uint8_t)Rb;
+            m[i].word = wR;
+        } else {
+            last[i] = (uint8_t)F;
+            m[i].word = wF;
+        }
+    }
+}
+
 }  // namespace
 
 struct orc_block {
@@ -706,6 +812,12 @@ void orc_set_map_settle(int kill) {
 }
 // Strand passes ended early since the last orc_set_map_settle.
 uint64_t orc_map_settled_count() { return g_map_settled.load(); }
+// Both strand passes interleaved (map_reads_both) instead of one after the
+// other; tie != 0 is the mutant without the + 1 of the forward bound.
+void orc_set_map_both(int on, int tie) {
+    g_map_both = on != 0;
+    g_map_both_tie = tie != 0;
+}
 
 int orc_make_params(uint32_t L, orc_params *p) {
     // preprocess.cpp:296-313
@@ -989,7 +1101,13 @@ orc_block *orc_block_run(const char *ascii, uint64_t n_reads, uint32_t L, int st
         const uint64_t *ref = B->hq_pg2.data();
         uint64_t index_count = 0;
         if (ref_len + 1 >= MAP_K) index_count = (ref_len - MAP_K + 1 + MAP_REF_STEP - 1) / MAP_REF_STEP;
-        if (L >= MAP_K) {
+        if (L >= MAP_K && g_map_both && index_count <= (1ULL << 30)) {
+            // both strands interleaved (test switch, see map_reads_both)
+            MapIndex ixf, ixr;
+            build_map_index(ixf, ref, 0, index_count, false);
+            build_map_index(ixr, ref, 0, index_count, true);
+            map_reads_both(ixf, ixr, ref, ref_len, L, W, g.data(), n_un, B->Nascii.data(), n_N, last, m);
+        } else if (L >= MAP_K) {
             for (int strand = 0; strand < 2; ++strand) {
                 bool rc = strand == 1;
                 // compress.cu:341-360: two index halves past 2^30 positions.

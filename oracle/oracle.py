@@ -69,6 +69,7 @@ def lib(kind="st"):
         "orc_block_free": (None, [vp]),
         "orc_set_map_settle": (None, [i32]),
         "orc_map_settled_count": (u64, []),
+        "orc_set_map_both": (None, [i32, i32]),
         "orc_block_streams": (i32, [vp, i32, i32]),
         "orc_marush36": (u32, [vp]),
         "orc_ref_match": (vp, [vp, u64, vp, u64, i32, i32]),

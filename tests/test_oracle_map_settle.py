@@ -39,14 +39,16 @@ CASES = {
 }
 
 
-def run(reads, kill):
+def run(reads, kill, both=False, tie=False):
     orc.lib().orc_set_map_settle(kill)
+    orc.lib().orc_set_map_both(int(both), int(tie))
     try:
         b = orc.OracleBlock(reads, stop="all")
         out = {f: b.get(f).copy() for f in FIELDS}
         return out, int(orc.lib().orc_map_settled_count())
     finally:
         orc.lib().orc_set_map_settle(0)
+        orc.lib().orc_set_map_both(0, 0)
 
 
 def differs(a, b):
@@ -81,3 +83,31 @@ def test_weaker_rule_is_caught(blocks):
                 caught[kill].append(name)
     print(caught, flush=True)
     assert caught[1] and caught[2]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_interleaved_passes_with_the_rule_keep_every_result(blocks, name):
+    """Both strand passes interleaved probe by probe (as k_map_both runs
+    them), without and with the early end of each pass; the forward pass
+    then also ends once no unseen forward window can reach the reverse
+    best (B = min(forward best, reverse best + 1))."""
+    reads, plain = blocks[name]
+    got, _ = run(reads, 0, both=True)
+    assert not differs(plain, got), differs(plain, got)
+    got, fired = run(reads, 3, both=True)
+    print(name, "passes ended early (interleaved):", fired, flush=True)
+    assert not differs(plain, got), differs(plain, got)
+    assert fired > 0
+
+
+def test_interleaved_mutants_are_caught(blocks):
+    """Without the + 1 (a forward window equal to the reverse best would win
+    the tie and is skipped) or with a weaker rule, some result changes."""
+    caught = {"tie": [], "kill2": []}
+    for name, (reads, plain) in blocks.items():
+        if differs(plain, run(reads, 3, both=True, tie=True)[0]):
+            caught["tie"].append(name)
+        if differs(plain, run(reads, 2, both=True)[0]):
+            caught["kill2"].append(name)
+    print(caught, flush=True)
+    assert caught["tie"] and caught["kill2"]
