@@ -4,7 +4,11 @@
 #include <immintrin.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <thread>
 
 namespace curc {
@@ -152,10 +156,11 @@ uint64_t seg_cap(uint64_t bytes, uint32_t L) { return bytes / ((uint64_t)L + 4) 
 }  // namespace
 
 uint64_t fq_host_cap(uint64_t nb, uint32_t L, uint32_t T) {
-    return (nb / ((uint64_t)L + 4) + (uint64_t)T + 1) * (uint64_t)L;
+    return (nb / ((uint64_t)L + 4) + (uint64_t)FQH_RANGES * std::max<uint32_t>(T, 1) + 1) * (uint64_t)L;
 }
 
-void fq_host_parse(const uint8_t* B, uint64_t nb, uint32_t L, uint32_t T, uint8_t* out, FqHostParse* res) {
+void fq_host_parse(const uint8_t* B, uint64_t nb, uint32_t L, uint32_t T, uint8_t* out, FqHostParse* res,
+                   const FqEmit& emit) {
     FqHostParse r;
     if (nb == 0) {
         *res = r;
@@ -167,55 +172,102 @@ void fq_host_parse(const uint8_t* B, uint64_t nb, uint32_t L, uint32_t T, uint8_
         return;
     }
     T = std::max<uint32_t>(T, 1);
-    std::vector<uint64_t> S(T + 1);
+    const uint32_t R = FQH_RANGES * T;
+    std::vector<uint64_t> S(R + 1);
     S[0] = 0;
-    S[T] = nb;
-    for (uint32_t t = 1; t < T; ++t) S[t] = std::max(S[t - 1], guess_start(B, nb, nb * t / T, L));
-    std::vector<uint64_t> off(T + 1, 0), cap(T);
-    for (uint32_t t = 0; t < T; ++t) {
-        cap[t] = seg_cap(S[t + 1] - S[t], L);
-        off[t + 1] = off[t] + cap[t] * L;
+    S[R] = nb;
+    for (uint32_t q = 1; q < R; ++q) S[q] = std::max(S[q - 1], guess_start(B, nb, nb * q / R, L));
+    std::vector<uint64_t> off(R + 1, 0), cap(R);
+    for (uint32_t q = 0; q < R; ++q) {
+        cap[q] = seg_cap(S[q + 1] - S[q], L);
+        off[q + 1] = off[q] + cap[q] * L;
     }
-    std::vector<Walk> w(T);
-    {
-        std::vector<std::thread> th;
-        th.reserve(T);
-        for (uint32_t t = 1; t < T; ++t)
-            th.emplace_back(walk_any, B, nb, S[t], S[t + 1], L, out + off[t], cap[t], &w[t]);
-        walk_any(B, nb, S[0], S[1], L, out + off[0], cap[0], &w[0]);
-        for (auto& x : th) x.join();
-    }
-    bool ok = true;
-    for (uint32_t t = 0; t + 1 < T && ok; ++t)
-        if (S[t + 1] < nb && w[t].end != S[t + 1]) ok = false;
-    for (uint32_t t = 0; t < T && ok; ++t) ok = !w[t].overflow;
-    if (!ok) {
-        // a start guess was not a record start: one pass from the first byte
-        r.resync = true;
-        w.assign(1, Walk{});
-        cap.assign(1, seg_cap(nb, L));
-        off.assign(2, 0);
-        walk_any(B, nb, 0, nb, L, out, cap[0], &w[0]);
-        T = 1;
-    }
-    // the chain reaches the end of the bytes in the last thread whose range
-    // is not empty (later ones start at nb)
-    uint32_t last = 0;
-    for (uint32_t t = 0; t < T; ++t)
-        if (ok ? S[t] < nb : true) last = t;
+    std::vector<Walk> w(R);
+    // T workers take the ranges in order; this thread hands each range's
+    // sequences on as soon as it and every range before it are done
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<char> done(R, 0);
+    std::atomic<uint32_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const uint32_t q = next.fetch_add(1);
+            if (q >= R) return;
+            walk_any(B, nb, S[q], S[q + 1], L, out + off[q], cap[q], &w[q]);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                done[q] = 1;
+            }
+            cv.notify_all();
+        }
+    };
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (uint32_t t = 0; t < T; ++t) th.emplace_back(worker);
+    std::exception_ptr failed;
+    uint64_t base = 0;
     bool len_err = false;
-    for (uint32_t t = 0; t < T; ++t) {
-        len_err |= w[t].len_err;
-        r.n += w[t].recs;
+    auto hand_on = [&](uint64_t o, uint64_t count) {
+        if (!count) return;
+        const FqSeg g{o, count};
+        r.segs.push_back(g);
+        if (emit && !failed) {
+            try {
+                emit(g, base);
+            } catch (...) {
+                failed = std::current_exception();
+            }
+        }
+        base += count;
+    };
+    int broken = -1;  // first range whose successor's guess the chain did not land on
+    bool overflow = false;
+    uint32_t last = 0;
+    for (uint32_t q = 0; q < R; ++q) {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return done[q] != 0; });
+        }
+        if (w[q].overflow) {
+            broken = (int)q;
+            overflow = true;
+            break;
+        }
+        hand_on(off[q], w[q].copied);
+        r.n += w[q].recs;
+        len_err |= w[q].len_err;
+        if (S[q] < nb) last = q;
+        if (q + 1 < R && S[q + 1] < nb && w[q].end != S[q + 1]) {
+            broken = (int)q;
+            break;
+        }
     }
-    if (B[nb - 1] != '\n' || w[last].phase != 0 || w[last].end != nb) {
+    next.store(R);  // no more ranges for the workers
+    for (auto& x : th) x.join();
+    Walk tail;
+    const Walk* lw = &w[last];
+    if (broken >= 0) {
+        // a start guess was not a record start (or, never expected, a range
+        // overflowed): the rest of the bytes in one pass from the last record
+        // start the chain confirmed, into the space of the ranges after it
+        r.resync = true;
+        const uint32_t q = (uint32_t)broken;
+        const uint32_t q1 = overflow ? q : q + 1;
+        uint64_t tcap = 0;
+        for (uint32_t x = q1; x < R; ++x) tcap += cap[x];
+        walk_any(B, nb, overflow ? S[q] : w[q].end, nb, L, out + off[q1], tcap, &tail);
+        hand_on(off[q1], tail.copied);
+        r.n += tail.recs;
+        len_err |= tail.len_err;
+        lw = &tail;
+    }
+    if (failed) std::rethrow_exception(failed);
+    if (B[nb - 1] != '\n' || lw->phase != 0 || lw->end != nb) {
         r.err = FQH_TRUNC;
     } else if (len_err) {
         r.err = FQH_LEN;
-    } else {
-        for (uint32_t t = 0; t < T; ++t)
-            if (w[t].copied) r.segs.push_back(FqSeg{off[t], w[t].copied});
     }
+    if (r.err) r.segs.clear();
     *res = r;
 }
 

@@ -2535,12 +2535,16 @@ uint64_t fq_dev_bytes(const curc_fastq_job& j) {
     return ((j.bytes1 + 255) & ~255ull) + (j.fq2 ? j.bytes2 : 0) + 256;
 }
 
+// Rows a host-parsed file of a paired job may hold (a record takes at least
+// L + 4 bytes): the stride of each file's region in the device buffer.
+uint64_t fq_pair_rows(const curc_fastq_job& j) { return std::max(j.bytes1, j.bytes2) / ((uint64_t)j.L + 4) + 2; }
+
 // Device bytes a job needs: its FASTQ bytes, or with host parsing its
-// sequences (at most the FASTQ bytes: a record takes at least L + 4 bytes),
-// twice over for a paired block (each file's rows, then the interleaved rows).
+// sequences (at most the FASTQ bytes: a record takes at least L + 4 bytes);
+// a paired block holds the interleaved rows, then each file's rows.
 uint64_t fq_dev_bytes(const curc_fastq_job& j, uint32_t host_threads) {
     if (!host_threads || !j.fq2) return fq_dev_bytes(j);
-    return 2 * (j.bytes1 + j.bytes2) + 512;
+    return std::max(fq_dev_bytes(j), 4 * fq_pair_rows(j) * j.L + 512);
 }
 
 // Host threads for a file of nb bytes: ranges of at least 1 MB each.
@@ -2558,8 +2562,9 @@ __global__ void k_interleave_rows(const uint8_t* __restrict__ a, const uint8_t* 
 }
 
 // Host parse of a job's FASTQ bytes (fq_host_parse, `threads` threads) into
-// the pinned buffer c->stage(which), then copies of the sequences to the
-// device buffer `dev` (fq_dev_bytes(j, threads) bytes) on stream s.  The
+// the pinned buffer c->stage(which), with each parsed range's sequences
+// copied to the device buffer `dev` (fq_dev_bytes(j, threads) bytes) on
+// stream s as soon as it is ready, while the later ranges are parsed.  The
 // caller makes sure earlier copies out of that pinned buffer have ended.
 // Format errors throw InputError with the device parser's messages, in its
 // order (preprocess.cpp:147-199).
@@ -2575,8 +2580,18 @@ FqInput fq_stage_host(curc_ctx* c, const curc_fastq_job& j, uint32_t threads, in
         cap[f] = fq_host_cap(nb[f], j.L, th[f]);
     }
     uint8_t* h = c->stage(which, cap[0] + cap[1] + 64);
+    const uint64_t L = j.L;
+    // single-end rows straight to their place; a paired block's files to
+    // regions after the interleaved rows, interleaved on the device below
+    const uint64_t pr = nf == 2 ? fq_pair_rows(j) : 0;
     FqHostParse r[2];
-    for (int f = 0; f < nf; ++f) fq_host_parse(src[f], nb[f], j.L, th[f], h + (f ? cap[0] : 0), &r[f]);
+    for (int f = 0; f < nf; ++f) {
+        uint8_t* d = nf == 1 ? dev : dev + (2 + f) * pr * L;
+        const uint8_t* hf = h + (f ? cap[0] : 0);
+        fq_host_parse(src[f], nb[f], j.L, th[f], h + (f ? cap[0] : 0), &r[f], [&](const FqSeg& g, uint64_t first) {
+            CURC_HIP(hipMemcpyAsync(d + first * L, hf + g.off, g.count * L, hipMemcpyHostToDevice, s));
+        });
+    }
     for (int f = 0; f < nf; ++f)
         if (r[f].err == FQH_START) throw InputError("FASTQ block does not start with '@'");
     for (int f = 0; f < nf; ++f)
@@ -2589,20 +2604,9 @@ FqInput fq_stage_host(curc_ctx* c, const curc_fastq_job& j, uint32_t threads, in
     FqInput in;
     in.n_ascii = N;
     in.ascii = dev;
-    const uint64_t L = j.L;
-    for (int f = 0; f < nf; ++f) {
-        // single-end rows straight to their place; a paired block's files to
-        // the rows after the interleaved ones, then interleaved on the device
-        uint8_t* d = nf == 1 ? dev : dev + (2 + f) * n1 * L;
-        uint64_t at = 0;
-        for (const FqSeg& g : r[f].segs) {
-            CURC_HIP(hipMemcpyAsync(d + at * L, h + (f ? cap[0] : 0) + g.off, g.count * L, hipMemcpyHostToDevice, s));
-            at += g.count;
-        }
-    }
     if (nf == 2 && n1)
         hipLaunchKernelGGL(k_interleave_rows, dim3(grid_for(2 * n1 * L)), dim3(256), 0, s,
-                           (const uint8_t*)(dev + 2 * n1 * L), (const uint8_t*)(dev + 3 * n1 * L), n1, j.L, dev);
+                           (const uint8_t*)(dev + 2 * pr * L), (const uint8_t*)(dev + 3 * pr * L), n1, j.L, dev);
     return in;
 }
 
