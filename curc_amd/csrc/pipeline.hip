@@ -317,6 +317,27 @@ uint32_t read_cnt(curc_ctx* c, const uint32_t* p) {
     return v;
 }
 
+// Two stable compactions of one list (length *d_n, host bound n): in one
+// single-pass launch (k_compact2, look-back state in scratch slot S_LB) when
+// both fit its tile limit, else one after the other.  With the single pass,
+// its look-back error flag is copied to *lb_err on the stream: the caller
+// checks it after its next wait on the stream.
+template <class PA, class EA, class PB, class EB>
+void compact_pair(curc_ctx* c, const uint32_t* d_n, uint64_t n, PA pa, EA ea, uint32_t* outa, PB pb, EB eb,
+                  uint32_t* outb, uint32_t* lb_err) {
+    Scratch& S = c->S;
+    *lb_err = 0;
+    if (2 * compact2_tiles(n) <= COMPACT2_MAX_TILES) {
+        LookbackState* lb = S.as<LookbackState>(S_LB, 1);
+        fill<uint32_t>(c, (uint32_t*)lb, 4 + compact2_words(n, n), 0u);
+        compact2(d_n, pa, ea, outa, n, d_n, pb, eb, outb, n, lb, c->st);
+        CURC_HIP(hipMemcpyAsync(lb_err, &lb->err, 4, hipMemcpyDeviceToHost, c->st));
+    } else {
+        compact(d_n, pa, ea, outa, S, S_PRIM, c->st, n);
+        compact(d_n, pb, eb, outb, S, S_PRIM, c->st, n);
+    }
+}
+
 template <typename V, typename T>
 void d2h(curc_ctx* c, V& out, const T* src, uint64_t n) {
     static_assert(sizeof(typename V::value_type) == sizeof(T), "element size");
@@ -1384,10 +1405,11 @@ NlqOut nlq_stage(curc_ctx* x, hipEvent_t wait, const uint8_t* Nascii, uint64_t n
         uint32_t* pre = S.as<uint32_t>(S_PRE, n_nlq);
         uint32_t* suf = S.as<uint32_t>(S_SUF, n_nlq);
         set_cnt(x, cnt + C_TMP, n_nlq);
-        compact(cnt + C_TMP, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE, S, S_PRIM, st,
-                n_nlq);
-        compact(cnt + C_TMP, PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, S, S_PRIM, st,
-                n_nlq);
+        uint32_t lb_err = 0;
+        compact_pair(x, cnt + C_TMP, n_nlq, PredLinkState{nsorted, nprev}, EmitGather{nsorted, pre}, cnt + C_PRE,
+                     PredLinkState{nsorted, nnext}, EmitGather{nsorted, suf}, cnt + C_SUF, &lb_err);
+        CURC_HIP(stream_wait(st));
+        if (lb_err != 0) throw std::runtime_error("N-read list compaction: look-back did not complete");
         full_match<Enc3>(x, p3, W3, L, pre, suf, cnt, n_nlq, nnext, nprev, noff, "nlq");
         out.a = assemble(x, nsorted, n_nlq, nN, nnext, nprev, noff, L, cnt, S_NLQIDS);
         const AsmOut& a = out.a;
@@ -1692,10 +1714,15 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     // ---- a5: prefix k-mer index (compress.cu:1427-1471)
     uint32_t* va = S.as<uint32_t>(S_VA, (uint64_t)n + 1);
     uint32_t* sl0 = S.as<uint32_t>(S_SL0, (uint64_t)n + 1);
-    compact(cnt + C_TMP, PredLinkState{sorted, prev}, EmitGather{sorted, va}, cnt + C_NPRE, S, S_PRIM, st, n);
-    compact(cnt + C_TMP, PredLinkState{sorted, next}, EmitGather{sorted, sl0}, cnt + C_NSUF, S, S_PRIM, st, n);
+    // the prefix list (reads without a predecessor) and the suffix list
+    // (without a successor), in dedup order: one single-pass launch for both
+    // when they fit its tile limit
+    uint32_t lb_err = 0;
+    compact_pair(c, cnt + C_TMP, n, PredLinkState{sorted, prev}, EmitGather{sorted, va}, cnt + C_NPRE,
+                 PredLinkState{sorted, next}, EmitGather{sorted, sl0}, cnt + C_NSUF, &lb_err);
     const uint32_t npre = read_cnt(c, cnt + C_NPRE);
     const uint32_t nsuf = read_cnt(c, cnt + C_NSUF);
+    if (lb_err != 0) throw std::runtime_error("prefix / suffix list compaction: look-back did not complete");
     {
         const uint32_t H = hash_size(npre);
         const uint32_t mask = H - 1;
@@ -1975,8 +2002,11 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         uint32_t* pre = S.as<uint32_t>(S_PRE, (uint64_t)n_hq + 1);
         uint32_t* suf = S.as<uint32_t>(S_SUF, (uint64_t)n_hq + 1);
         set_cnt(c, cnt + C_TMP, n_hq);
-        compact(cnt + C_TMP, PredLinkState{hq, prev}, EmitGather{hq, pre}, cnt + C_PRE, S, S_PRIM, st, n_hq);
-        compact(cnt + C_TMP, PredLinkState{hq, next}, EmitGather{hq, suf}, cnt + C_SUF, S, S_PRIM, st, n_hq);
+        uint32_t lb_err = 0;
+        compact_pair(c, cnt + C_TMP, n_hq, PredLinkState{hq, prev}, EmitGather{hq, pre}, cnt + C_PRE,
+                     PredLinkState{hq, next}, EmitGather{hq, suf}, cnt + C_SUF, &lb_err);
+        CURC_HIP(stream_wait(st));
+        if (lb_err != 0) throw std::runtime_error("HQ list compaction: look-back did not complete");
         full_match<Enc2>(c, packed, W, L, pre, suf, cnt, n_hq, next, prev, off, "full_match");
     }
     if (keep || stop <= CURC_STAGE_FULLMATCH) {

@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(1024) k_compact_small(const uint32_t* __restri
 // launch's output is then invalid (the caller checks).
 constexpr uint32_t COMPACT2_ITEMS = 16;
 constexpr uint32_t COMPACT2_TILE = 256 * COMPACT2_ITEMS;
-constexpr uint32_t COMPACT2_MAX_TILES = 4096;  // per launch (both lists)
+constexpr uint32_t COMPACT2_MAX_TILES = 16384;  // per launch (both lists)
 constexpr uint64_t LB_AGG = 1ull << 32, LB_INC = 2ull << 32;
 constexpr uint32_t LB_SPIN_LIMIT = 1u << 26;
 
