@@ -147,23 +147,24 @@ __global__ void k_gather_word(const uint64_t* __restrict__ R, const uint32_t* __
 struct PredTied {
     const uint64_t* key;  // first words in sorted order
     uint64_t n;
+    uint64_t mask;        // the bits the sort ordered by
     __device__ bool operator()(uint32_t i) const {
-        const uint64_t k = key[i];
-        return (i > 0 && key[i - 1] == k) || (i + 1 < n && key[i + 1] == k);
+        const uint64_t k = key[i] & mask;
+        return (i > 0 && (key[i - 1] & mask) == k) || (i + 1 < n && (key[i + 1] & mask) == k);
     }
 };
 
 // Subset entry j: its read id, its index (the value the subset sorts
 // carry) and 1 when it starts a run.
 __global__ void k_tie_setup(const uint32_t* __restrict__ pos, const uint32_t* __restrict__ d_m,
-                            const uint64_t* __restrict__ key, const uint32_t* __restrict__ ids,
+                            const uint64_t* __restrict__ key, uint64_t mask, const uint32_t* __restrict__ ids,
                             uint32_t* __restrict__ sid, uint32_t* __restrict__ perm, uint32_t* __restrict__ head) {
     const uint32_t m = *d_m;
     GRID_STRIDE(j, m) {
         const uint32_t p = pos[j];
         sid[j] = ids[p];
         perm[j] = (uint32_t)j;
-        head[j] = (j == 0 || key[pos[j - 1]] != key[p]) ? 1u : 0u;
+        head[j] = (j == 0 || (key[pos[j - 1]] & mask) != (key[p] & mask)) ? 1u : 0u;
     }
 }
 

@@ -1639,15 +1639,21 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             const uint32_t nbases = (w == W - 1) ? (L - 32 * (W - 1)) : 32;
             return ((64 - 2 * nbases) / 8) * 8;
         };
-        // every read by its first word (stable, so ties keep ascending id)
+        // every read by its first 16 bases (the top 32 bits of its first
+        // word; stable, so ties keep ascending id): half the passes of the
+        // whole word, and reads that share their first 16 bases are nearly
+        // all reads of one locus, which share the next 16 as well
+        const uint32_t b0 = begin_bit(0);
+        const bool split = b0 < 32;  // bits [b0, 32) of the first word left for the runs
+        const uint64_t kmask = split ? 0xFFFFFFFF00000000ull : ~0ull;
         hipLaunchKernelGGL(k_gather_word, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)packed,
                            (const uint32_t*)v0, (uint64_t)n, W, 0u, k0);
-        radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, begin_bit(0), 64, S, S_PRIM, st);
-        if (W > 1 && n > 1) {
-            // reads in runs of equal first words, by the remaining words
+        radix_sort_pairs<uint64_t>(k0, v0, k1, v1, n, split ? 32u : b0, 64, S, S_PRIM, st);
+        if ((W > 1 || split) && n > 1) {
+            // reads in runs of equal sorted bits, by the rest of their words
             uint32_t* tpos = S.as<uint32_t>(S_TPOS, (uint64_t)n + 1);
             set_cnt(c, cnt + C_TMP, n);
-            compact(cnt + C_TMP, PredTied{k0, n}, EmitIota{tpos}, cnt + C_TIE, S, S_PRIM, st, n);
+            compact(cnt + C_TMP, PredTied{k0, n, kmask}, EmitIota{tpos}, cnt + C_TIE, S, S_PRIM, st, n);
             const uint32_t m = read_cnt(c, cnt + C_TIE);
             if (m > 0) {
                 uint32_t* sid = S.as<uint32_t>(S_TSID, (uint64_t)m + 1);
@@ -1659,8 +1665,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                 uint64_t* tk1 = S.as<uint64_t>(S_TK1, (uint64_t)m + 1);
                 const dim3 gm(grid_for(m));
                 hipLaunchKernelGGL(k_tie_setup, gm, dim3(256), 0, st, (const uint32_t*)tpos,
-                                   (const uint32_t*)(cnt + C_TIE), (const uint64_t*)k0, (const uint32_t*)v0, sid, p0,
-                                   head);
+                                   (const uint32_t*)(cnt + C_TIE), (const uint64_t*)k0, kmask, (const uint32_t*)v0,
+                                   sid, p0, head);
                 scan_excl(head, run, m, S, S_PRIM, st);
                 hipLaunchKernelGGL(k_tie_runid, gm, dim3(256), 0, st, (const uint32_t*)(cnt + C_TIE),
                                    (const uint32_t*)head, (const uint32_t*)run, run);
@@ -1670,6 +1676,12 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                     hipLaunchKernelGGL(k_gather_word_perm, gm, dim3(256), 0, st, (const uint64_t*)packed,
                                        (const uint32_t*)sid, (const uint32_t*)perm, (uint64_t)m, W, w, tk0);
                     radix_sort_pairs<uint64_t>(tk0, perm, tk1, permt, m, begin_bit(w), 64, S, S_PRIM, st);
+                }
+                if (split) {
+                    // the first word's remaining bits, most significant of the words after it
+                    hipLaunchKernelGGL(k_gather_word_perm, gm, dim3(256), 0, st, (const uint64_t*)packed,
+                                       (const uint32_t*)sid, (const uint32_t*)perm, (uint64_t)m, W, 0u, tk0);
+                    radix_sort_pairs<uint64_t>(tk0, perm, tk1, permt, m, b0, 32, S, S_PRIM, st);
                 }
                 // then by run number (stable): runs back in position order
                 uint32_t* rk0 = reinterpret_cast<uint32_t*>(tk0);

@@ -57,6 +57,32 @@ def test_config_small(gpu_ctx):
     run_both(gpu_ctx, ca.synth_reads(ca.config_small())[1])
 
 
+@pytest.mark.parametrize("L", [20, 24, 33, 40, 64, 100, 150])
+def test_dedup_order_of_reads_sharing_their_first_bases(gpu_ctx, L):
+    """The dedup sort orders reads by their first 16 bases and then the runs
+    of reads that share them by everything after: reads here come from 6
+    prefixes of 16 bases with tails from 40 variants (point changes of one
+    tail spread over every word, so ties reach every word), many exact
+    duplicates, and N-reads.  L = 20 and 24 end inside the first word, and
+    L = 33 ends one base into the second."""
+    rng = np.random.default_rng(900 + L)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    heads = acgt[rng.integers(0, 4, (6, 16))]
+    base = acgt[rng.integers(0, 4, L - 16)]
+    tails = np.repeat(base[None], 40, 0)
+    for t in range(1, 40):
+        j = rng.integers(0, L - 16, 1 + t % 3)
+        tails[t, j] = acgt[rng.integers(0, 4, len(j))]
+    n = 20_000
+    reads = np.concatenate([heads[rng.integers(0, 6, n)], tails[rng.integers(0, 40, n)]], axis=1)
+    nr = rng.choice(n, 300, replace=False)
+    reads[nr, rng.integers(0, L, 300)] = ord("N")
+    g = gpu_ctx.run_block(reads, stop_after=ca.STAGE_DEDUP, keep_stages=True)
+    o = orc.OracleBlock(reads, stop="dedup")
+    compare(g, o, STAGE_FIELDS[0])
+    g.free()
+
+
 @pytest.mark.parametrize("L", [75, 100, 250, 300])
 def test_read_lengths(gpu_ctx, L):
     cfg = ca.synth_config(seed=200 + L, genome_len=40_000, gc=0.48, n_reads=6_000, L=L,
