@@ -1447,6 +1447,20 @@ __global__ void __launch_bounds__(1024) k_fm_fused(ClaimState c0, ClaimState c1,
 
 // ------------------------------------------------ exact matching, groups ---
 constexpr uint32_t FM_SMALL = 16;  // groups up to this size are sorted in registers
+// Row width of the group table gmem (member slots per prefix position).
+// Over lists kept in place it is FM_SMALL.  Over compacted lists (gcap: the
+// table's entries) it is the largest power of two up to FM_ROW_MAX whose
+// rows for all np positions fit, so that the larger groups of the last
+// offsets (short lists) have their members listed too; the probe and the
+// resolution derive it alike from the same counter.
+constexpr uint32_t FM_ROW_MAX = 4096;
+constexpr uint32_t FM_WAVE_MAX = 62;  // groups one wavefront resolves (pos(0 .. m+1), a lane each)
+__device__ __forceinline__ uint32_t fm_row_width(uint64_t gcap, uint32_t np) {
+    uint32_t w = FM_SMALL;
+    while (w < FM_ROW_MAX && (uint64_t)np * (2 * w) <= gcap) w <<= 1;
+    return w;
+}
+constexpr uint32_t FM_ROW = 0xFFFFFFFEu;  // FmCoop::base of a group whose members are in its row
 // The exact-matching loop of k_fm_probe_s / k_fm_resolve_s, over lists kept
 // in place (cs.nf set) or compacted after every offset (cs.nf null).
 //
@@ -1462,9 +1476,9 @@ constexpr uint32_t FM_SMALL = 16;  // groups up to this size are sorted in regis
 // Each suffix with an equal free non-self prefix joins its group (the first
 // equal position of the list, grp) and records that prefix (choice): it
 // takes the next rank of the group's count gcnt, goes into the group's row
-// of gmem when the rank is below FM_SMALL (so that the resolution reads a
-// small group with one load), and into the group's member list (head /
-// memb) for larger ones.  Rank 0 sets its lead flag, by which the
+// of gmem when the rank is below the row width (fm_row_width; so that the
+// resolution reads a small group with one load), and into the group's
+// member list (head / memb) for larger ones.  Rank 0 sets its lead flag, by which the
 // resolution finds the group.  No counter is shared by all suffixes.
 // zero_p / zero_n: a buffer to clear on the way (the look-back status words
 // of the compaction that follows, between compacted offsets).
@@ -1472,9 +1486,10 @@ template <class E>
 __global__ void __launch_bounds__(256) k_fm_probe_s(ClaimState cs, const uint32_t* __restrict__ rng, uint32_t bp,
                                                     uint32_t off, const uint64_t* __restrict__ vb, uint64_t vnw,
                                                     uint8_t* __restrict__ lead, uint32_t* __restrict__ gcnt,
-                                                    uint32_t* __restrict__ gmem, uint32_t* __restrict__ zero_p,
-                                                    uint64_t zero_n) {
+                                                    uint32_t* __restrict__ gmem, uint64_t gcap,
+                                                    uint32_t* __restrict__ zero_p, uint64_t zero_n) {
     const uint32_t ns = *cs.d_ns, np = *cs.d_np;
+    const uint32_t rw = fm_row_width(gcap, np);
     const bool inplace = cs.nf != nullptr;
     // a buffer the next compaction needs cleared (its look-back status)
     GRID_STRIDE(z, zero_n) zero_p[z] = 0;
@@ -1536,7 +1551,7 @@ __global__ void __launch_bounds__(256) k_fm_probe_s(ClaimState cs, const uint32_
         cs.choice[i] = ch;
         cs.grp[i] = lo;
         const uint32_t r = atomicAdd(&gcnt[lo], 1u);
-        if (r < FM_SMALL) gmem[(uint64_t)lo * FM_SMALL + r] = (uint32_t)i;
+        if (r < rw) gmem[(uint64_t)lo * rw + r] = (uint32_t)i;
         cs.memb[i] = atomicExch(&cs.head[lo], (uint32_t)i);
         if (r == 0) lead[i] = 1;
     }
@@ -1773,9 +1788,10 @@ __device__ __forceinline__ void fm_small_group(const ClaimState& cs, uint32_t (&
 // Groups of more than FM_SMALL members, up to FM_COOP_MAX, are sorted by
 // the whole workgroup in LDS (bitonic) after its scan; the walk then reads
 // the members' reads and the run's prefixes from LDS.  A group's members
-// come from the pool (base, m) or, for compacted lists of at most
-// FM_GATHER_MAX suffixes (base INV), from a scan of the group marks, which
-// costs less than following the member list one load at a time.
+// come from the group's row of the table (base FM_ROW, m), from the pool
+// (base, m) or, for compacted lists of at most FM_GATHER_MAX suffixes (base
+// INV), from a scan of the group marks, which costs less than following the
+// member list one load at a time.
 constexpr uint32_t FM_COOP_MAX = RESOLVE_COOP_MAX - 2;
 struct FmCoop {
     uint64_t key[RESOLVE_COOP_MAX];  // sort keys; then the members' reads (u32) and the run's prefixes (u32)
@@ -1785,7 +1801,8 @@ struct FmCoop {
 };
 
 template <class E>
-__device__ __forceinline__ void fm_coop_groups(const ClaimState& cs, FmCoop& co, uint32_t off) {
+__device__ __forceinline__ void fm_coop_groups(const ClaimState& cs, FmCoop& co, uint32_t off,
+                                               const uint32_t* __restrict__ rows, uint32_t rw) {
     __syncthreads();
     const uint32_t nb = min(co.n, RESOLVE_COOP_LIST);
     const uint32_t ns = *cs.d_ns;
@@ -1812,7 +1829,7 @@ __device__ __forceinline__ void fm_coop_groups(const ClaimState& cs, FmCoop& co,
             }
         } else {
             m = co.m[gi];
-            const uint32_t* arr = cs.pool + co.base[gi];
+            const uint32_t* arr = co.base[gi] == FM_ROW ? rows + (uint64_t)g * rw : cs.pool + co.base[gi];
             for (uint32_t x = threadIdx.x; x < m; x += blockDim.x) co.mem[x] = arr[x];
             __syncthreads();
         }
@@ -1879,28 +1896,98 @@ __device__ __forceinline__ void fm_coop_groups(const ClaimState& cs, FmCoop& co,
     }
 }
 
+// 64-bit value of lane (lane ^ j) / of lane k, as two 32-bit shuffles.
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t j) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, (int)j, 64), hi = __shfl_xor((uint32_t)(v >> 32), (int)j, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// A group of FM_SMALL + 1 .. FM_WAVE_MAX members over compacted lists,
+// resolved by one wavefront (all 64 lanes call it with the same g, m): lane
+// k loads member k from the group's row, the members are sorted by a
+// bitonic network across the lanes (the order of fm_before; the reads are
+// loaded only to break a tie of the keys), lane j holds pos(j) of the run,
+// and the walk of fm_walk_free runs on every lane alike with the values
+// taken from their lanes (lane 0 writes the links).
+template <class E>
+__device__ __forceinline__ void fm_wave_group(const ClaimState& cs, uint32_t g, uint32_t m, uint32_t off,
+                                              const uint32_t* __restrict__ row, uint32_t lane) {
+    uint32_t u = lane < m ? row[lane] : INV;
+    uint32_t sid = u != INV ? cs.sl[u] : INV;
+    uint64_t key = u != INV ? rev_prefix_key<E>(cs.R, cs.W, sid, off) : ~0ull;
+#pragma unroll
+    for (uint32_t k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t pu = __shfl_xor(u, (int)j, 64), ps = __shfl_xor(sid, (int)j, 64);
+            const uint64_t pkey = shfl_xor_u64(key, j);
+            // the lower lane of a pair keeps the earlier member in an ascending run
+            const bool first = ((lane & j) == 0) == ((lane & k) == 0);
+            const bool sw = first ? fm_before<E>(cs, pu, pkey, ps, u, key, sid, off)
+                                  : fm_before<E>(cs, u, key, sid, pu, pkey, ps, off);
+            if (sw) {
+                u = pu;
+                sid = ps;
+                key = pkey;
+            }
+        }
+    }
+    // the run's prefixes at positions g .. g + m + 1 (INV from the first unequal one)
+    const uint32_t np = *cs.d_np, len = cs.L - off, n1 = min(E::BPW, len);
+    const uint64_t* rs0 = cs.R + (uint64_t)__shfl(sid, 0, 64) * cs.W;
+    const uint64_t sx00 = E::word(rs0, off, n1);
+    const uint32_t p = g + lane;
+    const uint32_t posl =
+        (lane <= m + 1 && p < np && cmp_ps<E>(cs.R, cs.W, cs.pl[p], cs.pk[p], rs0, sx00, off, len) == 0) ? cs.pl[p]
+                                                                                                         : INV;
+    // fm_walk_free; a, b <= m + 1 < 64
+    uint32_t a = 0, b = 1;
+    for (uint32_t k = 0; k < m; ++k) {
+        const uint32_t pa = __shfl(posl, (int)a, 64);
+        if (pa == INV) break;
+        const uint32_t s = __shfl(sid, (int)k, 64), mk = __shfl(u, (int)k, 64);
+        if (pa != s) {
+            if (lane == 0) fm_link(cs, mk, s, g + a, pa, off);
+            a = b;
+            b = b + 1;
+        } else {
+            const uint32_t pb = __shfl(posl, (int)b, 64);
+            if (pb != INV) {
+                if (lane == 0) fm_link(cs, mk, s, g + b, pb, off);
+                b = b + 1;
+            }
+        }
+    }
+    if (u != INV) cs.grp[u] = INV;
+    if (lane == 0) cs.head[g] = INV;
+}
+
 // Resolution of the groups k_fm_probe_s formed, found by their lead flags
 // (cleared here).  A lone member links to its choice (claim_assign's walk
 // gives the same prefix for one member); groups of up to FM_SMALL members
 // are sorted in registers by the thread that found them, larger ones by
 // the workgroup (up to FM_COOP_MAX and RESOLVE_COOP_LIST per workgroup;
-// beyond, by that thread, as in claim_resolve_body).  Their members are
-// walked in sorted order (claim_assign; fm_walk_free when the lists are
-// compacted).
+// beyond, by that thread, as in claim_resolve_body).  Over compacted lists,
+// groups whose members all sit in the row (up to FM_WAVE_MAX) are resolved
+// by the wavefront that found them (fm_wave_group) instead.  Their members
+// are walked in sorted order (claim_assign; fm_walk_free when the lists are
+// compacted).  The list is split into one run of positions per wavefront,
+// so that the groups of a short list spread over all of them.
 template <class E>
 __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t off, uint8_t* __restrict__ lead,
-                                                      uint32_t* __restrict__ gcnt, const uint32_t* __restrict__ gmem) {
+                                                      uint32_t* __restrict__ gcnt, const uint32_t* __restrict__ gmem,
+                                                      uint64_t gcap) {
     __shared__ FmCoop co;
     if (threadIdx.x == 0) co.n = 0;
     __syncthreads();
     const uint32_t ns = *cs.d_ns;
-    auto group = [&](uint32_t i) {
-        const uint32_t g = cs.grp[i];
-        uint32_t m = gcnt[g];
+    const uint32_t rw = fm_row_width(gcap, *cs.d_np);
+    const uint32_t wmax = (gcap && !cs.nf) ? min(rw, FM_WAVE_MAX) : 0u;
+    auto group = [&](uint32_t g, uint32_t m) {
         gcnt[g] = 0;
         if (m <= FM_SMALL) {
             uint32_t mem[FM_SMALL];
-            const uint4* row = reinterpret_cast<const uint4*>(gmem + (uint64_t)g * FM_SMALL);
+            const uint4* row = reinterpret_cast<const uint4*>(gmem + (uint64_t)g * rw);
 #pragma unroll
             for (uint32_t q = 0; q < FM_SMALL / 4; ++q) {
                 const uint4 v = row[q];
@@ -1922,6 +2009,16 @@ __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t of
         }
         const uint32_t u0 = cs.head[g];
         uint32_t u = u0;
+        if (!cs.nf && gcap && m <= rw && m <= FM_COOP_MAX) {
+            // every member is in the group's row
+            const uint32_t slot = atomicAdd(&co.n, 1u);
+            if (slot < RESOLVE_COOP_LIST) {
+                co.g[slot] = g;
+                co.base[slot] = FM_ROW;
+                co.m[slot] = m;
+                return;
+            }
+        }
         if (!cs.nf && ns <= FM_GATHER_MAX) {
             const uint32_t slot = atomicAdd(&co.n, 1u);
             if (slot < RESOLVE_COOP_LIST) {
@@ -1946,17 +2043,38 @@ __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t of
         }
         fm_serial_group<E>(cs, g, off);
     };
-    // each workgroup scans its own run of positions, so that the groups of
-    // a short list still spread over all workgroups and threads
-    const uint32_t per = (ns + gridDim.x - 1) / gridDim.x;
-    const uint64_t beg = (uint64_t)blockIdx.x * per;
+    // each wavefront scans its own run of positions
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint64_t nwv = (uint64_t)gridDim.x * wpb;
+    const uint64_t wid = (uint64_t)blockIdx.x * wpb + (threadIdx.x >> 6);
+    const uint64_t per = (ns + nwv - 1) / nwv;
+    const uint64_t beg = wid * per;
     const uint64_t end = min<uint64_t>(beg + per, ns);
-    for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-        if (!lead[i]) continue;
-        lead[i] = 0;
-        group((uint32_t)i);
+    for (uint64_t i0 = beg; i0 < end; i0 += 64) {
+        const uint64_t i = i0 + lane;
+        uint32_t wg = INV, wm = 0;
+        if (i < end && lead[i]) {
+            lead[i] = 0;
+            const uint32_t g = cs.grp[i];
+            const uint32_t m = gcnt[g];
+            if (m > FM_SMALL && m <= wmax) {
+                gcnt[g] = 0;
+                wg = g;
+                wm = m;
+            } else {
+                group(g, m);
+            }
+        }
+        uint64_t bal = __ballot(wg != INV);
+        while (bal) {
+            const int k = __builtin_ctzll(bal);
+            bal &= bal - 1;
+            const uint32_t gk = __shfl(wg, k, 64), mk = __shfl(wm, k, 64);
+            fm_wave_group<E>(cs, gk, mk, off, gmem + (uint64_t)gk * rw, lane);
+        }
     }
-    fm_coop_groups<E>(cs, co, off);
+    fm_coop_groups<E>(cs, co, off, gmem, rw);
 }
 
 // Minimizer-blocked filter of the prefixes' first 32 bases (the blocks and

@@ -612,10 +612,10 @@ void fm_static_phase(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, uin
         ClaimState cs{R, W, L, sl, d_ns, pl, pk, d_np, next, prev, offset, choice, grp, nullptr, nullptr,
                       head, memb, resl, cnt + C_RESN, pool, cnt + C_POOL, kpool, slinked, plinked, work, nf};
         hipLaunchKernelGGL(k_fm_probe_s<E>, dim3(g), dim3(256), 0, st, cs, ranged ? (const uint32_t*)rng : nullptr, bp,
-                           off, (vb && off < vmax) ? (const uint64_t*)vb : nullptr, vnw, lead, gcnt, gmem,
+                           off, (vb && off < vmax) ? (const uint64_t*)vb : nullptr, vnw, lead, gcnt, gmem, 0ull,
                            (uint32_t*)nullptr, 0ull);
         hipLaunchKernelGGL((k_fm_resolve_s<E>), dim3(3 * c->cus), dim3(256), 0, st, cs, off, lead, gcnt,
-                           (const uint32_t*)gmem);
+                           (const uint32_t*)gmem, 0ull);
     }
     hipLaunchKernelGGL(k_fm_count_work, dim3(grid_for(n_max, 1024)), dim3(256), 0, st, (const uint32_t*)sl,
                        (const uint32_t*)d_ns, (const uint32_t*)next, (const uint16_t*)offset, Ls,
@@ -668,6 +668,8 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
     // per group (prefix position): member count and the first FM_SMALL members
     uint32_t* gcnt = S.as<uint32_t>(S_FMGC, n_max);
     uint32_t* gmem = S.as<uint32_t>(S_FMGM, (uint64_t)n_max * FM_SMALL);
+    // over the compacted lists of the last offsets, rows widen as the lists shrink
+    const uint64_t gcap = (uint64_t)n_max * FM_SMALL;
     fill<uint32_t>(c, gcnt, n_max, 0u);
     fill<uint32_t>(c, grp, n_max, INV);  // group marks (see k_fm_resolve_s)
     uint32_t* owner = S.as<uint32_t>(S_OWNER, (uint64_t)n_max + 1);
@@ -755,10 +757,10 @@ void full_match(curc_ctx* c, const uint64_t* R, uint32_t W, uint32_t L, const ui
                           kpool, slinked, plinked, work + 4};
             hipLaunchKernelGGL(k_fm_probe_s<E>, dim3(g), dim3(256), 0, c->st, cs,
                                ranged ? (const uint32_t*)rng : nullptr, bp, off, (const uint64_t*)nullptr, 0ull, lead,
-                               gcnt, gmem, lb_status, lb_words);
+                               gcnt, gmem, gcap, lb_status, lb_words);
             // three workgroups per CU (the cooperative sort's LDS allows three)
             hipLaunchKernelGGL((k_fm_resolve_s<E>), dim3(3 * c->cus), dim3(256), 0, c->st, cs, off, lead, gcnt,
-                               (const uint32_t*)gmem);
+                               (const uint32_t*)gmem, gcap);
             compact_lists(cur);
             cur = 1 - cur;
         }
