@@ -8,9 +8,9 @@
 //  2. pointer jumping along prev gives, per node, the bases emitted before it
 //     in its chain, its index in the chain, its head and the minimum rank in
 //     its chain (or, for a cycle, in the whole cycle);
-//  3. one thread per cycle (the member of minimum rank, i.e. the read the
-//     reference visits first) finds the reference's cut and the cycle is
-//     ranked again as a path;
+//  3. each cycle is ranked from its member of minimum rank (the read the
+//     reference visits first), the reference's cut is found from those
+//     distances, and the cycle is ranked again as a path;
 //  4. chains are emitted in order of their minimum rank, which is the order in
 //     which the reference's visit loop first touches them.
 #pragma once
@@ -81,35 +81,70 @@ __global__ void k_asm_round(const uint32_t* __restrict__ list, uint64_t n, const
     }
 }
 
-// Cycle nodes keep a live jump pointer after enough rounds; the member of
-// minimum rank cuts its cycle where the reference does (compress.cu:125-145)
-// and lists the cycle's members for re-ranking.
-__global__ void k_asm_cycles(uint64_t n, const JumpState* __restrict__ st, uint32_t* __restrict__ nxtR,
-                             uint32_t* __restrict__ prvR, const uint16_t* __restrict__ offR,
-                             uint32_t* __restrict__ members, uint32_t* __restrict__ n_members) {
+// Cycle nodes keep a live jump pointer after enough rounds; their window
+// holds the whole cycle, so m is the cycle's minimum rank j0, the member the
+// reference's visit loop meets first.  The reference cuts the cycle after
+// the first read of largest offset met walking next from j0 (compress.cu:
+// 125-145): the walk visits next[j0] (distance 1), ..., j0 last (distance
+// len).  Here that cut is found in parallel:
+//  1. k_asm_cyc_list lists the cycle members and records their cycle (cid);
+//  2. k_asm_cyc_open opens each cycle at j0 (no predecessor) and the jump
+//     rounds rank the members as a path: c = 1 + distance from j0;
+//  3. k_asm_cyc_best takes, per cycle, the largest (offset, -distance) with
+//     j0 at the largest distance, which is that first read of largest offset;
+//  4. k_asm_cyc_cut cuts after it; the cycles are then ranked again as paths.
+__global__ void k_asm_cyc_list(uint64_t n, const JumpState* __restrict__ st, uint32_t* __restrict__ members,
+                               uint32_t* __restrict__ n_members, uint32_t* __restrict__ cid,
+                               unsigned long long* __restrict__ best) {
     GRID_STRIDE(j, n) {
-        JumpState a = st[j];
-        if (a.p == INV || a.m != (uint32_t)j) continue;
-        uint32_t t = nxtR[j];
-        uint32_t mo = t;
-        uint16_t mof = offR[t];
-        uint32_t len = 1;
-        while (t != (uint32_t)j) {
-            t = nxtR[t];
-            ++len;
-            if (offR[t] > mof) {
-                mof = offR[t];
-                mo = t;
-            }
-        }
-        uint32_t base = atomicAdd(n_members, len);
-        t = (uint32_t)j;
-        for (uint32_t k = 0; k < len; ++k) {
-            members[base + k] = t;
-            t = nxtR[t];
-        }
-        uint32_t hd = nxtR[mo];
-        nxtR[mo] = INV;
+        const JumpState a = st[j];
+        if (a.p == INV) continue;
+        members[atomicAdd(n_members, 1u)] = (uint32_t)j;
+        cid[j] = a.m;
+        if (a.m == (uint32_t)j) best[j] = 0ull;
+    }
+}
+
+__global__ void k_asm_cyc_open(const uint32_t* __restrict__ members, uint64_t nm, const uint32_t* __restrict__ cid,
+                               const uint32_t* __restrict__ prvR, JumpState* __restrict__ st) {
+    GRID_STRIDE(q, nm) {
+        const uint32_t j = members[q];
+        JumpState a;
+        a.p = cid[j] == j ? INV : prvR[j];
+        a.c = 1;
+        a.h = j;
+        a.m = j;
+        a.s = 0;
+        st[j] = a;
+    }
+}
+
+// Order key of a member in the walk from its cycle's j0: larger offset
+// first, then smaller distance (never 0: the key of every member is > 0).
+__device__ __forceinline__ unsigned long long cyc_key(uint32_t j, const uint32_t* cid, const JumpState* st,
+                                                      const uint16_t* offR) {
+    const uint32_t d = cid[j] == j ? 0xFFFFFFFEu : st[j].c - 1;
+    return ((unsigned long long)offR[j] << 32) | (0xFFFFFFFFu - d);
+}
+
+__global__ void k_asm_cyc_best(const uint32_t* __restrict__ members, uint64_t nm, const uint32_t* __restrict__ cid,
+                               const JumpState* __restrict__ st, const uint16_t* __restrict__ offR,
+                               unsigned long long* __restrict__ best) {
+    GRID_STRIDE(q, nm) {
+        const uint32_t j = members[q];
+        atomicMax(&best[cid[j]], cyc_key(j, cid, st, offR));
+    }
+}
+
+__global__ void k_asm_cyc_cut(const uint32_t* __restrict__ members, uint64_t nm, const uint32_t* __restrict__ cid,
+                              const JumpState* __restrict__ st, const uint16_t* __restrict__ offR,
+                              const unsigned long long* __restrict__ best, uint32_t* __restrict__ nxtR,
+                              uint32_t* __restrict__ prvR) {
+    GRID_STRIDE(q, nm) {
+        const uint32_t j = members[q];
+        if (cyc_key(j, cid, st, offR) != best[cid[j]]) continue;
+        const uint32_t hd = nxtR[j];
+        nxtR[j] = INV;
         prvR[hd] = INV;
     }
 }

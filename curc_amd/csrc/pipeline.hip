@@ -481,27 +481,44 @@ AsmOut assemble(curc_ctx* c, const uint32_t* V, uint32_t n, uint64_t id_space, c
         cur = 1 - cur;
     }
     uint32_t* members = S.as<uint32_t>(S_A_MEMB, n);
+    uint32_t* ckey = S.as<uint32_t>(S_A_CKEY, n);
+    uint64_t* clen = S.as<uint64_t>(S_A_CLEN, n);
+    // the cycles' members (ckey / clen hold their cycle and its cut key
+    // until k_asm_tails writes them at chain heads)
+    uint32_t* cid = ckey;
+    unsigned long long* best = reinterpret_cast<unsigned long long*>(clen);
     set_cnt(c, cnt + C_ASM, 0);
-    hipLaunchKernelGGL(k_asm_cycles, dim3(g), dim3(256), 0, st, (uint64_t)n, (const JumpState*)js[cur], nxtR, prvR,
-                       (const uint16_t*)offR, members, cnt + C_ASM);
+    hipLaunchKernelGGL(k_asm_cyc_list, dim3(g), dim3(256), 0, st, (uint64_t)n, (const JumpState*)js[cur], members,
+                       cnt + C_ASM, cid, best);
     const uint32_t ncyc = read_cnt(c, cnt + C_ASM);
     if (ncyc) {
-        // re-rank the cut cycles as paths; the other nodes are final in both
-        // buffers after this copy
+        // the other nodes are final in both buffers after this copy
         CURC_HIP(hipMemcpyAsync(js[1 - cur], js[cur], (uint64_t)n * sizeof(JumpState), hipMemcpyDeviceToDevice, st));
         const uint32_t gc = grid_for(ncyc);
-        hipLaunchKernelGGL(k_asm_init, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
-                           (const uint32_t*)nxtR, (const uint32_t*)prvR, (const uint16_t*)offR, L, js[cur]);
         uint32_t Kc = 1;
         while ((1ull << Kc) <= ncyc) ++Kc;
+        // each cycle ranked from its minimum-rank member, cut where the reference cuts it
+        hipLaunchKernelGGL(k_asm_cyc_open, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
+                           (const uint32_t*)cid, (const uint32_t*)prvR, js[cur]);
+        for (uint32_t r = 0; r < Kc; ++r) {
+            hipLaunchKernelGGL(k_asm_round, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
+                               (const JumpState*)js[cur], js[1 - cur]);
+            cur = 1 - cur;
+        }
+        hipLaunchKernelGGL(k_asm_cyc_best, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
+                           (const uint32_t*)cid, (const JumpState*)js[cur], (const uint16_t*)offR, best);
+        hipLaunchKernelGGL(k_asm_cyc_cut, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
+                           (const uint32_t*)cid, (const JumpState*)js[cur], (const uint16_t*)offR,
+                           (const unsigned long long*)best, nxtR, prvR);
+        // the cut cycles ranked again as paths
+        hipLaunchKernelGGL(k_asm_init, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
+                           (const uint32_t*)nxtR, (const uint32_t*)prvR, (const uint16_t*)offR, L, js[cur]);
         for (uint32_t r = 0; r < Kc; ++r) {
             hipLaunchKernelGGL(k_asm_round, dim3(gc), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)ncyc,
                                (const JumpState*)js[cur], js[1 - cur]);
             cur = 1 - cur;
         }
     }
-    uint32_t* ckey = S.as<uint32_t>(S_A_CKEY, n);
-    uint64_t* clen = S.as<uint64_t>(S_A_CLEN, n);
     uint32_t* ccnt = S.as<uint32_t>(S_A_CCNT, n);
     hipLaunchKernelGGL(k_asm_tails, dim3(g), dim3(256), 0, st, (uint64_t)n, (const JumpState*)js[cur],
                        (const uint32_t*)nxtR, ckey, clen, ccnt);
