@@ -169,8 +169,8 @@ struct Tuning {
     uint32_t ov_grid = 16384;    // workgroups (cap) of the overlap probe and claim finalisation
     uint32_t fm_grid = 4096;     // workgroups (cap) of the exact-matching probe and claim finalisation
     uint32_t map_grid = 32768;   // workgroups (cap) of the mapping kernels
-    uint32_t radix_items = 8;    // keys per thread of a radix-sort tile (8, 16, 32)
-    uint32_t filter_bits = 16;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
+    uint32_t radix_items = 16;   // keys per thread of a radix-sort tile (8, 16, 32)
+    uint32_t filter_bits = 32;   // prefix k-mer filter bits per prefix (8 .. 128, power of two)
     uint32_t fm_fused = 4096;    // exact-matching lists up to this bound run in one fused launch (0: never)
     uint32_t map_both = 2;       // both strand passes in one launch on a two-strand index of 128-byte lines (1) or 32-byte slots (2)
     uint32_t ov_queue = 3;       // overlap probe: 0 one suffix per thread, 1 queued in LDS, 2 queued with verdicts,

@@ -419,9 +419,9 @@ typedef struct curc_tuning {
     uint32_t ov_grid;     /* >= 1 (default 16384) */
     uint32_t fm_grid;     /* >= 1 (default 4096) */
     uint32_t map_grid;    /* >= 1 (default 32768) */
-    uint32_t radix_items; /* 8, 16 or 32 (default 8) */
+    uint32_t radix_items; /* 8, 16 or 32 (default 16) */
     uint32_t workers;     /* 1 .. CURC_BATCH_MAX_WORKERS (default 4) */
-    uint32_t filter_bits; /* overlap k-mer filter bits per prefix: 8 .. 128, a power of two (default 16) */
+    uint32_t filter_bits; /* overlap k-mer filter bits per prefix: 8 .. 128, a power of two (default 32) */
     uint32_t fm_fused;    /* exact-matching lists up to this bound run their offset loop in one
                              single-workgroup launch; 0: never (default 4096) */
     uint32_t ov_queue;    /* overlap probe: 1 queues the suffixes that pass the k-mer filter in
