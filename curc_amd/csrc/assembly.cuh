@@ -81,6 +81,115 @@ __global__ void k_asm_round(const uint32_t* __restrict__ list, uint64_t n, const
     }
 }
 
+// List ranking by a ruling set.  The jump rounds over all n nodes cost
+// log2(n) passes over the states; instead the chain heads and about one
+// node in 8 (a multiplicative hash of the rank: fixed, not random) are
+// rulers, each ruler walks its sublist along next up to the next ruler, and
+// only the rulers are ranked by jump rounds, on the sums of the sublists
+// before them.  A node's final state (the sums from its chain head to
+// itself, the head, the minimum rank) is its ruler's exclusive prefix plus
+// its own local sums, the same values the rounds over all nodes give.  A
+// cycle holding a ruler is ranked around by its rulers (its rulers' pointers
+// stay live and their windows wrap the cycle); a cycle without one is
+// ranked by the rounds over its members alone.
+// Slot of this lane's entry in a list that the active lanes of the
+// wavefront that set `want` append to, with one atomic per wavefront (every
+// active lane calls it).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+    const uint64_t act = __ballot(1);
+    const uint64_t bal = __ballot(want);
+    const int leader = __builtin_ctzll(act);
+    uint32_t base = 0;
+    if (bal && (int)(threadIdx.x & 63u) == leader) base = atomicAdd(counter, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(bal & lanemask_lt());
+}
+
+constexpr uint32_t ASM_RULER_SHIFT = 29;  // rulers: top 3 bits of the rank's hash clear
+__device__ __forceinline__ bool asm_ruler_hash(uint32_t j) { return ((j * 0x9E3779B1u) >> ASM_RULER_SHIFT) == 0; }
+
+// The rulers in rank order (a stable compaction): rlist, their index ridx,
+// and the state of a chain head (nothing before it; a later ruler's state is
+// written by the walk of the ruler before it).
+struct PredRuler {
+    const uint32_t* prvR;
+    __device__ bool operator()(uint32_t j) const { return prvR[j] == INV || asm_ruler_hash(j); }
+};
+struct EmitRuler {
+    const uint32_t* prvR;
+    uint32_t* rlist;
+    uint32_t* ridx;
+    JumpState* rs;
+    __device__ void operator()(uint32_t j, uint32_t q) const {
+        rlist[q] = j;
+        ridx[j] = q;
+        if (prvR[j] == INV) {
+            JumpState a;
+            a.p = INV;
+            a.c = 0;
+            a.h = j;
+            a.m = INV;
+            a.s = 0;
+            rs[q] = a;
+        }
+    }
+};
+
+// Ruler q walks its sublist: each node gets its ruler and the sums from the
+// ruler to itself (loc); the next ruler gets the sums of the whole sublist
+// and q as its predecessor.
+__global__ void k_asm_walk(uint64_t nr, const uint32_t* __restrict__ rlist, const uint32_t* __restrict__ nxtR,
+                           const uint16_t* __restrict__ offR, uint32_t L, const uint32_t* __restrict__ ridx,
+                           uint32_t* __restrict__ rul, JumpState* __restrict__ loc, JumpState* __restrict__ rs) {
+    GRID_STRIDE(q, nr) {
+        uint32_t x = rlist[q];
+        JumpState a;
+        a.p = (uint32_t)q;
+        a.c = 0;
+        a.h = 0;
+        a.m = INV;
+        a.s = 0;
+        for (;;) {
+            const uint32_t y = nxtR[x];
+            a.c += 1;
+            a.s += y == INV ? L : (uint32_t)offR[x];
+            a.m = min(a.m, x);
+            rul[x] = (uint32_t)q;
+            loc[x] = a;
+            if (y == INV) break;
+            // y has a predecessor (x), so it is a ruler by its hash alone
+            if (asm_ruler_hash(y)) {
+                rs[ridx[y]] = a;
+                break;
+            }
+            x = y;
+        }
+    }
+}
+
+// Final states from the rulers' ranks: nodes of a cycle without a ruler are
+// listed for the rounds over their members.
+__global__ void k_asm_final(uint64_t n, const uint32_t* __restrict__ rul, const JumpState* __restrict__ loc,
+                            const JumpState* __restrict__ rs, JumpState* __restrict__ st,
+                            uint32_t* __restrict__ rest, uint32_t* __restrict__ n_rest) {
+    GRID_STRIDE(j, n) {
+        const uint32_t q = rul[j];
+        const uint32_t slot = wave_append(n_rest, q == INV);
+        if (q == INV) {
+            rest[slot] = (uint32_t)j;
+            continue;
+        }
+        const JumpState r = rs[q], l = loc[j];
+        JumpState a;
+        a.p = r.p;
+        a.c = r.c + l.c;
+        a.h = r.h;
+        a.m = min(r.m, l.m);
+        a.s = r.s + l.s;
+        st[j] = a;
+    }
+}
+
 // Cycle nodes keep a live jump pointer after enough rounds; their window
 // holds the whole cycle, so m is the cycle's minimum rank j0, the member the
 // reference's visit loop meets first.  The reference cuts the cycle after
@@ -98,8 +207,9 @@ __global__ void k_asm_cyc_list(uint64_t n, const JumpState* __restrict__ st, uin
                                unsigned long long* __restrict__ best) {
     GRID_STRIDE(j, n) {
         const JumpState a = st[j];
+        const uint32_t slot = wave_append(n_members, a.p != INV);
         if (a.p == INV) continue;
-        members[atomicAdd(n_members, 1u)] = (uint32_t)j;
+        members[slot] = (uint32_t)j;
         cid[j] = a.m;
         if (a.m == (uint32_t)j) best[j] = 0ull;
     }

@@ -107,6 +107,7 @@ enum Slot {
     S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2, S_OVPASS, S_WORK, S_FMNF, S_FMVB, S_FMFILT, S_FMLEAD, S_FMGC, S_FMGM,
     S_NMASK,
     S_OVLIST,
+    S_A_RS0, S_A_RS1, S_A_RUL, S_A_RIDX, S_A_RLIST,
     S_LAST_NAMED,
     S_PRIM = 236  // primitives use slots 236.. (scan recursion, sort hist, compaction)
 };
@@ -468,19 +469,53 @@ AsmOut assemble(curc_ctx* c, const uint32_t* V, uint32_t n, uint64_t id_space, c
     hipLaunchKernelGGL(k_asm_rank, dim3(g), dim3(256), 0, st, V, (uint64_t)n, rank);
     hipLaunchKernelGGL(k_asm_links, dim3(g), dim3(256), 0, st, V, (uint64_t)n, next, prev, offset,
                        (const uint32_t*)rank, nxtR, prvR, offR);
-    hipLaunchKernelGGL(k_asm_init, dim3(g), dim3(256), 0, st, (const uint32_t*)nullptr, (uint64_t)n,
-                       (const uint32_t*)nxtR, (const uint32_t*)prvR, (const uint16_t*)offR, L, js[0]);
-    // 2^K > n >= any chain or cycle length, so every path node's pointer runs
-    // out and every cycle node's window wraps its whole cycle.
-    uint32_t K = 1;
-    while ((1ull << K) <= n) ++K;
+    // list ranking by a ruling set (assembly.cuh): rulers, their walks, jump
+    // rounds over the rulers (2^Kr > rulers >= ruler chain or cycle length),
+    // then every node's state into js[0]
     int cur = 0;
-    for (uint32_t r = 0; r < K; ++r) {
-        hipLaunchKernelGGL(k_asm_round, dim3(g), dim3(256), 0, st, (const uint32_t*)nullptr, (uint64_t)n,
-                           (const JumpState*)js[cur], js[1 - cur]);
-        cur = 1 - cur;
-    }
     uint32_t* members = S.as<uint32_t>(S_A_MEMB, n);
+    {
+        uint32_t* rul = S.as<uint32_t>(S_A_RUL, n);
+        uint32_t* ridx = S.as<uint32_t>(S_A_RIDX, n);
+        uint32_t* rlist = S.as<uint32_t>(S_A_RLIST, n);
+        JumpState* rs0 = S.as<JumpState>(S_A_RS0, n);
+        fill<uint32_t>(c, rul, n, INV);
+        set_cnt(c, cnt + C_TMP, n);
+        compact(cnt + C_TMP, PredRuler{prvR}, EmitRuler{prvR, rlist, ridx, rs0}, cnt + C_ASM, S, S_PRIM, st, n);
+        const uint32_t nr = read_cnt(c, cnt + C_ASM);
+        JumpState* rs[2] = {rs0, S.as<JumpState>(S_A_RS1, nr)};
+        const uint32_t gr = grid_for(nr);
+        hipLaunchKernelGGL(k_asm_walk, dim3(gr), dim3(256), 0, st, (uint64_t)nr, (const uint32_t*)rlist,
+                           (const uint32_t*)nxtR, (const uint16_t*)offR, L, (const uint32_t*)ridx, rul, js[1], rs0);
+        uint32_t Kr = 1;
+        while ((1ull << Kr) <= nr) ++Kr;
+        int rc = 0;
+        for (uint32_t r = 0; r < Kr; ++r) {
+            hipLaunchKernelGGL(k_asm_round, dim3(gr), dim3(256), 0, st, (const uint32_t*)nullptr, (uint64_t)nr,
+                               (const JumpState*)rs[rc], rs[1 - rc]);
+            rc = 1 - rc;
+        }
+        set_cnt(c, cnt + C_ASM, 0);
+        hipLaunchKernelGGL(k_asm_final, dim3(g), dim3(256), 0, st, (uint64_t)n, (const uint32_t*)rul,
+                           (const JumpState*)js[1], (const JumpState*)rs[rc], js[0], members, cnt + C_ASM);
+        const uint32_t nu = read_cnt(c, cnt + C_ASM);
+        if (nu) {
+            // cycles without a ruler: the rounds over their members (an even
+            // number, so that the states end in js[0]; more rounds only widen
+            // a cycle's window, which still wraps the cycle)
+            const uint32_t gu = grid_for(nu);
+            hipLaunchKernelGGL(k_asm_init, dim3(gu), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)nu,
+                               (const uint32_t*)nxtR, (const uint32_t*)prvR, (const uint16_t*)offR, L, js[0]);
+            uint32_t Ku = 1;
+            while ((1ull << Ku) <= nu) ++Ku;
+            Ku += Ku & 1u;
+            for (uint32_t r = 0; r < Ku; ++r) {
+                hipLaunchKernelGGL(k_asm_round, dim3(gu), dim3(256), 0, st, (const uint32_t*)members, (uint64_t)nu,
+                                   (const JumpState*)js[cur], js[1 - cur]);
+                cur = 1 - cur;
+            }
+        }
+    }
     uint32_t* ckey = S.as<uint32_t>(S_A_CKEY, n);
     uint64_t* clen = S.as<uint64_t>(S_A_CLEN, n);
     // the cycles' members (ckey / clen hold their cycle and its cut key
