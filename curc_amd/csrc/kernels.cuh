@@ -193,9 +193,14 @@ __global__ void k_tie_scatter(const uint32_t* __restrict__ pos, const uint32_t* 
 }
 
 // compress.cu:1409-1420: adjacent equal reads in sorted order are linked.
+// eqf[i]: the read at dedup position i equals the one at i + 1 (0 at n - 1),
+// so that the reads without a predecessor or successor in dedup order are
+// found from a sequential pass (PredNoPred / PredNoSucc) instead of gathers
+// of next and prev.
 __global__ void k_dedup_link(const uint64_t* __restrict__ R, const uint32_t* __restrict__ sorted, uint64_t n,
                              uint32_t W, uint32_t* __restrict__ next, uint32_t* __restrict__ prev,
-                             uint16_t* __restrict__ offset) {
+                             uint16_t* __restrict__ offset, uint8_t* __restrict__ eqf) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) eqf[n - 1] = 0;
     GRID_STRIDE(i, n - 1) {
         uint32_t a = sorted[i], b = sorted[i + 1];
         const uint64_t* ra = R + (uint64_t)a * W;
@@ -206,6 +211,7 @@ __global__ void k_dedup_link(const uint64_t* __restrict__ R, const uint32_t* __r
                 eq = false;
                 break;
             }
+        eqf[i] = eq ? 1 : 0;
         if (eq) {
             next[a] = b;
             prev[b] = a;
@@ -213,6 +219,17 @@ __global__ void k_dedup_link(const uint64_t* __restrict__ R, const uint32_t* __r
         }
     }
 }
+
+// Dedup positions without a predecessor / successor (the dedup links are the
+// only links when the prefix and suffix lists are formed).
+struct PredNoPred {
+    const uint8_t* eqf;
+    __device__ bool operator()(uint32_t i) const { return i == 0 || eqf[i - 1] == 0; }
+};
+struct PredNoSucc {
+    const uint8_t* eqf;
+    __device__ bool operator()(uint32_t i) const { return eqf[i] == 0; }
+};
 
 struct PredLinkState {
     const uint32_t* list;

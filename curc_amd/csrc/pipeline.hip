@@ -107,9 +107,9 @@ enum Slot {
     S_D_GS, S_D_FLAT, S_D_OUT1, S_D_OUT2, S_OVPASS, S_WORK, S_FMNF, S_FMVB, S_FMFILT, S_FMLEAD, S_FMGC, S_FMGM,
     S_NMASK,
     S_OVLIST,
-    S_A_RS0, S_A_RS1, S_A_RUL, S_A_RIDX, S_A_RLIST,
+    S_A_RS0, S_A_RS1, S_A_RUL, S_A_RIDX, S_A_RLIST, S_DEQ,
     S_LAST_NAMED,
-    S_PRIM = 236  // primitives use slots 236.. (scan recursion, sort hist, compaction)
+    S_PRIM = 240  // primitives use slots 240.. (scan recursion, sort hist, compaction)
 };
 
 // Device counters (u32) in slot S_CNT.
@@ -1455,7 +1455,8 @@ NlqOut nlq_stage(curc_ctx* x, hipEvent_t wait, const uint8_t* Nascii, uint64_t n
         CURC_HIP(hipMemcpyAsync(nsorted, v0, (uint64_t)n_nlq * 4, hipMemcpyDeviceToDevice, st));
         if (n_nlq > 1)
             hipLaunchKernelGGL(k_dedup_link, dim3(grid_for(n_nlq)), dim3(256), 0, st, (const uint64_t*)p3,
-                               (const uint32_t*)nsorted, (uint64_t)n_nlq, W3, nnext, nprev, noff);
+                               (const uint32_t*)nsorted, (uint64_t)n_nlq, W3, nnext, nprev, noff,
+                               S.as<uint8_t>(S_DEQ, (uint64_t)n_nlq + 1));
         uint32_t* pre = S.as<uint32_t>(S_PRE, n_nlq);
         uint32_t* suf = S.as<uint32_t>(S_SUF, n_nlq);
         set_cnt(x, cnt + C_TMP, n_nlq);
@@ -1677,6 +1678,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     uint32_t* prev = S.as<uint32_t>(S_PREV, N + 1);
     uint16_t* off = S.as<uint16_t>(S_OFF, N + 1);
     uint32_t* sorted;
+    uint8_t* eqf = S.as<uint8_t>(S_DEQ, (uint64_t)n + 1);  // see k_dedup_link
     {
         Region r(c, "dedup");
         fill<uint32_t>(c, next, N, INV);
@@ -1751,7 +1753,9 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
         sorted = v0;
         if (n > 1)
             hipLaunchKernelGGL(k_dedup_link, dim3(grid_for(n)), dim3(256), 0, st, (const uint64_t*)packed,
-                               (const uint32_t*)sorted, (uint64_t)n, W, next, prev, off);
+                               (const uint32_t*)sorted, (uint64_t)n, W, next, prev, off, eqf);
+        else
+            fill<uint8_t>(c, eqf, 1, (uint8_t)0);
         c->add_bytes("dedup", (double)n * 8 * W + 2.0 * n * (8 * W + 4), (double)n);
     }
     // keep the sorted ids in a stable slot (v0/v1 are reused by later sorts)
@@ -1784,8 +1788,8 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
     // (without a successor), in dedup order: one single-pass launch for both
     // when they fit its tile limit
     uint32_t lb_err = 0;
-    compact_pair(c, cnt + C_TMP, n, PredLinkState{sorted, prev}, EmitGather{sorted, va}, cnt + C_NPRE,
-                 PredLinkState{sorted, next}, EmitGather{sorted, sl0}, cnt + C_NSUF, &lb_err);
+    compact_pair(c, cnt + C_TMP, n, PredNoPred{eqf}, EmitGather{sorted, va}, cnt + C_NPRE, PredNoSucc{eqf},
+                 EmitGather{sorted, sl0}, cnt + C_NSUF, &lb_err);
     const uint32_t npre = read_cnt(c, cnt + C_NPRE);
     const uint32_t nsuf = read_cnt(c, cnt + C_NSUF);
     if (lb_err != 0) throw std::runtime_error("prefix / suffix list compaction: look-back did not complete");
