@@ -2041,12 +2041,14 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
                            (const uint32_t*)prev, (const uint16_t*)off, N, res);
         hipLaunchKernelGGL(k_prune_unlink, dim3(grid_for(N)), dim3(256), 0, st, next, prev, (const uint8_t*)res, N);
         set_cnt(c, cnt + C_TMP, n);
-        compact(cnt + C_TMP, PredRes{sorted, res, true}, EmitGather{sorted, hq}, cnt + C_HQ, S, S_PRIM, st, n);
-        compact(cnt + C_TMP, PredRes{sorted, res, false}, EmitGather{sorted, unm}, cnt + C_UNM, S, S_PRIM, st, n);
+        // both lists of the partition in one single-pass launch
+        compact_pair(c, cnt + C_TMP, n, PredRes{sorted, res, true}, EmitGather{sorted, hq}, cnt + C_HQ,
+                     PredRes{sorted, res, false}, EmitGather{sorted, unm}, cnt + C_UNM, &lb_err);
         c->add_bytes("prune", 28.0 * N, (double)N);
     }
     n_hq = read_cnt(c, cnt + C_HQ);
     n_unm = read_cnt(c, cnt + C_UNM);
+    if (lb_err != 0) throw std::runtime_error("HQ / unmatched partition: look-back did not complete");
     if (ov_counted)  // per probe: the suffix's k-mer, the bucket's bounds; per search: a prefix's id and words
         c->add_bytes("overlap", (double)c->ov_work[0] * 32 + (double)c->ov_work[1] * (4 + 8.0 * W),
                      (double)c->ov_work[0]);
