@@ -2068,28 +2068,58 @@ __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t of
     const uint64_t per = (ns + nwv - 1) / nwv;
     const uint64_t beg = wid * per;
     const uint64_t end = min<uint64_t>(beg + per, ns);
-    for (uint64_t i0 = beg; i0 < end; i0 += 64) {
-        const uint64_t i = i0 + lane;
-        uint32_t wg = INV, wm = 0;
-        if (i < end && lead[i]) {
-            lead[i] = 0;
-            const uint32_t g = cs.grp[i];
-            const uint32_t m = gcnt[g];
-            if (m > FM_SMALL && m <= wmax) {
-                gcnt[g] = 0;
-                wg = g;
-                wm = m;
-            } else {
-                group(g, m);
+    // FM_SCAN positions at a time: their lead flags loaded together, the
+    // leads listed in LDS (the cooperative sort's key space, unused until
+    // fm_coop_groups), then resolved a lead per lane, so that the groups'
+    // chains of loads overlap instead of following each 64 flags
+    constexpr uint32_t FM_SCAN = 512;
+    static_assert(4 * FM_SCAN * 4 <= sizeof(co.key), "wave lead lists");
+    uint32_t* wl = reinterpret_cast<uint32_t*>(co.key) + (threadIdx.x >> 6) * FM_SCAN;
+    const uint64_t lt = lanemask_lt();
+    for (uint64_t i0 = beg; i0 < end; i0 += FM_SCAN) {
+        uint8_t f[FM_SCAN / 64];
+#pragma unroll
+        for (uint32_t r = 0; r < FM_SCAN / 64; ++r) {
+            const uint64_t i = i0 + r * 64 + lane;
+            f[r] = i < end ? lead[i] : (uint8_t)0;
+        }
+        uint32_t wn = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < FM_SCAN / 64; ++r) {
+            const bool hit = f[r] != 0;
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const uint64_t i = i0 + r * 64 + lane;
+                lead[i] = 0;
+                wl[wn + (uint32_t)__popcll(bal & lt)] = (uint32_t)i;
+            }
+            wn += (uint32_t)__popcll(bal);
+        }
+        __builtin_amdgcn_wave_barrier();  // the list is read by other lanes of the wave
+        for (uint32_t k0 = 0; k0 < wn; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            uint32_t wg = INV, wm = 0;
+            if (k < wn) {
+                const uint32_t i = wl[k];
+                const uint32_t g = cs.grp[i];
+                const uint32_t m = gcnt[g];
+                if (m > FM_SMALL && m <= wmax) {
+                    gcnt[g] = 0;
+                    wg = g;
+                    wm = m;
+                } else {
+                    group(g, m);
+                }
+            }
+            uint64_t bal = __ballot(wg != INV);
+            while (bal) {
+                const int kk = __builtin_ctzll(bal);
+                bal &= bal - 1;
+                const uint32_t gk = __shfl(wg, kk, 64), mk = __shfl(wm, kk, 64);
+                fm_wave_group<E>(cs, gk, mk, off, gmem + (uint64_t)gk * rw, lane);
             }
         }
-        uint64_t bal = __ballot(wg != INV);
-        while (bal) {
-            const int k = __builtin_ctzll(bal);
-            bal &= bal - 1;
-            const uint32_t gk = __shfl(wg, k, 64), mk = __shfl(wm, k, 64);
-            fm_wave_group<E>(cs, gk, mk, off, gmem + (uint64_t)gk * rw, lane);
-        }
+        __builtin_amdgcn_wave_barrier();  // before the next chunk's list overwrites this one
     }
     fm_coop_groups<E>(cs, co, off, gmem, rw);
 }
