@@ -1919,16 +1919,23 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t j) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// A group of FM_SMALL + 1 .. FM_WAVE_MAX members over compacted lists,
+// A group of up to FM_WAVE_MAX members whose members are all in its row,
 // resolved by one wavefront (all 64 lanes call it with the same g, m): lane
-// k loads member k from the group's row, the members are sorted by a
-// bitonic network across the lanes (the order of fm_before; the reads are
-// loaded only to break a tie of the keys), lane j holds pos(j) of the run,
-// and the walk of fm_walk_free runs on every lane alike with the values
-// taken from their lanes (lane 0 writes the links).
+// k loads member k from the row, the members are sorted by a bitonic
+// network across the lanes (the order of fm_before; the reads are loaded
+// only to break a tie of the keys), and the walk of fm_walk_free runs on
+// every lane alike over the run's free prefixes (lane 0 writes the links).
+// Compacted lists: every position of the run is free, lane j holds the
+// j-th.  Lists in place (cs.nf): lane j checks position g + j (equal to the
+// group's suffix and not linked), and the free ones are listed in lp (the
+// wave's 128 words of LDS: positions, then prefixes) in order; if the run
+// goes on past the 64 positions and they hold fewer than m + 2 free ones,
+// nothing is done and false is returned (the caller resolves the group as
+// before).
 template <class E>
-__device__ __forceinline__ void fm_wave_group(const ClaimState& cs, uint32_t g, uint32_t m, uint32_t off,
-                                              const uint32_t* __restrict__ row, uint32_t lane) {
+__device__ __forceinline__ bool fm_wave_group(const ClaimState& cs, uint32_t g, uint32_t m, uint32_t off,
+                                              const uint32_t* __restrict__ row, uint32_t lane,
+                                              uint32_t* __restrict__ lp) {
     uint32_t u = lane < m ? row[lane] : INV;
     uint32_t sid = u != INV ? cs.sl[u] : INV;
     uint64_t key = u != INV ? rev_prefix_key<E>(cs.R, cs.W, sid, off) : ~0ull;
@@ -1949,14 +1956,31 @@ __device__ __forceinline__ void fm_wave_group(const ClaimState& cs, uint32_t g, 
             }
         }
     }
-    // the run's prefixes at positions g .. g + m + 1 (INV from the first unequal one)
     const uint32_t np = *cs.d_np, len = cs.L - off, n1 = min(E::BPW, len);
     const uint64_t* rs0 = cs.R + (uint64_t)__shfl(sid, 0, 64) * cs.W;
     const uint64_t sx00 = E::word(rs0, off, n1);
     const uint32_t p = g + lane;
-    const uint32_t posl =
-        (lane <= m + 1 && p < np && cmp_ps<E>(cs.R, cs.W, cs.pl[p], cs.pk[p], rs0, sx00, off, len) == 0) ? cs.pl[p]
-                                                                                                         : INV;
+    // pos(j) / px(j): the prefix and position of the run's j-th free
+    // prefix, INV past the run; j <= m + 1 < 64
+    uint32_t posl = INV;
+    if (!cs.nf) {
+        if (lane <= m + 1 && p < np && cmp_ps<E>(cs.R, cs.W, cs.pl[p], cs.pk[p], rs0, sx00, off, len) == 0)
+            posl = cs.pl[p];
+    } else {
+        const bool eq = p < np && cmp_ps<E>(cs.R, cs.W, cs.pl[p], cs.pk[p], rs0, sx00, off, len) == 0;
+        const bool fr = eq && cs.plinked[p] == 0;
+        const uint64_t eb = __ballot(eq), fb = __ballot(fr);
+        if (eb == ~0ull && (uint32_t)__popcll(fb) < m + 2) return false;
+        const uint32_t nfree = (uint32_t)__popcll(fb);
+        if (fr) {
+            const uint32_t j = (uint32_t)__popcll(fb & lanemask_lt());
+            lp[j] = p;
+            lp[64 + j] = cs.pl[p];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < nfree) posl = lp[64 + lane];
+    }
+    auto px = [&](uint32_t j) -> uint32_t { return cs.nf ? lp[j] : g + j; };
     // fm_walk_free; a, b <= m + 1 < 64
     uint32_t a = 0, b = 1;
     for (uint32_t k = 0; k < m; ++k) {
@@ -1964,19 +1988,21 @@ __device__ __forceinline__ void fm_wave_group(const ClaimState& cs, uint32_t g, 
         if (pa == INV) break;
         const uint32_t s = __shfl(sid, (int)k, 64), mk = __shfl(u, (int)k, 64);
         if (pa != s) {
-            if (lane == 0) fm_link(cs, mk, s, g + a, pa, off);
+            if (lane == 0) fm_link(cs, mk, s, px(a), pa, off);
             a = b;
             b = b + 1;
         } else {
             const uint32_t pb = __shfl(posl, (int)b, 64);
             if (pb != INV) {
-                if (lane == 0) fm_link(cs, mk, s, g + b, pb, off);
+                if (lane == 0) fm_link(cs, mk, s, px(b), pb, off);
                 b = b + 1;
             }
         }
     }
     if (u != INV) cs.grp[u] = INV;
     if (lane == 0) cs.head[g] = INV;
+    __builtin_amdgcn_wave_barrier();  // lp is read before the wave's next group overwrites it
+    return true;
 }
 
 // Resolution of the groups k_fm_probe_s formed, found by their lead flags
@@ -1999,7 +2025,11 @@ __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t of
     __syncthreads();
     const uint32_t ns = *cs.d_ns;
     const uint32_t rw = fm_row_width(gcap, *cs.d_np);
-    const uint32_t wmax = (gcap && !cs.nf) ? min(rw, FM_WAVE_MAX) : 0u;
+    // groups the wavefront resolves: over compacted lists 17 .. 62 members
+    // (smaller ones by a lane in registers), over lists in place 2 .. 16
+    // (a lane's walk would follow the skip pointers one load at a time)
+    const uint32_t wmin = cs.nf ? 1u : FM_SMALL;
+    const uint32_t wmax = cs.nf ? FM_SMALL : (gcap ? min(rw, FM_WAVE_MAX) : 0u);
     auto group = [&](uint32_t g, uint32_t m) {
         gcnt[g] = 0;
         if (m <= FM_SMALL) {
@@ -2073,8 +2103,9 @@ __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t of
     // fm_coop_groups), then resolved a lead per lane, so that the groups'
     // chains of loads overlap instead of following each 64 flags
     constexpr uint32_t FM_SCAN = 512;
-    static_assert(4 * FM_SCAN * 4 <= sizeof(co.key), "wave lead lists");
+    static_assert(4 * (FM_SCAN + 128) * 4 <= sizeof(co.key), "wave lead lists and run lists");
     uint32_t* wl = reinterpret_cast<uint32_t*>(co.key) + (threadIdx.x >> 6) * FM_SCAN;
+    uint32_t* lp = reinterpret_cast<uint32_t*>(co.key) + 4 * FM_SCAN + (threadIdx.x >> 6) * 128;
     const uint64_t lt = lanemask_lt();
     for (uint64_t i0 = beg; i0 < end; i0 += FM_SCAN) {
         uint8_t f[FM_SCAN / 64];
@@ -2103,8 +2134,7 @@ __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t of
                 const uint32_t i = wl[k];
                 const uint32_t g = cs.grp[i];
                 const uint32_t m = gcnt[g];
-                if (m > FM_SMALL && m <= wmax) {
-                    gcnt[g] = 0;
+                if (m > wmin && m <= wmax) {
                     wg = g;
                     wm = m;
                 } else {
@@ -2116,7 +2146,12 @@ __global__ void __launch_bounds__(256) k_fm_resolve_s(ClaimState cs, uint32_t of
                 const int kk = __builtin_ctzll(bal);
                 bal &= bal - 1;
                 const uint32_t gk = __shfl(wg, kk, 64), mk = __shfl(wm, kk, 64);
-                fm_wave_group<E>(cs, gk, mk, off, gmem + (uint64_t)gk * rw, lane);
+                const bool done = fm_wave_group<E>(cs, gk, mk, off, gmem + (uint64_t)gk * rw, lane, lp);
+                if (done) {
+                    if (lane == 0) gcnt[gk] = 0;
+                } else if (lane == kk) {
+                    group(gk, mk);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();  // before the next chunk's list overwrites this one
