@@ -92,19 +92,6 @@ __global__ void k_asm_round(const uint32_t* __restrict__ list, uint64_t n, const
 // cycle holding a ruler is ranked around by its rulers (its rulers' pointers
 // stay live and their windows wrap the cycle); a cycle without one is
 // ranked by the rounds over its members alone.
-// Slot of this lane's entry in a list that the active lanes of the
-// wavefront that set `want` append to, with one atomic per wavefront (every
-// active lane calls it).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
-    const uint64_t act = __ballot(1);
-    const uint64_t bal = __ballot(want);
-    const int leader = __builtin_ctzll(act);
-    uint32_t base = 0;
-    if (bal && (int)(threadIdx.x & 63u) == leader) base = atomicAdd(counter, (uint32_t)__popcll(bal));
-    base = __shfl(base, leader, 64);
-    return base + (uint32_t)__popcll(bal & lanemask_lt());
-}
-
 constexpr uint32_t ASM_RULER_SHIFT = 29;  // rulers: top 3 bits of the rank's hash clear
 __device__ __forceinline__ bool asm_ruler_hash(uint32_t j) { return ((j * 0x9E3779B1u) >> ASM_RULER_SHIFT) == 0; }
 

@@ -19,6 +19,19 @@ namespace curc {
 #define GRID_STRIDE(i, n) \
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
 
+// Slot of this lane's entry in a list that the active lanes of the
+// wavefront that set `want` append to, with one atomic per wavefront (every
+// active lane calls it).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+    const uint64_t act = __ballot(1);
+    const uint64_t bal = __ballot(want);
+    const int leader = __builtin_ctzll(act);
+    uint32_t base = 0;
+    if (bal && (int)(threadIdx.x & 63u) == leader) base = atomicAdd(counter, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(bal & lanemask_lt());
+}
+
 // Plain device copy in 16-byte words (the measured bandwidth ceiling the
 // benchmark reports beside the vendor peak).
 __global__ void __launch_bounds__(256) k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
@@ -894,9 +907,11 @@ __global__ void __launch_bounds__(256) k_ov_finalize_q(ClaimState cs, uint32_t o
         const uint32_t ch = cs.choice[i];
         cs.owner[ch] = INV;
         const uint32_t g = cs.grp[i];
-        if (cs.contended[g]) {
+        const bool cont = cs.contended[g] != 0;
+        // one atomic on the shared count per wavefront, not per member
+        const uint32_t q = wave_append(cs.res_n, cont);
+        if (cont) {
             cs.memb[i] = atomicExch(&cs.head[g], i);
-            const uint32_t q = atomicAdd(cs.res_n, 1u);
             cs.res_list[q] = i;
         } else {
             cs.link(i, ch, off);
