@@ -882,9 +882,16 @@ __global__ void __launch_bounds__(256) k_ov_probe_count(const uint32_t* __restri
         const uint16_t o = off[sl[i]];
         acc += o == INVOFF ? (uint64_t)(max_off - 1) : (uint64_t)o;
     }
-    // 64-bit totals per wave, one atomic per wave
+    // 64-bit totals per wave, then per workgroup: one atomic per workgroup
+    // (same-address atomics are serialised, about 10 ns each)
+    __shared__ uint64_t wsum[4];
     for (int d = 32; d >= 1; d >>= 1) acc += __shfl_down(acc, d, 64);
-    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((unsigned long long*)out, (unsigned long long)acc);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd((unsigned long long*)out, (unsigned long long)t);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_sum_u32(const uint32_t* __restrict__ a, uint32_t n,

@@ -2008,7 +2008,7 @@ void run_block(curc_ctx* c, const char* ascii, uint64_t N, uint32_t L, int stop,
             // (probes that passed the k-mer filter); read back with the
             // partition's counts
             if (nsuf)
-                hipLaunchKernelGGL(k_ov_probe_count, dim3(grid_for(nsuf)), dim3(256), 0, st, (const uint32_t*)sl[0],
+                hipLaunchKernelGGL(k_ov_probe_count, dim3(grid_for(nsuf, 1024)), dim3(256), 0, st, (const uint32_t*)sl[0],
                                    nsuf, (const uint16_t*)off, P.max_off, ovw);
             if (passed)
                 hipLaunchKernelGGL(k_sum_u32, dim3(grid_for(wscan ? gw : gq)), dim3(256), 0, st,
